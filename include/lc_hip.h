@@ -1,0 +1,212 @@
+/*
+ * lc_hip.h -- C ABI of the MI355X (gfx950) ray-tracing backend, library lc-backend-hip.so.
+ *
+ * Plain C: handles are uint64_t (the backend object pointer, as in the reference where a
+ * resource handle is the backend object's address), sizes are size_t, buffers are device
+ * memory owned by the library. No C++ or torch types cross this boundary.
+ *
+ * Each entry point names the reference interface it stands in for. Paths are relative to the
+ * LuisaCompute checkout:
+ *   DeviceInterface           include/luisa/runtime/rhi/device_interface.h:95-212
+ *   Command classes           include/luisa/runtime/rhi/command.h
+ *   pure-C device fn-table    include/luisa/rust/api_types.h:597-654 (LCDeviceInterface)
+ *
+ * Error behaviour: the reference aborts on fatal errors and returns the invalid handle
+ * (~0ull) for unsupported requests (include/luisa/runtime/rhi/resource.h:18-33). Here every
+ * call that can fail returns LC_HIP_OK (0) or a negative status, creators return
+ * LC_HIP_INVALID_HANDLE, and lc_hip_last_error() describes the last failure of the calling
+ * thread; the C++ DeviceInterface layer turns these into the reference's abort.
+ */
+#ifndef LC_HIP_H
+#define LC_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LC_HIP_INVALID_HANDLE UINT64_MAX
+
+enum {
+    LC_HIP_OK = 0,
+    LC_HIP_ERROR_INVALID_ARGUMENT = -1,
+    LC_HIP_ERROR_INVALID_HANDLE = -2,
+    LC_HIP_ERROR_NOT_BUILT = -3,
+    LC_HIP_ERROR_HIP = -4,
+    LC_HIP_ERROR_OUT_OF_MEMORY = -5,
+    LC_HIP_ERROR_UNSUPPORTED = -6,
+    LC_HIP_ERROR_DEVICE_FAULT = -7
+};
+
+/* Ray: include/luisa/runtime/rtx/ray.h:10-16 (32 B, align 16). */
+typedef struct lc_hip_ray {
+    float origin[3];
+    float t_min;
+    float direction[3];
+    float t_max;
+} lc_hip_ray;
+
+/* SurfaceHit: include/luisa/runtime/rtx/hit.h:30-37 (24 B, align 8). Miss: inst = prim = ~0u,
+   bary = (0, 0), t = ray t_max (src/rust/luisa_compute_backend_impl/src/cpu/accel.rs:500-507). */
+typedef struct lc_hip_surface_hit {
+    uint32_t inst;
+    uint32_t prim;
+    float bary[2];
+    float committed_ray_t;
+    uint32_t _pad;
+} lc_hip_surface_hit;
+
+/* AccelOption: include/luisa/runtime/rhi/resource.h:94-126 (hint 0 = FAST_TRACE, 1 = FAST_BUILD). */
+typedef struct lc_hip_accel_option {
+    uint32_t hint;
+    uint8_t allow_compaction;
+    uint8_t allow_update;
+    uint8_t _pad[2];
+} lc_hip_accel_option;
+
+/* AccelBuildCommand::Modification: include/luisa/runtime/rhi/command.h:534-551 (80 B, align 16).
+   flags: 1 primitive, 2 transform, 4 opaque on, 8 opaque off, 16 visibility, 32 user id. */
+typedef struct lc_hip_accel_modification {
+    uint32_t index;
+    uint32_t user_id;
+    uint32_t flags;
+    uint32_t vis_mask;
+    float affine[12]; /* row-major 3x4, object -> world (command.h:557-571) */
+    uint64_t primitive; /* mesh handle */
+    uint64_t _pad;
+} lc_hip_accel_modification;
+
+#define LC_HIP_MOD_PRIMITIVE 1u
+#define LC_HIP_MOD_TRANSFORM 2u
+#define LC_HIP_MOD_OPAQUE_ON 4u
+#define LC_HIP_MOD_OPAQUE_OFF 8u
+#define LC_HIP_MOD_VISIBILITY 16u
+#define LC_HIP_MOD_USER_ID 32u
+
+/* AccelBuildRequest: command.h:402-405 */
+#define LC_HIP_BUILD_PREFER_UPDATE 0u
+#define LC_HIP_BUILD_FORCE_BUILD 1u
+
+const char *lc_hip_last_error(void);
+
+/* ---- device: backend exports create/destroy/backend_device_names
+   (src/backends/cuda/cuda_device.cpp:1192-1228, looked up at src/runtime/context.cpp:85-87);
+   LCLibInterface::create_device (api_types.h:664) ---- */
+int lc_hip_device_count(void);
+/* index < 0 or SIZE_MAX-like values select device 0 (device_interface.h:85, cuda_device.cpp:1057). */
+uint64_t lc_hip_device_create(int32_t index);
+void lc_hip_device_destroy(uint64_t device);
+/* DeviceInterface::compute_warp_size (device_interface.h:113): 64 on CDNA4. */
+uint32_t lc_hip_device_warp_size(uint64_t device);
+/* DeviceInterface::native_handle (device_interface.h:112): the HIP device ordinal. */
+int lc_hip_device_native(uint64_t device);
+/* DeviceInterface::query (device_interface.h:189): "device_name", "cu_count", "warp_size". */
+int lc_hip_device_query(uint64_t device, const char *key, char *out, size_t capacity);
+
+/* ---- buffers: DeviceInterface::create_buffer / destroy_buffer (device_interface.h:116-121),
+   BufferCreationInfo {handle, native_handle, element_stride, total_size_bytes} ---- */
+uint64_t lc_hip_buffer_create(uint64_t device, size_t element_stride, size_t element_count, void **native_handle);
+void lc_hip_buffer_destroy(uint64_t device, uint64_t buffer);
+/* BufferUploadCommand / BufferDownloadCommand / BufferCopyCommand (command.h:128-233). The
+   upload source is consumed before the call returns; a download destination is valid after the
+   stream is synchronized. */
+int lc_hip_buffer_upload(uint64_t device, uint64_t stream, uint64_t buffer, size_t offset, size_t size,
+                         const void *data);
+int lc_hip_buffer_download(uint64_t device, uint64_t stream, uint64_t buffer, size_t offset, size_t size,
+                           void *data);
+int lc_hip_buffer_copy(uint64_t device, uint64_t stream, uint64_t src, size_t src_offset, uint64_t dst,
+                       size_t dst_offset, size_t size);
+
+/* ---- streams: create_stream / destroy_stream / synchronize_stream / dispatch callback
+   (device_interface.h:128-134; CUDAStream::callback, cuda_stream.cpp:142-170) ---- */
+uint64_t lc_hip_stream_create(uint64_t device, uint32_t stream_tag);
+void lc_hip_stream_destroy(uint64_t device, uint64_t stream);
+int lc_hip_stream_synchronize(uint64_t device, uint64_t stream);
+/* Runs fn(user) on a backend thread after all work enqueued so far on the stream completed. */
+int lc_hip_stream_add_callback(uint64_t device, uint64_t stream, void (*fn)(void *), void *user);
+void *lc_hip_stream_native(uint64_t device, uint64_t stream);
+
+/* ---- events: timeline fences (device_interface.h:170-178; CUDAEvent, cuda_event.cpp:21-63) ---- */
+uint64_t lc_hip_event_create(uint64_t device);
+void lc_hip_event_destroy(uint64_t device, uint64_t event);
+int lc_hip_event_signal(uint64_t device, uint64_t event, uint64_t stream, uint64_t value);
+int lc_hip_event_wait(uint64_t device, uint64_t event, uint64_t stream, uint64_t value);
+int lc_hip_event_synchronize(uint64_t device, uint64_t event, uint64_t value);
+int lc_hip_event_is_completed(uint64_t device, uint64_t event, uint64_t value);
+
+/* ---- meshes (BLAS): create_mesh / destroy_mesh (device_interface.h:180-181) and
+   MeshBuildCommand (command.h:407-444): vertex buffer byte range and stride (a multiple of 16,
+   src/runtime/rtx/mesh.cpp:42-49), triangle buffer byte range of uint32 triples. ---- */
+uint64_t lc_hip_mesh_create(uint64_t device, const lc_hip_accel_option *option);
+void lc_hip_mesh_destroy(uint64_t device, uint64_t mesh);
+int lc_hip_mesh_build(uint64_t device, uint64_t stream, uint64_t mesh, uint32_t request, uint64_t vertex_buffer,
+                      size_t vertex_offset, size_t vertex_size, size_t vertex_stride, uint64_t triangle_buffer,
+                      size_t triangle_offset, size_t triangle_size);
+
+/* ---- accels (TLAS): create_accel / destroy_accel (device_interface.h:186-187) and
+   AccelBuildCommand (command.h:531-616; semantics of AccelImpl::update,
+   src/rust/luisa_compute_backend_impl/src/cpu/accel.rs:324-447). ---- */
+uint64_t lc_hip_accel_create(uint64_t device, const lc_hip_accel_option *option);
+void lc_hip_accel_destroy(uint64_t device, uint64_t accel);
+int lc_hip_accel_build(uint64_t device, uint64_t stream, uint64_t accel, uint32_t request, uint32_t instance_count,
+                       const lc_hip_accel_modification *modifications, size_t modification_count,
+                       int update_instance_buffer_only);
+/* Instance queries of the DSL ops RAY_TRACING_INSTANCE_TRANSFORM / _USER_ID / _VISIBILITY_MASK
+   (include/luisa/ast/op.h:290-297), answered from the accel's instance table. */
+int lc_hip_accel_instance_info(uint64_t device, uint64_t accel, uint32_t index, float affine[12],
+                               uint32_t *vis_mask, uint32_t *user_id);
+
+/* ---- traversal: the lowering of CallOp::RAY_TRACING_TRACE_CLOSEST / _TRACE_ANY
+   (src/dsl/rtx/accel.cpp:136-150 -> lc_accel_trace_closest / lc_accel_trace_any,
+   src/backends/cuda/cuda_builtin/cuda_device_resource.h:1783-1829) applied to a batch:
+   hits[i] = accel.intersect(rays[i], {.visibility_mask = mask}). Byte offsets into the buffers;
+   rays are lc_hip_ray, hits lc_hip_surface_hit, any-hit results one byte (bool) per ray. ---- */
+int lc_hip_accel_trace_closest(uint64_t device, uint64_t stream, uint64_t accel, uint64_t ray_buffer,
+                               size_t ray_offset, uint64_t hit_buffer, size_t hit_offset, uint32_t ray_count,
+                               uint32_t vis_mask);
+int lc_hip_accel_trace_any(uint64_t device, uint64_t stream, uint64_t accel, uint64_t ray_buffer, size_t ray_offset,
+                           uint64_t hit_buffer, size_t hit_offset, uint32_t ray_count, uint32_t vis_mask);
+/* Counter-mode run of the same traversal: total BVH nodes visited and triangles tested. */
+int lc_hip_accel_trace_count(uint64_t device, uint64_t stream, uint64_t accel, uint64_t ray_buffer, size_t ray_offset,
+                             uint32_t ray_count, uint32_t vis_mask, int any, uint64_t *nodes_visited,
+                             uint64_t *triangles_tested);
+
+/* ---- statistics, tuning and timing (no reference counterpart: reporting for this backend) ---- */
+typedef struct lc_hip_bvh_stats {
+    uint32_t node_count;
+    uint32_t leaf_primitive_count;
+    uint32_t depth;
+    uint32_t primitive_count;
+    float bounds[6];
+    uint64_t node_bytes;
+    uint64_t primitive_bytes;
+} lc_hip_bvh_stats;
+int lc_hip_mesh_stats(uint64_t device, uint64_t mesh, lc_hip_bvh_stats *out);
+int lc_hip_accel_stats(uint64_t device, uint64_t accel, lc_hip_bvh_stats *out);
+/* kind 0 = closest, 1 = any. block in {64,128,256}; lds_stack in {8,16,24}; steps >= 1;
+   blocks_per_cu 0 = occupancy limit. */
+int lc_hip_trace_set_config(uint64_t device, int kind, int block, int lds_stack, int steps, int blocks_per_cu);
+int lc_hip_trace_get_config(uint64_t device, int kind, int *block, int *lds_stack, int *steps, int *blocks_per_cu);
+/* Build knob: 1 = also open multi-triangle leaf candidates while a node has free slots. */
+int lc_hip_build_set_split_small(uint64_t device, int enable);
+/* Kernel timing with HIP events recorded on the launch stream around each build/trace. */
+int lc_hip_timing_enable(uint64_t device, int enable);
+int lc_hip_timing_get(uint64_t device, const char *name, double *total_ms, uint64_t *launches);
+int lc_hip_timing_reset(uint64_t device);
+
+/* ---- synthetic workloads (PCG32, O'Neill 2014; initstate 42, initseq = workload number) ---- */
+void lc_hip_workload_icosphere_counts(int level, uint32_t *vertex_count, uint32_t *triangle_count);
+void lc_hip_workload_icosphere(int level, float *vertices4, uint32_t *triangles);
+void lc_hip_workload_rays_orbit_to_ball(uint64_t initstate, uint64_t initseq, uint64_t ray_count, float orbit,
+                                        lc_hip_ray *rays);
+void lc_hip_workload_soup(uint64_t initstate, uint64_t initseq, uint64_t triangle_count, float extent, float sigma,
+                          float *vertices4, uint32_t *triangles);
+void lc_hip_workload_rays_box_sphere(uint64_t initstate, uint64_t initseq, uint64_t skip, uint64_t ray_count, float lo,
+                                     float hi, lc_hip_ray *rays);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

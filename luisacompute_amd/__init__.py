@@ -1,0 +1,560 @@
+"""luisacompute_amd -- MI355X (gfx950) ray-tracing backend for the LuisaCompute programming model.
+
+Python host mirror of the reference runtime's ray-tracing path over the C ABI of
+lc-backend-hip.so (include/lc_hip.h). Names and argument meaning follow the reference:
+
+    reference (C++)                                   here
+    Context::create_device("cuda")                   Device() / create_device("hip")
+    device.create_stream()                           Device.create_stream()
+    device.create_buffer<T>(n)                       Device.create_buffer(stride, n)
+    stream << buffer.copy_from(p) << synchronize()   stream << buf.copy_from(a) ; stream.synchronize()
+    device.create_mesh(vb, tb, option)               Device.create_mesh(vb, tb, option)
+    device.create_accel(option)                      Device.create_accel(option)
+    accel.emplace_back(mesh, transform, mask, ...)   Accel.emplace_back(...)
+    stream << mesh.build() << accel.build()          same
+    accel.intersect(ray) / intersect_any(ray)        Accel.trace_closest / trace_any over ray buffers
+(include/luisa/runtime/{context,device,stream}.h, include/luisa/runtime/rtx/{mesh,accel}.h,
+src/runtime/rtx/accel.cpp:68-257, src/py/luisa/accel.py:58-228).
+
+The HIP library is required: there is no CPU fallback. Importing works without a GPU; creating a
+Device on a machine without one raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "lc-backend-hip.so")
+INVALID = 0xFFFFFFFFFFFFFFFF
+
+RAY_DTYPE = np.dtype([("origin", "<f4", (3,)), ("t_min", "<f4"), ("direction", "<f4", (3,)), ("t_max", "<f4")],
+                     align=True)
+HIT_DTYPE = np.dtype([("inst", "<u4"), ("prim", "<u4"), ("bary", "<f4", (2,)), ("committed_ray_t", "<f4"),
+                      ("_pad", "<u4")])
+assert RAY_DTYPE.itemsize == 32 and HIT_DTYPE.itemsize == 24
+
+
+class LCError(RuntimeError):
+    pass
+
+
+class AccelOption(ctypes.Structure):
+    """AccelOption (include/luisa/runtime/rhi/resource.h:94-126)."""
+    _fields_ = [("hint", ctypes.c_uint32), ("allow_compaction", ctypes.c_uint8), ("allow_update", ctypes.c_uint8),
+                ("_pad", ctypes.c_uint8 * 2)]
+    FAST_TRACE = 0
+    FAST_BUILD = 1
+
+
+class Modification(ctypes.Structure):
+    """AccelBuildCommand::Modification (include/luisa/runtime/rhi/command.h:534-551)."""
+    _fields_ = [("index", ctypes.c_uint32), ("user_id", ctypes.c_uint32), ("flags", ctypes.c_uint32),
+                ("vis_mask", ctypes.c_uint32), ("affine", ctypes.c_float * 12), ("primitive", ctypes.c_uint64),
+                ("_pad", ctypes.c_uint64)]
+    flag_primitive = 1
+    flag_transform = 2
+    flag_opaque_on = 4
+    flag_opaque_off = 8
+    flag_visibility = 16
+    flag_user_id = 32
+
+
+class BvhStats(ctypes.Structure):
+    _fields_ = [("node_count", ctypes.c_uint32), ("leaf_primitive_count", ctypes.c_uint32), ("depth", ctypes.c_uint32),
+                ("primitive_count", ctypes.c_uint32), ("bounds", ctypes.c_float * 6), ("node_bytes", ctypes.c_uint64),
+                ("primitive_bytes", ctypes.c_uint64)]
+
+
+assert ctypes.sizeof(Modification) == 80 and ctypes.sizeof(AccelOption) == 8
+
+_lib = None
+
+# name -> (restype, argtypes)
+_U64, _U32, _I32, _SZ, _P, _F = ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int32, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_float
+_SIGNATURES = {
+    "lc_hip_last_error": (ctypes.c_char_p, []),
+    "lc_hip_device_count": (ctypes.c_int, []),
+    "lc_hip_device_create": (_U64, [_I32]),
+    "lc_hip_device_destroy": (None, [_U64]),
+    "lc_hip_device_warp_size": (_U32, [_U64]),
+    "lc_hip_device_native": (ctypes.c_int, [_U64]),
+    "lc_hip_device_query": (ctypes.c_int, [_U64, ctypes.c_char_p, ctypes.c_char_p, _SZ]),
+    "lc_hip_buffer_create": (_U64, [_U64, _SZ, _SZ, ctypes.POINTER(_P)]),
+    "lc_hip_buffer_destroy": (None, [_U64, _U64]),
+    "lc_hip_buffer_upload": (ctypes.c_int, [_U64, _U64, _U64, _SZ, _SZ, _P]),
+    "lc_hip_buffer_download": (ctypes.c_int, [_U64, _U64, _U64, _SZ, _SZ, _P]),
+    "lc_hip_buffer_copy": (ctypes.c_int, [_U64, _U64, _U64, _SZ, _U64, _SZ, _SZ]),
+    "lc_hip_stream_create": (_U64, [_U64, _U32]),
+    "lc_hip_stream_destroy": (None, [_U64, _U64]),
+    "lc_hip_stream_synchronize": (ctypes.c_int, [_U64, _U64]),
+    "lc_hip_stream_add_callback": (ctypes.c_int, [_U64, _U64, _P, _P]),
+    "lc_hip_stream_native": (_P, [_U64, _U64]),
+    "lc_hip_event_create": (_U64, [_U64]),
+    "lc_hip_event_destroy": (None, [_U64, _U64]),
+    "lc_hip_event_signal": (ctypes.c_int, [_U64, _U64, _U64, _U64]),
+    "lc_hip_event_wait": (ctypes.c_int, [_U64, _U64, _U64, _U64]),
+    "lc_hip_event_synchronize": (ctypes.c_int, [_U64, _U64, _U64]),
+    "lc_hip_event_is_completed": (ctypes.c_int, [_U64, _U64, _U64]),
+    "lc_hip_mesh_create": (_U64, [_U64, ctypes.POINTER(AccelOption)]),
+    "lc_hip_mesh_destroy": (None, [_U64, _U64]),
+    "lc_hip_mesh_build": (ctypes.c_int, [_U64, _U64, _U64, _U32, _U64, _SZ, _SZ, _SZ, _U64, _SZ, _SZ]),
+    "lc_hip_accel_create": (_U64, [_U64, ctypes.POINTER(AccelOption)]),
+    "lc_hip_accel_destroy": (None, [_U64, _U64]),
+    "lc_hip_accel_build": (ctypes.c_int, [_U64, _U64, _U64, _U32, _U32, ctypes.POINTER(Modification), _SZ, ctypes.c_int]),
+    "lc_hip_accel_instance_info": (ctypes.c_int, [_U64, _U64, _U32, ctypes.POINTER(_F), ctypes.POINTER(_U32), ctypes.POINTER(_U32)]),
+    "lc_hip_accel_trace_closest": (ctypes.c_int, [_U64, _U64, _U64, _U64, _SZ, _U64, _SZ, _U32, _U32]),
+    "lc_hip_accel_trace_any": (ctypes.c_int, [_U64, _U64, _U64, _U64, _SZ, _U64, _SZ, _U32, _U32]),
+    "lc_hip_accel_trace_count": (ctypes.c_int, [_U64, _U64, _U64, _U64, _SZ, _U32, _U32, ctypes.c_int,
+                                                ctypes.POINTER(_U64), ctypes.POINTER(_U64)]),
+    "lc_hip_mesh_stats": (ctypes.c_int, [_U64, _U64, ctypes.POINTER(BvhStats)]),
+    "lc_hip_accel_stats": (ctypes.c_int, [_U64, _U64, ctypes.POINTER(BvhStats)]),
+    "lc_hip_trace_set_config": (ctypes.c_int, [_U64, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "lc_hip_trace_get_config": (ctypes.c_int, [_U64, ctypes.c_int] + [ctypes.POINTER(ctypes.c_int)] * 4),
+    "lc_hip_build_set_split_small": (ctypes.c_int, [_U64, ctypes.c_int]),
+    "lc_hip_timing_enable": (ctypes.c_int, [_U64, ctypes.c_int]),
+    "lc_hip_timing_get": (ctypes.c_int, [_U64, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_U64)]),
+    "lc_hip_timing_reset": (ctypes.c_int, [_U64]),
+    "lc_hip_workload_icosphere_counts": (None, [ctypes.c_int, ctypes.POINTER(_U32), ctypes.POINTER(_U32)]),
+    "lc_hip_workload_icosphere": (None, [ctypes.c_int, _P, _P]),
+    "lc_hip_workload_rays_orbit_to_ball": (None, [_U64, _U64, _U64, _F, _P]),
+    "lc_hip_workload_soup": (None, [_U64, _U64, _U64, _F, _F, _P, _P]),
+    "lc_hip_workload_rays_box_sphere": (None, [_U64, _U64, _U64, _U64, _F, _F, _P]),
+}
+EXPORTED_SYMBOLS = sorted(_SIGNATURES)
+
+
+def lib():
+    """Loads lc-backend-hip.so (raises if it has not been built: no fallback exists)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise LCError(f"{LIB_PATH} is missing: build it with `python -m luisacompute_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc, what=""):
+    if rc != 0:
+        raise LCError(f"{what}: {lib().lc_hip_last_error().decode()} (status {rc})")
+
+
+def _handle(h, what):
+    if h == INVALID:
+        raise LCError(f"{what}: {lib().lc_hip_last_error().decode()}")
+    return h
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+# ------------------------------------------------------------------ affine helpers
+def affine_from_matrix(m):
+    """Row-major 3x4 affine from a 4x4 matrix given as numpy [row][col] (command.h:557-571 encodes
+    a column-major float4x4 the same way: affine[4*r+c] = M[r][c])."""
+    m = np.asarray(m, np.float32)
+    if m.shape == (12,):
+        return m.copy()
+    if m.shape == (3, 4):
+        return m.reshape(12).copy()
+    return m[:3, :4].reshape(12).copy()
+
+
+IDENTITY = np.array([1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0], np.float32)
+
+
+# ------------------------------------------------------------------ commands
+class Command:
+    def execute(self, stream):
+        raise NotImplementedError
+
+
+class _Upload(Command):
+    def __init__(self, buf, array, offset):
+        self.buf, self.array, self.offset = buf, np.ascontiguousarray(array), offset
+
+    def execute(self, stream):
+        _check(lib().lc_hip_buffer_upload(stream.device.handle, stream.handle, self.buf.handle, self.offset,
+                                          self.array.nbytes, _ptr(self.array)), "buffer upload")
+
+
+class _Download(Command):
+    def __init__(self, buf, array, offset):
+        if not array.flags.c_contiguous:
+            raise LCError("download destination must be contiguous")
+        self.buf, self.array, self.offset = buf, array, offset
+
+    def execute(self, stream):
+        _check(lib().lc_hip_buffer_download(stream.device.handle, stream.handle, self.buf.handle, self.offset,
+                                            self.array.nbytes, _ptr(self.array)), "buffer download")
+
+
+class _Copy(Command):
+    def __init__(self, src, src_off, dst, dst_off, size):
+        self.a = (src, src_off, dst, dst_off, size)
+
+    def execute(self, stream):
+        src, so, dst, do, size = self.a
+        _check(lib().lc_hip_buffer_copy(stream.device.handle, stream.handle, src.handle, so, dst.handle, do, size),
+               "buffer copy")
+
+
+class _MeshBuild(Command):
+    def __init__(self, mesh, request):
+        self.mesh, self.request = mesh, request
+
+    def execute(self, stream):
+        m = self.mesh
+        _check(lib().lc_hip_mesh_build(stream.device.handle, stream.handle, m.handle, self.request, m.vb.handle,
+                                       m.v_offset, m.v_size, m.v_stride, m.tb.handle, m.t_offset, m.t_size),
+               "mesh build")
+
+
+class _AccelBuild(Command):
+    def __init__(self, accel, request, mods, instance_count, update_only):
+        self.accel, self.request, self.mods, self.n, self.update_only = accel, request, mods, instance_count, update_only
+
+    def execute(self, stream):
+        arr = (Modification * max(1, len(self.mods)))(*self.mods)
+        _check(lib().lc_hip_accel_build(stream.device.handle, stream.handle, self.accel.handle, self.request, self.n,
+                                        arr, len(self.mods), 1 if self.update_only else 0), "accel build")
+
+
+class _Synchronize(Command):
+    def execute(self, stream):
+        stream.synchronize()
+
+
+def synchronize():
+    return _Synchronize()
+
+
+# ------------------------------------------------------------------ resources
+class Device:
+    """DeviceInterface owner (one GPU). index < 0 selects the default device."""
+
+    def __init__(self, index=0):
+        self.handle = _handle(lib().lc_hip_device_create(index), "device create")
+
+    def __del__(self):
+        if getattr(self, "handle", None) is not None and _lib is not None:
+            lib().lc_hip_device_destroy(self.handle)
+            self.handle = None
+
+    def warp_size(self):
+        return lib().lc_hip_device_warp_size(self.handle)
+
+    def query(self, key):
+        out = ctypes.create_string_buffer(256)
+        _check(lib().lc_hip_device_query(self.handle, key.encode(), out, 256), f"query {key}")
+        return out.value.decode()
+
+    def create_stream(self):
+        return Stream(self)
+
+    def create_buffer(self, stride, count):
+        return Buffer(self, stride, count)
+
+    def create_buffer_from(self, array, stream=None):
+        array = np.ascontiguousarray(array)
+        stride = array.strides[0] if array.ndim > 0 else array.itemsize
+        buf = Buffer(self, stride, array.shape[0] if array.ndim > 0 else 1)
+        s = stream or Stream(self)
+        s << buf.copy_from(array)
+        return buf
+
+    def create_mesh(self, vertex_buffer, triangle_buffer, option=None, vertex_stride=16, vertex_offset=0,
+                    vertex_size=None, triangle_offset=0, triangle_size=None):
+        return Mesh(self, vertex_buffer, triangle_buffer, option, vertex_stride, vertex_offset, vertex_size,
+                    triangle_offset, triangle_size)
+
+    def create_accel(self, option=None):
+        return Accel(self, option)
+
+    def create_event(self):
+        return Event(self)
+
+    # configuration / measurement of this backend
+    def set_trace_config(self, kind, block, lds_stack, steps, blocks_per_cu=0):
+        _check(lib().lc_hip_trace_set_config(self.handle, 1 if kind == "any" else 0, block, lds_stack, steps,
+                                             blocks_per_cu), "trace config")
+
+    def trace_config(self, kind):
+        v = [ctypes.c_int() for _ in range(4)]
+        _check(lib().lc_hip_trace_get_config(self.handle, 1 if kind == "any" else 0, *[ctypes.byref(x) for x in v]),
+               "trace config")
+        return dict(block=v[0].value, lds_stack=v[1].value, steps=v[2].value, blocks_per_cu=v[3].value)
+
+    def set_build_split_small(self, enable):
+        _check(lib().lc_hip_build_set_split_small(self.handle, 1 if enable else 0), "build config")
+
+    def enable_timing(self, on=True):
+        _check(lib().lc_hip_timing_enable(self.handle, 1 if on else 0), "timing")
+
+    def timing(self, name):
+        ms, n = ctypes.c_double(), ctypes.c_uint64()
+        _check(lib().lc_hip_timing_get(self.handle, name.encode(), ctypes.byref(ms), ctypes.byref(n)), "timing")
+        return ms.value, n.value
+
+    def reset_timing(self):
+        _check(lib().lc_hip_timing_reset(self.handle), "timing")
+
+
+def create_device(name="hip", index=0):
+    if name not in ("hip", "HIP"):
+        raise LCError(f"backend '{name}' is not provided by this package")
+    return Device(index)
+
+
+class Stream:
+    def __init__(self, device, tag=1):
+        self.device = device
+        self.handle = _handle(lib().lc_hip_stream_create(device.handle, tag), "stream create")
+        self._callbacks = []
+
+    def __del__(self):
+        if getattr(self, "handle", None) is not None and _lib is not None and self.device.handle is not None:
+            lib().lc_hip_stream_destroy(self.device.handle, self.handle)
+            self.handle = None
+
+    def __lshift__(self, cmd):
+        if callable(cmd) and not isinstance(cmd, Command):
+            self.add_callback(cmd)
+        else:
+            cmd.execute(self)
+        return self
+
+    def synchronize(self):
+        _check(lib().lc_hip_stream_synchronize(self.device.handle, self.handle), "stream synchronize")
+        self._callbacks = []
+
+    def add_callback(self, fn):
+        cb = ctypes.CFUNCTYPE(None, ctypes.c_void_p)(lambda _: fn())
+        self._callbacks.append(cb)
+        _check(lib().lc_hip_stream_add_callback(self.device.handle, self.handle, ctypes.cast(cb, ctypes.c_void_p), None),
+               "stream callback")
+
+    def native(self):
+        return lib().lc_hip_stream_native(self.device.handle, self.handle)
+
+
+class Event:
+    """Timeline event (include/luisa/runtime/event.h): signal/wait/synchronize by value."""
+
+    def __init__(self, device):
+        self.device = device
+        self.handle = _handle(lib().lc_hip_event_create(device.handle), "event create")
+
+    def __del__(self):
+        if getattr(self, "handle", None) is not None and _lib is not None and self.device.handle is not None:
+            lib().lc_hip_event_destroy(self.device.handle, self.handle)
+            self.handle = None
+
+    def signal(self, stream, value):
+        _check(lib().lc_hip_event_signal(self.device.handle, self.handle, stream.handle, value), "event signal")
+
+    def wait(self, stream, value):
+        _check(lib().lc_hip_event_wait(self.device.handle, self.handle, stream.handle, value), "event wait")
+
+    def synchronize(self, value):
+        _check(lib().lc_hip_event_synchronize(self.device.handle, self.handle, value), "event synchronize")
+
+    def is_completed(self, value):
+        return lib().lc_hip_event_is_completed(self.device.handle, self.handle, value) == 1
+
+
+class Buffer:
+    def __init__(self, device, stride, count):
+        self.device, self.stride, self.count = device, int(stride), int(count)
+        self.nbytes = self.stride * self.count
+        p = ctypes.c_void_p()
+        self.handle = _handle(lib().lc_hip_buffer_create(device.handle, self.stride, self.count, ctypes.byref(p)),
+                              "buffer create")
+        self.native = p.value
+
+    def __del__(self):
+        if getattr(self, "handle", None) is not None and _lib is not None and self.device.handle is not None:
+            lib().lc_hip_buffer_destroy(self.device.handle, self.handle)
+            self.handle = None
+
+    def copy_from(self, array, offset=0):
+        return _Upload(self, array, offset)
+
+    def copy_to(self, array, offset=0):
+        return _Download(self, array, offset)
+
+    def copy_from_buffer(self, src, src_offset=0, dst_offset=0, size=None):
+        return _Copy(src, src_offset, self, dst_offset, self.nbytes - dst_offset if size is None else size)
+
+
+class Mesh:
+    """BLAS over a vertex buffer (16-byte stride multiple) and a uint32 triangle buffer."""
+
+    def __init__(self, device, vb, tb, option=None, vertex_stride=16, vertex_offset=0, vertex_size=None,
+                 triangle_offset=0, triangle_size=None):
+        self.device, self.vb, self.tb = device, vb, tb
+        self.v_stride, self.v_offset = vertex_stride, vertex_offset
+        self.v_size = vb.nbytes - vertex_offset if vertex_size is None else vertex_size
+        self.t_offset = triangle_offset
+        self.t_size = tb.nbytes - triangle_offset if triangle_size is None else triangle_size
+        opt = option or AccelOption(AccelOption.FAST_TRACE, 1, 0)
+        self.handle = _handle(lib().lc_hip_mesh_create(device.handle, ctypes.byref(opt)), "mesh create")
+
+    def __del__(self):
+        if getattr(self, "handle", None) is not None and _lib is not None and self.device.handle is not None:
+            lib().lc_hip_mesh_destroy(self.device.handle, self.handle)
+            self.handle = None
+
+    def build(self, request=1):
+        return _MeshBuild(self, request)
+
+    def stats(self):
+        s = BvhStats()
+        _check(lib().lc_hip_mesh_stats(self.device.handle, self.handle, ctypes.byref(s)), "mesh stats")
+        return s
+
+
+class Accel:
+    """TLAS with the reference's modification protocol (src/runtime/rtx/accel.cpp:68-257)."""
+
+    def __init__(self, device, option=None):
+        self.device = device
+        opt = option or AccelOption(AccelOption.FAST_TRACE, 1, 0)
+        self.handle = _handle(lib().lc_hip_accel_create(device.handle, ctypes.byref(opt)), "accel create")
+        self._meshes = []
+        self._mods = {}
+
+    def __del__(self):
+        if getattr(self, "handle", None) is not None and _lib is not None and self.device.handle is not None:
+            lib().lc_hip_accel_destroy(self.device.handle, self.handle)
+            self.handle = None
+
+    def __len__(self):
+        return len(self._meshes)
+
+    def _mod(self, index):
+        m = self._mods.get(index)
+        if m is None:
+            m = Modification()
+            m.index = index
+            self._mods[index] = m
+        return m
+
+    def emplace_back(self, mesh, transform=IDENTITY, visibility_mask=0xFF, opaque=True, user_id=0):
+        index = len(self._meshes)
+        self._meshes.append(mesh)
+        self.set(index, mesh, transform, visibility_mask, opaque, user_id)
+        return index
+
+    def set(self, index, mesh, transform=IDENTITY, visibility_mask=0xFF, opaque=True, user_id=0):
+        self._meshes[index] = mesh
+        m = self._mod(index)
+        m.flags = Modification.flag_primitive | Modification.flag_transform | Modification.flag_visibility | \
+            Modification.flag_user_id | (Modification.flag_opaque_on if opaque else Modification.flag_opaque_off)
+        m.primitive = mesh.handle
+        m.affine[:] = affine_from_matrix(transform).tolist()
+        m.vis_mask = visibility_mask & 0xFF
+        m.user_id = user_id
+
+    def pop_back(self):
+        self._meshes.pop()
+        self._mods.pop(len(self._meshes), None)
+
+    def set_transform_on_update(self, index, transform):
+        m = self._mod(index)
+        m.flags |= Modification.flag_transform
+        m.affine[:] = affine_from_matrix(transform).tolist()
+
+    def set_visibility_on_update(self, index, visibility_mask):
+        m = self._mod(index)
+        m.flags |= Modification.flag_visibility
+        m.vis_mask = visibility_mask & 0xFF
+
+    def set_user_id_on_update(self, index, user_id):
+        m = self._mod(index)
+        m.flags |= Modification.flag_user_id
+        m.user_id = user_id
+
+    def _take_mods(self):
+        mods = [self._mods[k] for k in sorted(self._mods)]
+        self._mods = {}
+        return mods
+
+    def build(self, request=1):
+        if not self._meshes:
+            raise LCError("Building acceleration structure without instances.")  # accel.cpp:72-73
+        return _AccelBuild(self, request, self._take_mods(), len(self._meshes), False)
+
+    def update_instance_buffer(self):
+        return _AccelBuild(self, 0, self._take_mods(), len(self._meshes), True)
+
+    def instance_info(self, index):
+        a = (ctypes.c_float * 12)()
+        vis, uid = ctypes.c_uint32(), ctypes.c_uint32()
+        _check(lib().lc_hip_accel_instance_info(self.device.handle, self.handle, index, a, ctypes.byref(vis),
+                                                ctypes.byref(uid)), "instance info")
+        return np.array(a[:], np.float32), vis.value, uid.value
+
+    def stats(self):
+        s = BvhStats()
+        _check(lib().lc_hip_accel_stats(self.device.handle, self.handle, ctypes.byref(s)), "accel stats")
+        return s
+
+    # CallOp::RAY_TRACING_TRACE_CLOSEST / _TRACE_ANY over a batch of rays
+    def trace_closest(self, stream, ray_buffer, hit_buffer, count=None, vis_mask=0xFF, ray_offset=0, hit_offset=0):
+        n = ray_buffer.count if count is None else count
+        _check(lib().lc_hip_accel_trace_closest(self.device.handle, stream.handle, self.handle, ray_buffer.handle,
+                                                ray_offset, hit_buffer.handle, hit_offset, n, vis_mask), "trace_closest")
+
+    def trace_any(self, stream, ray_buffer, hit_buffer, count=None, vis_mask=0xFF, ray_offset=0, hit_offset=0):
+        n = ray_buffer.count if count is None else count
+        _check(lib().lc_hip_accel_trace_any(self.device.handle, stream.handle, self.handle, ray_buffer.handle,
+                                            ray_offset, hit_buffer.handle, hit_offset, n, vis_mask), "trace_any")
+
+    def trace_count(self, stream, ray_buffer, count=None, vis_mask=0xFF, any_hit=False, ray_offset=0):
+        n = ray_buffer.count if count is None else count
+        a, b = ctypes.c_uint64(), ctypes.c_uint64()
+        _check(lib().lc_hip_accel_trace_count(self.device.handle, stream.handle, self.handle, ray_buffer.handle,
+                                              ray_offset, n, vis_mask, 1 if any_hit else 0, ctypes.byref(a),
+                                              ctypes.byref(b)), "trace_count")
+        return a.value, b.value
+
+
+# ------------------------------------------------------------------ synthetic workloads
+def icosphere(level):
+    nv, nt = ctypes.c_uint32(), ctypes.c_uint32()
+    lib().lc_hip_workload_icosphere_counts(level, ctypes.byref(nv), ctypes.byref(nt))
+    v = np.zeros((nv.value, 4), np.float32)
+    t = np.zeros((nt.value, 3), np.uint32)
+    lib().lc_hip_workload_icosphere(level, _ptr(v), _ptr(t))
+    return v, t
+
+
+def rays_orbit_to_ball(n, initstate=42, initseq=1, orbit=3.0):
+    r = np.zeros(n, RAY_DTYPE)
+    lib().lc_hip_workload_rays_orbit_to_ball(initstate, initseq, n, orbit, _ptr(r))
+    return r
+
+
+def soup(ntris, initstate=42, initseq=2, extent=100.0, sigma=0.5):
+    v = np.zeros((3 * ntris, 4), np.float32)
+    t = np.zeros((ntris, 3), np.uint32)
+    lib().lc_hip_workload_soup(initstate, initseq, ntris, extent, sigma, _ptr(v), _ptr(t))
+    return v, t
+
+
+def rays_box_sphere(n, initstate=42, initseq=2, skip=0, lo=0.0, hi=100.0):
+    r = np.zeros(n, RAY_DTYPE)
+    lib().lc_hip_workload_rays_box_sphere(initstate, initseq, skip, n, lo, hi, _ptr(r))
+    return r
+
+
+def rays_as_floats(rays):
+    """(n, 8) float32 view of a RAY_DTYPE array."""
+    return np.ascontiguousarray(rays).view(np.float32).reshape(-1, 8)

@@ -1,0 +1,71 @@
+// lc_hip_bvh.h -- in-HBM layouts shared by the BVH build and traversal kernels (gfx950).
+//
+// A BLAS (one Mesh) lives in two device arrays:
+//   nodes : WideNode[node_count]   80 B each, 16-B aligned, root at index 0
+//   tris  : TriRecord[tri_count]   48 B each, leaf triangles copied out of the user buffers
+// A TLAS (one Accel) is the same WideNode layout whose leaf "triangles" are instance indices,
+// plus an InstanceRecord array.
+//
+// WideNode is an 8-wide node with child boxes quantised to 8 bits per plane relative to the
+// node origin p and a per-axis power-of-two scale 2^(e-127) (the compressed layout of Ylitie,
+// Karras, Laine, "Efficient Incoherent Ray Traversal on GPUs Through Compressed Wide BVHs",
+// HPG 2017). Slot s of a node holds the child lying toward octant s (bit k of s set = positive
+// along axis k); a ray visits slots in order i ^ oct(ray), which approximates front to back.
+#pragma once
+
+#include <stdint.h>
+
+namespace lchip {
+
+// Word layout of a WideNode as five 16-byte vectors:
+//   w0 = { p.x, p.y, p.z (float bits), ex | ey << 8 | ez << 16 | imask << 24 }
+//   w1 = { child_base, tri_base, meta[0..3], meta[4..7] }
+//   w2 = { qlo_x[0..3], qlo_x[4..7], qlo_y[0..3], qlo_y[4..7] }
+//   w3 = { qlo_z[0..3], qlo_z[4..7], qhi_x[0..3], qhi_x[4..7] }
+//   w4 = { qhi_y[0..3], qhi_y[4..7], qhi_z[0..3], qhi_z[4..7] }
+// imask bit s: slot s is an internal child, stored at child_base + popcount(imask & ((1<<s)-1)).
+// meta[s] == 0: empty slot. Leaf slot: meta = count << 5 | offset, its triangles are
+// tris[tri_base + offset .. + count). Internal slot: meta = 0xFF.
+struct alignas(16) WideNode {
+    uint32_t w[20];
+};
+static_assert(sizeof(WideNode) == 80, "WideNode must be 80 bytes");
+
+// Leaf triangle: three vertices as float4; v0.w holds the mesh-local primitive index bits.
+struct alignas(16) TriRecord {
+    float v0[4];
+    float v1[4];
+    float v2[4];
+};
+static_assert(sizeof(TriRecord) == 48, "TriRecord must be 48 bytes");
+
+// Instance as seen by traversal: world->object affine (row-major 3x4), BLAS pointers.
+struct alignas(16) InstanceRecord {
+    float inv[12];       // world -> object
+    float fwd[12];       // object -> world (reported by instance_transform)
+    uint64_t blas_nodes; // device address of the BLAS WideNode array (0 = no geometry)
+    uint64_t blas_tris;  // device address of the BLAS TriRecord array
+    uint32_t vis_mask;   // instance visibility (default 0xff)
+    uint32_t user_id;
+    uint32_t flags;      // bit0: opaque
+    uint32_t blas_depth; // wide depth of the BLAS
+};
+static_assert(sizeof(InstanceRecord) == 128, "InstanceRecord must be 128 bytes");
+
+constexpr uint32_t kLeafMax = 3;         // triangles per leaf slot (<= 3 keeps a node's tris in 24 bits)
+constexpr uint32_t kCollapseLaunches = 98;   // wide levels launched per build (depth <= 96)
+constexpr uint32_t kMaxCollapseLevels = 100; // level_count entries
+constexpr uint32_t kLeafBit = 0x80000000u; // binary-node reference: leaf (sorted position)
+
+// Per-build device-side counters (one per BLAS/TLAS build).
+struct BuildCounters {
+    uint32_t node_count;  // wide nodes allocated (root = 0)
+    uint32_t tri_count;   // triangle records written
+    uint32_t depth;       // wide levels that held work
+    uint32_t root_ref;    // binary root reference
+    int32_t cbounds[6];   // centroid bounds as order-preserving ints
+    float bounds[6];      // root box lo.xyz, hi.xyz
+    uint32_t level_count[kMaxCollapseLevels];
+};
+
+} // namespace lchip

@@ -1,0 +1,175 @@
+// lc_hip_internal.h -- host-side objects of the HIP backend and the kernel entry points they use.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <condition_variable>
+#include <cstdint>
+#include <deque>
+#include <functional>
+#include <map>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "lc_hip_bvh.h"
+
+namespace lchip {
+
+// ---- kernels (bvh_build.hip) ----
+size_t bvh_scratch_bytes(uint32_t n);
+size_t bvh_node_capacity(uint32_t n);
+hipError_t bvh_build(hipStream_t stream, void *scratch, size_t scratch_bytes, const void *vbase, uint64_t vstride,
+                     uint64_t nverts, const uint32_t *tri, const float4 *boxes, uint32_t n, WideNode *nodes,
+                     TriRecord *tris, uint32_t *prim_out, BuildCounters **counters_out, uint32_t split_small,
+                     int grid_collapse, uint32_t *error_flag);
+
+// ---- kernels (trace.hip) ----
+struct TraceParams {
+    const float4 *rays;   // 2 float4 per ray (Ray layout)
+    uint32_t count;
+    uint32_t vis_mask;    // ray visibility mask
+    uint32_t *ray_counter;
+    void *out_hits;       // 24-B SurfaceHit per ray (closest)
+    uint8_t *out_any;     // 1 byte per ray (any)
+    // single-instance form
+    const WideNode *blas_nodes;
+    const TriRecord *blas_tris;
+    float inst_inv[12];
+    uint32_t inst_mask;
+    // two-level form
+    const WideNode *tlas_nodes;
+    const uint32_t *tlas_prims;
+    const InstanceRecord *instances;
+    uint32_t empty;       // nothing to traverse: every ray misses
+    uint2 *spill;
+    uint32_t spill_depth; // spill entries per lane
+    int steps;            // traversal steps between ray refills
+    unsigned long long *stats; // [nodes visited, triangles tested] when counting
+    uint32_t *error;
+};
+
+bool trace_config_supported(int block, int slds);
+int trace_blocks_per_cu(bool any, bool two, bool count, int block, int slds);
+hipError_t trace_launch(const TraceParams &P, hipStream_t stream, bool any, bool two, bool count, int grid, int block,
+                        int slds);
+
+// ---- host objects ----
+struct TraceConfig {
+    int block = 256;
+    int slds = 16;
+    int steps = 32;
+    int blocks_per_cu = 0; // 0: occupancy limit
+};
+
+struct TimingStat {
+    double total_ms = 0.0;
+    uint64_t launches = 0;
+};
+
+struct Device {
+    int index = 0;
+    int cu_count = 0;
+    std::string name;
+    std::mutex mu;
+    TraceConfig cfg[2]; // [closest, any]
+    uint32_t split_small = 1;
+    // per-device working memory shared by launches (guarded by mu)
+    void *spill = nullptr;
+    size_t spill_bytes = 0;
+    uint32_t *ray_counter = nullptr;
+    uint32_t *error = nullptr;
+    unsigned long long *stats = nullptr;
+    // kernel timing with HIP events on the launch stream
+    bool timing = false;
+    struct Pending {
+        std::string name;
+        hipEvent_t a, b;
+    };
+    std::vector<Pending> pending;
+    std::map<std::string, TimingStat> timings;
+};
+
+struct Buffer {
+    Device *dev = nullptr;
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    size_t elem_stride = 0;
+};
+
+struct Stream {
+    Device *dev = nullptr;
+    hipStream_t s = nullptr;
+    std::mutex mu;
+    // host callbacks run on a worker thread once the GPU has passed their point in the stream
+    std::thread worker;
+    std::mutex qmu;
+    std::condition_variable qcv;
+    std::deque<std::pair<hipEvent_t, std::function<void()>>> q;
+    bool stop = false;
+};
+
+struct HostCounters {
+    uint32_t node_count, tri_count, depth, root_ref;
+    int32_t cbounds[6];
+    float bounds[6];
+    uint32_t tail_level_count; // level_count[kCollapseLaunches]: must be 0
+    uint32_t error;
+};
+
+struct Mesh {
+    Device *dev = nullptr;
+    uint32_t hint = 0;
+    bool allow_compaction = true, allow_update = false;
+    std::mutex mu;
+    uint32_t ntris = 0;
+    size_t node_cap = 0, tri_cap = 0;
+    WideNode *nodes = nullptr;
+    TriRecord *tris = nullptr;
+    void *scratch = nullptr;
+    size_t scratch_bytes = 0;
+    HostCounters *host = nullptr; // pinned
+    hipEvent_t ready = nullptr;   // counters copied to host
+    bool built = false;
+    bool stats_valid = false;
+    HostCounters stats{};
+    uint64_t build_epoch = 0;     // bumps on every build (TLAS records refer to it)
+};
+
+struct InstanceHost {
+    Mesh *mesh = nullptr;
+    float affine[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
+    uint32_t vis_mask = 0xFFu;
+    uint32_t user_id = 0;
+    bool opaque = true;
+};
+
+struct Accel {
+    Device *dev = nullptr;
+    uint32_t hint = 0;
+    bool allow_compaction = true, allow_update = false;
+    std::mutex mu;
+    std::vector<InstanceHost> inst;
+    InstanceRecord *records = nullptr;
+    size_t record_cap = 0;
+    std::vector<InstanceRecord> host_records;
+    // TLAS
+    float4 *boxes = nullptr;
+    size_t box_cap = 0;
+    WideNode *tlas_nodes = nullptr;
+    uint32_t *tlas_prims = nullptr;
+    size_t tlas_cap = 0;
+    void *tlas_scratch = nullptr;
+    size_t tlas_scratch_bytes = 0;
+    HostCounters *tlas_host = nullptr;
+    hipEvent_t tlas_ready = nullptr;
+    bool tlas_valid = false;      // TLAS matches the instance set
+    bool tlas_stats_valid = false;
+    HostCounters tlas_stats{};
+    bool built = false;
+    uint32_t max_blas_depth = 0;
+};
+
+} // namespace lchip

@@ -1,0 +1,964 @@
+// runtime.cpp -- host side of lc-backend-hip.so: devices, buffers, streams, events, mesh/accel
+// builds and trace launches behind the C ABI of include/lc_hip.h.
+//
+// Structure follows the reference's CUDA backend (CUDADevice / CUDAStream / CUDAMesh /
+// CUDAAccel, src/backends/cuda/cuda_device.cpp, cuda_stream.cpp, cuda_mesh.cpp, cuda_accel.cpp)
+// with OptiX replaced by the kernels of bvh_build.hip and trace.hip.
+#include "lc_hip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "lc_hip_internal.h"
+
+using namespace lchip;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg) {
+    g_last_error = msg;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char *what) {
+    g_last_error = std::string(what) + ": " + hipGetErrorString(e);
+    return LC_HIP_ERROR_HIP;
+}
+
+#define LC_HIP_TRY(expr)                                      \
+    do {                                                      \
+        hipError_t _e = (expr);                               \
+        if (_e != hipSuccess) return hip_fail(_e, #expr);     \
+    } while (0)
+
+template<typename T>
+T *obj(uint64_t h) { return (h == 0 || h == LC_HIP_INVALID_HANDLE) ? nullptr : reinterpret_cast<T *>(h); }
+
+template<typename T>
+uint64_t handle(T *p) { return reinterpret_cast<uint64_t>(p); }
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(const Device *d) {
+        hipGetDevice(&prev);
+        if (d && prev != d->index) hipSetDevice(d->index);
+    }
+    ~DeviceGuard() {
+        int cur = -1;
+        hipGetDevice(&cur);
+        if (prev >= 0 && cur != prev) hipSetDevice(prev);
+    }
+};
+
+hipStream_t native_stream(uint64_t s) {
+    Stream *st = obj<Stream>(s);
+    return st ? st->s : nullptr;
+}
+
+// Grows a device allocation to at least `bytes` (contents are not preserved).
+hipError_t ensure(void *&ptr, size_t &cap, size_t bytes) {
+    if (cap >= bytes && ptr) return hipSuccess;
+    if (ptr) {
+        hipError_t e = hipFree(ptr);
+        if (e != hipSuccess) return e;
+        ptr = nullptr;
+        cap = 0;
+    }
+    size_t b = std::max<size_t>(bytes, 256);
+    hipError_t e = hipMalloc(&ptr, b);
+    if (e == hipSuccess) cap = b;
+    return e;
+}
+
+// Timed region on a stream: records HIP events around the launches when timing is on.
+struct TimedRegion {
+    Device *dev;
+    hipStream_t s;
+    const char *name;
+    hipEvent_t a = nullptr, b = nullptr;
+    TimedRegion(Device *d, hipStream_t st, const char *n) : dev(d), s(st), name(n) {
+        if (dev->timing) {
+            hipEventCreate(&a);
+            hipEventCreate(&b);
+            hipEventRecord(a, s);
+        }
+    }
+    ~TimedRegion() {
+        if (a) {
+            hipEventRecord(b, s);
+            dev->pending.push_back({name, a, b});
+        }
+    }
+};
+
+void collect_timings(Device *dev) {
+    for (auto &p : dev->pending) {
+        hipEventSynchronize(p.b);
+        float ms = 0.0f;
+        hipEventElapsedTime(&ms, p.a, p.b);
+        auto &t = dev->timings[p.name];
+        t.total_ms += ms;
+        t.launches++;
+        hipEventDestroy(p.a);
+        hipEventDestroy(p.b);
+    }
+    dev->pending.clear();
+}
+
+void stream_worker(Stream *st) {
+    for (;;) {
+        std::pair<hipEvent_t, std::function<void()>> item;
+        {
+            std::unique_lock<std::mutex> lk(st->qmu);
+            st->qcv.wait(lk, [&] { return st->stop || !st->q.empty(); });
+            if (st->q.empty()) return;
+            item = std::move(st->q.front());
+            st->q.pop_front();
+        }
+        hipEventSynchronize(item.first);
+        hipEventDestroy(item.first);
+        item.second();
+    }
+}
+
+// Reads back the counters of a build (waits for the copy enqueued after the build).
+int mesh_stats_sync(Mesh *m) {
+    if (!m->built) return fail(LC_HIP_ERROR_NOT_BUILT, "mesh is not built");
+    if (!m->stats_valid) {
+        if (m->ntris > 0) {
+            LC_HIP_TRY(hipEventSynchronize(m->ready));
+            m->stats = *m->host;
+            if (m->stats.tail_level_count != 0)
+                return fail(LC_HIP_ERROR_DEVICE_FAULT, "BVH collapse exceeded the level bound");
+            if (m->stats.tri_count != m->ntris)
+                return fail(LC_HIP_ERROR_DEVICE_FAULT, "BVH build lost primitives");
+            if (m->stats.error & 2u)
+                return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "triangle index out of range of the vertex buffer");
+        } else {
+            std::memset(&m->stats, 0, sizeof(m->stats));
+        }
+        m->stats_valid = true;
+    }
+    return LC_HIP_OK;
+}
+
+void invert_affine(const float m[12], float out[12]) {
+    double a = m[0], b = m[1], c = m[2], d = m[4], e = m[5], f = m[6], g = m[8], h = m[9], i = m[10];
+    double A = e * i - f * h, B = -(d * i - f * g), C = d * h - e * g;
+    double det = a * A + b * B + c * C;
+    double id = det != 0.0 ? 1.0 / det : 0.0;
+    double r[9] = {A * id, -(b * i - c * h) * id, (b * f - c * e) * id,
+                   B * id, (a * i - c * g) * id, -(a * f - c * d) * id,
+                   C * id, -(a * h - b * g) * id, (a * e - b * d) * id};
+    double tx = m[3], ty = m[7], tz = m[11];
+    for (int row = 0; row < 3; row++) {
+        out[row * 4 + 0] = (float)r[row * 3 + 0];
+        out[row * 4 + 1] = (float)r[row * 3 + 1];
+        out[row * 4 + 2] = (float)r[row * 3 + 2];
+        out[row * 4 + 3] = (float)(-(r[row * 3 + 0] * tx + r[row * 3 + 1] * ty + r[row * 3 + 2] * tz));
+    }
+}
+
+void copy_counters_async(BuildCounters *dctr, HostCounters *host, uint32_t *error, hipStream_t s) {
+    hipMemcpyAsync(host, dctr, offsetof(BuildCounters, level_count), hipMemcpyDeviceToHost, s);
+    hipMemcpyAsync(&host->tail_level_count, &dctr->level_count[kCollapseLaunches], sizeof(uint32_t),
+                   hipMemcpyDeviceToHost, s);
+    hipMemcpyAsync(&host->error, error, sizeof(uint32_t), hipMemcpyDeviceToHost, s);
+}
+
+} // namespace
+
+extern "C" {
+
+const char *lc_hip_last_error(void) { return g_last_error.c_str(); }
+
+int lc_hip_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+uint64_t lc_hip_device_create(int32_t index) {
+    int n = lc_hip_device_count();
+    if (n <= 0) {
+        fail(LC_HIP_ERROR_UNSUPPORTED, "no HIP device");
+        return LC_HIP_INVALID_HANDLE;
+    }
+    if (index < 0 || index >= n) index = 0;
+    auto dev = std::make_unique<Device>();
+    dev->index = index;
+    hipDeviceProp_t prop{};
+    if (hipGetDeviceProperties(&prop, index) != hipSuccess) {
+        fail(LC_HIP_ERROR_HIP, "hipGetDeviceProperties failed");
+        return LC_HIP_INVALID_HANDLE;
+    }
+    dev->cu_count = prop.multiProcessorCount;
+    dev->name = prop.gcnArchName;
+    DeviceGuard g(dev.get());
+    void *p = nullptr;
+    if (hipMalloc(&p, 256) != hipSuccess) {
+        fail(LC_HIP_ERROR_OUT_OF_MEMORY, "device scratch allocation failed");
+        return LC_HIP_INVALID_HANDLE;
+    }
+    hipMemset(p, 0, 256);
+    dev->ray_counter = static_cast<uint32_t *>(p);
+    dev->error = dev->ray_counter + 16;
+    dev->stats = reinterpret_cast<unsigned long long *>(static_cast<char *>(p) + 128);
+    dev->cfg[0] = TraceConfig{256, 16, 32, 0};
+    dev->cfg[1] = TraceConfig{256, 16, 32, 0};
+    return handle(dev.release());
+}
+
+void lc_hip_device_destroy(uint64_t device) {
+    Device *dev = obj<Device>(device);
+    if (!dev) return;
+    DeviceGuard g(dev);
+    hipDeviceSynchronize();
+    collect_timings(dev);
+    if (dev->spill) hipFree(dev->spill);
+    if (dev->ray_counter) hipFree(dev->ray_counter);
+    delete dev;
+}
+
+uint32_t lc_hip_device_warp_size(uint64_t device) {
+    Device *dev = obj<Device>(device);
+    if (!dev) return 0;
+    int ws = 64;
+    hipDeviceGetAttribute(&ws, hipDeviceAttributeWarpSize, dev->index);
+    return (uint32_t)ws;
+}
+
+int lc_hip_device_native(uint64_t device) {
+    Device *dev = obj<Device>(device);
+    return dev ? dev->index : -1;
+}
+
+int lc_hip_device_query(uint64_t device, const char *key, char *out, size_t capacity) {
+    Device *dev = obj<Device>(device);
+    if (!dev || !key || !out || capacity == 0) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "bad query arguments");
+    std::string v;
+    std::string k(key);
+    if (k == "device_name") v = dev->name;
+    else if (k == "cu_count") v = std::to_string(dev->cu_count);
+    else if (k == "warp_size") v = std::to_string(lc_hip_device_warp_size(device));
+    else return fail(LC_HIP_ERROR_UNSUPPORTED, "unknown query key");
+    std::snprintf(out, capacity, "%s", v.c_str());
+    return LC_HIP_OK;
+}
+
+uint64_t lc_hip_buffer_create(uint64_t device, size_t element_stride, size_t element_count, void **native_handle) {
+    Device *dev = obj<Device>(device);
+    if (!dev || element_stride == 0) {
+        fail(LC_HIP_ERROR_INVALID_ARGUMENT, "bad buffer arguments");
+        return LC_HIP_INVALID_HANDLE;
+    }
+    DeviceGuard g(dev);
+    auto b = std::make_unique<Buffer>();
+    b->dev = dev;
+    b->elem_stride = element_stride;
+    b->bytes = element_stride * element_count;
+    if (hipMalloc(&b->ptr, std::max<size_t>(b->bytes, 16)) != hipSuccess) {
+        fail(LC_HIP_ERROR_OUT_OF_MEMORY, "hipMalloc failed for buffer");
+        return LC_HIP_INVALID_HANDLE;
+    }
+    if (native_handle) *native_handle = b->ptr;
+    return handle(b.release());
+}
+
+void lc_hip_buffer_destroy(uint64_t device, uint64_t buffer) {
+    Device *dev = obj<Device>(device);
+    Buffer *b = obj<Buffer>(buffer);
+    if (!dev || !b) return;
+    DeviceGuard g(dev);
+    hipFree(b->ptr);
+    delete b;
+}
+
+int lc_hip_buffer_upload(uint64_t device, uint64_t stream, uint64_t buffer, size_t offset, size_t size, const void *data) {
+    Device *dev = obj<Device>(device);
+    Buffer *b = obj<Buffer>(buffer);
+    if (!dev || !b || !data) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad upload handles");
+    if (offset + size > b->bytes) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "upload out of buffer range");
+    DeviceGuard g(dev);
+    hipStream_t s = native_stream(stream);
+    // Pageable source: the copy is staged before hipMemcpyAsync returns; synchronizing the
+    // stream here makes "consumed at dispatch" hold for any source memory.
+    LC_HIP_TRY(hipMemcpyAsync(static_cast<char *>(b->ptr) + offset, data, size, hipMemcpyHostToDevice, s));
+    LC_HIP_TRY(hipStreamSynchronize(s));
+    return LC_HIP_OK;
+}
+
+int lc_hip_buffer_download(uint64_t device, uint64_t stream, uint64_t buffer, size_t offset, size_t size, void *data) {
+    Device *dev = obj<Device>(device);
+    Buffer *b = obj<Buffer>(buffer);
+    if (!dev || !b || !data) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad download handles");
+    if (offset + size > b->bytes) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "download out of buffer range");
+    DeviceGuard g(dev);
+    LC_HIP_TRY(hipMemcpyAsync(data, static_cast<char *>(b->ptr) + offset, size, hipMemcpyDeviceToHost,
+                              native_stream(stream)));
+    return LC_HIP_OK;
+}
+
+int lc_hip_buffer_copy(uint64_t device, uint64_t stream, uint64_t src, size_t src_offset, uint64_t dst, size_t dst_offset,
+                       size_t size) {
+    Device *dev = obj<Device>(device);
+    Buffer *a = obj<Buffer>(src), *b = obj<Buffer>(dst);
+    if (!dev || !a || !b) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad copy handles");
+    if (src_offset + size > a->bytes || dst_offset + size > b->bytes)
+        return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "copy out of buffer range");
+    DeviceGuard g(dev);
+    LC_HIP_TRY(hipMemcpyAsync(static_cast<char *>(b->ptr) + dst_offset, static_cast<char *>(a->ptr) + src_offset, size,
+                              hipMemcpyDeviceToDevice, native_stream(stream)));
+    return LC_HIP_OK;
+}
+
+uint64_t lc_hip_stream_create(uint64_t device, uint32_t) {
+    Device *dev = obj<Device>(device);
+    if (!dev) {
+        fail(LC_HIP_ERROR_INVALID_HANDLE, "bad device");
+        return LC_HIP_INVALID_HANDLE;
+    }
+    DeviceGuard g(dev);
+    auto st = std::make_unique<Stream>();
+    st->dev = dev;
+    if (hipStreamCreateWithFlags(&st->s, hipStreamNonBlocking) != hipSuccess) {
+        fail(LC_HIP_ERROR_HIP, "hipStreamCreate failed");
+        return LC_HIP_INVALID_HANDLE;
+    }
+    Stream *raw = st.get();
+    st->worker = std::thread([raw] { stream_worker(raw); });
+    return handle(st.release());
+}
+
+void lc_hip_stream_destroy(uint64_t device, uint64_t stream) {
+    Device *dev = obj<Device>(device);
+    Stream *st = obj<Stream>(stream);
+    if (!dev || !st) return;
+    DeviceGuard g(dev);
+    hipStreamSynchronize(st->s);
+    {
+        std::lock_guard<std::mutex> lk(st->qmu);
+        st->stop = true;
+    }
+    st->qcv.notify_all();
+    if (st->worker.joinable()) st->worker.join();
+    hipStreamDestroy(st->s);
+    delete st;
+}
+
+int lc_hip_stream_synchronize(uint64_t device, uint64_t stream) {
+    Device *dev = obj<Device>(device);
+    if (!dev) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad device");
+    DeviceGuard g(dev);
+    LC_HIP_TRY(hipStreamSynchronize(native_stream(stream)));
+    return LC_HIP_OK;
+}
+
+int lc_hip_stream_add_callback(uint64_t device, uint64_t stream, void (*fn)(void *), void *user) {
+    Device *dev = obj<Device>(device);
+    Stream *st = obj<Stream>(stream);
+    if (!dev || !st || !fn) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad callback arguments");
+    DeviceGuard g(dev);
+    hipEvent_t e;
+    LC_HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    LC_HIP_TRY(hipEventRecord(e, st->s));
+    {
+        std::lock_guard<std::mutex> lk(st->qmu);
+        st->q.emplace_back(e, [fn, user] { fn(user); });
+    }
+    st->qcv.notify_one();
+    return LC_HIP_OK;
+}
+
+void *lc_hip_stream_native(uint64_t, uint64_t stream) { return native_stream(stream); }
+
+// ---- events: a timeline value maps to the HIP event recorded by the signal of that value ----
+struct Event {
+    Device *dev;
+    std::mutex mu;
+    std::map<uint64_t, hipEvent_t> signals;
+};
+
+uint64_t lc_hip_event_create(uint64_t device) {
+    Device *dev = obj<Device>(device);
+    if (!dev) {
+        fail(LC_HIP_ERROR_INVALID_HANDLE, "bad device");
+        return LC_HIP_INVALID_HANDLE;
+    }
+    auto *e = new Event;
+    e->dev = dev;
+    return handle(e);
+}
+
+void lc_hip_event_destroy(uint64_t device, uint64_t event) {
+    Event *e = obj<Event>(event);
+    if (!e || !obj<Device>(device)) return;
+    DeviceGuard g(e->dev);
+    for (auto &kv : e->signals) {
+        hipEventSynchronize(kv.second);
+        hipEventDestroy(kv.second);
+    }
+    delete e;
+}
+
+int lc_hip_event_signal(uint64_t device, uint64_t event, uint64_t stream, uint64_t value) {
+    Event *e = obj<Event>(event);
+    if (!e || !obj<Device>(device)) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad event");
+    DeviceGuard g(e->dev);
+    hipEvent_t h;
+    LC_HIP_TRY(hipEventCreateWithFlags(&h, hipEventDisableTiming));
+    LC_HIP_TRY(hipEventRecord(h, native_stream(stream)));
+    std::lock_guard<std::mutex> lk(e->mu);
+    auto it = e->signals.find(value);
+    if (it != e->signals.end()) {
+        hipEventDestroy(it->second);
+        it->second = h;
+    } else {
+        e->signals.emplace(value, h);
+    }
+    // drop fences of older values that have completed
+    for (auto i = e->signals.begin(); i != e->signals.end() && i->first < value;) {
+        if (hipEventQuery(i->second) == hipSuccess) {
+            hipEventDestroy(i->second);
+            i = e->signals.erase(i);
+        } else {
+            ++i;
+        }
+    }
+    return LC_HIP_OK;
+}
+
+static hipEvent_t event_for(Event *e, uint64_t value, bool &done) {
+    std::lock_guard<std::mutex> lk(e->mu);
+    auto it = e->signals.lower_bound(value);
+    if (it == e->signals.end()) {
+        // no signal at or above value pending: completed if an older one was pruned
+        done = true;
+        return nullptr;
+    }
+    done = false;
+    return it->second;
+}
+
+int lc_hip_event_wait(uint64_t device, uint64_t event, uint64_t stream, uint64_t value) {
+    Event *e = obj<Event>(event);
+    if (!e || !obj<Device>(device)) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad event");
+    DeviceGuard g(e->dev);
+    bool done;
+    hipEvent_t h = event_for(e, value, done);
+    if (h) LC_HIP_TRY(hipStreamWaitEvent(native_stream(stream), h, 0));
+    return LC_HIP_OK;
+}
+
+int lc_hip_event_synchronize(uint64_t device, uint64_t event, uint64_t value) {
+    Event *e = obj<Event>(event);
+    if (!e || !obj<Device>(device)) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad event");
+    DeviceGuard g(e->dev);
+    bool done;
+    hipEvent_t h = event_for(e, value, done);
+    if (h) LC_HIP_TRY(hipEventSynchronize(h));
+    return LC_HIP_OK;
+}
+
+int lc_hip_event_is_completed(uint64_t device, uint64_t event, uint64_t value) {
+    Event *e = obj<Event>(event);
+    if (!e || !obj<Device>(device)) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad event");
+    DeviceGuard g(e->dev);
+    bool done;
+    hipEvent_t h = event_for(e, value, done);
+    if (!h) return 1;
+    return hipEventQuery(h) == hipSuccess ? 1 : 0;
+}
+
+// ---- meshes ----
+uint64_t lc_hip_mesh_create(uint64_t device, const lc_hip_accel_option *option) {
+    Device *dev = obj<Device>(device);
+    if (!dev) {
+        fail(LC_HIP_ERROR_INVALID_HANDLE, "bad device");
+        return LC_HIP_INVALID_HANDLE;
+    }
+    DeviceGuard g(dev);
+    auto m = std::make_unique<Mesh>();
+    m->dev = dev;
+    if (option) {
+        m->hint = option->hint;
+        m->allow_compaction = option->allow_compaction != 0;
+        m->allow_update = option->allow_update != 0;
+    }
+    if (hipHostMalloc((void **)&m->host, sizeof(HostCounters), hipHostMallocDefault) != hipSuccess ||
+        hipEventCreateWithFlags(&m->ready, hipEventDisableTiming) != hipSuccess) {
+        fail(LC_HIP_ERROR_OUT_OF_MEMORY, "mesh host counters");
+        return LC_HIP_INVALID_HANDLE;
+    }
+    return handle(m.release());
+}
+
+void lc_hip_mesh_destroy(uint64_t device, uint64_t mesh) {
+    Device *dev = obj<Device>(device);
+    Mesh *m = obj<Mesh>(mesh);
+    if (!dev || !m) return;
+    DeviceGuard g(dev);
+    hipDeviceSynchronize();
+    if (m->nodes) hipFree(m->nodes);
+    if (m->tris) hipFree(m->tris);
+    if (m->scratch) hipFree(m->scratch);
+    if (m->host) hipHostFree(m->host);
+    if (m->ready) hipEventDestroy(m->ready);
+    delete m;
+}
+
+int lc_hip_mesh_build(uint64_t device, uint64_t stream, uint64_t mesh, uint32_t, uint64_t vertex_buffer,
+                      size_t vertex_offset, size_t vertex_size, size_t vertex_stride, uint64_t triangle_buffer,
+                      size_t triangle_offset, size_t triangle_size) {
+    Device *dev = obj<Device>(device);
+    Mesh *m = obj<Mesh>(mesh);
+    Buffer *vb = obj<Buffer>(vertex_buffer), *tb = obj<Buffer>(triangle_buffer);
+    if (!dev || !m || !vb || !tb) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad mesh build handles");
+    // src/runtime/rtx/mesh.cpp:29-49
+    if (vertex_stride == 0 || vertex_stride % 16 != 0) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "vertex stride must be a multiple of 16");
+    if (vertex_offset % vertex_stride != 0 || vertex_size % vertex_stride != 0)
+        return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "vertex range not aligned to the stride");
+    if (triangle_offset % 4 != 0 || triangle_size % 12 != 0)
+        return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "triangle range not aligned to uint3");
+    if (vertex_offset + vertex_size > vb->bytes || triangle_offset + triangle_size > tb->bytes)
+        return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "mesh range outside its buffer");
+    uint64_t ntris64 = triangle_size / 12, nverts = vertex_size / vertex_stride;
+    if (ntris64 >= 0x7FFFFFFFull) return fail(LC_HIP_ERROR_UNSUPPORTED, "more than 2^31 triangles");
+    uint32_t n = (uint32_t)ntris64;
+    if (n > 0 && nverts == 0) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "triangles without vertices");
+    std::lock_guard<std::mutex> lk(m->mu);
+    DeviceGuard g(dev);
+    hipStream_t s = native_stream(stream);
+    size_t ncap = bvh_node_capacity(n);
+    void *p;
+    size_t cap;
+    p = m->nodes; cap = m->node_cap * sizeof(WideNode);
+    LC_HIP_TRY(ensure(p, cap, ncap * sizeof(WideNode)));
+    m->nodes = static_cast<WideNode *>(p); m->node_cap = cap / sizeof(WideNode);
+    p = m->tris; cap = m->tri_cap * sizeof(TriRecord);
+    LC_HIP_TRY(ensure(p, cap, std::max<size_t>(n, 1) * sizeof(TriRecord)));
+    m->tris = static_cast<TriRecord *>(p); m->tri_cap = cap / sizeof(TriRecord);
+    LC_HIP_TRY(ensure(m->scratch, m->scratch_bytes, bvh_scratch_bytes(n)));
+    m->ntris = n;
+    m->built = true;
+    m->stats_valid = false;
+    m->build_epoch++;
+    if (n == 0) return LC_HIP_OK;
+    {
+        std::lock_guard<std::mutex> dl(dev->mu);
+        LC_HIP_TRY(hipMemsetAsync(dev->error, 0, sizeof(uint32_t), s));
+        TimedRegion tr(dev, s, "mesh_build");
+        BuildCounters *ctr = nullptr;
+        hipError_t e = bvh_build(s, m->scratch, m->scratch_bytes, static_cast<char *>(vb->ptr) + vertex_offset,
+                                 vertex_stride, nverts,
+                                 reinterpret_cast<const uint32_t *>(static_cast<char *>(tb->ptr) + triangle_offset),
+                                 nullptr, n, m->nodes, m->tris, nullptr, &ctr, dev->split_small, 4 * dev->cu_count,
+                                 dev->error);
+        if (e != hipSuccess) return hip_fail(e, "bvh_build");
+        copy_counters_async(ctr, m->host, dev->error, s);
+    }
+    LC_HIP_TRY(hipEventRecord(m->ready, s));
+    return LC_HIP_OK;
+}
+
+int lc_hip_mesh_stats(uint64_t device, uint64_t mesh, lc_hip_bvh_stats *out) {
+    Device *dev = obj<Device>(device);
+    Mesh *m = obj<Mesh>(mesh);
+    if (!dev || !m || !out) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad mesh stats handles");
+    DeviceGuard g(dev);
+    int rc = mesh_stats_sync(m);
+    if (rc != LC_HIP_OK) return rc;
+    out->node_count = m->stats.node_count;
+    out->leaf_primitive_count = m->stats.tri_count;
+    out->depth = m->stats.depth;
+    out->primitive_count = m->ntris;
+    std::memcpy(out->bounds, m->stats.bounds, sizeof(out->bounds));
+    out->node_bytes = (uint64_t)m->stats.node_count * sizeof(WideNode);
+    out->primitive_bytes = (uint64_t)m->stats.tri_count * sizeof(TriRecord);
+    return LC_HIP_OK;
+}
+
+// ---- accels ----
+uint64_t lc_hip_accel_create(uint64_t device, const lc_hip_accel_option *option) {
+    Device *dev = obj<Device>(device);
+    if (!dev) {
+        fail(LC_HIP_ERROR_INVALID_HANDLE, "bad device");
+        return LC_HIP_INVALID_HANDLE;
+    }
+    DeviceGuard g(dev);
+    auto a = std::make_unique<Accel>();
+    a->dev = dev;
+    if (option) {
+        a->hint = option->hint;
+        a->allow_compaction = option->allow_compaction != 0;
+        a->allow_update = option->allow_update != 0;
+    }
+    if (hipHostMalloc((void **)&a->tlas_host, sizeof(HostCounters), hipHostMallocDefault) != hipSuccess ||
+        hipEventCreateWithFlags(&a->tlas_ready, hipEventDisableTiming) != hipSuccess) {
+        fail(LC_HIP_ERROR_OUT_OF_MEMORY, "accel host counters");
+        return LC_HIP_INVALID_HANDLE;
+    }
+    return handle(a.release());
+}
+
+void lc_hip_accel_destroy(uint64_t device, uint64_t accel) {
+    Device *dev = obj<Device>(device);
+    Accel *a = obj<Accel>(accel);
+    if (!dev || !a) return;
+    DeviceGuard g(dev);
+    hipDeviceSynchronize();
+    if (a->records) hipFree(a->records);
+    if (a->boxes) hipFree(a->boxes);
+    if (a->tlas_nodes) hipFree(a->tlas_nodes);
+    if (a->tlas_prims) hipFree(a->tlas_prims);
+    if (a->tlas_scratch) hipFree(a->tlas_scratch);
+    if (a->tlas_host) hipHostFree(a->tlas_host);
+    if (a->tlas_ready) hipEventDestroy(a->tlas_ready);
+    delete a;
+}
+
+static void xform_point_h(const float m[12], const float p[3], float out[3]) {
+    for (int r = 0; r < 3; r++) out[r] = m[4 * r + 0] * p[0] + m[4 * r + 1] * p[1] + m[4 * r + 2] * p[2] + m[4 * r + 3];
+}
+
+int lc_hip_accel_build(uint64_t device, uint64_t stream, uint64_t accel, uint32_t, uint32_t instance_count,
+                       const lc_hip_accel_modification *mods, size_t nmods, int update_instance_buffer_only) {
+    Device *dev = obj<Device>(device);
+    Accel *a = obj<Accel>(accel);
+    if (!dev || !a) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad accel build handles");
+    if (nmods > 0 && !mods) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "null modifications");
+    std::lock_guard<std::mutex> lk(a->mu);
+    DeviceGuard g(dev);
+    hipStream_t s = native_stream(stream);
+    // AccelImpl::update (cpu/accel.rs:324-447): resize, then apply each modification in order.
+    a->inst.resize(instance_count);
+    for (size_t k = 0; k < nmods; k++) {
+        const auto &md = mods[k];
+        if (md.index >= instance_count) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "modification index out of range");
+        InstanceHost &in = a->inst[md.index];
+        if (md.flags & LC_HIP_MOD_PRIMITIVE) {
+            Mesh *m = obj<Mesh>(md.primitive);
+            if (!m) return fail(LC_HIP_ERROR_INVALID_HANDLE, "instance primitive is not a mesh");
+            if (!m->built) return fail(LC_HIP_ERROR_NOT_BUILT, "Mesh not built");
+            in.mesh = m;
+            std::memcpy(in.affine, md.affine, sizeof(in.affine));
+            in.vis_mask = 0xFFu;
+            in.user_id = md.user_id;
+            in.opaque = true;
+        }
+        if (md.flags & LC_HIP_MOD_OPAQUE_ON) in.opaque = true;
+        else if (md.flags & LC_HIP_MOD_OPAQUE_OFF) in.opaque = false;
+        if (md.flags & LC_HIP_MOD_TRANSFORM) std::memcpy(in.affine, md.affine, sizeof(in.affine));
+        if (md.flags & LC_HIP_MOD_VISIBILITY) in.vis_mask = md.vis_mask;
+        if (md.flags & LC_HIP_MOD_USER_ID) in.user_id = md.user_id;
+    }
+    // instance records (need each BLAS's depth and bounds: wait for its counters)
+    a->host_records.resize(instance_count);
+    uint32_t max_depth = 0;
+    for (uint32_t i = 0; i < instance_count; i++) {
+        InstanceHost &in = a->inst[i];
+        InstanceRecord &r = a->host_records[i];
+        std::memset(&r, 0, sizeof(r));
+        std::memcpy(r.fwd, in.affine, sizeof(r.fwd));
+        invert_affine(in.affine, r.inv);
+        r.vis_mask = in.vis_mask;
+        r.user_id = in.user_id;
+        r.flags = in.opaque ? 1u : 0u;
+        if (in.mesh) {
+            int rc = mesh_stats_sync(in.mesh);
+            if (rc != LC_HIP_OK) return rc;
+            if (in.mesh->ntris > 0) {
+                r.blas_nodes = reinterpret_cast<uint64_t>(in.mesh->nodes);
+                r.blas_tris = reinterpret_cast<uint64_t>(in.mesh->tris);
+                r.blas_depth = in.mesh->stats.depth;
+                max_depth = std::max(max_depth, r.blas_depth);
+            }
+        }
+    }
+    a->max_blas_depth = max_depth;
+    void *p = a->records;
+    size_t cap = a->record_cap * sizeof(InstanceRecord);
+    LC_HIP_TRY(ensure(p, cap, std::max<size_t>(instance_count, 1) * sizeof(InstanceRecord)));
+    a->records = static_cast<InstanceRecord *>(p);
+    a->record_cap = cap / sizeof(InstanceRecord);
+    if (instance_count) {
+        LC_HIP_TRY(hipMemcpyAsync(a->records, a->host_records.data(), instance_count * sizeof(InstanceRecord),
+                                  hipMemcpyHostToDevice, s));
+    }
+    if (update_instance_buffer_only) {
+        LC_HIP_TRY(hipStreamSynchronize(s)); // host_records is reused by the next build
+        return LC_HIP_OK;
+    }
+    a->built = true;
+    a->tlas_valid = false;
+    a->tlas_stats_valid = false;
+    if (instance_count > 1) {
+        // world boxes of instances: 8 transformed BLAS-box corners, padded by a relative epsilon
+        std::vector<float> boxes(8 * (size_t)instance_count);
+        for (uint32_t i = 0; i < instance_count; i++) {
+            float lo[3] = {3e38f, 3e38f, 3e38f}, hi[3] = {-3e38f, -3e38f, -3e38f};
+            const InstanceHost &in = a->inst[i];
+            if (in.mesh && in.mesh->ntris > 0) {
+                const float *b = in.mesh->stats.bounds;
+                for (int c = 0; c < 8; c++) {
+                    float q[3] = {(c & 1) ? b[3] : b[0], (c & 2) ? b[4] : b[1], (c & 4) ? b[5] : b[2]}, w[3];
+                    xform_point_h(in.affine, q, w);
+                    for (int k = 0; k < 3; k++) { lo[k] = std::min(lo[k], w[k]); hi[k] = std::max(hi[k], w[k]); }
+                }
+                for (int k = 0; k < 3; k++) {
+                    float pad = 1e-5f * std::max({std::fabs(lo[k]), std::fabs(hi[k]), 1.0f});
+                    lo[k] -= pad;
+                    hi[k] += pad;
+                }
+            } else {
+                // no geometry: a degenerate box far outside any scene (never entered: blas_nodes == 0)
+                for (int k = 0; k < 3; k++) { lo[k] = hi[k] = 3e38f; }
+            }
+            float *o = &boxes[8 * (size_t)i];
+            o[0] = lo[0]; o[1] = lo[1]; o[2] = lo[2]; o[3] = 0.0f;
+            o[4] = hi[0]; o[5] = hi[1]; o[6] = hi[2]; o[7] = 0.0f;
+        }
+        p = a->boxes; cap = a->box_cap;
+        LC_HIP_TRY(ensure(p, cap, boxes.size() * sizeof(float)));
+        a->boxes = static_cast<float4 *>(p); a->box_cap = cap;
+        LC_HIP_TRY(hipMemcpyAsync(a->boxes, boxes.data(), boxes.size() * sizeof(float), hipMemcpyHostToDevice, s));
+        size_t ncap = bvh_node_capacity(instance_count);
+        if (a->tlas_cap < ncap) {
+            if (a->tlas_nodes) hipFree(a->tlas_nodes);
+            if (a->tlas_prims) hipFree(a->tlas_prims);
+            a->tlas_nodes = nullptr;
+            a->tlas_prims = nullptr;
+            a->tlas_cap = 0;
+            LC_HIP_TRY(hipMalloc(&a->tlas_nodes, ncap * sizeof(WideNode)));
+            LC_HIP_TRY(hipMalloc(&a->tlas_prims, ncap * sizeof(uint32_t)));
+            a->tlas_cap = ncap;
+        }
+        LC_HIP_TRY(ensure(a->tlas_scratch, a->tlas_scratch_bytes, bvh_scratch_bytes(instance_count)));
+        {
+            std::lock_guard<std::mutex> dl(dev->mu);
+            LC_HIP_TRY(hipMemsetAsync(dev->error, 0, sizeof(uint32_t), s));
+            TimedRegion tr(dev, s, "accel_build");
+            BuildCounters *ctr = nullptr;
+            hipError_t e = bvh_build(s, a->tlas_scratch, a->tlas_scratch_bytes, nullptr, 0, 0, nullptr, a->boxes,
+                                     instance_count, a->tlas_nodes, nullptr, a->tlas_prims, &ctr, 0,
+                                     std::max(1, dev->cu_count), dev->error);
+            if (e != hipSuccess) return hip_fail(e, "tlas build");
+            copy_counters_async(ctr, a->tlas_host, dev->error, s);
+        }
+        LC_HIP_TRY(hipEventRecord(a->tlas_ready, s));
+        a->tlas_valid = true;
+    }
+    // host_records / boxes staging must outlive the async copies
+    LC_HIP_TRY(hipStreamSynchronize(s));
+    return LC_HIP_OK;
+}
+
+static int accel_tlas_sync(Accel *a) {
+    if (!a->tlas_stats_valid && a->tlas_valid) {
+        LC_HIP_TRY(hipEventSynchronize(a->tlas_ready));
+        a->tlas_stats = *a->tlas_host;
+        if (a->tlas_stats.tail_level_count != 0) return fail(LC_HIP_ERROR_DEVICE_FAULT, "TLAS collapse exceeded the level bound");
+        if (a->tlas_stats.tri_count != a->inst.size()) return fail(LC_HIP_ERROR_DEVICE_FAULT, "TLAS build lost instances");
+        a->tlas_stats_valid = true;
+    }
+    return LC_HIP_OK;
+}
+
+int lc_hip_accel_stats(uint64_t device, uint64_t accel, lc_hip_bvh_stats *out) {
+    Device *dev = obj<Device>(device);
+    Accel *a = obj<Accel>(accel);
+    if (!dev || !a || !out) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad accel stats handles");
+    DeviceGuard g(dev);
+    if (!a->built) return fail(LC_HIP_ERROR_NOT_BUILT, "accel is not built");
+    std::memset(out, 0, sizeof(*out));
+    out->primitive_count = (uint32_t)a->inst.size();
+    if (a->tlas_valid) {
+        int rc = accel_tlas_sync(a);
+        if (rc != LC_HIP_OK) return rc;
+        out->node_count = a->tlas_stats.node_count;
+        out->leaf_primitive_count = a->tlas_stats.tri_count;
+        out->depth = a->tlas_stats.depth;
+        std::memcpy(out->bounds, a->tlas_stats.bounds, sizeof(out->bounds));
+        out->node_bytes = (uint64_t)out->node_count * sizeof(WideNode);
+        out->primitive_bytes = (uint64_t)a->inst.size() * sizeof(InstanceRecord);
+    }
+    return LC_HIP_OK;
+}
+
+int lc_hip_accel_instance_info(uint64_t device, uint64_t accel, uint32_t index, float affine[12], uint32_t *vis_mask,
+                               uint32_t *user_id) {
+    Accel *a = obj<Accel>(accel);
+    if (!obj<Device>(device) || !a) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad accel");
+    std::lock_guard<std::mutex> lk(a->mu);
+    if (index >= a->inst.size()) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "instance index out of range");
+    if (affine) std::memcpy(affine, a->inst[index].affine, sizeof(float) * 12);
+    if (vis_mask) *vis_mask = a->inst[index].vis_mask;
+    if (user_id) *user_id = a->inst[index].user_id;
+    return LC_HIP_OK;
+}
+
+static int trace_common(uint64_t device, uint64_t stream, uint64_t accel, uint64_t ray_buffer, size_t ray_offset,
+                        uint64_t hit_buffer, size_t hit_offset, uint32_t count, uint32_t vis_mask, bool any, bool counting,
+                        uint64_t *n_nodes, uint64_t *n_tris) {
+    Device *dev = obj<Device>(device);
+    Accel *a = obj<Accel>(accel);
+    Buffer *rb = obj<Buffer>(ray_buffer);
+    Buffer *hb = counting ? nullptr : obj<Buffer>(hit_buffer);
+    if (!dev || !a || !rb || (!counting && !hb)) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad trace handles");
+    if (ray_offset % 16 != 0) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "ray offset must be 16-byte aligned");
+    if (ray_offset + (size_t)count * 32 > rb->bytes) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "rays outside the ray buffer");
+    if (!counting) {
+        size_t stride = any ? 1 : 24;
+        if (!any && hit_offset % 8 != 0) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "hit offset must be 8-byte aligned");
+        if (hit_offset + (size_t)count * stride > hb->bytes) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "hits outside the hit buffer");
+    }
+    std::lock_guard<std::mutex> lk(a->mu);
+    if (!a->built) return fail(LC_HIP_ERROR_NOT_BUILT, "accel is not built");
+    DeviceGuard g(dev);
+    hipStream_t s = native_stream(stream);
+    TraceParams P;
+    std::memset(&P, 0, sizeof(P));
+    P.rays = reinterpret_cast<const float4 *>(static_cast<char *>(rb->ptr) + ray_offset);
+    P.count = count;
+    P.vis_mask = vis_mask;
+    if (!counting) {
+        if (any) P.out_any = reinterpret_cast<uint8_t *>(static_cast<char *>(hb->ptr) + hit_offset);
+        else P.out_hits = static_cast<char *>(hb->ptr) + hit_offset;
+    }
+    bool two = a->inst.size() > 1;
+    uint32_t stack_need = 0;
+    if (!two) {
+        const InstanceRecord *r = a->host_records.empty() ? nullptr : &a->host_records[0];
+        if (!r || r->blas_nodes == 0) {
+            P.empty = 1;
+        } else {
+            P.blas_nodes = reinterpret_cast<const WideNode *>(r->blas_nodes);
+            P.blas_tris = reinterpret_cast<const TriRecord *>(r->blas_tris);
+            std::memcpy(P.inst_inv, r->inv, sizeof(P.inst_inv));
+            P.inst_mask = r->vis_mask;
+            stack_need = r->blas_depth + 1;
+        }
+    } else {
+        int rc = accel_tlas_sync(a);
+        if (rc != LC_HIP_OK) return rc;
+        P.tlas_nodes = a->tlas_nodes;
+        P.tlas_prims = a->tlas_prims;
+        P.instances = a->records;
+        stack_need = a->tlas_stats.depth + 2 + a->max_blas_depth + 1;
+    }
+    std::lock_guard<std::mutex> dl(dev->mu);
+    const TraceConfig &cfg = dev->cfg[any ? 1 : 0];
+    int bpc = cfg.blocks_per_cu > 0 ? cfg.blocks_per_cu : trace_blocks_per_cu(any, two, counting, cfg.block, cfg.slds);
+    uint64_t want = ((uint64_t)count + cfg.block - 1) / cfg.block;
+    int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)bpc * dev->cu_count, want));
+    P.spill_depth = stack_need > (uint32_t)cfg.slds ? stack_need - (uint32_t)cfg.slds : 0u;
+    size_t spill_bytes = (size_t)grid * cfg.block * P.spill_depth * sizeof(uint2);
+    LC_HIP_TRY(ensure(dev->spill, dev->spill_bytes, spill_bytes));
+    P.spill = static_cast<uint2 *>(dev->spill);
+    P.ray_counter = dev->ray_counter;
+    P.error = dev->error;
+    P.stats = dev->stats;
+    P.steps = cfg.steps;
+    if (counting) LC_HIP_TRY(hipMemsetAsync(dev->stats, 0, 2 * sizeof(unsigned long long), s));
+    {
+        TimedRegion tr(dev, s, counting ? "trace_count" : (any ? "trace_any" : "trace_closest"));
+        hipError_t e = trace_launch(P, s, any, two, counting, grid, cfg.block, cfg.slds);
+        if (e != hipSuccess) return hip_fail(e, "trace launch");
+    }
+    if (counting) {
+        unsigned long long st[2] = {0, 0};
+        LC_HIP_TRY(hipMemcpyAsync(st, dev->stats, sizeof(st), hipMemcpyDeviceToHost, s));
+        LC_HIP_TRY(hipStreamSynchronize(s));
+        if (n_nodes) *n_nodes = st[0];
+        if (n_tris) *n_tris = st[1];
+    }
+    return LC_HIP_OK;
+}
+
+int lc_hip_accel_trace_closest(uint64_t device, uint64_t stream, uint64_t accel, uint64_t ray_buffer, size_t ray_offset,
+                               uint64_t hit_buffer, size_t hit_offset, uint32_t ray_count, uint32_t vis_mask) {
+    return trace_common(device, stream, accel, ray_buffer, ray_offset, hit_buffer, hit_offset, ray_count, vis_mask, false,
+                        false, nullptr, nullptr);
+}
+
+int lc_hip_accel_trace_any(uint64_t device, uint64_t stream, uint64_t accel, uint64_t ray_buffer, size_t ray_offset,
+                           uint64_t hit_buffer, size_t hit_offset, uint32_t ray_count, uint32_t vis_mask) {
+    return trace_common(device, stream, accel, ray_buffer, ray_offset, hit_buffer, hit_offset, ray_count, vis_mask, true,
+                        false, nullptr, nullptr);
+}
+
+int lc_hip_accel_trace_count(uint64_t device, uint64_t stream, uint64_t accel, uint64_t ray_buffer, size_t ray_offset,
+                             uint32_t ray_count, uint32_t vis_mask, int any, uint64_t *nodes_visited,
+                             uint64_t *triangles_tested) {
+    return trace_common(device, stream, accel, ray_buffer, ray_offset, 0, 0, ray_count, vis_mask, any != 0, true,
+                        nodes_visited, triangles_tested);
+}
+
+int lc_hip_trace_set_config(uint64_t device, int kind, int block, int lds_stack, int steps, int blocks_per_cu) {
+    Device *dev = obj<Device>(device);
+    if (!dev || kind < 0 || kind > 1) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "bad trace config target");
+    if (!trace_config_supported(block, lds_stack) || steps < 1 || blocks_per_cu < 0)
+        return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "unsupported trace config");
+    std::lock_guard<std::mutex> lk(dev->mu);
+    dev->cfg[kind] = TraceConfig{block, lds_stack, steps, blocks_per_cu};
+    return LC_HIP_OK;
+}
+
+int lc_hip_trace_get_config(uint64_t device, int kind, int *block, int *lds_stack, int *steps, int *blocks_per_cu) {
+    Device *dev = obj<Device>(device);
+    if (!dev || kind < 0 || kind > 1) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "bad trace config target");
+    std::lock_guard<std::mutex> lk(dev->mu);
+    const TraceConfig &c = dev->cfg[kind];
+    if (block) *block = c.block;
+    if (lds_stack) *lds_stack = c.slds;
+    if (steps) *steps = c.steps;
+    if (blocks_per_cu) *blocks_per_cu = c.blocks_per_cu;
+    return LC_HIP_OK;
+}
+
+int lc_hip_build_set_split_small(uint64_t device, int enable) {
+    Device *dev = obj<Device>(device);
+    if (!dev) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad device");
+    dev->split_small = enable ? 1u : 0u;
+    return LC_HIP_OK;
+}
+
+int lc_hip_timing_enable(uint64_t device, int enable) {
+    Device *dev = obj<Device>(device);
+    if (!dev) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad device");
+    std::lock_guard<std::mutex> lk(dev->mu);
+    dev->timing = enable != 0;
+    return LC_HIP_OK;
+}
+
+int lc_hip_timing_get(uint64_t device, const char *name, double *total_ms, uint64_t *launches) {
+    Device *dev = obj<Device>(device);
+    if (!dev || !name) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "bad timing query");
+    DeviceGuard g(dev);
+    std::lock_guard<std::mutex> lk(dev->mu);
+    collect_timings(dev);
+    auto it = dev->timings.find(name);
+    if (total_ms) *total_ms = it == dev->timings.end() ? 0.0 : it->second.total_ms;
+    if (launches) *launches = it == dev->timings.end() ? 0 : it->second.launches;
+    return LC_HIP_OK;
+}
+
+int lc_hip_timing_reset(uint64_t device) {
+    Device *dev = obj<Device>(device);
+    if (!dev) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad device");
+    DeviceGuard g(dev);
+    std::lock_guard<std::mutex> lk(dev->mu);
+    collect_timings(dev);
+    dev->timings.clear();
+    return LC_HIP_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,198 @@
+"""ctypes binding of the CPU oracle (liblc_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, the smoke check and the benchmark's CPU-baseline
+leg, as the checker. The product package never imports this module.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liblc_oracle.so")
+
+HIT_DTYPE = np.dtype([("inst", "<u4"), ("prim", "<u4"), ("u", "<f4"), ("v", "<f4"), ("t", "<f4"), ("pad", "<u4")])
+HIT2_DTYPE = np.dtype([("inst", "<u4"), ("prim", "<u4"), ("u", "<f4"), ("v", "<f4"), ("t", "<f4"), ("t2", "<f4")])
+BF_DTYPE = np.dtype([("prim", "<u4"), ("pad", "<u4"), ("t", "<f8"), ("u", "<f8"), ("v", "<f8"), ("t2", "<f8")])
+
+
+class Pcg32(ctypes.Structure):
+    _fields_ = [("state", ctypes.c_uint64), ("inc", ctypes.c_uint64)]
+
+
+def build():
+    """Compile the oracle in place (make)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        P, U64, U32, I = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int
+        F = ctypes.c_float
+        sig = {
+            "orc_pcg32_seed": (None, [ctypes.POINTER(Pcg32), U64, U64]),
+            "orc_pcg32_next": (U32, [ctypes.POINTER(Pcg32)]),
+            "orc_pcg32_advance": (None, [ctypes.POINTER(Pcg32), U64]),
+            "orc_pcg32_float": (F, [ctypes.POINTER(Pcg32)]),
+            "orc_icosphere_counts": (None, [I, ctypes.POINTER(U32), ctypes.POINTER(U32)]),
+            "orc_icosphere": (None, [I, P, P]),
+            "orc_rays_orbit_to_ball": (None, [U64, U64, U64, F, P]),
+            "orc_soup": (None, [U64, U64, U64, F, F, P, P]),
+            "orc_rays_box_sphere": (None, [U64, U64, U64, U64, F, F, P]),
+            "orc_mesh_build": (P, [P, U64, U64, P, U64, I]),
+            "orc_mesh_free": (None, [P]),
+            "orc_mesh_node_count": (U64, [P]),
+            "orc_scene_create": (P, []),
+            "orc_scene_add": (None, [P, P, P, U32]),
+            "orc_scene_commit": (None, [P]),
+            "orc_scene_free": (None, [P]),
+            "orc_trace_closest": (None, [P, P, U64, U32, P, I]),
+            "orc_trace_closest2": (None, [P, P, U64, U32, P, I]),
+            "orc_trace_any": (None, [P, P, U64, U32, P, I]),
+            "orc_brute_closest_f64": (None, [P, U64, P, U64, P, U64, P, I]),
+            "orc_watertight": (I, [P, P, P, P, P, F, F, P, P, P]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Rng:
+    def __init__(self, initstate, initseq):
+        self.s = Pcg32()
+        lib().orc_pcg32_seed(ctypes.byref(self.s), initstate, initseq)
+
+    def next(self):
+        return lib().orc_pcg32_next(ctypes.byref(self.s))
+
+    def advance(self, delta):
+        lib().orc_pcg32_advance(ctypes.byref(self.s), delta)
+
+    def float(self):
+        return lib().orc_pcg32_float(ctypes.byref(self.s))
+
+
+def icosphere(level):
+    nv, nt = ctypes.c_uint32(), ctypes.c_uint32()
+    lib().orc_icosphere_counts(level, ctypes.byref(nv), ctypes.byref(nt))
+    v = np.zeros((nv.value, 4), np.float32)
+    t = np.zeros((nt.value, 3), np.uint32)
+    lib().orc_icosphere(level, _p(v), _p(t))
+    return v, t
+
+
+def rays_orbit_to_ball(n, initstate=42, initseq=1, orbit=3.0):
+    r = np.zeros((n, 8), np.float32)
+    lib().orc_rays_orbit_to_ball(initstate, initseq, n, orbit, _p(r))
+    return r
+
+
+def soup(ntris, initstate=42, initseq=2, extent=100.0, sigma=0.5):
+    v = np.zeros((3 * ntris, 4), np.float32)
+    t = np.zeros((ntris, 3), np.uint32)
+    lib().orc_soup(initstate, initseq, ntris, extent, sigma, _p(v), _p(t))
+    return v, t
+
+
+def rays_box_sphere(n, initstate=42, initseq=2, skip=0, lo=0.0, hi=100.0):
+    r = np.zeros((n, 8), np.float32)
+    lib().orc_rays_box_sphere(initstate, initseq, skip, n, lo, hi, _p(r))
+    return r
+
+
+class Mesh:
+    def __init__(self, verts4, tris, nthreads=0):
+        self.verts = np.ascontiguousarray(verts4, np.float32)
+        self.tris = np.ascontiguousarray(tris, np.uint32)
+        stride = self.verts.strides[0]
+        self.h = lib().orc_mesh_build(_p(self.verts), stride, self.verts.shape[0], _p(self.tris),
+                                      self.tris.shape[0], nthreads)
+
+    def node_count(self):
+        return lib().orc_mesh_node_count(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_mesh_free(self.h)
+            self.h = None
+
+
+IDENTITY = np.array([1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0], np.float32)
+
+
+class Scene:
+    """Two-level scene: instances of oracle meshes with row-major 3x4 affines."""
+
+    def __init__(self):
+        self.h = lib().orc_scene_create()
+        self._keep = []
+
+    def add(self, mesh, affine=IDENTITY, mask=0xFF):
+        a = np.ascontiguousarray(np.asarray(affine, np.float32).reshape(12))
+        self._keep.append((mesh, a))
+        lib().orc_scene_add(self.h, mesh.h, _p(a), mask)
+        return self
+
+    def commit(self):
+        lib().orc_scene_commit(self.h)
+        return self
+
+    def trace_closest(self, rays, mask=0xFF, nthreads=0):
+        rays = np.ascontiguousarray(rays, np.float32)
+        out = np.zeros(rays.shape[0], HIT_DTYPE)
+        lib().orc_trace_closest(self.h, _p(rays), rays.shape[0], mask, _p(out), nthreads)
+        return out
+
+    def trace_closest2(self, rays, mask=0xFF, nthreads=0):
+        rays = np.ascontiguousarray(rays, np.float32)
+        out = np.zeros(rays.shape[0], HIT2_DTYPE)
+        lib().orc_trace_closest2(self.h, _p(rays), rays.shape[0], mask, _p(out), nthreads)
+        return out
+
+    def trace_any(self, rays, mask=0xFF, nthreads=0):
+        rays = np.ascontiguousarray(rays, np.float32)
+        out = np.zeros(rays.shape[0], np.uint8)
+        lib().orc_trace_any(self.h, _p(rays), rays.shape[0], mask, _p(out), nthreads)
+        return out
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_scene_free(self.h)
+            self.h = None
+
+
+def single_mesh_scene(verts4, tris, nthreads=0):
+    m = Mesh(verts4, tris, nthreads)
+    return Scene().add(m).commit()
+
+
+def brute_closest_f64(verts4, tris, rays, nthreads=0):
+    verts4 = np.ascontiguousarray(verts4, np.float32)
+    tris = np.ascontiguousarray(tris, np.uint32)
+    rays = np.ascontiguousarray(rays, np.float32)
+    out = np.zeros(rays.shape[0], BF_DTYPE)
+    lib().orc_brute_closest_f64(_p(verts4), verts4.strides[0], _p(tris), tris.shape[0], _p(rays),
+                                rays.shape[0], _p(out), nthreads)
+    return out
+
+
+def watertight(o, d, v0, v1, v2, tmin=0.0, tmax=1e30):
+    a = [np.ascontiguousarray(x, np.float32) for x in (o, d, v0, v1, v2)]
+    t, u, v = ctypes.c_float(), ctypes.c_float(), ctypes.c_float()
+    h = lib().orc_watertight(*[_p(x) for x in a], tmin, tmax, ctypes.byref(t), ctypes.byref(u), ctypes.byref(v))
+    return bool(h), t.value, u.value, v.value

@@ -1,0 +1,98 @@
+/*
+ * lc_oracle.h -- CPU restatement of the reference CPU backend's ray-tracing semantics.
+ *
+ * TEST INFRASTRUCTURE ONLY. This library is the checker for the HIP backend: only the
+ * test suite, the smoke check and the benchmark's CPU-baseline leg may load it. The product
+ * (luisacompute_amd/) never links, calls or falls back to it.
+ *
+ * What it restates (reference paths relative to the LuisaCompute checkout):
+ *   - hit semantics of AccelImpl::trace_closest / trace_any
+ *       src/rust/luisa_compute_backend_impl/src/cpu/accel.rs:449-535
+ *     miss -> inst = prim = 0xFFFFFFFF, bary = (0,0), t = ray.tmax; any-hit -> bool.
+ *   - mesh input interpretation (FLOAT3 at a byte stride, UINT3 indices)  cpu/accel.rs:215-237
+ *   - instance semantics (instance id = index, default mask 0xff, row-major 3x4 affine,
+ *     world->object ray transform without renormalisation)               cpu/accel.rs:324-447
+ *   - FTZ/DAZ on the worker threads                                      cpu/mod.rs:503-516
+ * The reference delegates the arithmetic to Embree 4 (embree_sys 0.1.11, un-vendored), which
+ * cannot be built offline. This file restates the published algorithms instead:
+ *   - watertight ray/triangle test: Woop, Benthin, Wald, "Watertight Ray/Triangle
+ *     Intersection", JCGT 2(1), 2013 (with the double-precision edge fallback);
+ *   - binned-SAH BVH (Wald 2007 "On fast construction of SAH-based BVHs");
+ *   - robust slab test: Ize, "Robust BVH Ray Traversal", JCGT 2(2), 2013;
+ *   - PCG32: O'Neill 2014, pcg32_srandom_r / pcg32_random_r / pcg32_advance_r.
+ * plus an independent double-precision Moller-Trumbore brute force used to pin it.
+ *
+ * Pinning status per function is listed in oracle/README.md.
+ */
+#ifndef LC_ORACLE_H
+#define LC_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- PCG32 (O'Neill 2014, pcg-c-basic) ---- */
+typedef struct orc_pcg32 { uint64_t state; uint64_t inc; } orc_pcg32;
+void     orc_pcg32_seed(orc_pcg32 *rng, uint64_t initstate, uint64_t initseq);
+uint32_t orc_pcg32_next(orc_pcg32 *rng);
+void     orc_pcg32_advance(orc_pcg32 *rng, uint64_t delta);
+float    orc_pcg32_float(orc_pcg32 *rng); /* (x >> 8) * 2^-24 */
+
+/* ---- synthetic workload generators (restated independently of the product's) ---- */
+/* icosphere: level-L subdivided icosahedron, radius 1. verts: 4 floats per vertex (16-B stride). */
+void orc_icosphere_counts(int level, uint32_t *nverts, uint32_t *ntris);
+void orc_icosphere(int level, float *verts4, uint32_t *tris);
+/* rays from a sphere of radius `orbit` toward uniform points in the unit ball (8 floats/ray). */
+void orc_rays_orbit_to_ball(uint64_t initstate, uint64_t initseq, uint64_t nrays, float orbit,
+                            float *rays);
+/* triangle soup: centroid U[0,extent]^3, vertices centroid + N(0, sigma^2) per axis.
+   Stream offsets: triangle i consumes draws [13 i, 13 i + 13). */
+void orc_soup(uint64_t initstate, uint64_t initseq, uint64_t ntris, float extent, float sigma,
+              float *verts4, uint32_t *tris);
+/* rays with origin U[lo,hi]^3 and direction uniform on S^2; ray j consumes draws
+   [skip + 5 j, skip + 5 j + 5) of the stream. */
+void orc_rays_box_sphere(uint64_t initstate, uint64_t initseq, uint64_t skip, uint64_t nrays,
+                         float lo, float hi, float *rays);
+
+/* ---- geometry / scene ---- */
+typedef struct orc_hit { uint32_t inst; uint32_t prim; float u; float v; float t; uint32_t pad; } orc_hit;
+typedef struct orc_hit2 { uint32_t inst; uint32_t prim; float u; float v; float t; float t2; } orc_hit2;
+
+void *orc_mesh_build(const float *verts, uint64_t vstride_bytes, uint64_t nverts,
+                     const uint32_t *tris, uint64_t ntris, int nthreads);
+void  orc_mesh_free(void *mesh);
+uint64_t orc_mesh_node_count(const void *mesh);
+
+void *orc_scene_create(void);
+/* affine: row-major 3x4 (object -> world); mask: instance visibility (default 0xff). */
+void  orc_scene_add(void *scene, const void *mesh, const float affine[12], uint32_t mask);
+void  orc_scene_commit(void *scene);
+void  orc_scene_free(void *scene);
+
+/* rays: 8 floats each {ox,oy,oz,tmin,dx,dy,dz,tmax}. */
+void orc_trace_closest(const void *scene, const float *rays, uint64_t nrays, uint32_t mask,
+                       orc_hit *hits, int nthreads);
+/* closest hit plus the second-nearest distinct-primitive hit distance t2 (inf if none). */
+void orc_trace_closest2(const void *scene, const float *rays, uint64_t nrays, uint32_t mask,
+                        orc_hit2 *hits, int nthreads);
+void orc_trace_any(const void *scene, const float *rays, uint64_t nrays, uint32_t mask,
+                   uint8_t *hit, int nthreads);
+
+/* Independent check: double-precision Moller-Trumbore brute force over one mesh (no BVH).
+   out per ray: prim (UINT32_MAX on miss), t, u, v (doubles) and second-nearest t2. */
+typedef struct orc_bf_hit { uint32_t prim; uint32_t pad; double t, u, v, t2; } orc_bf_hit;
+void orc_brute_closest_f64(const float *verts, uint64_t vstride_bytes, const uint32_t *tris,
+                           uint64_t ntris, const float *rays, uint64_t nrays, orc_bf_hit *out,
+                           int nthreads);
+
+/* single watertight test exposed for unit tests (returns 1 on hit inside [tmin,tmax]). */
+int orc_watertight(const float o[3], const float d[3], const float v0[3], const float v1[3],
+                   const float v2[3], float tmin, float tmax, float *t, float *u, float *v);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
