@@ -99,18 +99,24 @@ struct TimedRegion {
     }
 };
 
-void collect_timings(Device *dev) {
+hipError_t collect_timings(Device *dev) {
+    hipError_t err = hipSuccess;
     for (auto &p : dev->pending) {
-        hipEventSynchronize(p.b);
+        hipError_t e = hipEventSynchronize(p.b);
         float ms = 0.0f;
-        hipEventElapsedTime(&ms, p.a, p.b);
-        auto &t = dev->timings[p.name];
-        t.total_ms += ms;
-        t.launches++;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, p.a, p.b);
+        if (e == hipSuccess) {
+            auto &t = dev->timings[p.name];
+            t.total_ms += ms;
+            t.launches++;
+        } else if (err == hipSuccess) {
+            err = e;
+        }
         hipEventDestroy(p.a);
         hipEventDestroy(p.b);
     }
     dev->pending.clear();
+    return err;
 }
 
 void stream_worker(Stream *st) {
@@ -944,7 +950,7 @@ int lc_hip_timing_get(uint64_t device, const char *name, double *total_ms, uint6
     if (!dev || !name) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "bad timing query");
     DeviceGuard g(dev);
     std::lock_guard<std::mutex> lk(dev->mu);
-    collect_timings(dev);
+    LC_HIP_TRY(collect_timings(dev));
     auto it = dev->timings.find(name);
     if (total_ms) *total_ms = it == dev->timings.end() ? 0.0 : it->second.total_ms;
     if (launches) *launches = it == dev->timings.end() ? 0 : it->second.launches;
@@ -956,8 +962,9 @@ int lc_hip_timing_reset(uint64_t device) {
     if (!dev) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad device");
     DeviceGuard g(dev);
     std::lock_guard<std::mutex> lk(dev->mu);
-    collect_timings(dev);
+    hipError_t e = collect_timings(dev);
     dev->timings.clear();
+    if (e != hipSuccess) return hip_fail(e, "kernel timing");
     return LC_HIP_OK;
 }
 

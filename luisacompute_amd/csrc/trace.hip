@@ -356,7 +356,9 @@ __global__ __launch_bounds__(BLOCK) void k_trace(const TraceParams P) {
                 }
             }
             if (done) {
-                if (ANY) {
+                // the counting form has no output buffers (out_hits/out_any are null)
+                if (COUNT) {
+                } else if (ANY) {
                     P.out_any[ray_id] = any_hit ? 1u : 0u;
                 } else {
                     uint2 *h = reinterpret_cast<uint2 *>(reinterpret_cast<char *>(P.out_hits) + 24ull * ray_id);

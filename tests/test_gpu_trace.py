@@ -120,3 +120,19 @@ def test_deterministic_rebuild(device, stream):
     g2 = host.trace_closest(device, stream, a2, rays)
     assert np.array_equal(g1["prim"], g2["prim"])
     assert np.array_equal(g1["committed_ray_t"], g2["committed_ray_t"])
+
+
+def test_trace_count_mode(device, stream):
+    """Counter-mode traversal (no output buffers): node/triangle counts, and the same accel
+    still traces correctly afterwards."""
+    verts, tris = lc.icosphere(5)
+    rays = lc.rays_orbit_to_ball(65536, initseq=51)
+    accel = host.single_mesh_accel(device, stream, verts, tris)
+    rb = host.upload(device, stream, rays)
+    n_nodes, n_tris = accel.trace_count(stream, rb, len(rays))
+    assert n_nodes >= len(rays)          # every ray visits at least the root
+    assert n_tris >= len(rays) * 0.99    # nearly every ray tests its hit triangle
+    a_nodes, a_tris = accel.trace_count(stream, rb, len(rays), any_hit=True)
+    assert a_nodes <= n_nodes and a_tris <= n_tris
+    g = host.trace_closest(device, stream, accel, rays)
+    assert np.mean(g["prim"] != 0xFFFFFFFF) > 0.99
