@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 #include <cstring>
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_select.hpp>
 
 #include <cfloat>
 #include <cstdint>
@@ -83,7 +84,7 @@ __global__ void k_init(BuildCounters *ctr, uint32_t *slots, uint32_t nslots) {
         }
     }
     if (i < kMaxCollapseLevels) ctr->level_count[i] = 0;
-    for (uint32_t k = i; k < nslots; k += gridDim.x * blockDim.x) slots[k] = 0xFFFFFFFFu;
+    for (uint32_t k = i; k < nslots; k += gridDim.x * blockDim.x) slots[k] = 0u;
 }
 
 __device__ __forceinline__ float wave_min(float v) {
@@ -154,63 +155,151 @@ __global__ __launch_bounds__(kBlock) void k_morton(PrimSource src, uint32_t n, c
     ids[p] = p;
 }
 
-// Distance between sorted neighbours i and i+1 as the position of their highest differing bit;
-// equal keys fall back to the index so every split is well defined.
-__device__ __forceinline__ int split_level(const uint64_t *keys, uint32_t i) {
-    uint64_t d = keys[i] ^ keys[i + 1];
-    if (d != 0) return 64 + 63 - __clzll((long long)d);
-    return 31 - __clz((int)(i ^ (i + 1)));
+// Common-prefix length of sorted keys i and j (-1 outside [0, n)); equal keys continue with the
+// common prefix of the indices, so every pair of positions is ordered (Karras 2012, sec. 4).
+__device__ __forceinline__ int prefix_len(const uint64_t *keys, int n, int i, int j) {
+    if (j < 0 || j >= n) return -1;
+    uint64_t a = keys[i], b = keys[j];
+    if (a != b) return __clzll((long long)(a ^ b));
+    return 64 + __clz(i ^ j);
 }
 
 struct BinaryTree {
-    float4 *cbox;    // 4 per internal node: left lo, left hi, right lo, right hi
-    uint2 *child;    // per internal node: left ref, right ref
-    uint2 *range;    // per internal node: first, last sorted position
-    uint32_t *slots; // handoff word per internal node
+    float4 *cbox;          // 4 per internal node: left lo, left hi, right lo, right hi
+    uint2 *child;          // per internal node: left ref, right ref
+    uint2 *range;          // per internal node: first, last sorted position
+    uint32_t *slots;       // handoff word per internal node (0 = no child arrived yet)
+    uint32_t *parent_int;  // parent of internal node i
+    uint32_t *parent_leaf; // parent of leaf (sorted position) i
 };
 
-__global__ __launch_bounds__(kBlock) void k_lbvh(PrimSource src, uint32_t n, const uint64_t *keys, const uint32_t *ids,
-                                                 BinaryTree bt, BuildCounters *ctr, uint2 *queue0) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    float3 lo, hi;
-    prim_box(src, ids[i], lo, hi);
-    if (n == 1) {
-        ctr->root_ref = kLeafBit;
-        ctr->bounds[0] = lo.x; ctr->bounds[1] = lo.y; ctr->bounds[2] = lo.z;
-        ctr->bounds[3] = hi.x; ctr->bounds[4] = hi.y; ctr->bounds[5] = hi.z;
-        queue0[0] = make_uint2(kLeafBit, 0u);
-        ctr->level_count[0] = 1;
-        return;
+// Binary radix tree in one parallel pass: internal node i finds its key range by exponential and
+// binary search over prefix lengths and splits it at the highest differing bit (Karras,
+// "Maximizing Parallelism in the Construction of BVHs, Octrees, and k-d Trees", HPG 2012).
+// No synchronisation: every node depends on the sorted keys only. Root = internal node 0.
+__global__ __launch_bounds__(kBlock) void k_karras(uint32_t n, const uint64_t *keys, BinaryTree bt) {
+    const int N = (int)n;
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= N - 1) return;
+    const int d = prefix_len(keys, N, i, i + 1) > prefix_len(keys, N, i, i - 1) ? 1 : -1;
+    const int dmin = prefix_len(keys, N, i, i - d);
+    int lmax = 2;
+    while (prefix_len(keys, N, i, i + lmax * d) > dmin) lmax <<= 1;
+    int l = 0;
+    for (int t = lmax >> 1; t >= 1; t >>= 1)
+        if (prefix_len(keys, N, i, i + (l + t) * d) > dmin) l += t;
+    const int j = i + l * d;
+    const int dnode = prefix_len(keys, N, i, j);
+    int s = 0, t = l;
+    do {
+        t = (t + 1) >> 1;
+        if (prefix_len(keys, N, i, i + (s + t) * d) > dnode) s += t;
+    } while (t > 1);
+    const int gamma = i + s * d + min(d, 0);
+    const int lo = min(i, j), hi = max(i, j);
+    const uint32_t left = lo == gamma ? ((uint32_t)gamma | kLeafBit) : (uint32_t)gamma;
+    const uint32_t right = hi == gamma + 1 ? ((uint32_t)(gamma + 1) | kLeafBit) : (uint32_t)(gamma + 1);
+    bt.child[i] = make_uint2(left, right);
+    bt.range[i] = make_uint2((uint32_t)lo, (uint32_t)hi);
+    if (left & kLeafBit) bt.parent_leaf[gamma] = (uint32_t)i; else bt.parent_int[gamma] = (uint32_t)i;
+    if (right & kLeafBit) bt.parent_leaf[gamma + 1] = (uint32_t)i; else bt.parent_int[gamma + 1] = (uint32_t)i;
+}
+
+__device__ __forceinline__ uint32_t ref_first(const BinaryTree &bt, uint32_t r) {
+    return (r & kLeafBit) ? (r & ~kLeafBit) : bt.range[r].x;
+}
+__device__ __forceinline__ uint32_t ref_last(const BinaryTree &bt, uint32_t r) {
+    return (r & kLeafBit) ? (r & ~kLeafBit) : bt.range[r].y;
+}
+
+__device__ __forceinline__ void publish_root(BuildCounters *ctr, uint2 *queue0, uint32_t root, float3 lo, float3 hi) {
+    ctr->root_ref = root;
+    ctr->bounds[0] = lo.x; ctr->bounds[1] = lo.y; ctr->bounds[2] = lo.z;
+    ctr->bounds[3] = hi.x; ctr->bounds[4] = hi.y; ctr->bounds[5] = hi.z;
+    queue0[0] = make_uint2(root, 0u);
+    ctr->level_count[0] = 1;
+}
+
+constexpr uint32_t kNoEntry = 0xFFFFFFFFu;
+
+// Bottom-up boxes, one phase. The workgroup owns entries [e0, e1) -- complete subtrees ordered by
+// first leaf -- and so the leaf range [L0, L1]. A thread climbs from its entry while the parent's
+// key range lies inside [L0, L1]; then both children of that parent are climbed by threads of the
+// same workgroup, and the handoff (the first child to arrive stops, the second merges the boxes
+// and goes on) needs only workgroup-scope ordering. A parent whose range leaves the workgroup is
+// deferred: the child is written to pend[first leaf] and becomes an entry of the next phase.
+// whole != 0: the workgroup owns all leaves (last phase, one workgroup) or, with agent scope, the
+// fallback for inputs whose phases stop shrinking.
+template<bool AGENT>
+__global__ void k_bounds_phase(PrimSource src, uint32_t n, const uint32_t *ids, BinaryTree bt,
+                               const uint32_t *entries, uint32_t m, uint32_t *pend, BuildCounters *ctr,
+                               uint2 *queue0, uint32_t whole) {
+    const uint32_t e0 = blockIdx.x * blockDim.x;
+    if (e0 >= m) return;
+    const uint32_t e1 = min(e0 + blockDim.x, m);
+    uint32_t L0 = 0, L1 = n - 1;
+    if (!whole) {
+        L0 = ref_first(bt, entries ? entries[e0] : (e0 | kLeafBit));
+        L1 = ref_last(bt, entries ? entries[e1 - 1] : ((e1 - 1) | kLeafBit));
     }
-    uint32_t cur = i | kLeafBit, l = i, r = i;
+    const uint32_t e = e0 + threadIdx.x;
+    if (e >= e1) return;
+    uint32_t cur = entries ? entries[e] : (e | kLeafBit);
+    float3 lo, hi;
+    if (cur & kLeafBit) {
+        prim_box(src, ids[cur & ~kLeafBit], lo, hi);
+    } else {
+        const float4 *cb = bt.cbox + 4 * (uint64_t)cur;
+        float4 a = cb[0], b = cb[1], c = cb[2], d = cb[3];
+        lo = make_float3(fminf(a.x, c.x), fminf(a.y, c.y), fminf(a.z, c.z));
+        hi = make_float3(fmaxf(b.x, d.x), fmaxf(b.y, d.y), fmaxf(b.z, d.z));
+    }
     for (;;) {
-        bool is_left = (l == 0) || (r != n - 1 && split_level(keys, r) < split_level(keys, l - 1));
-        uint32_t parent = is_left ? r : l - 1;
-        uint32_t side = is_left ? 0u : 1u;
-        reinterpret_cast<uint32_t *>(&bt.child[parent])[side] = cur;
-        bt.cbox[4 * (uint64_t)parent + 2 * side] = make_float4(lo.x, lo.y, lo.z, 0.0f);
-        bt.cbox[4 * (uint64_t)parent + 2 * side + 1] = make_float4(hi.x, hi.y, hi.z, 0.0f);
-        uint32_t mine = is_left ? l : r;
-        uint32_t other = __hip_atomic_exchange(&bt.slots[parent], mine, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (other == 0xFFFFFFFFu) return; // the sibling finishes this node
-        float4 slo = bt.cbox[4 * (uint64_t)parent + 2 * (1u - side)];
-        float4 shi = bt.cbox[4 * (uint64_t)parent + 2 * (1u - side) + 1];
-        lo = make_float3(fminf(lo.x, slo.x), fminf(lo.y, slo.y), fminf(lo.z, slo.z));
-        hi = make_float3(fmaxf(hi.x, shi.x), fmaxf(hi.y, shi.y), fmaxf(hi.z, shi.z));
-        if (is_left) r = other; else l = other;
-        bt.range[parent] = make_uint2(l, r);
-        cur = parent;
-        if (l == 0 && r == n - 1) {
-            ctr->root_ref = parent;
-            ctr->bounds[0] = lo.x; ctr->bounds[1] = lo.y; ctr->bounds[2] = lo.z;
-            ctr->bounds[3] = hi.x; ctr->bounds[4] = hi.y; ctr->bounds[5] = hi.z;
-            queue0[0] = make_uint2(parent, 0u);
-            ctr->level_count[0] = 1;
+        if (cur == 0u) { // internal node 0 spans every leaf: the root
+            publish_root(ctr, queue0, 0u, lo, hi);
             return;
         }
+        const uint32_t p = (cur & kLeafBit) ? bt.parent_leaf[cur & ~kLeafBit] : bt.parent_int[cur];
+        const uint2 pr = bt.range[p];
+        if (pr.x < L0 || pr.y > L1) {
+            pend[ref_first(bt, cur)] = cur;
+            return;
+        }
+        const uint32_t side = bt.child[p].x == cur ? 0u : 1u;
+        bt.cbox[4 * (uint64_t)p + 2 * side] = make_float4(lo.x, lo.y, lo.z, 0.0f);
+        bt.cbox[4 * (uint64_t)p + 2 * side + 1] = make_float4(hi.x, hi.y, hi.z, 0.0f);
+        const uint32_t first = AGENT
+            ? __hip_atomic_exchange(&bt.slots[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT)
+            : __hip_atomic_exchange(&bt.slots[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (first == 0u) return; // the sibling's thread completes p
+        const float4 slo = bt.cbox[4 * (uint64_t)p + 2 * (1u - side)];
+        const float4 shi = bt.cbox[4 * (uint64_t)p + 2 * (1u - side) + 1];
+        lo = make_float3(fminf(lo.x, slo.x), fminf(lo.y, slo.y), fminf(lo.z, slo.z));
+        hi = make_float3(fmaxf(hi.x, shi.x), fmaxf(hi.y, shi.y), fmaxf(hi.z, shi.z));
+        cur = p;
     }
 }
+
+__global__ void k_single_leaf_root(PrimSource src, const uint32_t *ids, BuildCounters *ctr, uint2 *queue0) {
+    float3 lo, hi;
+    prim_box(src, ids[0], lo, hi);
+    publish_root(ctr, queue0, kLeafBit, lo, hi);
+}
+
+__global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+// Clears the pend slots of the entries just compacted out of it.
+__global__ void k_clear_pend(const uint32_t *entries, const uint32_t *count, uint32_t *pend, BinaryTree bt) {
+    const uint32_t m = *count;
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < m; e += gridDim.x * blockDim.x)
+        pend[ref_first(bt, entries[e])] = kNoEntry;
+}
+
+struct IsEntry {
+    __device__ bool operator()(uint32_t v) const { return v != kNoEntry; }
+};
 
 struct WideOut {
     WideNode *nodes;
@@ -432,6 +521,13 @@ static size_t sort_temp_bytes(uint32_t n) {
     return bytes;
 }
 
+static size_t select_temp_bytes(uint32_t n) {
+    size_t bytes = 0;
+    rocprim::select(nullptr, bytes, (const uint32_t *)nullptr, (uint32_t *)nullptr, (uint32_t *)nullptr, n, IsEntry{},
+                    (hipStream_t)0);
+    return bytes;
+}
+
 size_t bvh_scratch_bytes(uint32_t n) {
     size_t m = std::max<uint32_t>(n, 2u);
     size_t b = 0;
@@ -441,9 +537,12 @@ size_t bvh_scratch_bytes(uint32_t n) {
     b += m * sizeof(uint2) + 256;                   // child
     b += m * sizeof(uint2) + 256;                   // range
     b += m * sizeof(uint32_t) + 256;                // slots
-    b += 2 * m * sizeof(uint2) + 256 * 2;           // queues
+    b += 2 * m * sizeof(uint32_t) + 256 * 2;        // parents
+    b += 3 * m * sizeof(uint32_t) + 256 * 3;        // pend + two entry lists
+    b += 2 * m * sizeof(uint2) + 256 * 2;           // collapse queues
     b += sizeof(BuildCounters) + 256;               // counters
-    b += sort_temp_bytes((uint32_t)m) + 256;        // rocPRIM temp
+    b += sizeof(uint32_t) + 256;                    // selected count
+    b += std::max(sort_temp_bytes((uint32_t)m), select_temp_bytes((uint32_t)m)) + 256; // rocPRIM temp
     return b;
 }
 
@@ -451,11 +550,13 @@ size_t bvh_node_capacity(uint32_t n) { return std::max<uint32_t>(n, 2u); }
 
 // Builds a wide BVH over n primitives (triangles if boxes == nullptr, else boxes).
 // Output arrays: nodes (bvh_node_capacity(n) entries), tris (n entries) or prim_out (n entries).
-// The counters land in `counters_out` (device) for the caller to read back.
+// The counters land in `counters_out` (device) for the caller to read back. `pinned_word` is a
+// host-pinned word used to read the per-phase entry counts (the stream is synchronized once per
+// bottom-up phase; about log_64(n) phases).
 hipError_t bvh_build(hipStream_t stream, void *scratch, size_t scratch_bytes, const void *vbase, uint64_t vstride,
                      uint64_t nverts, const uint32_t *tri, const float4 *boxes, uint32_t n, WideNode *nodes,
                      TriRecord *tris, uint32_t *prim_out, BuildCounters **counters_out, uint32_t split_small,
-                     int grid_collapse, uint32_t *error_flag) {
+                     int grid_collapse, uint32_t *error_flag, uint32_t *pinned_word) {
     if (scratch_bytes < bvh_scratch_bytes(n)) return hipErrorInvalidValue;
     size_t m = std::max<uint32_t>(n, 2u);
     char *p = static_cast<char *>(scratch);
@@ -466,9 +567,14 @@ hipError_t bvh_build(hipStream_t stream, void *scratch, size_t scratch_bytes, co
     bt.child = carve<uint2>(p, m);
     bt.range = carve<uint2>(p, m);
     bt.slots = carve<uint32_t>(p, m);
+    bt.parent_int = carve<uint32_t>(p, m);
+    bt.parent_leaf = carve<uint32_t>(p, m);
+    uint32_t *pend = carve<uint32_t>(p, m);
+    uint32_t *ent[2] = {carve<uint32_t>(p, m), carve<uint32_t>(p, m)};
     uint2 *q0 = carve<uint2>(p, m), *q1 = carve<uint2>(p, m);
     BuildCounters *ctr = carve<BuildCounters>(p, 1);
-    size_t temp_bytes = sort_temp_bytes((uint32_t)m);
+    uint32_t *dcount = carve<uint32_t>(p, 1);
+    size_t temp_bytes = std::max(sort_temp_bytes((uint32_t)m), select_temp_bytes((uint32_t)m));
     void *temp = carve<char>(p, temp_bytes);
     *counters_out = ctr;
 
@@ -483,11 +589,53 @@ hipError_t bvh_build(hipStream_t stream, void *scratch, size_t scratch_bytes, co
     hipLaunchKernelGGL(k_morton, dim3(grid), dim3(kBlock), 0, stream, src, n, ctr, k0, i0);
     rocprim::double_buffer<uint64_t> kb(k0, k1);
     rocprim::double_buffer<uint32_t> vb(i0, i1);
-    hipError_t e = rocprim::radix_sort_pairs(temp, temp_bytes, kb, vb, n, 0, 63, stream);
+    size_t tb = temp_bytes;
+    hipError_t e = rocprim::radix_sort_pairs(temp, tb, kb, vb, n, 0, 63, stream);
     if (e != hipSuccess) return e;
     const uint64_t *keys = kb.current();
     const uint32_t *ids = vb.current();
-    hipLaunchKernelGGL(k_lbvh, dim3(grid), dim3(kBlock), 0, stream, src, n, keys, ids, bt, ctr, q0);
+    if (n == 1) {
+        hipLaunchKernelGGL(k_single_leaf_root, dim3(1), dim3(1), 0, stream, src, ids, ctr, q0);
+    } else {
+        hipLaunchKernelGGL(k_karras, dim3((n - 1 + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, n, keys, bt);
+        hipLaunchKernelGGL(k_fill_u32, dim3(std::min<uint32_t>(grid, 4096)), dim3(kBlock), 0, stream, pend, (uint64_t)n,
+                           kNoEntry);
+        // Bottom-up phases. Entries of phase 0 are the leaves in sorted order (entries == null).
+        const uint32_t kPhaseBlock = 256, kLastBlock = 1024;
+        const uint32_t *entries = nullptr;
+        uint32_t count = n;
+        int which = 0;
+        for (;;) {
+            if (count <= kLastBlock) {
+                uint32_t blk = (count + 63) / 64 * 64;
+                hipLaunchKernelGGL(k_bounds_phase<false>, dim3(1), dim3(blk), 0, stream, src, n, ids, bt, entries, count,
+                                   pend, ctr, q0, 1u);
+                break;
+            }
+            hipLaunchKernelGGL(k_bounds_phase<false>, dim3((count + kPhaseBlock - 1) / kPhaseBlock), dim3(kPhaseBlock), 0,
+                               stream, src, n, ids, bt, entries, count, pend, ctr, q0, 0u);
+            tb = temp_bytes;
+            e = rocprim::select(temp, tb, (const uint32_t *)pend, ent[which], dcount, n, IsEntry{}, stream);
+            if (e != hipSuccess) return e;
+            hipLaunchKernelGGL(k_clear_pend, dim3(256), dim3(kBlock), 0, stream, ent[which], dcount, pend, bt);
+            e = hipMemcpyAsync(pinned_word, dcount, sizeof(uint32_t), hipMemcpyDeviceToHost, stream);
+            if (e != hipSuccess) return e;
+            e = hipStreamSynchronize(stream);
+            if (e != hipSuccess) return e;
+            const uint32_t next = *pinned_word;
+            entries = ent[which];
+            which ^= 1;
+            if (next == 0) break; // the root was reached in this phase
+            if (next > count - count / 8) {
+                // Phases stopped shrinking (sibling subtrees keep straddling workgroup
+                // boundaries): finish with agent-scope handoffs over all remaining entries.
+                hipLaunchKernelGGL(k_bounds_phase<true>, dim3((next + kPhaseBlock - 1) / kPhaseBlock), dim3(kPhaseBlock),
+                                   0, stream, src, n, ids, bt, entries, next, pend, ctr, q0, 1u);
+                break;
+            }
+            count = next;
+        }
+    }
     WideOut out{nodes, tris, prim_out};
     // A root-to-leaf path of the radix tree crosses each of the 63 key bits and 32 index bits
     // at most once, and every wide level descends at least one binary level, so the wide depth

@@ -24,7 +24,7 @@ size_t bvh_node_capacity(uint32_t n);
 hipError_t bvh_build(hipStream_t stream, void *scratch, size_t scratch_bytes, const void *vbase, uint64_t vstride,
                      uint64_t nverts, const uint32_t *tri, const float4 *boxes, uint32_t n, WideNode *nodes,
                      TriRecord *tris, uint32_t *prim_out, BuildCounters **counters_out, uint32_t split_small,
-                     int grid_collapse, uint32_t *error_flag);
+                     int grid_collapse, uint32_t *error_flag, uint32_t *pinned_word);
 
 // ---- kernels (trace.hip) ----
 struct TraceParams {
@@ -117,6 +117,7 @@ struct HostCounters {
     float bounds[6];
     uint32_t tail_level_count; // level_count[kCollapseLaunches]: must be 0
     uint32_t error;
+    uint32_t phase_count;      // bottom-up phase entry count read back during a build
 };
 
 struct Mesh {

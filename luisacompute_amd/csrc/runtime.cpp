@@ -567,7 +567,7 @@ int lc_hip_mesh_build(uint64_t device, uint64_t stream, uint64_t mesh, uint32_t,
                                  vertex_stride, nverts,
                                  reinterpret_cast<const uint32_t *>(static_cast<char *>(tb->ptr) + triangle_offset),
                                  nullptr, n, m->nodes, m->tris, nullptr, &ctr, dev->split_small, 4 * dev->cu_count,
-                                 dev->error);
+                                 dev->error, &m->host->phase_count);
         if (e != hipSuccess) return hip_fail(e, "bvh_build");
         copy_counters_async(ctr, m->host, dev->error, s);
     }
@@ -755,7 +755,7 @@ int lc_hip_accel_build(uint64_t device, uint64_t stream, uint64_t accel, uint32_
             BuildCounters *ctr = nullptr;
             hipError_t e = bvh_build(s, a->tlas_scratch, a->tlas_scratch_bytes, nullptr, 0, 0, nullptr, a->boxes,
                                      instance_count, a->tlas_nodes, nullptr, a->tlas_prims, &ctr, 0,
-                                     std::max(1, dev->cu_count), dev->error);
+                                     std::max(1, dev->cu_count), dev->error, &a->tlas_host->phase_count);
             if (e != hipSuccess) return hip_fail(e, "tlas build");
             copy_counters_async(ctr, a->tlas_host, dev->error, s);
         }
