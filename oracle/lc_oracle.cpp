@@ -61,7 +61,13 @@ void parallel_for(uint64_t n, uint64_t chunk, int nthreads, const std::function<
         }
     };
     int nt = (int)std::min<uint64_t>((uint64_t)nthreads, nchunks);
-    if (nt <= 1) { worker(); return; }
+    if (nt <= 1) {
+        // run on the caller's thread without leaving FTZ/DAZ behind in it
+        const unsigned int csr = _mm_getcsr();
+        worker();
+        _mm_setcsr(csr);
+        return;
+    }
     std::vector<std::thread> ts;
     for (int i = 0; i < nt; i++) ts.emplace_back(worker);
     for (auto &t : ts) t.join();
@@ -753,10 +759,13 @@ void orc_brute_closest_f64(const float *verts, uint64_t vstride_bytes, const uin
 
 int orc_watertight(const float o[3], const float d[3], const float v0[3], const float v1[3], const float v2[3],
                    float tmin, float tmax, float *t, float *u, float *v) {
+    const unsigned int csr = _mm_getcsr();
     set_ftz_daz();
     WtRay r;
     wt_setup(r, o, d);
-    return wt_intersect(r, v0, v1, v2, tmin, tmax, *t, *u, *v) ? 1 : 0;
+    const int hit = wt_intersect(r, v0, v1, v2, tmin, tmax, *t, *u, *v) ? 1 : 0;
+    _mm_setcsr(csr);
+    return hit;
 }
 
 } // extern "C"
