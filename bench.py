@@ -83,6 +83,8 @@ def main():
     gen_s = time.perf_counter() - t0
 
     dev = lc.Device(0)
+    from luisacompute_amd import autotune
+    tuned = autotune.load_default(dev)  # block size / LDS stack / refill steps from the autotuner
     s = dev.create_stream()
     mesh = host.mesh_from_arrays(dev, s, verts, tris, build=False)
     accel = dev.create_accel()
@@ -180,6 +182,8 @@ def main():
             "compulsory_floor_gbs": round(floor_gbs, 2),
         },
         "input_gen_s": round(gen_s, 2),
+        "trace_config": {"closest": dev.trace_config("closest"), "any": dev.trace_config("any"),
+                         "source": "luisacompute_amd/autotune_table.json" if tuned else "defaults"},
     }
     if not a.no_cpu:
         result["cpu_baseline"] = cpu_baseline(verts, tris, lc.rays_as_floats(rays), a.cpu_sample_rays, a.rays)

@@ -184,6 +184,11 @@ typedef struct lc_hip_bvh_stats {
     uint64_t primitive_bytes;
 } lc_hip_bvh_stats;
 int lc_hip_mesh_stats(uint64_t device, uint64_t mesh, lc_hip_bvh_stats *out);
+/* Checks a built BLAS on the device. errors[k] counts violations of: 0 child index range,
+   1 triangle range, 2 leaf triangle inside its quantised slot box, 3 child node inside its parent
+   slot box, 4 each triangle record referenced once, 5 each primitive covered once, 6 empty slots
+   empty. All zero for a well-formed tree. */
+int lc_hip_mesh_validate(uint64_t device, uint64_t stream, uint64_t mesh, uint32_t errors[7]);
 int lc_hip_accel_stats(uint64_t device, uint64_t accel, lc_hip_bvh_stats *out);
 /* kind 0 = closest, 1 = any. block in {64,128,256}; lds_stack in {8,16,24}; steps >= 1;
    blocks_per_cu 0 = occupancy limit. */

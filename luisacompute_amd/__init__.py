@@ -108,6 +108,7 @@ _SIGNATURES = {
     "lc_hip_accel_trace_count": (ctypes.c_int, [_U64, _U64, _U64, _U64, _SZ, _U32, _U32, ctypes.c_int,
                                                 ctypes.POINTER(_U64), ctypes.POINTER(_U64)]),
     "lc_hip_mesh_stats": (ctypes.c_int, [_U64, _U64, ctypes.POINTER(BvhStats)]),
+    "lc_hip_mesh_validate": (ctypes.c_int, [_U64, _U64, _U64, ctypes.POINTER(_U32)]),
     "lc_hip_accel_stats": (ctypes.c_int, [_U64, _U64, ctypes.POINTER(BvhStats)]),
     "lc_hip_trace_set_config": (ctypes.c_int, [_U64, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "lc_hip_trace_get_config": (ctypes.c_int, [_U64, ctypes.c_int] + [ctypes.POINTER(ctypes.c_int)] * 4),
@@ -418,6 +419,15 @@ class Mesh:
         s = BvhStats()
         _check(lib().lc_hip_mesh_stats(self.device.handle, self.handle, ctypes.byref(s)), "mesh stats")
         return s
+
+    VALIDATION_CHECKS = ("child_index", "triangle_range", "leaf_in_slot_box", "child_in_slot_box",
+                         "record_once", "primitive_once", "empty_slot")
+
+    def validate(self, stream):
+        """Device-side BVH invariant check; returns {check: violation count} (all zero when sound)."""
+        e = (ctypes.c_uint32 * 7)()
+        _check(lib().lc_hip_mesh_validate(self.device.handle, stream.handle, self.handle, e), "mesh validate")
+        return dict(zip(self.VALIDATION_CHECKS, e[:]))
 
 
 class Accel:

@@ -1,0 +1,92 @@
+"""Traversal autotuner: block size, LDS stack depth, steps between ray refills and blocks per CU,
+chosen per trace kind (closest / any) by timing the real kernels on a workload with HIP events.
+
+The table is JSON ({"closest": {...}, "any": {...}, "device": ..., "workload": ...}) and is applied
+with `apply(device, table)`. `python -m luisacompute_amd.autotune --out table.json` tunes on the
+soup workload (scaled by --tris/--rays).
+"""
+import argparse
+import itertools
+import json
+import os
+
+from . import Device, IDENTITY
+from . import host
+
+BLOCKS = (64, 128, 256)
+STACKS = (8, 16, 24)
+STEPS = (8, 32, 128)
+BLOCKS_PER_CU = (0,)
+
+DEFAULT_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "autotune_table.json")
+
+
+def apply(device, table):
+    for kind in ("closest", "any"):
+        c = table.get(kind)
+        if c:
+            device.set_trace_config(kind, c["block"], c["lds_stack"], c["steps"], c.get("blocks_per_cu", 0))
+
+
+def load_default(device):
+    """Applies the shipped table if present; returns it (or None)."""
+    if os.path.exists(DEFAULT_TABLE):
+        with open(DEFAULT_TABLE) as f:
+            t = json.load(f)
+        apply(device, t)
+        return t
+    return None
+
+
+def tune(device, stream, accel, ray_buffer, hit_buffer, any_buffer, count, repeats=3, space=None, log=print):
+    space = space or list(itertools.product(BLOCKS, STACKS, STEPS, BLOCKS_PER_CU))
+    result = {}
+    for kind, buf in (("closest", hit_buffer), ("any", any_buffer)):
+        best = None
+        for block, slds, steps, bpc in space:
+            device.set_trace_config(kind, block, slds, steps, bpc)
+            fn = accel.trace_closest if kind == "closest" else accel.trace_any
+            fn(stream, ray_buffer, buf, count)  # warm
+            stream.synchronize()
+            device.reset_timing()
+            device.enable_timing(True)
+            for _ in range(repeats):
+                fn(stream, ray_buffer, buf, count)
+            stream.synchronize()
+            device.enable_timing(False)
+            ms, n = device.timing("trace_" + kind)
+            ms /= max(n, 1)
+            log(f"  {kind:7s} block={block:3d} stack={slds:2d} steps={steps:3d} bpc={bpc}: {ms:.3f} ms "
+                f"({count / ms / 1e3:.1f} Mrays/s)")
+            if best is None or ms < best[0]:
+                best = (ms, dict(block=block, lds_stack=slds, steps=steps, blocks_per_cu=bpc))
+        result[kind] = dict(best[1], ms=round(best[0], 4), mrays_s=round(count / best[0] / 1e3, 2))
+        device.set_trace_config(kind, **{k: v for k, v in best[1].items()})
+    return result
+
+
+def main():
+    import luisacompute_amd as lc
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--tris", type=int, default=1 << 24)
+    ap.add_argument("--rays", type=int, default=1 << 23)
+    ap.add_argument("--out", default=DEFAULT_TABLE)
+    a = ap.parse_args()
+    dev = Device(0)
+    s = dev.create_stream()
+    v, t = lc.soup(a.tris)
+    rays = lc.rays_box_sphere(a.rays, skip=13 * a.tris)
+    accel = host.single_mesh_accel(dev, s, v, t)
+    rb = host.upload(dev, s, rays)
+    hb = dev.create_buffer(24, a.rays)
+    ab = dev.create_buffer(1, a.rays)
+    res = tune(dev, s, accel, rb, hb, ab, a.rays)
+    res["device"] = dev.query("device_name")
+    res["workload"] = f"soup {a.tris} tris, {a.rays} rays"
+    with open(a.out, "w") as f:
+        json.dump(res, f, indent=1)
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

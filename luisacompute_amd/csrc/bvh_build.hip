@@ -1,18 +1,19 @@
-// bvh_build.hip -- GPU BVH construction for gfx950: Morton LBVH -> greedy 8-wide collapse with
+// bvh_build.hip -- GPU BVH construction for gfx950: Morton LBVH -> 8-wide collapse with
 // conservatively quantised child boxes.
 //
 // Replaces the reference's OptiX BLAS/IAS build (src/backends/cuda/cuda_primitive.cpp:20-113,
 // cuda_accel.cpp:26-126) and Embree's rtcCommitScene (src/rust/luisa_compute_backend_impl/src/
-// cpu/accel.rs:205-260). Neither reference path has a software builder; the algorithm here is:
-//   1. centroid bounds (wave64 shuffle reduction + one atomic per block),
-//   2. 63-bit Morton codes, radix sort (rocPRIM, keys + primitive ids),
-//   3. binary radix tree and its boxes in one bottom-up pass (Apetrei, "Fast and Simple
-//      Agglomerative LBVH Construction", CGVC 2014) with acquire/release handoff between the two
-//      children of each node,
-//   4. top-down collapse to 8-wide nodes, one launch per wide level with device-side work
-//      counts (no host round trip between levels); each node greedily opens its widest child
-//      until it has 8 children, assigns children to octant slots, quantises child boxes to
-//      8 bits per plane and copies its leaf triangles into a contiguous block.
+// cpu/accel.rs:205-260). Neither reference path has a software builder; the steps here are:
+//   1. centroid bounds (wave64 shuffle reduction + one atomic per block); primitive boxes are
+//      stored once, in input order, by the Morton kernel;
+//   2. 30-bit Morton codes (10 bits per axis), radix sort of (code, primitive id) (rocPRIM);
+//   3. binary radix tree from the sorted codes alone (Karras 2012), equal codes ordered by index;
+//   4. node boxes bottom-up in phases whose child handoffs stay inside one workgroup (so they
+//      only need workgroup-scope ordering -- an agent-scope release/acquire writes back and
+//      invalidates the XCD-local L2 and made a single-pass climb 9x slower than the whole build);
+//   5. top-down collapse to 8-wide nodes, one launch per wide level, 8 lanes per node: each lane
+//      holds one child, the group opens its widest child until it has 8, assigns children to
+//      octant slots, quantises each child box to 8 bits per plane and copies leaf triangles.
 // The same builder serves the TLAS (primitives = instance world boxes).
 #include <hip/hip_runtime.h>
 #include <cstring>
@@ -30,6 +31,7 @@ namespace lchip {
 namespace {
 
 constexpr int kBlock = 256;
+constexpr uint32_t kNoEntry = 0xFFFFFFFFu;
 
 __device__ __forceinline__ int f2ord(float f) {
     int i = __float_as_int(f);
@@ -67,6 +69,13 @@ __device__ __forceinline__ void prim_box(const PrimSource &s, uint32_t p, float3
     float4 a = load_vertex(s, i0), b = load_vertex(s, i1), c = load_vertex(s, i2);
     lo = make_float3(fminf(a.x, fminf(b.x, c.x)), fminf(a.y, fminf(b.y, c.y)), fminf(a.z, fminf(b.z, c.z)));
     hi = make_float3(fmaxf(a.x, fmaxf(b.x, c.x)), fmaxf(a.y, fmaxf(b.y, c.y)), fmaxf(a.z, fmaxf(b.z, c.z)));
+}
+
+// Stored primitive box (written by k_morton; for box inputs, the input itself).
+__device__ __forceinline__ void stored_box(const float4 *pbox, uint32_t p, float3 &lo, float3 &hi) {
+    float4 a = pbox[2 * (uint64_t)p], b = pbox[2 * (uint64_t)p + 1];
+    lo = make_float3(a.x, a.y, a.z);
+    hi = make_float3(b.x, b.y, b.z);
 }
 
 __global__ void k_init(BuildCounters *ctr, uint32_t *slots, uint32_t nslots) {
@@ -121,18 +130,18 @@ __global__ __launch_bounds__(kBlock) void k_centroid_bounds(PrimSource src, uint
     }
 }
 
-__device__ __forceinline__ uint64_t expand21(uint32_t v) {
-    uint64_t x = v & 0x1fffffu;
-    x = (x | x << 32) & 0x1f00000000ffffull;
-    x = (x | x << 16) & 0x1f0000ff0000ffull;
-    x = (x | x << 8) & 0x100f00f00f00f00full;
-    x = (x | x << 4) & 0x10c30c30c30c30c3ull;
-    x = (x | x << 2) & 0x1249249249249249ull;
-    return x;
+__device__ __forceinline__ uint32_t expand10(uint32_t v) {
+    v &= 0x3FFu;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
 }
 
+// 30-bit Morton code of the box centroid in the centroid bounds; stores the box for later passes.
 __global__ __launch_bounds__(kBlock) void k_morton(PrimSource src, uint32_t n, const BuildCounters *ctr,
-                                                   uint64_t *keys, uint32_t *ids) {
+                                                   uint32_t *keys, uint32_t *ids, float4 *pbox) {
     uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     float lo[3], sc[3];
@@ -140,28 +149,32 @@ __global__ __launch_bounds__(kBlock) void k_morton(PrimSource src, uint32_t n, c
         float l = ord2f(ctr->cbounds[a]), h = ord2f(ctr->cbounds[3 + a]);
         float ext = h - l;
         lo[a] = l;
-        sc[a] = ext > 0.0f ? 2097152.0f / ext : 0.0f;
+        sc[a] = ext > 0.0f ? 1024.0f / ext : 0.0f;
     }
     float3 blo, bhi;
     prim_box(src, p, blo, bhi);
+    if (!src.boxes) {
+        pbox[2 * (uint64_t)p] = make_float4(blo.x, blo.y, blo.z, 0.0f);
+        pbox[2 * (uint64_t)p + 1] = make_float4(bhi.x, bhi.y, bhi.z, 0.0f);
+    }
     float c[3] = {0.5f * (blo.x + bhi.x), 0.5f * (blo.y + bhi.y), 0.5f * (blo.z + bhi.z)};
     uint32_t q[3];
     for (int a = 0; a < 3; a++) {
         float f = (c[a] - lo[a]) * sc[a];
-        f = fminf(fmaxf(f, 0.0f), 2097151.0f);
+        f = fminf(fmaxf(f, 0.0f), 1023.0f);
         q[a] = (uint32_t)f;
     }
-    keys[p] = expand21(q[0]) << 2 | expand21(q[1]) << 1 | expand21(q[2]);
+    keys[p] = expand10(q[0]) << 2 | expand10(q[1]) << 1 | expand10(q[2]);
     ids[p] = p;
 }
 
 // Common-prefix length of sorted keys i and j (-1 outside [0, n)); equal keys continue with the
 // common prefix of the indices, so every pair of positions is ordered (Karras 2012, sec. 4).
-__device__ __forceinline__ int prefix_len(const uint64_t *keys, int n, int i, int j) {
+__device__ __forceinline__ int prefix_len(const uint32_t *keys, int n, int i, int j) {
     if (j < 0 || j >= n) return -1;
-    uint64_t a = keys[i], b = keys[j];
-    if (a != b) return __clzll((long long)(a ^ b));
-    return 64 + __clz(i ^ j);
+    uint32_t a = keys[i], b = keys[j];
+    if (a != b) return __clz((int)(a ^ b));
+    return 32 + __clz(i ^ j);
 }
 
 struct BinaryTree {
@@ -177,7 +190,7 @@ struct BinaryTree {
 // binary search over prefix lengths and splits it at the highest differing bit (Karras,
 // "Maximizing Parallelism in the Construction of BVHs, Octrees, and k-d Trees", HPG 2012).
 // No synchronisation: every node depends on the sorted keys only. Root = internal node 0.
-__global__ __launch_bounds__(kBlock) void k_karras(uint32_t n, const uint64_t *keys, BinaryTree bt) {
+__global__ __launch_bounds__(kBlock) void k_karras(uint32_t n, const uint32_t *keys, BinaryTree bt) {
     const int N = (int)n;
     const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (i >= N - 1) return;
@@ -211,6 +224,11 @@ __device__ __forceinline__ uint32_t ref_first(const BinaryTree &bt, uint32_t r) 
 __device__ __forceinline__ uint32_t ref_last(const BinaryTree &bt, uint32_t r) {
     return (r & kLeafBit) ? (r & ~kLeafBit) : bt.range[r].y;
 }
+__device__ __forceinline__ uint32_t ref_count(const BinaryTree &bt, uint32_t r) {
+    if (r & kLeafBit) return 1u;
+    const uint2 rg = bt.range[r];
+    return rg.y - rg.x + 1u;
+}
 
 __device__ __forceinline__ void publish_root(BuildCounters *ctr, uint2 *queue0, uint32_t root, float3 lo, float3 hi) {
     ctr->root_ref = root;
@@ -219,8 +237,6 @@ __device__ __forceinline__ void publish_root(BuildCounters *ctr, uint2 *queue0, 
     queue0[0] = make_uint2(root, 0u);
     ctr->level_count[0] = 1;
 }
-
-constexpr uint32_t kNoEntry = 0xFFFFFFFFu;
 
 // Bottom-up boxes, one phase. The workgroup owns entries [e0, e1) -- complete subtrees ordered by
 // first leaf -- and so the leaf range [L0, L1]. A thread climbs from its entry while the parent's
@@ -231,7 +247,7 @@ constexpr uint32_t kNoEntry = 0xFFFFFFFFu;
 // whole != 0: the workgroup owns all leaves (last phase, one workgroup) or, with agent scope, the
 // fallback for inputs whose phases stop shrinking.
 template<bool AGENT>
-__global__ void k_bounds_phase(PrimSource src, uint32_t n, const uint32_t *ids, BinaryTree bt,
+__global__ void k_bounds_phase(const float4 *pbox, uint32_t n, const uint32_t *ids, BinaryTree bt,
                                const uint32_t *entries, uint32_t m, uint32_t *pend, BuildCounters *ctr,
                                uint2 *queue0, uint32_t whole) {
     const uint32_t e0 = blockIdx.x * blockDim.x;
@@ -247,7 +263,7 @@ __global__ void k_bounds_phase(PrimSource src, uint32_t n, const uint32_t *ids, 
     uint32_t cur = entries ? entries[e] : (e | kLeafBit);
     float3 lo, hi;
     if (cur & kLeafBit) {
-        prim_box(src, ids[cur & ~kLeafBit], lo, hi);
+        stored_box(pbox, ids[cur & ~kLeafBit], lo, hi);
     } else {
         const float4 *cb = bt.cbox + 4 * (uint64_t)cur;
         float4 a = cb[0], b = cb[1], c = cb[2], d = cb[3];
@@ -280,9 +296,9 @@ __global__ void k_bounds_phase(PrimSource src, uint32_t n, const uint32_t *ids, 
     }
 }
 
-__global__ void k_single_leaf_root(PrimSource src, const uint32_t *ids, BuildCounters *ctr, uint2 *queue0) {
+__global__ void k_single_leaf_root(const float4 *pbox, const uint32_t *ids, BuildCounters *ctr, uint2 *queue0) {
     float3 lo, hi;
-    prim_box(src, ids[0], lo, hi);
+    stored_box(pbox, ids[0], lo, hi);
     publish_root(ctr, queue0, kLeafBit, lo, hi);
 }
 
@@ -343,163 +359,299 @@ __device__ __forceinline__ uint32_t quant_hi(float v, float p, float s) {
     return qi;
 }
 
-// One wide level: each work item (binary ref, wide slot) becomes one WideNode.
-__global__ __launch_bounds__(kBlock) void k_collapse(PrimSource src, const uint32_t *ids, BinaryTree bt,
-                                                     BuildCounters *ctr, uint32_t level, const uint2 *qin,
-                                                     uint2 *qout, WideOut out, uint32_t split_small) {
+// Reductions over the 8 lanes of a collapse group.
+__device__ __forceinline__ float g8_min(float v) {
+    v = fminf(v, __shfl_xor(v, 1, 8)); v = fminf(v, __shfl_xor(v, 2, 8)); return fminf(v, __shfl_xor(v, 4, 8));
+}
+__device__ __forceinline__ float g8_max(float v) {
+    v = fmaxf(v, __shfl_xor(v, 1, 8)); v = fmaxf(v, __shfl_xor(v, 2, 8)); return fmaxf(v, __shfl_xor(v, 4, 8));
+}
+__device__ __forceinline__ uint32_t g8_or(uint32_t v) {
+    v |= __shfl_xor(v, 1, 8); v |= __shfl_xor(v, 2, 8); return v | __shfl_xor(v, 4, 8);
+}
+__device__ __forceinline__ uint64_t g8_max_u64(uint64_t v) {
+    for (int o = 1; o < 8; o <<= 1) {
+        uint64_t w = __shfl_xor(v, o, 8);
+        v = w > v ? w : v;
+    }
+    return v;
+}
+__device__ __forceinline__ uint32_t ord_u32(float f) {
+    uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+// One wide level. Each group of 8 lanes turns one work item (binary ref, wide node index) into a
+// WideNode; lane k of the group holds child candidate k. All shuffles are executed by every lane
+// of a group (groups without an item compute on empty data), and each group's control flow is
+// uniform across its 8 lanes.
+__global__ __launch_bounds__(kBlock) void k_collapse(const float4 *pbox, const uint32_t *ids, PrimSource src,
+                                                     BinaryTree bt, BuildCounters *ctr, uint32_t level,
+                                                     const uint2 *qin, uint2 *qout, WideOut out,
+                                                     uint32_t split_small) {
     const uint32_t count = ctr->level_count[level];
     if (count == 0) return;
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicMax(&ctr->depth, level + 1);
-    for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < count; w += gridDim.x * blockDim.x) {
-        uint2 item = qin[w];
-        uint32_t ref[8], cnt[8];
-        float3 clo[8], chi[8];
-        int nc;
-        auto ref_count = [&](uint32_t r) -> uint32_t {
-            if (r & kLeafBit) return 1u;
-            uint2 rg = bt.range[r];
-            return rg.y - rg.x + 1u;
-        };
-        if (item.x & kLeafBit) {
+    const uint32_t lane = threadIdx.x & 7u;
+    const uint32_t groups = blockDim.x >> 3;
+    for (uint32_t base = blockIdx.x * groups; base < count; base += gridDim.x * groups) {
+        const uint32_t item_idx = base + (threadIdx.x >> 3);
+        const bool active = item_idx < count;
+        const uint2 item = active ? qin[item_idx] : make_uint2(kLeafBit, 0u);
+        // ---- gather children: lanes 0/1 start with the binary node's two children
+        bool valid = false;
+        uint32_t ref = 0, cnt = 0;
+        float3 lo = make_float3(FLT_MAX, FLT_MAX, FLT_MAX), hi = make_float3(-FLT_MAX, -FLT_MAX, -FLT_MAX);
+        uint32_t nc;
+        if (!active) {
+            nc = 8;
+        } else if (item.x & kLeafBit) {
             nc = 1;
-            ref[0] = item.x;
-            cnt[0] = 1;
-            prim_box(src, ids[item.x & ~kLeafBit], clo[0], chi[0]);
+            if (lane == 0) {
+                ref = item.x;
+                cnt = 1;
+                valid = true;
+                stored_box(pbox, ids[item.x & ~kLeafBit], lo, hi);
+            }
         } else {
             nc = 2;
-            uint2 ch = bt.child[item.x];
-            const float4 *cb = bt.cbox + 4 * (uint64_t)item.x;
-            float4 a = cb[0], b = cb[1], c = cb[2], d = cb[3];
-            ref[0] = ch.x; clo[0] = make_float3(a.x, a.y, a.z); chi[0] = make_float3(b.x, b.y, b.z);
-            ref[1] = ch.y; clo[1] = make_float3(c.x, c.y, c.z); chi[1] = make_float3(d.x, d.y, d.z);
-            cnt[0] = ref_count(ch.x);
-            cnt[1] = ref_count(ch.y);
+            if (lane < 2) {
+                ref = lane == 0 ? bt.child[item.x].x : bt.child[item.x].y;
+                const float4 a = bt.cbox[4 * (uint64_t)item.x + 2 * lane], b = bt.cbox[4 * (uint64_t)item.x + 2 * lane + 1];
+                lo = make_float3(a.x, a.y, a.z);
+                hi = make_float3(b.x, b.y, b.z);
+                cnt = ref_count(bt, ref);
+                valid = true;
+            }
         }
-        // Greedy opening: widest child that must become internal, then (optionally) widest
-        // multi-triangle leaf candidate while slots remain.
-        while (nc < 8) {
-            int best = -1;
-            float best_area = -1.0f;
-            for (int j = 0; j < nc; j++) {
-                if (cnt[j] > kLeafMax) {
-                    float a = box_area(clo[j], chi[j]);
-                    if (a > best_area) { best_area = a; best = j; }
-                }
+        // ---- open the widest child (must-split before may-split) until 8 children
+        for (int it = 0; it < 7; it++) {
+            uint64_t key = 0;
+            if (valid) {
+                const uint32_t prio = cnt > kLeafMax ? 2u : ((split_small && !(ref & kLeafBit) && cnt > 1u) ? 1u : 0u);
+                if (prio) key = ((uint64_t)prio << 40) | ((uint64_t)__float_as_uint(fmaxf(box_area(lo, hi), 0.0f)) << 3) | (7u - lane);
             }
-            if (best < 0 && split_small) {
-                for (int j = 0; j < nc; j++) {
-                    if (!(ref[j] & kLeafBit) && cnt[j] > 1) {
-                        float a = box_area(clo[j], chi[j]);
-                        if (a > best_area) { best_area = a; best = j; }
-                    }
-                }
+            const uint64_t best = g8_max_u64(key);
+            if (nc >= 8 || best == 0) break; // uniform within the group
+            const uint32_t winner = 7u - (uint32_t)(best & 7u);
+            const uint32_t wref = __shfl(ref, (int)winner, 8);
+            if (lane == winner) {
+                const float4 a = bt.cbox[4 * (uint64_t)wref], b = bt.cbox[4 * (uint64_t)wref + 1];
+                ref = bt.child[wref].x;
+                lo = make_float3(a.x, a.y, a.z);
+                hi = make_float3(b.x, b.y, b.z);
+                cnt = ref_count(bt, ref);
+            } else if (lane == nc) {
+                const float4 c = bt.cbox[4 * (uint64_t)wref + 2], d = bt.cbox[4 * (uint64_t)wref + 3];
+                ref = bt.child[wref].y;
+                lo = make_float3(c.x, c.y, c.z);
+                hi = make_float3(d.x, d.y, d.z);
+                cnt = ref_count(bt, ref);
+                valid = true;
             }
-            if (best < 0) break;
-            uint32_t r = ref[best];
-            uint2 ch = bt.child[r];
-            const float4 *cb = bt.cbox + 4 * (uint64_t)r;
-            float4 a = cb[0], b = cb[1], c = cb[2], d = cb[3];
-            ref[best] = ch.x; clo[best] = make_float3(a.x, a.y, a.z); chi[best] = make_float3(b.x, b.y, b.z);
-            cnt[best] = ref_count(ch.x);
-            ref[nc] = ch.y; clo[nc] = make_float3(c.x, c.y, c.z); chi[nc] = make_float3(d.x, d.y, d.z);
-            cnt[nc] = ref_count(ch.y);
             nc++;
         }
-        // Node box, origin and scale.
-        float3 nlo = clo[0], nhi = chi[0];
-        for (int j = 1; j < nc; j++) {
-            nlo = make_float3(fminf(nlo.x, clo[j].x), fminf(nlo.y, clo[j].y), fminf(nlo.z, clo[j].z));
-            nhi = make_float3(fmaxf(nhi.x, chi[j].x), fmaxf(nhi.y, chi[j].y), fmaxf(nhi.z, chi[j].z));
-        }
-        uint32_t ex = pick_exponent(nlo.x, nhi.x), ey = pick_exponent(nlo.y, nhi.y), ez = pick_exponent(nlo.z, nhi.z);
-        float sx = __int_as_float((int)(ex << 23)), sy = __int_as_float((int)(ey << 23)), sz = __int_as_float((int)(ez << 23));
-        // Octant slot assignment: greedy max of dot(child centre - node centre, octant direction).
-        float3 pc = make_float3(0.5f * (nlo.x + nhi.x), 0.5f * (nlo.y + nhi.y), 0.5f * (nlo.z + nhi.z));
-        int slot_of[8];
-        uint32_t used_slots = 0, used_children = 0;
-        for (int k = 0; k < nc; k++) {
-            float best = -FLT_MAX;
-            int bc = 0, bs = 0;
-            for (int j = 0; j < nc; j++) {
-                if (used_children & (1u << j)) continue;
-                float dx = 0.5f * (clo[j].x + chi[j].x) - pc.x;
-                float dy = 0.5f * (clo[j].y + chi[j].y) - pc.y;
-                float dz = 0.5f * (clo[j].z + chi[j].z) - pc.z;
-                for (int s = 0; s < 8; s++) {
-                    if (used_slots & (1u << s)) continue;
-                    float v = ((s & 1) ? dx : -dx) + ((s & 2) ? dy : -dy) + ((s & 4) ? dz : -dz);
-                    if (v > best) { best = v; bc = j; bs = s; }
+        // ---- node box, origin and scales
+        const float3 nlo = make_float3(g8_min(lo.x), g8_min(lo.y), g8_min(lo.z));
+        const float3 nhi = make_float3(g8_max(hi.x), g8_max(hi.y), g8_max(hi.z));
+        const uint32_t ex = pick_exponent(nlo.x, nhi.x), ey = pick_exponent(nlo.y, nhi.y), ez = pick_exponent(nlo.z, nhi.z);
+        const float sx = __int_as_float((int)(ex << 23)), sy = __int_as_float((int)(ey << 23)), sz = __int_as_float((int)(ez << 23));
+        // ---- octant slots: repeatedly give the best remaining (child, free slot) pair its slot,
+        // scoring dot(child centre - node centre, octant direction)
+        const float3 pc = make_float3(0.5f * (nlo.x + nhi.x), 0.5f * (nlo.y + nhi.y), 0.5f * (nlo.z + nhi.z));
+        const float dx = 0.5f * (lo.x + hi.x) - pc.x, dy = 0.5f * (lo.y + hi.y) - pc.y, dz = 0.5f * (lo.z + hi.z) - pc.z;
+        uint32_t slot = 8u, used = 0u;
+        const uint32_t nvalid = active ? nc : 0u;
+        for (uint32_t k = 0; k < nvalid; k++) {
+            uint64_t key = 0;
+            if (valid && slot == 8u) {
+                float bv = -FLT_MAX;
+                uint32_t bs = (uint32_t)__ffs((int)(~used & 0xFFu)) - 1u; // lowest free slot (NaN scores)
+                for (uint32_t s = 0; s < 8; s++) {
+                    if (used & (1u << s)) continue;
+                    const float v = ((s & 1) ? dx : -dx) + ((s & 2) ? dy : -dy) + ((s & 4) ? dz : -dz);
+                    if (v > bv) { bv = v; bs = s; }
                 }
+                key = ((uint64_t)ord_u32(bv) << 8) | (bs << 3) | (7u - lane);
             }
-            used_children |= 1u << bc;
-            used_slots |= 1u << bs;
-            slot_of[bc] = bs;
+            const uint64_t best = g8_max_u64(key);
+            const uint32_t bs = (uint32_t)(best >> 3) & 7u;
+            if (lane == 7u - (uint32_t)(best & 7u)) slot = bs;
+            used |= 1u << bs;
         }
-        int child_at[8];
-        for (int s = 0; s < 8; s++) child_at[s] = -1;
-        for (int j = 0; j < nc; j++) child_at[slot_of[j]] = j;
-        // Allocation of internal children and leaf triangles.
-        uint32_t imask = 0, ninternal = 0, nleaf_tris = 0;
-        for (int s = 0; s < 8; s++) {
-            int j = child_at[s];
-            if (j < 0) continue;
-            if (cnt[j] > kLeafMax) { imask |= 1u << s; ninternal++; }
-            else nleaf_tris += cnt[j];
+        // ---- internal / leaf, ranks and triangle offsets in slot order
+        const bool internal = valid && cnt > kLeafMax;
+        const uint32_t imask = g8_or(internal ? (1u << slot) : 0u);
+        const uint32_t leafcnt = (valid && !internal) ? cnt : 0u;
+        uint32_t toff = 0, ttotal = 0;
+        for (int k = 0; k < 8; k++) {
+            const uint32_t sk = __shfl(slot, k, 8), ck = __shfl(leafcnt, k, 8);
+            ttotal += ck;
+            if (sk < slot) toff += ck;
         }
-        uint32_t child_base = ninternal ? atomicAdd(&ctr->node_count, ninternal) : 0u;
-        uint32_t tri_base = nleaf_tris ? atomicAdd(&ctr->tri_count, nleaf_tris) : 0u;
-        uint32_t qbase = ninternal ? atomicAdd(&ctr->level_count[level + 1], ninternal) : 0u;
-        uint32_t meta[2] = {0, 0};
-        uint32_t qlo[3][2] = {{0, 0}, {0, 0}, {0, 0}}, qhi[3][2] = {{0, 0}, {0, 0}, {0, 0}};
-        uint32_t rank = 0, toff = 0;
-        for (int s = 0; s < 8; s++) {
-            int j = child_at[s];
-            uint32_t sh = 8u * (uint32_t)(s & 3), wi = (uint32_t)(s >> 2);
-            uint32_t lx, ly, lz, hx, hy, hz, m;
-            if (j < 0) {
-                lx = ly = lz = 255u;
-                hx = hy = hz = 0u;
-                m = 0u;
+        const uint32_t ninternal = (uint32_t)__popc(imask);
+        uint32_t child_base = 0, tri_base = 0, qbase = 0;
+        if (lane == 0 && active) {
+            if (ninternal) {
+                child_base = atomicAdd(&ctr->node_count, ninternal);
+                qbase = atomicAdd(&ctr->level_count[level + 1], ninternal);
+            }
+            if (ttotal) tri_base = atomicAdd(&ctr->tri_count, ttotal);
+        }
+        child_base = __shfl(child_base, 0, 8);
+        tri_base = __shfl(tri_base, 0, 8);
+        qbase = __shfl(qbase, 0, 8);
+        const uint32_t rank = valid ? (uint32_t)__popc(imask & ((1u << slot) - 1u)) : 0u;
+        uint32_t meta = 0u, qlx = 0u, qly = 0u, qlz = 0u, qhx = 0u, qhy = 0u, qhz = 0u;
+        if (active && valid) {
+            qlx = quant_lo(lo.x, nlo.x, sx); qhx = quant_hi(hi.x, nlo.x, sx);
+            qly = quant_lo(lo.y, nlo.y, sy); qhy = quant_hi(hi.y, nlo.y, sy);
+            qlz = quant_lo(lo.z, nlo.z, sz); qhz = quant_hi(hi.z, nlo.z, sz);
+            if (internal) {
+                meta = 0xFFu;
+                qout[qbase + rank] = make_uint2(ref, child_base + rank);
             } else {
-                lx = quant_lo(clo[j].x, nlo.x, sx); hx = quant_hi(chi[j].x, nlo.x, sx);
-                ly = quant_lo(clo[j].y, nlo.y, sy); hy = quant_hi(chi[j].y, nlo.y, sy);
-                lz = quant_lo(clo[j].z, nlo.z, sz); hz = quant_hi(chi[j].z, nlo.z, sz);
-                if (imask & (1u << s)) {
-                    m = 0xFFu;
-                    qout[qbase + rank] = make_uint2(ref[j], child_base + rank);
-                    rank++;
-                } else {
-                    uint32_t first = (ref[j] & kLeafBit) ? (ref[j] & ~kLeafBit) : bt.range[ref[j]].x;
-                    m = (cnt[j] << 5) | toff;
-                    for (uint32_t k = 0; k < cnt[j]; k++) {
-                        uint32_t prim = ids[first + k];
-                        uint32_t dst = tri_base + toff + k;
-                        if (out.tris) {
-                            uint32_t i0 = src.tri[3 * (uint64_t)prim], i1 = src.tri[3 * (uint64_t)prim + 1], i2 = src.tri[3 * (uint64_t)prim + 2];
-                            float4 a = load_vertex(src, i0), b = load_vertex(src, i1), c = load_vertex(src, i2);
-                            float4 *tr = reinterpret_cast<float4 *>(out.tris + dst);
-                            tr[0] = make_float4(a.x, a.y, a.z, __uint_as_float(prim));
-                            tr[1] = make_float4(b.x, b.y, b.z, 0.0f);
-                            tr[2] = make_float4(c.x, c.y, c.z, 0.0f);
-                        } else {
-                            out.prim_out[dst] = prim;
-                        }
+                meta = (cnt << 5) | toff;
+                const uint32_t first = ref_first(bt, ref);
+                for (uint32_t k = 0; k < cnt; k++) {
+                    const uint32_t prim = ids[first + k];
+                    const uint32_t dst = tri_base + toff + k;
+                    if (out.tris) {
+                        const uint32_t i0 = src.tri[3 * (uint64_t)prim], i1 = src.tri[3 * (uint64_t)prim + 1],
+                                       i2 = src.tri[3 * (uint64_t)prim + 2];
+                        const float4 a = load_vertex(src, i0), b = load_vertex(src, i1), c = load_vertex(src, i2);
+                        float4 *tr = reinterpret_cast<float4 *>(out.tris + dst);
+                        tr[0] = make_float4(a.x, a.y, a.z, __uint_as_float(prim));
+                        tr[1] = make_float4(b.x, b.y, b.z, 0.0f);
+                        tr[2] = make_float4(c.x, c.y, c.z, 0.0f);
+                    } else {
+                        out.prim_out[dst] = prim;
                     }
-                    toff += cnt[j];
                 }
             }
-            meta[wi] |= m << sh;
-            qlo[0][wi] |= lx << sh; qlo[1][wi] |= ly << sh; qlo[2][wi] |= lz << sh;
-            qhi[0][wi] |= hx << sh; qhi[1][wi] |= hy << sh; qhi[2][wi] |= hz << sh;
         }
-        uint4 *dst = reinterpret_cast<uint4 *>(out.nodes + item.y);
-        dst[0] = make_uint4(__float_as_uint(nlo.x), __float_as_uint(nlo.y), __float_as_uint(nlo.z),
-                            ex | (ey << 8) | (ez << 16) | (imask << 24));
-        dst[1] = make_uint4(child_base, tri_base, meta[0], meta[1]);
-        dst[2] = make_uint4(qlo[0][0], qlo[0][1], qlo[1][0], qlo[1][1]);
-        dst[3] = make_uint4(qlo[2][0], qlo[2][1], qhi[0][0], qhi[0][1]);
-        dst[4] = make_uint4(qhi[1][0], qhi[1][1], qhi[2][0], qhi[2][1]);
+        // ---- assemble the node words by OR-reducing each lane's bytes at its slot position
+        const uint32_t sh = valid ? 8u * (slot & 3u) : 0u;
+        const bool hi_half = valid && slot >= 4u;
+        auto word = [&](uint32_t byte, bool upper) -> uint32_t {
+            const uint32_t c = valid ? (byte << sh) : 0u;
+            return g8_or((upper == hi_half) ? c : 0u);
+        };
+        const uint32_t occ = g8_or(valid ? (1u << slot) : 0u);
+        // empty slots: qlo = 255, qhi = 0, meta = 0 (never hit)
+        uint32_t empty_lo = 0u, empty_hi = 0u;
+        for (uint32_t s = 0; s < 8; s++)
+            if (!(occ & (1u << s))) {
+                if (s < 4) empty_lo |= 0xFFu << (8u * s); else empty_hi |= 0xFFu << (8u * (s - 4));
+            }
+        const uint32_t m0 = word(meta, false), m1 = word(meta, true);
+        const uint32_t lx0 = word(qlx, false) | empty_lo, lx1 = word(qlx, true) | empty_hi;
+        const uint32_t ly0 = word(qly, false) | empty_lo, ly1 = word(qly, true) | empty_hi;
+        const uint32_t lz0 = word(qlz, false) | empty_lo, lz1 = word(qlz, true) | empty_hi;
+        const uint32_t hx0 = word(qhx, false), hx1 = word(qhx, true);
+        const uint32_t hy0 = word(qhy, false), hy1 = word(qhy, true);
+        const uint32_t hz0 = word(qhz, false), hz1 = word(qhz, true);
+        if (active && lane < 5u) {
+            uint4 w;
+            if (lane == 0u) w = make_uint4(__float_as_uint(nlo.x), __float_as_uint(nlo.y), __float_as_uint(nlo.z),
+                                           ex | (ey << 8) | (ez << 16) | (imask << 24));
+            else if (lane == 1u) w = make_uint4(child_base, tri_base, m0, m1);
+            else if (lane == 2u) w = make_uint4(lx0, lx1, ly0, ly1);
+            else if (lane == 3u) w = make_uint4(lz0, lz1, hx0, hx1);
+            else w = make_uint4(hy0, hy1, hz0, hz1);
+            reinterpret_cast<uint4 *>(out.nodes + item.y)[lane] = w;
+        }
     }
+}
+
+// ---- validation (tests and debugging): structural and containment invariants of a built tree.
+// err[0] child index out of range      err[1] triangle range out of range
+// err[2] leaf triangle outside its slot box
+// err[3] child node extent outside its parent's slot box
+// err[4] a triangle record referenced != 1 times   err[5] a primitive id covered != 1 times
+// err[6] an empty slot with an occupied box (qlo <= qhi on every axis)
+struct DecodedSlot {
+    double lo[3], hi[3];
+};
+
+__device__ __forceinline__ uint32_t node_byte(const uint4 *w, int word, int slot) {
+    const uint32_t *u = reinterpret_cast<const uint32_t *>(w);
+    return (u[word + (slot >> 2)] >> (8 * (slot & 3))) & 0xFFu;
+}
+
+__device__ __forceinline__ DecodedSlot decode_slot(const uint4 *w, int s) {
+    DecodedSlot d;
+    const double p[3] = {__uint_as_float(w[0].x), __uint_as_float(w[0].y), __uint_as_float(w[0].z)};
+    // word indices (uint32 units): qlo x/y/z at 8/10/12, qhi x/y/z at 14/16/18
+    for (int a = 0; a < 3; a++) {
+        const double sc = __int_as_float((int)(((w[0].w >> (8 * a)) & 0xFFu) << 23));
+        d.lo[a] = p[a] + sc * (double)node_byte(w, 8 + 2 * a, s);
+        d.hi[a] = p[a] + sc * (double)node_byte(w, 14 + 2 * a, s);
+    }
+    return d;
+}
+
+__global__ void k_validate_nodes(const WideNode *nodes, uint32_t node_count, const TriRecord *tris, uint32_t tri_count,
+                                 uint32_t nprims, uint32_t *tri_marks, uint32_t *prim_marks, uint32_t *err) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= node_count) return;
+    uint4 w[5];
+    for (int k = 0; k < 5; k++) w[k] = reinterpret_cast<const uint4 *>(nodes + i)[k];
+    const uint32_t imask = w[0].w >> 24, child_base = w[1].x, tri_base = w[1].y;
+    for (int s = 0; s < 8; s++) {
+        const uint32_t meta = node_byte(w, 6, s);
+        const DecodedSlot b = decode_slot(w, s);
+        if (meta == 0u) {
+            if (node_byte(w, 8, s) <= node_byte(w, 14, s) && node_byte(w, 10, s) <= node_byte(w, 16, s) &&
+                node_byte(w, 12, s) <= node_byte(w, 18, s))
+                atomicAdd(&err[6], 1u);
+            continue;
+        }
+        if (imask & (1u << s)) {
+            const uint32_t c = child_base + (uint32_t)__popc(imask & ((1u << s) - 1u));
+            if (c >= node_count || c == 0u) { atomicAdd(&err[0], 1u); continue; }
+            uint4 cw[5];
+            for (int k = 0; k < 5; k++) cw[k] = reinterpret_cast<const uint4 *>(nodes + c)[k];
+            // The child's origin is its exact lower corner; its decoded slot tops exceed the exact
+            // upper corner by less than one child quantum.
+            const double cp[3] = {__uint_as_float(cw[0].x), __uint_as_float(cw[0].y), __uint_as_float(cw[0].z)};
+            bool bad = false;
+            for (int a = 0; a < 3; a++) bad |= cp[a] < b.lo[a];
+            for (int cs = 0; cs < 8; cs++) {
+                if (node_byte(cw, 6, cs) == 0u) continue;
+                const DecodedSlot cb = decode_slot(cw, cs);
+                for (int a = 0; a < 3; a++) {
+                    const double q = __int_as_float((int)(((cw[0].w >> (8 * a)) & 0xFFu) << 23));
+                    bad |= cb.hi[a] - q > b.hi[a];
+                }
+            }
+            if (bad) atomicAdd(&err[3], 1u);
+        } else {
+            const uint32_t off = meta & 31u, cnt = meta >> 5;
+            if (cnt == 0u || (uint64_t)tri_base + off + cnt > tri_count) { atomicAdd(&err[1], 1u); continue; }
+            for (uint32_t k = 0; k < cnt; k++) {
+                const uint32_t t = tri_base + off + k;
+                atomicAdd(&tri_marks[t], 1u);
+                if (tris) {
+                    const float *v = reinterpret_cast<const float *>(tris + t);
+                    const uint32_t prim = __float_as_uint(v[3]);
+                    if (prim < nprims) atomicAdd(&prim_marks[prim], 1u); else atomicAdd(&err[5], 1u);
+                    for (int q = 0; q < 3; q++)
+                        for (int a = 0; a < 3; a++) {
+                            const double x = v[4 * q + a];
+                            if (x < b.lo[a] || x > b.hi[a]) atomicAdd(&err[2], 1u);
+                        }
+                }
+            }
+        }
+    }
+}
+
+__global__ void k_validate_marks(const uint32_t *tri_marks, uint32_t tri_count, const uint32_t *prim_marks,
+                                 uint32_t nprims, uint32_t *err) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < tri_count && tri_marks[i] != 1u) atomicAdd(&err[4], 1u);
+    if (prim_marks && i < nprims && prim_marks[i] != 1u) atomicAdd(&err[5], 1u);
 }
 
 template<typename T>
@@ -512,12 +664,11 @@ T *carve(char *&p, size_t count) {
 
 } // namespace
 
-// Scratch bytes for building over n primitives (everything but the output arrays).
 static size_t sort_temp_bytes(uint32_t n) {
     size_t bytes = 0;
-    rocprim::double_buffer<uint64_t> k(nullptr, nullptr);
+    rocprim::double_buffer<uint32_t> k(nullptr, nullptr);
     rocprim::double_buffer<uint32_t> v(nullptr, nullptr);
-    rocprim::radix_sort_pairs(nullptr, bytes, k, v, n, 0, 63, (hipStream_t)0);
+    rocprim::radix_sort_pairs(nullptr, bytes, k, v, n, 0, 30, (hipStream_t)0);
     return bytes;
 }
 
@@ -528,11 +679,13 @@ static size_t select_temp_bytes(uint32_t n) {
     return bytes;
 }
 
+// Scratch bytes for building over n primitives (everything but the output arrays).
 size_t bvh_scratch_bytes(uint32_t n) {
     size_t m = std::max<uint32_t>(n, 2u);
     size_t b = 0;
-    b += 2 * m * sizeof(uint64_t) + 256 * 2;        // keys
+    b += 2 * m * sizeof(uint32_t) + 256 * 2;        // keys
     b += 2 * m * sizeof(uint32_t) + 256 * 2;        // ids
+    b += 2 * m * sizeof(float4) + 256;              // primitive boxes
     b += 4 * m * sizeof(float4) + 256;              // cbox
     b += m * sizeof(uint2) + 256;                   // child
     b += m * sizeof(uint2) + 256;                   // range
@@ -548,11 +701,29 @@ size_t bvh_scratch_bytes(uint32_t n) {
 
 size_t bvh_node_capacity(uint32_t n) { return std::max<uint32_t>(n, 2u); }
 
+// Checks a built tree; err7 (device, 7 words) receives the violation counts listed above.
+hipError_t bvh_validate(hipStream_t stream, const WideNode *nodes, uint32_t node_count, const TriRecord *tris,
+                        uint32_t tri_count, uint32_t nprims, uint32_t *marks, uint32_t *err7) {
+    hipError_t e = hipMemsetAsync(marks, 0, sizeof(uint32_t) * ((size_t)tri_count + nprims), stream);
+    if (e != hipSuccess) return e;
+    e = hipMemsetAsync(err7, 0, 7 * sizeof(uint32_t), stream);
+    if (e != hipSuccess) return e;
+    uint32_t *prim_marks = tris ? marks + tri_count : nullptr;
+    if (node_count)
+        hipLaunchKernelGGL(k_validate_nodes, dim3((node_count + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, nodes,
+                           node_count, tris, tri_count, nprims, marks, prim_marks, err7);
+    const uint32_t nmax = std::max(tri_count, nprims);
+    if (nmax)
+        hipLaunchKernelGGL(k_validate_marks, dim3((nmax + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, marks, tri_count,
+                           prim_marks, nprims, err7);
+    return hipGetLastError();
+}
+
 // Builds a wide BVH over n primitives (triangles if boxes == nullptr, else boxes).
 // Output arrays: nodes (bvh_node_capacity(n) entries), tris (n entries) or prim_out (n entries).
 // The counters land in `counters_out` (device) for the caller to read back. `pinned_word` is a
-// host-pinned word used to read the per-phase entry counts (the stream is synchronized once per
-// bottom-up phase; about log_64(n) phases).
+// host-pinned word used to read entry and level counts: the stream is synchronized once per
+// bottom-up phase (about log_64(n) phases) and once per chunk of collapse levels.
 hipError_t bvh_build(hipStream_t stream, void *scratch, size_t scratch_bytes, const void *vbase, uint64_t vstride,
                      uint64_t nverts, const uint32_t *tri, const float4 *boxes, uint32_t n, WideNode *nodes,
                      TriRecord *tris, uint32_t *prim_out, BuildCounters **counters_out, uint32_t split_small,
@@ -560,8 +731,9 @@ hipError_t bvh_build(hipStream_t stream, void *scratch, size_t scratch_bytes, co
     if (scratch_bytes < bvh_scratch_bytes(n)) return hipErrorInvalidValue;
     size_t m = std::max<uint32_t>(n, 2u);
     char *p = static_cast<char *>(scratch);
-    uint64_t *k0 = carve<uint64_t>(p, m), *k1 = carve<uint64_t>(p, m);
+    uint32_t *k0 = carve<uint32_t>(p, m), *k1 = carve<uint32_t>(p, m);
     uint32_t *i0 = carve<uint32_t>(p, m), *i1 = carve<uint32_t>(p, m);
+    float4 *pbox_own = carve<float4>(p, 2 * m);
     BinaryTree bt;
     bt.cbox = carve<float4>(p, 4 * m);
     bt.child = carve<uint2>(p, m);
@@ -580,22 +752,23 @@ hipError_t bvh_build(hipStream_t stream, void *scratch, size_t scratch_bytes, co
 
     if (!boxes && n > 0 && nverts == 0) return hipErrorInvalidValue;
     PrimSource src{static_cast<const char *>(vbase), vstride, nverts, tri, boxes, error_flag};
+    const float4 *pbox = boxes ? boxes : pbox_own;
     const uint32_t nslots = n > 1 ? n - 1 : 0;
     hipLaunchKernelGGL(k_init, dim3(std::max<uint32_t>(1, std::min<uint32_t>((nslots + kBlock - 1) / kBlock, 4096))),
                        dim3(kBlock), 0, stream, ctr, bt.slots, nslots);
     if (n == 0) return hipGetLastError();
-    uint32_t grid = (n + kBlock - 1) / kBlock;
+    const uint32_t grid = (n + kBlock - 1) / kBlock;
     hipLaunchKernelGGL(k_centroid_bounds, dim3(std::min<uint32_t>(grid, 2048)), dim3(kBlock), 0, stream, src, n, ctr);
-    hipLaunchKernelGGL(k_morton, dim3(grid), dim3(kBlock), 0, stream, src, n, ctr, k0, i0);
-    rocprim::double_buffer<uint64_t> kb(k0, k1);
+    hipLaunchKernelGGL(k_morton, dim3(grid), dim3(kBlock), 0, stream, src, n, ctr, k0, i0, pbox_own);
+    rocprim::double_buffer<uint32_t> kb(k0, k1);
     rocprim::double_buffer<uint32_t> vb(i0, i1);
     size_t tb = temp_bytes;
-    hipError_t e = rocprim::radix_sort_pairs(temp, tb, kb, vb, n, 0, 63, stream);
+    hipError_t e = rocprim::radix_sort_pairs(temp, tb, kb, vb, n, 0, 30, stream);
     if (e != hipSuccess) return e;
-    const uint64_t *keys = kb.current();
+    const uint32_t *keys = kb.current();
     const uint32_t *ids = vb.current();
     if (n == 1) {
-        hipLaunchKernelGGL(k_single_leaf_root, dim3(1), dim3(1), 0, stream, src, ids, ctr, q0);
+        hipLaunchKernelGGL(k_single_leaf_root, dim3(1), dim3(1), 0, stream, pbox, ids, ctr, q0);
     } else {
         hipLaunchKernelGGL(k_karras, dim3((n - 1 + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, n, keys, bt);
         hipLaunchKernelGGL(k_fill_u32, dim3(std::min<uint32_t>(grid, 4096)), dim3(kBlock), 0, stream, pend, (uint64_t)n,
@@ -608,12 +781,12 @@ hipError_t bvh_build(hipStream_t stream, void *scratch, size_t scratch_bytes, co
         for (;;) {
             if (count <= kLastBlock) {
                 uint32_t blk = (count + 63) / 64 * 64;
-                hipLaunchKernelGGL(k_bounds_phase<false>, dim3(1), dim3(blk), 0, stream, src, n, ids, bt, entries, count,
+                hipLaunchKernelGGL(k_bounds_phase<false>, dim3(1), dim3(blk), 0, stream, pbox, n, ids, bt, entries, count,
                                    pend, ctr, q0, 1u);
                 break;
             }
             hipLaunchKernelGGL(k_bounds_phase<false>, dim3((count + kPhaseBlock - 1) / kPhaseBlock), dim3(kPhaseBlock), 0,
-                               stream, src, n, ids, bt, entries, count, pend, ctr, q0, 0u);
+                               stream, pbox, n, ids, bt, entries, count, pend, ctr, q0, 0u);
             tb = temp_bytes;
             e = rocprim::select(temp, tb, (const uint32_t *)pend, ent[which], dcount, n, IsEntry{}, stream);
             if (e != hipSuccess) return e;
@@ -630,22 +803,31 @@ hipError_t bvh_build(hipStream_t stream, void *scratch, size_t scratch_bytes, co
                 // Phases stopped shrinking (sibling subtrees keep straddling workgroup
                 // boundaries): finish with agent-scope handoffs over all remaining entries.
                 hipLaunchKernelGGL(k_bounds_phase<true>, dim3((next + kPhaseBlock - 1) / kPhaseBlock), dim3(kPhaseBlock),
-                                   0, stream, src, n, ids, bt, entries, next, pend, ctr, q0, 1u);
+                                   0, stream, pbox, n, ids, bt, entries, next, pend, ctr, q0, 1u);
                 break;
             }
             count = next;
         }
     }
     WideOut out{nodes, tris, prim_out};
-    // A root-to-leaf path of the radix tree crosses each of the 63 key bits and 32 index bits
-    // at most once, and every wide level descends at least one binary level, so the wide depth
-    // is at most 96: launching kCollapseLaunches levels always completes the tree (levels
-    // without work return at once). The host checks level_count[kCollapseLaunches] == 0.
+    // A root-to-leaf path of the radix tree crosses each of the 30 key bits and 32 index bits at
+    // most once, and every wide level descends at least one binary level, so the wide depth is
+    // at most 63 < kCollapseLaunches. Levels are issued in chunks; after each chunk the work
+    // count of the next level is read back and the loop stops once it is zero. The host checks
+    // level_count[kCollapseLaunches] == 0 after the build.
+    const uint32_t kChunk = 6;
     for (uint32_t level = 0; level < kCollapseLaunches; level++) {
         const uint2 *qin = (level & 1) ? q1 : q0;
         uint2 *qout = (level & 1) ? q0 : q1;
-        hipLaunchKernelGGL(k_collapse, dim3(grid_collapse), dim3(kBlock), 0, stream, src, ids, bt, ctr, level, qin,
+        hipLaunchKernelGGL(k_collapse, dim3(grid_collapse), dim3(kBlock), 0, stream, pbox, ids, src, bt, ctr, level, qin,
                            qout, out, split_small);
+        if ((level + 1) % kChunk == 0 && level + 1 < kCollapseLaunches) {
+            e = hipMemcpyAsync(pinned_word, &ctr->level_count[level + 1], sizeof(uint32_t), hipMemcpyDeviceToHost, stream);
+            if (e != hipSuccess) return e;
+            e = hipStreamSynchronize(stream);
+            if (e != hipSuccess) return e;
+            if (*pinned_word == 0) break;
+        }
     }
     return hipGetLastError();
 }

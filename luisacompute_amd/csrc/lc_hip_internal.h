@@ -25,6 +25,8 @@ hipError_t bvh_build(hipStream_t stream, void *scratch, size_t scratch_bytes, co
                      uint64_t nverts, const uint32_t *tri, const float4 *boxes, uint32_t n, WideNode *nodes,
                      TriRecord *tris, uint32_t *prim_out, BuildCounters **counters_out, uint32_t split_small,
                      int grid_collapse, uint32_t *error_flag, uint32_t *pinned_word);
+hipError_t bvh_validate(hipStream_t stream, const WideNode *nodes, uint32_t node_count, const TriRecord *tris,
+                        uint32_t tri_count, uint32_t nprims, uint32_t *marks, uint32_t *err7);
 
 // ---- kernels (trace.hip) ----
 struct TraceParams {

@@ -592,6 +592,28 @@ int lc_hip_mesh_stats(uint64_t device, uint64_t mesh, lc_hip_bvh_stats *out) {
     return LC_HIP_OK;
 }
 
+int lc_hip_mesh_validate(uint64_t device, uint64_t stream, uint64_t mesh, uint32_t errors[7]) {
+    Device *dev = obj<Device>(device);
+    Mesh *m = obj<Mesh>(mesh);
+    if (!dev || !m || !errors) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad mesh validate handles");
+    DeviceGuard g(dev);
+    int rc = mesh_stats_sync(m);
+    if (rc != LC_HIP_OK) return rc;
+    std::memset(errors, 0, 7 * sizeof(uint32_t));
+    if (m->ntris == 0) return LC_HIP_OK;
+    hipStream_t s = native_stream(stream);
+    void *scratch = nullptr;
+    LC_HIP_TRY(hipMalloc(&scratch, sizeof(uint32_t) * (2ull * m->ntris + 16)));
+    uint32_t *err = static_cast<uint32_t *>(scratch);
+    uint32_t *marks = err + 8;
+    hipError_t e = bvh_validate(s, m->nodes, m->stats.node_count, m->tris, m->stats.tri_count, m->ntris, marks, err);
+    if (e == hipSuccess) e = hipMemcpyAsync(errors, err, 7 * sizeof(uint32_t), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    hipFree(scratch);
+    if (e != hipSuccess) return hip_fail(e, "bvh validate");
+    return LC_HIP_OK;
+}
+
 // ---- accels ----
 uint64_t lc_hip_accel_create(uint64_t device, const lc_hip_accel_option *option) {
     Device *dev = obj<Device>(device);

@@ -136,3 +136,22 @@ def test_trace_count_mode(device, stream):
     assert a_nodes <= n_nodes and a_tris <= n_tris
     g = host.trace_closest(device, stream, accel, rays)
     assert np.mean(g["prim"] != 0xFFFFFFFF) > 0.99
+
+
+@pytest.mark.parametrize("kind,n", [("soup", 1), ("soup", 2), ("soup", 37), ("soup", 5000), ("soup", 300000),
+                                    ("ico", 6), ("dup", 20000)])
+def test_bvh_invariants(device, stream, kind, n):
+    """Every primitive in exactly one leaf, leaf triangles inside their quantised slot boxes,
+    child nodes inside their parents' slot boxes, indices in range (device-side check)."""
+    if kind == "ico":
+        verts, tris = lc.icosphere(n)
+    else:
+        verts, tris = lc.soup(n, initseq=60 + n)
+        if kind == "dup":  # many identical centroids: equal Morton codes everywhere
+            verts[:, :3] = np.round(verts[:, :3] / 10) * 10
+    print(f"[bvh {kind} {n}]", flush=True)
+    mesh = host.mesh_from_arrays(device, stream, verts, tris)
+    errs = mesh.validate(stream)
+    assert all(v == 0 for v in errs.values()), errs
+    st = mesh.stats()
+    assert st.leaf_primitive_count == len(tris) and st.depth <= 63
