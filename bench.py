@@ -35,7 +35,22 @@ def parse():
     ap.add_argument("--rays", type=int, default=1 << 23)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-sample-rays", type=int, default=1 << 18)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run 3 device-to-device copies of this many MiB (counter calibration) and exit")
     return ap.parse_args()
+
+
+def copy_probe(mib):
+    """Plain copy of known size: reading FETCH_SIZE / WRITE_SIZE for it calibrates the counters."""
+    import luisacompute_amd as lc
+    dev = lc.Device(0)
+    s = dev.create_stream()
+    n = mib << 20
+    a, b = dev.create_buffer(1, n), dev.create_buffer(1, n)
+    for _ in range(3):
+        s << b.copy_from_buffer(a)
+    s.synchronize()
+    print(json.dumps({"copy_probe_bytes": n, "copies": 3}), flush=True)
 
 
 def cpu_baseline(verts, tris, rays_f, sample_rays, total_rays):
@@ -70,6 +85,9 @@ def cpu_baseline(verts, tris, rays_f, sample_rays, total_rays):
 
 def main():
     a = parse()
+    if a.copy_probe_mib:
+        copy_probe(a.copy_probe_mib)
+        return
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (rays and scenes would shard "
               f"by replicating the BVH per GPU, see DESIGN.md)", file=sys.stderr)

@@ -194,6 +194,9 @@ int lc_hip_accel_stats(uint64_t device, uint64_t accel, lc_hip_bvh_stats *out);
    blocks_per_cu 0 = occupancy limit. */
 int lc_hip_trace_set_config(uint64_t device, int kind, int block, int lds_stack, int steps, int blocks_per_cu);
 int lc_hip_trace_get_config(uint64_t device, int kind, int *block, int *lds_stack, int *steps, int *blocks_per_cu);
+/* 1 = traverse rays in the order of a sort by (origin Morton code in the scene box, direction
+   octant) so neighbouring lanes share subtrees; results still land in each ray's own slot. */
+int lc_hip_trace_set_ray_sort(uint64_t device, int kind, int enable);
 /* Build knob: 1 = also open multi-triangle leaf candidates while a node has free slots. */
 int lc_hip_build_set_split_small(uint64_t device, int enable);
 /* Kernel timing with HIP events recorded on the launch stream around each build/trace. */

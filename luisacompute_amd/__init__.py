@@ -112,6 +112,7 @@ _SIGNATURES = {
     "lc_hip_accel_stats": (ctypes.c_int, [_U64, _U64, ctypes.POINTER(BvhStats)]),
     "lc_hip_trace_set_config": (ctypes.c_int, [_U64, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "lc_hip_trace_get_config": (ctypes.c_int, [_U64, ctypes.c_int] + [ctypes.POINTER(ctypes.c_int)] * 4),
+    "lc_hip_trace_set_ray_sort": (ctypes.c_int, [_U64, ctypes.c_int, ctypes.c_int]),
     "lc_hip_build_set_split_small": (ctypes.c_int, [_U64, ctypes.c_int]),
     "lc_hip_timing_enable": (ctypes.c_int, [_U64, ctypes.c_int]),
     "lc_hip_timing_get": (ctypes.c_int, [_U64, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_U64)]),
@@ -291,6 +292,10 @@ class Device:
         _check(lib().lc_hip_trace_get_config(self.handle, 1 if kind == "any" else 0, *[ctypes.byref(x) for x in v]),
                "trace config")
         return dict(block=v[0].value, lds_stack=v[1].value, steps=v[2].value, blocks_per_cu=v[3].value)
+
+    def set_ray_sort(self, kind, enable):
+        """Traverse rays in sorted-queue order (origin Morton code, direction octant)."""
+        _check(lib().lc_hip_trace_set_ray_sort(self.handle, 1 if kind == "any" else 0, 1 if enable else 0), "ray sort")
 
     def set_build_split_small(self, enable):
         _check(lib().lc_hip_build_set_split_small(self.handle, 1 if enable else 0), "build config")

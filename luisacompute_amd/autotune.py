@@ -1,5 +1,6 @@
-"""Traversal autotuner: block size, LDS stack depth, steps between ray refills and blocks per CU,
-chosen per trace kind (closest / any) by timing the real kernels on a workload with HIP events.
+"""Traversal autotuner: block size, LDS stack depth, steps between ray refills, blocks per CU and
+whether to traverse a sorted ray queue, chosen per trace kind (closest / any) by timing the real
+kernels (sort included) on a workload with HIP events.
 
 The table is JSON ({"closest": {...}, "any": {...}, "device": ..., "workload": ...}) and is applied
 with `apply(device, table)`. `python -m luisacompute_amd.autotune --out table.json` tunes on the
@@ -10,13 +11,14 @@ import itertools
 import json
 import os
 
-from . import Device, IDENTITY
+from . import Device
 from . import host
 
 BLOCKS = (64, 128, 256)
 STACKS = (8, 16, 24)
 STEPS = (8, 32, 128)
 BLOCKS_PER_CU = (0,)
+SORT = (False, True)
 
 DEFAULT_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "autotune_table.json")
 
@@ -26,6 +28,7 @@ def apply(device, table):
         c = table.get(kind)
         if c:
             device.set_trace_config(kind, c["block"], c["lds_stack"], c["steps"], c.get("blocks_per_cu", 0))
+            device.set_ray_sort(kind, c.get("sort_rays", False))
 
 
 def load_default(device):
@@ -38,30 +41,40 @@ def load_default(device):
     return None
 
 
+def _time(device, stream, fn, kind, repeats):
+    fn()  # warm
+    stream.synchronize()
+    device.reset_timing()
+    device.enable_timing(True)
+    for _ in range(repeats):
+        fn()
+    stream.synchronize()
+    device.enable_timing(False)
+    ms, n = device.timing("trace_" + kind)
+    return ms / max(n, 1)
+
+
 def tune(device, stream, accel, ray_buffer, hit_buffer, any_buffer, count, repeats=3, space=None, log=print):
+    """Two stages per kind: launch shape without sorting, then the sort flag on the best shapes."""
     space = space or list(itertools.product(BLOCKS, STACKS, STEPS, BLOCKS_PER_CU))
     result = {}
     for kind, buf in (("closest", hit_buffer), ("any", any_buffer)):
-        best = None
-        for block, slds, steps, bpc in space:
-            device.set_trace_config(kind, block, slds, steps, bpc)
-            fn = accel.trace_closest if kind == "closest" else accel.trace_any
-            fn(stream, ray_buffer, buf, count)  # warm
-            stream.synchronize()
-            device.reset_timing()
-            device.enable_timing(True)
-            for _ in range(repeats):
-                fn(stream, ray_buffer, buf, count)
-            stream.synchronize()
-            device.enable_timing(False)
-            ms, n = device.timing("trace_" + kind)
-            ms /= max(n, 1)
-            log(f"  {kind:7s} block={block:3d} stack={slds:2d} steps={steps:3d} bpc={bpc}: {ms:.3f} ms "
-                f"({count / ms / 1e3:.1f} Mrays/s)")
-            if best is None or ms < best[0]:
-                best = (ms, dict(block=block, lds_stack=slds, steps=steps, blocks_per_cu=bpc))
+        fn = (lambda: accel.trace_closest(stream, ray_buffer, buf, count)) if kind == "closest" else \
+             (lambda: accel.trace_any(stream, ray_buffer, buf, count))
+        timed = []
+        for srt in SORT:
+            device.set_ray_sort(kind, srt)
+            for block, slds, steps, bpc in space:
+                device.set_trace_config(kind, block, slds, steps, bpc)
+                ms = _time(device, stream, fn, kind, repeats)
+                log(f"  {kind:7s} sort={int(srt)} block={block:3d} stack={slds:2d} steps={steps:3d} bpc={bpc}: "
+                    f"{ms:.3f} ms ({count / ms / 1e3:.1f} Mrays/s)")
+                timed.append((ms, dict(block=block, lds_stack=slds, steps=steps, blocks_per_cu=bpc, sort_rays=srt)))
+        best = min(timed, key=lambda x: x[0])
         result[kind] = dict(best[1], ms=round(best[0], 4), mrays_s=round(count / best[0] / 1e3, 2))
-        device.set_trace_config(kind, **{k: v for k, v in best[1].items()})
+        c = best[1]
+        device.set_trace_config(kind, c["block"], c["lds_stack"], c["steps"], c["blocks_per_cu"])
+        device.set_ray_sort(kind, c["sort_rays"])
     return result
 
 

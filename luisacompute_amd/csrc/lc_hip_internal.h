@@ -34,6 +34,7 @@ struct TraceParams {
     uint32_t count;
     uint32_t vis_mask;    // ray visibility mask
     uint32_t *ray_counter;
+    const uint32_t *order; // optional: ray indices in traversal order (sorted queue), else identity
     void *out_hits;       // 24-B SurfaceHit per ray (closest)
     uint8_t *out_any;     // 1 byte per ray (any)
     // single-instance form
@@ -53,6 +54,11 @@ struct TraceParams {
     uint32_t *error;
 };
 
+// ---- kernels (raysort.hip) ----
+size_t ray_sort_scratch_bytes(uint32_t n);
+hipError_t ray_sort(hipStream_t stream, const float4 *rays, uint32_t n, const float bounds[6], void *scratch,
+                    size_t scratch_bytes, const uint32_t **order_out);
+
 bool trace_config_supported(int block, int slds);
 int trace_blocks_per_cu(bool any, bool two, bool count, int block, int slds);
 hipError_t trace_launch(const TraceParams &P, hipStream_t stream, bool any, bool two, bool count, int grid, int block,
@@ -64,6 +70,7 @@ struct TraceConfig {
     int slds = 16;
     int steps = 32;
     int blocks_per_cu = 0; // 0: occupancy limit
+    bool sort_rays = false; // traverse a queue sorted by origin Morton code + direction octant
 };
 
 struct TimingStat {
@@ -81,6 +88,8 @@ struct Device {
     // per-device working memory shared by launches (guarded by mu)
     void *spill = nullptr;
     size_t spill_bytes = 0;
+    void *sort_scratch = nullptr;
+    size_t sort_bytes = 0;
     uint32_t *ray_counter = nullptr;
     uint32_t *error = nullptr;
     unsigned long long *stats = nullptr;
@@ -173,6 +182,7 @@ struct Accel {
     HostCounters tlas_stats{};
     bool built = false;
     uint32_t max_blas_depth = 0;
+    float world_bounds[6] = {0, 0, 0, 0, 0, 0}; // union of instance world boxes (ray-sort keys)
 };
 
 } // namespace lchip

@@ -230,6 +230,7 @@ void lc_hip_device_destroy(uint64_t device) {
     hipDeviceSynchronize();
     collect_timings(dev);
     if (dev->spill) hipFree(dev->spill);
+    if (dev->sort_scratch) hipFree(dev->sort_scratch);
     if (dev->ray_counter) hipFree(dev->ray_counter);
     delete dev;
 }
@@ -728,9 +729,10 @@ int lc_hip_accel_build(uint64_t device, uint64_t stream, uint64_t accel, uint32_
     a->built = true;
     a->tlas_valid = false;
     a->tlas_stats_valid = false;
-    if (instance_count > 1) {
-        // world boxes of instances: 8 transformed BLAS-box corners, padded by a relative epsilon
-        std::vector<float> boxes(8 * (size_t)instance_count);
+    // world boxes of instances: 8 transformed BLAS-box corners, padded by a relative epsilon
+    std::vector<float> boxes(8 * (size_t)instance_count);
+    {
+        float wlo[3] = {3e38f, 3e38f, 3e38f}, whi[3] = {-3e38f, -3e38f, -3e38f};
         for (uint32_t i = 0; i < instance_count; i++) {
             float lo[3] = {3e38f, 3e38f, 3e38f}, hi[3] = {-3e38f, -3e38f, -3e38f};
             const InstanceHost &in = a->inst[i];
@@ -745,6 +747,8 @@ int lc_hip_accel_build(uint64_t device, uint64_t stream, uint64_t accel, uint32_
                     float pad = 1e-5f * std::max({std::fabs(lo[k]), std::fabs(hi[k]), 1.0f});
                     lo[k] -= pad;
                     hi[k] += pad;
+                    wlo[k] = std::min(wlo[k], lo[k]);
+                    whi[k] = std::max(whi[k], hi[k]);
                 }
             } else {
                 // no geometry: a degenerate box far outside any scene (never entered: blas_nodes == 0)
@@ -754,6 +758,12 @@ int lc_hip_accel_build(uint64_t device, uint64_t stream, uint64_t accel, uint32_
             o[0] = lo[0]; o[1] = lo[1]; o[2] = lo[2]; o[3] = 0.0f;
             o[4] = hi[0]; o[5] = hi[1]; o[6] = hi[2]; o[7] = 0.0f;
         }
+        for (int k = 0; k < 3; k++) {
+            a->world_bounds[k] = wlo[k] <= whi[k] ? wlo[k] : 0.0f;
+            a->world_bounds[3 + k] = wlo[k] <= whi[k] ? whi[k] : 0.0f;
+        }
+    }
+    if (instance_count > 1) {
         p = a->boxes; cap = a->box_cap;
         LC_HIP_TRY(ensure(p, cap, boxes.size() * sizeof(float)));
         a->boxes = static_cast<float4 *>(p); a->box_cap = cap;
@@ -896,8 +906,14 @@ static int trace_common(uint64_t device, uint64_t stream, uint64_t accel, uint64
     P.stats = dev->stats;
     P.steps = cfg.steps;
     if (counting) LC_HIP_TRY(hipMemsetAsync(dev->stats, 0, 2 * sizeof(unsigned long long), s));
+    const bool sort = cfg.sort_rays && !counting && count > 1 && !P.empty; // per-ray results do not depend on order
+    if (sort) LC_HIP_TRY(ensure(dev->sort_scratch, dev->sort_bytes, ray_sort_scratch_bytes(count)));
     {
         TimedRegion tr(dev, s, counting ? "trace_count" : (any ? "trace_any" : "trace_closest"));
+        if (sort) {
+            hipError_t e = ray_sort(s, P.rays, count, a->world_bounds, dev->sort_scratch, dev->sort_bytes, &P.order);
+            if (e != hipSuccess) return hip_fail(e, "ray sort");
+        }
         hipError_t e = trace_launch(P, s, any, two, counting, grid, cfg.block, cfg.slds);
         if (e != hipSuccess) return hip_fail(e, "trace launch");
     }
@@ -936,7 +952,8 @@ int lc_hip_trace_set_config(uint64_t device, int kind, int block, int lds_stack,
     if (!trace_config_supported(block, lds_stack) || steps < 1 || blocks_per_cu < 0)
         return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "unsupported trace config");
     std::lock_guard<std::mutex> lk(dev->mu);
-    dev->cfg[kind] = TraceConfig{block, lds_stack, steps, blocks_per_cu};
+    const bool sort_rays = dev->cfg[kind].sort_rays;
+    dev->cfg[kind] = TraceConfig{block, lds_stack, steps, blocks_per_cu, sort_rays};
     return LC_HIP_OK;
 }
 
@@ -949,6 +966,14 @@ int lc_hip_trace_get_config(uint64_t device, int kind, int *block, int *lds_stac
     if (lds_stack) *lds_stack = c.slds;
     if (steps) *steps = c.steps;
     if (blocks_per_cu) *blocks_per_cu = c.blocks_per_cu;
+    return LC_HIP_OK;
+}
+
+int lc_hip_trace_set_ray_sort(uint64_t device, int kind, int enable) {
+    Device *dev = obj<Device>(device);
+    if (!dev || kind < 0 || kind > 1) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "bad trace config target");
+    std::lock_guard<std::mutex> lk(dev->mu);
+    dev->cfg[kind].sort_rays = enable != 0;
     return LC_HIP_OK;
 }
 

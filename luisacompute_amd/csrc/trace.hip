@@ -222,8 +222,8 @@ __global__ __launch_bounds__(BLOCK) void k_trace(const TraceParams P) {
             if (!active && !exhausted) {
                 const uint32_t my = base + (uint32_t)__popcll(need & ((1ull << lane) - 1ull));
                 if (my < P.count) {
-                    ray_id = my;
-                    const float4 r0 = P.rays[2 * (uint64_t)my], r1 = P.rays[2 * (uint64_t)my + 1];
+                    ray_id = P.order ? P.order[my] : my; // sorted queue: fetch in key order
+                    const float4 r0 = P.rays[2 * (uint64_t)ray_id], r1 = P.rays[2 * (uint64_t)ray_id + 1];
                     wo[0] = r0.x; wo[1] = r0.y; wo[2] = r0.z; tmin = r0.w;
                     wd[0] = r1.x; wd[1] = r1.y; wd[2] = r1.z; tmax = r1.w;
                     t_best = FLT_MAX * 2.0f; // +inf

@@ -155,3 +155,26 @@ def test_bvh_invariants(device, stream, kind, n):
     assert all(v == 0 for v in errs.values()), errs
     st = mesh.stats()
     assert st.leaf_primitive_count == len(tris) and st.depth <= 63
+
+
+@pytest.mark.parametrize("kind", ["closest", "any"])
+def test_sorted_queue_matches_unsorted(device, stream, kind):
+    """Sorting the ray queue changes only the traversal order, never a result."""
+    n = 1 << 17
+    verts, tris = lc.soup(n, initseq=71)
+    rays = lc.rays_box_sphere(1 << 16, initseq=72)
+    accel = host.single_mesh_accel(device, stream, verts, tris)
+    fn = host.trace_closest if kind == "closest" else host.trace_any
+    try:
+        device.set_ray_sort(kind, False)
+        a = fn(device, stream, accel, rays)
+        device.set_ray_sort(kind, True)
+        b = fn(device, stream, accel, rays)
+    finally:
+        device.set_ray_sort(kind, False)
+    assert np.array_equal(a, b)
+    sc = ob.single_mesh_scene(verts, tris)
+    if kind == "closest":
+        compare_closest(b, sc.trace_closest2(lc.rays_as_floats(rays)), rays)
+    else:
+        compare_any(b, sc.trace_any(lc.rays_as_floats(rays)))
