@@ -128,9 +128,11 @@ def main():
     wall = time.perf_counter() - t0
     dev.enable_timing(False)
     build_ms, nb = dev.timing("mesh_build")
-    trace_ms, nt = dev.timing("trace_closest")
+    trace_ms, nt = dev.timing("trace_closest")          # entry point: ray sort + traversal
+    kern_ms, nk = dev.timing("trace_closest_kernel")    # traversal kernel alone (roofline)
     build_ms /= max(nb, 1)
     trace_ms /= max(nt, 1)
+    kern_ms /= max(nk, 1)
     ms_per_step = wall * 1e3 / a.steps
 
     # any-hit on the same BVH (reported beside the headline)
@@ -149,9 +151,12 @@ def main():
     nodes_per_ray = n_nodes / a.rays
     tris_per_ray = n_tris / a.rays
     bytes_per_ray = RAY_BYTES + HIT_BYTES + NODE_BYTES * nodes_per_ray + TRI_BYTES * tris_per_ray
-    achieved = a.rays * bytes_per_ray / (trace_ms * 1e-3) / 1e9
-    floor_gbs = (a.rays * (RAY_BYTES + HIT_BYTES) + st.node_bytes + st.primitive_bytes) / (trace_ms * 1e-3) / 1e9
+    achieved = a.rays * bytes_per_ray / (kern_ms * 1e-3) / 1e9
+    floor_gbs = (a.rays * (RAY_BYTES + HIT_BYTES) + st.node_bytes + st.primitive_bytes) / (kern_ms * 1e-3) / 1e9
 
+    # HBM bytes per launch of the traversal kernel: FETCH_SIZE + WRITE_SIZE from two rocprofv3
+    # --pmc runs of this command, each scaled by what the counter reads for a 1 GiB device copy
+    # (profiles/trace_closest_soup16m_traffic.md)
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "trace_closest_soup16m_traffic.json")
     if os.path.exists(tpath):
@@ -181,6 +186,7 @@ def main():
         "build_ms": round(build_ms, 3),
         "build_mtris_s": round(a.tris / (build_ms * 1e-3) / 1e6, 2),
         "trace_closest_ms": round(trace_ms, 3),
+        "trace_closest_kernel_ms": round(kern_ms, 3),
         "trace_closest_mrays_s": round(a.rays / (trace_ms * 1e-3) / 1e6, 2),
         "trace_any_ms": round(any_ms, 3),
         "trace_any_mrays_s": round(a.rays / (any_ms * 1e-3) / 1e6, 2),
@@ -190,7 +196,8 @@ def main():
         "tris_per_ray": round(tris_per_ray, 3),
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_trace<closest> (persistent wave64 traversal)",
+            "kernel": "k_trace<closest> (persistent wave64 traversal, sorted ray queue)",
+            "kernel_ms": round(kern_ms, 3),
             "achieved": round(achieved, 2),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

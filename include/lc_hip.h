@@ -102,10 +102,10 @@ void lc_hip_device_destroy(uint64_t device);
 uint32_t lc_hip_device_warp_size(uint64_t device);
 /* DeviceInterface::native_handle (device_interface.h:112): the HIP device ordinal. */
 int lc_hip_device_native(uint64_t device);
-/* DeviceInterface::query (device_interface.h:189): "device_name", "cu_count", "warp_size". */
+/* DeviceInterface::query (device_interface.h:187): "device_name", "cu_count", "warp_size". */
 int lc_hip_device_query(uint64_t device, const char *key, char *out, size_t capacity);
 
-/* ---- buffers: DeviceInterface::create_buffer / destroy_buffer (device_interface.h:116-121),
+/* ---- buffers: DeviceInterface::create_buffer / destroy_buffer (device_interface.h:116-122),
    BufferCreationInfo {handle, native_handle, element_stride, total_size_bytes} ---- */
 uint64_t lc_hip_buffer_create(uint64_t device, size_t element_stride, size_t element_count, void **native_handle);
 void lc_hip_buffer_destroy(uint64_t device, uint64_t buffer);
@@ -120,7 +120,7 @@ int lc_hip_buffer_copy(uint64_t device, uint64_t stream, uint64_t src, size_t sr
                        size_t dst_offset, size_t size);
 
 /* ---- streams: create_stream / destroy_stream / synchronize_stream / dispatch callback
-   (device_interface.h:128-134; CUDAStream::callback, cuda_stream.cpp:142-170) ---- */
+   (device_interface.h:136-139; CUDAStream::callback, cuda_stream.cpp:142-170) ---- */
 uint64_t lc_hip_stream_create(uint64_t device, uint32_t stream_tag);
 void lc_hip_stream_destroy(uint64_t device, uint64_t stream);
 int lc_hip_stream_synchronize(uint64_t device, uint64_t stream);
@@ -128,7 +128,7 @@ int lc_hip_stream_synchronize(uint64_t device, uint64_t stream);
 int lc_hip_stream_add_callback(uint64_t device, uint64_t stream, void (*fn)(void *), void *user);
 void *lc_hip_stream_native(uint64_t device, uint64_t stream);
 
-/* ---- events: timeline fences (device_interface.h:170-178; CUDAEvent, cuda_event.cpp:21-63) ---- */
+/* ---- events: timeline fences (device_interface.h:163-168; CUDAEvent, cuda_event.cpp:21-63) ---- */
 uint64_t lc_hip_event_create(uint64_t device);
 void lc_hip_event_destroy(uint64_t device, uint64_t event);
 int lc_hip_event_signal(uint64_t device, uint64_t event, uint64_t stream, uint64_t value);
@@ -136,7 +136,7 @@ int lc_hip_event_wait(uint64_t device, uint64_t event, uint64_t stream, uint64_t
 int lc_hip_event_synchronize(uint64_t device, uint64_t event, uint64_t value);
 int lc_hip_event_is_completed(uint64_t device, uint64_t event, uint64_t value);
 
-/* ---- meshes (BLAS): create_mesh / destroy_mesh (device_interface.h:180-181) and
+/* ---- meshes (BLAS): create_mesh / destroy_mesh (device_interface.h:171-172) and
    MeshBuildCommand (command.h:407-444): vertex buffer byte range and stride (a multiple of 16,
    src/runtime/rtx/mesh.cpp:42-49), triangle buffer byte range of uint32 triples. ---- */
 uint64_t lc_hip_mesh_create(uint64_t device, const lc_hip_accel_option *option);
@@ -145,7 +145,7 @@ int lc_hip_mesh_build(uint64_t device, uint64_t stream, uint64_t mesh, uint32_t 
                       size_t vertex_offset, size_t vertex_size, size_t vertex_stride, uint64_t triangle_buffer,
                       size_t triangle_offset, size_t triangle_size);
 
-/* ---- accels (TLAS): create_accel / destroy_accel (device_interface.h:186-187) and
+/* ---- accels (TLAS): create_accel / destroy_accel (device_interface.h:183-184) and
    AccelBuildCommand (command.h:531-616; semantics of AccelImpl::update,
    src/rust/luisa_compute_backend_impl/src/cpu/accel.rs:324-447). ---- */
 uint64_t lc_hip_accel_create(uint64_t device, const lc_hip_accel_option *option);

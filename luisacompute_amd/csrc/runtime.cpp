@@ -909,11 +909,13 @@ static int trace_common(uint64_t device, uint64_t stream, uint64_t accel, uint64
     const bool sort = cfg.sort_rays && !counting && count > 1 && !P.empty; // per-ray results do not depend on order
     if (sort) LC_HIP_TRY(ensure(dev->sort_scratch, dev->sort_bytes, ray_sort_scratch_bytes(count)));
     {
+        // "trace_*": the whole entry point (sort + traversal); "trace_*_kernel": traversal only
         TimedRegion tr(dev, s, counting ? "trace_count" : (any ? "trace_any" : "trace_closest"));
         if (sort) {
             hipError_t e = ray_sort(s, P.rays, count, a->world_bounds, dev->sort_scratch, dev->sort_bytes, &P.order);
             if (e != hipSuccess) return hip_fail(e, "ray sort");
         }
+        TimedRegion tk(dev, s, counting ? "trace_count_kernel" : (any ? "trace_any_kernel" : "trace_closest_kernel"));
         hipError_t e = trace_launch(P, s, any, two, counting, grid, cfg.block, cfg.slds);
         if (e != hipSuccess) return hip_fail(e, "trace launch");
     }
