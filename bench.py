@@ -37,7 +37,54 @@ def parse():
     ap.add_argument("--cpu-sample-rays", type=int, default=1 << 18)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run 3 device-to-device copies of this many MiB (counter calibration) and exit")
+    ap.add_argument("--workload", default="soup", choices=["soup", "instanced"],
+                    help="soup: the headline line; instanced: 64 ray batches over a 1,024-instance TLAS")
     return ap.parse_args()
+
+
+def bench_instanced(a):
+    """1,024 instances of 8 meshes, 64 closest-hit batches (log-uniform 4,096..4,194,304 rays):
+    one persistent launch for all batches vs one launch per batch."""
+    import luisacompute_amd as lc
+    from luisacompute_amd import autotune, host, workloads
+    w = workloads.Instanced()
+    dev = lc.Device(0)
+    autotune.load_default(dev)
+    s = dev.create_stream()
+    meshes = [host.mesh_from_arrays(dev, s, v, t) for v, t in w.meshes]
+    accel = dev.create_accel()
+    for aff, k in zip(w.affines, w.mesh_ids):
+        accel.emplace_back(meshes[int(k)], aff)
+    s << accel.build()
+    bufs = []
+    for j in range(len(w.batch_sizes)):
+        r = w.batch_rays(j)
+        bufs.append((host.upload(dev, s, r), dev.create_buffer(24, len(r)), len(r)))
+    s.synchronize()
+    total = w.total_rays
+
+    def timed(fn, reps):
+        fn()
+        s.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        s.synchronize()
+        return (time.perf_counter() - t0) / reps * 1e3
+
+    one = timed(lambda: accel.trace_batched(s, bufs), a.steps)
+    per = timed(lambda: [accel.trace_closest(s, rb, hb, n) for rb, hb, n in bufs], a.steps)
+    st = accel.stats()
+    print(json.dumps({
+        "metric": "Mrays/s instanced closest-hit, 64 batches",
+        "value": round(total / (one * 1e-3) / 1e6, 2), "unit": "Mrays/s", "n_gpus": 1, "steps": a.steps,
+        "ms_per_step": round(one, 3), "per_batch_launches_ms": round(per, 3),
+        "per_batch_launches_mrays_s": round(total / (per * 1e-3) / 1e6, 2),
+        "rays": total, "batches": len(w.batch_sizes), "instances": len(w.mesh_ids),
+        "unique_triangles": int(sum(len(t) for _, t in w.meshes)), "tlas_depth": st.depth,
+        "config": {"workload": "1,024 instances of icospheres L3-L8 + soups 200k/500k; 64 batches log-uniform "
+                               "4,096..4,194,304 rays, origin U[-60,60]^3, dir U(S^2)"},
+        "dtype": "f32", "data": "synthetic"}), flush=True)
 
 
 def copy_probe(mib):
@@ -92,6 +139,9 @@ def main():
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (rays and scenes would shard "
               f"by replicating the BVH per GPU, see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if a.workload == "instanced":
+        bench_instanced(a)
+        return
     import luisacompute_amd as lc
     from luisacompute_amd import host
 

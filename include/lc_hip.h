@@ -168,6 +168,21 @@ int lc_hip_accel_trace_closest(uint64_t device, uint64_t stream, uint64_t accel,
                                uint32_t vis_mask);
 int lc_hip_accel_trace_any(uint64_t device, uint64_t stream, uint64_t accel, uint64_t ray_buffer, size_t ray_offset,
                            uint64_t hit_buffer, size_t hit_offset, uint32_t ray_count, uint32_t vis_mask);
+/* Several ray batches traced by one persistent launch per 64 batches (the batches form one queue,
+   sorted together when ray sorting is on); each batch has its own buffers and visibility mask.
+   The same per-ray results as one trace call per batch. */
+typedef struct lc_hip_ray_batch {
+    uint64_t ray_buffer;
+    size_t ray_offset;
+    uint64_t hit_buffer; /* lc_hip_surface_hit per ray (closest) or one byte per ray (any) */
+    size_t hit_offset;
+    uint32_t ray_count;
+    uint32_t vis_mask;
+} lc_hip_ray_batch;
+int lc_hip_accel_trace_closest_batched(uint64_t device, uint64_t stream, uint64_t accel,
+                                       const lc_hip_ray_batch *batches, size_t batch_count);
+int lc_hip_accel_trace_any_batched(uint64_t device, uint64_t stream, uint64_t accel, const lc_hip_ray_batch *batches,
+                                   size_t batch_count);
 /* Counter-mode run of the same traversal: total BVH nodes visited and triangles tested. */
 int lc_hip_accel_trace_count(uint64_t device, uint64_t stream, uint64_t accel, uint64_t ray_buffer, size_t ray_offset,
                              uint32_t ray_count, uint32_t vis_mask, int any, uint64_t *nodes_visited,
@@ -213,6 +228,14 @@ void lc_hip_workload_soup(uint64_t initstate, uint64_t initseq, uint64_t triangl
                           float *vertices4, uint32_t *triangles);
 void lc_hip_workload_rays_box_sphere(uint64_t initstate, uint64_t initseq, uint64_t skip, uint64_t ray_count, float lo,
                                      float hi, lc_hip_ray *rays);
+/* Instances: 8 draws each -- mesh id floor(u * mesh_count), uniform rotation (Shoemake 1992,
+   3 draws), translation U[-extent, extent]^3, scale U[scale_lo, scale_hi]; affines row-major 3x4. */
+void lc_hip_workload_instances(uint64_t initstate, uint64_t initseq, uint32_t instance_count, uint32_t mesh_count,
+                               float translate_extent, float scale_lo, float scale_hi, float *affines,
+                               uint32_t *mesh_ids);
+/* Batch sizes log-uniform in [lo, hi] (one draw each, after `skip` draws). */
+void lc_hip_workload_batch_sizes(uint64_t initstate, uint64_t initseq, uint64_t skip, uint32_t batch_count,
+                                 uint32_t lo, uint32_t hi, uint32_t *sizes);
 
 #ifdef __cplusplus
 }

@@ -60,6 +60,12 @@ class Modification(ctypes.Structure):
     flag_user_id = 32
 
 
+class RayBatch(ctypes.Structure):
+    """lc_hip_ray_batch: one batch of a batched trace launch."""
+    _fields_ = [("ray_buffer", ctypes.c_uint64), ("ray_offset", ctypes.c_size_t), ("hit_buffer", ctypes.c_uint64),
+                ("hit_offset", ctypes.c_size_t), ("ray_count", ctypes.c_uint32), ("vis_mask", ctypes.c_uint32)]
+
+
 class BvhStats(ctypes.Structure):
     _fields_ = [("node_count", ctypes.c_uint32), ("leaf_primitive_count", ctypes.c_uint32), ("depth", ctypes.c_uint32),
                 ("primitive_count", ctypes.c_uint32), ("bounds", ctypes.c_float * 6), ("node_bytes", ctypes.c_uint64),
@@ -122,6 +128,10 @@ _SIGNATURES = {
     "lc_hip_workload_rays_orbit_to_ball": (None, [_U64, _U64, _U64, _F, _P]),
     "lc_hip_workload_soup": (None, [_U64, _U64, _U64, _F, _F, _P, _P]),
     "lc_hip_workload_rays_box_sphere": (None, [_U64, _U64, _U64, _U64, _F, _F, _P]),
+    "lc_hip_workload_instances": (None, [_U64, _U64, _U32, _U32, _F, _F, _F, _P, _P]),
+    "lc_hip_workload_batch_sizes": (None, [_U64, _U64, _U64, _U32, _U32, _U32, _P]),
+    "lc_hip_accel_trace_closest_batched": (ctypes.c_int, [_U64, _U64, _U64, _P, _SZ]),
+    "lc_hip_accel_trace_any_batched": (ctypes.c_int, [_U64, _U64, _U64, _P, _SZ]),
 }
 EXPORTED_SYMBOLS = sorted(_SIGNATURES)
 
@@ -532,6 +542,16 @@ class Accel:
         _check(lib().lc_hip_accel_trace_any(self.device.handle, stream.handle, self.handle, ray_buffer.handle,
                                             ray_offset, hit_buffer.handle, hit_offset, n, vis_mask), "trace_any")
 
+    def trace_batched(self, stream, batches, any_hit=False):
+        """batches: list of (ray_buffer, hit_buffer, count[, vis_mask]); one persistent launch per 64."""
+        arr = (RayBatch * max(1, len(batches)))()
+        for i, b in enumerate(batches):
+            rb, hb, n = b[0], b[1], b[2]
+            arr[i] = RayBatch(rb.handle, 0, hb.handle, 0, n, b[3] if len(b) > 3 else 0xFF)
+        fn = lib().lc_hip_accel_trace_any_batched if any_hit else lib().lc_hip_accel_trace_closest_batched
+        _check(fn(self.device.handle, stream.handle, self.handle, ctypes.cast(arr, ctypes.c_void_p), len(batches)),
+               "trace_batched")
+
     def trace_count(self, stream, ray_buffer, count=None, vis_mask=0xFF, any_hit=False, ray_offset=0):
         n = ray_buffer.count if count is None else count
         a, b = ctypes.c_uint64(), ctypes.c_uint64()
@@ -568,6 +588,20 @@ def rays_box_sphere(n, initstate=42, initseq=2, skip=0, lo=0.0, hi=100.0):
     r = np.zeros(n, RAY_DTYPE)
     lib().lc_hip_workload_rays_box_sphere(initstate, initseq, skip, n, lo, hi, _ptr(r))
     return r
+
+
+def instances(count, mesh_count, initstate=42, initseq=4, extent=50.0, scale_lo=0.5, scale_hi=2.0):
+    """Instance affines (count, 12) and mesh ids: Shoemake rotation, translation, scale."""
+    a = np.zeros((count, 12), np.float32)
+    m = np.zeros(count, np.uint32)
+    lib().lc_hip_workload_instances(initstate, initseq, count, mesh_count, extent, scale_lo, scale_hi, _ptr(a), _ptr(m))
+    return a, m
+
+
+def batch_sizes(count, lo, hi, initstate=42, initseq=4, skip=0):
+    s = np.zeros(count, np.uint32)
+    lib().lc_hip_workload_batch_sizes(initstate, initseq, skip, count, lo, hi, _ptr(s))
+    return s
 
 
 def rays_as_floats(rays):

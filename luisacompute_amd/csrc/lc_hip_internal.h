@@ -29,14 +29,25 @@ hipError_t bvh_validate(hipStream_t stream, const WideNode *nodes, uint32_t node
                         uint32_t tri_count, uint32_t nprims, uint32_t *marks, uint32_t *err7);
 
 // ---- kernels (trace.hip) ----
+// A batch of rays traced by one launch: rays [start, start + count) of the launch's queue are
+// this batch's rays 0..count-1. Up to kMaxBatches batches ride in the kernel arguments.
+constexpr int kMaxBatches = 64;
+struct RayBatch {
+    const float4 *rays; // 2 float4 per ray (Ray layout)
+    void *out;          // 24-B SurfaceHit per ray (closest) or 1 byte per ray (any); null when counting
+    uint32_t start;     // first queue index of the batch (ascending across batches)
+    uint32_t mask;      // ray visibility mask of the batch
+};
+struct BatchTable {
+    RayBatch b[kMaxBatches];
+    uint32_t n;
+};
+
 struct TraceParams {
-    const float4 *rays;   // 2 float4 per ray (Ray layout)
-    uint32_t count;
-    uint32_t vis_mask;    // ray visibility mask
+    BatchTable batches;
+    uint32_t count;        // rays over all batches
     uint32_t *ray_counter;
-    const uint32_t *order; // optional: ray indices in traversal order (sorted queue), else identity
-    void *out_hits;       // 24-B SurfaceHit per ray (closest)
-    uint8_t *out_any;     // 1 byte per ray (any)
+    const uint32_t *order; // optional: queue indices in traversal order (sorted queue), else identity
     // single-instance form
     const WideNode *blas_nodes;
     const TriRecord *blas_tris;
@@ -56,8 +67,16 @@ struct TraceParams {
 
 // ---- kernels (raysort.hip) ----
 size_t ray_sort_scratch_bytes(uint32_t n);
-hipError_t ray_sort(hipStream_t stream, const float4 *rays, uint32_t n, const float bounds[6], void *scratch,
+hipError_t ray_sort(hipStream_t stream, const BatchTable &batches, uint32_t n, const float bounds[6], void *scratch,
                     size_t scratch_bytes, const uint32_t **order_out);
+
+// Queue index -> batch (batches ordered by start).
+__host__ __device__ inline uint32_t find_batch(const BatchTable &t, uint32_t q) {
+    uint32_t b = 0;
+    for (uint32_t step = kMaxBatches / 2; step > 0; step >>= 1)
+        if (b + step < t.n && t.b[b + step].start <= q) b += step;
+    return b;
+}
 
 bool trace_config_supported(int block, int slds);
 int trace_blocks_per_cu(bool any, bool two, bool count, int block, int slds);

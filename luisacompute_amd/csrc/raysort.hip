@@ -28,11 +28,14 @@ __device__ __forceinline__ uint32_t expand9(uint32_t v) {
     return v;
 }
 
-__global__ __launch_bounds__(256) void k_ray_keys(const float4 *rays, uint32_t n, float3 lo, float3 scale,
+__global__ __launch_bounds__(256) void k_ray_keys(const BatchTable batches, uint32_t n, float3 lo, float3 scale,
                                                   uint32_t *keys, uint32_t *idx) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const float4 o = rays[2 * (uint64_t)i], d = rays[2 * (uint64_t)i + 1];
+    const uint32_t b = find_batch(batches, i);
+    const float4 *rays = batches.b[b].rays;
+    const uint32_t r = i - batches.b[b].start;
+    const float4 o = rays[2 * (uint64_t)r], d = rays[2 * (uint64_t)r + 1];
     const float q[3] = {(o.x - lo.x) * scale.x, (o.y - lo.y) * scale.y, (o.z - lo.z) * scale.z};
     uint32_t c[3];
     for (int a = 0; a < 3; a++) c[a] = (uint32_t)fminf(fmaxf(q[a], 0.0f), 511.0f); // NaN -> 0
@@ -56,7 +59,7 @@ size_t ray_sort_scratch_bytes(uint32_t n) { return 4 * align256((size_t)n * 4) +
 
 // Sorts ray indices by (origin Morton code in [lo, hi], direction octant). Returns the order
 // (n indices) inside `scratch`.
-hipError_t ray_sort(hipStream_t stream, const float4 *rays, uint32_t n, const float bounds[6], void *scratch,
+hipError_t ray_sort(hipStream_t stream, const BatchTable &batches, uint32_t n, const float bounds[6], void *scratch,
                     size_t scratch_bytes, const uint32_t **order_out) {
     if (scratch_bytes < ray_sort_scratch_bytes(n)) return hipErrorInvalidValue;
     char *p = static_cast<char *>(scratch);
@@ -71,7 +74,7 @@ hipError_t ray_sort(hipStream_t stream, const float4 *rays, uint32_t n, const fl
     sc.x = ext[0] > 0.0f ? 512.0f / ext[0] : 0.0f;
     sc.y = ext[1] > 0.0f ? 512.0f / ext[1] : 0.0f;
     sc.z = ext[2] > 0.0f ? 512.0f / ext[2] : 0.0f;
-    hipLaunchKernelGGL(k_ray_keys, dim3((n + 255) / 256), dim3(256), 0, stream, rays, n, lo, sc, k0, i0);
+    hipLaunchKernelGGL(k_ray_keys, dim3((n + 255) / 256), dim3(256), 0, stream, batches, n, lo, sc, k0, i0);
     rocprim::double_buffer<uint32_t> kb(k0, k1), vb(i0, i1);
     hipError_t e = rocprim::radix_sort_pairs(temp, tb, kb, vb, n, 0, 30, stream);
     if (e != hipSuccess) return e;

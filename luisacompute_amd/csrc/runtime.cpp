@@ -843,35 +843,40 @@ int lc_hip_accel_instance_info(uint64_t device, uint64_t accel, uint32_t index, 
     return LC_HIP_OK;
 }
 
-static int trace_common(uint64_t device, uint64_t stream, uint64_t accel, uint64_t ray_buffer, size_t ray_offset,
-                        uint64_t hit_buffer, size_t hit_offset, uint32_t count, uint32_t vis_mask, bool any, bool counting,
-                        uint64_t *n_nodes, uint64_t *n_tris) {
-    Device *dev = obj<Device>(device);
-    Accel *a = obj<Accel>(accel);
-    Buffer *rb = obj<Buffer>(ray_buffer);
-    Buffer *hb = counting ? nullptr : obj<Buffer>(hit_buffer);
-    if (!dev || !a || !rb || (!counting && !hb)) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad trace handles");
-    if (ray_offset % 16 != 0) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "ray offset must be 16-byte aligned");
-    if (ray_offset + (size_t)count * 32 > rb->bytes) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "rays outside the ray buffer");
-    if (!counting) {
-        size_t stride = any ? 1 : 24;
-        if (!any && hit_offset % 8 != 0) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "hit offset must be 8-byte aligned");
-        if (hit_offset + (size_t)count * stride > hb->bytes) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "hits outside the hit buffer");
-    }
-    std::lock_guard<std::mutex> lk(a->mu);
-    if (!a->built) return fail(LC_HIP_ERROR_NOT_BUILT, "accel is not built");
-    DeviceGuard g(dev);
-    hipStream_t s = native_stream(stream);
+// One launch over up to kMaxBatches ray batches (queue = the batches back to back).
+static int trace_launch_batches(Device *dev, Accel *a, hipStream_t s, const lc_hip_ray_batch *batches, size_t nb,
+                                bool any, bool counting, uint64_t *n_nodes, uint64_t *n_tris) {
     TraceParams P;
     std::memset(&P, 0, sizeof(P));
-    P.rays = reinterpret_cast<const float4 *>(static_cast<char *>(rb->ptr) + ray_offset);
-    P.count = count;
-    P.vis_mask = vis_mask;
-    if (!counting) {
-        if (any) P.out_any = reinterpret_cast<uint8_t *>(static_cast<char *>(hb->ptr) + hit_offset);
-        else P.out_hits = static_cast<char *>(hb->ptr) + hit_offset;
+    uint64_t total = 0;
+    for (size_t k = 0; k < nb; k++) {
+        const lc_hip_ray_batch &bt = batches[k];
+        Buffer *rb = obj<Buffer>(bt.ray_buffer);
+        Buffer *hb = counting ? nullptr : obj<Buffer>(bt.hit_buffer);
+        if (!rb || (!counting && !hb)) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad trace buffer handles");
+        if (bt.ray_offset % 16 != 0) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "ray offset must be 16-byte aligned");
+        if (bt.ray_offset + (size_t)bt.ray_count * 32 > rb->bytes)
+            return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "rays outside the ray buffer");
+        if (!counting) {
+            const size_t stride = any ? 1 : 24;
+            if (!any && bt.hit_offset % 8 != 0) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "hit offset must be 8-byte aligned");
+            if (bt.hit_offset + (size_t)bt.ray_count * stride > hb->bytes)
+                return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "hits outside the hit buffer");
+        }
+        RayBatch &rbt = P.batches.b[k];
+        rbt.rays = reinterpret_cast<const float4 *>(static_cast<char *>(rb->ptr) + bt.ray_offset);
+        rbt.out = counting ? nullptr : static_cast<char *>(hb->ptr) + bt.hit_offset;
+        rbt.start = (uint32_t)total;
+        rbt.mask = bt.vis_mask;
+        total += bt.ray_count;
     }
-    bool two = a->inst.size() > 1;
+    if (total >= 0xFFFFFFFFull) return fail(LC_HIP_ERROR_UNSUPPORTED, "more than 2^32 - 1 rays in one launch");
+    P.batches.n = (uint32_t)nb;
+    P.count = (uint32_t)total;
+    if (counting && n_nodes) *n_nodes = 0;
+    if (counting && n_tris) *n_tris = 0;
+    if (total == 0) return LC_HIP_OK;
+    const bool two = a->inst.size() > 1;
     uint32_t stack_need = 0;
     if (!two) {
         const InstanceRecord *r = a->host_records.empty() ? nullptr : &a->host_records[0];
@@ -895,7 +900,7 @@ static int trace_common(uint64_t device, uint64_t stream, uint64_t accel, uint64
     std::lock_guard<std::mutex> dl(dev->mu);
     const TraceConfig &cfg = dev->cfg[any ? 1 : 0];
     int bpc = cfg.blocks_per_cu > 0 ? cfg.blocks_per_cu : trace_blocks_per_cu(any, two, counting, cfg.block, cfg.slds);
-    uint64_t want = ((uint64_t)count + cfg.block - 1) / cfg.block;
+    uint64_t want = (total + cfg.block - 1) / cfg.block;
     int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)bpc * dev->cu_count, want));
     P.spill_depth = stack_need > (uint32_t)cfg.slds ? stack_need - (uint32_t)cfg.slds : 0u;
     size_t spill_bytes = (size_t)grid * cfg.block * P.spill_depth * sizeof(uint2);
@@ -906,13 +911,13 @@ static int trace_common(uint64_t device, uint64_t stream, uint64_t accel, uint64
     P.stats = dev->stats;
     P.steps = cfg.steps;
     if (counting) LC_HIP_TRY(hipMemsetAsync(dev->stats, 0, 2 * sizeof(unsigned long long), s));
-    const bool sort = cfg.sort_rays && !counting && count > 1 && !P.empty; // per-ray results do not depend on order
-    if (sort) LC_HIP_TRY(ensure(dev->sort_scratch, dev->sort_bytes, ray_sort_scratch_bytes(count)));
+    const bool sort = cfg.sort_rays && !counting && total > 1 && !P.empty; // per-ray results do not depend on order
+    if (sort) LC_HIP_TRY(ensure(dev->sort_scratch, dev->sort_bytes, ray_sort_scratch_bytes(P.count)));
     {
         // "trace_*": the whole entry point (sort + traversal); "trace_*_kernel": traversal only
         TimedRegion tr(dev, s, counting ? "trace_count" : (any ? "trace_any" : "trace_closest"));
         if (sort) {
-            hipError_t e = ray_sort(s, P.rays, count, a->world_bounds, dev->sort_scratch, dev->sort_bytes, &P.order);
+            hipError_t e = ray_sort(s, P.batches, P.count, a->world_bounds, dev->sort_scratch, dev->sort_bytes, &P.order);
             if (e != hipSuccess) return hip_fail(e, "ray sort");
         }
         TimedRegion tk(dev, s, counting ? "trace_count_kernel" : (any ? "trace_any_kernel" : "trace_closest_kernel"));
@@ -929,23 +934,56 @@ static int trace_common(uint64_t device, uint64_t stream, uint64_t accel, uint64
     return LC_HIP_OK;
 }
 
+static int trace_batches(uint64_t device, uint64_t stream, uint64_t accel, const lc_hip_ray_batch *batches, size_t nb,
+                         bool any, bool counting, uint64_t *n_nodes, uint64_t *n_tris) {
+    Device *dev = obj<Device>(device);
+    Accel *a = obj<Accel>(accel);
+    if (!dev || !a) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad trace handles");
+    if (nb > 0 && !batches) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "null batch table");
+    std::lock_guard<std::mutex> lk(a->mu);
+    if (!a->built) return fail(LC_HIP_ERROR_NOT_BUILT, "accel is not built");
+    DeviceGuard g(dev);
+    hipStream_t s = native_stream(stream);
+    uint64_t nodes = 0, tris = 0;
+    for (size_t k = 0; k < nb; k += kMaxBatches) {
+        uint64_t cn = 0, ct = 0;
+        int rc = trace_launch_batches(dev, a, s, batches + k, std::min<size_t>(kMaxBatches, nb - k), any, counting, &cn, &ct);
+        if (rc != LC_HIP_OK) return rc;
+        nodes += cn;
+        tris += ct;
+    }
+    if (n_nodes) *n_nodes = nodes;
+    if (n_tris) *n_tris = tris;
+    return LC_HIP_OK;
+}
+
 int lc_hip_accel_trace_closest(uint64_t device, uint64_t stream, uint64_t accel, uint64_t ray_buffer, size_t ray_offset,
                                uint64_t hit_buffer, size_t hit_offset, uint32_t ray_count, uint32_t vis_mask) {
-    return trace_common(device, stream, accel, ray_buffer, ray_offset, hit_buffer, hit_offset, ray_count, vis_mask, false,
-                        false, nullptr, nullptr);
+    const lc_hip_ray_batch b{ray_buffer, ray_offset, hit_buffer, hit_offset, ray_count, vis_mask};
+    return trace_batches(device, stream, accel, &b, 1, false, false, nullptr, nullptr);
 }
 
 int lc_hip_accel_trace_any(uint64_t device, uint64_t stream, uint64_t accel, uint64_t ray_buffer, size_t ray_offset,
                            uint64_t hit_buffer, size_t hit_offset, uint32_t ray_count, uint32_t vis_mask) {
-    return trace_common(device, stream, accel, ray_buffer, ray_offset, hit_buffer, hit_offset, ray_count, vis_mask, true,
-                        false, nullptr, nullptr);
+    const lc_hip_ray_batch b{ray_buffer, ray_offset, hit_buffer, hit_offset, ray_count, vis_mask};
+    return trace_batches(device, stream, accel, &b, 1, true, false, nullptr, nullptr);
+}
+
+int lc_hip_accel_trace_closest_batched(uint64_t device, uint64_t stream, uint64_t accel, const lc_hip_ray_batch *batches,
+                                       size_t batch_count) {
+    return trace_batches(device, stream, accel, batches, batch_count, false, false, nullptr, nullptr);
+}
+
+int lc_hip_accel_trace_any_batched(uint64_t device, uint64_t stream, uint64_t accel, const lc_hip_ray_batch *batches,
+                                   size_t batch_count) {
+    return trace_batches(device, stream, accel, batches, batch_count, true, false, nullptr, nullptr);
 }
 
 int lc_hip_accel_trace_count(uint64_t device, uint64_t stream, uint64_t accel, uint64_t ray_buffer, size_t ray_offset,
                              uint32_t ray_count, uint32_t vis_mask, int any, uint64_t *nodes_visited,
                              uint64_t *triangles_tested) {
-    return trace_common(device, stream, accel, ray_buffer, ray_offset, 0, 0, ray_count, vis_mask, any != 0, true,
-                        nodes_visited, triangles_tested);
+    const lc_hip_ray_batch b{ray_buffer, ray_offset, 0, 0, ray_count, vis_mask};
+    return trace_batches(device, stream, accel, &b, 1, any != 0, true, nodes_visited, triangles_tested);
 }
 
 int lc_hip_trace_set_config(uint64_t device, int kind, int block, int lds_stack, int steps, int blocks_per_cu) {

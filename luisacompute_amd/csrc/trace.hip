@@ -196,7 +196,9 @@ __global__ __launch_bounds__(BLOCK) void k_trace(const TraceParams P) {
     };
 
     bool active = false, exhausted = false;
-    uint32_t ray_id = 0;
+    uint32_t ray_id = 0;    // index within the batch
+    uint32_t batch = 0;     // batch of the current ray
+    uint32_t ray_mask = 0;  // its visibility mask
     RaySetup R;           // current (object space when inside an instance)
     float wo[3], wd[3];   // world-space origin/direction (two-level)
     float tmin = 0.0f, tmax = 0.0f;
@@ -222,8 +224,12 @@ __global__ __launch_bounds__(BLOCK) void k_trace(const TraceParams P) {
             if (!active && !exhausted) {
                 const uint32_t my = base + (uint32_t)__popcll(need & ((1ull << lane) - 1ull));
                 if (my < P.count) {
-                    ray_id = P.order ? P.order[my] : my; // sorted queue: fetch in key order
-                    const float4 r0 = P.rays[2 * (uint64_t)ray_id], r1 = P.rays[2 * (uint64_t)ray_id + 1];
+                    const uint32_t q = P.order ? P.order[my] : my; // sorted queue: fetch in key order
+                    batch = find_batch(P.batches, q);
+                    ray_id = q - P.batches.b[batch].start;
+                    ray_mask = P.batches.b[batch].mask;
+                    const float4 *rays = P.batches.b[batch].rays;
+                    const float4 r0 = rays[2 * (uint64_t)ray_id], r1 = rays[2 * (uint64_t)ray_id + 1];
                     wo[0] = r0.x; wo[1] = r0.y; wo[2] = r0.z; tmin = r0.w;
                     wd[0] = r1.x; wd[1] = r1.y; wd[2] = r1.z; tmax = r1.w;
                     t_best = FLT_MAX * 2.0f; // +inf
@@ -243,7 +249,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace(const TraceParams P) {
                         xform_vector(P.inst_inv, wd, R.d);
                         ray_setup(R);
                         g_base = 0;
-                        g_mask = (P.empty || (P.inst_mask & P.vis_mask & 0xFFu) == 0u) ? 0u : ((1u << R.oct) | (1u << 8));
+                        g_mask = (P.empty || (P.inst_mask & ray_mask & 0xFFu) == 0u) ? 0u : ((1u << R.oct) | (1u << 8));
                     }
                     active = true;
                 } else {
@@ -278,7 +284,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace(const TraceParams P) {
                         const InstanceRecord *ir = P.instances + id;
                         const uint32_t imask = ir->vis_mask;
                         const uint64_t bn = ir->blas_nodes;
-                        if (bn != 0u && (imask & P.vis_mask & 0xFFu) != 0u) {
+                        if (bn != 0u && (imask & ray_mask & 0xFFu) != 0u) {
                             if (t_mask != 0u) {
                                 const uint2 e = make_uint2(t_base, t_mask | kInstGroup);
                                 push(e);
@@ -359,9 +365,9 @@ __global__ __launch_bounds__(BLOCK) void k_trace(const TraceParams P) {
                 // the counting form has no output buffers (out_hits/out_any are null)
                 if (COUNT) {
                 } else if (ANY) {
-                    P.out_any[ray_id] = any_hit ? 1u : 0u;
+                    static_cast<uint8_t *>(P.batches.b[batch].out)[ray_id] = any_hit ? 1u : 0u;
                 } else {
-                    uint2 *h = reinterpret_cast<uint2 *>(reinterpret_cast<char *>(P.out_hits) + 24ull * ray_id);
+                    uint2 *h = reinterpret_cast<uint2 *>(static_cast<char *>(P.batches.b[batch].out) + 24ull * ray_id);
                     if (b_prim != 0xFFFFFFFFu) {
                         h[0] = make_uint2(b_inst, b_prim);
                         h[1] = make_uint2(__float_as_uint(bu), __float_as_uint(bv));

@@ -177,6 +177,43 @@ void lc_hip_workload_soup(uint64_t initstate, uint64_t initseq, uint64_t triangl
     });
 }
 
+void lc_hip_workload_instances(uint64_t initstate, uint64_t initseq, uint32_t instance_count, uint32_t mesh_count,
+                               float translate_extent, float scale_lo, float scale_hi, float *affines,
+                               uint32_t *mesh_ids) {
+    // 8 draws per instance: mesh, rotation (Shoemake 1992 uniform quaternion), translation, scale
+    Pcg rng(initstate, initseq);
+    for (uint32_t i = 0; i < instance_count; i++) {
+        const double um = rng.uniform();
+        mesh_ids[i] = std::min<uint32_t>(mesh_count - 1, (uint32_t)(um * mesh_count));
+        const double u1 = rng.uniform(), u2 = rng.uniform(), u3 = rng.uniform();
+        const double x = std::sqrt(1.0 - u1) * std::sin(kTwoPi * u2), y = std::sqrt(1.0 - u1) * std::cos(kTwoPi * u2);
+        const double z = std::sqrt(u1) * std::sin(kTwoPi * u3), w = std::sqrt(u1) * std::cos(kTwoPi * u3);
+        double t[3];
+        for (double &c : t) c = (double)translate_extent * (2.0 * rng.uniform() - 1.0);
+        const double s = (double)scale_lo + ((double)scale_hi - (double)scale_lo) * rng.uniform();
+        const double R[3][3] = {{1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)},
+                                {2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)},
+                                {2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)}};
+        float *a = affines + 12 * (size_t)i;
+        for (int r = 0; r < 3; r++) {
+            for (int c = 0; c < 3; c++) a[4 * r + c] = (float)(s * R[r][c]);
+            a[4 * r + 3] = (float)t[r];
+        }
+    }
+}
+
+void lc_hip_workload_batch_sizes(uint64_t initstate, uint64_t initseq, uint64_t skip, uint32_t batch_count,
+                                 uint32_t lo, uint32_t hi, uint32_t *sizes) {
+    // log-uniform in [lo, hi], rounded
+    Pcg rng(initstate, initseq);
+    rng.skip(skip);
+    const double a = std::log((double)lo), b = std::log((double)hi);
+    for (uint32_t i = 0; i < batch_count; i++) {
+        const double v = std::exp(a + (b - a) * rng.uniform());
+        sizes[i] = (uint32_t)std::min<double>(hi, std::max<double>(lo, std::floor(v + 0.5)));
+    }
+}
+
 void lc_hip_workload_rays_box_sphere(uint64_t initstate, uint64_t initseq, uint64_t skip, uint64_t ray_count, float lo,
                                      float hi, lc_hip_ray *rays) {
     chunked(ray_count, [&](uint64_t b, uint64_t e) {

@@ -47,6 +47,8 @@ def lib():
             "orc_rays_orbit_to_ball": (None, [U64, U64, U64, F, P]),
             "orc_soup": (None, [U64, U64, U64, F, F, P, P]),
             "orc_rays_box_sphere": (None, [U64, U64, U64, U64, F, F, P]),
+            "orc_instances": (None, [U64, U64, U32, U32, F, F, F, P, P]),
+            "orc_batch_sizes": (None, [U64, U64, U64, U32, U32, U32, P]),
             "orc_mesh_build": (P, [P, U64, U64, P, U64, I]),
             "orc_mesh_free": (None, [P]),
             "orc_mesh_node_count": (U64, [P]),
@@ -113,6 +115,19 @@ def rays_box_sphere(n, initstate=42, initseq=2, skip=0, lo=0.0, hi=100.0):
     r = np.zeros((n, 8), np.float32)
     lib().orc_rays_box_sphere(initstate, initseq, skip, n, lo, hi, _p(r))
     return r
+
+
+def instances(count, mesh_count, initstate=42, initseq=4, extent=50.0, slo=0.5, shi=2.0):
+    a = np.zeros((count, 12), np.float32)
+    m = np.zeros(count, np.uint32)
+    lib().orc_instances(initstate, initseq, count, mesh_count, extent, slo, shi, _p(a), _p(m))
+    return a, m
+
+
+def batch_sizes(count, lo, hi, initstate=42, initseq=4, skip=0):
+    s = np.zeros(count, np.uint32)
+    lib().orc_batch_sizes(initstate, initseq, skip, count, lo, hi, _p(s))
+    return s
 
 
 class Mesh:

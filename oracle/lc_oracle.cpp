@@ -601,6 +601,48 @@ void orc_rays_box_sphere(uint64_t initstate, uint64_t initseq, uint64_t skip, ui
     });
 }
 
+void orc_instances(uint64_t initstate, uint64_t initseq, uint32_t count, uint32_t mesh_count, float extent, float slo,
+                   float shi, float *affines, uint32_t *mesh_ids) {
+    const double two_pi = 6.283185307179586476925286766559;
+    orc_pcg32 rng;
+    orc_pcg32_seed(&rng, initstate, initseq);
+    for (uint32_t i = 0; i < count; i++) {
+        double m = (double)pcg_float(&rng) * mesh_count;
+        mesh_ids[i] = std::min<uint32_t>(mesh_count - 1, (uint32_t)m);
+        // Shoemake, "Uniform random rotations", Graphics Gems III (1992)
+        double u1 = pcg_float(&rng), u2 = pcg_float(&rng), u3 = pcg_float(&rng);
+        double q[4] = {std::sqrt(1.0 - u1) * std::sin(two_pi * u2), std::sqrt(1.0 - u1) * std::cos(two_pi * u2),
+                       std::sqrt(u1) * std::sin(two_pi * u3), std::sqrt(u1) * std::cos(two_pi * u3)};
+        double tr[3];
+        for (int k = 0; k < 3; k++) tr[k] = (double)extent * (2.0 * (double)pcg_float(&rng) - 1.0);
+        double sc = (double)slo + ((double)shi - (double)slo) * (double)pcg_float(&rng);
+        const double x = q[0], y = q[1], z = q[2], w = q[3];
+        const double rot[9] = {1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w),
+                               2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w),
+                               2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)};
+        float *a = affines + 12 * (size_t)i;
+        for (int r = 0; r < 3; r++) {
+            a[4 * r + 0] = (float)(sc * rot[3 * r + 0]);
+            a[4 * r + 1] = (float)(sc * rot[3 * r + 1]);
+            a[4 * r + 2] = (float)(sc * rot[3 * r + 2]);
+            a[4 * r + 3] = (float)tr[r];
+        }
+    }
+}
+
+void orc_batch_sizes(uint64_t initstate, uint64_t initseq, uint64_t skip, uint32_t count, uint32_t lo, uint32_t hi,
+                     uint32_t *sizes) {
+    orc_pcg32 rng;
+    orc_pcg32_seed(&rng, initstate, initseq);
+    orc_pcg32_advance(&rng, skip);
+    const double la = std::log((double)lo), lb = std::log((double)hi);
+    for (uint32_t i = 0; i < count; i++) {
+        double v = std::exp(la + (lb - la) * (double)pcg_float(&rng));
+        v = std::floor(v + 0.5);
+        sizes[i] = (uint32_t)std::min<double>(hi, std::max<double>(lo, v));
+    }
+}
+
 void *orc_mesh_build(const float *verts, uint64_t vstride_bytes, uint64_t nverts, const uint32_t *tris,
                      uint64_t ntris, int nthreads) {
     auto *m = new Mesh;

@@ -57,6 +57,14 @@ void orc_soup(uint64_t initstate, uint64_t initseq, uint64_t ntris, float extent
 void orc_rays_box_sphere(uint64_t initstate, uint64_t initseq, uint64_t skip, uint64_t nrays,
                          float lo, float hi, float *rays);
 
+/* instances: per instance 8 draws -- mesh floor(u * mesh_count), Shoemake uniform rotation,
+   translation U[-extent, extent]^3, scale U[slo, shi]; affines row-major 3x4 (12 floats). */
+void orc_instances(uint64_t initstate, uint64_t initseq, uint32_t count, uint32_t mesh_count, float extent, float slo,
+                   float shi, float *affines, uint32_t *mesh_ids);
+/* log-uniform batch sizes in [lo, hi], one draw each after `skip` draws */
+void orc_batch_sizes(uint64_t initstate, uint64_t initseq, uint64_t skip, uint32_t count, uint32_t lo, uint32_t hi,
+                     uint32_t *sizes);
+
 /* ---- geometry / scene ---- */
 typedef struct orc_hit { uint32_t inst; uint32_t prim; float u; float v; float t; uint32_t pad; } orc_hit;
 typedef struct orc_hit2 { uint32_t inst; uint32_t prim; float u; float v; float t; float t2; } orc_hit2;

@@ -177,3 +177,46 @@ def test_update_modifications(device, stream):
     assert g["inst"][0] == 0 and abs(g["committed_ray_t"][0] - 5.0) < 0.05
     aff, vis, uid = accel.instance_info(0)
     assert aff[11] == -1 and vis == 0xFF
+
+
+def test_instanced_batches_one_launch(device, stream):
+    """The instanced workload scaled down (8 small meshes, 1,024 instances, 64 batches of 16-4096
+    rays): one batched launch == one launch per batch == the oracle."""
+    from luisacompute_amd import workloads
+    w = workloads.Instanced(small=True)
+    gm = [host.mesh_from_arrays(device, stream, v, t) for v, t in w.meshes]
+    om = [ob.Mesh(v, t) for v, t in w.meshes]
+    accel = device.create_accel()
+    sc = ob.Scene()
+    for a, k in zip(w.affines, w.mesh_ids):
+        accel.emplace_back(gm[int(k)], a)
+        sc.add(om[int(k)], a)
+    stream << accel.build()
+    sc.commit()
+    rays = [w.batch_rays(j) for j in range(len(w.batch_sizes))]
+    rbufs = [host.upload(device, stream, r) for r in rays]
+    hbufs = [device.create_buffer(24, len(r)) for r in rays]
+    accel.trace_batched(stream, [(rb, hb, len(r)) for rb, hb, r in zip(rbufs, hbufs, rays)])
+    got = []
+    for hb, r in zip(hbufs, rays):
+        out = np.zeros(len(r), lc.HIT_DTYPE)
+        stream << hb.copy_to(out)
+        got.append(out)
+    stream.synchronize()
+    allr = np.concatenate(rays)
+    allg = np.concatenate(got)
+    single = np.concatenate([host.trace_closest(device, stream, accel, r) for r in rays])
+    assert np.array_equal(allg, single)
+    rep = compare_closest(allg, sc.trace_closest2(lc.rays_as_floats(allr)), allr)
+    print(f"[instanced small] {rep}", flush=True)
+    assert rep["hits"] > 0.05 * len(allr)
+    # any-hit, batched
+    abufs = [device.create_buffer(1, len(r)) for r in rays]
+    accel.trace_batched(stream, [(rb, ab, len(r)) for rb, ab, r in zip(rbufs, abufs, rays)], any_hit=True)
+    ga = []
+    for ab, r in zip(abufs, rays):
+        out = np.zeros(len(r), np.uint8)
+        stream << ab.copy_to(out)
+        ga.append(out)
+    stream.synchronize()
+    compare_any(np.concatenate(ga), sc.trace_any(lc.rays_as_floats(allr)))

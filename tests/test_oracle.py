@@ -262,3 +262,25 @@ def test_instance_transform_invariance_oracle():
     assert both.sum() > 4500
     assert np.mean(hi["prim"][both] == hw["prim"][both]) > 0.999
     assert np.allclose(hi["t"][both], hw["t"][both], rtol=1e-4)
+
+
+def test_instance_generator():
+    """Shoemake rotations: s * R with R orthonormal and det +1; translations and scales in range;
+    the product generator equals the oracle's."""
+    a, m = ob.instances(1024, 8)
+    M = a.reshape(-1, 3, 4)[:, :, :3].astype(np.float64)
+    s = np.cbrt(np.linalg.det(M))
+    assert np.all(s > 0.5 - 1e-5) and np.all(s < 2.0 + 1e-5)
+    R = M / s[:, None, None]
+    assert np.allclose(R @ R.transpose(0, 2, 1), np.eye(3), atol=1e-5)
+    t = a.reshape(-1, 3, 4)[:, :, 3]
+    assert t.min() >= -50 and t.max() <= 50
+    assert set(np.unique(m)) == set(range(8))
+    # uniform rotations: the rotated x axis is uniform on the sphere (mean ~ 0)
+    assert np.allclose(R[:, :, 0].mean(0), 0, atol=0.1)
+    a2, m2 = lc.instances(1024, 8)
+    assert np.array_equal(a, a2) and np.array_equal(m, m2)
+    sz = ob.batch_sizes(5000, 4096, 4194304)
+    assert sz.min() >= 4096 and sz.max() <= 4194304
+    assert abs(np.mean(np.log(sz)) - 0.5 * (np.log(4096) + np.log(4194304))) < 0.15  # log-uniform
+    assert np.array_equal(sz, lc.batch_sizes(5000, 4096, 4194304))
