@@ -188,6 +188,53 @@ int lc_hip_accel_trace_count(uint64_t device, uint64_t stream, uint64_t accel, u
                              uint32_t ray_count, uint32_t vis_mask, int any, uint64_t *nodes_visited,
                              uint64_t *triangles_tested);
 
+/* ---- path tracing: the DSL program of src/tests/test_path_tracing.cpp:165-258 (camera ray,
+   closest hit, emission with the balance heuristic, one area-light sample with a shadow ray
+   through offset_ray_origin, cosine-weighted bounce, Russian roulette), run as a built-in
+   kernel over an accel whose instance i is shaded with pt_instances[i]. The reference runs the
+   same program through its AST codegen; here it is compiled into the library, in two forms:
+   LC_HIP_PT_MEGAKERNEL (one launch, a lane per pixel) and LC_HIP_PT_WAVEFRONT (per-bounce
+   compacted ray queues, persistent trace launches, optional queue sort and hipGraph replay).
+   Both give the same image bit for bit. ---- */
+typedef struct lc_hip_pt_instance {
+    uint64_t vertex_buffer;   /* float3 positions at vertex_stride bytes (the mesh's buffer) */
+    size_t vertex_offset;
+    size_t vertex_stride;
+    uint64_t triangle_buffer; /* uint3 per triangle */
+    size_t triangle_offset;
+    float albedo[3];          /* Lambertian reflectance */
+    uint32_t emissive;        /* 1: this instance is the light's geometry; a hit returns light_emission */
+} lc_hip_pt_instance;
+typedef struct lc_hip_pt_scene {
+    float light_position[3];  /* the quad light: position + a u + b v, (a, b) in [0,1)^2 */
+    float light_u[3];
+    float light_v[3];
+    float light_emission[3];  /* radiance, emitted toward normalize(cross(u, v)) */
+    float camera_origin[3];
+    float camera_right[3];    /* pixel (x, y) looks along normalize(fx right - fy up + forward), */
+    float camera_up[3];       /* fx, fy in [-1, 1] along the shorter image side */
+    float camera_forward[3];
+} lc_hip_pt_scene;
+enum { LC_HIP_PT_MEGAKERNEL = 0, LC_HIP_PT_WAVEFRONT = 1 };
+typedef struct lc_hip_pt_options {
+    uint32_t width, height;
+    uint32_t spp;             /* samples per pixel of this call */
+    uint32_t sample_offset;   /* index of the first sample (progressive rendering) */
+    uint32_t max_depth;       /* closest-hit rays per path, 1..12 */
+    uint32_t rr_depth;        /* Russian roulette from this bounce on (0 = the camera ray's hit) */
+    uint64_t seed;            /* PCG32 initstate */
+    uint64_t sequence;        /* pixel i draws from PCG32 stream initseq = sequence + i */
+    uint32_t mode;            /* LC_HIP_PT_MEGAKERNEL or LC_HIP_PT_WAVEFRONT */
+    uint32_t chunk_spp;       /* wavefront: samples per wave of paths (0 = about 16M paths) */
+    uint32_t sort;            /* wavefront: bucket-sort continuation rays (origin cell, octant) */
+    uint32_t graph;           /* wavefront: capture the launches in a hipGraph, replay when unchanged */
+} lc_hip_pt_options;
+/* image: float4 per pixel = (mean radiance of the samples, 1), row-major. counters: 0 or a buffer
+   of two uint64 that receives (closest-hit rays traced, shadow rays traced). */
+int lc_hip_pathtrace(uint64_t device, uint64_t stream, uint64_t accel, const lc_hip_pt_instance *pt_instances,
+                     uint32_t instance_count, const lc_hip_pt_scene *scene, const lc_hip_pt_options *options,
+                     uint64_t image_buffer, size_t image_offset, uint64_t counters_buffer);
+
 /* ---- statistics, tuning and timing (no reference counterpart: reporting for this backend) ---- */
 typedef struct lc_hip_bvh_stats {
     uint32_t node_count;

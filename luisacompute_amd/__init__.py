@@ -72,7 +72,31 @@ class BvhStats(ctypes.Structure):
                 ("primitive_bytes", ctypes.c_uint64)]
 
 
+class PtInstance(ctypes.Structure):
+    """lc_hip_pt_instance: shading data of one accel instance."""
+    _fields_ = [("vertex_buffer", ctypes.c_uint64), ("vertex_offset", ctypes.c_size_t), ("vertex_stride", ctypes.c_size_t),
+                ("triangle_buffer", ctypes.c_uint64), ("triangle_offset", ctypes.c_size_t),
+                ("albedo", ctypes.c_float * 3), ("emissive", ctypes.c_uint32)]
+
+
+class PtScene(ctypes.Structure):
+    """lc_hip_pt_scene: quad light and pinhole camera."""
+    _fields_ = [(n, ctypes.c_float * 3) for n in ("light_position", "light_u", "light_v", "light_emission",
+                                                   "camera_origin", "camera_right", "camera_up", "camera_forward")]
+
+
+class PtOptions(ctypes.Structure):
+    """lc_hip_pt_options."""
+    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("spp", ctypes.c_uint32),
+                ("sample_offset", ctypes.c_uint32), ("max_depth", ctypes.c_uint32), ("rr_depth", ctypes.c_uint32),
+                ("seed", ctypes.c_uint64), ("sequence", ctypes.c_uint64), ("mode", ctypes.c_uint32),
+                ("chunk_spp", ctypes.c_uint32), ("sort", ctypes.c_uint32), ("graph", ctypes.c_uint32)]
+
+
+PT_MEGAKERNEL, PT_WAVEFRONT = 0, 1
+
 assert ctypes.sizeof(Modification) == 80 and ctypes.sizeof(AccelOption) == 8
+assert ctypes.sizeof(PtInstance) == 56 and ctypes.sizeof(PtScene) == 96 and ctypes.sizeof(PtOptions) == 56
 
 _lib = None
 
@@ -132,6 +156,8 @@ _SIGNATURES = {
     "lc_hip_workload_batch_sizes": (None, [_U64, _U64, _U64, _U32, _U32, _U32, _P]),
     "lc_hip_accel_trace_closest_batched": (ctypes.c_int, [_U64, _U64, _U64, _P, _SZ]),
     "lc_hip_accel_trace_any_batched": (ctypes.c_int, [_U64, _U64, _U64, _P, _SZ]),
+    "lc_hip_pathtrace": (ctypes.c_int, [_U64, _U64, _U64, ctypes.POINTER(PtInstance), _U32, ctypes.POINTER(PtScene),
+                                        ctypes.POINTER(PtOptions), _U64, _SZ, _U64]),
 }
 EXPORTED_SYMBOLS = sorted(_SIGNATURES)
 
@@ -551,6 +577,14 @@ class Accel:
         fn = lib().lc_hip_accel_trace_any_batched if any_hit else lib().lc_hip_accel_trace_closest_batched
         _check(fn(self.device.handle, stream.handle, self.handle, ctypes.cast(arr, ctypes.c_void_p), len(batches)),
                "trace_batched")
+
+    def pathtrace(self, stream, instances, scene, options, image_buffer, image_offset=0, counters=None):
+        """Renders into image_buffer (float4 per pixel). instances: one PtInstance per accel
+        instance; scene: PtScene; options: PtOptions; counters: optional Buffer of 2 uint64."""
+        arr = (PtInstance * max(1, len(instances)))(*instances)
+        _check(lib().lc_hip_pathtrace(self.device.handle, stream.handle, self.handle, arr, len(instances),
+                                      ctypes.byref(scene), ctypes.byref(options), image_buffer.handle, image_offset,
+                                      counters.handle if counters is not None else 0), "pathtrace")
 
     def trace_count(self, stream, ray_buffer, count=None, vis_mask=0xFF, any_hit=False, ray_offset=0):
         n = ray_buffer.count if count is None else count

@@ -27,8 +27,8 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unus
 HIP_FLAGS = [f"--offload-arch={ARCH}", "-fgpu-flush-denormals-to-zero",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-munsafe-fp-atomics"]
 
-SOURCES = ["bvh_build.hip", "trace.hip", "raysort.hip", "runtime.cpp", "workload.cpp"]
-HEADERS = ["lc_hip_bvh.h", "lc_hip_internal.h"]
+SOURCES = ["bvh_build.hip", "trace.hip", "raysort.hip", "pathtracer.hip", "runtime.cpp", "workload.cpp"]
+HEADERS = ["lc_hip_bvh.h", "lc_hip_internal.h", "traverse.h"]
 
 
 def _newer(src, dst):
@@ -44,7 +44,6 @@ def _compile(src, force):
     cmd = [HIPCC] + COMMON + ["-c", path, "-o", obj]
     if src.endswith(".hip"):
         cmd[1:1] = HIP_FLAGS
-        cmd += ["-x", "hip"] if False else []
     subprocess.run(cmd, check=True)
     return obj
 

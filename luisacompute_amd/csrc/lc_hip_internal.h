@@ -43,11 +43,8 @@ struct BatchTable {
     uint32_t n;
 };
 
-struct TraceParams {
-    BatchTable batches;
-    uint32_t count;        // rays over all batches
-    uint32_t *ray_counter;
-    const uint32_t *order; // optional: queue indices in traversal order (sorted queue), else identity
+// What a ray is traced against: one instance's BLAS (single-instance form) or TLAS + BLASes.
+struct Geometry {
     // single-instance form
     const WideNode *blas_nodes;
     const TriRecord *blas_tris;
@@ -58,6 +55,15 @@ struct TraceParams {
     const uint32_t *tlas_prims;
     const InstanceRecord *instances;
     uint32_t empty;       // nothing to traverse: every ray misses
+};
+
+struct TraceParams {
+    BatchTable batches;
+    uint32_t count;        // rays over all batches (the capacity when count_ptr is set)
+    const uint32_t *count_ptr; // optional: ray count in device memory (queues built on the GPU)
+    uint32_t *ray_counter;
+    const uint32_t *order; // optional: queue indices in traversal order (sorted queue), else identity
+    Geometry geo;
     uint2 *spill;
     uint32_t spill_depth; // spill entries per lane
     int steps;            // traversal steps between ray refills
@@ -77,6 +83,44 @@ __host__ __device__ inline uint32_t find_batch(const BatchTable &t, uint32_t q) 
         if (b + step < t.n && t.b[b + step].start <= q) b += step;
     return b;
 }
+
+// ---- kernels (pathtracer.hip) ----
+// Shading data of one accel instance (its mesh buffers, object->world affine, material).
+struct PtInstance {
+    const char *vertices;       // float3 at vstride bytes
+    uint64_t vstride;
+    const uint32_t *triangles;  // uint3 per triangle
+    uint32_t vertex_count;      // indices are clamped to it
+    uint32_t _pad;
+    float fwd[12];              // object -> world, row-major 3x4
+    float albedo[3];
+    uint32_t emissive;          // 1: a hit returns the light's emission
+};
+
+// One render's constants (kernel argument).
+struct PtScene {
+    const PtInstance *inst;
+    float light_pos[3], light_u[3], light_v[3], light_e[3];
+    float light_n[3], light_area; // normalize(cross(u, v)), |cross(u, v)|
+    float cam_o[3], cam_r[3], cam_u[3], cam_f[3];
+    uint32_t width, height, spp, sample_offset, max_depth, rr_depth;
+    uint64_t seed, sequence;      // PCG32 initstate; pixel i draws from stream initseq = sequence + i
+};
+
+struct PtOptions {
+    uint32_t wavefront;   // 0: megakernel, 1: wavefront
+    uint32_t chunk_spp;   // wavefront: samples per wave
+    uint32_t sort;        // wavefront: bucket-sort continuation rays by origin cell + direction octant
+    uint32_t graph;       // wavefront: capture the launches in a hipGraph and replay it
+};
+
+struct Device;
+// Enqueues one render on `stream`; image = float4 per pixel (mean radiance over the samples, 1).
+int pt_render(Device *dev, hipStream_t stream, const Geometry &G, bool two, uint32_t stack_need, const float bounds[6],
+              const PtScene &scene, const std::vector<PtInstance> &inst, const PtOptions &opt, float4 *image,
+              unsigned long long *counts, std::string &err);
+void pt_release(Device *dev);
+void pt_release_stream(Device *dev, hipStream_t stream);
 
 bool trace_config_supported(int block, int slds);
 int trace_blocks_per_cu(bool any, bool two, bool count, int block, int slds);
@@ -120,6 +164,7 @@ struct Device {
     };
     std::vector<Pending> pending;
     std::map<std::string, TimingStat> timings;
+    void *pt = nullptr; // path-tracer workspace (pathtracer.hip), released with the device
 };
 
 struct Buffer {

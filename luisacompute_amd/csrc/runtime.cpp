@@ -229,6 +229,7 @@ void lc_hip_device_destroy(uint64_t device) {
     DeviceGuard g(dev);
     hipDeviceSynchronize();
     collect_timings(dev);
+    pt_release(dev);
     if (dev->spill) hipFree(dev->spill);
     if (dev->sort_scratch) hipFree(dev->sort_scratch);
     if (dev->ray_counter) hipFree(dev->ray_counter);
@@ -357,6 +358,7 @@ void lc_hip_stream_destroy(uint64_t device, uint64_t stream) {
     }
     st->qcv.notify_all();
     if (st->worker.joinable()) st->worker.join();
+    pt_release_stream(dev, st->s);
     hipStreamDestroy(st->s);
     delete st;
 }
@@ -810,6 +812,34 @@ static int accel_tlas_sync(Accel *a) {
     return LC_HIP_OK;
 }
 
+// What traversal needs from a built accel: one BLAS (single instance) or the TLAS; the stack
+// entries a ray can need (node groups per wide level + TLAS leaf entries).
+static int accel_geometry(Accel *a, Geometry &G, bool &two, uint32_t &stack_need) {
+    std::memset(&G, 0, sizeof(G));
+    two = a->inst.size() > 1;
+    stack_need = 0;
+    if (!two) {
+        const InstanceRecord *r = a->host_records.empty() ? nullptr : &a->host_records[0];
+        if (!r || r->blas_nodes == 0) {
+            G.empty = 1;
+        } else {
+            G.blas_nodes = reinterpret_cast<const WideNode *>(r->blas_nodes);
+            G.blas_tris = reinterpret_cast<const TriRecord *>(r->blas_tris);
+            std::memcpy(G.inst_inv, r->inv, sizeof(G.inst_inv));
+            G.inst_mask = r->vis_mask;
+            stack_need = r->blas_depth + 1;
+        }
+    } else {
+        int rc = accel_tlas_sync(a);
+        if (rc != LC_HIP_OK) return rc;
+        G.tlas_nodes = a->tlas_nodes;
+        G.tlas_prims = a->tlas_prims;
+        G.instances = a->records;
+        stack_need = a->tlas_stats.depth + 2 + a->max_blas_depth + 1;
+    }
+    return LC_HIP_OK;
+}
+
 int lc_hip_accel_stats(uint64_t device, uint64_t accel, lc_hip_bvh_stats *out) {
     Device *dev = obj<Device>(device);
     Accel *a = obj<Accel>(accel);
@@ -876,27 +906,10 @@ static int trace_launch_batches(Device *dev, Accel *a, hipStream_t s, const lc_h
     if (counting && n_nodes) *n_nodes = 0;
     if (counting && n_tris) *n_tris = 0;
     if (total == 0) return LC_HIP_OK;
-    const bool two = a->inst.size() > 1;
+    bool two = false;
     uint32_t stack_need = 0;
-    if (!two) {
-        const InstanceRecord *r = a->host_records.empty() ? nullptr : &a->host_records[0];
-        if (!r || r->blas_nodes == 0) {
-            P.empty = 1;
-        } else {
-            P.blas_nodes = reinterpret_cast<const WideNode *>(r->blas_nodes);
-            P.blas_tris = reinterpret_cast<const TriRecord *>(r->blas_tris);
-            std::memcpy(P.inst_inv, r->inv, sizeof(P.inst_inv));
-            P.inst_mask = r->vis_mask;
-            stack_need = r->blas_depth + 1;
-        }
-    } else {
-        int rc = accel_tlas_sync(a);
-        if (rc != LC_HIP_OK) return rc;
-        P.tlas_nodes = a->tlas_nodes;
-        P.tlas_prims = a->tlas_prims;
-        P.instances = a->records;
-        stack_need = a->tlas_stats.depth + 2 + a->max_blas_depth + 1;
-    }
+    int grc = accel_geometry(a, P.geo, two, stack_need);
+    if (grc != LC_HIP_OK) return grc;
     std::lock_guard<std::mutex> dl(dev->mu);
     const TraceConfig &cfg = dev->cfg[any ? 1 : 0];
     int bpc = cfg.blocks_per_cu > 0 ? cfg.blocks_per_cu : trace_blocks_per_cu(any, two, counting, cfg.block, cfg.slds);
@@ -911,7 +924,7 @@ static int trace_launch_batches(Device *dev, Accel *a, hipStream_t s, const lc_h
     P.stats = dev->stats;
     P.steps = cfg.steps;
     if (counting) LC_HIP_TRY(hipMemsetAsync(dev->stats, 0, 2 * sizeof(unsigned long long), s));
-    const bool sort = cfg.sort_rays && !counting && total > 1 && !P.empty; // per-ray results do not depend on order
+    const bool sort = cfg.sort_rays && !counting && total > 1 && !P.geo.empty; // per-ray results do not depend on order
     if (sort) LC_HIP_TRY(ensure(dev->sort_scratch, dev->sort_bytes, ray_sort_scratch_bytes(P.count)));
     {
         // "trace_*": the whole entry point (sort + traversal); "trace_*_kernel": traversal only
@@ -984,6 +997,107 @@ int lc_hip_accel_trace_count(uint64_t device, uint64_t stream, uint64_t accel, u
                              uint64_t *triangles_tested) {
     const lc_hip_ray_batch b{ray_buffer, ray_offset, 0, 0, ray_count, vis_mask};
     return trace_batches(device, stream, accel, &b, 1, any != 0, true, nodes_visited, triangles_tested);
+}
+
+int lc_hip_pathtrace(uint64_t device, uint64_t stream, uint64_t accel, const lc_hip_pt_instance *pt_instances,
+                     uint32_t instance_count, const lc_hip_pt_scene *scene, const lc_hip_pt_options *options,
+                     uint64_t image_buffer, size_t image_offset, uint64_t counters_buffer) {
+    Device *dev = obj<Device>(device);
+    Accel *a = obj<Accel>(accel);
+    Buffer *img = obj<Buffer>(image_buffer);
+    if (!dev || !a || !img) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad pathtrace handles");
+    if (!scene || !options || (instance_count > 0 && !pt_instances))
+        return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "null pathtrace argument");
+    const lc_hip_pt_options &o = *options;
+    if (o.width == 0 || o.height == 0 || o.spp == 0) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "empty image or no samples");
+    if ((uint64_t)o.width * o.height >= (1ull << 31)) return fail(LC_HIP_ERROR_UNSUPPORTED, "image too large");
+    if (o.max_depth < 1 || o.max_depth > 12) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "max_depth must be in 1..12");
+    if (o.mode > 1) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "unknown pathtrace mode");
+    if ((uint64_t)o.sample_offset + o.spp > 0xFFFFFFFFull) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "sample index overflow");
+    const uint64_t npix = (uint64_t)o.width * o.height;
+    if (image_offset % 16 != 0 || image_offset + npix * 16 > img->bytes)
+        return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "image outside the image buffer (float4 per pixel, 16-B aligned)");
+    unsigned long long *counts = nullptr;
+    if (counters_buffer) {
+        Buffer *cb = obj<Buffer>(counters_buffer);
+        if (!cb || cb->bytes < 16) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "counters buffer must hold two uint64");
+        counts = static_cast<unsigned long long *>(cb->ptr);
+    }
+    std::lock_guard<std::mutex> lk(a->mu);
+    if (!a->built) return fail(LC_HIP_ERROR_NOT_BUILT, "accel is not built");
+    if (instance_count != a->inst.size())
+        return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "one lc_hip_pt_instance per accel instance");
+    std::vector<PtInstance> table(instance_count);
+    for (uint32_t i = 0; i < instance_count; i++) {
+        const lc_hip_pt_instance &pi = pt_instances[i];
+        Buffer *vb = obj<Buffer>(pi.vertex_buffer);
+        Buffer *tb = obj<Buffer>(pi.triangle_buffer);
+        if (!vb || !tb) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad pathtrace instance buffers");
+        if (pi.vertex_stride < 12 || pi.vertex_stride % 4 != 0 || pi.vertex_offset % 4 != 0 || pi.triangle_offset % 4 != 0)
+            return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "vertex stride/offsets must be 4-byte aligned, stride >= 12");
+        if (pi.vertex_offset + 12 > vb->bytes) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "no vertex in the vertex buffer");
+        const uint64_t nv = (vb->bytes - pi.vertex_offset - 12) / pi.vertex_stride + 1;
+        const Mesh *m = a->inst[i].mesh;
+        const uint64_t nt = m ? m->ntris : 0;
+        if (pi.triangle_offset + nt * 12 > tb->bytes)
+            return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "triangle buffer smaller than the instance's mesh");
+        PtInstance &t = table[i];
+        std::memset(&t, 0, sizeof(t));
+        t.vertices = static_cast<const char *>(vb->ptr) + pi.vertex_offset;
+        t.vstride = pi.vertex_stride;
+        t.triangles = reinterpret_cast<const uint32_t *>(static_cast<const char *>(tb->ptr) + pi.triangle_offset);
+        t.vertex_count = (uint32_t)std::min<uint64_t>(nv, 0xFFFFFFFFull);
+        std::memcpy(t.fwd, a->inst[i].affine, sizeof(t.fwd));
+        std::memcpy(t.albedo, pi.albedo, sizeof(t.albedo));
+        t.emissive = pi.emissive ? 1u : 0u;
+    }
+    PtScene S;
+    std::memset(&S, 0, sizeof(S));
+    for (int k = 0; k < 3; k++) {
+        S.light_pos[k] = scene->light_position[k];
+        S.light_u[k] = scene->light_u[k];
+        S.light_v[k] = scene->light_v[k];
+        S.light_e[k] = scene->light_emission[k];
+        S.cam_o[k] = scene->camera_origin[k];
+        S.cam_r[k] = scene->camera_right[k];
+        S.cam_u[k] = scene->camera_up[k];
+        S.cam_f[k] = scene->camera_forward[k];
+    }
+    {
+        // light normal and area: normalize(cross(u, v)), |cross(u, v)| in single precision
+        const float *u = S.light_u, *v = S.light_v;
+        const float c[3] = {u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0]};
+        const float len = std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+        if (!(len > 0.0f)) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "degenerate light quad");
+        for (int k = 0; k < 3; k++) S.light_n[k] = c[k] / len;
+        S.light_area = len;
+    }
+    S.width = o.width;
+    S.height = o.height;
+    S.spp = o.spp;
+    S.sample_offset = o.sample_offset;
+    S.max_depth = o.max_depth;
+    S.rr_depth = o.rr_depth;
+    S.seed = o.seed;
+    S.sequence = o.sequence;
+    PtOptions po{o.mode == LC_HIP_PT_WAVEFRONT ? 1u : 0u, o.chunk_spp, o.sort ? 1u : 0u, o.graph ? 1u : 0u};
+    DeviceGuard g(dev);
+    hipStream_t s = native_stream(stream);
+    Geometry G;
+    bool two = false;
+    uint32_t stack_need = 0;
+    int rc = accel_geometry(a, G, two, stack_need);
+    if (rc != LC_HIP_OK) return rc;
+    float4 *image = reinterpret_cast<float4 *>(static_cast<char *>(img->ptr) + image_offset);
+    std::string err;
+    int r;
+    {
+        TimedRegion tr(dev, s, po.wavefront ? "pathtrace_wavefront" : "pathtrace_megakernel");
+        r = pt_render(dev, s, G, two, stack_need, a->world_bounds, S, table, po, image, counts, err);
+    }
+    if (r == -2) return fail(LC_HIP_ERROR_UNSUPPORTED, err);
+    if (r != 0) return fail(LC_HIP_ERROR_HIP, err);
+    return LC_HIP_OK;
 }
 
 int lc_hip_trace_set_config(uint64_t device, int kind, int block, int lds_stack, int steps, int blocks_per_cu) {

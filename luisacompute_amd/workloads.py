@@ -50,3 +50,138 @@ class Instanced:
     @property
     def total_rays(self):
         return int(np.sum(self.batch_sizes, dtype=np.uint64))
+
+
+# ---------------------------------------------------------------------------------------------
+# Box room of the path-tracing workload.
+def pcg32_floats(initstate, initseq, n):
+    """n PCG32 floats (x >> 8) * 2^-24 of stream (initstate, initseq) (O'Neill 2014)."""
+    M = (1 << 64) - 1
+    mult = 6364136223846793005
+    inc = ((initseq << 1) | 1) & M
+    state = 0
+
+    def step(s):
+        old = s
+        s = (old * mult + inc) & M
+        xs = (((old >> 18) ^ old) >> 27) & 0xFFFFFFFF
+        rot = old >> 59
+        return s, ((xs >> rot) | (xs << ((-rot) & 31))) & 0xFFFFFFFF
+
+    state, _ = step(state)
+    state = (state + initstate) & M
+    state, _ = step(state)
+    out = []
+    for _ in range(n):
+        state, x = step(state)
+        out.append(np.float32((x >> 8) * 2.0 ** -24))
+    return np.array(out, np.float32)
+
+
+def grid_quad(origin, a, b, n):
+    """n x n quads spanning origin + [0,1] a + [0,1] b, two triangles each, facing cross(a, b)."""
+    origin, a, b = (np.asarray(x, np.float64) for x in (origin, a, b))
+    s = np.arange(n + 1, dtype=np.float64) / n
+    I, J = np.meshgrid(s, s, indexing="ij")
+    p = origin + I[..., None] * a + J[..., None] * b
+    verts = np.zeros(((n + 1) * (n + 1), 4), np.float32)
+    verts[:, :3] = p.reshape(-1, 3)
+    i, j = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    v00 = (i * (n + 1) + j).ravel()
+    v10 = ((i + 1) * (n + 1) + j).ravel()
+    v11 = ((i + 1) * (n + 1) + j + 1).ravel()
+    v01 = (i * (n + 1) + j + 1).ravel()
+    tris = np.empty((2 * n * n, 3), np.uint32)
+    tris[0::2] = np.stack([v00, v10, v11], 1)
+    tris[1::2] = np.stack([v00, v11, v01], 1)
+    return verts, tris
+
+
+def merge(parts):
+    verts, tris, base = [], [], 0
+    for v, t in parts:
+        verts.append(v)
+        tris.append(t + base)
+        base += v.shape[0]
+    return np.concatenate(verts), np.concatenate(tris).astype(np.uint32)
+
+
+def block(lo, hi, m):
+    """Axis-aligned box [lo, hi], m x m quads per face, faces pointing outward."""
+    lo, hi = np.asarray(lo, np.float64), np.asarray(hi, np.float64)
+    e = hi - lo
+    X, Y, Z = np.array([e[0], 0, 0]), np.array([0, e[1], 0]), np.array([0, 0, e[2]])
+    faces = [((hi[0], lo[1], lo[2]), Y, Z), (lo, Z, Y),                 # +x, -x
+             ((lo[0], hi[1], lo[2]), Z, X), (lo, X, Z),                 # +y, -y
+             ((lo[0], lo[1], hi[2]), X, Y), (lo, Y, X)]                 # +z, -z
+    return merge([grid_quad(o, a, b, m) for o, a, b in faces])
+
+
+class BoxRoom:
+    """The path-tracing workload's scene: a closed room [-1,1] x [0,2] x [-1,1] of 6 walls, each
+    wall_quads^2 quads (2 triangles each) facing inward; 2 axis-aligned blocks on the floor, each
+    face block_quads^2 quads; a quad light of radiance `emission` just below the ceiling, facing
+    down. Albedos: U[0.1, 0.9] per channel for the 6 walls and 2 blocks, in that order, from PCG32
+    stream (42, 3). Instances (identity transforms): floor, ceiling, back, front, left, right,
+    short block, tall block, light. The camera sits inside the room by the front wall, looking
+    along -z with a vertical half-angle of 45 degrees (the light is in view)."""
+
+    LIGHT_POSITION = (-0.24, 1.98, -0.22)
+    LIGHT_U = (0.47, 0.0, 0.0)
+    LIGHT_V = (0.0, 0.0, 0.38)
+
+    def __init__(self, wall_quads=512, block_quads=128, emission=15.0, initstate=42, initseq=3, albedo=None):
+        n = wall_quads
+        walls = [((-1, 0, -1), (0, 0, 2), (2, 0, 0)),   # floor, facing +y
+                 ((-1, 2, -1), (2, 0, 0), (0, 0, 2)),   # ceiling, -y
+                 ((-1, 0, -1), (2, 0, 0), (0, 2, 0)),   # back, +z
+                 ((-1, 0, 1), (0, 2, 0), (2, 0, 0)),    # front, -z
+                 ((-1, 0, -1), (0, 2, 0), (0, 0, 2)),   # left, +x
+                 ((1, 0, -1), (0, 0, 2), (0, 2, 0))]    # right, -x
+        self.meshes = [grid_quad(o, a, b, n) for o, a, b in walls]
+        self.meshes.append(block((0.05, 0.0, -0.05), (0.65, 0.6, 0.55), block_quads))    # short block
+        self.meshes.append(block((-0.65, 0.0, -0.65), (-0.05, 1.2, -0.05), block_quads))  # tall block
+        p, u, v = (np.asarray(x, np.float64) for x in (self.LIGHT_POSITION, self.LIGHT_U, self.LIGHT_V))
+        lv = np.zeros((4, 4), np.float32)
+        lv[:, :3] = [p, p + u, p + v, p + u + v]
+        self.meshes.append((lv, np.array([[0, 1, 2], [1, 3, 2]], np.uint32)))
+        if albedo is None:
+            albedo = 0.1 + 0.8 * pcg32_floats(initstate, initseq, 24).astype(np.float64).reshape(8, 3)
+        self.albedo = np.zeros((9, 3), np.float32)
+        self.albedo[:8] = np.asarray(albedo, np.float32).reshape(8, 3)
+        self.emissive = np.array([0] * 8 + [1], np.uint32)
+        self.emission = (emission, emission, emission)
+        t = 1.0
+        self.camera = dict(origin=(0.0, 1.0, 0.98), right=(t, 0.0, 0.0), up=(0.0, t, 0.0), forward=(0.0, 0.0, -1.0))
+
+    @property
+    def triangle_count(self):
+        return sum(t.shape[0] for _, t in self.meshes)
+
+    def scene(self):
+        from . import PtScene
+        s = PtScene()
+        for name, val in (("light_position", self.LIGHT_POSITION), ("light_u", self.LIGHT_U),
+                          ("light_v", self.LIGHT_V), ("light_emission", self.emission),
+                          ("camera_origin", self.camera["origin"]), ("camera_right", self.camera["right"]),
+                          ("camera_up", self.camera["up"]), ("camera_forward", self.camera["forward"])):
+            getattr(s, name)[:] = [float(np.float32(x)) for x in val]
+        return s
+
+    def build(self, device, stream):
+        """Uploads the meshes, builds them and an accel of identity instances; returns
+        (accel, meshes, pt_instances)."""
+        from . import PtInstance, host
+        meshes = [host.mesh_from_arrays(device, stream, v, t) for v, t in self.meshes]
+        accel = device.create_accel()
+        for m in meshes:
+            accel.emplace_back(m)
+        stream << accel.build()
+        inst = []
+        for k, m in enumerate(meshes):
+            pi = PtInstance(m.vb.handle, 0, 16, m.tb.handle, 0)
+            pi.albedo[:] = [float(x) for x in self.albedo[k]]
+            pi.emissive = int(self.emissive[k])
+            inst.append(pi)
+        accel._keep = meshes
+        return accel, meshes, inst

@@ -21,6 +21,17 @@ class Pcg32(ctypes.Structure):
     _fields_ = [("state", ctypes.c_uint64), ("inc", ctypes.c_uint64)]
 
 
+class PtParams(ctypes.Structure):
+    """orc_pt_params."""
+    _fields_ = [(n, ctypes.c_float * 3) for n in ("light_position", "light_u", "light_v", "light_emission",
+                                                   "camera_origin", "camera_right", "camera_up", "camera_forward")] + \
+               [(n, ctypes.c_uint32) for n in ("width", "height", "spp", "sample_offset", "max_depth", "rr_depth")] + \
+               [("seed", ctypes.c_uint64), ("sequence", ctypes.c_uint64)]
+
+
+PT_MATERIAL_DTYPE = np.dtype([("albedo", "<f4", (3,)), ("emissive", "<u4")])
+
+
 def build():
     """Compile the oracle in place (make)."""
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
@@ -61,6 +72,10 @@ def lib():
             "orc_trace_any": (None, [P, P, U64, U32, P, I]),
             "orc_brute_closest_f64": (None, [P, U64, P, U64, P, U64, P, I]),
             "orc_watertight": (I, [P, P, P, P, P, F, F, P, P, P]),
+            "orc_pathtrace": (None, [P, P, ctypes.POINTER(PtParams), U64, U64, P, P, I]),
+            "orc_sincos_2pi": (None, [F, ctypes.POINTER(F), ctypes.POINTER(F)]),
+            "orc_pathtrace_rays": (None, [P, P, ctypes.POINTER(PtParams), U64, U64, P, U64, ctypes.POINTER(U64), P, U64,
+                                          ctypes.POINTER(U64)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -189,6 +204,56 @@ class Scene:
         if getattr(self, "h", None):
             lib().orc_scene_free(self.h)
             self.h = None
+
+
+def pathtrace(scene, albedo, emissive, light, camera, width, height, spp, max_depth=8, rr_depth=3, seed=42,
+              sequence=3 << 32, sample_offset=0, pixel_range=None, nthreads=0):
+    """Oracle path tracer over a committed Scene. albedo: (instances, 3); emissive: (instances,);
+    light: (position, u, v, emission); camera: dict(origin, right, up, forward). Returns
+    (image float32 (height, width, 4), closest-hit rays, shadow rays); pixels outside
+    pixel_range = (begin, end) stay zero."""
+    mats = np.zeros(len(emissive), PT_MATERIAL_DTYPE)
+    mats["albedo"] = np.asarray(albedo, np.float32).reshape(-1, 3)
+    mats["emissive"] = np.asarray(emissive, np.uint32)
+    P = _pt_params(light, camera, width, height, spp, max_depth, rr_depth, seed, sequence, sample_offset)
+    img = np.zeros((height, width, 4), np.float32)
+    counts = np.zeros(2, np.uint64)
+    b, e = pixel_range if pixel_range is not None else (0, width * height)
+    lib().orc_pathtrace(scene.h, _p(mats), ctypes.byref(P), b, e, _p(img), _p(counts), nthreads)
+    return img, int(counts[0]), int(counts[1])
+
+
+def _pt_params(light, camera, width, height, spp, max_depth, rr_depth, seed, sequence, sample_offset):
+    P = PtParams()
+    for name, val in zip(("light_position", "light_u", "light_v", "light_emission"), light):
+        getattr(P, name)[:] = [float(np.float32(x)) for x in val]
+    for name in ("origin", "right", "up", "forward"):
+        getattr(P, "camera_" + name)[:] = [float(np.float32(x)) for x in camera[name]]
+    P.width, P.height, P.spp, P.sample_offset, P.max_depth, P.rr_depth = width, height, spp, sample_offset, max_depth, rr_depth
+    P.seed, P.sequence = seed, sequence
+    return P
+
+
+def pathtrace_rays(scene, albedo, emissive, light, camera, width, height, spp, max_depth=8, rr_depth=3, seed=42,
+                   sequence=3 << 32, sample_offset=0, pixel_range=None, cap=1 << 22):
+    """The rays the oracle's path tracer traces: (closest-hit rays, shadow rays), (n, 8) float32."""
+    mats = np.zeros(len(emissive), PT_MATERIAL_DTYPE)
+    mats["albedo"] = np.asarray(albedo, np.float32).reshape(-1, 3)
+    mats["emissive"] = np.asarray(emissive, np.uint32)
+    P = _pt_params(light, camera, width, height, spp, max_depth, rr_depth, seed, sequence, sample_offset)
+    c = np.zeros((cap, 8), np.float32)
+    s = np.zeros((cap, 8), np.float32)
+    nc, ns = ctypes.c_uint64(), ctypes.c_uint64()
+    b, e = pixel_range if pixel_range is not None else (0, width * height)
+    lib().orc_pathtrace_rays(scene.h, _p(mats), ctypes.byref(P), b, e, _p(c), cap, ctypes.byref(nc), _p(s), cap,
+                             ctypes.byref(ns))
+    return c[:min(cap, nc.value)], s[:min(cap, ns.value)]
+
+
+def sincos_2pi(u):
+    s, c = ctypes.c_float(), ctypes.c_float()
+    lib().orc_sincos_2pi(float(u), ctypes.byref(s), ctypes.byref(c))
+    return s.value, c.value
 
 
 def single_mesh_scene(verts4, tris, nthreads=0):

@@ -269,8 +269,12 @@ void build_bvh(Bvh &bvh, const std::vector<Box> &boxes, int nthreads) {
     bvh.nodes.resize(bvh.node_count.load());
 }
 
-// Robust slab test (Ize 2013): the far distance is scaled by 1 + 2*gamma(3) so that a box
-// touched by the ray is never culled by rounding.
+// Conservative slab test. Ize 2013 scales the box's far distance by 1 + 2*gamma(3) so a box the
+// ray touches survives rounding; the clip against the ray's [tmin, tmax] (and the closest-hit
+// culling distance) must be widened too, or a triangle lying on the box face whose computed t
+// is <= tmax is culled when the box's computed entry rounds above tmax. So the interval test
+// is widened by 2^-12 of the larger distance on both sides: culling only skips work, and the
+// triangle test alone decides what is hit.
 struct SlabRay {
     float o[3], inv[3];
 };
@@ -284,20 +288,18 @@ inline void slab_setup(SlabRay &s, const float o[3], const float d[3]) {
     }
 }
 
-constexpr float kRobust = 1.0f + 2.0f * (3.0f * 0x1p-24f) / (1.0f - 3.0f * 0x1p-24f);
-
 inline bool slab_hit(const SlabRay &s, const Box &b, float tmin, float tmax, float &tnear) {
     float tn = tmin, tf = tmax;
     for (int a = 0; a < 3; a++) {
         float t0 = (b.lo[a] - s.o[a]) * s.inv[a];
         float t1 = (b.hi[a] - s.o[a]) * s.inv[a];
         if (t0 > t1) std::swap(t0, t1);
-        t1 *= kRobust;
         tn = t0 > tn ? t0 : tn;
         tf = t1 < tf ? t1 : tf;
     }
     tnear = tn;
-    return tn <= tf;
+    const float m = 0x1p-12f * std::max(std::fabs(tn), std::fabs(tf));
+    return tn - m <= tf + m;
 }
 
 struct Mesh {
@@ -379,7 +381,11 @@ bool trace_mesh(const Mesh &m, const float o[3], const float d[3], float tmin, f
                                   tmin, tmax, t, u, v))
                     continue;
                 if (ANY) return true;
-                if (t < st.t_best) {
+                // nearer, or as near with a smaller (instance, primitive) key: the result does
+                // not depend on the BVH or the visiting order
+                const bool nearer = t != st.t_best ? t < st.t_best
+                                                   : (inst_id != st.inst ? inst_id < st.inst : p < st.prim);
+                if (nearer) {
                     if (st.track2) st.t2 = st.t_best;
                     st.t_best = t; st.inst = inst_id; st.prim = p; st.u = u; st.v = v;
                 } else if (st.track2 && t < st.t2) {
@@ -811,3 +817,254 @@ int orc_watertight(const float o[3], const float d[3], const float v0[3], const 
 }
 
 } // extern "C"
+
+// ------------------------------------------------------------------------------------------
+// Path tracer: the DSL program of src/tests/test_path_tracing.cpp:165-258 restated on the
+// scene above (camera ray, closest hit, triangle_interpolate, geometric normal, emission with
+// the balance heuristic, one light sample through offset_ray_origin plus a shadow ray, a
+// cosine-weighted bounce in the make_onb frame, Russian roulette with q = max(luminance, 0.05)),
+// with the fixed choices stated in lc_oracle.h. Single precision, no contraction, one operation
+// at a time in a fixed order.
+namespace {
+
+constexpr uint32_t kPtDrawsPerSample = 64;
+constexpr float kPtInvPi = 0.318309886183790672f;
+
+struct F3 {
+    float x, y, z;
+};
+inline F3 f3(float x, float y, float z) { return F3{x, y, z}; }
+inline F3 f3sub(F3 a, F3 b) { return f3(a.x - b.x, a.y - b.y, a.z - b.z); }
+inline float f3dot(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline F3 f3cross(F3 a, F3 b) { return f3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+inline F3 f3normalize(F3 a) {
+    const float l = std::sqrt(f3dot(a, a));
+    return f3(a.x / l, a.y / l, a.z / l);
+}
+
+// sin(2 pi u), cos(2 pi u), u in [0,1): quadrant k = floor(4u) (exact), theta = (4u - k) pi/2,
+// Taylor series of sin to degree 11 and cos to degree 12 (truncation below 6e-8 on [0, pi/2)).
+inline void pt_sincos_2pi(float u, float &s, float &c) {
+    const float x = u * 4.0f;
+    const int k = (int)x;
+    const float f = x - (float)k;
+    const float t = f * 1.57079632679489662f;
+    const float t2 = t * t;
+    const float ps = t * (1.0f + t2 * (-1.0f / 6.0f + t2 * (1.0f / 120.0f + t2 * (-1.0f / 5040.0f +
+                     t2 * (1.0f / 362880.0f + t2 * (-1.0f / 39916800.0f))))));
+    const float pc = 1.0f + t2 * (-1.0f / 2.0f + t2 * (1.0f / 24.0f + t2 * (-1.0f / 720.0f +
+                     t2 * (1.0f / 40320.0f + t2 * (-1.0f / 3628800.0f + t2 * (1.0f / 479001600.0f))))));
+    if ((k & 3) == 0) { s = ps; c = pc; }
+    else if ((k & 3) == 1) { s = pc; c = -ps; }
+    else if ((k & 3) == 2) { s = -ps; c = -pc; }
+    else { s = -pc; c = ps; }
+}
+
+// offset_ray_origin (src/dsl/rtx/ray.cpp:16-24)
+inline float pt_offset1(float p, float n) {
+    const int32_t of = (int32_t)(256.0f * n);
+    uint32_t bits;
+    std::memcpy(&bits, &p, 4);
+    bits += (uint32_t)(p < 0.0f ? -of : of);
+    float pi;
+    std::memcpy(&pi, &bits, 4);
+    return std::fabs(p) < (1.0f / 32.0f) ? p + (1.0f / 65536.0f) * n : pi;
+}
+inline F3 pt_offset(F3 p, F3 n) { return f3(pt_offset1(p.x, n.x), pt_offset1(p.y, n.y), pt_offset1(p.z, n.z)); }
+
+struct PtLight {
+    F3 pos, u, v, e, n;
+    float area;
+};
+
+F3 pt_vertex(const Instance &in, uint32_t idx) {
+    const float *p = &in.mesh->v[3 * (size_t)idx];
+    float w[3];
+    xform_point(in.fwd, p, w);
+    return f3(w[0], w[1], w[2]);
+}
+
+// One sample of pixel (x, y); returns the path's radiance.
+// Optional record of every ray a path traces (8 floats each).
+struct PtRecord {
+    std::vector<float> closest, shadow;
+};
+
+F3 pt_sample(const Scene &sc, const orc_pt_material *mats, const orc_pt_params &P, const PtLight &Lt, uint32_t x,
+             uint32_t y, uint32_t sample, uint64_t &n_closest, uint64_t &n_shadow, PtRecord *rec = nullptr) {
+    const uint32_t pixel = y * P.width + x;
+    orc_pcg32 rng;
+    orc_pcg32_seed(&rng, P.seed, P.sequence + pixel);
+    orc_pcg32_advance(&rng, (uint64_t)kPtDrawsPerSample * sample);
+    // camera ray
+    const float rx = pcg_float(&rng);
+    const float ry = pcg_float(&rng);
+    const float frame = (float)std::min(P.width, P.height);
+    const float fx = (((float)x + rx) * 2.0f - (float)P.width) / frame;
+    const float fy = (((float)y + ry) * 2.0f - (float)P.height) / frame;
+    F3 d = f3normalize(f3(fx * P.camera_right[0] - fy * P.camera_up[0] + P.camera_forward[0],
+                          fx * P.camera_right[1] - fy * P.camera_up[1] + P.camera_forward[1],
+                          fx * P.camera_right[2] - fy * P.camera_up[2] + P.camera_forward[2]));
+    F3 o = f3(P.camera_origin[0], P.camera_origin[1], P.camera_origin[2]);
+    F3 beta = f3(1.0f, 1.0f, 1.0f), L = f3(0.0f, 0.0f, 0.0f);
+    float pdf_bsdf = 0.0f;
+    for (uint32_t depth = 0; depth < P.max_depth; depth++) {
+        const float ray[8] = {o.x, o.y, o.z, 0.0f, d.x, d.y, d.z, std::numeric_limits<float>::max()};
+        TraceState st{std::numeric_limits<float>::infinity(), std::numeric_limits<float>::infinity(), UINT32_MAX,
+                      UINT32_MAX, 0.0f, 0.0f, false};
+        trace_scene<false>(sc, ray, 0xFFu, st);
+        n_closest++;
+        if (rec) rec->closest.insert(rec->closest.end(), ray, ray + 8);
+        if (st.prim == UINT32_MAX) break;
+        const Instance &in = sc.inst[st.inst];
+        const uint32_t *tri = &in.mesh->tri[3 * (size_t)st.prim];
+        const F3 p0 = pt_vertex(in, tri[0]), p1 = pt_vertex(in, tri[1]), p2 = pt_vertex(in, tri[2]);
+        const float bu = st.u, bv = st.v;
+        const float w0 = 1.0f - bu - bv;
+        const F3 p = f3(w0 * p0.x + bu * p1.x + bv * p2.x, w0 * p0.y + bu * p1.y + bv * p2.y,
+                        w0 * p0.z + bu * p1.z + bv * p2.z);
+        const F3 n = f3normalize(f3cross(f3sub(p1, p0), f3sub(p2, p0)));
+        const float cos_wo = -d.x * n.x + -d.y * n.y + -d.z * n.z;
+        if (cos_wo < 1e-4f) break;
+        const orc_pt_material &mat = mats[st.inst];
+        if (mat.emissive) {
+            if (depth == 0) {
+                L = f3(L.x + Lt.e.x, L.y + Lt.e.y, L.z + Lt.e.z);
+            } else {
+                const F3 dp = f3sub(p, o);
+                const float pdf_light = f3dot(dp, dp) / (Lt.area * cos_wo);
+                const float mis = pdf_bsdf / std::fmax(pdf_bsdf + pdf_light, 1e-4f);
+                L = f3(L.x + mis * beta.x * Lt.e.x, L.y + mis * beta.y * Lt.e.y, L.z + mis * beta.z * Lt.e.z);
+            }
+            break;
+        }
+        const float ulx = pcg_float(&rng);
+        const float uly = pcg_float(&rng);
+        const F3 pl = f3(Lt.pos.x + ulx * Lt.u.x + uly * Lt.v.x, Lt.pos.y + ulx * Lt.u.y + uly * Lt.v.y,
+                         Lt.pos.z + ulx * Lt.u.z + uly * Lt.v.z);
+        const F3 pp = pt_offset(p, n);
+        const F3 ppl = pt_offset(pl, Lt.n);
+        const F3 dl = f3sub(ppl, pp);
+        const float d_light = std::sqrt(f3dot(dl, dl));
+        const F3 wi = f3normalize(dl);
+        const float cos_wi_light = f3dot(wi, n);
+        const float cos_light = -f3dot(Lt.n, wi);
+        const F3 albedo = f3(mat.albedo[0], mat.albedo[1], mat.albedo[2]);
+        if (cos_wi_light > 1e-4f && cos_light > 1e-4f) {
+            const float pdf_light = (d_light * d_light) / (Lt.area * cos_light);
+            const float pdf_b = cos_wi_light * kPtInvPi;
+            const float mis = pdf_light / std::fmax(pdf_light + pdf_b, 1e-4f);
+            const float den = std::fmax(pdf_light, 1e-4f);
+            const F3 c = f3(beta.x * (albedo.x * kPtInvPi * cos_wi_light) * mis * Lt.e.x / den,
+                            beta.y * (albedo.y * kPtInvPi * cos_wi_light) * mis * Lt.e.y / den,
+                            beta.z * (albedo.z * kPtInvPi * cos_wi_light) * mis * Lt.e.z / den);
+            const F3 so = pt_offset(pp, n);
+            const float sray[8] = {so.x, so.y, so.z, 0.0f, wi.x, wi.y, wi.z, d_light};
+            TraceState sst{std::numeric_limits<float>::infinity(), std::numeric_limits<float>::infinity(), UINT32_MAX,
+                           UINT32_MAX, 0.0f, 0.0f, false};
+            const bool occluded = trace_scene<true>(sc, sray, 0xFFu, sst);
+            n_shadow++;
+            if (rec) rec->shadow.insert(rec->shadow.end(), sray, sray + 8);
+            if (!occluded) L = f3(L.x + c.x, L.y + c.y, L.z + c.z);
+        }
+        const float ux = pcg_float(&rng);
+        const float uy = pcg_float(&rng);
+        const float r = std::sqrt(ux);
+        float sp, cp;
+        pt_sincos_2pi(uy, sp, cp);
+        const F3 wl = f3(r * cp, r * sp, std::sqrt(1.0f - ux));
+        const F3 bn = f3normalize(std::fabs(n.x) > std::fabs(n.z) ? f3(-n.y, n.x, 0.0f) : f3(0.0f, -n.z, n.y));
+        const F3 tn = f3normalize(f3cross(bn, n));
+        d = f3(wl.x * tn.x + wl.y * bn.x + wl.z * n.x, wl.x * tn.y + wl.y * bn.y + wl.z * n.y,
+               wl.x * tn.z + wl.y * bn.z + wl.z * n.z);
+        o = pp;
+        pdf_bsdf = std::fabs(wl.z) * kPtInvPi;
+        beta = f3(beta.x * albedo.x, beta.y * albedo.y, beta.z * albedo.z);
+        if (depth >= P.rr_depth) {
+            const float l = 0.212671f * beta.x + 0.715160f * beta.y + 0.072169f * beta.z;
+            if (l == 0.0f) break;
+            const float q = std::fmax(l, 0.05f);
+            const float u_rr = pcg_float(&rng);
+            if (u_rr >= q) break;
+            const float iq = 1.0f / q;
+            beta = f3(beta.x * iq, beta.y * iq, beta.z * iq);
+        }
+    }
+    if (std::isnan(L.x) || std::isnan(L.y) || std::isnan(L.z)) L = f3(0.0f, 0.0f, 0.0f);
+    return L;
+}
+
+} // namespace
+
+extern "C" void orc_sincos_2pi(float u, float *s, float *c) { pt_sincos_2pi(u, *s, *c); }
+
+extern "C" void orc_pathtrace_rays(const void *scene, const orc_pt_material *mats, const orc_pt_params *params,
+                                   uint64_t pixel_begin, uint64_t pixel_end, float *closest, uint64_t closest_cap,
+                                   uint64_t *closest_count, float *shadow, uint64_t shadow_cap, uint64_t *shadow_count) {
+    const unsigned int csr = _mm_getcsr();
+    set_ftz_daz();
+    const Scene &sc = *static_cast<const Scene *>(scene);
+    const orc_pt_params P = *params;
+    PtLight Lt;
+    Lt.pos = f3(P.light_position[0], P.light_position[1], P.light_position[2]);
+    Lt.u = f3(P.light_u[0], P.light_u[1], P.light_u[2]);
+    Lt.v = f3(P.light_v[0], P.light_v[1], P.light_v[2]);
+    Lt.e = f3(P.light_emission[0], P.light_emission[1], P.light_emission[2]);
+    const F3 c = f3cross(Lt.u, Lt.v);
+    Lt.area = std::sqrt(f3dot(c, c));
+    Lt.n = f3(c.x / Lt.area, c.y / Lt.area, c.z / Lt.area);
+    PtRecord rec;
+    uint64_t nc = 0, ns = 0;
+    const uint64_t npix = std::min<uint64_t>(pixel_end, (uint64_t)P.width * P.height);
+    for (uint64_t pixel = pixel_begin; pixel < npix; pixel++)
+        for (uint32_t s = 0; s < P.spp; s++)
+            pt_sample(sc, mats, P, Lt, (uint32_t)(pixel % P.width), (uint32_t)(pixel / P.width), P.sample_offset + s, nc,
+                      ns, &rec);
+    _mm_setcsr(csr);
+    *closest_count = rec.closest.size() / 8;
+    *shadow_count = rec.shadow.size() / 8;
+    std::memcpy(closest, rec.closest.data(), sizeof(float) * std::min<uint64_t>(rec.closest.size(), 8 * closest_cap));
+    std::memcpy(shadow, rec.shadow.data(), sizeof(float) * std::min<uint64_t>(rec.shadow.size(), 8 * shadow_cap));
+}
+
+extern "C" void orc_pathtrace(const void *scene, const orc_pt_material *mats, const orc_pt_params *params,
+                              uint64_t pixel_begin, uint64_t pixel_end, float *image, uint64_t counts[2],
+                              int nthreads) {
+    const Scene &sc = *static_cast<const Scene *>(scene);
+    const orc_pt_params P = *params;
+    PtLight Lt;
+    Lt.pos = f3(P.light_position[0], P.light_position[1], P.light_position[2]);
+    Lt.u = f3(P.light_u[0], P.light_u[1], P.light_u[2]);
+    Lt.v = f3(P.light_v[0], P.light_v[1], P.light_v[2]);
+    Lt.e = f3(P.light_emission[0], P.light_emission[1], P.light_emission[2]);
+    const F3 c = f3cross(Lt.u, Lt.v);
+    Lt.area = std::sqrt(f3dot(c, c));
+    Lt.n = f3(c.x / Lt.area, c.y / Lt.area, c.z / Lt.area);
+    std::atomic<uint64_t> nc{0}, ns{0};
+    const uint64_t npix = std::min<uint64_t>(pixel_end, (uint64_t)P.width * P.height);
+    if (pixel_begin >= npix) return;
+    parallel_for(npix - pixel_begin, 16, nthreads, [&](uint64_t b, uint64_t e) {
+        uint64_t n_closest = 0, n_shadow = 0;
+        for (uint64_t k = b; k < e; k++) {
+            const uint64_t pixel = pixel_begin + k;
+            const uint32_t x = (uint32_t)(pixel % P.width), y = (uint32_t)(pixel / P.width);
+            F3 sum = f3(0.0f, 0.0f, 0.0f);
+            for (uint32_t s = 0; s < P.spp; s++) {
+                const F3 L = pt_sample(sc, mats, P, Lt, x, y, P.sample_offset + s, n_closest, n_shadow);
+                sum = f3(sum.x + L.x, sum.y + L.y, sum.z + L.z);
+            }
+            const float fs = (float)P.spp;
+            float *out = image + 4 * pixel;
+            out[0] = sum.x / fs;
+            out[1] = sum.y / fs;
+            out[2] = sum.z / fs;
+            out[3] = 1.0f;
+        }
+        nc += n_closest;
+        ns += n_shadow;
+    });
+    if (counts) {
+        counts[0] += nc.load();
+        counts[1] += ns.load();
+    }
+}

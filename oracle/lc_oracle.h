@@ -96,6 +96,34 @@ void orc_brute_closest_f64(const float *verts, uint64_t vstride_bytes, const uin
                            uint64_t ntris, const float *rays, uint64_t nrays, orc_bf_hit *out,
                            int nthreads);
 
+/* ---- path tracer: the DSL program of src/tests/test_path_tracing.cpp:165-258 on a scene whose
+   instance i is shaded with mats[i]. Fixed choices (the product's path tracer makes the same):
+   pixel (x, y) looks along normalize(fx right - fy up + forward), fx = (2 (x + jx) - width) /
+   min(width, height), fy likewise, with a fresh jitter (jx, jy) per sample; pixel i draws from
+   PCG32 stream (seed, sequence + i), sample s starting 64 s draws into it; per bounce two draws
+   for the light sample, two for the cosine-weighted direction, one for Russian roulette from
+   bounce rr_depth on (q = max(luminance(beta), 0.05)); at most max_depth closest-hit rays per
+   path; a shadow ray only when the light sample can contribute; sin/cos of the sampled angle
+   from the degree 11/12 Taylor polynomials after an exact quadrant reduction; a NaN path
+   counts as zero. image: 4 floats per pixel (mean radiance over the samples, 1); only pixels
+   [pixel_begin, pixel_end) are written. counts[0] += closest-hit rays, counts[1] += shadow rays. */
+typedef struct orc_pt_material { float albedo[3]; uint32_t emissive; } orc_pt_material;
+typedef struct orc_pt_params {
+    float light_position[3], light_u[3], light_v[3], light_emission[3];
+    float camera_origin[3], camera_right[3], camera_up[3], camera_forward[3];
+    uint32_t width, height, spp, sample_offset, max_depth, rr_depth;
+    uint64_t seed, sequence;
+} orc_pt_params;
+void orc_pathtrace(const void *scene, const orc_pt_material *mats, const orc_pt_params *params,
+                   uint64_t pixel_begin, uint64_t pixel_end, float *image, uint64_t counts[2], int nthreads);
+/* every ray the path tracer traces for pixels [pixel_begin, pixel_end) (8 floats each, in path
+   order): closest-hit rays and shadow rays; counts are the totals (arrays hold up to cap). */
+void orc_pathtrace_rays(const void *scene, const orc_pt_material *mats, const orc_pt_params *params,
+                        uint64_t pixel_begin, uint64_t pixel_end, float *closest, uint64_t closest_cap,
+                        uint64_t *closest_count, float *shadow, uint64_t shadow_cap, uint64_t *shadow_count);
+/* the path tracer's sin(2 pi u), cos(2 pi u) (exposed for the accuracy test) */
+void orc_sincos_2pi(float u, float *s, float *c);
+
 /* single watertight test exposed for unit tests (returns 1 on hit inside [tmin,tmax]). */
 int orc_watertight(const float o[3], const float d[3], const float v0[3], const float v1[3],
                    const float v2[3], float tmin, float tmax, float *t, float *u, float *v);
