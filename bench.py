@@ -45,6 +45,8 @@ def parse():
     ap.add_argument("--pt-spp", type=int, default=64)
     ap.add_argument("--pt-chunk", type=int, default=0, help="wavefront samples per wave (0: about 16M paths)")
     ap.add_argument("--pt-sort", type=int, default=1)
+    ap.add_argument("--pt-fetch", default="256,8", help="wavefront traces: slots per claim, queue segments")
+    ap.add_argument("--pt-no-mega", action="store_true", help="skip the megakernel comparison")
     return ap.parse_args()
 
 
@@ -76,6 +78,7 @@ def bench_pathtracer(a):
         o = lc.PtOptions()
         o.width, o.height, o.spp, o.sample_offset, o.max_depth, o.rr_depth = w, h, spp, 0, 8, 3
         o.seed, o.sequence, o.mode, o.chunk_spp, o.sort, o.graph = 42, 3 << 32, mode, a.pt_chunk, a.pt_sort, graph
+        o.fetch_chunk, o.fetch_segments = (int(x) for x in a.pt_fetch.split(","))
         return o
 
     def run(mode, reps, warm):
@@ -93,9 +96,9 @@ def bench_pathtracer(a):
         return ms, int(cnt[0]), int(cnt[1])
 
     wf_ms, nc, ns = run(lc.PT_WAVEFRONT, a.steps, a.warmup)
-    mk_ms, mc, ms_ = run(lc.PT_MEGAKERNEL, max(1, a.steps // 2), 1)
     image = np.zeros((h, w, 4), np.float32)
     s << img.copy_to(image) << lc.synchronize()
+    mk_ms, mc, ms_ = (None, nc, ns) if a.pt_no_mega else run(lc.PT_MEGAKERNEL, max(1, a.steps // 2), 1)
     rays = nc + ns
     paths = w * h * spp
     out = {
@@ -105,12 +108,13 @@ def bench_pathtracer(a):
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "samples_per_s": round(paths / (wf_ms * 1e-3), 1),
         "rays_closest": nc, "rays_shadow": ns,
-        "megakernel_ms": round(mk_ms, 3), "megakernel_mrays_s": round((mc + ms_) / (mk_ms * 1e-3) / 1e6, 2),
-        "wavefront_over_megakernel": round(mk_ms / wf_ms, 3),
+        "megakernel_ms": mk_ms and round(mk_ms, 3),
+        "megakernel_mrays_s": mk_ms and round((mc + ms_) / (mk_ms * 1e-3) / 1e6, 2),
+        "wavefront_over_megakernel": mk_ms and round(mk_ms / wf_ms, 3),
         "same_ray_counts": (mc, ms_) == (nc, ns),
         "image_mean": float(image[..., :3].mean()),
         "config": {"workload": f"box room {room.triangle_count} tris, {w}x{h}, {spp} spp, depth 8, RR from 3, "
-                               f"chunk_spp {a.pt_chunk or 'auto'}, sort {a.pt_sort}, hipGraph replay"},
+                               f"chunk_spp {a.pt_chunk or 'auto'}, sort {a.pt_sort}, fetch {a.pt_fetch}, hipGraph replay"},
     }
     if not a.no_cpu:
         from oracle import binding as ob

@@ -37,7 +37,8 @@ enum {
     LC_HIP_ERROR_HIP = -4,
     LC_HIP_ERROR_OUT_OF_MEMORY = -5,
     LC_HIP_ERROR_UNSUPPORTED = -6,
-    LC_HIP_ERROR_DEVICE_FAULT = -7
+    LC_HIP_ERROR_DEVICE_FAULT = -7,
+    LC_HIP_ERROR_TIMEOUT = -8
 };
 
 /* Ray: include/luisa/runtime/rtx/ray.h:10-16 (32 B, align 16). */
@@ -195,13 +196,10 @@ int lc_hip_accel_trace_count(uint64_t device, uint64_t stream, uint64_t accel, u
    same program through its AST codegen; here it is compiled into the library, in two forms:
    LC_HIP_PT_MEGAKERNEL (one launch, a lane per pixel) and LC_HIP_PT_WAVEFRONT (per-bounce
    compacted ray queues, persistent trace launches, optional queue sort and hipGraph replay).
-   Both give the same image bit for bit. ---- */
+   Both give the same image bit for bit. Hit points and normals come from the triangles the
+   instance's mesh was built from (the reference program reads the same vertices through its
+   bindless heap). ---- */
 typedef struct lc_hip_pt_instance {
-    uint64_t vertex_buffer;   /* float3 positions at vertex_stride bytes (the mesh's buffer) */
-    size_t vertex_offset;
-    size_t vertex_stride;
-    uint64_t triangle_buffer; /* uint3 per triangle */
-    size_t triangle_offset;
     float albedo[3];          /* Lambertian reflectance */
     uint32_t emissive;        /* 1: this instance is the light's geometry; a hit returns light_emission */
 } lc_hip_pt_instance;
@@ -228,12 +226,59 @@ typedef struct lc_hip_pt_options {
     uint32_t chunk_spp;       /* wavefront: samples per wave of paths (0 = about 16M paths) */
     uint32_t sort;            /* wavefront: bucket-sort continuation rays (origin cell, octant) */
     uint32_t graph;           /* wavefront: capture the launches in a hipGraph, replay when unchanged */
+    uint32_t fetch_chunk;     /* wavefront traces: queue slots a wave claims per atomic (0: exact need) */
+    uint32_t fetch_segments;  /* wavefront traces: 1, or 8 (a queue segment per XCD); 0 = chunk 256, 8 */
 } lc_hip_pt_options;
 /* image: float4 per pixel = (mean radiance of the samples, 1), row-major. counters: 0 or a buffer
    of two uint64 that receives (closest-hit rays traced, shadow rays traced). */
 int lc_hip_pathtrace(uint64_t device, uint64_t stream, uint64_t accel, const lc_hip_pt_instance *pt_instances,
                      uint32_t instance_count, const lc_hip_pt_scene *scene, const lc_hip_pt_options *options,
                      uint64_t image_buffer, size_t image_offset, uint64_t counters_buffer);
+
+/* ---- render serving (no reference counterpart: a serving loop over the path tracer above).
+   Scenes (an accel + shading table + camera/light) stay resident; requests are admitted in
+   arrival order while their accumulators fit the HBM budget, and the samples of all admitted
+   requests of a scene are batched into shared wavefront waves (continuous batching: a request
+   that arrives while others run joins the next wave). Each request's image equals a
+   lc_hip_pathtrace render of it alone. The accel must not be rebuilt while the server uses it. */
+enum { LC_HIP_REQUEST_QUEUED = 0, LC_HIP_REQUEST_ACTIVE = 1, LC_HIP_REQUEST_DONE = 2, LC_HIP_REQUEST_FAILED = 3 };
+typedef struct lc_hip_server_options {
+    uint64_t hbm_budget_bytes;  /* accumulator arena: admission budget (float4 per pixel per request) */
+    uint64_t wave_paths;        /* path capacity of a scene's wave */
+    uint32_t max_active;        /* admitted requests at once */
+    uint32_t sort;              /* waves: bucket-sort continuation rays */
+    uint32_t graph;             /* waves: replay each scene's wave from a hipGraph */
+    uint32_t fetch_chunk;       /* waves' traces: see lc_hip_pt_options */
+    uint32_t fetch_segments;
+    uint32_t _pad;
+} lc_hip_server_options;
+typedef struct lc_hip_render_request {
+    uint32_t scene, width, height, spp;
+    uint64_t seed, sequence;    /* PCG32: pixel i draws from stream (seed, sequence + i) */
+    uint64_t image_buffer;      /* float4 per pixel, written when the request completes */
+    size_t image_offset;
+} lc_hip_render_request;
+typedef struct lc_hip_request_status {
+    uint32_t state;             /* LC_HIP_REQUEST_* */
+    uint32_t samples_done;
+    double t_submit, t_admit, t_done; /* seconds since the server was created; -1 = not yet */
+} lc_hip_request_status;
+typedef struct lc_hip_serving_stats {
+    uint64_t waves, rays_closest, rays_shadow, requests_done;
+    uint32_t peak_active, active, queued, _pad;
+    uint64_t peak_bytes;
+} lc_hip_serving_stats;
+uint64_t lc_hip_server_create(uint64_t device, const lc_hip_server_options *options);
+void lc_hip_server_destroy(uint64_t device, uint64_t server);
+int lc_hip_server_add_scene(uint64_t device, uint64_t server, uint64_t accel, const lc_hip_pt_instance *pt_instances,
+                            uint32_t instance_count, const lc_hip_pt_scene *scene, uint32_t max_depth,
+                            uint32_t rr_depth, uint32_t *scene_id);
+int lc_hip_server_submit(uint64_t device, uint64_t server, const lc_hip_render_request *request, uint64_t *request_id);
+int lc_hip_server_query(uint64_t device, uint64_t server, uint64_t request_id, lc_hip_request_status *status);
+/* LC_HIP_OK when done; LC_HIP_ERROR_TIMEOUT when not finished within the timeout. */
+int lc_hip_server_wait(uint64_t device, uint64_t server, uint64_t request_id, double timeout_seconds);
+int lc_hip_server_drain(uint64_t device, uint64_t server, double timeout_seconds);
+int lc_hip_server_stats(uint64_t device, uint64_t server, lc_hip_serving_stats *stats);
 
 /* ---- statistics, tuning and timing (no reference counterpart: reporting for this backend) ---- */
 typedef struct lc_hip_bvh_stats {
@@ -259,6 +304,11 @@ int lc_hip_trace_get_config(uint64_t device, int kind, int *block, int *lds_stac
 /* 1 = traverse rays in the order of a sort by (origin Morton code in the scene box, direction
    octant) so neighbouring lanes share subtrees; results still land in each ray's own slot. */
 int lc_hip_trace_set_ray_sort(uint64_t device, int kind, int enable);
+/* How a persistent trace launch hands out queue slots: a wave claims `chunk` slots per atomic
+   (0: exactly what its idle lanes need, one atomic per refill round) from one queue counter
+   (segments 1) or from a segment of the queue per XCD (segments 8). */
+int lc_hip_trace_set_fetch(uint64_t device, int kind, uint32_t chunk, uint32_t segments);
+int lc_hip_trace_get_fetch(uint64_t device, int kind, uint32_t *chunk, uint32_t *segments);
 /* Build knob: 1 = also open multi-triangle leaf candidates while a node has free slots. */
 int lc_hip_build_set_split_small(uint64_t device, int enable);
 /* Kernel timing with HIP events recorded on the launch stream around each build/trace. */

@@ -73,10 +73,8 @@ class BvhStats(ctypes.Structure):
 
 
 class PtInstance(ctypes.Structure):
-    """lc_hip_pt_instance: shading data of one accel instance."""
-    _fields_ = [("vertex_buffer", ctypes.c_uint64), ("vertex_offset", ctypes.c_size_t), ("vertex_stride", ctypes.c_size_t),
-                ("triangle_buffer", ctypes.c_uint64), ("triangle_offset", ctypes.c_size_t),
-                ("albedo", ctypes.c_float * 3), ("emissive", ctypes.c_uint32)]
+    """lc_hip_pt_instance: material of one accel instance (geometry comes from its mesh)."""
+    _fields_ = [("albedo", ctypes.c_float * 3), ("emissive", ctypes.c_uint32)]
 
 
 class PtScene(ctypes.Structure):
@@ -90,13 +88,44 @@ class PtOptions(ctypes.Structure):
     _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("spp", ctypes.c_uint32),
                 ("sample_offset", ctypes.c_uint32), ("max_depth", ctypes.c_uint32), ("rr_depth", ctypes.c_uint32),
                 ("seed", ctypes.c_uint64), ("sequence", ctypes.c_uint64), ("mode", ctypes.c_uint32),
-                ("chunk_spp", ctypes.c_uint32), ("sort", ctypes.c_uint32), ("graph", ctypes.c_uint32)]
+                ("chunk_spp", ctypes.c_uint32), ("sort", ctypes.c_uint32), ("graph", ctypes.c_uint32),
+                ("fetch_chunk", ctypes.c_uint32), ("fetch_segments", ctypes.c_uint32)]
 
 
 PT_MEGAKERNEL, PT_WAVEFRONT = 0, 1
 
+
+class ServerOptions(ctypes.Structure):
+    """lc_hip_server_options."""
+    _fields_ = [("hbm_budget_bytes", ctypes.c_uint64), ("wave_paths", ctypes.c_uint64), ("max_active", ctypes.c_uint32),
+                ("sort", ctypes.c_uint32), ("graph", ctypes.c_uint32), ("fetch_chunk", ctypes.c_uint32),
+                ("fetch_segments", ctypes.c_uint32), ("_pad", ctypes.c_uint32)]
+
+
+class RenderRequest(ctypes.Structure):
+    """lc_hip_render_request."""
+    _fields_ = [("scene", ctypes.c_uint32), ("width", ctypes.c_uint32), ("height", ctypes.c_uint32),
+                ("spp", ctypes.c_uint32), ("seed", ctypes.c_uint64), ("sequence", ctypes.c_uint64),
+                ("image_buffer", ctypes.c_uint64), ("image_offset", ctypes.c_size_t)]
+
+
+class RequestStatus(ctypes.Structure):
+    """lc_hip_request_status."""
+    _fields_ = [("state", ctypes.c_uint32), ("samples_done", ctypes.c_uint32), ("t_submit", ctypes.c_double),
+                ("t_admit", ctypes.c_double), ("t_done", ctypes.c_double)]
+
+
+class ServingStats(ctypes.Structure):
+    """lc_hip_serving_stats."""
+    _fields_ = [("waves", ctypes.c_uint64), ("rays_closest", ctypes.c_uint64), ("rays_shadow", ctypes.c_uint64),
+                ("requests_done", ctypes.c_uint64), ("peak_active", ctypes.c_uint32), ("active", ctypes.c_uint32),
+                ("queued", ctypes.c_uint32), ("_pad", ctypes.c_uint32), ("peak_bytes", ctypes.c_uint64)]
+
+
+REQUEST_QUEUED, REQUEST_ACTIVE, REQUEST_DONE, REQUEST_FAILED = 0, 1, 2, 3
+
 assert ctypes.sizeof(Modification) == 80 and ctypes.sizeof(AccelOption) == 8
-assert ctypes.sizeof(PtInstance) == 56 and ctypes.sizeof(PtScene) == 96 and ctypes.sizeof(PtOptions) == 56
+assert ctypes.sizeof(PtInstance) == 16 and ctypes.sizeof(PtScene) == 96 and ctypes.sizeof(PtOptions) == 64
 
 _lib = None
 
@@ -143,6 +172,8 @@ _SIGNATURES = {
     "lc_hip_trace_set_config": (ctypes.c_int, [_U64, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "lc_hip_trace_get_config": (ctypes.c_int, [_U64, ctypes.c_int] + [ctypes.POINTER(ctypes.c_int)] * 4),
     "lc_hip_trace_set_ray_sort": (ctypes.c_int, [_U64, ctypes.c_int, ctypes.c_int]),
+    "lc_hip_trace_set_fetch": (ctypes.c_int, [_U64, ctypes.c_int, _U32, _U32]),
+    "lc_hip_trace_get_fetch": (ctypes.c_int, [_U64, ctypes.c_int, ctypes.POINTER(_U32), ctypes.POINTER(_U32)]),
     "lc_hip_build_set_split_small": (ctypes.c_int, [_U64, ctypes.c_int]),
     "lc_hip_timing_enable": (ctypes.c_int, [_U64, ctypes.c_int]),
     "lc_hip_timing_get": (ctypes.c_int, [_U64, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_U64)]),
@@ -158,6 +189,15 @@ _SIGNATURES = {
     "lc_hip_accel_trace_any_batched": (ctypes.c_int, [_U64, _U64, _U64, _P, _SZ]),
     "lc_hip_pathtrace": (ctypes.c_int, [_U64, _U64, _U64, ctypes.POINTER(PtInstance), _U32, ctypes.POINTER(PtScene),
                                         ctypes.POINTER(PtOptions), _U64, _SZ, _U64]),
+    "lc_hip_server_create": (_U64, [_U64, ctypes.POINTER(ServerOptions)]),
+    "lc_hip_server_destroy": (None, [_U64, _U64]),
+    "lc_hip_server_add_scene": (ctypes.c_int, [_U64, _U64, _U64, ctypes.POINTER(PtInstance), _U32, ctypes.POINTER(PtScene),
+                                               _U32, _U32, ctypes.POINTER(_U32)]),
+    "lc_hip_server_submit": (ctypes.c_int, [_U64, _U64, ctypes.POINTER(RenderRequest), ctypes.POINTER(_U64)]),
+    "lc_hip_server_query": (ctypes.c_int, [_U64, _U64, _U64, ctypes.POINTER(RequestStatus)]),
+    "lc_hip_server_wait": (ctypes.c_int, [_U64, _U64, _U64, ctypes.c_double]),
+    "lc_hip_server_drain": (ctypes.c_int, [_U64, _U64, ctypes.c_double]),
+    "lc_hip_server_stats": (ctypes.c_int, [_U64, _U64, ctypes.POINTER(ServingStats)]),
 }
 EXPORTED_SYMBOLS = sorted(_SIGNATURES)
 
@@ -332,6 +372,16 @@ class Device:
     def set_ray_sort(self, kind, enable):
         """Traverse rays in sorted-queue order (origin Morton code, direction octant)."""
         _check(lib().lc_hip_trace_set_ray_sort(self.handle, 1 if kind == "any" else 0, 1 if enable else 0), "ray sort")
+
+    def set_fetch(self, kind, chunk, segments):
+        """Queue-slot hand-out of persistent trace launches (kind 0 closest, 1 any)."""
+        _check(lib().lc_hip_trace_set_fetch(self.handle, 1 if kind == "any" else 0, chunk, segments), "set_fetch")
+
+    def fetch(self, kind):
+        c, s = ctypes.c_uint32(), ctypes.c_uint32()
+        _check(lib().lc_hip_trace_get_fetch(self.handle, 1 if kind == "any" else 0, ctypes.byref(c), ctypes.byref(s)),
+               "fetch")
+        return c.value, s.value
 
     def set_build_split_small(self, enable):
         _check(lib().lc_hip_build_set_split_small(self.handle, 1 if enable else 0), "build config")
@@ -593,6 +643,57 @@ class Accel:
                                               ray_offset, n, vis_mask, 1 if any_hit else 0, ctypes.byref(a),
                                               ctypes.byref(b)), "trace_count")
         return a.value, b.value
+
+
+# ------------------------------------------------------------------ render serving
+class Server:
+    """Render-serving loop (lc_hip_server_*): resident scenes, HBM-budget admission, continuous
+    batching of all admitted requests of a scene into shared wavefront waves."""
+
+    def __init__(self, device, hbm_budget=1 << 30, wave_paths=8 << 20, max_active=64, sort=1, graph=1,
+                 fetch=(256, 8)):
+        self.device = device
+        o = ServerOptions(hbm_budget, wave_paths, max_active, sort, graph, fetch[0], fetch[1], 0)
+        self.handle = _handle(lib().lc_hip_server_create(device.handle, ctypes.byref(o)), "server create")
+        self._keep = []
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().lc_hip_server_destroy(self.device.handle, self.handle)
+            self.handle = None
+
+    def __del__(self):
+        self.close()
+
+    def add_scene(self, accel, instances, scene, max_depth=8, rr_depth=3):
+        arr = (PtInstance * max(1, len(instances)))(*instances)
+        sid = ctypes.c_uint32()
+        _check(lib().lc_hip_server_add_scene(self.device.handle, self.handle, accel.handle, arr, len(instances),
+                                             ctypes.byref(scene), max_depth, rr_depth, ctypes.byref(sid)), "add_scene")
+        self._keep.append(accel)
+        return sid.value
+
+    def submit(self, scene, width, height, spp, image_buffer, seed=42, sequence=0, image_offset=0):
+        q = RenderRequest(scene, width, height, spp, seed, sequence, image_buffer.handle, image_offset)
+        rid = ctypes.c_uint64()
+        _check(lib().lc_hip_server_submit(self.device.handle, self.handle, ctypes.byref(q), ctypes.byref(rid)), "submit")
+        return rid.value
+
+    def query(self, request_id):
+        st = RequestStatus()
+        _check(lib().lc_hip_server_query(self.device.handle, self.handle, request_id, ctypes.byref(st)), "query")
+        return st
+
+    def wait(self, request_id, timeout=600.0):
+        _check(lib().lc_hip_server_wait(self.device.handle, self.handle, request_id, timeout), "wait")
+
+    def drain(self, timeout=600.0):
+        _check(lib().lc_hip_server_drain(self.device.handle, self.handle, timeout), "drain")
+
+    def stats(self):
+        st = ServingStats()
+        _check(lib().lc_hip_server_stats(self.device.handle, self.handle, ctypes.byref(st)), "stats")
+        return st
 
 
 # ------------------------------------------------------------------ synthetic workloads

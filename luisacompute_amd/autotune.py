@@ -19,6 +19,7 @@ STACKS = (8, 16, 24)
 STEPS = (8, 32, 128)
 BLOCKS_PER_CU = (0,)
 SORT = (False, True)
+FETCH = ((0, 1), (64, 1), (256, 1), (64, 8), (256, 8))  # (slots per claim, queue segments)
 
 DEFAULT_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "autotune_table.json")
 
@@ -29,6 +30,7 @@ def apply(device, table):
         if c:
             device.set_trace_config(kind, c["block"], c["lds_stack"], c["steps"], c.get("blocks_per_cu", 0))
             device.set_ray_sort(kind, c.get("sort_rays", False))
+            device.set_fetch(kind, c.get("fetch_chunk", 0), c.get("fetch_segments", 1))
 
 
 def load_default(device):
@@ -71,10 +73,20 @@ def tune(device, stream, accel, ray_buffer, hit_buffer, any_buffer, count, repea
                     f"{ms:.3f} ms ({count / ms / 1e3:.1f} Mrays/s)")
                 timed.append((ms, dict(block=block, lds_stack=slds, steps=steps, blocks_per_cu=bpc, sort_rays=srt)))
         best = min(timed, key=lambda x: x[0])
-        result[kind] = dict(best[1], ms=round(best[0], 4), mrays_s=round(count / best[0] / 1e3, 2))
-        c = best[1]
+        c = dict(best[1])
         device.set_trace_config(kind, c["block"], c["lds_stack"], c["steps"], c["blocks_per_cu"])
         device.set_ray_sort(kind, c["sort_rays"])
+        # third stage: how the launch hands out queue slots, on the best shape
+        fetched = []
+        for chunk, segs in FETCH:
+            device.set_fetch(kind, chunk, segs)
+            ms = _time(device, stream, fn, kind, repeats)
+            log(f"  {kind:7s} fetch chunk={chunk:3d} segments={segs}: {ms:.3f} ms")
+            fetched.append((ms, chunk, segs))
+        ms, chunk, segs = min(fetched)
+        device.set_fetch(kind, chunk, segs)
+        c.update(fetch_chunk=chunk, fetch_segments=segs)
+        result[kind] = dict(c, ms=round(ms, 4), mrays_s=round(count / ms / 1e3, 2))
     return result
 
 

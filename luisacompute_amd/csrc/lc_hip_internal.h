@@ -43,6 +43,12 @@ struct BatchTable {
     uint32_t n;
 };
 
+// A trace launch splits its queue into kFetchSegments contiguous segments, one per XCD, each
+// with its own slot counter on its own 64-byte line (ray_counter[s * kCounterStride]).
+constexpr uint32_t kFetchSegments = 8;
+constexpr uint32_t kCounterStride = 16;
+constexpr uint32_t kRayCounterWords = kFetchSegments * kCounterStride;
+
 // What a ray is traced against: one instance's BLAS (single-instance form) or TLAS + BLASes.
 struct Geometry {
     // single-instance form
@@ -69,6 +75,9 @@ struct TraceParams {
     int steps;            // traversal steps between ray refills
     unsigned long long *stats; // [nodes visited, triangles tested] when counting
     uint32_t *error;
+    uint32_t hit_record;  // closest hits carry the leaf record index in their padding word
+    uint32_t fetch_chunk;    // queue slots a wave claims per atomic (0: exactly what its idle lanes need)
+    uint32_t fetch_segments; // 1: one queue counter; kFetchSegments: one segment + counter per XCD
 };
 
 // ---- kernels (raysort.hip) ----
@@ -85,13 +94,10 @@ __host__ __device__ inline uint32_t find_batch(const BatchTable &t, uint32_t q) 
 }
 
 // ---- kernels (pathtracer.hip) ----
-// Shading data of one accel instance (its mesh buffers, object->world affine, material).
+// Shading data of one accel instance: its BLAS leaf records (the triangles' vertices as the
+// build copied them), object->world affine, material.
 struct PtInstance {
-    const char *vertices;       // float3 at vstride bytes
-    uint64_t vstride;
-    const uint32_t *triangles;  // uint3 per triangle
-    uint32_t vertex_count;      // indices are clamped to it
-    uint32_t _pad;
+    const TriRecord *tris;
     float fwd[12];              // object -> world, row-major 3x4
     float albedo[3];
     uint32_t emissive;          // 1: a hit returns the light's emission
@@ -112,6 +118,40 @@ struct PtOptions {
     uint32_t chunk_spp;   // wavefront: samples per wave
     uint32_t sort;        // wavefront: bucket-sort continuation rays by origin cell + direction octant
     uint32_t graph;       // wavefront: capture the launches in a hipGraph and replay it
+    uint32_t fetch_chunk;    // wavefront traces: queue slots a wave claims per atomic (see TraceParams)
+    uint32_t fetch_segments; // wavefront traces: 1 or kFetchSegments queue segments
+};
+
+// One job's share of a wavefront wave: samples [sample0, sample0 + nsamples) of every pixel of
+// one image. A wave holds up to kMaxWaveJobs jobs; its paths are laid out job after job,
+// sample-major then pixel within a job.
+constexpr uint32_t kMaxWaveJobs = 256;
+struct PtJob {
+    float4 *acc;          // running sum over the job's samples (float4 per pixel)
+    float4 *out;          // final image, written by the job's last wave (may equal acc)
+    uint32_t width, height;
+    uint32_t sample0;     // absolute index of the job's first sample in this wave
+    uint32_t nsamples;    // samples of the job in this wave
+    uint32_t path_base;   // first path of the job in the wave
+    uint32_t pix_base;    // first accumulate thread of the job in the wave
+    uint32_t first;       // 1: the accumulator starts from zero
+    uint32_t finish_spp;  // != 0: the wave ends the job: out = acc / finish_spp with alpha 1
+    uint64_t seed, sequence; // PCG32 initstate; pixel i draws from stream initseq = sequence + i
+};
+static_assert(sizeof(PtJob) == 64, "PtJob is 64 bytes");
+struct PtWaveTable {
+    uint32_t njobs, paths, pixels, _pad;
+    PtJob jobs[kMaxWaveJobs];
+};
+
+// A scene as the wavefront kernels see it.
+struct PtBinding {
+    Geometry G;
+    bool two = false;
+    uint32_t stack_need = 0;
+    float bounds[6] = {0, 0, 0, 0, 0, 0};
+    PtScene S{};          // light, camera, max_depth, rr_depth (per-render fields unused)
+    std::vector<PtInstance> inst;
 };
 
 struct Device;
@@ -119,8 +159,40 @@ struct Device;
 int pt_render(Device *dev, hipStream_t stream, const Geometry &G, bool two, uint32_t stack_need, const float bounds[6],
               const PtScene &scene, const std::vector<PtInstance> &inst, const PtOptions &opt, float4 *image,
               unsigned long long *counts, std::string &err);
+// Enqueues one wavefront wave of `njobs` jobs (at most `capacity` paths) on `stream`, with the
+// stream's workspace; counts[0..1] += closest-hit / shadow rays traced.
+int pt_wave(Device *dev, hipStream_t stream, const PtBinding &B, const PtJob *jobs, uint32_t njobs, size_t capacity,
+            const PtOptions &opt, unsigned long long *counts, std::string &err);
 void pt_release(Device *dev);
 void pt_release_stream(Device *dev, hipStream_t stream);
+
+// ---- serving loop (serve.cpp) ----
+struct ServeOptions {
+    size_t hbm_budget = 1ull << 30; // accumulator arena (admission budget), bytes
+    size_t wave_paths = 8u << 20;   // path capacity of a scene's wave
+    uint32_t max_active = 64;       // admitted requests at once
+    PtOptions pt{};                 // wavefront options of the waves
+};
+struct ServeStatus {
+    int state;
+    uint32_t samples_done;
+    double t_submit, t_admit, t_done; // seconds since the server started (-1: not yet)
+};
+struct ServeStats {
+    uint64_t waves, rays_closest, rays_shadow, requests_done;
+    uint32_t peak_active, active, queued;
+    uint64_t peak_bytes;
+};
+struct Server;
+Server *serve_create(Device *dev, const ServeOptions &opt, std::string &err);
+void serve_destroy(Server *s);
+int serve_add_scene(Server *s, const PtBinding &B, std::string &err); // scene id, or -1
+int serve_submit(Server *s, uint32_t scene, uint32_t width, uint32_t height, uint32_t spp, uint64_t seed,
+                 uint64_t sequence, float4 *out, uint64_t *id, std::string &err);
+int serve_query(Server *s, uint64_t id, ServeStatus *st);   // -1: unknown id
+int serve_wait(Server *s, uint64_t id, double timeout_s);   // 0 done, 1 timeout, -1 unknown, -2 failed
+int serve_drain(Server *s, double timeout_s);               // 0: every request finished, 1 timeout
+void serve_stats(Server *s, ServeStats *st);
 
 bool trace_config_supported(int block, int slds);
 int trace_blocks_per_cu(bool any, bool two, bool count, int block, int slds);
@@ -134,6 +206,8 @@ struct TraceConfig {
     int steps = 32;
     int blocks_per_cu = 0; // 0: occupancy limit
     bool sort_rays = false; // traverse a queue sorted by origin Morton code + direction octant
+    uint32_t fetch_chunk = 0;    // see TraceParams
+    uint32_t fetch_segments = 1;
 };
 
 struct TimingStat {

@@ -2,13 +2,19 @@
 //
 //   megakernel : one persistent launch; a wave takes an 8x8 pixel tile, a lane runs its pixel's
 //                samples one after another, tracing through the Walker of traverse.h;
-//   wavefront  : samples are rendered in chunks ("waves" of paths); per bounce a persistent
-//                closest-hit launch over the compacted ray queue, a shading launch that appends
-//                shadow rays and continuation rays to compacted queues (one atomic per wave),
-//                a persistent any-hit launch over the shadow queue, and a resolve launch; the
-//                continuation queue is optionally bucket-sorted by origin cell and direction
-//                octant. Queue lengths stay on the GPU, so the whole render is captured in one
-//                hipGraph and replayed.
+//   wavefront  : samples are rendered in waves of paths; a wave holds up to kMaxWaveJobs jobs
+//                (an image's sample range each: one render's chunk, or several requests of
+//                the serving loop). A path's state (ray, beta, pdf, radiance, RNG, path id)
+//                travels inside the queue entry, so every launch reads and writes its queue
+//                sequentially. Per bounce: a persistent closest-hit launch over the queue (the
+//                hit carries its BVH leaf record, so shading reads one 48-B record instead of
+//                gathering indices and vertices), a shading launch that appends continuation
+//                and shadow rays to compacted queues (one atomic per block iteration), a
+//                persistent any-hit launch over the shadow queue, a resolve launch, and
+//                optionally a bucket sort that physically reorders the continuation queue by
+//                origin cell and direction octant. Queue lengths and the wave's job table stay
+//                in device memory, so a wave's launches are captured once in a hipGraph and
+//                replayed for every wave.
 //
 // The integrator restates the DSL program of src/tests/test_path_tracing.cpp:165-258
 // (camera ray, closest hit, triangle_interpolate, geometric normal, emission with the balance
@@ -44,6 +50,7 @@ namespace pt {
 constexpr uint32_t kDrawsPerSample = 64; // RNG draws reserved per sample: 2 + 5 per bounce
 constexpr float kInvPi = 0.318309886183790672f;
 constexpr uint32_t kMiss = 0xFFFFFFFFu;
+constexpr uint32_t kEnded = 0x80000000u; // shadow target: the path ended (index into the final radiance)
 
 // ---- PCG32 (O'Neill 2014: pcg32_srandom_r, pcg32_random_r; jump-ahead of Brown 1994) ----
 struct Pcg {
@@ -77,9 +84,6 @@ __device__ __forceinline__ void pcg_advance(Pcg &r, uint64_t delta) {
     r.state = acc_mult * r.state + acc_plus;
 }
 __device__ __forceinline__ float pcg_float(Pcg &r) { return (float)(pcg_next(r) >> 8) * 0x1p-24f; }
-__device__ __forceinline__ uint64_t pixel_inc(const PtScene &S, uint32_t pixel) {
-    return ((S.sequence + pixel) << 1u) | 1u;
-}
 
 // ---- float3 arithmetic, every operation spelled out (no contraction, fixed order) ----
 struct V3 {
@@ -126,23 +130,20 @@ __device__ __forceinline__ V3 offset_ray_origin(V3 p, V3 n) {
     return v3(offset1(p.x, n.x), offset1(p.y, n.y), offset1(p.z, n.z));
 }
 
-__device__ __forceinline__ V3 world_vertex(const PtInstance &I, uint32_t idx) {
-    idx = min(idx, I.vertex_count - 1u); // out-of-range indices read the last vertex, never past the buffer
-    const float *v = reinterpret_cast<const float *>(I.vertices + (uint64_t)idx * I.vstride);
-    const float x = v[0], y = v[1], z = v[2];
-    const float *m = I.fwd;
-    return v3(m[0] * x + m[1] * y + m[2] * z + m[3], m[4] * x + m[5] * y + m[6] * z + m[7],
-              m[8] * x + m[9] * y + m[10] * z + m[11]);
+__device__ __forceinline__ V3 to_world(const float *m, float4 v) {
+    return v3(m[0] * v.x + m[1] * v.y + m[2] * v.z + m[3], m[4] * v.x + m[5] * v.y + m[6] * v.z + m[7],
+              m[8] * v.x + m[9] * v.y + m[10] * v.z + m[11]);
 }
 
 // Camera ray through pixel (x, y) with a jitter of two draws; the film spans [-1, 1] along the
 // shorter image side: direction = normalize(fx * right - fy * up + forward).
-__device__ __forceinline__ void camera_ray(const PtScene &S, uint32_t x, uint32_t y, Pcg &rng, V3 &o, V3 &d) {
+__device__ __forceinline__ void camera_ray(const PtScene &S, uint32_t width, uint32_t height, uint32_t x, uint32_t y,
+                                           Pcg &rng, V3 &o, V3 &d) {
     const float rx = pcg_float(rng);
     const float ry = pcg_float(rng);
-    const float frame = (float)(S.width < S.height ? S.width : S.height);
-    const float fx = (((float)x + rx) * 2.0f - (float)S.width) / frame;
-    const float fy = (((float)y + ry) * 2.0f - (float)S.height) / frame;
+    const float frame = (float)(width < height ? width : height);
+    const float fx = (((float)x + rx) * 2.0f - (float)width) / frame;
+    const float fy = (((float)y + ry) * 2.0f - (float)height) / frame;
     const V3 w = v3(fx * S.cam_r[0] - fy * S.cam_u[0] + S.cam_f[0], fx * S.cam_r[1] - fy * S.cam_u[1] + S.cam_f[1],
                     fx * S.cam_r[2] - fy * S.cam_u[2] + S.cam_f[2]);
     d = normalize(w);
@@ -159,17 +160,18 @@ struct Bounce {
     V3 no, nd;
 };
 
-// One path vertex at bounce `depth` (0: the camera ray's hit). Emission is added to L here; the
-// light sample's contribution is returned for the caller to add after the shadow ray.
-__device__ __forceinline__ void shade(const PtScene &S, uint32_t depth, V3 ro, V3 rd, uint32_t inst, uint32_t prim,
+// One path vertex at bounce `depth` (0: the camera ray's hit); `rec` is the hit triangle's leaf
+// record in the instance's BLAS. Emission is added to L here; the light sample's contribution
+// is returned for the caller to add after the shadow ray.
+__device__ __forceinline__ void shade(const PtScene &S, uint32_t depth, V3 ro, V3 rd, uint32_t inst, uint32_t rec,
                                       float bu, float bv, Pcg &rng, V3 &beta, float &pdf_bsdf, V3 &L, Bounce &out) {
     out.shadow = false;
     out.cont = false;
     if (inst == kMiss) return;
     const PtInstance &I = S.inst[inst];
-    const uint32_t *tri = I.triangles + 3ull * prim;
-    const uint32_t i0 = tri[0], i1 = tri[1], i2 = tri[2];
-    const V3 p0 = world_vertex(I, i0), p1 = world_vertex(I, i1), p2 = world_vertex(I, i2);
+    const float4 *tr = reinterpret_cast<const float4 *>(I.tris + rec);
+    const float4 a = tr[0], b = tr[1], c = tr[2];
+    const V3 p0 = to_world(I.fwd, a), p1 = to_world(I.fwd, b), p2 = to_world(I.fwd, c);
     // triangle_interpolate: (1 - u - v) a + u b + v c
     const float w0 = 1.0f - bu - bv;
     const V3 p = v3(w0 * p0.x + bu * p1.x + bv * p2.x, w0 * p0.y + bu * p1.y + bv * p2.y,
@@ -243,7 +245,7 @@ __device__ __forceinline__ void shade(const PtScene &S, uint32_t depth, V3 ro, V
     out.cont = depth + 1u < S.max_depth;
 }
 
-__device__ __forceinline__ bool has_nan(V3 a) { return isnan(a.x) || isnan(a.y) || isnan(a.z); }
+__device__ __forceinline__ bool has_nan(float x, float y, float z) { return isnan(x) || isnan(y) || isnan(z); }
 
 // ---------------------------------------------------------------------------------------------
 // megakernel
@@ -256,8 +258,8 @@ __global__ __launch_bounds__(BLOCK) void k_pt_mega(const PtScene S, const Geomet
     const uint64_t glane = (uint64_t)blockIdx.x * BLOCK + tid;
     trav::Walker<false, TWO, false, BLOCK, SLDS> Wc;
     trav::Walker<true, TWO, false, BLOCK, SLDS> Wa;
-    Wc.init(stk, tid, spill + glane * spill_depth, spill_depth, error);
-    Wa.init(stk, tid, spill + glane * spill_depth, spill_depth, error);
+    Wc.init(G, stk, tid, spill + glane * spill_depth, spill_depth, error);
+    Wa.init(G, stk, tid, spill + glane * spill_depth, spill_depth, error);
     const uint32_t tiles_x = (S.width + 7u) / 8u, tiles = tiles_x * ((S.height + 7u) / 8u);
     unsigned long long n_closest = 0, n_shadow = 0;
     for (;;) {
@@ -274,7 +276,7 @@ __global__ __launch_bounds__(BLOCK) void k_pt_mega(const PtScene S, const Geomet
                 pcg_seed(rng, S.seed, S.sequence + pixel);
                 pcg_advance(rng, (uint64_t)kDrawsPerSample * (S.sample_offset + s));
                 V3 o, d;
-                camera_ray(S, x, y, rng, o, d);
+                camera_ray(S, S.width, S.height, x, y, rng, o, d);
                 V3 beta = v3(1.0f, 1.0f, 1.0f), L = v3(0.0f, 0.0f, 0.0f);
                 float pdf = 0.0f;
                 for (uint32_t depth = 0; depth < S.max_depth; depth++) {
@@ -283,7 +285,7 @@ __global__ __launch_bounds__(BLOCK) void k_pt_mega(const PtScene S, const Geomet
                     }
                     n_closest++;
                     Bounce b;
-                    shade(S, depth, o, d, Wc.b_inst, Wc.b_prim, Wc.bu, Wc.bv, rng, beta, pdf, L, b);
+                    shade(S, depth, o, d, Wc.b_inst, Wc.b_rec, Wc.bu, Wc.bv, rng, beta, pdf, L, b);
                     if (b.shadow) {
                         Wa.start(G, b.so.x, b.so.y, b.so.z, 0.0f, b.wi.x, b.wi.y, b.wi.z, b.d_light, 0xFFu);
                         while (!Wa.run(G, 64)) {
@@ -295,7 +297,7 @@ __global__ __launch_bounds__(BLOCK) void k_pt_mega(const PtScene S, const Geomet
                     o = b.no;
                     d = b.nd;
                 }
-                if (has_nan(L)) L = v3(0.0f, 0.0f, 0.0f);
+                if (has_nan(L.x, L.y, L.z)) L = v3(0.0f, 0.0f, 0.0f);
                 sum = v3(sum.x + L.x, sum.y + L.y, sum.z + L.z);
             }
             const float fs = (float)S.spp;
@@ -314,155 +316,223 @@ __global__ __launch_bounds__(BLOCK) void k_pt_mega(const PtScene S, const Geomet
 
 // ---------------------------------------------------------------------------------------------
 // wavefront
-struct Queues {
-    uint64_t *rng;    // per path: PCG32 state (the increment follows from the pixel)
-    float4 *beta;     // per path: beta.xyz, pdf_bsdf
-    float4 *L;        // per path: radiance
-    float4 *qr[2];    // ray queues (ping-pong), 2 float4 per ray
-    uint32_t *qp[2];  // path of each queued ray
-    uint2 *hits;      // closest hits of the current queue (24 B per ray)
-    float4 *sr;       // shadow rays
-    uint32_t *sp;     // path of each shadow ray
-    float4 *sc;       // contribution of each shadow ray
-    uint8_t *occ;     // any-hit results of the shadow queue
-    uint16_t *keys;   // bucket of each continuation ray
-    uint32_t *order;  // sorted queue -> queue index
-    uint32_t *hist;   // [bin][block] counts, then offsets
-    uint32_t *bin_total, *bin_base;
-    uint32_t *ctr;    // [0..1] queue lengths, [2] shadow queue length, [3] megakernel tiles,
-                      // [4] closest ray counter, [5] any ray counter
+struct Queue {         // one ray queue, one array per field (coalesced per field)
+    float4 *ray;       // 2 per entry: origin, tmin / direction, tmax
+    float4 *beta;      // beta.xyz, pdf_bsdf
+    float4 *rad;       // radiance so far .xyz, path id bits in .w
+    uint4 *rng;        // PCG32 state (x, y) and increment (z, w)
+    uint16_t *key;     // sort bucket (continuation queues)
 };
+
+struct Queues {
+    Queue q[3];        // rotating: shade reads one, appends to the next, the sort writes the spare
+    uint2 *hits;       // closest hits of the current queue (leaf record in the padding word)
+    float4 *sr;        // shadow rays, 2 per entry
+    float4 *sc;        // contribution .xyz; .w bits: continuation slot, or path | kEnded
+    uint8_t *occ;      // any-hit results of the shadow queue
+    float4 *Lf;        // final radiance per path of the wave
+    uint32_t *hist;    // [bin][block] counts, then offsets
+    uint32_t *bin_total, *bin_base;
+    uint32_t *ctr;     // [0..2] queue lengths, [3] shadow queue length, [4] megakernel tiles,
+                       // [kCtrClosest..) / [kCtrAny..) the trace launches' segment counters
+};
+constexpr uint32_t kCtrClosest = 64, kCtrAny = kCtrClosest + kRayCounterWords;
+constexpr uint32_t kCtrWords = kCtrAny + kRayCounterWords;
 
 constexpr uint32_t kBins = 4096;     // 9-bit origin cell (3 bits per axis, Morton order) x octant
 constexpr uint32_t kSortBlocks = 1024;
 
-__global__ void k_pt_camera(const PtScene S, uint32_t sample_base, uint32_t chunk_spp, float4 *qr, uint32_t *qp,
-                            uint64_t *rng_state, float4 *beta, float4 *L, uint32_t *count) {
-    const uint32_t npix = S.width * S.height;
-    const uint64_t total = (uint64_t)npix * chunk_spp;
-    const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p == 0) *count = (uint32_t)total;
-    if (p >= total) return;
-    const uint32_t s_local = (uint32_t)(p / npix), pixel = (uint32_t)(p % npix);
-    const uint32_t x = pixel % S.width, y = pixel / S.width;
-    Pcg rng;
-    pcg_seed(rng, S.seed, S.sequence + pixel);
-    pcg_advance(rng, (uint64_t)kDrawsPerSample * (S.sample_offset + sample_base + s_local));
-    V3 o, d;
-    camera_ray(S, x, y, rng, o, d);
-    qr[2 * p] = make_float4(o.x, o.y, o.z, 0.0f);
-    qr[2 * p + 1] = make_float4(d.x, d.y, d.z, FLT_MAX);
-    qp[p] = (uint32_t)p;
-    rng_state[p] = rng.state;
-    beta[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-    L[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+struct SortBox {
+    float lo[3], scale[3]; // cell = (o - lo) * scale, 8 cells per axis
+};
+
+__device__ __forceinline__ uint32_t spread3(uint32_t v) { // 3 bits -> positions 0, 3, 6
+    return (v & 1u) | ((v & 2u) << 2) | ((v & 4u) << 4);
 }
 
-__global__ __launch_bounds__(256) void k_pt_shade(const PtScene S, uint32_t depth, const Queues Q, uint32_t cur,
-                                                  const uint32_t *order) {
+__device__ __forceinline__ uint32_t bucket_of(V3 o, V3 d, const SortBox &B) {
+    const float oo[3] = {o.x, o.y, o.z};
+    uint32_t c[3];
+    for (int a = 0; a < 3; a++) {
+        float f = (oo[a] - B.lo[a]) * B.scale[a];
+        f = fminf(fmaxf(f, 0.0f), 7.0f);
+        c[a] = (uint32_t)f;
+    }
+    const uint32_t oct = (d.x < 0.0f ? 1u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 4u : 0u);
+    return ((spread3(c[0]) | (spread3(c[1]) << 1) | (spread3(c[2]) << 2)) << 3) | oct;
+}
+
+// Last k in [0, n) with key[k] <= v (key ascending, key[0] = 0).
+__device__ __forceinline__ uint32_t search_le(const uint32_t *key, uint32_t n, uint32_t v) {
+    uint32_t lo = 0;
+    for (uint32_t step = kMaxWaveJobs / 2; step > 0; step >>= 1)
+        if (lo + step < n && key[lo + step] <= v) lo += step;
+    return lo;
+}
+
+constexpr int kGridBlock = 256;
+
+// Path p of the wave is, within its job, sample sample0 + local / npix of pixel local % npix.
+__global__ __launch_bounds__(kGridBlock) void k_pt_camera(const PtScene S, const PtWaveTable *T, const Queues Q) {
+    __shared__ uint32_t base[kMaxWaveJobs];
+    const uint32_t njobs = T->njobs, paths = T->paths;
+    for (uint32_t k = threadIdx.x; k < njobs; k += blockDim.x) base[k] = T->jobs[k].path_base;
+    __syncthreads();
+    if (blockIdx.x == 0 && threadIdx.x == 0) Q.ctr[0] = paths;
+    const Queue &A = Q.q[0];
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < paths; p += gridDim.x * blockDim.x) {
+        const PtJob &J = T->jobs[search_le(base, njobs, p)];
+        const uint32_t npix = J.width * J.height;
+        const uint32_t local = p - J.path_base;
+        const uint32_t s_local = local / npix, pixel = local % npix;
+        const uint32_t x = pixel % J.width, y = pixel / J.width;
+        Pcg rng;
+        pcg_seed(rng, J.seed, J.sequence + pixel);
+        pcg_advance(rng, (uint64_t)kDrawsPerSample * (J.sample0 + s_local));
+        V3 o, d;
+        camera_ray(S, J.width, J.height, x, y, rng, o, d);
+        A.ray[2 * (uint64_t)p] = make_float4(o.x, o.y, o.z, 0.0f);
+        A.ray[2 * (uint64_t)p + 1] = make_float4(d.x, d.y, d.z, FLT_MAX);
+        A.beta[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+        A.rad[p] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(p));
+        A.rng[p] = make_uint4((uint32_t)rng.state, (uint32_t)(rng.state >> 32), (uint32_t)rng.inc,
+                              (uint32_t)(rng.inc >> 32));
+    }
+}
+
+// Appends are reserved once per block iteration (one atomic per queue for kShadeBlock items):
+// a single counter address serialises its atomics, so per-wave reservations would dominate.
+constexpr int kShadeBlock = 512;
+constexpr int kShadeWaves = kShadeBlock / 64;
+
+__global__ __launch_bounds__(kShadeBlock) void k_pt_shade(const PtScene S, uint32_t depth, const Queues Q,
+                                                          uint32_t cur, uint32_t nxt, int keys, SortBox B) {
+    __shared__ uint32_t s_off[2][2][kShadeWaves]; // [buffer][continuation, shadow][wave] offsets
+    __shared__ uint32_t s_base[2][2];
     const uint32_t count = Q.ctr[cur];
-    const uint32_t nxt = cur ^ 1u;
-    const uint32_t npix = S.width * S.height;
-    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const uint64_t lt = (1ull << lane) - 1ull;
     const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < count; base += stride) {
-        const uint32_t i = base + lane;
+    const Queue A = Q.q[cur], N = Q.q[nxt];
+    uint32_t buf = 0;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += stride, buf ^= 1u) {
+        const uint32_t i = base + threadIdx.x;
+        const bool valid = i < count;
         Bounce b;
         b.shadow = false;
         b.cont = false;
         uint32_t path = 0;
-        if (i < count) {
-            const uint32_t q = order ? order[i] : i;
-            path = Q.qp[cur][q];
-            const float4 r0 = Q.qr[cur][2 * (uint64_t)q], r1 = Q.qr[cur][2 * (uint64_t)q + 1];
-            const uint2 h0 = Q.hits[3 * (uint64_t)q], h1 = Q.hits[3 * (uint64_t)q + 1];
-            Pcg rng{Q.rng[path], pixel_inc(S, path % npix)};
-            const float4 bp = Q.beta[path];
-            const float4 lp = Q.L[path];
-            V3 beta = v3(bp.x, bp.y, bp.z), L = v3(lp.x, lp.y, lp.z);
-            float pdf = bp.w;
-            shade(S, depth, v3(r0.x, r0.y, r0.z), v3(r1.x, r1.y, r1.z), h0.x, h0.y, __uint_as_float(h1.x),
+        V3 beta = v3(0.0f, 0.0f, 0.0f), L = v3(0.0f, 0.0f, 0.0f);
+        float pdf = 0.0f;
+        Pcg rng{0u, 0u};
+        if (valid) {
+            const float4 r0 = A.ray[2 * (uint64_t)i], r1 = A.ray[2 * (uint64_t)i + 1];
+            const float4 bp = A.beta[i], lp = A.rad[i];
+            const uint4 rs = A.rng[i];
+            const uint2 h0 = Q.hits[3 * (uint64_t)i], h1 = Q.hits[3 * (uint64_t)i + 1], h2 = Q.hits[3 * (uint64_t)i + 2];
+            path = __float_as_uint(lp.w);
+            rng = Pcg{(uint64_t)rs.x | ((uint64_t)rs.y << 32), (uint64_t)rs.z | ((uint64_t)rs.w << 32)};
+            beta = v3(bp.x, bp.y, bp.z);
+            pdf = bp.w;
+            L = v3(lp.x, lp.y, lp.z);
+            shade(S, depth, v3(r0.x, r0.y, r0.z), v3(r1.x, r1.y, r1.z), h0.x, h2.y, __uint_as_float(h1.x),
                   __uint_as_float(h1.y), rng, beta, pdf, L, b);
-            Q.rng[path] = rng.state;
-            Q.beta[path] = make_float4(beta.x, beta.y, beta.z, pdf);
-            Q.L[path] = make_float4(L.x, L.y, L.z, 0.0f);
         }
-        const uint64_t ms = __ballot(b.shadow), mc = __ballot(b.cont);
-        uint32_t sbase = 0, cbase = 0;
+        // block-wide reservation of the continuation and shadow slots (double-buffered offsets,
+        // so one barrier pair per iteration suffices)
+        const uint64_t mc = __ballot(b.cont), ms = __ballot(b.shadow);
         if (lane == 0) {
-            if (ms) sbase = atomicAdd(Q.ctr + 2, (uint32_t)__popcll(ms));
-            if (mc) cbase = atomicAdd(Q.ctr + nxt, (uint32_t)__popcll(mc));
+            s_off[buf][0][wave] = (uint32_t)__popcll(mc);
+            s_off[buf][1][wave] = (uint32_t)__popcll(ms);
         }
-        sbase = __shfl(sbase, 0, 64);
-        cbase = __shfl(cbase, 0, 64);
-        if (b.shadow) {
-            const uint64_t k = sbase + (uint32_t)__popcll(ms & lt);
-            Q.sr[2 * k] = make_float4(b.so.x, b.so.y, b.so.z, 0.0f);
-            Q.sr[2 * k + 1] = make_float4(b.wi.x, b.wi.y, b.wi.z, b.d_light);
-            Q.sp[k] = path;
-            Q.sc[k] = make_float4(b.contrib.x, b.contrib.y, b.contrib.z, 0.0f);
+        __syncthreads();
+        if (threadIdx.x < 2) {
+            uint32_t run = 0;
+            for (int w = 0; w < kShadeWaves; w++) {
+                const uint32_t c = s_off[buf][threadIdx.x][w];
+                s_off[buf][threadIdx.x][w] = run;
+                run += c;
+            }
+            s_base[buf][threadIdx.x] = run ? atomicAdd(Q.ctr + (threadIdx.x == 0 ? nxt : 3u), run) : 0u;
         }
+        __syncthreads();
+        // continuation first: the shadow entry names its slot
+        const uint32_t k = s_base[buf][0] + s_off[buf][0][wave] + (uint32_t)__popcll(mc & lt);
         if (b.cont) {
-            const uint64_t k = cbase + (uint32_t)__popcll(mc & lt);
-            Q.qr[nxt][2 * k] = make_float4(b.no.x, b.no.y, b.no.z, 0.0f);
-            Q.qr[nxt][2 * k + 1] = make_float4(b.nd.x, b.nd.y, b.nd.z, FLT_MAX);
-            Q.qp[nxt][k] = path;
+            N.ray[2 * (uint64_t)k] = make_float4(b.no.x, b.no.y, b.no.z, 0.0f);
+            N.ray[2 * (uint64_t)k + 1] = make_float4(b.nd.x, b.nd.y, b.nd.z, FLT_MAX);
+            N.beta[k] = make_float4(beta.x, beta.y, beta.z, pdf);
+            N.rad[k] = make_float4(L.x, L.y, L.z, __uint_as_float(path));
+            N.rng[k] = make_uint4((uint32_t)rng.state, (uint32_t)(rng.state >> 32), (uint32_t)rng.inc,
+                                  (uint32_t)(rng.inc >> 32));
+            if (keys) N.key[k] = (uint16_t)bucket_of(b.no, b.nd, B);
+        } else if (valid) {
+            Q.Lf[path] = make_float4(L.x, L.y, L.z, 0.0f);
+        }
+        if (b.shadow) {
+            const uint64_t s = s_base[buf][1] + s_off[buf][1][wave] + (uint32_t)__popcll(ms & lt);
+            Q.sr[2 * s] = make_float4(b.so.x, b.so.y, b.so.z, 0.0f);
+            Q.sr[2 * s + 1] = make_float4(b.wi.x, b.wi.y, b.wi.z, b.d_light);
+            Q.sc[s] = make_float4(b.contrib.x, b.contrib.y, b.contrib.z, __uint_as_float(b.cont ? k : (path | kEnded)));
         }
     }
 }
 
-__global__ void k_pt_resolve(const Queues Q) {
-    const uint32_t count = Q.ctr[2];
+// Unoccluded light samples add their contribution to the continuation entry or, for a path
+// that ended this bounce, to its final radiance (one shadow ray per path per bounce: no races).
+__global__ void k_pt_resolve(const Queues Q, uint32_t nxt) {
+    const uint32_t count = Q.ctr[3];
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
         if (Q.occ[i]) continue;
-        const uint32_t path = Q.sp[i];
         const float4 c = Q.sc[i];
-        float4 l = Q.L[path];
+        const uint32_t t = __float_as_uint(c.w);
+        float4 *dst = (t & kEnded) ? Q.Lf + (t & ~kEnded) : Q.q[nxt].rad + t;
+        float4 l = *dst;
         l.x = l.x + c.x;
         l.y = l.y + c.y;
         l.z = l.z + c.z;
-        Q.L[path] = l;
+        *dst = l;
     }
 }
 
 // Queue lengths: clear the next queue and the shadow queue; tally the rays traced.
-__global__ void k_pt_counters(const Queues Q, uint32_t cur, unsigned long long *counts, int phase) {
+__global__ void k_pt_counters(const Queues Q, uint32_t cur, uint32_t nxt, unsigned long long *counts, int phase) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     if (phase == 0) {
         counts[0] += Q.ctr[cur];
-        Q.ctr[cur ^ 1u] = 0u;
-        Q.ctr[2] = 0u;
+        Q.ctr[nxt] = 0u;
+        Q.ctr[3] = 0u;
     } else {
-        counts[1] += Q.ctr[2];
+        counts[1] += Q.ctr[3];
     }
 }
 
-__global__ void k_pt_accumulate(const float4 *L, float4 *acc, uint32_t npix, uint32_t chunk_spp) {
-    const uint32_t pixel = blockIdx.x * blockDim.x + threadIdx.x;
-    if (pixel >= npix) return;
-    float4 a = acc[pixel];
-    for (uint32_t s = 0; s < chunk_spp; s++) {
-        const float4 l = L[(uint64_t)s * npix + pixel];
-        if (has_nan(v3(l.x, l.y, l.z))) continue; // a NaN path adds nothing
-        a.x = a.x + l.x;
-        a.y = a.y + l.y;
-        a.z = a.z + l.z;
+// Adds each job's samples of this wave to its accumulator in sample order (the order of the
+// megakernel's per-pixel sum); the job's last wave writes the mean.
+__global__ __launch_bounds__(kGridBlock) void k_pt_accumulate(const PtWaveTable *T, const float4 *Lf) {
+    __shared__ uint32_t base[kMaxWaveJobs];
+    const uint32_t njobs = T->njobs, pixels = T->pixels;
+    for (uint32_t k = threadIdx.x; k < njobs; k += blockDim.x) base[k] = T->jobs[k].pix_base;
+    __syncthreads();
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < pixels; t += gridDim.x * blockDim.x) {
+        const PtJob &J = T->jobs[search_le(base, njobs, t)];
+        const uint32_t npix = J.width * J.height;
+        const uint32_t pixel = t - J.pix_base;
+        float4 a = J.first ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : J.acc[pixel];
+        for (uint32_t s = 0; s < J.nsamples; s++) {
+            const float4 l = Lf[J.path_base + (uint64_t)s * npix + pixel];
+            if (has_nan(l.x, l.y, l.z)) continue; // a NaN path adds nothing
+            a.x = a.x + l.x;
+            a.y = a.y + l.y;
+            a.z = a.z + l.z;
+        }
+        if (J.finish_spp) {
+            const float fs = (float)J.finish_spp;
+            J.out[pixel] = make_float4(a.x / fs, a.y / fs, a.z / fs, 1.0f);
+        } else {
+            J.acc[pixel] = a;
+        }
     }
-    acc[pixel] = a;
-}
-
-__global__ void k_pt_finish(float4 *acc, uint32_t npix, uint32_t spp) {
-    const uint32_t pixel = blockIdx.x * blockDim.x + threadIdx.x;
-    if (pixel >= npix) return;
-    const float4 a = acc[pixel];
-    const float fs = (float)spp;
-    acc[pixel] = make_float4(a.x / fs, a.y / fs, a.z / fs, 1.0f);
-}
-
-__global__ void k_fill_f4(float4 *p, uint32_t n, float4 v) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
 }
 
 __global__ void k_fill_u64(unsigned long long *p, uint32_t n) {
@@ -476,39 +546,15 @@ __global__ void k_fill_u32(uint32_t *p, uint32_t n) {
 }
 
 // ---- bucket sort of the continuation queue (queue length read on the GPU) ----
-struct SortBox {
-    float lo[3], scale[3]; // cell = (o - lo) * scale, 8 cells per axis
-};
-
-__device__ __forceinline__ uint32_t spread3(uint32_t v) { // 3 bits -> positions 0, 3, 6
-    return (v & 1u) | ((v & 2u) << 2) | ((v & 4u) << 4);
-}
-
-__device__ __forceinline__ uint32_t bucket_of(const float4 r0, const float4 r1, const SortBox &B) {
-    uint32_t c[3];
-    const float o[3] = {r0.x, r0.y, r0.z};
-    for (int a = 0; a < 3; a++) {
-        float f = (o[a] - B.lo[a]) * B.scale[a];
-        f = fminf(fmaxf(f, 0.0f), 7.0f);
-        c[a] = (uint32_t)f;
-    }
-    const uint32_t oct = (r1.x < 0.0f ? 1u : 0u) | (r1.y < 0.0f ? 2u : 0u) | (r1.z < 0.0f ? 4u : 0u);
-    return ((spread3(c[0]) | (spread3(c[1]) << 1) | (spread3(c[2]) << 2)) << 3) | oct;
-}
-
 // Block b takes queue entries [b * seg, (b + 1) * seg), seg = ceil(count / gridDim.x).
-__global__ __launch_bounds__(256) void k_pt_hist(const Queues Q, uint32_t nxt, SortBox B) {
+__global__ __launch_bounds__(256) void k_pt_hist(const Queues Q, uint32_t nxt) {
     __shared__ uint32_t h[kBins];
     for (uint32_t k = threadIdx.x; k < kBins; k += blockDim.x) h[k] = 0u;
     __syncthreads();
     const uint32_t count = Q.ctr[nxt];
     const uint32_t seg = (count + gridDim.x - 1u) / gridDim.x;
     const uint32_t begin = min(blockIdx.x * seg, count), end = min(begin + seg, count);
-    for (uint32_t i = begin + threadIdx.x; i < end; i += blockDim.x) {
-        const uint32_t key = bucket_of(Q.qr[nxt][2 * (uint64_t)i], Q.qr[nxt][2 * (uint64_t)i + 1], B);
-        Q.keys[i] = (uint16_t)key;
-        atomicAdd(&h[key], 1u);
-    }
+    for (uint32_t i = begin + threadIdx.x; i < end; i += blockDim.x) atomicAdd(&h[Q.q[nxt].key[i]], 1u);
     __syncthreads();
     for (uint32_t k = threadIdx.x; k < kBins; k += blockDim.x) Q.hist[(uint64_t)k * gridDim.x + blockIdx.x] = h[k];
 }
@@ -563,24 +609,33 @@ __global__ __launch_bounds__(1024) void k_pt_scan_total(const Queues Q) {
     }
 }
 
-__global__ __launch_bounds__(256) void k_pt_scatter(const Queues Q, uint32_t nxt) {
+// Moves each entry of queue `from` to its bucket position in queue `to`.
+__global__ __launch_bounds__(256) void k_pt_scatter(const Queues Q, uint32_t from, uint32_t to) {
     __shared__ uint32_t pos[kBins];
     for (uint32_t k = threadIdx.x; k < kBins; k += blockDim.x)
         pos[k] = Q.bin_base[k] + Q.hist[(uint64_t)k * gridDim.x + blockIdx.x];
     __syncthreads();
-    const uint32_t count = Q.ctr[nxt];
+    const uint32_t count = Q.ctr[from];
+    if (blockIdx.x == 0 && threadIdx.x == 0) Q.ctr[to] = count;
     const uint32_t seg = (count + gridDim.x - 1u) / gridDim.x;
     const uint32_t begin = min(blockIdx.x * seg, count), end = min(begin + seg, count);
+    const Queue F = Q.q[from], T = Q.q[to];
     for (uint32_t i = begin + threadIdx.x; i < end; i += blockDim.x) {
-        const uint32_t at = atomicAdd(&pos[Q.keys[i]], 1u);
-        Q.order[at] = i;
+        const uint32_t at = atomicAdd(&pos[F.key[i]], 1u);
+        T.ray[2 * (uint64_t)at] = F.ray[2 * (uint64_t)i];
+        T.ray[2 * (uint64_t)at + 1] = F.ray[2 * (uint64_t)i + 1];
+        T.beta[at] = F.beta[i];
+        T.rad[at] = F.rad[i];
+        T.rng[at] = F.rng[i];
     }
 }
 
 // ---------------------------------------------------------------------------------------------
 // host side
+constexpr int kStaging = 8; // pinned wave-table staging slots (ring)
+
 struct Workspace {
-    size_t cap = 0;         // paths per chunk
+    size_t cap = 0;         // paths per wave
     void *mem = nullptr;
     Queues Q{};
     uint2 *spill = nullptr;
@@ -588,6 +643,10 @@ struct Workspace {
     PtInstance *inst = nullptr;
     size_t inst_cap = 0;
     std::vector<PtInstance> inst_host; // what the device table holds
+    PtWaveTable *table = nullptr;       // the device wave table the launches read
+    PtWaveTable *staging = nullptr;     // pinned host ring of kStaging tables
+    hipEvent_t staged[kStaging] = {};   // copy out of each staging slot done
+    int next_stage = 0;
     unsigned long long *counts = nullptr; // [closest rays, shadow rays] when the caller passes none
     uint64_t generation = 0;
     hipGraphExec_t exec = nullptr;
@@ -604,9 +663,12 @@ hipError_t workspace_reserve(Workspace &W, size_t cap) {
         if (e != hipSuccess) return e;
         W.mem = nullptr;
     }
-    const size_t sizes[] = {
-        8 * cap, 16 * cap, 16 * cap, 32 * cap, 32 * cap, 4 * cap, 4 * cap, 24 * cap, 32 * cap, 4 * cap, 16 * cap,
-        cap,     2 * cap,  4 * cap,  4ull * kBins * kSortBlocks, 4ull * kBins, 4ull * kBins, 64};
+    // per queue: ray 32 + beta 16 + rad 16 + rng 16 + key 2 bytes per entry
+    const size_t sizes[] = {32 * cap, 16 * cap, 16 * cap, 16 * cap, 2 * cap,   // q[0]
+                            32 * cap, 16 * cap, 16 * cap, 16 * cap, 2 * cap,   // q[1]
+                            32 * cap, 16 * cap, 16 * cap, 16 * cap, 2 * cap,   // q[2]
+                            24 * cap, 32 * cap, 16 * cap, cap, 16 * cap,       // hits, sr, sc, occ, Lf
+                            4ull * kBins * kSortBlocks, 4ull * kBins, 4ull * kBins, 4 * kCtrWords};
     size_t total = 0;
     for (size_t s : sizes) total += round_up(s);
     hipError_t e = hipMalloc(&W.mem, total);
@@ -621,8 +683,10 @@ hipError_t workspace_reserve(Workspace &W, size_t cap) {
         dst = reinterpret_cast<std::remove_reference_t<decltype(dst)>>(p);
         p += round_up(sizes[k++]);
     };
-    take(W.Q.rng); take(W.Q.beta); take(W.Q.L); take(W.Q.qr[0]); take(W.Q.qr[1]); take(W.Q.qp[0]); take(W.Q.qp[1]);
-    take(W.Q.hits); take(W.Q.sr); take(W.Q.sp); take(W.Q.sc); take(W.Q.occ); take(W.Q.keys); take(W.Q.order);
+    for (int q = 0; q < 3; q++) {
+        take(W.Q.q[q].ray); take(W.Q.q[q].beta); take(W.Q.q[q].rad); take(W.Q.q[q].rng); take(W.Q.q[q].key);
+    }
+    take(W.Q.hits); take(W.Q.sr); take(W.Q.sc); take(W.Q.occ); take(W.Q.Lf);
     take(W.Q.hist); take(W.Q.bin_total); take(W.Q.bin_base); take(W.Q.ctr);
     W.cap = cap;
     W.generation++;
@@ -678,10 +742,14 @@ struct PtState {
 
 static void free_workspace(Workspace *W) {
     drop_graph(*W);
+    for (hipEvent_t &e : W->staged)
+        if (e) hipEventDestroy(e);
     if (W->mem) hipFree(W->mem);
     if (W->spill) hipFree(W->spill);
     if (W->inst) hipFree(W->inst);
     if (W->counts) hipFree(W->counts);
+    if (W->table) hipFree(W->table);
+    if (W->staging) hipHostFree(W->staging);
     delete W;
 }
 
@@ -703,84 +771,96 @@ void pt_release_stream(Device *dev, hipStream_t s) {
     st->ws.erase(it);
 }
 
-int pt_render(Device *dev, hipStream_t s, const Geometry &G, bool two, uint32_t stack_need, const float bounds[6],
-              const PtScene &scene_in, const std::vector<PtInstance> &inst, const PtOptions &opt, float4 *image,
-              unsigned long long *counts, std::string &err) {
-#define PT_TRY(x)                                  \
-    do {                                           \
-        hipError_t e_ = (x);                       \
-        if (e_ != hipSuccess) {                    \
+#define PT_TRY(x)                                                 \
+    do {                                                          \
+        hipError_t e_ = (x);                                      \
+        if (e_ != hipSuccess) {                                   \
             err = std::string(#x) + ": " + hipGetErrorString(e_); \
-            return -1;                             \
-        }                                          \
+            return -1;                                            \
+        }                                                         \
     } while (0)
-    std::lock_guard<std::mutex> lk(dev->mu);
+
+// The stream's workspace with the instance table uploaded (re-uploaded only when it changes,
+// after the stream has drained the previous copy). Caller holds dev->mu.
+static int prepare(Device *dev, hipStream_t s, const std::vector<PtInstance> &inst, Workspace *&out, std::string &err) {
     if (!dev->pt) dev->pt = new PtState();
     PtState &state = *static_cast<PtState *>(dev->pt);
     Workspace *&slot = state.ws[s];
     if (!slot) slot = new Workspace();
     Workspace &W = *slot;
-    const uint32_t npix = scene_in.width * scene_in.height;
-    // instance table: re-uploaded only when it changes (after the stream has drained the last copy)
-    {
-        void *p = W.inst;
-        size_t have = W.inst_cap;
-        uint64_t gen = W.generation;
-        PT_TRY(grow(p, have, std::max<size_t>(1, inst.size()) * sizeof(PtInstance), gen));
-        if (p != W.inst) W.inst_host.clear();
-        W.inst = static_cast<PtInstance *>(p);
-        W.inst_cap = have;
-        W.generation = gen;
-        if (W.inst_host.size() != inst.size() ||
-            std::memcmp(W.inst_host.data(), inst.data(), inst.size() * sizeof(PtInstance)) != 0) {
-            PT_TRY(hipStreamSynchronize(s));
-            W.inst_host = inst;
-            PT_TRY(hipMemcpy(W.inst, W.inst_host.data(), inst.size() * sizeof(PtInstance), hipMemcpyHostToDevice));
-        }
+    out = slot;
+    void *p = W.inst;
+    size_t have = W.inst_cap;
+    uint64_t gen = W.generation;
+    PT_TRY(grow(p, have, std::max<size_t>(1, inst.size()) * sizeof(PtInstance), gen));
+    if (p != W.inst) W.inst_host.clear();
+    W.inst = static_cast<PtInstance *>(p);
+    W.inst_cap = have;
+    W.generation = gen;
+    if (W.inst_host.size() != inst.size() ||
+        std::memcmp(W.inst_host.data(), inst.data(), inst.size() * sizeof(PtInstance)) != 0) {
+        PT_TRY(hipStreamSynchronize(s));
+        W.inst_host = inst;
+        PT_TRY(hipMemcpy(W.inst, W.inst_host.data(), inst.size() * sizeof(PtInstance), hipMemcpyHostToDevice));
     }
     if (!W.counts) PT_TRY(hipMalloc(&W.counts, 2 * sizeof(unsigned long long)));
-    if (!counts) counts = W.counts;
-    PtScene S = scene_in;
-    S.inst = W.inst;
-
-    const TraceConfig cc = dev->cfg[0], ca = dev->cfg[1];
-    if (!opt.wavefront) {
-        const int bpc = two ? mega_blocks_per_cu<true>() : mega_blocks_per_cu<false>();
-        const int grid = bpc * dev->cu_count;
-        const uint32_t spill_depth = stack_need > (uint32_t)kMegaLds ? stack_need - kMegaLds : 0u;
-        void *sp = W.spill;
-        size_t have = W.spill_bytes;
-        PT_TRY(grow(sp, have, std::max<size_t>(16, (size_t)grid * kMegaBlock * spill_depth * sizeof(uint2)), W.generation));
-        W.spill = static_cast<uint2 *>(sp);
-        W.spill_bytes = have;
-        PT_TRY(workspace_reserve(W, std::max<size_t>(W.cap, 1)));
-        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, W.Q.ctr, 16u);
-        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, counts, 2u);
-        if (two)
-            hipLaunchKernelGGL((k_pt_mega<true, kMegaBlock, kMegaLds>), dim3(grid), dim3(kMegaBlock), 0, s, S, G, image,
-                               W.Q.ctr + 3, W.spill, spill_depth, dev->error, counts);
-        else
-            hipLaunchKernelGGL((k_pt_mega<false, kMegaBlock, kMegaLds>), dim3(grid), dim3(kMegaBlock), 0, s, S, G, image,
-                               W.Q.ctr + 3, W.spill, spill_depth, dev->error, counts);
-        PT_TRY(hipGetLastError());
-        return 0;
+    if (!W.table) {
+        PT_TRY(hipMalloc(&W.table, sizeof(PtWaveTable)));
+        PT_TRY(hipHostMalloc(reinterpret_cast<void **>(&W.staging), kStaging * sizeof(PtWaveTable), hipHostMallocDefault));
+        for (hipEvent_t &e : W.staged) PT_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     }
+    return 0;
+}
 
-    // wavefront
-    uint32_t chunk = opt.chunk_spp;
-    if (chunk == 0) chunk = (uint32_t)std::max<uint64_t>(1, (16ull << 20) / std::max<uint32_t>(1, npix));
-    chunk = std::min(chunk, S.spp);
-    const size_t cap = (size_t)npix * chunk;
-    if (cap >= 0xFFFFFFFFull) {
-        err = "a wavefront chunk must hold fewer than 2^32 paths";
+// Enqueues one wave: the wave table is staged through the pinned ring and copied to the device
+// table in stream order, then the wave's launches run (from the cached hipGraph when `graph`).
+static int run_wave(Device *dev, hipStream_t s, Workspace &W, const PtBinding &Bd, const PtScene &S,
+                    const PtJob *jobs, uint32_t njobs, size_t cap, const PtOptions &opt, unsigned long long *counts,
+                    std::string &err) {
+    if (njobs == 0) return 0;
+    if (njobs > kMaxWaveJobs) {
+        err = "too many jobs in one wave";
         return -2;
     }
+    if (cap >= (size_t)kEnded) {
+        err = "a wavefront wave must hold fewer than 2^31 paths";
+        return -2;
+    }
+    // table: path and pixel bases
+    {
+        const int slot = W.next_stage;
+        W.next_stage = (W.next_stage + 1) % kStaging;
+        PT_TRY(hipEventSynchronize(W.staged[slot])); // that slot's previous copy has been consumed
+        PtWaveTable &T = W.staging[slot];
+        uint64_t paths = 0, pixels = 0;
+        for (uint32_t j = 0; j < njobs; j++) {
+            T.jobs[j] = jobs[j];
+            T.jobs[j].path_base = (uint32_t)paths;
+            T.jobs[j].pix_base = (uint32_t)pixels;
+            const uint64_t npix = (uint64_t)jobs[j].width * jobs[j].height;
+            paths += npix * jobs[j].nsamples;
+            pixels += npix;
+        }
+        if (paths > cap) {
+            err = "wave exceeds its path capacity";
+            return -2;
+        }
+        T.njobs = njobs;
+        T.paths = (uint32_t)paths;
+        T.pixels = (uint32_t)pixels;
+        T._pad = 0;
+        const size_t bytes = offsetof(PtWaveTable, jobs) + njobs * sizeof(PtJob);
+        PT_TRY(hipMemcpyAsync(W.table, &T, bytes, hipMemcpyHostToDevice, s));
+        PT_TRY(hipEventRecord(W.staged[slot], s));
+    }
     PT_TRY(workspace_reserve(W, cap));
+    const TraceConfig cc = dev->cfg[0], ca = dev->cfg[1];
+    const bool two = Bd.two;
     const int bpc_c = cc.blocks_per_cu > 0 ? cc.blocks_per_cu : trace_blocks_per_cu(false, two, false, cc.block, cc.slds);
     const int bpc_a = ca.blocks_per_cu > 0 ? ca.blocks_per_cu : trace_blocks_per_cu(true, two, false, ca.block, ca.slds);
     const int grid_c = bpc_c * dev->cu_count, grid_a = bpc_a * dev->cu_count;
-    const uint32_t sd_c = stack_need > (uint32_t)cc.slds ? stack_need - cc.slds : 0u;
-    const uint32_t sd_a = stack_need > (uint32_t)ca.slds ? stack_need - ca.slds : 0u;
+    const uint32_t sd_c = Bd.stack_need > (uint32_t)cc.slds ? Bd.stack_need - cc.slds : 0u;
+    const uint32_t sd_a = Bd.stack_need > (uint32_t)ca.slds ? Bd.stack_need - ca.slds : 0u;
     {
         const size_t need = std::max<size_t>({16, (size_t)grid_c * cc.block * sd_c * sizeof(uint2),
                                               (size_t)grid_a * ca.block * sd_a * sizeof(uint2)});
@@ -790,21 +870,21 @@ int pt_render(Device *dev, hipStream_t s, const Geometry &G, bool two, uint32_t 
         W.spill = static_cast<uint2 *>(sp);
         W.spill_bytes = have;
     }
-    SortBox B;
+    SortBox SB;
     for (int a = 0; a < 3; a++) {
-        const float ext = bounds[3 + a] - bounds[a];
-        B.lo[a] = bounds[a];
-        B.scale[a] = ext > 0.0f ? 8.0f / ext : 0.0f;
+        const float ext = Bd.bounds[3 + a] - Bd.bounds[a];
+        SB.lo[a] = Bd.bounds[a];
+        SB.scale[a] = ext > 0.0f ? 8.0f / ext : 0.0f;
     }
-    // graph cache key: everything the captured launches depend on
+    // graph cache key: everything the captured launches depend on (the wave table is data)
     std::vector<uint8_t> key;
     auto put = [&](const void *p, size_t n) {
         const uint8_t *b = static_cast<const uint8_t *>(p);
         key.insert(key.end(), b, b + n);
     };
-    put(&G, sizeof(G)); put(&S, sizeof(S)); put(&opt, sizeof(opt)); put(&image, sizeof(image)); put(&counts, sizeof(counts));
-    put(&W.generation, sizeof(W.generation)); put(&cc, sizeof(cc)); put(&ca, sizeof(ca)); put(&B, sizeof(B));
-    put(&chunk, sizeof(chunk)); put(&two, sizeof(two)); put(&stack_need, sizeof(stack_need));
+    put(&Bd.G, sizeof(Bd.G)); put(&S, sizeof(S)); put(&opt, sizeof(opt)); put(&counts, sizeof(counts));
+    put(&W.generation, sizeof(W.generation)); put(&cc, sizeof(cc)); put(&ca, sizeof(ca)); put(&SB, sizeof(SB));
+    put(&cap, sizeof(cap)); put(&two, sizeof(two)); put(&Bd.stack_need, sizeof(Bd.stack_need));
     put(&dev->error, sizeof(dev->error));
     if (opt.graph && W.exec && key == W.key) {
         PT_TRY(hipGraphLaunch(W.exec, s));
@@ -814,62 +894,61 @@ int pt_render(Device *dev, hipStream_t s, const Geometry &G, bool two, uint32_t 
         drop_graph(W);
         PT_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     }
+    const int grid_shade_blocks = std::max(1, dev->cu_count * 4);
+    const int grid_small = std::max(1, dev->cu_count * 8);
     auto enqueue = [&]() -> hipError_t {
         const Queues &Q = W.Q;
-        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, counts, 2u);
-        hipLaunchKernelGGL(k_fill_f4, dim3((npix + 255) / 256), dim3(256), 0, s, image, npix, make_float4(0.f, 0.f, 0.f, 0.f));
-        for (uint32_t base = 0; base < S.spp; base += chunk) {
-            const uint32_t n_s = std::min(chunk, S.spp - base);
-            const uint64_t paths = (uint64_t)npix * n_s;
-            hipLaunchKernelGGL(k_pt_camera, dim3((unsigned)((paths + 255) / 256)), dim3(256), 0, s, S, base, n_s, Q.qr[0],
-                               Q.qp[0], Q.rng, Q.beta, Q.L, Q.ctr + 0);
-            uint32_t cur = 0;
-            bool sorted = false;
-            for (uint32_t depth = 0; depth < S.max_depth; depth++) {
-                TraceParams P;
-                std::memset(&P, 0, sizeof(P));
-                P.batches.n = 1;
-                P.batches.b[0] = RayBatch{Q.qr[cur], Q.hits, 0u, 0xFFu};
-                P.count = (uint32_t)cap;
-                P.count_ptr = Q.ctr + cur;
-                P.ray_counter = Q.ctr + 4;
-                P.order = sorted ? Q.order : nullptr;
-                P.geo = G;
-                P.spill = W.spill;
-                P.spill_depth = sd_c;
-                P.steps = cc.steps;
-                P.stats = nullptr;
-                P.error = dev->error;
-                hipError_t e = trace_launch(P, s, false, two, false, grid_c, cc.block, cc.slds);
-                if (e != hipSuccess) return e;
-                hipLaunchKernelGGL(k_pt_counters, dim3(1), dim3(64), 0, s, Q, cur, counts, 0);
-                hipLaunchKernelGGL(k_pt_shade, dim3(std::max(1, dev->cu_count * 8)), dim3(256), 0, s, S, depth, Q, cur,
-                                   sorted ? (const uint32_t *)Q.order : nullptr);
-                TraceParams A = P;
-                A.batches.b[0] = RayBatch{Q.sr, Q.occ, 0u, 0xFFu};
-                A.count_ptr = Q.ctr + 2;
-                A.ray_counter = Q.ctr + 5;
-                A.order = nullptr;
-                A.spill_depth = sd_a;
-                A.steps = ca.steps;
-                e = trace_launch(A, s, true, two, false, grid_a, ca.block, ca.slds);
-                if (e != hipSuccess) return e;
-                hipLaunchKernelGGL(k_pt_counters, dim3(1), dim3(64), 0, s, Q, cur, counts, 1);
-                hipLaunchKernelGGL(k_pt_resolve, dim3(std::max(1, dev->cu_count * 8)), dim3(256), 0, s, Q);
-                const uint32_t nxt = cur ^ 1u;
-                sorted = false;
-                if (opt.sort && depth + 1u < S.max_depth) {
-                    hipLaunchKernelGGL(k_pt_hist, dim3(kSortBlocks), dim3(256), 0, s, Q, nxt, B);
-                    hipLaunchKernelGGL(k_pt_scan_bins, dim3(kBins), dim3(256), 0, s, Q);
-                    hipLaunchKernelGGL(k_pt_scan_total, dim3(1), dim3(1024), 0, s, Q);
-                    hipLaunchKernelGGL(k_pt_scatter, dim3(kSortBlocks), dim3(256), 0, s, Q, nxt);
-                    sorted = true;
-                }
-                cur = nxt;
+        hipLaunchKernelGGL(k_pt_camera, dim3(grid_small), dim3(kGridBlock), 0, s, S, W.table, Q);
+        uint32_t cur = 0, nxt = 1, spare = 2;
+        for (uint32_t depth = 0; depth < S.max_depth; depth++) {
+            TraceParams P;
+            std::memset(&P, 0, sizeof(P));
+            P.batches.n = 1;
+            P.batches.b[0] = RayBatch{Q.q[cur].ray, Q.hits, 0u, 0xFFu};
+            P.count = (uint32_t)cap;
+            P.count_ptr = Q.ctr + cur;
+            P.ray_counter = Q.ctr + kCtrClosest;
+            P.order = nullptr;
+            P.geo = Bd.G;
+            P.spill = W.spill;
+            P.spill_depth = sd_c;
+            P.steps = cc.steps;
+            P.fetch_chunk = opt.fetch_chunk;
+            P.fetch_segments = opt.fetch_segments;
+            P.stats = nullptr;
+            P.error = dev->error;
+            P.hit_record = 1u;
+            hipError_t e = trace_launch(P, s, false, two, false, grid_c, cc.block, cc.slds);
+            if (e != hipSuccess) return e;
+            hipLaunchKernelGGL(k_pt_counters, dim3(1), dim3(64), 0, s, Q, cur, nxt, counts, 0);
+            const bool sort = opt.sort && depth + 1u < S.max_depth;
+            hipLaunchKernelGGL(k_pt_shade, dim3(grid_shade_blocks), dim3(kShadeBlock), 0, s, S, depth, Q, cur, nxt,
+                               sort ? 1 : 0, SB);
+            TraceParams A = P;
+            A.batches.b[0] = RayBatch{Q.sr, Q.occ, 0u, 0xFFu};
+            A.count_ptr = Q.ctr + 3;
+            A.ray_counter = Q.ctr + kCtrAny;
+            A.spill_depth = sd_a;
+            A.steps = ca.steps;
+            A.hit_record = 0u;
+            e = trace_launch(A, s, true, two, false, grid_a, ca.block, ca.slds);
+            if (e != hipSuccess) return e;
+            hipLaunchKernelGGL(k_pt_counters, dim3(1), dim3(64), 0, s, Q, cur, nxt, counts, 1);
+            hipLaunchKernelGGL(k_pt_resolve, dim3(grid_small), dim3(256), 0, s, Q, nxt);
+            if (sort) {
+                hipLaunchKernelGGL(k_pt_hist, dim3(kSortBlocks), dim3(256), 0, s, Q, nxt);
+                hipLaunchKernelGGL(k_pt_scan_bins, dim3(kBins), dim3(256), 0, s, Q);
+                hipLaunchKernelGGL(k_pt_scan_total, dim3(1), dim3(1024), 0, s, Q);
+                hipLaunchKernelGGL(k_pt_scatter, dim3(kSortBlocks), dim3(256), 0, s, Q, nxt, spare);
+                const uint32_t old = cur;
+                cur = spare;
+                spare = nxt;
+                nxt = old;
+            } else {
+                std::swap(cur, nxt);
             }
-            hipLaunchKernelGGL(k_pt_accumulate, dim3((npix + 255) / 256), dim3(256), 0, s, Q.L, image, npix, n_s);
         }
-        hipLaunchKernelGGL(k_pt_finish, dim3((npix + 255) / 256), dim3(256), 0, s, image, npix, S.spp);
+        hipLaunchKernelGGL(k_pt_accumulate, dim3(grid_small), dim3(kGridBlock), 0, s, W.table, Q.Lf);
         return hipGetLastError();
     };
     hipError_t e = enqueue();
@@ -892,7 +971,85 @@ int pt_render(Device *dev, hipStream_t s, const Geometry &G, bool two, uint32_t 
         return -1;
     }
     return 0;
-#undef PT_TRY
 }
+
+int pt_wave(Device *dev, hipStream_t s, const PtBinding &B, const PtJob *jobs, uint32_t njobs, size_t capacity,
+            const PtOptions &opt, unsigned long long *counts, std::string &err) {
+    std::lock_guard<std::mutex> lk(dev->mu);
+    Workspace *W = nullptr;
+    int r = prepare(dev, s, B.inst, W, err);
+    if (r != 0) return r;
+    PtScene S = B.S;
+    S.inst = W->inst;
+    return run_wave(dev, s, *W, B, S, jobs, njobs, capacity, opt, counts ? counts : W->counts, err);
+}
+
+int pt_render(Device *dev, hipStream_t s, const Geometry &G, bool two, uint32_t stack_need, const float bounds[6],
+              const PtScene &scene_in, const std::vector<PtInstance> &inst, const PtOptions &opt, float4 *image,
+              unsigned long long *counts, std::string &err) {
+    std::lock_guard<std::mutex> lk(dev->mu);
+    Workspace *Wp = nullptr;
+    int r = prepare(dev, s, inst, Wp, err);
+    if (r != 0) return r;
+    Workspace &W = *Wp;
+    if (!counts) counts = W.counts;
+    PtScene S = scene_in;
+    S.inst = W.inst;
+    const uint32_t npix = S.width * S.height;
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, counts, 2u);
+    if (!opt.wavefront) {
+        const int bpc = two ? mega_blocks_per_cu<true>() : mega_blocks_per_cu<false>();
+        const int grid = bpc * dev->cu_count;
+        const uint32_t spill_depth = stack_need > (uint32_t)kMegaLds ? stack_need - kMegaLds : 0u;
+        void *sp = W.spill;
+        size_t have = W.spill_bytes;
+        PT_TRY(grow(sp, have, std::max<size_t>(16, (size_t)grid * kMegaBlock * spill_depth * sizeof(uint2)), W.generation));
+        W.spill = static_cast<uint2 *>(sp);
+        W.spill_bytes = have;
+        PT_TRY(workspace_reserve(W, std::max<size_t>(W.cap, 1)));
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, W.Q.ctr, 16u);
+        if (two)
+            hipLaunchKernelGGL((k_pt_mega<true, kMegaBlock, kMegaLds>), dim3(grid), dim3(kMegaBlock), 0, s, S, G, image,
+                               W.Q.ctr + 4, W.spill, spill_depth, dev->error, counts);
+        else
+            hipLaunchKernelGGL((k_pt_mega<false, kMegaBlock, kMegaLds>), dim3(grid), dim3(kMegaBlock), 0, s, S, G, image,
+                               W.Q.ctr + 4, W.spill, spill_depth, dev->error, counts);
+        PT_TRY(hipGetLastError());
+        return 0;
+    }
+    // wavefront: one job per wave, chunk_spp samples per wave, accumulated in place in the image
+    uint32_t chunk = opt.chunk_spp;
+    if (chunk == 0) chunk = (uint32_t)std::max<uint64_t>(1, (16ull << 20) / std::max<uint32_t>(1, npix));
+    chunk = std::min(chunk, S.spp);
+    const size_t cap = (size_t)npix * chunk;
+    PtBinding Bd;
+    Bd.G = G;
+    Bd.two = two;
+    Bd.stack_need = stack_need;
+    std::memcpy(Bd.bounds, bounds, sizeof(Bd.bounds));
+    for (uint32_t base = 0; base < S.spp; base += chunk) {
+        PtJob J;
+        std::memset(&J, 0, sizeof(J));
+        J.acc = image;
+        J.out = image;
+        J.width = S.width;
+        J.height = S.height;
+        J.sample0 = S.sample_offset + base;
+        J.nsamples = std::min(chunk, S.spp - base);
+        J.first = base == 0 ? 1u : 0u;
+        J.finish_spp = base + J.nsamples == S.spp ? S.spp : 0u;
+        J.seed = S.seed;
+        J.sequence = S.sequence;
+        // the per-render fields do not reach the wave launches (the graph key stays the same)
+        PtScene Sw = S;
+        Sw.width = Sw.height = Sw.spp = Sw.sample_offset = 0;
+        Sw.seed = Sw.sequence = 0;
+        r = run_wave(dev, s, W, Bd, Sw, &J, 1, cap, opt, counts, err);
+        if (r != 0) return r;
+    }
+    return 0;
+}
+
+#undef PT_TRY
 
 } // namespace lchip

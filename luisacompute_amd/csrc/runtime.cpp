@@ -210,14 +210,15 @@ uint64_t lc_hip_device_create(int32_t index) {
     dev->name = prop.gcnArchName;
     DeviceGuard g(dev.get());
     void *p = nullptr;
-    if (hipMalloc(&p, 256) != hipSuccess) {
+    // [0, 512): per-segment ray slot counters; 512: error word; 576: two 64-bit counter-mode stats
+    if (hipMalloc(&p, 1024) != hipSuccess) {
         fail(LC_HIP_ERROR_OUT_OF_MEMORY, "device scratch allocation failed");
         return LC_HIP_INVALID_HANDLE;
     }
-    hipMemset(p, 0, 256);
+    hipMemset(p, 0, 1024);
     dev->ray_counter = static_cast<uint32_t *>(p);
-    dev->error = dev->ray_counter + 16;
-    dev->stats = reinterpret_cast<unsigned long long *>(static_cast<char *>(p) + 128);
+    dev->error = dev->ray_counter + kRayCounterWords;
+    dev->stats = reinterpret_cast<unsigned long long *>(static_cast<char *>(p) + 576);
     dev->cfg[0] = TraceConfig{256, 16, 32, 0};
     dev->cfg[1] = TraceConfig{256, 16, 32, 0};
     return handle(dev.release());
@@ -900,7 +901,8 @@ static int trace_launch_batches(Device *dev, Accel *a, hipStream_t s, const lc_h
         rbt.mask = bt.vis_mask;
         total += bt.ray_count;
     }
-    if (total >= 0xFFFFFFFFull) return fail(LC_HIP_ERROR_UNSUPPORTED, "more than 2^32 - 1 rays in one launch");
+    // the launch's slot counter runs past the ray count by at most one chunk per wave
+    if (total > 0xF0000000ull) return fail(LC_HIP_ERROR_UNSUPPORTED, "more than 0xF0000000 rays in one launch");
     P.batches.n = (uint32_t)nb;
     P.count = (uint32_t)total;
     if (counting && n_nodes) *n_nodes = 0;
@@ -923,6 +925,8 @@ static int trace_launch_batches(Device *dev, Accel *a, hipStream_t s, const lc_h
     P.error = dev->error;
     P.stats = dev->stats;
     P.steps = cfg.steps;
+    P.fetch_chunk = cfg.fetch_chunk;
+    P.fetch_segments = cfg.fetch_segments;
     if (counting) LC_HIP_TRY(hipMemsetAsync(dev->stats, 0, 2 * sizeof(unsigned long long), s));
     const bool sort = cfg.sort_rays && !counting && total > 1 && !P.geo.empty; // per-ray results do not depend on order
     if (sort) LC_HIP_TRY(ensure(dev->sort_scratch, dev->sort_bytes, ray_sort_scratch_bytes(P.count)));
@@ -999,6 +1003,52 @@ int lc_hip_accel_trace_count(uint64_t device, uint64_t stream, uint64_t accel, u
     return trace_batches(device, stream, accel, &b, 1, any != 0, true, nodes_visited, triangles_tested);
 }
 
+// Light and camera of a lc_hip_pt_scene; the light's normal and area are normalize(cross(u, v))
+// and |cross(u, v)| in single precision. False for a degenerate light quad.
+static bool pt_scene_from(const lc_hip_pt_scene &scene, PtScene &S) {
+    std::memset(&S, 0, sizeof(S));
+    for (int k = 0; k < 3; k++) {
+        S.light_pos[k] = scene.light_position[k];
+        S.light_u[k] = scene.light_u[k];
+        S.light_v[k] = scene.light_v[k];
+        S.light_e[k] = scene.light_emission[k];
+        S.cam_o[k] = scene.camera_origin[k];
+        S.cam_r[k] = scene.camera_right[k];
+        S.cam_u[k] = scene.camera_up[k];
+        S.cam_f[k] = scene.camera_forward[k];
+    }
+    const float *u = S.light_u, *v = S.light_v;
+    const float c[3] = {u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0]};
+    const float len = std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+    if (!(len > 0.0f)) return false;
+    for (int k = 0; k < 3; k++) S.light_n[k] = c[k] / len;
+    S.light_area = len;
+    return true;
+}
+
+// Shading table of an accel's instances (caller holds a->mu, instance_count == a->inst.size()).
+static std::vector<PtInstance> pt_table_from(Accel *a, const lc_hip_pt_instance *pt_instances, uint32_t instance_count) {
+    std::vector<PtInstance> table(instance_count);
+    for (uint32_t i = 0; i < instance_count; i++) {
+        const lc_hip_pt_instance &pi = pt_instances[i];
+        const Mesh *m = a->inst[i].mesh;
+        PtInstance &t = table[i];
+        std::memset(&t, 0, sizeof(t));
+        t.tris = (m && m->built) ? m->tris : nullptr; // hits only name records of built meshes
+        std::memcpy(t.fwd, a->inst[i].affine, sizeof(t.fwd));
+        std::memcpy(t.albedo, pi.albedo, sizeof(t.albedo));
+        t.emissive = pi.emissive ? 1u : 0u;
+    }
+    return table;
+}
+
+static bool pt_fetch_from(uint32_t chunk, uint32_t segments, PtOptions &po) {
+    if ((segments > 1 && segments != kFetchSegments) || chunk > 4096 || chunk % 64 != 0) return false;
+    po.fetch_chunk = segments == 0 ? 256u : chunk;
+    po.fetch_segments = segments == 0 ? kFetchSegments : segments;
+    return true;
+}
+
 int lc_hip_pathtrace(uint64_t device, uint64_t stream, uint64_t accel, const lc_hip_pt_instance *pt_instances,
                      uint32_t instance_count, const lc_hip_pt_scene *scene, const lc_hip_pt_options *options,
                      uint64_t image_buffer, size_t image_offset, uint64_t counters_buffer) {
@@ -1027,51 +1077,9 @@ int lc_hip_pathtrace(uint64_t device, uint64_t stream, uint64_t accel, const lc_
     if (!a->built) return fail(LC_HIP_ERROR_NOT_BUILT, "accel is not built");
     if (instance_count != a->inst.size())
         return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "one lc_hip_pt_instance per accel instance");
-    std::vector<PtInstance> table(instance_count);
-    for (uint32_t i = 0; i < instance_count; i++) {
-        const lc_hip_pt_instance &pi = pt_instances[i];
-        Buffer *vb = obj<Buffer>(pi.vertex_buffer);
-        Buffer *tb = obj<Buffer>(pi.triangle_buffer);
-        if (!vb || !tb) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad pathtrace instance buffers");
-        if (pi.vertex_stride < 12 || pi.vertex_stride % 4 != 0 || pi.vertex_offset % 4 != 0 || pi.triangle_offset % 4 != 0)
-            return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "vertex stride/offsets must be 4-byte aligned, stride >= 12");
-        if (pi.vertex_offset + 12 > vb->bytes) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "no vertex in the vertex buffer");
-        const uint64_t nv = (vb->bytes - pi.vertex_offset - 12) / pi.vertex_stride + 1;
-        const Mesh *m = a->inst[i].mesh;
-        const uint64_t nt = m ? m->ntris : 0;
-        if (pi.triangle_offset + nt * 12 > tb->bytes)
-            return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "triangle buffer smaller than the instance's mesh");
-        PtInstance &t = table[i];
-        std::memset(&t, 0, sizeof(t));
-        t.vertices = static_cast<const char *>(vb->ptr) + pi.vertex_offset;
-        t.vstride = pi.vertex_stride;
-        t.triangles = reinterpret_cast<const uint32_t *>(static_cast<const char *>(tb->ptr) + pi.triangle_offset);
-        t.vertex_count = (uint32_t)std::min<uint64_t>(nv, 0xFFFFFFFFull);
-        std::memcpy(t.fwd, a->inst[i].affine, sizeof(t.fwd));
-        std::memcpy(t.albedo, pi.albedo, sizeof(t.albedo));
-        t.emissive = pi.emissive ? 1u : 0u;
-    }
+    std::vector<PtInstance> table = pt_table_from(a, pt_instances, instance_count);
     PtScene S;
-    std::memset(&S, 0, sizeof(S));
-    for (int k = 0; k < 3; k++) {
-        S.light_pos[k] = scene->light_position[k];
-        S.light_u[k] = scene->light_u[k];
-        S.light_v[k] = scene->light_v[k];
-        S.light_e[k] = scene->light_emission[k];
-        S.cam_o[k] = scene->camera_origin[k];
-        S.cam_r[k] = scene->camera_right[k];
-        S.cam_u[k] = scene->camera_up[k];
-        S.cam_f[k] = scene->camera_forward[k];
-    }
-    {
-        // light normal and area: normalize(cross(u, v)), |cross(u, v)| in single precision
-        const float *u = S.light_u, *v = S.light_v;
-        const float c[3] = {u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0]};
-        const float len = std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
-        if (!(len > 0.0f)) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "degenerate light quad");
-        for (int k = 0; k < 3; k++) S.light_n[k] = c[k] / len;
-        S.light_area = len;
-    }
+    if (!pt_scene_from(*scene, S)) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "degenerate light quad");
     S.width = o.width;
     S.height = o.height;
     S.spp = o.spp;
@@ -1080,7 +1088,9 @@ int lc_hip_pathtrace(uint64_t device, uint64_t stream, uint64_t accel, const lc_
     S.rr_depth = o.rr_depth;
     S.seed = o.seed;
     S.sequence = o.sequence;
-    PtOptions po{o.mode == LC_HIP_PT_WAVEFRONT ? 1u : 0u, o.chunk_spp, o.sort ? 1u : 0u, o.graph ? 1u : 0u};
+    PtOptions po{o.mode == LC_HIP_PT_WAVEFRONT ? 1u : 0u, o.chunk_spp, o.sort ? 1u : 0u, o.graph ? 1u : 0u, 0u, 0u};
+    if (!pt_fetch_from(o.fetch_chunk, o.fetch_segments, po))
+        return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "fetch_chunk: a multiple of 64 up to 4096; fetch_segments: 0, 1 or 8");
     DeviceGuard g(dev);
     hipStream_t s = native_stream(stream);
     Geometry G;
@@ -1100,14 +1110,174 @@ int lc_hip_pathtrace(uint64_t device, uint64_t stream, uint64_t accel, const lc_
     return LC_HIP_OK;
 }
 
+// ---- render serving ----
+uint64_t lc_hip_server_create(uint64_t device, const lc_hip_server_options *options) {
+    Device *dev = obj<Device>(device);
+    if (!dev || !options) {
+        fail(LC_HIP_ERROR_INVALID_HANDLE, "bad server arguments");
+        return LC_HIP_INVALID_HANDLE;
+    }
+    const lc_hip_server_options &o = *options;
+    ServeOptions so;
+    so.hbm_budget = o.hbm_budget_bytes;
+    so.wave_paths = o.wave_paths;
+    so.max_active = o.max_active;
+    so.pt = PtOptions{1u, 0u, o.sort ? 1u : 0u, o.graph ? 1u : 0u, 0u, 0u};
+    if (so.wave_paths == 0 || so.wave_paths >= (1ull << 31) || so.max_active == 0 || so.hbm_budget == 0 ||
+        !pt_fetch_from(o.fetch_chunk, o.fetch_segments, so.pt)) {
+        fail(LC_HIP_ERROR_INVALID_ARGUMENT, "server options: wave_paths in 1..2^31-1, max_active and budget > 0, "
+                                            "fetch as for lc_hip_pathtrace");
+        return LC_HIP_INVALID_HANDLE;
+    }
+    DeviceGuard g(dev);
+    std::string err;
+    Server *s = serve_create(dev, so, err);
+    if (!s) {
+        fail(LC_HIP_ERROR_OUT_OF_MEMORY, err);
+        return LC_HIP_INVALID_HANDLE;
+    }
+    return handle(s);
+}
+
+void lc_hip_server_destroy(uint64_t device, uint64_t server) {
+    Device *dev = obj<Device>(device);
+    Server *s = obj<Server>(server);
+    if (!dev || !s) return;
+    DeviceGuard g(dev);
+    serve_destroy(s);
+}
+
+int lc_hip_server_add_scene(uint64_t device, uint64_t server, uint64_t accel, const lc_hip_pt_instance *pt_instances,
+                            uint32_t instance_count, const lc_hip_pt_scene *scene, uint32_t max_depth,
+                            uint32_t rr_depth, uint32_t *scene_id) {
+    Device *dev = obj<Device>(device);
+    Server *s = obj<Server>(server);
+    Accel *a = obj<Accel>(accel);
+    if (!dev || !s || !a) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad server scene handles");
+    if (!scene || !scene_id || (instance_count > 0 && !pt_instances))
+        return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "null server scene argument");
+    if (max_depth < 1 || max_depth > 12) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "max_depth must be in 1..12");
+    PtBinding B;
+    if (!pt_scene_from(*scene, B.S)) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "degenerate light quad");
+    B.S.max_depth = max_depth;
+    B.S.rr_depth = rr_depth;
+    DeviceGuard g(dev);
+    {
+        std::lock_guard<std::mutex> lk(a->mu);
+        if (!a->built) return fail(LC_HIP_ERROR_NOT_BUILT, "accel is not built");
+        if (instance_count != a->inst.size())
+            return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "one lc_hip_pt_instance per accel instance");
+        B.inst = pt_table_from(a, pt_instances, instance_count);
+        int rc = accel_geometry(a, B.G, B.two, B.stack_need);
+        if (rc != LC_HIP_OK) return rc;
+        std::memcpy(B.bounds, a->world_bounds, sizeof(B.bounds));
+    }
+    std::string err;
+    const int id = serve_add_scene(s, B, err);
+    if (id < 0) return fail(LC_HIP_ERROR_HIP, err);
+    *scene_id = (uint32_t)id;
+    return LC_HIP_OK;
+}
+
+int lc_hip_server_submit(uint64_t device, uint64_t server, const lc_hip_render_request *request, uint64_t *request_id) {
+    Device *dev = obj<Device>(device);
+    Server *s = obj<Server>(server);
+    if (!dev || !s) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad server handles");
+    if (!request || !request_id) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "null request");
+    const lc_hip_render_request &q = *request;
+    Buffer *img = obj<Buffer>(q.image_buffer);
+    if (!img) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad request image buffer");
+    const uint64_t npix = (uint64_t)q.width * q.height;
+    if (q.image_offset % 16 != 0 || q.image_offset + npix * 16 > img->bytes)
+        return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "image outside the image buffer (float4 per pixel, 16-B aligned)");
+    if ((uint64_t)q.spp > 0xFFFFFFFFull / 64) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "too many samples");
+    std::string err;
+    float4 *out = reinterpret_cast<float4 *>(static_cast<char *>(img->ptr) + q.image_offset);
+    if (serve_submit(s, q.scene, q.width, q.height, q.spp, q.seed, q.sequence, out, request_id, err) != 0)
+        return fail(LC_HIP_ERROR_INVALID_ARGUMENT, err);
+    return LC_HIP_OK;
+}
+
+int lc_hip_server_query(uint64_t device, uint64_t server, uint64_t request_id, lc_hip_request_status *status) {
+    Server *s = obj<Server>(server);
+    if (!obj<Device>(device) || !s || !status) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad server handles");
+    ServeStatus st;
+    if (serve_query(s, request_id, &st) != 0) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "unknown request");
+    status->state = (uint32_t)st.state;
+    status->samples_done = st.samples_done;
+    status->t_submit = st.t_submit;
+    status->t_admit = st.t_admit;
+    status->t_done = st.t_done;
+    return LC_HIP_OK;
+}
+
+int lc_hip_server_wait(uint64_t device, uint64_t server, uint64_t request_id, double timeout_seconds) {
+    Server *s = obj<Server>(server);
+    if (!obj<Device>(device) || !s) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad server handles");
+    const int r = serve_wait(s, request_id, timeout_seconds);
+    if (r == 1) return fail(LC_HIP_ERROR_TIMEOUT, "request not finished within the timeout");
+    if (r == -1) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "unknown request");
+    if (r == -2) return fail(LC_HIP_ERROR_HIP, "request failed");
+    return LC_HIP_OK;
+}
+
+int lc_hip_server_drain(uint64_t device, uint64_t server, double timeout_seconds) {
+    Server *s = obj<Server>(server);
+    if (!obj<Device>(device) || !s) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad server handles");
+    if (serve_drain(s, timeout_seconds) != 0) return fail(LC_HIP_ERROR_TIMEOUT, "requests still running");
+    return LC_HIP_OK;
+}
+
+int lc_hip_server_stats(uint64_t device, uint64_t server, lc_hip_serving_stats *stats) {
+    Device *dev = obj<Device>(device);
+    Server *s = obj<Server>(server);
+    if (!dev || !s || !stats) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad server handles");
+    DeviceGuard g(dev);
+    ServeStats st;
+    serve_stats(s, &st);
+    std::memset(stats, 0, sizeof(*stats));
+    stats->waves = st.waves;
+    stats->rays_closest = st.rays_closest;
+    stats->rays_shadow = st.rays_shadow;
+    stats->requests_done = st.requests_done;
+    stats->peak_active = st.peak_active;
+    stats->active = st.active;
+    stats->queued = st.queued;
+    stats->peak_bytes = st.peak_bytes;
+    return LC_HIP_OK;
+}
+
 int lc_hip_trace_set_config(uint64_t device, int kind, int block, int lds_stack, int steps, int blocks_per_cu) {
     Device *dev = obj<Device>(device);
     if (!dev || kind < 0 || kind > 1) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "bad trace config target");
     if (!trace_config_supported(block, lds_stack) || steps < 1 || blocks_per_cu < 0)
         return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "unsupported trace config");
     std::lock_guard<std::mutex> lk(dev->mu);
-    const bool sort_rays = dev->cfg[kind].sort_rays;
-    dev->cfg[kind] = TraceConfig{block, lds_stack, steps, blocks_per_cu, sort_rays};
+    TraceConfig &c = dev->cfg[kind];
+    c.block = block;
+    c.slds = lds_stack;
+    c.steps = steps;
+    c.blocks_per_cu = blocks_per_cu;
+    return LC_HIP_OK;
+}
+
+int lc_hip_trace_set_fetch(uint64_t device, int kind, uint32_t chunk, uint32_t segments) {
+    Device *dev = obj<Device>(device);
+    if (!dev || kind < 0 || kind > 1) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "bad trace config target");
+    if (chunk > 4096 || (chunk != 0 && chunk % 64 != 0) || (segments != 1 && segments != kFetchSegments))
+        return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "fetch chunk must be 0 or a multiple of 64 up to 4096; segments 1 or 8");
+    std::lock_guard<std::mutex> lk(dev->mu);
+    dev->cfg[kind].fetch_chunk = chunk;
+    dev->cfg[kind].fetch_segments = segments;
+    return LC_HIP_OK;
+}
+
+int lc_hip_trace_get_fetch(uint64_t device, int kind, uint32_t *chunk, uint32_t *segments) {
+    Device *dev = obj<Device>(device);
+    if (!dev || kind < 0 || kind > 1) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "bad trace config target");
+    std::lock_guard<std::mutex> lk(dev->mu);
+    if (chunk) *chunk = dev->cfg[kind].fetch_chunk;
+    if (segments) *segments = dev->cfg[kind].fetch_segments;
     return LC_HIP_OK;
 }
 

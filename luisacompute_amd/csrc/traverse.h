@@ -190,6 +190,7 @@ struct Walker {
     uint32_t ray_mask;
     float t_best, bu, bv;
     uint32_t b_prim, b_inst;
+    uint32_t b_rec;       // leaf record (TriRecord index in the hit instance's BLAS) of the best hit
     bool any_hit;
     uint32_t g_base, g_mask, t_base, t_mask;
     bool in_blas;
@@ -199,10 +200,16 @@ struct Walker {
     const TriRecord *tris;
     uint64_t n_nodes, n_tris;
 
-    __device__ __forceinline__ void init(uint2 (*s)[BLOCK], uint32_t t, uint2 *spill_lane, uint32_t depth,
-                                         uint32_t *err) {
+    __device__ __forceinline__ void init(const Geometry &G, uint2 (*s)[BLOCK], uint32_t t, uint2 *spill_lane,
+                                         uint32_t depth, uint32_t *err) {
         stk = s; tid = t; spill = spill_lane; spill_depth = depth; error = err;
         sp = 0; n_nodes = 0; n_tris = 0;
+        // single-instance form: the BLAS pointers never change (kept uniform)
+        in_blas = !TWO;
+        nodes = TWO ? G.tlas_nodes : G.blas_nodes;
+        tris = G.blas_tris;
+        sp_inst = 0;
+        cur_inst = 0;
     }
 
     // The stack never holds more than (wide depth - 1) node groups plus the TLAS leaf entries;
@@ -235,20 +242,16 @@ struct Walker {
             for (int k = 0; k < 3; k++) { R.o[k] = wo[k]; R.d[k] = wd[k]; }
             in_blas = false;
             nodes = G.tlas_nodes;
-            tris = G.blas_tris;
             ray_setup(R);
             g_mask = G.empty ? 0u : ((1u << R.oct) | (1u << 8));
+            sp_inst = 0;
+            cur_inst = 0;
         } else {
-            in_blas = true;
-            nodes = G.blas_nodes;
-            tris = G.blas_tris;
             xform_point(G.inst_inv, wo, R.o);
             xform_vector(G.inst_inv, wd, R.d);
             ray_setup(R);
             g_mask = (G.empty || (G.inst_mask & ray_mask & 0xFFu) == 0u) ? 0u : ((1u << R.oct) | (1u << 8));
         }
-        sp_inst = 0;
-        cur_inst = 0;
     }
 
     // Runs up to `steps` traversal steps; true when the ray is finished.
@@ -271,6 +274,7 @@ struct Walker {
                             t_best = t; bu = u; bv = v;
                             b_prim = prim;
                             b_inst = inst;
+                            b_rec = t_base + b;
                         }
                     }
                 } else {
@@ -350,13 +354,14 @@ struct Walker {
         return false;
     }
 
-    // Closest-hit record (24-B SurfaceHit as three uint2).
-    __device__ __forceinline__ void write_hit(uint2 *h) const {
+    // Closest-hit record (24-B SurfaceHit as three uint2). The padding word after t carries the
+    // leaf record index when `record` is set (queues internal to the path tracer), else 0.
+    __device__ __forceinline__ void write_hit(uint2 *h, bool record) const {
         const bool hit = b_prim != 0xFFFFFFFFu;
         const uint32_t t_bits = hit ? __float_as_uint(t_best) : __float_as_uint(tmax);
         h[0] = make_uint2(hit ? b_inst : 0xFFFFFFFFu, b_prim);
         h[1] = make_uint2(hit ? __float_as_uint(bu) : 0u, hit ? __float_as_uint(bv) : 0u);
-        h[2] = make_uint2(t_bits, 0u);
+        h[2] = make_uint2(t_bits, (hit && record) ? b_rec : 0u);
     }
 };
 

@@ -179,7 +179,7 @@ class BoxRoom:
         stream << accel.build()
         inst = []
         for k, m in enumerate(meshes):
-            pi = PtInstance(m.vb.handle, 0, 16, m.tb.handle, 0)
+            pi = PtInstance()
             pi.albedo[:] = [float(x) for x in self.albedo[k]]
             pi.emissive = int(self.emissive[k])
             inst.append(pi)

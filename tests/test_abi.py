@@ -38,6 +38,28 @@ def test_abi_struct_sizes():
     assert lc.HIT_DTYPE.fields["committed_ray_t"][1] == 16
 
 
+def test_c_struct_layouts_match_binding(tmp_path):
+    """Every struct of include/lc_hip.h, compiled by the C compiler, has the size the binding uses."""
+    pairs = {"lc_hip_ray": lc.RAY_DTYPE.itemsize, "lc_hip_surface_hit": lc.HIT_DTYPE.itemsize,
+             "lc_hip_accel_modification": ctypes.sizeof(lc.Modification),
+             "lc_hip_accel_option": ctypes.sizeof(lc.AccelOption), "lc_hip_ray_batch": ctypes.sizeof(lc.RayBatch),
+             "lc_hip_bvh_stats": ctypes.sizeof(lc.BvhStats), "lc_hip_pt_instance": ctypes.sizeof(lc.PtInstance),
+             "lc_hip_pt_scene": ctypes.sizeof(lc.PtScene), "lc_hip_pt_options": ctypes.sizeof(lc.PtOptions),
+             "lc_hip_server_options": ctypes.sizeof(lc.ServerOptions),
+             "lc_hip_render_request": ctypes.sizeof(lc.RenderRequest),
+             "lc_hip_request_status": ctypes.sizeof(lc.RequestStatus),
+             "lc_hip_serving_stats": ctypes.sizeof(lc.ServingStats)}
+    src = tmp_path / "sizes.c"
+    src.write_text('#include <stdio.h>\n#include "lc_hip.h"\nint main(void) {\n' +
+                   "".join(f'    printf("{k} %zu\\n", sizeof({k}));\n' for k in pairs) + "    return 0;\n}\n")
+    exe = tmp_path / "sizes"
+    import subprocess
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {l.split()[0]: int(l.split()[1]) for l in out if l.strip()}
+    assert got == pairs
+
+
 def test_no_gpu_here_fails_loudly():
     if lc.lib().lc_hip_device_count() > 0:
         pytest.skip("a GPU is visible")
