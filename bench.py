@@ -37,15 +37,20 @@ def parse():
     ap.add_argument("--cpu-sample-rays", type=int, default=1 << 18)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run 3 device-to-device copies of this many MiB (counter calibration) and exit")
-    ap.add_argument("--workload", default="soup", choices=["soup", "instanced", "pathtracer"],
+    ap.add_argument("--workload", default="soup", choices=["soup", "instanced", "pathtracer", "serving"],
                     help="soup: the headline line; instanced: 64 ray batches over a 1,024-instance TLAS; "
-                         "pathtracer: box room 1920x1080, 64 spp, depth 8 (wavefront vs megakernel)")
+                         "pathtracer: box room 1920x1080, 64 spp, depth 8 (wavefront vs megakernel); "
+                         "serving: render requests over 4 resident scenes (served vs plain)")
+    ap.add_argument("--serve-requests", type=int, default=48)
+    ap.add_argument("--serve-rate", type=float, default=0.0, help="arrivals/s (0: 0.8 x plain saturated throughput)")
+    ap.add_argument("--serve-budget-mib", type=int, default=1024)
+    ap.add_argument("--serve-wave-paths", type=int, default=8 << 20)
     ap.add_argument("--pt-width", type=int, default=1920)
     ap.add_argument("--pt-height", type=int, default=1080)
     ap.add_argument("--pt-spp", type=int, default=64)
     ap.add_argument("--pt-chunk", type=int, default=0, help="wavefront samples per wave (0: about 16M paths)")
     ap.add_argument("--pt-sort", type=int, default=1)
-    ap.add_argument("--pt-fetch", default="256,8", help="wavefront traces: slots per claim, queue segments")
+    ap.add_argument("--pt-fetch", default="0,1", help="wavefront traces: slots per claim (0: exact), queue segments")
     ap.add_argument("--pt-no-mega", action="store_true", help="skip the megakernel comparison")
     return ap.parse_args()
 
@@ -136,6 +141,133 @@ def bench_pathtracer(a):
             "value": round((c_c + c_s) / t_pt / 1e6, 4), "unit": "Mrays/s", "cores": nco, "kind": "port",
             "sample": f"oracle path tracer, {rows} image rows x {w} px x {spp} spp ({c_c + c_s} rays, {t_pt:.2f} s) "
                       f"on a BVH built in {t_build:.2f} s (build excluded, as for the GPU)"}
+    print(json.dumps(out), flush=True)
+
+
+def bench_serving(a):
+    """Render serving over four resident scenes (icosphere L6, 16.7M-triangle soup, box room,
+    1,024-instance TLAS). Requests: scene U{4}, square resolution U{256,512,1024,2048}, spp
+    U{1,4,16,64}, depth 8, PCG32 stream (42, 5). Two disciplines on the same requests:
+      plain  : one megakernel dispatch per request, in arrival order (one stream);
+      served : the serving loop (HBM-budget admission, continuous batching into shared waves,
+               scenes on their own streams).
+    Throughput is measured with every request queued at t = 0 (requests/s = N / makespan);
+    latency (p50/p99 of completion - arrival) with Poisson arrivals at a fixed rate, 0.8 x the
+    plain discipline's saturated throughput."""
+    import threading
+    import luisacompute_amd as lc
+    from luisacompute_amd import autotune, workloads
+    dev = lc.Device(0)
+    autotune.load_default(dev)
+    s = dev.create_stream()
+    t0 = time.perf_counter()
+    scenes = workloads.serving_scenes(dev, s)
+    s.synchronize()
+    t_setup = time.perf_counter() - t0
+    n = a.serve_requests
+    base = workloads.serving_requests(n, 0.0)
+    bufs = [dev.create_buffer(16, r["res"] * r["res"]) for r in base]
+    cnts = [dev.create_buffer(8, 2) for _ in base]
+
+    def plain(reqs, rate_mode):
+        lat = []
+        t_start = time.perf_counter()
+        for r in reqs:
+            if rate_mode:
+                wait = t_start + r["arrival"] - time.perf_counter()
+                if wait > 0:
+                    time.sleep(wait)
+            name, acc, inst, sc = scenes[r["scene"]]
+            o = lc.PtOptions()
+            o.width = o.height = r["res"]
+            o.spp, o.max_depth, o.rr_depth, o.seed, o.sequence, o.mode = r["spp"], 8, 3, 42, r["sequence"], lc.PT_MEGAKERNEL
+            acc.pathtrace(s, inst, sc, o, bufs[r["index"]], counters=cnts[r["index"]])
+            s.synchronize()
+            lat.append(time.perf_counter() - t_start - r["arrival"])
+        return time.perf_counter() - t_start, lat
+
+    def plain_rays(reqs):
+        tot = 0
+        for r in reqs:
+            c = np.zeros(2, np.uint64)
+            s << cnts[r["index"]].copy_to(c) << lc.synchronize()
+            tot += int(c.sum())
+        return tot
+
+    fetch = tuple(int(x) for x in a.pt_fetch.split(","))
+
+    def served(reqs, rate_mode):
+        srv = lc.Server(dev, hbm_budget=a.serve_budget_mib << 20, wave_paths=a.serve_wave_paths, max_active=64,
+                        sort=a.pt_sort, graph=1, fetch=fetch)
+        ids = [srv.add_scene(acc, inst, sc) for _, acc, inst, sc in scenes]
+        # warm: capture each scene's wave graph
+        for sid in ids:
+            srv.submit(sid, 16, 16, 1, bufs[0], seed=1, sequence=1)
+        srv.drain()
+        st0 = srv.stats()
+        rid = {}
+        t_start = time.perf_counter()
+
+        def submitter():
+            for r in reqs:
+                if rate_mode:
+                    wait = t_start + r["arrival"] - time.perf_counter()
+                    if wait > 0:
+                        time.sleep(wait)
+                rid[r["index"]] = srv.submit(ids[r["scene"]], r["res"], r["res"], r["spp"], bufs[r["index"]], seed=42,
+                                             sequence=r["sequence"])
+        th = threading.Thread(target=submitter)
+        th.start()
+        th.join()
+        srv.drain(timeout=1200)
+        makespan = time.perf_counter() - t_start
+        lat = []
+        for r in reqs:
+            q = srv.query(rid[r["index"]])
+            lat.append(q.t_done - q.t_submit)
+        st = srv.stats()
+        rays = (st.rays_closest + st.rays_shadow) - (st0.rays_closest + st0.rays_shadow)
+        out = dict(waves=st.waves - st0.waves, peak_active=st.peak_active, peak_accumulator_mib=st.peak_bytes / 2 ** 20)
+        srv.close()
+        return makespan, lat, rays, out
+
+    # warm the plain path (kernel load) on a tiny request per scene
+    for _, acc, inst, sc in scenes:
+        o = lc.PtOptions()
+        o.width = o.height = 16
+        o.spp, o.max_depth, o.rr_depth, o.seed, o.sequence, o.mode = 1, 8, 3, 42, 1, lc.PT_MEGAKERNEL
+        acc.pathtrace(s, inst, sc, o, bufs[0])
+    s.synchronize()
+    print(f"[serving] setup {t_setup:.1f} s; {n} requests; plain backlog", file=sys.stderr, flush=True)
+    mk_plain, _ = plain(base, False)
+    rays_plain = plain_rays(base)
+    print(f"[serving] plain backlog {mk_plain:.2f} s; served backlog", file=sys.stderr, flush=True)
+    mk_served, _, rays_served, info_b = served(base, False)
+    rate = a.serve_rate or 0.8 * n / mk_plain
+    timed = workloads.serving_requests(n, rate)
+    print(f"[serving] served backlog {mk_served:.2f} s; rate {rate:.2f}/s plain", file=sys.stderr, flush=True)
+    _, lat_plain = plain(timed, True)
+    print("[serving] served at rate", file=sys.stderr, flush=True)
+    _, lat_served, _, info_r = served(timed, True)
+
+    def pct(x, q):
+        return round(float(np.percentile(np.asarray(x), q)) * 1e3, 2)
+
+    out = {
+        "metric": "served renders: requests/s (saturated) and p50/p99 latency at a fixed Poisson rate",
+        "value": round(n / mk_served, 3), "unit": "requests/s", "n_gpus": 1, "steps": n, "warmup": 1,
+        "ms_per_step": round(mk_served / n * 1e3, 2), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "plain_requests_s": round(n / mk_plain, 3), "served_over_plain": round(mk_plain / mk_served, 3),
+        "served_mrays_s": round(rays_served / mk_served / 1e6, 1), "plain_mrays_s": round(rays_plain / mk_plain / 1e6, 1),
+        "rate_per_s": round(rate, 3),
+        "latency_ms": {"served_p50": pct(lat_served, 50), "served_p99": pct(lat_served, 99),
+                       "plain_p50": pct(lat_plain, 50), "plain_p99": pct(lat_plain, 99)},
+        "served_backlog": info_b, "served_at_rate": info_r,
+        "config": {"workload": f"{n} requests over 4 resident scenes (icosphere L6, soup 16.7M, box room 3.5M, "
+                               f"1,024-instance TLAS); res U{{256..2048}}^2, spp U{{1,4,16,64}}, depth 8; "
+                               f"budget {a.serve_budget_mib} MiB, wave {a.serve_wave_paths} paths"},
+    }
     print(json.dumps(out), flush=True)
 
 
@@ -241,6 +373,9 @@ def main():
         return
     if a.workload == "pathtracer":
         bench_pathtracer(a)
+        return
+    if a.workload == "serving":
+        bench_serving(a)
         return
     import luisacompute_amd as lc
     from luisacompute_amd import host

@@ -11,9 +11,11 @@
 //   4. node boxes bottom-up in phases whose child handoffs stay inside one workgroup (so they
 //      only need workgroup-scope ordering -- an agent-scope release/acquire writes back and
 //      invalidates the XCD-local L2 and made a single-pass climb 9x slower than the whole build);
+//      the first phase, over all leaves, hands off through LDS;
 //   5. top-down collapse to 8-wide nodes, one launch per wide level, 8 lanes per node: each lane
 //      holds one child, the group opens its widest child until it has 8, assigns children to
-//      octant slots, quantises each child box to 8 bits per plane and copies leaf triangles.
+//      octant slots, quantises each child box to 8 bits per plane and lists leaf primitives;
+//   6. triangle records in leaf order, one thread per record.
 // The same builder serves the TLAS (primitives = instance world boxes).
 #include <hip/hip_runtime.h>
 #include <cstring>
@@ -92,7 +94,7 @@ __global__ void k_init(BuildCounters *ctr, uint32_t *slots, uint32_t nslots) {
             ctr->bounds[3 + a] = 0.0f;
         }
     }
-    if (i < kMaxCollapseLevels) ctr->level_count[i] = 0;
+    if (i < kMaxCollapseLevels) ctr->level[i] = 0ull;
     for (uint32_t k = i; k < nslots; k += gridDim.x * blockDim.x) slots[k] = 0u;
 }
 
@@ -182,9 +184,17 @@ struct BinaryTree {
     uint2 *child;          // per internal node: left ref, right ref
     uint2 *range;          // per internal node: first, last sorted position
     uint32_t *slots;       // handoff word per internal node (0 = no child arrived yet)
-    uint32_t *parent_int;  // parent of internal node i
-    uint32_t *parent_leaf; // parent of leaf (sorted position) i
+    // Parent links of internal node i / leaf (sorted position) i: (parent, e | side << 31) where
+    // side is 1 for a right child and e is the parent's range end the child does not share (its
+    // last leaf for a left child, its first for a right one). A climbing thread that knows its
+    // node's range [f, l] gets the parent's range [f, e] or [e, l] from this one load.
+    uint2 *parent_int;
+    uint2 *parent_leaf;
 };
+
+__device__ __forceinline__ uint2 parent_link(uint32_t parent, uint32_t end, uint32_t side) {
+    return make_uint2(parent, end | (side << 31));
+}
 
 // Binary radix tree in one parallel pass: internal node i finds its key range by exponential and
 // binary search over prefix lengths and splits it at the highest differing bit (Karras,
@@ -214,8 +224,9 @@ __global__ __launch_bounds__(kBlock) void k_karras(uint32_t n, const uint32_t *k
     const uint32_t right = hi == gamma + 1 ? ((uint32_t)(gamma + 1) | kLeafBit) : (uint32_t)(gamma + 1);
     bt.child[i] = make_uint2(left, right);
     bt.range[i] = make_uint2((uint32_t)lo, (uint32_t)hi);
-    if (left & kLeafBit) bt.parent_leaf[gamma] = (uint32_t)i; else bt.parent_int[gamma] = (uint32_t)i;
-    if (right & kLeafBit) bt.parent_leaf[gamma + 1] = (uint32_t)i; else bt.parent_int[gamma + 1] = (uint32_t)i;
+    const uint2 lp = parent_link((uint32_t)i, (uint32_t)hi, 0u), rp = parent_link((uint32_t)i, (uint32_t)lo, 1u);
+    if (left & kLeafBit) bt.parent_leaf[gamma] = lp; else bt.parent_int[gamma] = lp;
+    if (right & kLeafBit) bt.parent_leaf[gamma + 1] = rp; else bt.parent_int[gamma + 1] = rp;
 }
 
 __device__ __forceinline__ uint32_t ref_first(const BinaryTree &bt, uint32_t r) {
@@ -235,7 +246,7 @@ __device__ __forceinline__ void publish_root(BuildCounters *ctr, uint2 *queue0, 
     ctr->bounds[0] = lo.x; ctr->bounds[1] = lo.y; ctr->bounds[2] = lo.z;
     ctr->bounds[3] = hi.x; ctr->bounds[4] = hi.y; ctr->bounds[5] = hi.z;
     queue0[0] = make_uint2(root, 0u);
-    ctr->level_count[0] = 1;
+    ctr->level[0] = 1ull << 32;
 }
 
 // Bottom-up boxes, one phase. The workgroup owns entries [e0, e1) -- complete subtrees ordered by
@@ -262,9 +273,14 @@ __global__ void k_bounds_phase(const float4 *pbox, uint32_t n, const uint32_t *i
     if (e >= e1) return;
     uint32_t cur = entries ? entries[e] : (e | kLeafBit);
     float3 lo, hi;
+    uint32_t f, l;
     if (cur & kLeafBit) {
-        stored_box(pbox, ids[cur & ~kLeafBit], lo, hi);
+        f = l = cur & ~kLeafBit;
+        stored_box(pbox, ids[f], lo, hi);
     } else {
+        const uint2 rg = bt.range[cur];
+        f = rg.x;
+        l = rg.y;
         const float4 *cb = bt.cbox + 4 * (uint64_t)cur;
         float4 a = cb[0], b = cb[1], c = cb[2], d = cb[3];
         lo = make_float3(fminf(a.x, c.x), fminf(a.y, c.y), fminf(a.z, c.z));
@@ -275,13 +291,13 @@ __global__ void k_bounds_phase(const float4 *pbox, uint32_t n, const uint32_t *i
             publish_root(ctr, queue0, 0u, lo, hi);
             return;
         }
-        const uint32_t p = (cur & kLeafBit) ? bt.parent_leaf[cur & ~kLeafBit] : bt.parent_int[cur];
-        const uint2 pr = bt.range[p];
-        if (pr.x < L0 || pr.y > L1) {
-            pend[ref_first(bt, cur)] = cur;
+        const uint2 link = (cur & kLeafBit) ? bt.parent_leaf[f] : bt.parent_int[cur];
+        const uint32_t p = link.x, side = link.y >> 31, end = link.y & ~kLeafBit;
+        const uint32_t pf = side ? end : f, pl = side ? l : end;
+        if (pf < L0 || pl > L1) {
+            pend[f] = cur;
             return;
         }
-        const uint32_t side = bt.child[p].x == cur ? 0u : 1u;
         bt.cbox[4 * (uint64_t)p + 2 * side] = make_float4(lo.x, lo.y, lo.z, 0.0f);
         bt.cbox[4 * (uint64_t)p + 2 * side + 1] = make_float4(hi.x, hi.y, hi.z, 0.0f);
         const uint32_t first = AGENT
@@ -293,6 +309,52 @@ __global__ void k_bounds_phase(const float4 *pbox, uint32_t n, const uint32_t *i
         lo = make_float3(fminf(lo.x, slo.x), fminf(lo.y, slo.y), fminf(lo.z, slo.z));
         hi = make_float3(fmaxf(hi.x, shi.x), fmaxf(hi.y, shi.y), fmaxf(hi.z, shi.z));
         cur = p;
+        f = pf;
+        l = pl;
+    }
+}
+
+// First bottom-up phase: workgroup b owns leaves [b B, b B + B) and climbs from every leaf. A node
+// whose range lies inside the run has its index inside the run too (a radix-tree node's index is
+// one end of its range), so both children of such a node hand off through LDS: each writes its
+// box to its side's LDS slot, then an LDS atomic picks the second arrival, which merges and goes
+// on. Boxes are also stored to cbox for the collapse. Parents reaching outside the run are
+// deferred to pend as in k_bounds_phase. Requires n > B (the root never lies inside a run).
+template<int B>
+__global__ __launch_bounds__(B) void k_bounds_leaves(const float4 *pbox, uint32_t n, const uint32_t *ids,
+                                                     BinaryTree bt, uint32_t *pend) {
+    __shared__ float s_box[B][2][6];
+    __shared__ uint32_t s_arrived[B];
+    const uint32_t L0 = blockIdx.x * B, L1 = min(L0 + (uint32_t)B - 1u, n - 1u);
+    s_arrived[threadIdx.x] = 0u;
+    __syncthreads();
+    const uint32_t leaf = L0 + threadIdx.x;
+    if (leaf > L1) return;
+    uint32_t cur = leaf | kLeafBit, f = leaf, l = leaf;
+    float3 lo, hi;
+    stored_box(pbox, ids[leaf], lo, hi);
+    for (;;) {
+        const uint2 link = (cur & kLeafBit) ? bt.parent_leaf[f] : bt.parent_int[cur];
+        const uint32_t p = link.x, side = link.y >> 31, end = link.y & ~kLeafBit;
+        const uint32_t pf = side ? end : f, pl = side ? l : end;
+        if (pf < L0 || pl > L1) {
+            pend[f] = cur;
+            return;
+        }
+        bt.cbox[4 * (uint64_t)p + 2 * side] = make_float4(lo.x, lo.y, lo.z, 0.0f);
+        bt.cbox[4 * (uint64_t)p + 2 * side + 1] = make_float4(hi.x, hi.y, hi.z, 0.0f);
+        const uint32_t k = p - L0;
+        float *mine = s_box[k][side];
+        mine[0] = lo.x; mine[1] = lo.y; mine[2] = lo.z;
+        mine[3] = hi.x; mine[4] = hi.y; mine[5] = hi.z;
+        const uint32_t before = __hip_atomic_fetch_add(&s_arrived[k], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (before == 0u) return; // the sibling's thread completes p
+        const float *other = s_box[k][1u - side];
+        lo = make_float3(fminf(lo.x, other[0]), fminf(lo.y, other[1]), fminf(lo.z, other[2]));
+        hi = make_float3(fmaxf(hi.x, other[3]), fmaxf(hi.y, other[4]), fmaxf(hi.z, other[5]));
+        cur = p;
+        f = pf;
+        l = pl;
     }
 }
 
@@ -384,15 +446,34 @@ __device__ __forceinline__ uint32_t ord_u32(float f) {
 // One wide level. Each group of 8 lanes turns one work item (binary ref, wide node index) into a
 // WideNode; lane k of the group holds child candidate k. All shuffles are executed by every lane
 // of a group (groups without an item compute on empty data), and each group's control flow is
-// uniform across its 8 lanes.
-__global__ __launch_bounds__(kBlock) void k_collapse(const float4 *pbox, const uint32_t *ids, PrimSource src,
-                                                     BinaryTree bt, BuildCounters *ctr, uint32_t level,
-                                                     const uint2 *qin, uint2 *qout, WideOut out,
-                                                     uint32_t split_small) {
-    const uint32_t count = ctr->level_count[level];
+// uniform across its 8 lanes; the loop over items is uniform across the block.
+//
+// A candidate carries its sorted range [first, last]: in the radix tree an internal node split at
+// gamma has children covering [first, gamma] and [gamma + 1, last] (child.x & ~kLeafBit == gamma
+// for leaf and internal children alike), so opening a child needs one round of loads (its child
+// pair and boxes) and no range lookups.
+//
+// Wide nodes of level l + 1 are numbered after those of levels 0..l in queue order, so a child's
+// node index is (nodes of levels 0..l) + its queue slot. Queue slots and triangle ranges are
+// reserved with one atomic per wave. Leaf slots write their primitive ids to out.prim_out in
+// triangle-record order; a triangle BLAS turns them into records afterwards (k_tri_records).
+__global__ __launch_bounds__(kBlock) void k_collapse(const float4 *pbox, const uint32_t *ids, BinaryTree bt,
+                                                     BuildCounters *ctr, uint32_t level, const uint2 *qin,
+                                                     uint2 *qout, WideOut out, uint32_t split_small) {
+    __shared__ __align__(16) uint32_t s_node[kBlock / 8][20];
+    __shared__ uint32_t s_wave[2][kBlock / 64];
+    __shared__ unsigned long long s_base;
+    const uint32_t count = (uint32_t)(ctr->level[level] >> 32);
     if (count == 0) return;
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicMax(&ctr->depth, level + 1);
+    uint32_t next_start = 0, tri_start = 0;
+    for (uint32_t j = 0; j <= level; j++) {
+        const uint64_t lw = ctr->level[j];
+        next_start += (uint32_t)(lw >> 32);
+        tri_start += (uint32_t)lw;
+    }
     const uint32_t lane = threadIdx.x & 7u;
+    const uint32_t wl = threadIdx.x & 63u;
     const uint32_t groups = blockDim.x >> 3;
     for (uint32_t base = blockIdx.x * groups; base < count; base += gridDim.x * groups) {
         const uint32_t item_idx = base + (threadIdx.x >> 3);
@@ -400,7 +481,7 @@ __global__ __launch_bounds__(kBlock) void k_collapse(const float4 *pbox, const u
         const uint2 item = active ? qin[item_idx] : make_uint2(kLeafBit, 0u);
         // ---- gather children: lanes 0/1 start with the binary node's two children
         bool valid = false;
-        uint32_t ref = 0, cnt = 0;
+        uint32_t ref = 0, first = 0, last = 0;
         float3 lo = make_float3(FLT_MAX, FLT_MAX, FLT_MAX), hi = make_float3(-FLT_MAX, -FLT_MAX, -FLT_MAX);
         uint32_t nc;
         if (!active) {
@@ -409,18 +490,21 @@ __global__ __launch_bounds__(kBlock) void k_collapse(const float4 *pbox, const u
             nc = 1;
             if (lane == 0) {
                 ref = item.x;
-                cnt = 1;
+                first = last = item.x & ~kLeafBit;
                 valid = true;
-                stored_box(pbox, ids[item.x & ~kLeafBit], lo, hi);
+                stored_box(pbox, ids[first], lo, hi);
             }
         } else {
             nc = 2;
             if (lane < 2) {
-                ref = lane == 0 ? bt.child[item.x].x : bt.child[item.x].y;
+                const uint2 ch = bt.child[item.x], rg = bt.range[item.x];
+                const uint32_t gamma = ch.x & ~kLeafBit;
+                ref = lane == 0 ? ch.x : ch.y;
+                first = lane == 0 ? rg.x : gamma + 1u;
+                last = lane == 0 ? gamma : rg.y;
                 const float4 a = bt.cbox[4 * (uint64_t)item.x + 2 * lane], b = bt.cbox[4 * (uint64_t)item.x + 2 * lane + 1];
                 lo = make_float3(a.x, a.y, a.z);
                 hi = make_float3(b.x, b.y, b.z);
-                cnt = ref_count(bt, ref);
                 valid = true;
             }
         }
@@ -428,6 +512,7 @@ __global__ __launch_bounds__(kBlock) void k_collapse(const float4 *pbox, const u
         for (int it = 0; it < 7; it++) {
             uint64_t key = 0;
             if (valid) {
+                const uint32_t cnt = last - first + 1u;
                 const uint32_t prio = cnt > kLeafMax ? 2u : ((split_small && !(ref & kLeafBit) && cnt > 1u) ? 1u : 0u);
                 if (prio) key = ((uint64_t)prio << 40) | ((uint64_t)__float_as_uint(fmaxf(box_area(lo, hi), 0.0f)) << 3) | (7u - lane);
             }
@@ -435,22 +520,22 @@ __global__ __launch_bounds__(kBlock) void k_collapse(const float4 *pbox, const u
             if (nc >= 8 || best == 0) break; // uniform within the group
             const uint32_t winner = 7u - (uint32_t)(best & 7u);
             const uint32_t wref = __shfl(ref, (int)winner, 8);
-            if (lane == winner) {
-                const float4 a = bt.cbox[4 * (uint64_t)wref], b = bt.cbox[4 * (uint64_t)wref + 1];
-                ref = bt.child[wref].x;
+            const uint32_t wlast = __shfl(last, (int)winner, 8);
+            if (lane == winner || lane == nc) {
+                const uint32_t side = lane == winner ? 0u : 1u;
+                const uint2 ch = bt.child[wref];
+                const float4 a = bt.cbox[4 * (uint64_t)wref + 2 * side], b = bt.cbox[4 * (uint64_t)wref + 2 * side + 1];
+                const uint32_t gamma = ch.x & ~kLeafBit;
+                ref = side ? ch.y : ch.x;
+                if (side) first = gamma + 1u;
+                last = side ? wlast : gamma;
                 lo = make_float3(a.x, a.y, a.z);
                 hi = make_float3(b.x, b.y, b.z);
-                cnt = ref_count(bt, ref);
-            } else if (lane == nc) {
-                const float4 c = bt.cbox[4 * (uint64_t)wref + 2], d = bt.cbox[4 * (uint64_t)wref + 3];
-                ref = bt.child[wref].y;
-                lo = make_float3(c.x, c.y, c.z);
-                hi = make_float3(d.x, d.y, d.z);
-                cnt = ref_count(bt, ref);
                 valid = true;
             }
             nc++;
         }
+        const uint32_t cnt = valid ? last - first + 1u : 0u;
         // ---- node box, origin and scales
         const float3 nlo = make_float3(g8_min(lo.x), g8_min(lo.y), g8_min(lo.z));
         const float3 nhi = make_float3(g8_max(hi.x), g8_max(hi.y), g8_max(hi.z));
@@ -489,78 +574,112 @@ __global__ __launch_bounds__(kBlock) void k_collapse(const float4 *pbox, const u
             ttotal += ck;
             if (sk < slot) toff += ck;
         }
-        const uint32_t ninternal = (uint32_t)__popc(imask);
-        uint32_t child_base = 0, tri_base = 0, qbase = 0;
-        if (lane == 0 && active) {
-            if (ninternal) {
-                child_base = atomicAdd(&ctr->node_count, ninternal);
-                qbase = atomicAdd(&ctr->level_count[level + 1], ninternal);
-            }
-            if (ttotal) tri_base = atomicAdd(&ctr->tri_count, ttotal);
+        // ---- reservations: inclusive scan of the 8 groups' counts (uniform within a group) over
+        // the wave, wave totals through LDS, one 64-bit atomic per block for the next level's
+        // queue slots (upper half) and this level's triangle records (lower half)
+        const uint32_t gi = active ? (uint32_t)__popc(imask) : 0u, gt = active ? ttotal : 0u;
+        uint32_t si = gi, st = gt;
+        for (int o = 8; o < 64; o <<= 1) {
+            const uint32_t ui = __shfl_up(si, (unsigned)o, 64), ut = __shfl_up(st, (unsigned)o, 64);
+            if (wl >= (uint32_t)o) { si += ui; st += ut; }
         }
-        child_base = __shfl(child_base, 0, 8);
-        tri_base = __shfl(tri_base, 0, 8);
-        qbase = __shfl(qbase, 0, 8);
+        const uint32_t wave = threadIdx.x >> 6;
+        if (wl == 63u) {
+            s_wave[0][wave] = si;
+            s_wave[1][wave] = st;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t bi = 0, btri = 0;
+            for (uint32_t w = 0; w < kBlock / 64; w++) {
+                bi += s_wave[0][w];
+                btri += s_wave[1][w];
+            }
+            unsigned long long b = 0;
+            if (bi | btri)
+                b = atomicAdd(reinterpret_cast<unsigned long long *>(&ctr->level[level + 1]),
+                              ((unsigned long long)bi << 32) | btri);
+            s_base = b;
+        }
+        __syncthreads();
+        uint32_t wpi = 0, wpt = 0;
+        for (uint32_t w = 0; w < wave; w++) {
+            wpi += s_wave[0][w];
+            wpt += s_wave[1][w];
+        }
+        const unsigned long long bbase = s_base;
+        const uint32_t qbase = (uint32_t)(bbase >> 32) + wpi + si - gi;
+        const uint32_t tri_base = tri_start + (uint32_t)bbase + wpt + st - gt;
+        const uint32_t child_base = next_start + qbase;
         const uint32_t rank = valid ? (uint32_t)__popc(imask & ((1u << slot) - 1u)) : 0u;
-        uint32_t meta = 0u, qlx = 0u, qly = 0u, qlz = 0u, qhx = 0u, qhy = 0u, qhz = 0u;
+        uint32_t meta = 0u;
         if (active && valid) {
-            qlx = quant_lo(lo.x, nlo.x, sx); qhx = quant_hi(hi.x, nlo.x, sx);
-            qly = quant_lo(lo.y, nlo.y, sy); qhy = quant_hi(hi.y, nlo.y, sy);
-            qlz = quant_lo(lo.z, nlo.z, sz); qhz = quant_hi(hi.z, nlo.z, sz);
             if (internal) {
                 meta = 0xFFu;
                 qout[qbase + rank] = make_uint2(ref, child_base + rank);
             } else {
                 meta = (cnt << 5) | toff;
-                const uint32_t first = ref_first(bt, ref);
-                for (uint32_t k = 0; k < cnt; k++) {
-                    const uint32_t prim = ids[first + k];
-                    const uint32_t dst = tri_base + toff + k;
-                    if (out.tris) {
-                        const uint32_t i0 = src.tri[3 * (uint64_t)prim], i1 = src.tri[3 * (uint64_t)prim + 1],
-                                       i2 = src.tri[3 * (uint64_t)prim + 2];
-                        const float4 a = load_vertex(src, i0), b = load_vertex(src, i1), c = load_vertex(src, i2);
-                        float4 *tr = reinterpret_cast<float4 *>(out.tris + dst);
-                        tr[0] = make_float4(a.x, a.y, a.z, __uint_as_float(prim));
-                        tr[1] = make_float4(b.x, b.y, b.z, 0.0f);
-                        tr[2] = make_float4(c.x, c.y, c.z, 0.0f);
-                    } else {
-                        out.prim_out[dst] = prim;
-                    }
-                }
+                uint32_t prim[kLeafMax];
+                for (uint32_t k = 0; k < kLeafMax; k++) prim[k] = k < cnt ? ids[first + k] : 0u;
+                for (uint32_t k = 0; k < kLeafMax; k++)
+                    if (k < cnt) out.prim_out[tri_base + toff + k] = prim[k];
             }
         }
-        // ---- assemble the node words by OR-reducing each lane's bytes at its slot position
-        const uint32_t sh = valid ? 8u * (slot & 3u) : 0u;
-        const bool hi_half = valid && slot >= 4u;
-        auto word = [&](uint32_t byte, bool upper) -> uint32_t {
-            const uint32_t c = valid ? (byte << sh) : 0u;
-            return g8_or((upper == hi_half) ? c : 0u);
-        };
-        const uint32_t occ = g8_or(valid ? (1u << slot) : 0u);
-        // empty slots: qlo = 255, qhi = 0, meta = 0 (never hit)
-        uint32_t empty_lo = 0u, empty_hi = 0u;
-        for (uint32_t s = 0; s < 8; s++)
-            if (!(occ & (1u << s))) {
-                if (s < 4) empty_lo |= 0xFFu << (8u * s); else empty_hi |= 0xFFu << (8u * (s - 4));
-            }
-        const uint32_t m0 = word(meta, false), m1 = word(meta, true);
-        const uint32_t lx0 = word(qlx, false) | empty_lo, lx1 = word(qlx, true) | empty_hi;
-        const uint32_t ly0 = word(qly, false) | empty_lo, ly1 = word(qly, true) | empty_hi;
-        const uint32_t lz0 = word(qlz, false) | empty_lo, lz1 = word(qlz, true) | empty_hi;
-        const uint32_t hx0 = word(qhx, false), hx1 = word(qhx, true);
-        const uint32_t hy0 = word(qhy, false), hy1 = word(qhy, true);
-        const uint32_t hz0 = word(qhz, false), hz1 = word(qhz, true);
-        if (active && lane < 5u) {
-            uint4 w;
-            if (lane == 0u) w = make_uint4(__float_as_uint(nlo.x), __float_as_uint(nlo.y), __float_as_uint(nlo.z),
-                                           ex | (ey << 8) | (ez << 16) | (imask << 24));
-            else if (lane == 1u) w = make_uint4(child_base, tri_base, m0, m1);
-            else if (lane == 2u) w = make_uint4(lx0, lx1, ly0, ly1);
-            else if (lane == 3u) w = make_uint4(lz0, lz1, hx0, hx1);
-            else w = make_uint4(hy0, hy1, hz0, hz1);
-            reinterpret_cast<uint4 *>(out.nodes + item.y)[lane] = w;
+        // ---- assemble the node in LDS: header words from lane 0, each child's bytes at its slot;
+        // empty slots keep qlo = 255, qhi = 0, meta = 0 (never hit). Lanes 0..4 store the 80 B.
+        uint32_t *nw = s_node[threadIdx.x >> 3];
+        uint8_t *nb = reinterpret_cast<uint8_t *>(nw);
+        for (uint32_t w = lane; w < 20u; w += 8u) nw[w] = (w >= 8u && w < 14u) ? 0xFFFFFFFFu : 0u;
+        __syncthreads();
+        if (lane == 0u) {
+            nw[0] = __float_as_uint(nlo.x);
+            nw[1] = __float_as_uint(nlo.y);
+            nw[2] = __float_as_uint(nlo.z);
+            nw[3] = ex | (ey << 8) | (ez << 16) | (imask << 24);
+            nw[4] = child_base;
+            nw[5] = tri_base;
         }
+        if (valid) {
+            nb[24 + slot] = (uint8_t)meta;
+            nb[32 + slot] = (uint8_t)quant_lo(lo.x, nlo.x, sx);
+            nb[40 + slot] = (uint8_t)quant_lo(lo.y, nlo.y, sy);
+            nb[48 + slot] = (uint8_t)quant_lo(lo.z, nlo.z, sz);
+            nb[56 + slot] = (uint8_t)quant_hi(hi.x, nlo.x, sx);
+            nb[64 + slot] = (uint8_t)quant_hi(hi.y, nlo.y, sy);
+            nb[72 + slot] = (uint8_t)quant_hi(hi.z, nlo.z, sz);
+        }
+        __syncthreads();
+        if (active && lane < 5u)
+            reinterpret_cast<uint4 *>(out.nodes + item.y)[lane] = reinterpret_cast<const uint4 *>(nw)[lane];
+        __syncthreads();
+    }
+}
+
+// Totals of the per-level words (one thread).
+__global__ void k_finish_counts(BuildCounters *ctr) {
+    uint32_t nodes = 0, tris = 0;
+    for (uint32_t l = 0; l < kMaxCollapseLevels; l++) {
+        const uint64_t w = ctr->level[l];
+        nodes += (uint32_t)(w >> 32);
+        tris += (uint32_t)w;
+    }
+    ctr->node_count = nodes;
+    ctr->tri_count = tris;
+}
+
+// Triangle records in leaf order: record i holds the vertices of primitive order[i] (prim id in
+// v0.w). One thread per record: coalesced 48-B writes, gathered vertex reads.
+__global__ __launch_bounds__(kBlock) void k_tri_records(const uint32_t *order, PrimSource src, TriRecord *tris,
+                                                        uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t prim = order[i];
+        const uint32_t i0 = src.tri[3 * (uint64_t)prim], i1 = src.tri[3 * (uint64_t)prim + 1],
+                       i2 = src.tri[3 * (uint64_t)prim + 2];
+        const float4 a = load_vertex(src, i0), b = load_vertex(src, i1), c = load_vertex(src, i2);
+        float4 *tr = reinterpret_cast<float4 *>(tris + i);
+        tr[0] = make_float4(a.x, a.y, a.z, __uint_as_float(prim));
+        tr[1] = make_float4(b.x, b.y, b.z, 0.0f);
+        tr[2] = make_float4(c.x, c.y, c.z, 0.0f);
     }
 }
 
@@ -690,7 +809,7 @@ size_t bvh_scratch_bytes(uint32_t n) {
     b += m * sizeof(uint2) + 256;                   // child
     b += m * sizeof(uint2) + 256;                   // range
     b += m * sizeof(uint32_t) + 256;                // slots
-    b += 2 * m * sizeof(uint32_t) + 256 * 2;        // parents
+    b += 2 * m * sizeof(uint2) + 256 * 2;           // parent links
     b += 3 * m * sizeof(uint32_t) + 256 * 3;        // pend + two entry lists
     b += 2 * m * sizeof(uint2) + 256 * 2;           // collapse queues
     b += sizeof(BuildCounters) + 256;               // counters
@@ -739,8 +858,8 @@ hipError_t bvh_build(hipStream_t stream, void *scratch, size_t scratch_bytes, co
     bt.child = carve<uint2>(p, m);
     bt.range = carve<uint2>(p, m);
     bt.slots = carve<uint32_t>(p, m);
-    bt.parent_int = carve<uint32_t>(p, m);
-    bt.parent_leaf = carve<uint32_t>(p, m);
+    bt.parent_int = carve<uint2>(p, m);
+    bt.parent_leaf = carve<uint2>(p, m);
     uint32_t *pend = carve<uint32_t>(p, m);
     uint32_t *ent[2] = {carve<uint32_t>(p, m), carve<uint32_t>(p, m)};
     uint2 *q0 = carve<uint2>(p, m), *q1 = carve<uint2>(p, m);
@@ -774,7 +893,8 @@ hipError_t bvh_build(hipStream_t stream, void *scratch, size_t scratch_bytes, co
         hipLaunchKernelGGL(k_fill_u32, dim3(std::min<uint32_t>(grid, 4096)), dim3(kBlock), 0, stream, pend, (uint64_t)n,
                            kNoEntry);
         // Bottom-up phases. Entries of phase 0 are the leaves in sorted order (entries == null).
-        const uint32_t kPhaseBlock = 256, kLastBlock = 1024;
+        constexpr uint32_t kPhaseBlock = 256, kLastBlock = 1024, kLeafRun = 512;
+        static_assert(kLastBlock >= kLeafRun, "the leaf phase needs n > its run length");
         const uint32_t *entries = nullptr;
         uint32_t count = n;
         int which = 0;
@@ -785,8 +905,12 @@ hipError_t bvh_build(hipStream_t stream, void *scratch, size_t scratch_bytes, co
                                    pend, ctr, q0, 1u);
                 break;
             }
-            hipLaunchKernelGGL(k_bounds_phase<false>, dim3((count + kPhaseBlock - 1) / kPhaseBlock), dim3(kPhaseBlock), 0,
-                               stream, pbox, n, ids, bt, entries, count, pend, ctr, q0, 0u);
+            if (!entries)
+                hipLaunchKernelGGL(k_bounds_leaves<kLeafRun>, dim3((n + kLeafRun - 1) / kLeafRun), dim3(kLeafRun), 0,
+                                   stream, pbox, n, ids, bt, pend);
+            else
+                hipLaunchKernelGGL(k_bounds_phase<false>, dim3((count + kPhaseBlock - 1) / kPhaseBlock),
+                                   dim3(kPhaseBlock), 0, stream, pbox, n, ids, bt, entries, count, pend, ctr, q0, 0u);
             tb = temp_bytes;
             e = rocprim::select(temp, tb, (const uint32_t *)pend, ent[which], dcount, n, IsEntry{}, stream);
             if (e != hipSuccess) return e;
@@ -809,26 +933,33 @@ hipError_t bvh_build(hipStream_t stream, void *scratch, size_t scratch_bytes, co
             count = next;
         }
     }
-    WideOut out{nodes, tris, prim_out};
+    // triangle BLAS: leaf primitive ids go to `pend` (free after the bottom-up phases) and become
+    // triangle records in one pass after the last level
+    WideOut out{nodes, tris, tris ? pend : prim_out};
     // A root-to-leaf path of the radix tree crosses each of the 30 key bits and 32 index bits at
     // most once, and every wide level descends at least one binary level, so the wide depth is
     // at most 63 < kCollapseLaunches. Levels are issued in chunks; after each chunk the work
     // count of the next level is read back and the loop stops once it is zero. The host checks
-    // level_count[kCollapseLaunches] == 0 after the build.
+    // that level kCollapseLaunches holds no nodes after the build.
     const uint32_t kChunk = 6;
     for (uint32_t level = 0; level < kCollapseLaunches; level++) {
         const uint2 *qin = (level & 1) ? q1 : q0;
         uint2 *qout = (level & 1) ? q0 : q1;
-        hipLaunchKernelGGL(k_collapse, dim3(grid_collapse), dim3(kBlock), 0, stream, pbox, ids, src, bt, ctr, level, qin,
+        hipLaunchKernelGGL(k_collapse, dim3(grid_collapse), dim3(kBlock), 0, stream, pbox, ids, bt, ctr, level, qin,
                            qout, out, split_small);
         if ((level + 1) % kChunk == 0 && level + 1 < kCollapseLaunches) {
-            e = hipMemcpyAsync(pinned_word, &ctr->level_count[level + 1], sizeof(uint32_t), hipMemcpyDeviceToHost, stream);
+            e = hipMemcpyAsync(pinned_word, level_nodes_word(ctr, level + 1), sizeof(uint32_t), hipMemcpyDeviceToHost,
+                               stream);
             if (e != hipSuccess) return e;
             e = hipStreamSynchronize(stream);
             if (e != hipSuccess) return e;
             if (*pinned_word == 0) break;
         }
     }
+    hipLaunchKernelGGL(k_finish_counts, dim3(1), dim3(1), 0, stream, ctr);
+    if (tris)
+        hipLaunchKernelGGL(k_tri_records, dim3(std::min<uint32_t>(grid, 1u << 20)), dim3(kBlock), 0, stream, pend, src,
+                           tris, n);
     return hipGetLastError();
 }
 

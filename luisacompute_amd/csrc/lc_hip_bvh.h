@@ -13,6 +13,7 @@
 // along axis k); a ray visits slots in order i ^ oct(ray), which approximates front to back.
 #pragma once
 
+#include <stddef.h>
 #include <stdint.h>
 
 namespace lchip {
@@ -54,18 +55,26 @@ static_assert(sizeof(InstanceRecord) == 128, "InstanceRecord must be 128 bytes")
 
 constexpr uint32_t kLeafMax = 3;         // triangles per leaf slot (<= 3 keeps a node's tris in 24 bits)
 constexpr uint32_t kCollapseLaunches = 98;   // wide levels launched per build (depth <= 96)
-constexpr uint32_t kMaxCollapseLevels = 100; // level_count entries
+constexpr uint32_t kMaxCollapseLevels = 100; // per-level counter words
 constexpr uint32_t kLeafBit = 0x80000000u; // binary-node reference: leaf (sorted position)
 
 // Per-build device-side counters (one per BLAS/TLAS build).
 struct BuildCounters {
-    uint32_t node_count;  // wide nodes allocated (root = 0)
-    uint32_t tri_count;   // triangle records written
+    uint32_t node_count;  // wide nodes (root = 0), summed from the levels after the collapse
+    uint32_t tri_count;   // triangle records, summed likewise
     uint32_t depth;       // wide levels that held work
     uint32_t root_ref;    // binary root reference
     int32_t cbounds[6];   // centroid bounds as order-preserving ints
     float bounds[6];      // root box lo.xyz, hi.xyz
-    uint32_t level_count[kMaxCollapseLevels];
+    // wide level l: (nodes << 32) | triangle records of its leaves; one 64-bit atomic reserves
+    // both. On the little-endian device the node count is the word's upper 32-bit half.
+    uint64_t level[kMaxCollapseLevels];
 };
+static_assert(offsetof(BuildCounters, level) % 8 == 0, "level words must be 8-B aligned");
+
+// Address of level l's node count (the upper half of its word), for 4-byte device-to-host reads.
+inline uint32_t *level_nodes_word(BuildCounters *c, uint32_t l) {
+    return reinterpret_cast<uint32_t *>(&c->level[l]) + 1;
+}
 
 } // namespace lchip

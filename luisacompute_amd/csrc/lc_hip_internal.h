@@ -264,7 +264,7 @@ struct HostCounters {
     uint32_t node_count, tri_count, depth, root_ref;
     int32_t cbounds[6];
     float bounds[6];
-    uint32_t tail_level_count; // level_count[kCollapseLaunches]: must be 0
+    uint32_t tail_level_count; // nodes at wide level kCollapseLaunches: must be 0
     uint32_t error;
     uint32_t phase_count;      // bottom-up phase entry count read back during a build
 };

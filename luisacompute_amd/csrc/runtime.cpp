@@ -174,8 +174,8 @@ void invert_affine(const float m[12], float out[12]) {
 }
 
 void copy_counters_async(BuildCounters *dctr, HostCounters *host, uint32_t *error, hipStream_t s) {
-    hipMemcpyAsync(host, dctr, offsetof(BuildCounters, level_count), hipMemcpyDeviceToHost, s);
-    hipMemcpyAsync(&host->tail_level_count, &dctr->level_count[kCollapseLaunches], sizeof(uint32_t),
+    hipMemcpyAsync(host, dctr, offsetof(BuildCounters, level), hipMemcpyDeviceToHost, s);
+    hipMemcpyAsync(&host->tail_level_count, level_nodes_word(dctr, kCollapseLaunches), sizeof(uint32_t),
                    hipMemcpyDeviceToHost, s);
     hipMemcpyAsync(&host->error, error, sizeof(uint32_t), hipMemcpyDeviceToHost, s);
 }
@@ -570,7 +570,7 @@ int lc_hip_mesh_build(uint64_t device, uint64_t stream, uint64_t mesh, uint32_t,
         hipError_t e = bvh_build(s, m->scratch, m->scratch_bytes, static_cast<char *>(vb->ptr) + vertex_offset,
                                  vertex_stride, nverts,
                                  reinterpret_cast<const uint32_t *>(static_cast<char *>(tb->ptr) + triangle_offset),
-                                 nullptr, n, m->nodes, m->tris, nullptr, &ctr, dev->split_small, 4 * dev->cu_count,
+                                 nullptr, n, m->nodes, m->tris, nullptr, &ctr, dev->split_small, 5 * dev->cu_count,
                                  dev->error, &m->host->phase_count);
         if (e != hipSuccess) return hip_fail(e, "bvh_build");
         copy_counters_async(ctr, m->host, dev->error, s);

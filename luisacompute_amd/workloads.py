@@ -53,6 +53,81 @@ class Instanced:
 
 
 # ---------------------------------------------------------------------------------------------
+# Serving workload: four resident scenes and a Poisson request stream.
+RESOLUTIONS = (256, 512, 1024, 2048)
+SPPS = (1, 4, 16, 64)
+
+
+def serving_requests(n, rate, initstate=42, initseq=5):
+    """n requests (PCG32 stream (initstate, initseq), 4 draws each): scene floor(4u), square
+    resolution from RESOLUTIONS, spp from SPPS, Poisson arrivals at `rate` per second
+    (exponential gaps -ln(1 - u) / rate). Returns a list of dicts with the arrival time."""
+    u = pcg32_floats(initstate, initseq, 4 * n).astype(np.float64).reshape(n, 4)
+    t = 0.0
+    out = []
+    for k in range(n):
+        t += -np.log1p(-u[k, 3]) / rate if rate > 0 else 0.0
+        out.append(dict(index=k, scene=int(u[k, 0] * 4), res=RESOLUTIONS[int(u[k, 1] * 4)], spp=SPPS[int(u[k, 2] * 4)],
+                        arrival=t, sequence=(5 << 32) + (k << 22)))
+    return out
+
+
+def pt_scene(light_position, light_u, light_v, emission, origin, right, up, forward):
+    from . import PtScene
+    s = PtScene()
+    for name, val in (("light_position", light_position), ("light_u", light_u), ("light_v", light_v),
+                      ("light_emission", emission), ("camera_origin", origin), ("camera_right", right),
+                      ("camera_up", up), ("camera_forward", forward)):
+        getattr(s, name)[:] = [float(np.float32(x)) for x in val]
+    return s
+
+
+def serving_scenes(device, stream, small=False):
+    """The four resident scenes of the serving workload: (name, accel, pt_instances, PtScene, keep).
+    icosphere L6 (C1), the 16,777,216-triangle soup (C2), the box room (C3) and the 1,024-instance
+    TLAS (C4); each lit by a quad light above it and viewed from outside (the room from inside).
+    small=True scales every scene down for tests."""
+    from . import PtInstance, host
+    scenes = []
+    # 1. icosphere
+    v, t = icosphere(3 if small else 6)
+    acc = host.single_mesh_accel(device, stream, v, t)
+    inst = [PtInstance()]
+    inst[0].albedo[:] = [0.7, 0.5, 0.3]
+    scenes.append(("icosphere", acc, inst, pt_scene((-1, 3, -1), (2, 0, 0), (0, 0, 2), (10, 10, 10),
+                                                    (0, 0, 3.5), (0.5, 0, 0), (0, 0.5, 0), (0, 0, -1))))
+    # 2. soup
+    ntri = 20000 if small else 1 << 24
+    v, t = soup(ntri, initstate=42, initseq=2)
+    acc = host.single_mesh_accel(device, stream, v, t)
+    inst = [PtInstance()]
+    inst[0].albedo[:] = [0.5, 0.5, 0.5]
+    scenes.append(("soup", acc, inst, pt_scene((20, 150, 20), (60, 0, 0), (0, 0, 60), (20, 20, 20),
+                                               (50, 50, -60), (-0.6, 0, 0), (0, 0.6, 0), (0, 0, 1))))
+    # 3. box room
+    room = BoxRoom(wall_quads=16, block_quads=4) if small else BoxRoom()
+    acc, _, inst = room.build(device, stream)
+    scenes.append(("box room", acc, inst, room.scene()))
+    # 4. instanced
+    w = Instanced(small=small)
+    gm = [host.mesh_from_arrays(device, stream, mv, mt) for mv, mt in w.meshes]
+    acc = device.create_accel()
+    for aff, k in zip(w.affines, w.mesh_ids):
+        acc.emplace_back(gm[int(k)], aff)
+    stream << acc.build()
+    acc._keep = gm
+    alb = 0.1 + 0.8 * pcg32_floats(42, 6, 3 * len(w.mesh_ids)).astype(np.float64).reshape(-1, 3)
+    inst = []
+    for k in range(len(w.mesh_ids)):
+        p = PtInstance()
+        p.albedo[:] = [float(x) for x in alb[k]]
+        inst.append(p)
+    scenes.append(("instanced", acc, inst, pt_scene((-50, 120, -50), (100, 0, 0), (0, 0, 100), (5, 5, 5),
+                                                    (0, 0, -140), (-0.5, 0, 0), (0, 0.5, 0), (0, 0, 1))))
+    return scenes
+
+
+# ---------------------------------------------------------------------------------------------
 # Box room of the path-tracing workload.
 def pcg32_floats(initstate, initseq, n):
     """n PCG32 floats (x >> 8) * 2^-24 of stream (initstate, initseq) (O'Neill 2014)."""

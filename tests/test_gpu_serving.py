@@ -64,7 +64,9 @@ def test_requests_in_the_mix_equal_alone(device, stream, scenes):
     srv.drain(timeout=300)
     st = srv.stats()
     assert st.requests_done == len(reqs)
-    assert 1 <= st.peak_active <= 3            # the budget admits at most three 64x48 requests
+    # all 12 accumulators (about 360 KB) never fit the budget at once; small requests may stay
+    # admitted alongside larger ones up to max_active
+    assert 1 <= st.peak_active <= 8
     assert st.peak_bytes <= 3 * 64 * 48 * 16 + 4096
     assert st.waves >= 2
     for rid, sc, w, h, spp, seed, seq, buf in reqs:
