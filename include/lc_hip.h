@@ -227,7 +227,7 @@ typedef struct lc_hip_pt_options {
     uint32_t sort;            /* wavefront: bucket-sort continuation rays (origin cell, octant) */
     uint32_t graph;           /* wavefront: capture the launches in a hipGraph, replay when unchanged */
     uint32_t fetch_chunk;     /* wavefront traces: queue slots a wave claims per atomic (0: exact need) */
-    uint32_t fetch_segments;  /* wavefront traces: 1, or 8 (a queue segment per XCD); 0 = chunk 256, 8 */
+    uint32_t fetch_segments;  /* wavefront traces: 1, or 8 (a queue segment per XCD); 0 = default (exact, 1) */
 } lc_hip_pt_options;
 /* image: float4 per pixel = (mean radiance of the samples, 1), row-major. counters: 0 or a buffer
    of two uint64 that receives (closest-hit rays traced, shadow rays traced). */

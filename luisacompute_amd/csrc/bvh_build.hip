@@ -33,6 +33,8 @@ namespace lchip {
 namespace {
 
 constexpr int kBlock = 256;
+// Collapse workgroup: groups of 8 lanes, one queue/triangle reservation per workgroup iteration.
+constexpr int kCollapseBlock = 256;
 constexpr uint32_t kNoEntry = 0xFFFFFFFFu;
 
 __device__ __forceinline__ int f2ord(float f) {
@@ -457,11 +459,11 @@ __device__ __forceinline__ uint32_t ord_u32(float f) {
 // node index is (nodes of levels 0..l) + its queue slot. Queue slots and triangle ranges are
 // reserved with one atomic per wave. Leaf slots write their primitive ids to out.prim_out in
 // triangle-record order; a triangle BLAS turns them into records afterwards (k_tri_records).
-__global__ __launch_bounds__(kBlock) void k_collapse(const float4 *pbox, const uint32_t *ids, BinaryTree bt,
+__global__ __launch_bounds__(kCollapseBlock) void k_collapse(const float4 *pbox, const uint32_t *ids, BinaryTree bt,
                                                      BuildCounters *ctr, uint32_t level, const uint2 *qin,
                                                      uint2 *qout, WideOut out, uint32_t split_small) {
-    __shared__ __align__(16) uint32_t s_node[kBlock / 8][20];
-    __shared__ uint32_t s_wave[2][kBlock / 64];
+    __shared__ __align__(16) uint32_t s_node[kCollapseBlock / 8][20];
+    __shared__ uint32_t s_wave[2][kCollapseBlock / 64];
     __shared__ unsigned long long s_base;
     const uint32_t count = (uint32_t)(ctr->level[level] >> 32);
     if (count == 0) return;
@@ -591,7 +593,7 @@ __global__ __launch_bounds__(kBlock) void k_collapse(const float4 *pbox, const u
         __syncthreads();
         if (threadIdx.x == 0) {
             uint32_t bi = 0, btri = 0;
-            for (uint32_t w = 0; w < kBlock / 64; w++) {
+            for (uint32_t w = 0; w < kCollapseBlock / 64; w++) {
                 bi += s_wave[0][w];
                 btri += s_wave[1][w];
             }
@@ -846,8 +848,15 @@ hipError_t bvh_validate(hipStream_t stream, const WideNode *nodes, uint32_t node
 hipError_t bvh_build(hipStream_t stream, void *scratch, size_t scratch_bytes, const void *vbase, uint64_t vstride,
                      uint64_t nverts, const uint32_t *tri, const float4 *boxes, uint32_t n, WideNode *nodes,
                      TriRecord *tris, uint32_t *prim_out, BuildCounters **counters_out, uint32_t split_small,
-                     int grid_collapse, uint32_t *error_flag, uint32_t *pinned_word) {
+                     int cu_count, uint32_t *error_flag, uint32_t *pinned_word) {
     if (scratch_bytes < bvh_scratch_bytes(n)) return hipErrorInvalidValue;
+    // the collapse grid fills the CUs once (its loop strides over the level's items)
+    static const int collapse_blocks_per_cu = [] {
+        int b = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_collapse, kCollapseBlock, 0) != hipSuccess || b < 1) b = 1;
+        return b;
+    }();
+    const int grid_collapse = std::max(1, cu_count) * collapse_blocks_per_cu;
     size_t m = std::max<uint32_t>(n, 2u);
     char *p = static_cast<char *>(scratch);
     uint32_t *k0 = carve<uint32_t>(p, m), *k1 = carve<uint32_t>(p, m);
@@ -945,7 +954,7 @@ hipError_t bvh_build(hipStream_t stream, void *scratch, size_t scratch_bytes, co
     for (uint32_t level = 0; level < kCollapseLaunches; level++) {
         const uint2 *qin = (level & 1) ? q1 : q0;
         uint2 *qout = (level & 1) ? q0 : q1;
-        hipLaunchKernelGGL(k_collapse, dim3(grid_collapse), dim3(kBlock), 0, stream, pbox, ids, bt, ctr, level, qin,
+        hipLaunchKernelGGL(k_collapse, dim3(grid_collapse), dim3(kCollapseBlock), 0, stream, pbox, ids, bt, ctr, level, qin,
                            qout, out, split_small);
         if ((level + 1) % kChunk == 0 && level + 1 < kCollapseLaunches) {
             e = hipMemcpyAsync(pinned_word, level_nodes_word(ctr, level + 1), sizeof(uint32_t), hipMemcpyDeviceToHost,

@@ -24,7 +24,7 @@ size_t bvh_node_capacity(uint32_t n);
 hipError_t bvh_build(hipStream_t stream, void *scratch, size_t scratch_bytes, const void *vbase, uint64_t vstride,
                      uint64_t nverts, const uint32_t *tri, const float4 *boxes, uint32_t n, WideNode *nodes,
                      TriRecord *tris, uint32_t *prim_out, BuildCounters **counters_out, uint32_t split_small,
-                     int grid_collapse, uint32_t *error_flag, uint32_t *pinned_word);
+                     int cu_count, uint32_t *error_flag, uint32_t *pinned_word);
 hipError_t bvh_validate(hipStream_t stream, const WideNode *nodes, uint32_t node_count, const TriRecord *tris,
                         uint32_t tri_count, uint32_t nprims, uint32_t *marks, uint32_t *err7);
 

@@ -570,7 +570,7 @@ int lc_hip_mesh_build(uint64_t device, uint64_t stream, uint64_t mesh, uint32_t,
         hipError_t e = bvh_build(s, m->scratch, m->scratch_bytes, static_cast<char *>(vb->ptr) + vertex_offset,
                                  vertex_stride, nverts,
                                  reinterpret_cast<const uint32_t *>(static_cast<char *>(tb->ptr) + triangle_offset),
-                                 nullptr, n, m->nodes, m->tris, nullptr, &ctr, dev->split_small, 5 * dev->cu_count,
+                                 nullptr, n, m->nodes, m->tris, nullptr, &ctr, dev->split_small, dev->cu_count,
                                  dev->error, &m->host->phase_count);
         if (e != hipSuccess) return hip_fail(e, "bvh_build");
         copy_counters_async(ctr, m->host, dev->error, s);
@@ -1044,8 +1044,9 @@ static std::vector<PtInstance> pt_table_from(Accel *a, const lc_hip_pt_instance 
 
 static bool pt_fetch_from(uint32_t chunk, uint32_t segments, PtOptions &po) {
     if ((segments > 1 && segments != kFetchSegments) || chunk > 4096 || chunk % 64 != 0) return false;
-    po.fetch_chunk = segments == 0 ? 256u : chunk;
-    po.fetch_segments = segments == 0 ? kFetchSegments : segments;
+    // default: exact claims from one counter (the per-bounce queues are sorted already)
+    po.fetch_chunk = segments == 0 ? 0u : chunk;
+    po.fetch_segments = segments == 0 ? 1u : segments;
     return true;
 }
 
