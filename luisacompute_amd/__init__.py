@@ -47,6 +47,12 @@ class AccelOption(ctypes.Structure):
     FAST_BUILD = 1
 
 
+# AccelBuildRequest (include/luisa/runtime/rhi/command.h:402-405); build() defaults to
+# PREFER_UPDATE like Mesh::build / Accel::build (include/luisa/runtime/rtx/mesh.h:120)
+BUILD_PREFER_UPDATE = 0
+BUILD_FORCE_BUILD = 1
+
+
 class Modification(ctypes.Structure):
     """AccelBuildCommand::Modification (include/luisa/runtime/rhi/command.h:534-551)."""
     _fields_ = [("index", ctypes.c_uint32), ("user_id", ctypes.c_uint32), ("flags", ctypes.c_uint32),
@@ -503,7 +509,9 @@ class Mesh:
             lib().lc_hip_mesh_destroy(self.device.handle, self.handle)
             self.handle = None
 
-    def build(self, request=1):
+    def build(self, request=BUILD_PREFER_UPDATE):
+        """Builds, or refits when the mesh allows updates, was built before over the same buffer
+        ranges, and the request is PREFER_UPDATE."""
         return _MeshBuild(self, request)
 
     def stats(self):
@@ -587,7 +595,7 @@ class Accel:
         self._mods = {}
         return mods
 
-    def build(self, request=1):
+    def build(self, request=BUILD_PREFER_UPDATE):
         if not self._meshes:
             raise LCError("Building acceleration structure without instances.")  # accel.cpp:72-73
         return _AccelBuild(self, request, self._take_mods(), len(self._meshes), False)

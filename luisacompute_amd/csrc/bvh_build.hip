@@ -685,6 +685,107 @@ __global__ __launch_bounds__(kBlock) void k_tri_records(const uint32_t *order, P
     }
 }
 
+// ---- refit: new vertex positions, same topology (a MeshBuildCommand that updates, see
+// src/backends/cuda/cuda_mesh.cpp:30-69). Every record reloads its triangle's vertices; then the
+// wide levels, deepest first, recompute each slot box from the child's exact box (internal) or
+// the leaf's records, and requantise against the new node box. Slot order, child and triangle
+// bases and leaf ranges are kept.
+
+// Records: the primitive id in v0.w names the triangle to reload.
+__global__ __launch_bounds__(kBlock) void k_refit_records(PrimSource src, TriRecord *tris, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        float4 *tr = reinterpret_cast<float4 *>(tris + i);
+        const uint32_t prim = __float_as_uint(tr[0].w);
+        const uint32_t i0 = src.tri[3 * (uint64_t)prim], i1 = src.tri[3 * (uint64_t)prim + 1],
+                       i2 = src.tri[3 * (uint64_t)prim + 2];
+        const float4 a = load_vertex(src, i0), b = load_vertex(src, i1), c = load_vertex(src, i2);
+        tr[0] = make_float4(a.x, a.y, a.z, __uint_as_float(prim));
+        tr[1] = make_float4(b.x, b.y, b.z, 0.0f);
+        tr[2] = make_float4(c.x, c.y, c.z, 0.0f);
+    }
+}
+
+// One wide level; 8 lanes per node, lane k = slot k. exact[2 i], exact[2 i + 1]: node i's box.
+__global__ __launch_bounds__(kCollapseBlock) void k_refit_level(BuildCounters *ctr, uint32_t level, WideNode *nodes,
+                                                                const TriRecord *tris, float4 *exact) {
+    __shared__ __align__(16) uint32_t s_node[kCollapseBlock / 8][20];
+    const uint32_t count = (uint32_t)(ctr->level[level] >> 32);
+    uint32_t start = 0;
+    for (uint32_t j = 0; j < level; j++) start += (uint32_t)(ctr->level[j] >> 32);
+    const uint32_t lane = threadIdx.x & 7u;
+    const uint32_t groups = blockDim.x >> 3;
+    for (uint32_t base = blockIdx.x * groups; base < count; base += gridDim.x * groups) {
+        const uint32_t item = base + (threadIdx.x >> 3);
+        const bool active = item < count;
+        const uint32_t node = start + item;
+        uint4 w0 = make_uint4(0u, 0u, 0u, 0u), w1 = make_uint4(0u, 0u, 0u, 0u);
+        if (active) {
+            const uint4 *src = reinterpret_cast<const uint4 *>(nodes + node);
+            w0 = src[0];
+            w1 = src[1];
+        }
+        const uint32_t imask = w0.w >> 24;
+        const uint32_t meta = ((lane < 4u ? w1.z : w1.w) >> (8u * (lane & 3u))) & 0xFFu;
+        float3 lo = make_float3(FLT_MAX, FLT_MAX, FLT_MAX), hi = make_float3(-FLT_MAX, -FLT_MAX, -FLT_MAX);
+        if (meta != 0u) {
+            if (imask & (1u << lane)) {
+                const uint32_t c = w1.x + (uint32_t)__popc(imask & ((1u << lane) - 1u));
+                const float4 a = exact[2 * (uint64_t)c], b = exact[2 * (uint64_t)c + 1];
+                lo = make_float3(a.x, a.y, a.z);
+                hi = make_float3(b.x, b.y, b.z);
+            } else {
+                const uint32_t cnt = meta >> 5, off = meta & 31u;
+                for (uint32_t k = 0; k < cnt; k++) {
+                    const float4 *tr = reinterpret_cast<const float4 *>(tris + w1.y + off + k);
+                    for (int v = 0; v < 3; v++) {
+                        const float4 p = tr[v];
+                        lo = make_float3(fminf(lo.x, p.x), fminf(lo.y, p.y), fminf(lo.z, p.z));
+                        hi = make_float3(fmaxf(hi.x, p.x), fmaxf(hi.y, p.y), fmaxf(hi.z, p.z));
+                    }
+                }
+            }
+        }
+        const float3 nlo = make_float3(g8_min(lo.x), g8_min(lo.y), g8_min(lo.z));
+        const float3 nhi = make_float3(g8_max(hi.x), g8_max(hi.y), g8_max(hi.z));
+        const uint32_t ex = pick_exponent(nlo.x, nhi.x), ey = pick_exponent(nlo.y, nhi.y), ez = pick_exponent(nlo.z, nhi.z);
+        const float sx = __int_as_float((int)(ex << 23)), sy = __int_as_float((int)(ey << 23)), sz = __int_as_float((int)(ez << 23));
+        uint32_t *nw = s_node[threadIdx.x >> 3];
+        uint8_t *nb = reinterpret_cast<uint8_t *>(nw);
+        for (uint32_t w = 8u + lane; w < 20u; w += 8u) nw[w] = w < 14u ? 0xFFFFFFFFu : 0u;
+        if (lane == 0u) {
+            nw[0] = __float_as_uint(nlo.x);
+            nw[1] = __float_as_uint(nlo.y);
+            nw[2] = __float_as_uint(nlo.z);
+            nw[3] = ex | (ey << 8) | (ez << 16) | (imask << 24);
+            nw[4] = w1.x;
+            nw[5] = w1.y;
+            nw[6] = w1.z;
+            nw[7] = w1.w;
+        }
+        __syncthreads();
+        if (active && meta != 0u) {
+            nb[32 + lane] = (uint8_t)quant_lo(lo.x, nlo.x, sx);
+            nb[40 + lane] = (uint8_t)quant_lo(lo.y, nlo.y, sy);
+            nb[48 + lane] = (uint8_t)quant_lo(lo.z, nlo.z, sz);
+            nb[56 + lane] = (uint8_t)quant_hi(hi.x, nlo.x, sx);
+            nb[64 + lane] = (uint8_t)quant_hi(hi.y, nlo.y, sy);
+            nb[72 + lane] = (uint8_t)quant_hi(hi.z, nlo.z, sz);
+        }
+        __syncthreads();
+        if (active && lane < 5u)
+            reinterpret_cast<uint4 *>(nodes + node)[lane] = reinterpret_cast<const uint4 *>(nw)[lane];
+        if (active && lane == 0u) {
+            exact[2 * (uint64_t)node] = make_float4(nlo.x, nlo.y, nlo.z, 0.0f);
+            exact[2 * (uint64_t)node + 1] = make_float4(nhi.x, nhi.y, nhi.z, 0.0f);
+            if (node == 0u) {
+                ctr->bounds[0] = nlo.x; ctr->bounds[1] = nlo.y; ctr->bounds[2] = nlo.z;
+                ctr->bounds[3] = nhi.x; ctr->bounds[4] = nhi.y; ctr->bounds[5] = nhi.z;
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // ---- validation (tests and debugging): structural and containment invariants of a built tree.
 // err[0] child index out of range      err[1] triangle range out of range
 // err[2] leaf triangle outside its slot box
@@ -783,9 +884,7 @@ T *carve(char *&p, size_t count) {
     return r;
 }
 
-} // namespace
-
-static size_t sort_temp_bytes(uint32_t n) {
+size_t sort_temp_bytes(uint32_t n) {
     size_t bytes = 0;
     rocprim::double_buffer<uint32_t> k(nullptr, nullptr);
     rocprim::double_buffer<uint32_t> v(nullptr, nullptr);
@@ -793,12 +892,49 @@ static size_t sort_temp_bytes(uint32_t n) {
     return bytes;
 }
 
-static size_t select_temp_bytes(uint32_t n) {
+size_t select_temp_bytes(uint32_t n) {
     size_t bytes = 0;
     rocprim::select(nullptr, bytes, (const uint32_t *)nullptr, (uint32_t *)nullptr, (uint32_t *)nullptr, n, IsEntry{},
                     (hipStream_t)0);
     return bytes;
 }
+
+// The scratch arrays of a build over n primitives (bvh_scratch_bytes(n) bytes). The counters
+// stay valid after the build; a refit reuses them and the cbox array.
+struct ScratchLayout {
+    uint32_t *k0, *k1, *i0, *i1;
+    float4 *pbox_own;
+    BinaryTree bt;
+    uint32_t *pend, *ent[2];
+    uint2 *q0, *q1;
+    BuildCounters *ctr;
+    uint32_t *dcount;
+    size_t temp_bytes;
+    void *temp;
+    ScratchLayout(void *scratch, uint32_t n) {
+        const size_t m = std::max<uint32_t>(n, 2u);
+        char *p = static_cast<char *>(scratch);
+        k0 = carve<uint32_t>(p, m); k1 = carve<uint32_t>(p, m);
+        i0 = carve<uint32_t>(p, m); i1 = carve<uint32_t>(p, m);
+        pbox_own = carve<float4>(p, 2 * m);
+        bt.cbox = carve<float4>(p, 4 * m);
+        bt.child = carve<uint2>(p, m);
+        bt.range = carve<uint2>(p, m);
+        bt.slots = carve<uint32_t>(p, m);
+        bt.parent_int = carve<uint2>(p, m);
+        bt.parent_leaf = carve<uint2>(p, m);
+        pend = carve<uint32_t>(p, m);
+        ent[0] = carve<uint32_t>(p, m);
+        ent[1] = carve<uint32_t>(p, m);
+        q0 = carve<uint2>(p, m); q1 = carve<uint2>(p, m);
+        ctr = carve<BuildCounters>(p, 1);
+        dcount = carve<uint32_t>(p, 1);
+        temp_bytes = std::max(sort_temp_bytes((uint32_t)m), select_temp_bytes((uint32_t)m));
+        temp = carve<char>(p, temp_bytes);
+    }
+};
+
+} // namespace
 
 // Scratch bytes for building over n primitives (everything but the output arrays).
 size_t bvh_scratch_bytes(uint32_t n) {
@@ -857,25 +993,17 @@ hipError_t bvh_build(hipStream_t stream, void *scratch, size_t scratch_bytes, co
         return b;
     }();
     const int grid_collapse = std::max(1, cu_count) * collapse_blocks_per_cu;
-    size_t m = std::max<uint32_t>(n, 2u);
-    char *p = static_cast<char *>(scratch);
-    uint32_t *k0 = carve<uint32_t>(p, m), *k1 = carve<uint32_t>(p, m);
-    uint32_t *i0 = carve<uint32_t>(p, m), *i1 = carve<uint32_t>(p, m);
-    float4 *pbox_own = carve<float4>(p, 2 * m);
-    BinaryTree bt;
-    bt.cbox = carve<float4>(p, 4 * m);
-    bt.child = carve<uint2>(p, m);
-    bt.range = carve<uint2>(p, m);
-    bt.slots = carve<uint32_t>(p, m);
-    bt.parent_int = carve<uint2>(p, m);
-    bt.parent_leaf = carve<uint2>(p, m);
-    uint32_t *pend = carve<uint32_t>(p, m);
-    uint32_t *ent[2] = {carve<uint32_t>(p, m), carve<uint32_t>(p, m)};
-    uint2 *q0 = carve<uint2>(p, m), *q1 = carve<uint2>(p, m);
-    BuildCounters *ctr = carve<BuildCounters>(p, 1);
-    uint32_t *dcount = carve<uint32_t>(p, 1);
-    size_t temp_bytes = std::max(sort_temp_bytes((uint32_t)m), select_temp_bytes((uint32_t)m));
-    void *temp = carve<char>(p, temp_bytes);
+    ScratchLayout L(scratch, n);
+    uint32_t *k0 = L.k0, *k1 = L.k1, *i0 = L.i0, *i1 = L.i1;
+    float4 *pbox_own = L.pbox_own;
+    BinaryTree bt = L.bt;
+    uint32_t *pend = L.pend;
+    uint32_t *ent[2] = {L.ent[0], L.ent[1]};
+    uint2 *q0 = L.q0, *q1 = L.q1;
+    BuildCounters *ctr = L.ctr;
+    uint32_t *dcount = L.dcount;
+    const size_t temp_bytes = L.temp_bytes;
+    void *temp = L.temp;
     *counters_out = ctr;
 
     if (!boxes && n > 0 && nverts == 0) return hipErrorInvalidValue;
@@ -969,6 +1097,29 @@ hipError_t bvh_build(hipStream_t stream, void *scratch, size_t scratch_bytes, co
     if (tris)
         hipLaunchKernelGGL(k_tri_records, dim3(std::min<uint32_t>(grid, 1u << 20)), dim3(kBlock), 0, stream, pend, src,
                            tris, n);
+    return hipGetLastError();
+}
+
+// Refits a triangle BLAS built by bvh_build over the same scratch (its counters hold the level
+// sizes) for new vertex positions; depth = the build's wide depth.
+hipError_t bvh_refit(hipStream_t stream, void *scratch, size_t scratch_bytes, const void *vbase, uint64_t vstride,
+                     uint64_t nverts, const uint32_t *tri, uint32_t n, uint32_t depth, WideNode *nodes,
+                     TriRecord *tris, BuildCounters **counters_out, int cu_count, uint32_t *error_flag) {
+    if (scratch_bytes < bvh_scratch_bytes(n) || nverts == 0) return hipErrorInvalidValue;
+    ScratchLayout L(scratch, n);
+    *counters_out = L.ctr;
+    if (n == 0) return hipSuccess;
+    static const int refit_blocks_per_cu = [] {
+        int b = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_refit_level, kCollapseBlock, 0) != hipSuccess || b < 1) b = 1;
+        return b;
+    }();
+    PrimSource src{static_cast<const char *>(vbase), vstride, nverts, tri, nullptr, error_flag};
+    hipLaunchKernelGGL(k_refit_records, dim3(std::min<uint32_t>((n + kBlock - 1) / kBlock, 1u << 20)), dim3(kBlock), 0,
+                       stream, src, tris, n);
+    const int grid = std::max(1, cu_count) * refit_blocks_per_cu;
+    for (uint32_t level = depth; level-- > 0;)
+        hipLaunchKernelGGL(k_refit_level, dim3(grid), dim3(kCollapseBlock), 0, stream, L.ctr, level, nodes, tris, L.bt.cbox);
     return hipGetLastError();
 }
 

@@ -25,6 +25,9 @@ hipError_t bvh_build(hipStream_t stream, void *scratch, size_t scratch_bytes, co
                      uint64_t nverts, const uint32_t *tri, const float4 *boxes, uint32_t n, WideNode *nodes,
                      TriRecord *tris, uint32_t *prim_out, BuildCounters **counters_out, uint32_t split_small,
                      int cu_count, uint32_t *error_flag, uint32_t *pinned_word);
+hipError_t bvh_refit(hipStream_t stream, void *scratch, size_t scratch_bytes, const void *vbase, uint64_t vstride,
+                     uint64_t nverts, const uint32_t *tri, uint32_t n, uint32_t depth, WideNode *nodes,
+                     TriRecord *tris, BuildCounters **counters_out, int cu_count, uint32_t *error_flag);
 hipError_t bvh_validate(hipStream_t stream, const WideNode *nodes, uint32_t node_count, const TriRecord *tris,
                         uint32_t tri_count, uint32_t nprims, uint32_t *marks, uint32_t *err7);
 
@@ -285,7 +288,12 @@ struct Mesh {
     bool built = false;
     bool stats_valid = false;
     HostCounters stats{};
-    uint64_t build_epoch = 0;     // bumps on every build (TLAS records refer to it)
+    uint64_t build_epoch = 0;     // bumps on every build or refit (TLAS records refer to it)
+    uint64_t refits = 0;          // builds served by a refit
+    // buffer ranges of the last build: an update over the same ranges refits
+    const char *v_addr = nullptr;
+    const uint32_t *t_addr = nullptr;
+    size_t v_size = 0, v_stride = 0, t_size = 0;
 };
 
 struct InstanceHost {

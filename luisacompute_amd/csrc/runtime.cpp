@@ -525,7 +525,7 @@ void lc_hip_mesh_destroy(uint64_t device, uint64_t mesh) {
     delete m;
 }
 
-int lc_hip_mesh_build(uint64_t device, uint64_t stream, uint64_t mesh, uint32_t, uint64_t vertex_buffer,
+int lc_hip_mesh_build(uint64_t device, uint64_t stream, uint64_t mesh, uint32_t request, uint64_t vertex_buffer,
                       size_t vertex_offset, size_t vertex_size, size_t vertex_stride, uint64_t triangle_buffer,
                       size_t triangle_offset, size_t triangle_size) {
     Device *dev = obj<Device>(device);
@@ -547,6 +547,38 @@ int lc_hip_mesh_build(uint64_t device, uint64_t stream, uint64_t mesh, uint32_t,
     std::lock_guard<std::mutex> lk(m->mu);
     DeviceGuard g(dev);
     hipStream_t s = native_stream(stream);
+    const char *vaddr = static_cast<const char *>(vb->ptr) + vertex_offset;
+    const uint32_t *taddr = reinterpret_cast<const uint32_t *>(static_cast<const char *>(tb->ptr) + triangle_offset);
+    // update (refit) iff built, allowed, requested and over the same buffer ranges
+    // (src/backends/cuda/cuda_mesh.cpp:41-60); anything else rebuilds
+    const bool refit = request == LC_HIP_BUILD_PREFER_UPDATE && m->built && m->allow_update && n > 0 &&
+                       m->ntris == n && vaddr == m->v_addr && vertex_size == m->v_size &&
+                       vertex_stride == m->v_stride && taddr == m->t_addr && triangle_size == m->t_size;
+    m->v_addr = vaddr;
+    m->v_size = vertex_size;
+    m->v_stride = vertex_stride;
+    m->t_addr = taddr;
+    m->t_size = triangle_size;
+    if (refit) {
+        const int rc = mesh_stats_sync(m); // the wide depth of the build being refitted
+        if (rc != LC_HIP_OK) return rc;
+        const uint32_t depth = m->stats.depth;
+        m->stats_valid = false;
+        m->build_epoch++;
+        {
+            std::lock_guard<std::mutex> dl(dev->mu);
+            LC_HIP_TRY(hipMemsetAsync(dev->error, 0, sizeof(uint32_t), s));
+            TimedRegion tr(dev, s, "mesh_refit");
+            BuildCounters *ctr = nullptr;
+            hipError_t e = bvh_refit(s, m->scratch, m->scratch_bytes, vaddr, vertex_stride, nverts, taddr, n, depth,
+                                     m->nodes, m->tris, &ctr, dev->cu_count, dev->error);
+            if (e != hipSuccess) return hip_fail(e, "bvh_refit");
+            copy_counters_async(ctr, m->host, dev->error, s);
+        }
+        m->refits++;
+        LC_HIP_TRY(hipEventRecord(m->ready, s));
+        return LC_HIP_OK;
+    }
     size_t ncap = bvh_node_capacity(n);
     void *p;
     size_t cap;
@@ -567,11 +599,9 @@ int lc_hip_mesh_build(uint64_t device, uint64_t stream, uint64_t mesh, uint32_t,
         LC_HIP_TRY(hipMemsetAsync(dev->error, 0, sizeof(uint32_t), s));
         TimedRegion tr(dev, s, "mesh_build");
         BuildCounters *ctr = nullptr;
-        hipError_t e = bvh_build(s, m->scratch, m->scratch_bytes, static_cast<char *>(vb->ptr) + vertex_offset,
-                                 vertex_stride, nverts,
-                                 reinterpret_cast<const uint32_t *>(static_cast<char *>(tb->ptr) + triangle_offset),
-                                 nullptr, n, m->nodes, m->tris, nullptr, &ctr, dev->split_small, dev->cu_count,
-                                 dev->error, &m->host->phase_count);
+        hipError_t e = bvh_build(s, m->scratch, m->scratch_bytes, vaddr, vertex_stride, nverts, taddr, nullptr, n,
+                                 m->nodes, m->tris, nullptr, &ctr, dev->split_small, dev->cu_count, dev->error,
+                                 &m->host->phase_count);
         if (e != hipSuccess) return hip_fail(e, "bvh_build");
         copy_counters_async(ctr, m->host, dev->error, s);
     }
@@ -661,6 +691,8 @@ static void xform_point_h(const float m[12], const float p[3], float out[3]) {
     for (int r = 0; r < 3; r++) out[r] = m[4 * r + 0] * p[0] + m[4 * r + 1] * p[1] + m[4 * r + 2] * p[2] + m[4 * r + 3];
 }
 
+// The request is not consulted: an update rebuilds the TLAS over the instance boxes (a few
+// thousand instances at most; the boxes move with the transforms anyway).
 int lc_hip_accel_build(uint64_t device, uint64_t stream, uint64_t accel, uint32_t, uint32_t instance_count,
                        const lc_hip_accel_modification *mods, size_t nmods, int update_instance_buffer_only) {
     Device *dev = obj<Device>(device);
