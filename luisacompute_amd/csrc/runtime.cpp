@@ -846,7 +846,8 @@ static int accel_tlas_sync(Accel *a) {
 }
 
 // What traversal needs from a built accel: one BLAS (single instance) or the TLAS; the stack
-// entries a ray can need (node groups per wide level + TLAS leaf entries).
+// entries a ray can need: per wide level a node group and a postponed leaf group (speculative
+// traversal), plus the TLAS leaf entries.
 static int accel_geometry(Accel *a, Geometry &G, bool &two, uint32_t &stack_need) {
     std::memset(&G, 0, sizeof(G));
     two = a->inst.size() > 1;
@@ -860,7 +861,7 @@ static int accel_geometry(Accel *a, Geometry &G, bool &two, uint32_t &stack_need
             G.blas_tris = reinterpret_cast<const TriRecord *>(r->blas_tris);
             std::memcpy(G.inst_inv, r->inv, sizeof(G.inst_inv));
             G.inst_mask = r->vis_mask;
-            stack_need = r->blas_depth + 1;
+            stack_need = 2 * (r->blas_depth + 1);
         }
     } else {
         int rc = accel_tlas_sync(a);
@@ -868,7 +869,7 @@ static int accel_geometry(Accel *a, Geometry &G, bool &two, uint32_t &stack_need
         G.tlas_nodes = a->tlas_nodes;
         G.tlas_prims = a->tlas_prims;
         G.instances = a->records;
-        stack_need = a->tlas_stats.depth + 2 + a->max_blas_depth + 1;
+        stack_need = 2 * (a->tlas_stats.depth + 2 + a->max_blas_depth + 1);
     }
     return LC_HIP_OK;
 }

@@ -165,6 +165,19 @@ __device__ __forceinline__ void load_node(const WideNode *nodes, uint32_t idx, u
 }
 
 constexpr uint32_t kInstGroup = 0x80000000u; // stack entry tag: remaining instances of a TLAS leaf
+constexpr uint32_t kTriGroup = 0x40000000u;  // stack entry tag: postponed triangles of a BLAS node
+
+// Wave-uniform step choice (speculative traversal, Aila and Laine, "Understanding the Efficiency
+// of Ray Traversal on GPUs", HPG 2009): a step tests triangles only when at least
+// kTriQuarters/4 of the wave's live lanes hold triangles; otherwise it visits nodes, and lanes
+// that hold triangles but still have a node group visit a node too (their triangles wait, and
+// are pushed as a tagged stack entry if the new node brings leaves of its own). Each step then
+// runs one of the two code paths instead of both.
+constexpr bool kSpeculate = true;      // closest hit
+constexpr bool kSpeculateAny = false;  // any hit (a lane tests its triangles first: a hit ends it)
+constexpr bool kSpeculateNodes = true; // triangle-holding lanes visit nodes in node steps
+constexpr int kTriQuarters = 2;
+constexpr int kTriQuartersAny = 2;
 
 // (t, inst, prim) is nearer than the current best, or as near with a smaller (inst, prim) key.
 __device__ __forceinline__ bool nearer(float t, uint32_t inst, uint32_t prim, float t_best, uint32_t b_inst,
@@ -256,9 +269,19 @@ struct Walker {
 
     // Runs up to `steps` traversal steps; true when the ray is finished.
     __device__ __forceinline__ bool run(const Geometry &G, int steps) {
+        constexpr bool SPEC = ANY ? kSpeculateAny : kSpeculate;
         for (int it = 0; it < steps; it++) {
             const float tcull = ANY ? tmax : fminf(tmax, t_best);
-            if (t_mask != 0u) {
+            bool tri_step = t_mask != 0u;
+            if (SPEC) {
+                if (t_mask == 0u && (g_mask & 0xFFu) == 0u && sp == 0) return true;
+                const uint32_t n_live = (uint32_t)__popcll(__ballot(1));
+                const uint32_t n_tri = (uint32_t)__popcll(__ballot(t_mask != 0u));
+                tri_step = 4u * n_tri >= (uint32_t)(ANY ? kTriQuartersAny : kTriQuarters) * n_live;
+                if (tri_step && t_mask == 0u) continue;
+                if (!tri_step && !kSpeculateNodes && t_mask != 0u) continue;
+            }
+            if (tri_step) {
                 const uint32_t b = (uint32_t)__builtin_ctz(t_mask);
                 t_mask &= t_mask - 1u;
                 if (!TWO || in_blas) {
@@ -308,6 +331,7 @@ struct Walker {
                 continue;
             }
             if ((g_mask & 0xFFu) == 0u) {
+                if (SPEC && t_mask != 0u) continue; // holds triangles and has no node to visit
                 if (TWO && in_blas && sp == sp_inst) {
                     // leave the instance: back to the world-space ray and the TLAS
                     for (int k = 0; k < 3; k++) { R.o[k] = wo[k]; R.d[k] = wd[k]; }
@@ -320,6 +344,11 @@ struct Walker {
                 if (TWO && (e.y & kInstGroup)) {
                     t_base = e.x;
                     t_mask = e.y & ~kInstGroup;
+                    continue;
+                }
+                if (SPEC && (e.y & kTriGroup)) {
+                    t_base = e.x;
+                    t_mask = e.y & ~kTriGroup;
                     continue;
                 }
                 g_base = e.x;
@@ -342,13 +371,23 @@ struct Walker {
             g_base = w1.x;
             g_mask = permute8(hits & nim, R.oct) | (nim << 8);
             uint32_t leaf = hits & ~nim;
-            t_base = w1.y;
-            t_mask = 0u;
+            uint32_t nt = 0u;
             while (leaf) {
                 const uint32_t s = (uint32_t)__builtin_ctz(leaf);
                 leaf &= leaf - 1u;
                 const uint32_t m = ((s < 4 ? w1.z : w1.w) >> (8 * (s & 3))) & 0xFFu;
-                t_mask |= ((1u << (m >> 5)) - 1u) << (m & 31u);
+                nt |= ((1u << (m >> 5)) - 1u) << (m & 31u);
+            }
+            if (SPEC) {
+                if (nt != 0u) {
+                    // the postponed group goes to the stack (instances in the TLAS, triangles in a BLAS)
+                    if (t_mask != 0u) push(make_uint2(t_base, t_mask | ((TWO && !in_blas) ? kInstGroup : kTriGroup)));
+                    t_base = w1.y;
+                    t_mask = nt;
+                }
+            } else {
+                t_base = w1.y;
+                t_mask = nt;
             }
         }
         return false;
