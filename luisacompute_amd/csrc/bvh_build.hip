@@ -20,7 +20,7 @@
 #include <hip/hip_runtime.h>
 #include <cstring>
 #include <rocprim/device/device_radix_sort.hpp>
-#include <rocprim/device/device_select.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include <cfloat>
 #include <cstdint>
@@ -36,6 +36,10 @@ constexpr int kBlock = 256;
 // Collapse workgroup: groups of 8 lanes, one queue/triangle reservation per workgroup iteration.
 constexpr int kCollapseBlock = 256;
 constexpr uint32_t kNoEntry = 0xFFFFFFFFu;
+// bottom-up phases: entries per workgroup, the single-workgroup last phase, leaves per run of
+// the first phase
+constexpr uint32_t kPhaseBlock = 256, kLastBlock = 1024, kLeafRun = 512;
+static_assert(kLastBlock >= kLeafRun && kLeafRun >= kPhaseBlock, "phase sizes");
 
 __device__ __forceinline__ int f2ord(float f) {
     int i = __float_as_int(f);
@@ -251,20 +255,40 @@ __device__ __forceinline__ void publish_root(BuildCounters *ctr, uint2 *queue0, 
     ctr->level[0] = 1ull << 32;
 }
 
+// Deferred entries of a phase, compacted per workgroup in thread order (which is first-leaf
+// order: a thread's final subtree contains its starting entry): out[blockIdx.x * blockDim.x + r],
+// count in bcount[blockIdx.x]. Every thread of the workgroup calls it once.
+__device__ __forceinline__ void compact_deferred(uint32_t deferred, uint32_t *out, uint32_t *bcount) {
+    __shared__ uint32_t s_wave_count[16];
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const bool has = deferred != kNoEntry;
+    const uint64_t mask = __ballot(has);
+    const uint32_t rank = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+    if (lane == 0u) s_wave_count[wave] = (uint32_t)__popcll(mask);
+    __syncthreads();
+    uint32_t base = 0, total = 0;
+    for (uint32_t w = 0; w < ((blockDim.x + 63u) >> 6); w++) {
+        if (w < wave) base += s_wave_count[w];
+        total += s_wave_count[w];
+    }
+    if (has) out[blockIdx.x * blockDim.x + base + rank] = deferred;
+    if (threadIdx.x == 0u) bcount[blockIdx.x] = total;
+}
+
 // Bottom-up boxes, one phase. The workgroup owns entries [e0, e1) -- complete subtrees ordered by
 // first leaf -- and so the leaf range [L0, L1]. A thread climbs from its entry while the parent's
 // key range lies inside [L0, L1]; then both children of that parent are climbed by threads of the
 // same workgroup, and the handoff (the first child to arrive stops, the second merges the boxes
 // and goes on) needs only workgroup-scope ordering. A parent whose range leaves the workgroup is
-// deferred: the child is written to pend[first leaf] and becomes an entry of the next phase.
+// deferred: the child becomes an entry of the next phase (compact_deferred into `pend`).
 // whole != 0: the workgroup owns all leaves (last phase, one workgroup) or, with agent scope, the
-// fallback for inputs whose phases stop shrinking.
+// fallback for inputs whose phases stop shrinking; nothing is deferred then.
 template<bool AGENT>
 __global__ void k_bounds_phase(const float4 *pbox, uint32_t n, const uint32_t *ids, BinaryTree bt,
-                               const uint32_t *entries, uint32_t m, uint32_t *pend, BuildCounters *ctr,
-                               uint2 *queue0, uint32_t whole) {
+                               const uint32_t *entries, uint32_t m, uint32_t *pend, uint32_t *bcount,
+                               BuildCounters *ctr, uint2 *queue0, uint32_t whole) {
     const uint32_t e0 = blockIdx.x * blockDim.x;
-    if (e0 >= m) return;
+    if (e0 >= m) return; // whole workgroups only
     const uint32_t e1 = min(e0 + blockDim.x, m);
     uint32_t L0 = 0, L1 = n - 1;
     if (!whole) {
@@ -272,48 +296,51 @@ __global__ void k_bounds_phase(const float4 *pbox, uint32_t n, const uint32_t *i
         L1 = ref_last(bt, entries ? entries[e1 - 1] : ((e1 - 1) | kLeafBit));
     }
     const uint32_t e = e0 + threadIdx.x;
-    if (e >= e1) return;
-    uint32_t cur = entries ? entries[e] : (e | kLeafBit);
-    float3 lo, hi;
-    uint32_t f, l;
-    if (cur & kLeafBit) {
-        f = l = cur & ~kLeafBit;
-        stored_box(pbox, ids[f], lo, hi);
-    } else {
-        const uint2 rg = bt.range[cur];
-        f = rg.x;
-        l = rg.y;
-        const float4 *cb = bt.cbox + 4 * (uint64_t)cur;
-        float4 a = cb[0], b = cb[1], c = cb[2], d = cb[3];
-        lo = make_float3(fminf(a.x, c.x), fminf(a.y, c.y), fminf(a.z, c.z));
-        hi = make_float3(fmaxf(b.x, d.x), fmaxf(b.y, d.y), fmaxf(b.z, d.z));
-    }
-    for (;;) {
-        if (cur == 0u) { // internal node 0 spans every leaf: the root
-            publish_root(ctr, queue0, 0u, lo, hi);
-            return;
+    uint32_t deferred = kNoEntry;
+    if (e < e1) {
+        uint32_t cur = entries ? entries[e] : (e | kLeafBit);
+        float3 lo, hi;
+        uint32_t f, l;
+        if (cur & kLeafBit) {
+            f = l = cur & ~kLeafBit;
+            stored_box(pbox, ids[f], lo, hi);
+        } else {
+            const uint2 rg = bt.range[cur];
+            f = rg.x;
+            l = rg.y;
+            const float4 *cb = bt.cbox + 4 * (uint64_t)cur;
+            float4 a = cb[0], b = cb[1], c = cb[2], d = cb[3];
+            lo = make_float3(fminf(a.x, c.x), fminf(a.y, c.y), fminf(a.z, c.z));
+            hi = make_float3(fmaxf(b.x, d.x), fmaxf(b.y, d.y), fmaxf(b.z, d.z));
         }
-        const uint2 link = (cur & kLeafBit) ? bt.parent_leaf[f] : bt.parent_int[cur];
-        const uint32_t p = link.x, side = link.y >> 31, end = link.y & ~kLeafBit;
-        const uint32_t pf = side ? end : f, pl = side ? l : end;
-        if (pf < L0 || pl > L1) {
-            pend[f] = cur;
-            return;
+        for (;;) {
+            if (cur == 0u) { // internal node 0 spans every leaf: the root
+                publish_root(ctr, queue0, 0u, lo, hi);
+                break;
+            }
+            const uint2 link = (cur & kLeafBit) ? bt.parent_leaf[f] : bt.parent_int[cur];
+            const uint32_t p = link.x, side = link.y >> 31, end = link.y & ~kLeafBit;
+            const uint32_t pf = side ? end : f, pl = side ? l : end;
+            if (pf < L0 || pl > L1) {
+                deferred = cur;
+                break;
+            }
+            bt.cbox[4 * (uint64_t)p + 2 * side] = make_float4(lo.x, lo.y, lo.z, 0.0f);
+            bt.cbox[4 * (uint64_t)p + 2 * side + 1] = make_float4(hi.x, hi.y, hi.z, 0.0f);
+            const uint32_t first = AGENT
+                ? __hip_atomic_exchange(&bt.slots[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT)
+                : __hip_atomic_exchange(&bt.slots[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (first == 0u) break; // the sibling's thread completes p
+            const float4 slo = bt.cbox[4 * (uint64_t)p + 2 * (1u - side)];
+            const float4 shi = bt.cbox[4 * (uint64_t)p + 2 * (1u - side) + 1];
+            lo = make_float3(fminf(lo.x, slo.x), fminf(lo.y, slo.y), fminf(lo.z, slo.z));
+            hi = make_float3(fmaxf(hi.x, shi.x), fmaxf(hi.y, shi.y), fmaxf(hi.z, shi.z));
+            cur = p;
+            f = pf;
+            l = pl;
         }
-        bt.cbox[4 * (uint64_t)p + 2 * side] = make_float4(lo.x, lo.y, lo.z, 0.0f);
-        bt.cbox[4 * (uint64_t)p + 2 * side + 1] = make_float4(hi.x, hi.y, hi.z, 0.0f);
-        const uint32_t first = AGENT
-            ? __hip_atomic_exchange(&bt.slots[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT)
-            : __hip_atomic_exchange(&bt.slots[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
-        if (first == 0u) return; // the sibling's thread completes p
-        const float4 slo = bt.cbox[4 * (uint64_t)p + 2 * (1u - side)];
-        const float4 shi = bt.cbox[4 * (uint64_t)p + 2 * (1u - side) + 1];
-        lo = make_float3(fminf(lo.x, slo.x), fminf(lo.y, slo.y), fminf(lo.z, slo.z));
-        hi = make_float3(fmaxf(hi.x, shi.x), fmaxf(hi.y, shi.y), fmaxf(hi.z, shi.z));
-        cur = p;
-        f = pf;
-        l = pl;
     }
+    if (!whole) compact_deferred(deferred, pend, bcount);
 }
 
 // First bottom-up phase: workgroup b owns leaves [b B, b B + B) and climbs from every leaf. A node
@@ -321,43 +348,56 @@ __global__ void k_bounds_phase(const float4 *pbox, uint32_t n, const uint32_t *i
 // one end of its range), so both children of such a node hand off through LDS: each writes its
 // box to its side's LDS slot, then an LDS atomic picks the second arrival, which merges and goes
 // on. Boxes are also stored to cbox for the collapse. Parents reaching outside the run are
-// deferred to pend as in k_bounds_phase. Requires n > B (the root never lies inside a run).
+// deferred as in k_bounds_phase. Requires n > B (the root never lies inside a run).
 template<int B>
 __global__ __launch_bounds__(B) void k_bounds_leaves(const float4 *pbox, uint32_t n, const uint32_t *ids,
-                                                     BinaryTree bt, uint32_t *pend) {
+                                                     BinaryTree bt, uint32_t *pend, uint32_t *bcount) {
     __shared__ float s_box[B][2][6];
     __shared__ uint32_t s_arrived[B];
     const uint32_t L0 = blockIdx.x * B, L1 = min(L0 + (uint32_t)B - 1u, n - 1u);
     s_arrived[threadIdx.x] = 0u;
     __syncthreads();
     const uint32_t leaf = L0 + threadIdx.x;
-    if (leaf > L1) return;
-    uint32_t cur = leaf | kLeafBit, f = leaf, l = leaf;
-    float3 lo, hi;
-    stored_box(pbox, ids[leaf], lo, hi);
-    for (;;) {
-        const uint2 link = (cur & kLeafBit) ? bt.parent_leaf[f] : bt.parent_int[cur];
-        const uint32_t p = link.x, side = link.y >> 31, end = link.y & ~kLeafBit;
-        const uint32_t pf = side ? end : f, pl = side ? l : end;
-        if (pf < L0 || pl > L1) {
-            pend[f] = cur;
-            return;
+    uint32_t deferred = kNoEntry;
+    if (leaf <= L1) {
+        uint32_t cur = leaf | kLeafBit, f = leaf, l = leaf;
+        float3 lo, hi;
+        stored_box(pbox, ids[leaf], lo, hi);
+        for (;;) {
+            const uint2 link = (cur & kLeafBit) ? bt.parent_leaf[f] : bt.parent_int[cur];
+            const uint32_t p = link.x, side = link.y >> 31, end = link.y & ~kLeafBit;
+            const uint32_t pf = side ? end : f, pl = side ? l : end;
+            if (pf < L0 || pl > L1) {
+                deferred = cur;
+                break;
+            }
+            bt.cbox[4 * (uint64_t)p + 2 * side] = make_float4(lo.x, lo.y, lo.z, 0.0f);
+            bt.cbox[4 * (uint64_t)p + 2 * side + 1] = make_float4(hi.x, hi.y, hi.z, 0.0f);
+            const uint32_t k = p - L0;
+            float *mine = s_box[k][side];
+            mine[0] = lo.x; mine[1] = lo.y; mine[2] = lo.z;
+            mine[3] = hi.x; mine[4] = hi.y; mine[5] = hi.z;
+            const uint32_t before = __hip_atomic_fetch_add(&s_arrived[k], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (before == 0u) break; // the sibling's thread completes p
+            const float *other = s_box[k][1u - side];
+            lo = make_float3(fminf(lo.x, other[0]), fminf(lo.y, other[1]), fminf(lo.z, other[2]));
+            hi = make_float3(fmaxf(hi.x, other[3]), fmaxf(hi.y, other[4]), fmaxf(hi.z, other[5]));
+            cur = p;
+            f = pf;
+            l = pl;
         }
-        bt.cbox[4 * (uint64_t)p + 2 * side] = make_float4(lo.x, lo.y, lo.z, 0.0f);
-        bt.cbox[4 * (uint64_t)p + 2 * side + 1] = make_float4(hi.x, hi.y, hi.z, 0.0f);
-        const uint32_t k = p - L0;
-        float *mine = s_box[k][side];
-        mine[0] = lo.x; mine[1] = lo.y; mine[2] = lo.z;
-        mine[3] = hi.x; mine[4] = hi.y; mine[5] = hi.z;
-        const uint32_t before = __hip_atomic_fetch_add(&s_arrived[k], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
-        if (before == 0u) return; // the sibling's thread completes p
-        const float *other = s_box[k][1u - side];
-        lo = make_float3(fminf(lo.x, other[0]), fminf(lo.y, other[1]), fminf(lo.z, other[2]));
-        hi = make_float3(fmaxf(hi.x, other[3]), fmaxf(hi.y, other[4]), fmaxf(hi.z, other[5]));
-        cur = p;
-        f = pf;
-        l = pl;
     }
+    compact_deferred(deferred, pend, bcount);
+}
+
+// Next phase's entries: workgroup b of the phase left bcount[b] entries at src[b * run]; they go
+// to dst[boff[b] ...] (boff = exclusive scan of bcount). The last workgroup writes the total.
+__global__ void k_gather_deferred(const uint32_t *src, const uint32_t *bcount, const uint32_t *boff, uint32_t run,
+                                  uint32_t nblocks, uint32_t *dst, uint32_t *total) {
+    const uint32_t b = blockIdx.x;
+    const uint32_t c = bcount[b], o = boff[b];
+    for (uint32_t t = threadIdx.x; t < c; t += blockDim.x) dst[o + t] = src[(uint64_t)b * run + t];
+    if (b == nblocks - 1u && threadIdx.x == 0u) *total = o + c;
 }
 
 __global__ void k_single_leaf_root(const float4 *pbox, const uint32_t *ids, BuildCounters *ctr, uint2 *queue0) {
@@ -365,21 +405,6 @@ __global__ void k_single_leaf_root(const float4 *pbox, const uint32_t *ids, Buil
     stored_box(pbox, ids[0], lo, hi);
     publish_root(ctr, queue0, kLeafBit, lo, hi);
 }
-
-__global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) p[i] = v;
-}
-
-// Clears the pend slots of the entries just compacted out of it.
-__global__ void k_clear_pend(const uint32_t *entries, const uint32_t *count, uint32_t *pend, BinaryTree bt) {
-    const uint32_t m = *count;
-    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < m; e += gridDim.x * blockDim.x)
-        pend[ref_first(bt, entries[e])] = kNoEntry;
-}
-
-struct IsEntry {
-    __device__ bool operator()(uint32_t v) const { return v != kNoEntry; }
-};
 
 struct WideOut {
     WideNode *nodes;
@@ -892,12 +917,15 @@ size_t sort_temp_bytes(uint32_t n) {
     return bytes;
 }
 
-size_t select_temp_bytes(uint32_t n) {
+size_t scan_temp_bytes(uint32_t n) {
     size_t bytes = 0;
-    rocprim::select(nullptr, bytes, (const uint32_t *)nullptr, (uint32_t *)nullptr, (uint32_t *)nullptr, n, IsEntry{},
-                    (hipStream_t)0);
+    rocprim::exclusive_scan(nullptr, bytes, (const uint32_t *)nullptr, (uint32_t *)nullptr, 0u, (size_t)n,
+                            rocprim::plus<uint32_t>(), (hipStream_t)0);
     return bytes;
 }
+
+// Workgroups of a bottom-up phase (runs of >= kPhaseBlock entries or leaves).
+size_t phase_blocks(size_t m) { return m / kPhaseBlock + 1; }
 
 // The scratch arrays of a build over n primitives (bvh_scratch_bytes(n) bytes). The counters
 // stay valid after the build; a refit reuses them and the cbox array.
@@ -906,6 +934,7 @@ struct ScratchLayout {
     float4 *pbox_own;
     BinaryTree bt;
     uint32_t *pend, *ent[2];
+    uint32_t *bcount, *boff; // per-workgroup deferred-entry counts of a phase and their scan
     uint2 *q0, *q1;
     BuildCounters *ctr;
     uint32_t *dcount;
@@ -927,9 +956,11 @@ struct ScratchLayout {
         ent[0] = carve<uint32_t>(p, m);
         ent[1] = carve<uint32_t>(p, m);
         q0 = carve<uint2>(p, m); q1 = carve<uint2>(p, m);
+        bcount = carve<uint32_t>(p, phase_blocks(m));
+        boff = carve<uint32_t>(p, phase_blocks(m));
         ctr = carve<BuildCounters>(p, 1);
         dcount = carve<uint32_t>(p, 1);
-        temp_bytes = std::max(sort_temp_bytes((uint32_t)m), select_temp_bytes((uint32_t)m));
+        temp_bytes = std::max(sort_temp_bytes((uint32_t)m), scan_temp_bytes((uint32_t)phase_blocks(m)));
         temp = carve<char>(p, temp_bytes);
     }
 };
@@ -949,10 +980,11 @@ size_t bvh_scratch_bytes(uint32_t n) {
     b += m * sizeof(uint32_t) + 256;                // slots
     b += 2 * m * sizeof(uint2) + 256 * 2;           // parent links
     b += 3 * m * sizeof(uint32_t) + 256 * 3;        // pend + two entry lists
+    b += 2 * phase_blocks(m) * sizeof(uint32_t) + 256 * 2; // phase workgroup counts + scan
     b += 2 * m * sizeof(uint2) + 256 * 2;           // collapse queues
     b += sizeof(BuildCounters) + 256;               // counters
-    b += sizeof(uint32_t) + 256;                    // selected count
-    b += std::max(sort_temp_bytes((uint32_t)m), select_temp_bytes((uint32_t)m)) + 256; // rocPRIM temp
+    b += sizeof(uint32_t) + 256;                    // entry count
+    b += std::max(sort_temp_bytes((uint32_t)m), scan_temp_bytes((uint32_t)phase_blocks(m))) + 256; // rocPRIM temp
     return b;
 }
 
@@ -1027,11 +1059,9 @@ hipError_t bvh_build(hipStream_t stream, void *scratch, size_t scratch_bytes, co
         hipLaunchKernelGGL(k_single_leaf_root, dim3(1), dim3(1), 0, stream, pbox, ids, ctr, q0);
     } else {
         hipLaunchKernelGGL(k_karras, dim3((n - 1 + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, n, keys, bt);
-        hipLaunchKernelGGL(k_fill_u32, dim3(std::min<uint32_t>(grid, 4096)), dim3(kBlock), 0, stream, pend, (uint64_t)n,
-                           kNoEntry);
         // Bottom-up phases. Entries of phase 0 are the leaves in sorted order (entries == null).
-        constexpr uint32_t kPhaseBlock = 256, kLastBlock = 1024, kLeafRun = 512;
-        static_assert(kLastBlock >= kLeafRun, "the leaf phase needs n > its run length");
+        // Each phase leaves its deferred entries compacted per workgroup in `pend`; a scan of the
+        // workgroup counts places them in the next entry list.
         const uint32_t *entries = nullptr;
         uint32_t count = n;
         int which = 0;
@@ -1039,19 +1069,23 @@ hipError_t bvh_build(hipStream_t stream, void *scratch, size_t scratch_bytes, co
             if (count <= kLastBlock) {
                 uint32_t blk = (count + 63) / 64 * 64;
                 hipLaunchKernelGGL(k_bounds_phase<false>, dim3(1), dim3(blk), 0, stream, pbox, n, ids, bt, entries, count,
-                                   pend, ctr, q0, 1u);
+                                   pend, L.bcount, ctr, q0, 1u);
                 break;
             }
+            const uint32_t run = entries ? kPhaseBlock : kLeafRun;
+            const uint32_t nblocks = ((entries ? count : n) + run - 1) / run;
             if (!entries)
-                hipLaunchKernelGGL(k_bounds_leaves<kLeafRun>, dim3((n + kLeafRun - 1) / kLeafRun), dim3(kLeafRun), 0,
-                                   stream, pbox, n, ids, bt, pend);
+                hipLaunchKernelGGL(k_bounds_leaves<kLeafRun>, dim3(nblocks), dim3(kLeafRun), 0, stream, pbox, n, ids, bt,
+                                   pend, L.bcount);
             else
-                hipLaunchKernelGGL(k_bounds_phase<false>, dim3((count + kPhaseBlock - 1) / kPhaseBlock),
-                                   dim3(kPhaseBlock), 0, stream, pbox, n, ids, bt, entries, count, pend, ctr, q0, 0u);
+                hipLaunchKernelGGL(k_bounds_phase<false>, dim3(nblocks), dim3(kPhaseBlock), 0, stream, pbox, n, ids, bt,
+                                   entries, count, pend, L.bcount, ctr, q0, 0u);
             tb = temp_bytes;
-            e = rocprim::select(temp, tb, (const uint32_t *)pend, ent[which], dcount, n, IsEntry{}, stream);
+            e = rocprim::exclusive_scan(temp, tb, (const uint32_t *)L.bcount, L.boff, 0u, (size_t)nblocks,
+                                        rocprim::plus<uint32_t>(), stream);
             if (e != hipSuccess) return e;
-            hipLaunchKernelGGL(k_clear_pend, dim3(256), dim3(kBlock), 0, stream, ent[which], dcount, pend, bt);
+            hipLaunchKernelGGL(k_gather_deferred, dim3(nblocks), dim3(run), 0, stream, (const uint32_t *)pend,
+                               (const uint32_t *)L.bcount, (const uint32_t *)L.boff, run, nblocks, ent[which], dcount);
             e = hipMemcpyAsync(pinned_word, dcount, sizeof(uint32_t), hipMemcpyDeviceToHost, stream);
             if (e != hipSuccess) return e;
             e = hipStreamSynchronize(stream);
@@ -1064,7 +1098,7 @@ hipError_t bvh_build(hipStream_t stream, void *scratch, size_t scratch_bytes, co
                 // Phases stopped shrinking (sibling subtrees keep straddling workgroup
                 // boundaries): finish with agent-scope handoffs over all remaining entries.
                 hipLaunchKernelGGL(k_bounds_phase<true>, dim3((next + kPhaseBlock - 1) / kPhaseBlock), dim3(kPhaseBlock),
-                                   0, stream, pbox, n, ids, bt, entries, next, pend, ctr, q0, 1u);
+                                   0, stream, pbox, n, ids, bt, entries, next, pend, L.bcount, ctr, q0, 1u);
                 break;
             }
             count = next;
