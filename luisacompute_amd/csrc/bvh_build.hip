@@ -79,11 +79,57 @@ __device__ __forceinline__ void prim_box(const PrimSource &s, uint32_t p, float3
     hi = make_float3(fmaxf(a.x, fmaxf(b.x, c.x)), fmaxf(a.y, fmaxf(b.y, c.y)), fmaxf(a.z, fmaxf(b.z, c.z)));
 }
 
-// Stored primitive box (written by k_morton; for box inputs, the input itself).
+// Input box of primitive p (box builds: the TLAS's instance boxes).
 __device__ __forceinline__ void stored_box(const float4 *pbox, uint32_t p, float3 &lo, float3 &hi) {
     float4 a = pbox[2 * (uint64_t)p], b = pbox[2 * (uint64_t)p + 1];
     lo = make_float3(a.x, a.y, a.z);
     hi = make_float3(b.x, b.y, b.z);
+}
+
+// Leaf boxes by sorted position: triangle builds read the records written in sorted order
+// (srec, coalesced with the sort order), box builds the input boxes through the sorted ids.
+struct LeafBoxes {
+    const float4 *pbox;
+    const uint32_t *ids;
+    const TriRecord *srec;
+};
+
+__device__ __forceinline__ void record_box(const TriRecord *r, float3 &lo, float3 &hi) {
+    const float4 *v = reinterpret_cast<const float4 *>(r);
+    const float4 a = v[0], b = v[1], c = v[2];
+    lo = make_float3(fminf(a.x, fminf(b.x, c.x)), fminf(a.y, fminf(b.y, c.y)), fminf(a.z, fminf(b.z, c.z)));
+    hi = make_float3(fmaxf(a.x, fmaxf(b.x, c.x)), fmaxf(a.y, fmaxf(b.y, c.y)), fmaxf(a.z, fmaxf(b.z, c.z)));
+}
+
+__device__ __forceinline__ void leaf_box(const LeafBoxes &lb, uint32_t pos, float3 &lo, float3 &hi) {
+    if (lb.srec) record_box(lb.srec + pos, lo, hi);
+    else stored_box(lb.pbox, lb.ids[pos], lo, hi);
+}
+
+// Copies record *src to *dst; returns its box.
+__device__ __forceinline__ void copy_record(const TriRecord *src, TriRecord *dst, float3 &lo, float3 &hi) {
+    const float4 *v = reinterpret_cast<const float4 *>(src);
+    const float4 a = v[0], b = v[1], c = v[2];
+    float4 *d = reinterpret_cast<float4 *>(dst);
+    d[0] = a;
+    d[1] = b;
+    d[2] = c;
+    lo = make_float3(fminf(a.x, fminf(b.x, c.x)), fminf(a.y, fminf(b.y, c.y)), fminf(a.z, fminf(b.z, c.z)));
+    hi = make_float3(fmaxf(a.x, fmaxf(b.x, c.x)), fmaxf(a.y, fmaxf(b.y, c.y)), fmaxf(a.z, fmaxf(b.z, c.z)));
+}
+
+// Triangle record of primitive `prim` (prim id in v0.w) at *dst; returns its box.
+__device__ __forceinline__ void write_record(const PrimSource &src, uint32_t prim, TriRecord *dst, float3 &lo,
+                                             float3 &hi) {
+    const uint32_t i0 = src.tri[3 * (uint64_t)prim], i1 = src.tri[3 * (uint64_t)prim + 1],
+                   i2 = src.tri[3 * (uint64_t)prim + 2];
+    const float4 a = load_vertex(src, i0), b = load_vertex(src, i1), c = load_vertex(src, i2);
+    float4 *tr = reinterpret_cast<float4 *>(dst);
+    tr[0] = make_float4(a.x, a.y, a.z, __uint_as_float(prim));
+    tr[1] = make_float4(b.x, b.y, b.z, 0.0f);
+    tr[2] = make_float4(c.x, c.y, c.z, 0.0f);
+    lo = make_float3(fminf(a.x, fminf(b.x, c.x)), fminf(a.y, fminf(b.y, c.y)), fminf(a.z, fminf(b.z, c.z)));
+    hi = make_float3(fmaxf(a.x, fmaxf(b.x, c.x)), fmaxf(a.y, fmaxf(b.y, c.y)), fmaxf(a.z, fmaxf(b.z, c.z)));
 }
 
 __global__ void k_init(BuildCounters *ctr, uint32_t *slots, uint32_t nslots) {
@@ -147,9 +193,10 @@ __device__ __forceinline__ uint32_t expand10(uint32_t v) {
     return v;
 }
 
-// 30-bit Morton code of the box centroid in the centroid bounds; stores the box for later passes.
+// 30-bit Morton code of the box centroid in the centroid bounds; a triangle build also writes
+// each triangle's record in primitive order (prec), read back in sorted order by the leaf phase.
 __global__ __launch_bounds__(kBlock) void k_morton(PrimSource src, uint32_t n, const BuildCounters *ctr,
-                                                   uint32_t *keys, uint32_t *ids, float4 *pbox) {
+                                                   uint32_t *keys, uint32_t *ids, TriRecord *prec) {
     uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     float lo[3], sc[3];
@@ -160,11 +207,8 @@ __global__ __launch_bounds__(kBlock) void k_morton(PrimSource src, uint32_t n, c
         sc[a] = ext > 0.0f ? 1024.0f / ext : 0.0f;
     }
     float3 blo, bhi;
-    prim_box(src, p, blo, bhi);
-    if (!src.boxes) {
-        pbox[2 * (uint64_t)p] = make_float4(blo.x, blo.y, blo.z, 0.0f);
-        pbox[2 * (uint64_t)p + 1] = make_float4(bhi.x, bhi.y, bhi.z, 0.0f);
-    }
+    if (prec) write_record(src, p, prec + p, blo, bhi);
+    else prim_box(src, p, blo, bhi);
     float c[3] = {0.5f * (blo.x + bhi.x), 0.5f * (blo.y + bhi.y), 0.5f * (blo.z + bhi.z)};
     uint32_t q[3];
     for (int a = 0; a < 3; a++) {
@@ -284,7 +328,7 @@ __device__ __forceinline__ void compact_deferred(uint32_t deferred, uint32_t *ou
 // whole != 0: the workgroup owns all leaves (last phase, one workgroup) or, with agent scope, the
 // fallback for inputs whose phases stop shrinking; nothing is deferred then.
 template<bool AGENT>
-__global__ void k_bounds_phase(const float4 *pbox, uint32_t n, const uint32_t *ids, BinaryTree bt,
+__global__ void k_bounds_phase(LeafBoxes lb, uint32_t n, BinaryTree bt,
                                const uint32_t *entries, uint32_t m, uint32_t *pend, uint32_t *bcount,
                                BuildCounters *ctr, uint2 *queue0, uint32_t whole) {
     const uint32_t e0 = blockIdx.x * blockDim.x;
@@ -303,7 +347,7 @@ __global__ void k_bounds_phase(const float4 *pbox, uint32_t n, const uint32_t *i
         uint32_t f, l;
         if (cur & kLeafBit) {
             f = l = cur & ~kLeafBit;
-            stored_box(pbox, ids[f], lo, hi);
+            leaf_box(lb, f, lo, hi);
         } else {
             const uint2 rg = bt.range[cur];
             f = rg.x;
@@ -348,9 +392,10 @@ __global__ void k_bounds_phase(const float4 *pbox, uint32_t n, const uint32_t *i
 // one end of its range), so both children of such a node hand off through LDS: each writes its
 // box to its side's LDS slot, then an LDS atomic picks the second arrival, which merges and goes
 // on. Boxes are also stored to cbox for the collapse. Parents reaching outside the run are
-// deferred as in k_bounds_phase. Requires n > B (the root never lies inside a run).
+// deferred as in k_bounds_phase. Requires n > B (the root never lies inside a run). A triangle
+// build (srec != null) also writes each leaf's triangle record at its sorted position.
 template<int B>
-__global__ __launch_bounds__(B) void k_bounds_leaves(const float4 *pbox, uint32_t n, const uint32_t *ids,
+__global__ __launch_bounds__(B) void k_bounds_leaves(LeafBoxes lb, const TriRecord *prec, TriRecord *srec, uint32_t n,
                                                      BinaryTree bt, uint32_t *pend, uint32_t *bcount) {
     __shared__ float s_box[B][2][6];
     __shared__ uint32_t s_arrived[B];
@@ -362,7 +407,8 @@ __global__ __launch_bounds__(B) void k_bounds_leaves(const float4 *pbox, uint32_
     if (leaf <= L1) {
         uint32_t cur = leaf | kLeafBit, f = leaf, l = leaf;
         float3 lo, hi;
-        stored_box(pbox, ids[leaf], lo, hi);
+        if (srec) copy_record(prec + lb.ids[leaf], srec + leaf, lo, hi); // triangle records in sorted order
+        else stored_box(lb.pbox, lb.ids[leaf], lo, hi);
         for (;;) {
             const uint2 link = (cur & kLeafBit) ? bt.parent_leaf[f] : bt.parent_int[cur];
             const uint32_t p = link.x, side = link.y >> 31, end = link.y & ~kLeafBit;
@@ -400,10 +446,18 @@ __global__ void k_gather_deferred(const uint32_t *src, const uint32_t *bcount, c
     if (b == nblocks - 1u && threadIdx.x == 0u) *total = o + c;
 }
 
-__global__ void k_single_leaf_root(const float4 *pbox, const uint32_t *ids, BuildCounters *ctr, uint2 *queue0) {
+__global__ void k_single_leaf_root(LeafBoxes lb, BuildCounters *ctr, uint2 *queue0) {
     float3 lo, hi;
-    stored_box(pbox, ids[0], lo, hi);
+    leaf_box(lb, 0u, lo, hi);
     publish_root(ctr, queue0, kLeafBit, lo, hi);
+}
+
+// Triangle records in sorted order for builds too small for the leaf phase.
+__global__ void k_sorted_records(const TriRecord *prec, const uint32_t *ids, TriRecord *srec, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        float3 lo, hi;
+        copy_record(prec + ids[i], srec + i, lo, hi);
+    }
 }
 
 struct WideOut {
@@ -482,9 +536,10 @@ __device__ __forceinline__ uint32_t ord_u32(float f) {
 //
 // Wide nodes of level l + 1 are numbered after those of levels 0..l in queue order, so a child's
 // node index is (nodes of levels 0..l) + its queue slot. Queue slots and triangle ranges are
-// reserved with one atomic per wave. Leaf slots write their primitive ids to out.prim_out in
-// triangle-record order; a triangle BLAS turns them into records afterwards (k_tri_records).
-__global__ __launch_bounds__(kCollapseBlock) void k_collapse(const float4 *pbox, const uint32_t *ids, BinaryTree bt,
+// reserved with one 64-bit atomic per workgroup iteration. Leaf slots list their primitives in
+// out.prim_out in triangle-record order: primitive ids for a box build, sorted positions for a
+// triangle build, whose records are then copied from the sorted records (k_tri_records).
+__global__ __launch_bounds__(kCollapseBlock) void k_collapse(LeafBoxes lb, BinaryTree bt,
                                                      BuildCounters *ctr, uint32_t level, const uint2 *qin,
                                                      uint2 *qout, WideOut out, uint32_t split_small) {
     __shared__ __align__(16) uint32_t s_node[kCollapseBlock / 8][20];
@@ -519,7 +574,7 @@ __global__ __launch_bounds__(kCollapseBlock) void k_collapse(const float4 *pbox,
                 ref = item.x;
                 first = last = item.x & ~kLeafBit;
                 valid = true;
-                stored_box(pbox, ids[first], lo, hi);
+                leaf_box(lb, first, lo, hi);
             }
         } else {
             nc = 2;
@@ -647,7 +702,8 @@ __global__ __launch_bounds__(kCollapseBlock) void k_collapse(const float4 *pbox,
             } else {
                 meta = (cnt << 5) | toff;
                 uint32_t prim[kLeafMax];
-                for (uint32_t k = 0; k < kLeafMax; k++) prim[k] = k < cnt ? ids[first + k] : 0u;
+                // triangle builds list sorted positions (records are in srec), box builds ids
+                for (uint32_t k = 0; k < kLeafMax; k++) prim[k] = k < cnt ? (lb.srec ? first + k : lb.ids[first + k]) : 0u;
                 for (uint32_t k = 0; k < kLeafMax; k++)
                     if (k < cnt) out.prim_out[tri_base + toff + k] = prim[k];
             }
@@ -694,19 +750,17 @@ __global__ void k_finish_counts(BuildCounters *ctr) {
     ctr->tri_count = tris;
 }
 
-// Triangle records in leaf order: record i holds the vertices of primitive order[i] (prim id in
-// v0.w). One thread per record: coalesced 48-B writes, gathered vertex reads.
-__global__ __launch_bounds__(kBlock) void k_tri_records(const uint32_t *order, PrimSource src, TriRecord *tris,
+// Triangle records in leaf order: record i = sorted record order[i]. A node's leaf triangles are
+// a few nearby sorted positions, so the reads stay close to sequential; writes are coalesced.
+__global__ __launch_bounds__(kBlock) void k_tri_records(const uint32_t *order, const TriRecord *srec, TriRecord *tris,
                                                         uint32_t n) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const uint32_t prim = order[i];
-        const uint32_t i0 = src.tri[3 * (uint64_t)prim], i1 = src.tri[3 * (uint64_t)prim + 1],
-                       i2 = src.tri[3 * (uint64_t)prim + 2];
-        const float4 a = load_vertex(src, i0), b = load_vertex(src, i1), c = load_vertex(src, i2);
-        float4 *tr = reinterpret_cast<float4 *>(tris + i);
-        tr[0] = make_float4(a.x, a.y, a.z, __uint_as_float(prim));
-        tr[1] = make_float4(b.x, b.y, b.z, 0.0f);
-        tr[2] = make_float4(c.x, c.y, c.z, 0.0f);
+        const float4 *src = reinterpret_cast<const float4 *>(srec + order[i]);
+        float4 *dst = reinterpret_cast<float4 *>(tris + i);
+        const float4 a = src[0], b = src[1], c = src[2];
+        dst[0] = a;
+        dst[1] = b;
+        dst[2] = c;
     }
 }
 
@@ -931,7 +985,7 @@ size_t phase_blocks(size_t m) { return m / kPhaseBlock + 1; }
 // stay valid after the build; a refit reuses them and the cbox array.
 struct ScratchLayout {
     uint32_t *k0, *k1, *i0, *i1;
-    float4 *pbox_own;
+    TriRecord *srec, *prec; // triangle builds: records in sorted and in primitive order
     BinaryTree bt;
     uint32_t *pend, *ent[2];
     uint32_t *bcount, *boff; // per-workgroup deferred-entry counts of a phase and their scan
@@ -945,7 +999,8 @@ struct ScratchLayout {
         char *p = static_cast<char *>(scratch);
         k0 = carve<uint32_t>(p, m); k1 = carve<uint32_t>(p, m);
         i0 = carve<uint32_t>(p, m); i1 = carve<uint32_t>(p, m);
-        pbox_own = carve<float4>(p, 2 * m);
+        srec = carve<TriRecord>(p, m);
+        prec = carve<TriRecord>(p, m);
         bt.cbox = carve<float4>(p, 4 * m);
         bt.child = carve<uint2>(p, m);
         bt.range = carve<uint2>(p, m);
@@ -973,7 +1028,7 @@ size_t bvh_scratch_bytes(uint32_t n) {
     size_t b = 0;
     b += 2 * m * sizeof(uint32_t) + 256 * 2;        // keys
     b += 2 * m * sizeof(uint32_t) + 256 * 2;        // ids
-    b += 2 * m * sizeof(float4) + 256;              // primitive boxes
+    b += 2 * m * sizeof(TriRecord) + 256 * 2;       // triangle records, sorted and primitive order
     b += 4 * m * sizeof(float4) + 256;              // cbox
     b += m * sizeof(uint2) + 256;                   // child
     b += m * sizeof(uint2) + 256;                   // range
@@ -1027,7 +1082,6 @@ hipError_t bvh_build(hipStream_t stream, void *scratch, size_t scratch_bytes, co
     const int grid_collapse = std::max(1, cu_count) * collapse_blocks_per_cu;
     ScratchLayout L(scratch, n);
     uint32_t *k0 = L.k0, *k1 = L.k1, *i0 = L.i0, *i1 = L.i1;
-    float4 *pbox_own = L.pbox_own;
     BinaryTree bt = L.bt;
     uint32_t *pend = L.pend;
     uint32_t *ent[2] = {L.ent[0], L.ent[1]};
@@ -1040,14 +1094,14 @@ hipError_t bvh_build(hipStream_t stream, void *scratch, size_t scratch_bytes, co
 
     if (!boxes && n > 0 && nverts == 0) return hipErrorInvalidValue;
     PrimSource src{static_cast<const char *>(vbase), vstride, nverts, tri, boxes, error_flag};
-    const float4 *pbox = boxes ? boxes : pbox_own;
     const uint32_t nslots = n > 1 ? n - 1 : 0;
     hipLaunchKernelGGL(k_init, dim3(std::max<uint32_t>(1, std::min<uint32_t>((nslots + kBlock - 1) / kBlock, 4096))),
                        dim3(kBlock), 0, stream, ctr, bt.slots, nslots);
     if (n == 0) return hipGetLastError();
     const uint32_t grid = (n + kBlock - 1) / kBlock;
     hipLaunchKernelGGL(k_centroid_bounds, dim3(std::min<uint32_t>(grid, 2048)), dim3(kBlock), 0, stream, src, n, ctr);
-    hipLaunchKernelGGL(k_morton, dim3(grid), dim3(kBlock), 0, stream, src, n, ctr, k0, i0, pbox_own);
+    TriRecord *prec = boxes ? nullptr : L.prec;
+    hipLaunchKernelGGL(k_morton, dim3(grid), dim3(kBlock), 0, stream, src, n, ctr, k0, i0, prec);
     rocprim::double_buffer<uint32_t> kb(k0, k1);
     rocprim::double_buffer<uint32_t> vb(i0, i1);
     size_t tb = temp_bytes;
@@ -1055,8 +1109,15 @@ hipError_t bvh_build(hipStream_t stream, void *scratch, size_t scratch_bytes, co
     if (e != hipSuccess) return e;
     const uint32_t *keys = kb.current();
     const uint32_t *ids = vb.current();
+    // leaf boxes: triangle builds keep their records in sorted order (written by the leaf phase,
+    // or here when the build is too small for it); box builds read the input boxes
+    TriRecord *srec = boxes ? nullptr : L.srec;
+    const LeafBoxes lb{boxes, ids, srec};
+    if (srec && n <= kLastBlock)
+        hipLaunchKernelGGL(k_sorted_records, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, stream,
+                           (const TriRecord *)prec, ids, srec, n);
     if (n == 1) {
-        hipLaunchKernelGGL(k_single_leaf_root, dim3(1), dim3(1), 0, stream, pbox, ids, ctr, q0);
+        hipLaunchKernelGGL(k_single_leaf_root, dim3(1), dim3(1), 0, stream, lb, ctr, q0);
     } else {
         hipLaunchKernelGGL(k_karras, dim3((n - 1 + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, n, keys, bt);
         // Bottom-up phases. Entries of phase 0 are the leaves in sorted order (entries == null).
@@ -1068,17 +1129,18 @@ hipError_t bvh_build(hipStream_t stream, void *scratch, size_t scratch_bytes, co
         for (;;) {
             if (count <= kLastBlock) {
                 uint32_t blk = (count + 63) / 64 * 64;
-                hipLaunchKernelGGL(k_bounds_phase<false>, dim3(1), dim3(blk), 0, stream, pbox, n, ids, bt, entries, count,
+                hipLaunchKernelGGL(k_bounds_phase<false>, dim3(1), dim3(blk), 0, stream, lb, n, bt, entries, count,
                                    pend, L.bcount, ctr, q0, 1u);
                 break;
             }
             const uint32_t run = entries ? kPhaseBlock : kLeafRun;
             const uint32_t nblocks = ((entries ? count : n) + run - 1) / run;
             if (!entries)
-                hipLaunchKernelGGL(k_bounds_leaves<kLeafRun>, dim3(nblocks), dim3(kLeafRun), 0, stream, pbox, n, ids, bt,
-                                   pend, L.bcount);
+                hipLaunchKernelGGL(k_bounds_leaves<kLeafRun>, dim3(nblocks), dim3(kLeafRun), 0, stream, lb,
+                                   (const TriRecord *)prec, srec, n,
+                                   bt, pend, L.bcount);
             else
-                hipLaunchKernelGGL(k_bounds_phase<false>, dim3(nblocks), dim3(kPhaseBlock), 0, stream, pbox, n, ids, bt,
+                hipLaunchKernelGGL(k_bounds_phase<false>, dim3(nblocks), dim3(kPhaseBlock), 0, stream, lb, n, bt,
                                    entries, count, pend, L.bcount, ctr, q0, 0u);
             tb = temp_bytes;
             e = rocprim::exclusive_scan(temp, tb, (const uint32_t *)L.bcount, L.boff, 0u, (size_t)nblocks,
@@ -1098,14 +1160,14 @@ hipError_t bvh_build(hipStream_t stream, void *scratch, size_t scratch_bytes, co
                 // Phases stopped shrinking (sibling subtrees keep straddling workgroup
                 // boundaries): finish with agent-scope handoffs over all remaining entries.
                 hipLaunchKernelGGL(k_bounds_phase<true>, dim3((next + kPhaseBlock - 1) / kPhaseBlock), dim3(kPhaseBlock),
-                                   0, stream, pbox, n, ids, bt, entries, next, pend, L.bcount, ctr, q0, 1u);
+                                   0, stream, lb, n, bt, entries, next, pend, L.bcount, ctr, q0, 1u);
                 break;
             }
             count = next;
         }
     }
-    // triangle BLAS: leaf primitive ids go to `pend` (free after the bottom-up phases) and become
-    // triangle records in one pass after the last level
+    // triangle BLAS: leaf sorted positions go to `pend` (free after the bottom-up phases) and the
+    // records are copied from the sorted records in one pass after the last level
     WideOut out{nodes, tris, tris ? pend : prim_out};
     // A root-to-leaf path of the radix tree crosses each of the 30 key bits and 32 index bits at
     // most once, and every wide level descends at least one binary level, so the wide depth is
@@ -1116,7 +1178,7 @@ hipError_t bvh_build(hipStream_t stream, void *scratch, size_t scratch_bytes, co
     for (uint32_t level = 0; level < kCollapseLaunches; level++) {
         const uint2 *qin = (level & 1) ? q1 : q0;
         uint2 *qout = (level & 1) ? q0 : q1;
-        hipLaunchKernelGGL(k_collapse, dim3(grid_collapse), dim3(kCollapseBlock), 0, stream, pbox, ids, bt, ctr, level, qin,
+        hipLaunchKernelGGL(k_collapse, dim3(grid_collapse), dim3(kCollapseBlock), 0, stream, lb, bt, ctr, level, qin,
                            qout, out, split_small);
         if ((level + 1) % kChunk == 0 && level + 1 < kCollapseLaunches) {
             e = hipMemcpyAsync(pinned_word, level_nodes_word(ctr, level + 1), sizeof(uint32_t), hipMemcpyDeviceToHost,
@@ -1129,8 +1191,8 @@ hipError_t bvh_build(hipStream_t stream, void *scratch, size_t scratch_bytes, co
     }
     hipLaunchKernelGGL(k_finish_counts, dim3(1), dim3(1), 0, stream, ctr);
     if (tris)
-        hipLaunchKernelGGL(k_tri_records, dim3(std::min<uint32_t>(grid, 1u << 20)), dim3(kBlock), 0, stream, pend, src,
-                           tris, n);
+        hipLaunchKernelGGL(k_tri_records, dim3(std::min<uint32_t>(grid, 1u << 20)), dim3(kBlock), 0, stream, pend,
+                           (const TriRecord *)srec, tris, n);
     return hipGetLastError();
 }
 
