@@ -44,13 +44,13 @@ def parse():
     ap.add_argument("--serve-requests", type=int, default=48)
     ap.add_argument("--serve-rate", type=float, default=0.0, help="arrivals/s (0: 0.8 x plain saturated throughput)")
     ap.add_argument("--serve-budget-mib", type=int, default=1024)
-    ap.add_argument("--serve-wave-paths", type=int, default=8 << 20)
+    ap.add_argument("--serve-wave-paths", type=int, default=32 << 20)
     ap.add_argument("--pt-width", type=int, default=1920)
     ap.add_argument("--pt-height", type=int, default=1080)
     ap.add_argument("--pt-spp", type=int, default=64)
-    ap.add_argument("--pt-chunk", type=int, default=0, help="wavefront samples per wave (0: about 16M paths)")
-    ap.add_argument("--pt-sort", type=int, default=1)
-    ap.add_argument("--pt-fetch", default="0,1", help="wavefront traces: slots per claim (0: exact), queue segments")
+    ap.add_argument("--pt-chunk", type=int, default=0, help="wavefront samples per wave (0: about 32M paths)")
+    ap.add_argument("--pt-sort", type=int, default=0, help="wavefront: bucket-sort continuation rays")
+    ap.add_argument("--pt-fetch", default="128,8", help="wavefront traces: slots per claim (0: exact), queue segments")
     ap.add_argument("--pt-no-mega", action="store_true", help="skip the megakernel comparison")
     return ap.parse_args()
 
@@ -389,7 +389,9 @@ def main():
     from luisacompute_amd import autotune
     tuned = autotune.load_default(dev)  # block size / LDS stack / refill steps from the autotuner
     s = dev.create_stream()
-    mesh = host.mesh_from_arrays(dev, s, verts, tris, build=False)
+    # the build is inside every step: the BLAS is built with the FAST_BUILD hint (3 triangles per leaf)
+    build_opt = lc.AccelOption(lc.AccelOption.FAST_BUILD, 1, 0)
+    mesh = host.mesh_from_arrays(dev, s, verts, tris, build=False, option=build_opt)
     accel = dev.create_accel()
     accel.emplace_back(mesh, lc.IDENTITY)
     rb = host.upload(dev, s, rays)
@@ -432,7 +434,7 @@ def main():
 
     # BLAS update (refit) of the same triangles on a mesh that allows updates, reported beside
     # the full build
-    umesh = dev.create_mesh(mesh.vb, mesh.tb, option=lc.AccelOption(lc.AccelOption.FAST_TRACE, 1, 1))
+    umesh = dev.create_mesh(mesh.vb, mesh.tb, option=lc.AccelOption(lc.AccelOption.FAST_BUILD, 1, 1))
     s << umesh.build()
     s.synchronize()
     dev.reset_timing()
@@ -478,7 +480,7 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": "triangle soup 16,777,216 tris (centroid U[0,100]^3, sigma 0.5) + 8,388,608 rays "
-                        "(origin U[0,100]^3, dir U(S^2)); step = BLAS build + TLAS + trace_closest",
+                        "(origin U[0,100]^3, dir U(S^2)); step = BLAS build (FAST_BUILD hint) + TLAS + trace_closest",
             "triangles": a.tris,
             "rays": a.rays,
             "seed": "pcg32(42, 2)",

@@ -59,7 +59,10 @@ typedef struct lc_hip_surface_hit {
     uint32_t _pad;
 } lc_hip_surface_hit;
 
-/* AccelOption: include/luisa/runtime/rhi/resource.h:94-126 (hint 0 = FAST_TRACE, 1 = FAST_BUILD). */
+/* AccelOption: include/luisa/runtime/rhi/resource.h:94-126. FAST_TRACE builds one primitive per
+   BVH leaf slot, FAST_BUILD up to three (fewer nodes, shorter build, more triangle tests). */
+#define LC_HIP_ACCEL_FAST_TRACE 0u
+#define LC_HIP_ACCEL_FAST_BUILD 1u
 typedef struct lc_hip_accel_option {
     uint32_t hint;
     uint8_t allow_compaction;
@@ -227,7 +230,7 @@ typedef struct lc_hip_pt_options {
     uint32_t sort;            /* wavefront: bucket-sort continuation rays (origin cell, octant) */
     uint32_t graph;           /* wavefront: capture the launches in a hipGraph, replay when unchanged */
     uint32_t fetch_chunk;     /* wavefront traces: queue slots a wave claims per atomic (0: exact need) */
-    uint32_t fetch_segments;  /* wavefront traces: 1, or 8 (a queue segment per XCD); 0 = default (exact, 1) */
+    uint32_t fetch_segments;  /* wavefront traces: 1, or 8 (a queue segment per XCD); 0 = default (128, 8) */
 } lc_hip_pt_options;
 /* image: float4 per pixel = (mean radiance of the samples, 1), row-major. counters: 0 or a buffer
    of two uint64 that receives (closest-hit rays traced, shadow rays traced). */

@@ -658,8 +658,8 @@ class Server:
     """Render-serving loop (lc_hip_server_*): resident scenes, HBM-budget admission, continuous
     batching of all admitted requests of a scene into shared wavefront waves."""
 
-    def __init__(self, device, hbm_budget=1 << 30, wave_paths=8 << 20, max_active=64, sort=1, graph=1,
-                 fetch=(0, 1)):
+    def __init__(self, device, hbm_budget=1 << 30, wave_paths=32 << 20, max_active=64, sort=0, graph=1,
+                 fetch=(128, 8)):
         self.device = device
         o = ServerOptions(hbm_budget, wave_paths, max_active, sort, graph, fetch[0], fetch[1], 0)
         self.handle = _handle(lib().lc_hip_server_create(device.handle, ctypes.byref(o)), "server create")

@@ -541,7 +541,8 @@ __device__ __forceinline__ uint32_t ord_u32(float f) {
 // triangle build, whose records are then copied from the sorted records (k_tri_records).
 __global__ __launch_bounds__(kCollapseBlock) void k_collapse(LeafBoxes lb, BinaryTree bt,
                                                      BuildCounters *ctr, uint32_t level, const uint2 *qin,
-                                                     uint2 *qout, WideOut out, uint32_t split_small) {
+                                                     uint2 *qout, WideOut out, uint32_t split_small,
+                                                     uint32_t leaf_max) {
     __shared__ __align__(16) uint32_t s_node[kCollapseBlock / 8][20];
     __shared__ uint32_t s_wave[2][kCollapseBlock / 64];
     __shared__ unsigned long long s_base;
@@ -595,7 +596,7 @@ __global__ __launch_bounds__(kCollapseBlock) void k_collapse(LeafBoxes lb, Binar
             uint64_t key = 0;
             if (valid) {
                 const uint32_t cnt = last - first + 1u;
-                const uint32_t prio = cnt > kLeafMax ? 2u : ((split_small && !(ref & kLeafBit) && cnt > 1u) ? 1u : 0u);
+                const uint32_t prio = cnt > leaf_max ? 2u : ((split_small && !(ref & kLeafBit) && cnt > 1u) ? 1u : 0u);
                 if (prio) key = ((uint64_t)prio << 40) | ((uint64_t)__float_as_uint(fmaxf(box_area(lo, hi), 0.0f)) << 3) | (7u - lane);
             }
             const uint64_t best = g8_max_u64(key);
@@ -647,7 +648,7 @@ __global__ __launch_bounds__(kCollapseBlock) void k_collapse(LeafBoxes lb, Binar
             used |= 1u << bs;
         }
         // ---- internal / leaf, ranks and triangle offsets in slot order
-        const bool internal = valid && cnt > kLeafMax;
+        const bool internal = valid && cnt > leaf_max;
         const uint32_t imask = g8_or(internal ? (1u << slot) : 0u);
         const uint32_t leafcnt = (valid && !internal) ? cnt : 0u;
         uint32_t toff = 0, ttotal = 0;
@@ -1071,7 +1072,8 @@ hipError_t bvh_validate(hipStream_t stream, const WideNode *nodes, uint32_t node
 hipError_t bvh_build(hipStream_t stream, void *scratch, size_t scratch_bytes, const void *vbase, uint64_t vstride,
                      uint64_t nverts, const uint32_t *tri, const float4 *boxes, uint32_t n, WideNode *nodes,
                      TriRecord *tris, uint32_t *prim_out, BuildCounters **counters_out, uint32_t split_small,
-                     int cu_count, uint32_t *error_flag, uint32_t *pinned_word) {
+                     uint32_t leaf_max, int cu_count, uint32_t *error_flag, uint32_t *pinned_word) {
+    if (leaf_max < 1 || leaf_max > kLeafMax) return hipErrorInvalidValue;
     if (scratch_bytes < bvh_scratch_bytes(n)) return hipErrorInvalidValue;
     // the collapse grid fills the CUs once (its loop strides over the level's items)
     static const int collapse_blocks_per_cu = [] {
@@ -1179,7 +1181,7 @@ hipError_t bvh_build(hipStream_t stream, void *scratch, size_t scratch_bytes, co
         const uint2 *qin = (level & 1) ? q1 : q0;
         uint2 *qout = (level & 1) ? q0 : q1;
         hipLaunchKernelGGL(k_collapse, dim3(grid_collapse), dim3(kCollapseBlock), 0, stream, lb, bt, ctr, level, qin,
-                           qout, out, split_small);
+                           qout, out, split_small, leaf_max);
         if ((level + 1) % kChunk == 0 && level + 1 < kCollapseLaunches) {
             e = hipMemcpyAsync(pinned_word, level_nodes_word(ctr, level + 1), sizeof(uint32_t), hipMemcpyDeviceToHost,
                                stream);

@@ -24,7 +24,7 @@ size_t bvh_node_capacity(uint32_t n);
 hipError_t bvh_build(hipStream_t stream, void *scratch, size_t scratch_bytes, const void *vbase, uint64_t vstride,
                      uint64_t nverts, const uint32_t *tri, const float4 *boxes, uint32_t n, WideNode *nodes,
                      TriRecord *tris, uint32_t *prim_out, BuildCounters **counters_out, uint32_t split_small,
-                     int cu_count, uint32_t *error_flag, uint32_t *pinned_word);
+                     uint32_t leaf_max, int cu_count, uint32_t *error_flag, uint32_t *pinned_word);
 hipError_t bvh_refit(hipStream_t stream, void *scratch, size_t scratch_bytes, const void *vbase, uint64_t vstride,
                      uint64_t nverts, const uint32_t *tri, uint32_t n, uint32_t depth, WideNode *nodes,
                      TriRecord *tris, BuildCounters **counters_out, int cu_count, uint32_t *error_flag);
@@ -172,7 +172,7 @@ void pt_release_stream(Device *dev, hipStream_t stream);
 // ---- serving loop (serve.cpp) ----
 struct ServeOptions {
     size_t hbm_budget = 1ull << 30; // accumulator arena (admission budget), bytes
-    size_t wave_paths = 8u << 20;   // path capacity of a scene's wave
+    size_t wave_paths = 32u << 20;  // path capacity of a scene's wave
     uint32_t max_active = 64;       // admitted requests at once
     PtOptions pt{};                 // wavefront options of the waves
 };

@@ -1019,7 +1019,7 @@ int pt_render(Device *dev, hipStream_t s, const Geometry &G, bool two, uint32_t 
     }
     // wavefront: one job per wave, chunk_spp samples per wave, accumulated in place in the image
     uint32_t chunk = opt.chunk_spp;
-    if (chunk == 0) chunk = (uint32_t)std::max<uint64_t>(1, (16ull << 20) / std::max<uint32_t>(1, npix));
+    if (chunk == 0) chunk = (uint32_t)std::max<uint64_t>(1, (32ull << 20) / std::max<uint32_t>(1, npix));
     chunk = std::min(chunk, S.spp);
     const size_t cap = (size_t)npix * chunk;
     PtBinding Bd;

@@ -13,8 +13,8 @@ def upload(device, stream, array):
     return buf
 
 
-def mesh_from_arrays(device, stream, verts4, tris, build=True):
-    """verts4: (n, 4) float32 (16-B stride); tris: (m, 3) uint32."""
+def mesh_from_arrays(device, stream, verts4, tris, build=True, option=None):
+    """verts4: (n, 4) float32 (16-B stride); tris: (m, 3) uint32; option: AccelOption or None."""
     verts4 = np.ascontiguousarray(verts4, np.float32)
     tris = np.ascontiguousarray(tris, np.uint32).reshape(-1, 3)
     if verts4.ndim != 2 or verts4.shape[1] != 4:
@@ -25,7 +25,8 @@ def mesh_from_arrays(device, stream, verts4, tris, build=True):
         stream << vb.copy_from(verts4)
     if tris.size:
         stream << tb.copy_from(tris)
-    mesh = device.create_mesh(vb, tb, vertex_size=16 * verts4.shape[0], triangle_size=12 * tris.shape[0])
+    mesh = device.create_mesh(vb, tb, option=option, vertex_size=16 * verts4.shape[0],
+                              triangle_size=12 * tris.shape[0])
     if build:
         stream << mesh.build()
     return mesh
