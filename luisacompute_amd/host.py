@@ -32,9 +32,9 @@ def mesh_from_arrays(device, stream, verts4, tris, build=True, option=None):
     return mesh
 
 
-def single_mesh_accel(device, stream, verts4, tris):
-    mesh = mesh_from_arrays(device, stream, verts4, tris)
-    accel = device.create_accel()
+def single_mesh_accel(device, stream, verts4, tris, option=None):
+    mesh = mesh_from_arrays(device, stream, verts4, tris, option=option)
+    accel = device.create_accel(option)
     accel.emplace_back(mesh, IDENTITY)
     stream << accel.build()
     accel._keep = [mesh]

@@ -178,3 +178,30 @@ def test_sorted_queue_matches_unsorted(device, stream, kind):
         compare_closest(b, sc.trace_closest2(lc.rays_as_floats(rays)), rays)
     else:
         compare_any(b, sc.trace_any(lc.rays_as_floats(rays)))
+
+
+@pytest.mark.parametrize("hint", ["fast_trace", "fast_build"])
+def test_leaf_size_hints_match_oracle(device, stream, hint):
+    """FAST_TRACE builds single-triangle leaf slots, FAST_BUILD up to three: both trace like the
+    oracle, and the trees satisfy the containment invariants."""
+    h = lc.AccelOption.FAST_TRACE if hint == "fast_trace" else lc.AccelOption.FAST_BUILD
+    verts, tris = lc.soup(1 << 16, initseq=91, extent=20.0, sigma=0.4)
+    rays = lc.rays_box_sphere(1 << 15, initseq=92, lo=-2.0, hi=22.0)
+    print(f"[hint {hint}]", flush=True)
+    opt = lc.AccelOption(h, 1, 0)
+    accel = host.single_mesh_accel(device, stream, verts, tris, option=opt)
+    g = host.trace_closest(device, stream, accel, rays)
+    ga = host.trace_any(device, stream, accel, rays)
+    sc = _oracle_scene(verts, tris)
+    rf = lc.rays_as_floats(host.as_rays(rays))
+    rep = compare_closest(g, sc.trace_closest2(rf), rays)
+    compare_any(ga, sc.trace_any(rf))
+    assert rep["hits"] > 0
+    mesh = accel._keep[0]
+    errs = mesh.validate(stream)
+    assert all(v == 0 for v in errs.values()), errs
+    st = mesh.stats()
+    assert st.leaf_primitive_count == len(tris)
+    # single-triangle leaves need more wide nodes than three-triangle leaves
+    if hint == "fast_trace":
+        assert st.node_count > len(tris) // 8
