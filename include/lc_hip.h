@@ -158,9 +158,27 @@ int lc_hip_accel_build(uint64_t device, uint64_t stream, uint64_t accel, uint32_
                        const lc_hip_accel_modification *modifications, size_t modification_count,
                        int update_instance_buffer_only);
 /* Instance queries of the DSL ops RAY_TRACING_INSTANCE_TRANSFORM / _USER_ID / _VISIBILITY_MASK
-   (include/luisa/ast/op.h:290-297), answered from the accel's instance table. */
+   (include/luisa/ast/op.h:290-297), answered from the accel's instance table (host call; device
+   writes issued since the last build are included). */
 int lc_hip_accel_instance_info(uint64_t device, uint64_t accel, uint32_t index, float affine[12],
                                uint32_t *vis_mask, uint32_t *user_id);
+/* The in-kernel instance ops RAY_TRACING_INSTANCE_TRANSFORM / _USER_ID / _VISIBILITY_MASK and
+   RAY_TRACING_SET_INSTANCE_* (op.h:290-297; CPU backend cpu/stream.rs:582-660, cpu/accel.rs:537-579)
+   as stream-ordered batches on the device: item i reads or writes `field` of instance
+   indices[i] (uint32 buffer). Transforms are 12 floats per item (row-major 3x4 object -> world,
+   as in lc_hip_accel_modification), user ids and visibility masks one uint32. Reads see the
+   latest writes; writes reach traversal at the accel's next build (as the reference's dirty
+   instances do at the next update). Out-of-range indices read zeros / are skipped and make the
+   next build (or instance info) fail with LC_HIP_ERROR_INVALID_ARGUMENT. Writes of one batch to
+   the same instance race, as in-kernel writes do. */
+#define LC_HIP_INSTANCE_TRANSFORM 0u
+#define LC_HIP_INSTANCE_USER_ID 1u
+#define LC_HIP_INSTANCE_VISIBILITY 2u
+int lc_hip_accel_instance_read(uint64_t device, uint64_t stream, uint64_t accel, uint32_t field, uint64_t index_buffer,
+                               size_t index_offset, uint64_t out_buffer, size_t out_offset, uint32_t count);
+int lc_hip_accel_instance_write(uint64_t device, uint64_t stream, uint64_t accel, uint32_t field,
+                                uint64_t index_buffer, size_t index_offset, uint64_t in_buffer, size_t in_offset,
+                                uint32_t count);
 
 /* ---- traversal: the lowering of CallOp::RAY_TRACING_TRACE_CLOSEST / _TRACE_ANY
    (src/dsl/rtx/accel.cpp:136-150 -> lc_accel_trace_closest / lc_accel_trace_any,

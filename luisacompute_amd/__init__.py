@@ -129,6 +129,7 @@ class ServingStats(ctypes.Structure):
 
 
 REQUEST_QUEUED, REQUEST_ACTIVE, REQUEST_DONE, REQUEST_FAILED = 0, 1, 2, 3
+INSTANCE_TRANSFORM, INSTANCE_USER_ID, INSTANCE_VISIBILITY = 0, 1, 2
 
 assert ctypes.sizeof(Modification) == 80 and ctypes.sizeof(AccelOption) == 8
 assert ctypes.sizeof(PtInstance) == 16 and ctypes.sizeof(PtScene) == 96 and ctypes.sizeof(PtOptions) == 64
@@ -168,6 +169,8 @@ _SIGNATURES = {
     "lc_hip_accel_destroy": (None, [_U64, _U64]),
     "lc_hip_accel_build": (ctypes.c_int, [_U64, _U64, _U64, _U32, _U32, ctypes.POINTER(Modification), _SZ, ctypes.c_int]),
     "lc_hip_accel_instance_info": (ctypes.c_int, [_U64, _U64, _U32, ctypes.POINTER(_F), ctypes.POINTER(_U32), ctypes.POINTER(_U32)]),
+    "lc_hip_accel_instance_read": (ctypes.c_int, [_U64, _U64, _U64, _U32, _U64, _SZ, _U64, _SZ, _U32]),
+    "lc_hip_accel_instance_write": (ctypes.c_int, [_U64, _U64, _U64, _U32, _U64, _SZ, _U64, _SZ, _U32]),
     "lc_hip_accel_trace_closest": (ctypes.c_int, [_U64, _U64, _U64, _U64, _SZ, _U64, _SZ, _U32, _U32]),
     "lc_hip_accel_trace_any": (ctypes.c_int, [_U64, _U64, _U64, _U64, _SZ, _U64, _SZ, _U32, _U32]),
     "lc_hip_accel_trace_count": (ctypes.c_int, [_U64, _U64, _U64, _U64, _SZ, _U32, _U32, ctypes.c_int,
@@ -609,6 +612,20 @@ class Accel:
         _check(lib().lc_hip_accel_instance_info(self.device.handle, self.handle, index, a, ctypes.byref(vis),
                                                 ctypes.byref(uid)), "instance info")
         return np.array(a[:], np.float32), vis.value, uid.value
+
+    # in-kernel instance ops (RAY_TRACING_[SET_]INSTANCE_*) as device batches over index buffers;
+    # field: INSTANCE_TRANSFORM (12 floats per item), INSTANCE_USER_ID, INSTANCE_VISIBILITY (uint32)
+    def instance_read(self, stream, field, index_buffer, out_buffer, count=None, index_offset=0, out_offset=0):
+        n = index_buffer.count if count is None else count
+        _check(lib().lc_hip_accel_instance_read(self.device.handle, stream.handle, self.handle, field,
+                                                index_buffer.handle, index_offset, out_buffer.handle, out_offset, n),
+               "instance read")
+
+    def instance_write(self, stream, field, index_buffer, in_buffer, count=None, index_offset=0, in_offset=0):
+        n = index_buffer.count if count is None else count
+        _check(lib().lc_hip_accel_instance_write(self.device.handle, stream.handle, self.handle, field,
+                                                 index_buffer.handle, index_offset, in_buffer.handle, in_offset, n),
+               "instance write")
 
     def stats(self):
         s = BvhStats()

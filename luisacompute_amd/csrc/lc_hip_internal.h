@@ -31,6 +31,23 @@ hipError_t bvh_refit(hipStream_t stream, void *scratch, size_t scratch_bytes, co
 hipError_t bvh_validate(hipStream_t stream, const WideNode *nodes, uint32_t node_count, const TriRecord *tris,
                         uint32_t tri_count, uint32_t nprims, uint32_t *marks, uint32_t *err7);
 
+// ---- kernels (instances.hip) ----
+// An accel's user-visible instance fields on the device (the table the in-kernel instance ops
+// read and write). dirty: bit f set when field f was written since the last build.
+enum { kInstanceTransform = 0, kInstanceUserId = 1, kInstanceVisibility = 2 };
+struct InstanceState {
+    float affine[12];     // object -> world, row-major 3x4
+    uint32_t user_id;
+    uint32_t vis_mask;
+    uint32_t dirty;
+    uint32_t _pad;
+};
+static_assert(sizeof(InstanceState) == 64, "InstanceState is 64 bytes");
+// Item i reads (write = false: into data) or writes (from data) field `field` of instance idx[i];
+// 12 floats per item for transforms, one uint32 otherwise. Indices >= count set *err.
+hipError_t instance_ops(hipStream_t stream, InstanceState *st, uint32_t count, uint32_t field, bool write,
+                        const uint32_t *idx, uint32_t n, void *data, uint32_t *err);
+
 // ---- kernels (trace.hip) ----
 // A batch of rays traced by one launch: rays [start, start + count) of the launch's queue are
 // this batch's rays 0..count-1. Up to kMaxBatches batches ride in the kernel arguments.
@@ -329,6 +346,14 @@ struct Accel {
     bool built = false;
     uint32_t max_blas_depth = 0;
     float world_bounds[6] = {0, 0, 0, 0, 0, 0}; // union of instance world boxes (ray-sort keys)
+    // device-side instance table for the in-kernel instance ops (state[count] is the
+    // out-of-range-index flag word); written at every build, folded back into `inst` before the
+    // next build when instance writes were issued
+    InstanceState *state = nullptr;
+    std::vector<InstanceState> state_host; // staging of the last upload
+    size_t state_cap = 0;
+    uint32_t state_count = 0;
+    bool state_written = false;
 };
 
 } // namespace lchip

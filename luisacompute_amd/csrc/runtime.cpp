@@ -689,7 +689,28 @@ void lc_hip_accel_destroy(uint64_t device, uint64_t accel) {
     if (a->tlas_scratch) hipFree(a->tlas_scratch);
     if (a->tlas_host) hipHostFree(a->tlas_host);
     if (a->tlas_ready) hipEventDestroy(a->tlas_ready);
+    if (a->state) hipFree(a->state);
     delete a;
+}
+
+// Folds instance writes issued on the device (lc_hip_accel_instance_write) into the host
+// instance set. Called with the accel locked and its device current.
+static int fold_instance_writes(Accel *a) {
+    if (!a->state_written || !a->state) return LC_HIP_OK;
+    LC_HIP_TRY(hipDeviceSynchronize());
+    std::vector<InstanceState> st((size_t)a->state_count + 1);
+    LC_HIP_TRY(hipMemcpy(st.data(), a->state, st.size() * sizeof(InstanceState), hipMemcpyDeviceToHost));
+    a->state_written = false;
+    uint32_t err = 0;
+    std::memcpy(&err, &st[a->state_count], sizeof(err));
+    for (uint32_t i = 0; i < a->state_count && i < a->inst.size(); i++) {
+        const InstanceState &x = st[i];
+        if (x.dirty & (1u << kInstanceTransform)) std::memcpy(a->inst[i].affine, x.affine, sizeof(float) * 12);
+        if (x.dirty & (1u << kInstanceUserId)) a->inst[i].user_id = x.user_id;
+        if (x.dirty & (1u << kInstanceVisibility)) a->inst[i].vis_mask = x.vis_mask;
+    }
+    if (err) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "an instance read or write named an instance out of range");
+    return LC_HIP_OK;
 }
 
 static void xform_point_h(const float m[12], const float p[3], float out[3]) {
@@ -707,6 +728,12 @@ int lc_hip_accel_build(uint64_t device, uint64_t stream, uint64_t accel, uint32_
     std::lock_guard<std::mutex> lk(a->mu);
     DeviceGuard g(dev);
     hipStream_t s = native_stream(stream);
+    // instance writes issued on the device since the last build come first (cpu/accel.rs:335-343
+    // applies dirty instances before the modifications)
+    {
+        const int rc = fold_instance_writes(a);
+        if (rc != LC_HIP_OK) return rc;
+    }
     // AccelImpl::update (cpu/accel.rs:324-447): resize, then apply each modification in order.
     a->inst.resize(instance_count);
     for (size_t k = 0; k < nmods; k++) {
@@ -761,6 +788,24 @@ int lc_hip_accel_build(uint64_t device, uint64_t stream, uint64_t accel, uint32_
     if (instance_count) {
         LC_HIP_TRY(hipMemcpyAsync(a->records, a->host_records.data(), instance_count * sizeof(InstanceRecord),
                                   hipMemcpyHostToDevice, s));
+    }
+    // the device-side instance table the in-kernel instance ops see (state[count] = error word)
+    {
+        void *sp = a->state;
+        size_t scap = a->state_cap * sizeof(InstanceState);
+        LC_HIP_TRY(ensure(sp, scap, ((size_t)instance_count + 1) * sizeof(InstanceState)));
+        a->state = static_cast<InstanceState *>(sp);
+        a->state_cap = scap / sizeof(InstanceState);
+        a->state_host.assign((size_t)instance_count + 1, InstanceState{});
+        for (uint32_t i = 0; i < instance_count; i++) {
+            std::memcpy(a->state_host[i].affine, a->inst[i].affine, sizeof(float) * 12);
+            a->state_host[i].user_id = a->inst[i].user_id;
+            a->state_host[i].vis_mask = a->inst[i].vis_mask;
+        }
+        LC_HIP_TRY(hipMemcpyAsync(a->state, a->state_host.data(), a->state_host.size() * sizeof(InstanceState),
+                                  hipMemcpyHostToDevice, s));
+        a->state_count = instance_count;
+        a->state_written = false;
     }
     if (update_instance_buffer_only) {
         LC_HIP_TRY(hipStreamSynchronize(s)); // host_records is reused by the next build
@@ -906,11 +951,52 @@ int lc_hip_accel_instance_info(uint64_t device, uint64_t accel, uint32_t index, 
     Accel *a = obj<Accel>(accel);
     if (!obj<Device>(device) || !a) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad accel");
     std::lock_guard<std::mutex> lk(a->mu);
+    DeviceGuard g(a->dev);
+    {
+        const int rc = fold_instance_writes(a);
+        if (rc != LC_HIP_OK) return rc;
+    }
     if (index >= a->inst.size()) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "instance index out of range");
     if (affine) std::memcpy(affine, a->inst[index].affine, sizeof(float) * 12);
     if (vis_mask) *vis_mask = a->inst[index].vis_mask;
     if (user_id) *user_id = a->inst[index].user_id;
     return LC_HIP_OK;
+}
+
+// In-kernel instance ops as batches over index buffers (see lc_hip.h).
+static int instance_batch(uint64_t device, uint64_t stream, uint64_t accel, uint32_t field, bool write,
+                          uint64_t index_buffer, size_t index_offset, uint64_t data_buffer, size_t data_offset,
+                          uint32_t count) {
+    Device *dev = obj<Device>(device);
+    Accel *a = obj<Accel>(accel);
+    Buffer *ib = obj<Buffer>(index_buffer), *db = obj<Buffer>(data_buffer);
+    if (!dev || !a || !ib || !db) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad instance op handles");
+    if (field > kInstanceVisibility) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "instance field: 0 transform, 1 user id, 2 visibility");
+    const size_t item = field == kInstanceTransform ? 12 * sizeof(float) : sizeof(uint32_t);
+    if (index_offset % 4 != 0 || data_offset % 4 != 0) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "offsets must be 4-byte aligned");
+    if (index_offset + (size_t)count * 4 > ib->bytes || data_offset + (size_t)count * item > db->bytes)
+        return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "instance op range outside its buffer");
+    std::lock_guard<std::mutex> lk(a->mu);
+    if (!a->state) return fail(LC_HIP_ERROR_NOT_BUILT, "accel not built");
+    DeviceGuard g(dev);
+    hipStream_t s = native_stream(stream);
+    const uint32_t *idx = reinterpret_cast<const uint32_t *>(static_cast<const char *>(ib->ptr) + index_offset);
+    void *data = static_cast<char *>(db->ptr) + data_offset;
+    uint32_t *err = reinterpret_cast<uint32_t *>(a->state + a->state_count);
+    LC_HIP_TRY(instance_ops(s, a->state, a->state_count, field, write, idx, count, data, err));
+    if (write || count) a->state_written = true; // also reports out-of-range reads at the next fold
+    return LC_HIP_OK;
+}
+
+int lc_hip_accel_instance_read(uint64_t device, uint64_t stream, uint64_t accel, uint32_t field, uint64_t index_buffer,
+                               size_t index_offset, uint64_t out_buffer, size_t out_offset, uint32_t count) {
+    return instance_batch(device, stream, accel, field, false, index_buffer, index_offset, out_buffer, out_offset, count);
+}
+
+int lc_hip_accel_instance_write(uint64_t device, uint64_t stream, uint64_t accel, uint32_t field,
+                                uint64_t index_buffer, size_t index_offset, uint64_t in_buffer, size_t in_offset,
+                                uint32_t count) {
+    return instance_batch(device, stream, accel, field, true, index_buffer, index_offset, in_buffer, in_offset, count);
 }
 
 // One launch over up to kMaxBatches ray batches (queue = the batches back to back).
