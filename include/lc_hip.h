@@ -59,6 +59,20 @@ typedef struct lc_hip_surface_hit {
     uint32_t _pad;
 } lc_hip_surface_hit;
 
+/* CommittedHit: include/luisa/runtime/rtx/hit.h:18-26 (24 B, align 8), the result of a ray
+   query. hit_type: LC_HIP_HIT_MISS / _SURFACE / _PROCEDURAL (HitType, hit.h:7-14). Miss: inst =
+   prim = ~0u, bary = (0, 0), committed_ray_t = ray t_max; procedural hits report bary = (0, 0). */
+#define LC_HIP_HIT_MISS 0u
+#define LC_HIP_HIT_SURFACE 1u
+#define LC_HIP_HIT_PROCEDURAL 2u
+typedef struct lc_hip_committed_hit {
+    uint32_t inst;
+    uint32_t prim;
+    float bary[2];
+    uint32_t hit_type;
+    float committed_ray_t;
+} lc_hip_committed_hit;
+
 /* AccelOption: include/luisa/runtime/rhi/resource.h:94-126. FAST_TRACE builds one primitive per
    BVH leaf slot, FAST_BUILD up to three (fewer nodes, shorter build, more triangle tests). */
 #define LC_HIP_ACCEL_FAST_TRACE 0u
@@ -149,6 +163,17 @@ int lc_hip_mesh_build(uint64_t device, uint64_t stream, uint64_t mesh, uint32_t 
                       size_t vertex_offset, size_t vertex_size, size_t vertex_stride, uint64_t triangle_buffer,
                       size_t triangle_offset, size_t triangle_size);
 
+/* ---- procedural primitives: create_procedural_primitive / destroy_procedural_primitive
+   (device_interface.h:175-176) and ProceduralPrimitiveBuildCommand (command.h:489-510): a BLAS
+   over a byte range of AABBs (6 floats, min then max: include/luisa/runtime/rtx/aabb.h:7-10).
+   An accel instances it like a mesh (its handle goes where a mesh handle goes); plain traces do
+   not see it (no intersection functions, cpu/accel.rs:476-484), ray queries report its
+   candidates to the procedural callback. ---- */
+uint64_t lc_hip_procedural_create(uint64_t device, const lc_hip_accel_option *option);
+void lc_hip_procedural_destroy(uint64_t device, uint64_t primitive);
+int lc_hip_procedural_build(uint64_t device, uint64_t stream, uint64_t primitive, uint32_t request,
+                            uint64_t aabb_buffer, size_t aabb_offset, size_t aabb_size);
+
 /* ---- accels (TLAS): create_accel / destroy_accel (device_interface.h:183-184) and
    AccelBuildCommand (command.h:531-616; semantics of AccelImpl::update,
    src/rust/luisa_compute_backend_impl/src/cpu/accel.rs:324-447). ---- */
@@ -209,6 +234,34 @@ int lc_hip_accel_trace_any_batched(uint64_t device, uint64_t stream, uint64_t ac
 int lc_hip_accel_trace_count(uint64_t device, uint64_t stream, uint64_t accel, uint64_t ray_buffer, size_t ray_offset,
                              uint32_t ray_count, uint32_t vis_mask, int any, uint64_t *nodes_visited,
                              uint64_t *triangles_tested);
+
+/* ---- ray queries: RayQueryAll / RayQueryAny (accel.traverse / traverse_any with
+   on_surface_candidate / on_procedural_candidate, include/luisa/dsl/rtx/ray_query.h:11-243;
+   semantics of AccelImpl::ray_query, src/rust/luisa_compute_backend_impl/src/cpu/accel.rs:582-800)
+   applied to a batch with built-in candidate callbacks: hits[i] (lc_hip_committed_hit) =
+   accel.traverse(rays[i], mask) (terminate_on_first = 0) or traverse_any (1). Surface candidates
+   of opaque instances are committed without a callback; of other instances per surface_mode:
+   COMMIT, IGNORE, or CUTOUT (commit iff with w = 1 - u - v the lengths |(w, u)|, |(u, v)|,
+   |(v, w)| are all below `cutout`, src/tests/test_procedural.cpp:86-97). Procedural candidates
+   per procedural_mode: IGNORE, or SPHERE (the sphere inscribed in the primitive's AABB, radius
+   half its smallest extent, hit by the object-space ray, test_procedural.cpp:98-120); a committed
+   distance counts inside [t_min, t_max). traverse returns the nearest committed candidate (ties:
+   smaller instance, then primitive); traverse_any the first one found. HIP kernels of their own
+   use the device template in ray_query.h with their own callbacks. ---- */
+#define LC_HIP_QUERY_SURFACE_COMMIT 0u
+#define LC_HIP_QUERY_SURFACE_IGNORE 1u
+#define LC_HIP_QUERY_SURFACE_CUTOUT 2u
+#define LC_HIP_QUERY_PROCEDURAL_IGNORE 0u
+#define LC_HIP_QUERY_PROCEDURAL_SPHERE 1u
+typedef struct lc_hip_query_policy {
+    uint32_t surface_mode;
+    float cutout;
+    uint32_t procedural_mode;
+    uint32_t terminate_on_first;
+} lc_hip_query_policy;
+int lc_hip_accel_query(uint64_t device, uint64_t stream, uint64_t accel, uint64_t ray_buffer, size_t ray_offset,
+                       uint64_t hit_buffer, size_t hit_offset, uint32_t ray_count, uint32_t vis_mask,
+                       const lc_hip_query_policy *policy);
 
 /* ---- path tracing: the DSL program of src/tests/test_path_tracing.cpp:165-258 (camera ray,
    closest hit, emission with the balance heuristic, one area-light sample with a shadow ray

@@ -32,7 +32,15 @@ RAY_DTYPE = np.dtype([("origin", "<f4", (3,)), ("t_min", "<f4"), ("direction", "
                      align=True)
 HIT_DTYPE = np.dtype([("inst", "<u4"), ("prim", "<u4"), ("bary", "<f4", (2,)), ("committed_ray_t", "<f4"),
                       ("_pad", "<u4")])
-assert RAY_DTYPE.itemsize == 32 and HIT_DTYPE.itemsize == 24
+# CommittedHit (include/luisa/runtime/rtx/hit.h:18-26): the result of a ray query
+COMMITTED_HIT_DTYPE = np.dtype([("inst", "<u4"), ("prim", "<u4"), ("bary", "<f4", (2,)), ("hit_type", "<u4"),
+                                ("committed_ray_t", "<f4")])
+assert RAY_DTYPE.itemsize == 32 and HIT_DTYPE.itemsize == 24 and COMMITTED_HIT_DTYPE.itemsize == 24
+# HitType (include/luisa/runtime/rtx/hit.h:7-14)
+HIT_MISS, HIT_SURFACE, HIT_PROCEDURAL = 0, 1, 2
+# built-in candidate callbacks of the batched ray query (lc_hip_query_policy)
+QUERY_SURFACE_COMMIT, QUERY_SURFACE_IGNORE, QUERY_SURFACE_CUTOUT = 0, 1, 2
+QUERY_PROCEDURAL_IGNORE, QUERY_PROCEDURAL_SPHERE = 0, 1
 
 
 class LCError(RuntimeError):
@@ -51,6 +59,12 @@ class AccelOption(ctypes.Structure):
 # PREFER_UPDATE like Mesh::build / Accel::build (include/luisa/runtime/rtx/mesh.h:120)
 BUILD_PREFER_UPDATE = 0
 BUILD_FORCE_BUILD = 1
+
+
+class QueryPolicy(ctypes.Structure):
+    """lc_hip_query_policy: built-in surface / procedural candidate callbacks of a ray query."""
+    _fields_ = [("surface_mode", ctypes.c_uint32), ("cutout", ctypes.c_float), ("procedural_mode", ctypes.c_uint32),
+                ("terminate_on_first", ctypes.c_uint32)]
 
 
 class Modification(ctypes.Structure):
@@ -165,6 +179,11 @@ _SIGNATURES = {
     "lc_hip_mesh_create": (_U64, [_U64, ctypes.POINTER(AccelOption)]),
     "lc_hip_mesh_destroy": (None, [_U64, _U64]),
     "lc_hip_mesh_build": (ctypes.c_int, [_U64, _U64, _U64, _U32, _U64, _SZ, _SZ, _SZ, _U64, _SZ, _SZ]),
+    "lc_hip_procedural_create": (_U64, [_U64, ctypes.POINTER(AccelOption)]),
+    "lc_hip_procedural_destroy": (None, [_U64, _U64]),
+    "lc_hip_procedural_build": (ctypes.c_int, [_U64, _U64, _U64, _U32, _U64, _SZ, _SZ]),
+    "lc_hip_accel_query": (ctypes.c_int, [_U64, _U64, _U64, _U64, _SZ, _U64, _SZ, _U32, _U32,
+                                          ctypes.POINTER(QueryPolicy)]),
     "lc_hip_accel_create": (_U64, [_U64, ctypes.POINTER(AccelOption)]),
     "lc_hip_accel_destroy": (None, [_U64, _U64]),
     "lc_hip_accel_build": (ctypes.c_int, [_U64, _U64, _U64, _U32, _U32, ctypes.POINTER(Modification), _SZ, ctypes.c_int]),
@@ -303,6 +322,16 @@ class _MeshBuild(Command):
                "mesh build")
 
 
+class _ProceduralBuild(Command):
+    def __init__(self, prim, request):
+        self.prim, self.request = prim, request
+
+    def execute(self, stream):
+        p = self.prim
+        _check(lib().lc_hip_procedural_build(stream.device.handle, stream.handle, p.handle, self.request, p.ab.handle,
+                                             p.offset, p.size), "procedural build")
+
+
 class _AccelBuild(Command):
     def __init__(self, accel, request, mods, instance_count, update_only):
         self.accel, self.request, self.mods, self.n, self.update_only = accel, request, mods, instance_count, update_only
@@ -360,6 +389,9 @@ class Device:
                     vertex_size=None, triangle_offset=0, triangle_size=None):
         return Mesh(self, vertex_buffer, triangle_buffer, option, vertex_stride, vertex_offset, vertex_size,
                     triangle_offset, triangle_size)
+
+    def create_procedural_primitive(self, aabb_buffer, option=None, offset=0, size=None):
+        return ProceduralPrimitive(self, aabb_buffer, option, offset, size)
 
     def create_accel(self, option=None):
         return Accel(self, option)
@@ -532,6 +564,32 @@ class Mesh:
         return dict(zip(self.VALIDATION_CHECKS, e[:]))
 
 
+class ProceduralPrimitive:
+    """BLAS over a buffer of AABBs (6 floats each: min, max; include/luisa/runtime/rtx/aabb.h:7-10),
+    ProceduralPrimitive (include/luisa/runtime/rtx/procedural_primitive.h). Instanced by an Accel
+    like a Mesh; reported to ray queries only."""
+
+    def __init__(self, device, aabb_buffer, option=None, offset=0, size=None):
+        self.device, self.ab, self.offset = device, aabb_buffer, offset
+        self.size = aabb_buffer.nbytes - offset if size is None else size
+        opt = option or AccelOption(AccelOption.FAST_TRACE, 1, 0)
+        self.handle = _handle(lib().lc_hip_procedural_create(device.handle, ctypes.byref(opt)),
+                              "procedural primitive create")
+
+    def __del__(self):
+        if getattr(self, "handle", None) is not None and _lib is not None and self.device.handle is not None:
+            lib().lc_hip_procedural_destroy(self.device.handle, self.handle)
+            self.handle = None
+
+    def build(self, request=BUILD_PREFER_UPDATE):
+        return _ProceduralBuild(self, request)
+
+    def stats(self):
+        s = BvhStats()
+        _check(lib().lc_hip_mesh_stats(self.device.handle, self.handle, ctypes.byref(s)), "procedural stats")
+        return s
+
+
 class Accel:
     """TLAS with the reference's modification protocol (src/runtime/rtx/accel.cpp:68-257)."""
 
@@ -642,6 +700,15 @@ class Accel:
         n = ray_buffer.count if count is None else count
         _check(lib().lc_hip_accel_trace_any(self.device.handle, stream.handle, self.handle, ray_buffer.handle,
                                             ray_offset, hit_buffer.handle, hit_offset, n, vis_mask), "trace_any")
+
+    # RayQueryAll / RayQueryAny (accel.traverse / traverse_any) over a batch with built-in callbacks;
+    # hits: COMMITTED_HIT_DTYPE per ray
+    def query(self, stream, ray_buffer, hit_buffer, count=None, vis_mask=0xFF, surface=QUERY_SURFACE_COMMIT,
+              cutout=0.8, procedural=QUERY_PROCEDURAL_SPHERE, terminate_on_first=False, ray_offset=0, hit_offset=0):
+        n = ray_buffer.count if count is None else count
+        pol = QueryPolicy(surface, cutout, procedural, 1 if terminate_on_first else 0)
+        _check(lib().lc_hip_accel_query(self.device.handle, stream.handle, self.handle, ray_buffer.handle, ray_offset,
+                                        hit_buffer.handle, hit_offset, n, vis_mask, ctypes.byref(pol)), "query")
 
     def trace_batched(self, stream, batches, any_hit=False):
         """batches: list of (ray_buffer, hit_buffer, count[, vis_mask]); one persistent launch per 64."""

@@ -27,9 +27,9 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unus
 HIP_FLAGS = [f"--offload-arch={ARCH}", "-fgpu-flush-denormals-to-zero",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-munsafe-fp-atomics"]
 
-SOURCES = ["bvh_build.hip", "trace.hip", "raysort.hip", "pathtracer.hip", "instances.hip", "serve.cpp", "runtime.cpp",
+SOURCES = ["bvh_build.hip", "trace.hip", "raysort.hip", "pathtracer.hip", "instances.hip", "query.hip", "serve.cpp", "runtime.cpp",
            "workload.cpp"]
-HEADERS = ["lc_hip_bvh.h", "lc_hip_internal.h", "traverse.h"]
+HEADERS = ["lc_hip_bvh.h", "lc_hip_internal.h", "traverse.h", "ray_query.h"]
 
 
 def _newer(src, dst):

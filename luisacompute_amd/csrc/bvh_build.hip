@@ -866,6 +866,26 @@ __global__ __launch_bounds__(kCollapseBlock) void k_refit_level(BuildCounters *c
     }
 }
 
+// ---- procedural primitives: AABBs (packed float3 min, float3 max: 24 B, include/luisa/runtime/
+// rtx/aabb.h:7-10) -> the builder's box layout, and leaf records in leaf order after the build.
+__global__ void k_aabbs_to_boxes(const float *aabb, uint32_t n, float4 *boxes) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float *a = aabb + 6 * (uint64_t)i;
+        boxes[2 * (uint64_t)i] = make_float4(a[0], a[1], a[2], 0.0f);
+        boxes[2 * (uint64_t)i + 1] = make_float4(a[3], a[4], a[5], 0.0f);
+    }
+}
+
+__global__ void k_proc_records(const uint32_t *ids, const float4 *boxes, uint32_t n, ProcRecord *out) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t p = ids[i];
+        const float4 lo = boxes[2 * (uint64_t)p], hi = boxes[2 * (uint64_t)p + 1];
+        float4 *o = reinterpret_cast<float4 *>(out + i);
+        o[0] = make_float4(lo.x, lo.y, lo.z, __uint_as_float(p));
+        o[1] = make_float4(hi.x, hi.y, hi.z, 0.0f);
+    }
+}
+
 // ---- validation (tests and debugging): structural and containment invariants of a built tree.
 // err[0] child index out of range      err[1] triangle range out of range
 // err[2] leaf triangle outside its slot box
@@ -1195,6 +1215,20 @@ hipError_t bvh_build(hipStream_t stream, void *scratch, size_t scratch_bytes, co
     if (tris)
         hipLaunchKernelGGL(k_tri_records, dim3(std::min<uint32_t>(grid, 1u << 20)), dim3(kBlock), 0, stream, pend,
                            (const TriRecord *)srec, tris, n);
+    return hipGetLastError();
+}
+
+hipError_t aabbs_to_boxes(hipStream_t stream, const float *aabb, uint32_t n, float4 *boxes) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_aabbs_to_boxes, dim3(std::min<uint32_t>((n + kBlock - 1) / kBlock, 1u << 16)), dim3(kBlock), 0,
+                       stream, aabb, n, boxes);
+    return hipGetLastError();
+}
+
+hipError_t proc_records(hipStream_t stream, const uint32_t *ids, const float4 *boxes, uint32_t n, ProcRecord *out) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_proc_records, dim3(std::min<uint32_t>((n + kBlock - 1) / kBlock, 1u << 16)), dim3(kBlock), 0,
+                       stream, ids, boxes, n, out);
     return hipGetLastError();
 }
 

@@ -40,19 +40,29 @@ struct alignas(16) TriRecord {
 };
 static_assert(sizeof(TriRecord) == 48, "TriRecord must be 48 bytes");
 
+// Leaf record of a procedural primitive (an AABB, ProceduralPrimitiveBuildCommand,
+// include/luisa/runtime/rhi/command.h:489-510): lo.w holds the primitive index bits. A procedural
+// BLAS has one primitive per leaf slot.
+struct alignas(16) ProcRecord {
+    float lo[4];
+    float hi[4];
+};
+static_assert(sizeof(ProcRecord) == 32, "ProcRecord must be 32 bytes");
+
 // Instance as seen by traversal: world->object affine (row-major 3x4), BLAS pointers.
 struct alignas(16) InstanceRecord {
     float inv[12];       // world -> object
     float fwd[12];       // object -> world (reported by instance_transform)
     uint64_t blas_nodes; // device address of the BLAS WideNode array (0 = no geometry)
-    uint64_t blas_tris;  // device address of the BLAS TriRecord array
+    uint64_t blas_tris;  // device address of the BLAS TriRecord (ProcRecord if procedural) array
     uint32_t vis_mask;   // instance visibility (default 0xff)
     uint32_t user_id;
-    uint32_t flags;      // bit0: opaque
+    uint32_t flags;      // kInstOpaque | kInstProcedural
     uint32_t blas_depth; // wide depth of the BLAS
 };
 static_assert(sizeof(InstanceRecord) == 128, "InstanceRecord must be 128 bytes");
 
+constexpr uint32_t kInstOpaque = 1u, kInstProcedural = 2u; // InstanceRecord::flags
 constexpr uint32_t kLeafMax = 3;         // triangles per leaf slot (<= 3 keeps a node's tris in 24 bits)
 constexpr uint32_t kCollapseLaunches = 98;   // wide levels launched per build (depth <= 96)
 constexpr uint32_t kMaxCollapseLevels = 100; // per-level counter words

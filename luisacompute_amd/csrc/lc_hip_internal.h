@@ -28,6 +28,10 @@ hipError_t bvh_build(hipStream_t stream, void *scratch, size_t scratch_bytes, co
 hipError_t bvh_refit(hipStream_t stream, void *scratch, size_t scratch_bytes, const void *vbase, uint64_t vstride,
                      uint64_t nverts, const uint32_t *tri, uint32_t n, uint32_t depth, WideNode *nodes,
                      TriRecord *tris, BuildCounters **counters_out, int cu_count, uint32_t *error_flag);
+// Procedural primitives: packed 24-B AABBs -> 2 float4 per box; leaf records from the build's
+// leaf-order primitive ids.
+hipError_t aabbs_to_boxes(hipStream_t stream, const float *aabb, uint32_t n, float4 *boxes);
+hipError_t proc_records(hipStream_t stream, const uint32_t *ids, const float4 *boxes, uint32_t n, ProcRecord *out);
 hipError_t bvh_validate(hipStream_t stream, const WideNode *nodes, uint32_t node_count, const TriRecord *tris,
                         uint32_t tri_count, uint32_t nprims, uint32_t *marks, uint32_t *err7);
 
@@ -214,6 +218,36 @@ int serve_wait(Server *s, uint64_t id, double timeout_s);   // 0 done, 1 timeout
 int serve_drain(Server *s, double timeout_s);               // 0: every request finished, 1 timeout
 void serve_stats(Server *s, ServeStats *st);
 
+// ---- ray queries (query.hip, ray_query.h) ----
+struct QueryGeometry {
+    const WideNode *tlas_nodes;      // two-level form
+    const uint32_t *tlas_prims;
+    const InstanceRecord *instances; // instance 0 alone when `two` is 0
+    uint32_t two;
+    uint32_t empty;                  // nothing to traverse: every query misses
+};
+// built-in candidate callbacks of the batched query (values of lc_hip_query_policy)
+enum { kQuerySurfaceCommit = 0, kQuerySurfaceIgnore = 1, kQuerySurfaceCutout = 2 };
+enum { kQueryProceduralIgnore = 0, kQueryProceduralSphere = 1 };
+struct QueryParams {
+    const float4 *rays; // 2 float4 per ray
+    void *hits;         // 24-B CommittedHit per ray
+    uint64_t count;
+    uint32_t mask;
+    uint32_t first;     // terminate on the first commit (traverse_any)
+    QueryGeometry geo;
+    uint2 *spill;
+    uint32_t spill_depth;
+    uint32_t *error;
+    uint32_t surface_mode;
+    float cutout;
+    uint32_t procedural_mode;
+};
+int query_block();
+int query_lds_entries();
+int query_blocks_per_cu();
+hipError_t query_launch(const QueryParams &P, hipStream_t stream, int grid);
+
 bool trace_config_supported(int block, int slds);
 int trace_blocks_per_cu(bool any, bool two, bool count, int block, int slds);
 hipError_t trace_launch(const TraceParams &P, hipStream_t stream, bool any, bool two, bool count, int grid, int block,
@@ -307,6 +341,13 @@ struct Mesh {
     HostCounters stats{};
     uint64_t build_epoch = 0;     // bumps on every build or refit (TLAS records refer to it)
     uint64_t refits = 0;          // builds served by a refit
+    // procedural primitive (an AABB BLAS; created by lc_hip_procedural_create): leaf records in
+    // `proc`, the build's input boxes and leaf-order ids in proc_boxes / proc_ids
+    bool procedural = false;
+    ProcRecord *proc = nullptr;
+    float4 *proc_boxes = nullptr;
+    uint32_t *proc_ids = nullptr;
+    size_t proc_cap = 0;
     // buffer ranges of the last build: an update over the same ranges refits
     const char *v_addr = nullptr;
     const uint32_t *t_addr = nullptr;

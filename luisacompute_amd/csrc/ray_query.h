@@ -1,0 +1,210 @@
+// ray_query.h -- ray queries over the BVH8 layout: traverse / traverse_any with surface and
+// procedural candidate callbacks (include/luisa/dsl/rtx/ray_query.h:11-243), for HIP kernels that
+// include this header and pass their callbacks as a policy object.
+//
+// Semantics follow the reference CPU backend's AccelImpl::ray_query
+// (src/rust/luisa_compute_backend_impl/src/cpu/accel.rs:582-800):
+//   - every triangle the ray hits with t in [tmin, tmax] in an instance whose mask passes is a
+//     surface candidate; a candidate of an opaque instance is committed without the callback
+//     (the callback is disabled for opaque instances, cpu/accel.rs:378-393), otherwise the
+//     callback decides;
+//   - every procedural primitive whose AABB the ray reaches is a procedural candidate; the
+//     callback returns the distance it commits, which counts only inside [tmin, tmax);
+//   - traverse: the result is the nearest committed candidate; traverse_any (terminate on first):
+//     the first committed candidate ends the query;
+//   - result: CommittedHit {inst, prim, bary, hit_type, committed_ray_t}
+//     (include/luisa/runtime/rtx/hit.h:18-26); a miss reports inst = prim = 0xFFFFFFFF,
+//     bary = (0, 0), hit_type 0, t = ray.tmax; a procedural hit reports bary = (0, 0).
+// Callbacks that are functions of the candidate alone make the nearest-commit result independent
+// of the visiting order; ties at equal t go to the smaller (instance, primitive) pair, as in
+// trace_closest.
+//
+// The walk is one ray per lane without the wave-uniform speculation of the trace walker: a
+// lane finishes a node's leaf records before it visits the next node. Procedural BLASes hold one
+// ProcRecord per leaf slot; the callback receives the object-space ray (and the world-space one),
+// so geometry inside the AABB does not depend on how conservatively boxes are culled.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cstdint>
+
+#include "lc_hip_bvh.h"
+#include "lc_hip_internal.h"
+#include "traverse.h"
+
+namespace lchip {
+namespace rq {
+
+constexpr uint32_t kHitMiss = 0u, kHitSurface = 1u, kHitProcedural = 2u;
+
+struct CommittedHit {
+    uint32_t inst, prim;
+    float u, v;
+    uint32_t type;
+    float t;
+};
+
+// Per-lane stack: the first SLDS entries in column `tid` of an [SLDS][BLOCK] LDS array, the rest
+// in a global spill area of `depth` entries (sized by the host from the built depths; the bound
+// check only guards memory and flags an error).
+template<int BLOCK, int SLDS>
+struct LaneStack {
+    uint2 (*lds)[BLOCK];
+    uint32_t tid;
+    uint2 *spill;
+    uint32_t depth;
+    uint32_t *error;
+    int sp;
+    __device__ __forceinline__ void push(uint2 e) {
+        if (sp < SLDS) lds[sp][tid] = e;
+        else if ((uint32_t)(sp - SLDS) < depth) spill[sp - SLDS] = e;
+        else atomicOr(error, 1u);
+        sp++;
+    }
+    __device__ __forceinline__ uint2 pop() {
+        sp--;
+        if (sp < SLDS) return lds[sp][tid];
+        if ((uint32_t)(sp - SLDS) < depth) return spill[sp - SLDS];
+        return make_uint2(0u, 0u);
+    }
+};
+
+// One query. Policy provides
+//   bool surface(uint32_t inst, uint32_t prim, float u, float v, float t) const
+//   bool procedural(uint32_t inst, uint32_t prim, const float obj_o[3], const float obj_d[3],
+//                   const float world_o[3], const float world_d[3], float4 lo, float4 hi,
+//                   float tmax, float &t) const
+// (`tmax`: the current committed distance, as candidate.ray().t_max() reports it).
+template<class Policy, int BLOCK, int SLDS>
+__device__ CommittedHit ray_query(const QueryGeometry &Q, LaneStack<BLOCK, SLDS> &S, const float4 r0, const float4 r1,
+                                  uint32_t ray_mask, bool first, const Policy &pol) {
+    CommittedHit best{0xFFFFFFFFu, 0xFFFFFFFFu, 0.0f, 0.0f, kHitMiss, r1.w};
+    if (Q.empty) return best;
+    const float wo[3] = {r0.x, r0.y, r0.z}, wd[3] = {r1.x, r1.y, r1.z};
+    const float tmin = r0.w, tmax = r1.w;
+    float t_best = FLT_MAX * 2.0f; // +inf
+    trav::RaySetup R;
+    const WideNode *nodes = Q.tlas_nodes;
+    const char *recs = nullptr;
+    uint32_t cur_inst = 0, cur_flags = 0;
+    bool in_blas = false;
+    int sp_inst = 0;
+    uint32_t g_base = 0, g_mask = 0, t_base = 0, t_mask = 0;
+    S.sp = 0;
+    // enters instance `id` if it has geometry and its mask passes (pushing what the TLAS walk
+    // still holds); false otherwise
+    auto enter = [&](uint32_t id) -> bool {
+        const InstanceRecord *ir = Q.instances + id;
+        const uint64_t bn = ir->blas_nodes;
+        if (bn == 0u || (ir->vis_mask & ray_mask & 0xFFu) == 0u) return false;
+        if (t_mask != 0u) S.push(make_uint2(t_base, t_mask | trav::kInstGroup));
+        t_mask = 0u;
+        if ((g_mask & 0xFFu) != 0u) S.push(make_uint2(g_base, g_mask));
+        trav::xform_point(ir->inv, wo, R.o);
+        trav::xform_vector(ir->inv, wd, R.d);
+        trav::ray_setup(R);
+        nodes = reinterpret_cast<const WideNode *>(bn);
+        recs = reinterpret_cast<const char *>(ir->blas_tris);
+        cur_inst = id;
+        cur_flags = ir->flags;
+        in_blas = true;
+        sp_inst = S.sp;
+        g_base = 0;
+        g_mask = (1u << R.oct) | (1u << 8);
+        return true;
+    };
+    if (Q.two) {
+        for (int k = 0; k < 3; k++) { R.o[k] = wo[k]; R.d[k] = wd[k]; }
+        trav::ray_setup(R);
+        g_mask = (1u << R.oct) | (1u << 8);
+    } else if (!enter(0u)) {
+        return best;
+    }
+    for (;;) {
+        if (t_mask != 0u) {
+            const uint32_t b = (uint32_t)__builtin_ctz(t_mask);
+            t_mask &= t_mask - 1u;
+            if (!in_blas) {
+                enter(Q.tlas_prims[t_base + b]);
+                continue;
+            }
+            const uint32_t rec = t_base + b;
+            if (cur_flags & kInstProcedural) {
+                const float4 *pr = reinterpret_cast<const float4 *>(recs + 32ull * rec);
+                const float4 lo = pr[0], hi = pr[1];
+                const uint32_t prim = __float_as_uint(lo.w);
+                float t = 0.0f;
+                const float t_far = fminf(tmax, t_best);
+                if (pol.procedural(cur_inst, prim, R.o, R.d, wo, wd, lo, hi, t_far, t) && t >= tmin && t < tmax &&
+                    trav::nearer(t, cur_inst, prim, t_best, best.inst, best.prim)) {
+                    t_best = t;
+                    best = CommittedHit{cur_inst, prim, 0.0f, 0.0f, kHitProcedural, t};
+                    if (first) return best;
+                }
+            } else {
+                const float4 *tr = reinterpret_cast<const float4 *>(recs + 48ull * rec);
+                const float4 a = tr[0], bb = tr[1], c = tr[2];
+                float t, u, v;
+                if (trav::wt_intersect(R, a, bb, c, tmin, tmax, t, u, v)) {
+                    const uint32_t prim = __float_as_uint(a.w);
+                    if (trav::nearer(t, cur_inst, prim, t_best, best.inst, best.prim) &&
+                        ((cur_flags & kInstOpaque) || pol.surface(cur_inst, prim, u, v, t))) {
+                        t_best = t;
+                        best = CommittedHit{cur_inst, prim, u, v, kHitSurface, t};
+                        if (first) return best;
+                    }
+                }
+            }
+            continue;
+        }
+        if ((g_mask & 0xFFu) == 0u) {
+            if (in_blas && S.sp == sp_inst) {
+                if (!Q.two) return best;
+                // leave the instance: back to the world-space ray and the TLAS
+                for (int k = 0; k < 3; k++) { R.o[k] = wo[k]; R.d[k] = wd[k]; }
+                trav::ray_setup(R);
+                nodes = Q.tlas_nodes;
+                in_blas = false;
+            }
+            if (S.sp == 0) return best;
+            const uint2 e = S.pop();
+            if (e.y & trav::kInstGroup) {
+                t_base = e.x;
+                t_mask = e.y & ~trav::kInstGroup;
+                continue;
+            }
+            g_base = e.x;
+            g_mask = e.y;
+        }
+        // visit the next node of the current group; its leaf records come before its children
+        uint32_t hm = g_mask & 0xFFu;
+        const uint32_t im = (g_mask >> 8) & 0xFFu;
+        const uint32_t i = (uint32_t)__builtin_ctz(hm);
+        hm &= hm - 1u;
+        const uint32_t slot = i ^ R.oct;
+        const uint32_t node = g_base + (uint32_t)__popc(im & ((1u << slot) - 1u));
+        if (hm != 0u) S.push(make_uint2(g_base, hm | (im << 8)));
+        uint4 w0, w1, w2, w3, w4;
+        trav::load_node(nodes, node, w0, w1, w2, w3, w4);
+        const float tcull = first ? tmax : fminf(tmax, t_best);
+        const uint32_t hits = trav::intersect_node(R, w0, w1, w2, w3, w4, tmin, tcull);
+        const uint32_t nim = w0.w >> 24;
+        g_base = w1.x;
+        g_mask = trav::permute8(hits & nim, R.oct) | (nim << 8);
+        uint32_t leaf = hits & ~nim;
+        uint32_t nt = 0u;
+        while (leaf) {
+            const uint32_t s = (uint32_t)__builtin_ctz(leaf);
+            leaf &= leaf - 1u;
+            const uint32_t m = ((s < 4 ? w1.z : w1.w) >> (8 * (s & 3))) & 0xFFu;
+            nt |= ((1u << (m >> 5)) - 1u) << (m & 31u);
+        }
+        t_base = w1.y;
+        t_mask = nt;
+    }
+}
+
+} // namespace rq
+} // namespace lchip

@@ -524,10 +524,78 @@ void lc_hip_mesh_destroy(uint64_t device, uint64_t mesh) {
     hipDeviceSynchronize();
     if (m->nodes) hipFree(m->nodes);
     if (m->tris) hipFree(m->tris);
+    if (m->proc) hipFree(m->proc);
+    if (m->proc_boxes) hipFree(m->proc_boxes);
+    if (m->proc_ids) hipFree(m->proc_ids);
     if (m->scratch) hipFree(m->scratch);
     if (m->host) hipHostFree(m->host);
     if (m->ready) hipEventDestroy(m->ready);
     delete m;
+}
+
+// ---- procedural primitives: create_procedural_primitive (device_interface.h:175-176) and
+// ProceduralPrimitiveBuildCommand (command.h:489-510): a BLAS over AABBs, one per leaf slot ----
+uint64_t lc_hip_procedural_create(uint64_t device, const lc_hip_accel_option *option) {
+    const uint64_t h = lc_hip_mesh_create(device, option);
+    if (h != LC_HIP_INVALID_HANDLE) obj<Mesh>(h)->procedural = true;
+    return h;
+}
+
+void lc_hip_procedural_destroy(uint64_t device, uint64_t primitive) { lc_hip_mesh_destroy(device, primitive); }
+
+int lc_hip_procedural_build(uint64_t device, uint64_t stream, uint64_t primitive, uint32_t, uint64_t aabb_buffer,
+                            size_t aabb_offset, size_t aabb_size) {
+    Device *dev = obj<Device>(device);
+    Mesh *m = obj<Mesh>(primitive);
+    Buffer *ab = obj<Buffer>(aabb_buffer);
+    if (!dev || !m || !ab) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad procedural build handles");
+    if (!m->procedural) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "not a procedural primitive");
+    if (aabb_offset % 4 != 0 || aabb_size % 24 != 0)
+        return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "AABB range must be 4-byte aligned and a multiple of 24 bytes");
+    if (aabb_offset + aabb_size > ab->bytes) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "AABB range outside its buffer");
+    const uint64_t n64 = aabb_size / 24;
+    if (n64 >= 0x7FFFFFFFull) return fail(LC_HIP_ERROR_UNSUPPORTED, "more than 2^31 primitives");
+    const uint32_t n = (uint32_t)n64;
+    std::lock_guard<std::mutex> lk(m->mu);
+    DeviceGuard g(dev);
+    hipStream_t s = native_stream(stream);
+    void *p;
+    size_t cap;
+    p = m->nodes; cap = m->node_cap * sizeof(WideNode);
+    LC_HIP_TRY(ensure(p, cap, bvh_node_capacity(n) * sizeof(WideNode)));
+    m->nodes = static_cast<WideNode *>(p); m->node_cap = cap / sizeof(WideNode);
+    const size_t pn = std::max<size_t>(n, 1);
+    if (pn > m->proc_cap) {
+        if (m->proc) hipFree(m->proc);
+        if (m->proc_boxes) hipFree(m->proc_boxes);
+        if (m->proc_ids) hipFree(m->proc_ids);
+        m->proc = nullptr; m->proc_boxes = nullptr; m->proc_ids = nullptr; m->proc_cap = 0;
+        LC_HIP_TRY(hipMalloc(&m->proc, pn * sizeof(ProcRecord)));
+        LC_HIP_TRY(hipMalloc(&m->proc_boxes, 2 * pn * sizeof(float4)));
+        LC_HIP_TRY(hipMalloc(&m->proc_ids, pn * sizeof(uint32_t)));
+        m->proc_cap = pn;
+    }
+    LC_HIP_TRY(ensure(m->scratch, m->scratch_bytes, bvh_scratch_bytes(n)));
+    m->ntris = n;
+    m->built = true;
+    m->stats_valid = false;
+    m->build_epoch++;
+    if (n == 0) return LC_HIP_OK;
+    {
+        std::lock_guard<std::mutex> dl(dev->mu);
+        LC_HIP_TRY(hipMemsetAsync(dev->error, 0, sizeof(uint32_t), s));
+        TimedRegion tr(dev, s, "procedural_build");
+        const float *aabb = reinterpret_cast<const float *>(static_cast<const char *>(ab->ptr) + aabb_offset);
+        LC_HIP_TRY(aabbs_to_boxes(s, aabb, n, m->proc_boxes));
+        BuildCounters *ctr = nullptr;
+        hipError_t e = bvh_build(s, m->scratch, m->scratch_bytes, nullptr, 0, 0, nullptr, m->proc_boxes, n, m->nodes,
+                                 nullptr, m->proc_ids, &ctr, 0, 1u, dev->cu_count, dev->error, &m->host->phase_count);
+        if (e != hipSuccess) return hip_fail(e, "procedural bvh_build");
+        LC_HIP_TRY(proc_records(s, m->proc_ids, m->proc_boxes, n, m->proc));
+        copy_counters_async(ctr, m->host, dev->error, s);
+    }
+    LC_HIP_TRY(hipEventRecord(m->ready, s));
+    return LC_HIP_OK;
 }
 
 int lc_hip_mesh_build(uint64_t device, uint64_t stream, uint64_t mesh, uint32_t request, uint64_t vertex_buffer,
@@ -537,6 +605,7 @@ int lc_hip_mesh_build(uint64_t device, uint64_t stream, uint64_t mesh, uint32_t 
     Mesh *m = obj<Mesh>(mesh);
     Buffer *vb = obj<Buffer>(vertex_buffer), *tb = obj<Buffer>(triangle_buffer);
     if (!dev || !m || !vb || !tb) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad mesh build handles");
+    if (m->procedural) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "a procedural primitive is built from AABBs");
     // src/runtime/rtx/mesh.cpp:29-49
     if (vertex_stride == 0 || vertex_stride % 16 != 0) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "vertex stride must be a multiple of 16");
     if (vertex_offset % vertex_stride != 0 || vertex_size % vertex_stride != 0)
@@ -767,13 +836,15 @@ int lc_hip_accel_build(uint64_t device, uint64_t stream, uint64_t accel, uint32_
         invert_affine(in.affine, r.inv);
         r.vis_mask = in.vis_mask;
         r.user_id = in.user_id;
-        r.flags = in.opaque ? 1u : 0u;
+        r.flags = in.opaque ? kInstOpaque : 0u;
         if (in.mesh) {
             int rc = mesh_stats_sync(in.mesh);
             if (rc != LC_HIP_OK) return rc;
             if (in.mesh->ntris > 0) {
                 r.blas_nodes = reinterpret_cast<uint64_t>(in.mesh->nodes);
-                r.blas_tris = reinterpret_cast<uint64_t>(in.mesh->tris);
+                r.blas_tris = in.mesh->procedural ? reinterpret_cast<uint64_t>(in.mesh->proc)
+                                                  : reinterpret_cast<uint64_t>(in.mesh->tris);
+                if (in.mesh->procedural) r.flags |= kInstProcedural;
                 r.blas_depth = in.mesh->stats.depth;
                 max_depth = std::max(max_depth, r.blas_depth);
             }
@@ -905,8 +976,8 @@ static int accel_geometry(Accel *a, Geometry &G, bool &two, uint32_t &stack_need
     stack_need = 0;
     if (!two) {
         const InstanceRecord *r = a->host_records.empty() ? nullptr : &a->host_records[0];
-        if (!r || r->blas_nodes == 0) {
-            G.empty = 1;
+        if (!r || r->blas_nodes == 0 || (r->flags & kInstProcedural)) {
+            G.empty = 1; // nothing, or a procedural primitive (seen by ray queries only)
         } else {
             G.blas_nodes = reinterpret_cast<const WideNode *>(r->blas_nodes);
             G.blas_tris = reinterpret_cast<const TriRecord *>(r->blas_tris);
@@ -1126,6 +1197,73 @@ int lc_hip_accel_trace_count(uint64_t device, uint64_t stream, uint64_t accel, u
                              uint64_t *triangles_tested) {
     const lc_hip_ray_batch b{ray_buffer, ray_offset, 0, 0, ray_count, vis_mask};
     return trace_batches(device, stream, accel, &b, 1, any != 0, true, nodes_visited, triangles_tested);
+}
+
+// Ray queries (accel.traverse / traverse_any, include/luisa/dsl/rtx/ray_query.h:11-243) over a
+// batch with the built-in candidate callbacks of query.hip. Unlike plain traces they see the
+// procedural instances too; stack entries: per wide level one node group (the walk finishes a
+// node's leaf records before its children), plus the TLAS leaf entries.
+int lc_hip_accel_query(uint64_t device, uint64_t stream, uint64_t accel, uint64_t ray_buffer, size_t ray_offset,
+                       uint64_t hit_buffer, size_t hit_offset, uint32_t ray_count, uint32_t vis_mask,
+                       const lc_hip_query_policy *policy) {
+    Device *dev = obj<Device>(device);
+    Accel *a = obj<Accel>(accel);
+    Buffer *rb = obj<Buffer>(ray_buffer), *hb = obj<Buffer>(hit_buffer);
+    if (!dev || !a || !rb || !hb) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad query handles");
+    if (!policy) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "null query policy");
+    if (policy->surface_mode > LC_HIP_QUERY_SURFACE_CUTOUT || policy->procedural_mode > LC_HIP_QUERY_PROCEDURAL_SPHERE)
+        return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "unknown query callback");
+    if (ray_offset % 16 != 0) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "ray offset must be 16-byte aligned");
+    if (hit_offset % 8 != 0) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "hit offset must be 8-byte aligned");
+    if (ray_offset + (size_t)ray_count * 32 > rb->bytes) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "rays outside the ray buffer");
+    if (hit_offset + (size_t)ray_count * 24 > hb->bytes) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "hits outside the hit buffer");
+    std::lock_guard<std::mutex> lk(a->mu);
+    if (!a->built) return fail(LC_HIP_ERROR_NOT_BUILT, "accel is not built");
+    DeviceGuard g(dev);
+    hipStream_t s = native_stream(stream);
+    if (ray_count == 0) return LC_HIP_OK;
+    QueryParams P;
+    std::memset(&P, 0, sizeof(P));
+    uint32_t stack_need = 0;
+    const uint32_t n_inst = (uint32_t)a->inst.size();
+    if (n_inst == 0) {
+        P.geo.empty = 1;
+    } else if (n_inst == 1) {
+        const InstanceRecord &r = a->host_records[0];
+        P.geo.instances = a->records;
+        P.geo.empty = r.blas_nodes == 0 ? 1u : 0u;
+        stack_need = r.blas_depth + 2;
+    } else {
+        int rc = accel_tlas_sync(a);
+        if (rc != LC_HIP_OK) return rc;
+        P.geo.two = 1;
+        P.geo.tlas_nodes = a->tlas_nodes;
+        P.geo.tlas_prims = a->tlas_prims;
+        P.geo.instances = a->records;
+        stack_need = a->tlas_stats.depth + 2 + a->max_blas_depth + 2;
+    }
+    P.rays = reinterpret_cast<const float4 *>(static_cast<char *>(rb->ptr) + ray_offset);
+    P.hits = static_cast<char *>(hb->ptr) + hit_offset;
+    P.count = ray_count;
+    P.mask = vis_mask;
+    P.first = policy->terminate_on_first ? 1u : 0u;
+    P.surface_mode = policy->surface_mode;
+    P.cutout = policy->cutout;
+    P.procedural_mode = policy->procedural_mode;
+    std::lock_guard<std::mutex> dl(dev->mu);
+    const int block = query_block();
+    const uint64_t want = ((uint64_t)ray_count + block - 1) / block;
+    const int grid = (int)std::max<uint64_t>(
+        1, std::min<uint64_t>((uint64_t)query_blocks_per_cu() * std::max(1, dev->cu_count), want));
+    const uint32_t slds = (uint32_t)query_lds_entries();
+    P.spill_depth = stack_need > slds ? stack_need - slds : 0u;
+    LC_HIP_TRY(ensure(dev->spill, dev->spill_bytes, (size_t)grid * block * P.spill_depth * sizeof(uint2)));
+    P.spill = static_cast<uint2 *>(dev->spill);
+    P.error = dev->error;
+    TimedRegion tr(dev, s, P.first ? "query_any" : "query");
+    hipError_t e = query_launch(P, s, grid);
+    if (e != hipSuccess) return hip_fail(e, "query launch");
+    return LC_HIP_OK;
 }
 
 // Light and camera of a lc_hip_pt_scene; the light's normal and area are normalize(cross(u, v))

@@ -306,7 +306,9 @@ struct Walker {
                     const InstanceRecord *ir = G.instances + id;
                     const uint32_t imask = ir->vis_mask;
                     const uint64_t bn = ir->blas_nodes;
-                    if (bn != 0u && (imask & ray_mask & 0xFFu) != 0u) {
+                    // procedural primitives report only to ray queries (plain traces run without
+                    // intersection functions, cpu/accel.rs:476-484)
+                    if (bn != 0u && (imask & ray_mask & 0xFFu) != 0u && !(ir->flags & kInstProcedural)) {
                         if (t_mask != 0u) push(make_uint2(t_base, t_mask | kInstGroup));
                         t_mask = 0u;
                         if ((g_mask & 0xFFu) != 0u) push(make_uint2(g_base, g_mask));

@@ -1,7 +1,8 @@
 """Host conveniences over the runtime mirror: numpy arrays in, numpy arrays out."""
 import numpy as np
 
-from . import HIT_DTYPE, RAY_DTYPE, Accel, Device, IDENTITY, LCError, synchronize
+from . import (COMMITTED_HIT_DTYPE, HIT_DTYPE, QUERY_PROCEDURAL_SPHERE, QUERY_SURFACE_COMMIT, RAY_DTYPE, Accel, Device,
+               IDENTITY, LCError, synchronize)
 
 
 def upload(device, stream, array):
@@ -30,6 +31,18 @@ def mesh_from_arrays(device, stream, verts4, tris, build=True, option=None):
     if build:
         stream << mesh.build()
     return mesh
+
+
+def procedural_from_aabbs(device, stream, aabbs, build=True, option=None):
+    """aabbs: (n, 6) float32 rows min.xyz, max.xyz; returns the built ProceduralPrimitive."""
+    aabbs = np.ascontiguousarray(aabbs, np.float32).reshape(-1, 6)
+    ab = device.create_buffer(24, max(1, aabbs.shape[0]))
+    if aabbs.size:
+        stream << ab.copy_from(aabbs)
+    prim = device.create_procedural_primitive(ab, option=option, size=24 * aabbs.shape[0])
+    if build:
+        stream << prim.build()
+    return prim
 
 
 def single_mesh_accel(device, stream, verts4, tris, option=None):
@@ -68,5 +81,19 @@ def trace_any(device, stream, accel, rays, vis_mask=0xFF):
     out = np.zeros(n, np.uint8)
     if n:
         accel.trace_any(stream, rb, hb, n, vis_mask)
+        stream << hb.copy_to(out) << synchronize()
+    return out
+
+
+def query(device, stream, accel, rays, vis_mask=0xFF, surface=QUERY_SURFACE_COMMIT, cutout=0.8,
+          procedural=QUERY_PROCEDURAL_SPHERE, terminate_on_first=False):
+    """Ray query per ray with built-in callbacks; COMMITTED_HIT_DTYPE array."""
+    rays = as_rays(rays)
+    n = rays.shape[0]
+    rb = upload(device, stream, rays)
+    hb = device.create_buffer(24, max(1, n))
+    out = np.zeros(n, COMMITTED_HIT_DTYPE)
+    if n:
+        accel.query(stream, rb, hb, n, vis_mask, surface, cutout, procedural, terminate_on_first)
         stream << hb.copy_to(out) << synchronize()
     return out

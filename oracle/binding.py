@@ -14,11 +14,19 @@ _LIB_PATH = os.path.join(_HERE, "liblc_oracle.so")
 
 HIT_DTYPE = np.dtype([("inst", "<u4"), ("prim", "<u4"), ("u", "<f4"), ("v", "<f4"), ("t", "<f4"), ("pad", "<u4")])
 HIT2_DTYPE = np.dtype([("inst", "<u4"), ("prim", "<u4"), ("u", "<f4"), ("v", "<f4"), ("t", "<f4"), ("t2", "<f4")])
+COMMITTED_DTYPE = np.dtype([("inst", "<u4"), ("prim", "<u4"), ("u", "<f4"), ("v", "<f4"), ("hit_type", "<u4"),
+                            ("t", "<f4")])
 BF_DTYPE = np.dtype([("prim", "<u4"), ("pad", "<u4"), ("t", "<f8"), ("u", "<f8"), ("v", "<f8"), ("t2", "<f8")])
 
 
 class Pcg32(ctypes.Structure):
     _fields_ = [("state", ctypes.c_uint64), ("inc", ctypes.c_uint64)]
+
+
+class QueryPolicy(ctypes.Structure):
+    """orc_query_policy: surface_mode 0 commit / 1 ignore / 2 cutout; procedural_mode 0 ignore / 1 sphere."""
+    _fields_ = [("surface_mode", ctypes.c_uint32), ("cutout", ctypes.c_float), ("procedural_mode", ctypes.c_uint32),
+                ("terminate_on_first", ctypes.c_uint32)]
 
 
 class PtParams(ctypes.Structure):
@@ -65,6 +73,12 @@ def lib():
             "orc_mesh_node_count": (U64, [P]),
             "orc_scene_create": (P, []),
             "orc_scene_add": (None, [P, P, P, U32]),
+            "orc_scene_add_ex": (None, [P, P, P, P, U32, U32]),
+            "orc_procedural_create": (P, [P, U64]),
+            "orc_procedural_free": (None, [P]),
+            "orc_query": (None, [P, P, U64, U32, ctypes.POINTER(QueryPolicy), P, I]),
+            "orc_query_candidate": (I, [P, P, U32, ctypes.POINTER(QueryPolicy), U32, U32, U32, ctypes.POINTER(F),
+                                        ctypes.POINTER(F), ctypes.POINTER(F)]),
             "orc_scene_commit": (None, [P]),
             "orc_scene_free": (None, [P]),
             "orc_trace_closest": (None, [P, P, U64, U32, P, I]),
@@ -162,6 +176,19 @@ class Mesh:
             self.h = None
 
 
+class Procedural:
+    """AABB list (n, 6): min.xyz, max.xyz per primitive."""
+
+    def __init__(self, aabbs):
+        self.aabbs = np.ascontiguousarray(aabbs, np.float32).reshape(-1, 6)
+        self.h = lib().orc_procedural_create(_p(self.aabbs), self.aabbs.shape[0])
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_procedural_free(self.h)
+            self.h = None
+
+
 IDENTITY = np.array([1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0], np.float32)
 
 
@@ -172,11 +199,32 @@ class Scene:
         self.h = lib().orc_scene_create()
         self._keep = []
 
-    def add(self, mesh, affine=IDENTITY, mask=0xFF):
+    def add(self, mesh, affine=IDENTITY, mask=0xFF, opaque=True):
+        """mesh: a Mesh or a Procedural."""
         a = np.ascontiguousarray(np.asarray(affine, np.float32).reshape(12))
         self._keep.append((mesh, a))
-        lib().orc_scene_add(self.h, mesh.h, _p(a), mask)
+        if isinstance(mesh, Procedural):
+            lib().orc_scene_add_ex(self.h, None, mesh.h, _p(a), mask, 1 if opaque else 0)
+        else:
+            lib().orc_scene_add_ex(self.h, mesh.h, None, _p(a), mask, 1 if opaque else 0)
         return self
+
+    def query(self, rays, mask=0xFF, surface=0, cutout=0.8, procedural=1, nthreads=0):
+        """Nearest committed candidate per ray (COMMITTED_DTYPE)."""
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        out = np.zeros(rays.shape[0], COMMITTED_DTYPE)
+        pol = QueryPolicy(surface, cutout, procedural, 0)
+        lib().orc_query(self.h, _p(rays), rays.shape[0], mask, ctypes.byref(pol), _p(out), nthreads)
+        return out
+
+    def query_candidate(self, ray, inst, prim, hit_type, mask=0xFF, surface=0, cutout=0.8, procedural=1):
+        """(t, u, v) if candidate (inst, prim) of kind hit_type is committed for `ray`, else None."""
+        ray = np.ascontiguousarray(ray, np.float32).reshape(8)
+        pol = QueryPolicy(surface, cutout, procedural, 0)
+        t, u, v = ctypes.c_float(), ctypes.c_float(), ctypes.c_float()
+        ok = lib().orc_query_candidate(self.h, _p(ray), mask, ctypes.byref(pol), int(inst), int(prim), int(hit_type),
+                                       ctypes.byref(t), ctypes.byref(u), ctypes.byref(v))
+        return (t.value, u.value, v.value) if ok else None
 
     def commit(self):
         lib().orc_scene_commit(self.h)

@@ -309,11 +309,18 @@ struct Mesh {
     Box bounds;
 };
 
+// Procedural primitive: AABBs (6 floats: min.xyz, max.xyz), include/luisa/runtime/rtx/aabb.h:7-10.
+struct Procedural {
+    std::vector<float> aabb;
+};
+
 struct Instance {
-    const Mesh *mesh;
+    const Mesh *mesh;       // null for a procedural instance
+    const Procedural *proc; // null for a mesh instance
     float fwd[12]; // object -> world, row-major 3x4
     float inv[12]; // world -> object, row-major 3x4
     uint32_t mask;
+    bool opaque;
 };
 
 struct Scene {
@@ -432,7 +439,9 @@ bool trace_scene(const Scene &s, const float *ray, uint32_t mask, TraceState &st
             for (uint32_t k = 0; k < n.count; k++) {
                 uint32_t id = s.tlas.order[n.first + k];
                 const Instance &in = s.inst[id];
-                if ((in.mask & mask & 0xffu) == 0u) continue;
+                // procedural primitives answer ray queries only (cpu/accel.rs:476-484: plain
+                // traces run without intersection functions)
+                if (!in.mesh || (in.mask & mask & 0xffu) == 0u) continue;
                 float lo[3], ld[3];
                 xform_point(in.inv, o, lo);
                 xform_vector(in.inv, d, ld);
@@ -677,14 +686,21 @@ uint64_t orc_mesh_node_count(const void *mesh) { return static_cast<const Mesh *
 
 void *orc_scene_create(void) { return new Scene; }
 
-void orc_scene_add(void *scene, const void *mesh, const float affine[12], uint32_t mask) {
+void orc_scene_add_ex(void *scene, const void *mesh, const void *procedural, const float affine[12], uint32_t mask,
+                      uint32_t opaque) {
     auto *s = static_cast<Scene *>(scene);
     Instance in;
     in.mesh = static_cast<const Mesh *>(mesh);
+    in.proc = static_cast<const Procedural *>(procedural);
     std::memcpy(in.fwd, affine, sizeof(in.fwd));
     invert_affine(affine, in.inv);
     in.mask = mask;
+    in.opaque = opaque != 0;
     s->inst.push_back(in);
+}
+
+void orc_scene_add(void *scene, const void *mesh, const float affine[12], uint32_t mask) {
+    orc_scene_add_ex(scene, mesh, nullptr, affine, mask, 1u);
 }
 
 void orc_scene_commit(void *scene) {
@@ -693,7 +709,7 @@ void orc_scene_commit(void *scene) {
     for (size_t i = 0; i < s->inst.size(); i++) {
         const Instance &in = s->inst[i];
         boxes[i].empty();
-        if (in.mesh->tri.empty()) continue;
+        if (!in.mesh || in.mesh->tri.empty()) continue; // procedural: not in the plain-trace TLAS
         const Box &b = in.mesh->bounds;
         for (int c = 0; c < 8; c++) {
             float p[3] = {(c & 1) ? b.hi[0] : b.lo[0], (c & 2) ? b.hi[1] : b.lo[1], (c & 4) ? b.hi[2] : b.lo[2]};
@@ -1068,3 +1084,123 @@ extern "C" void orc_pathtrace(const void *scene, const orc_pt_material *mats, co
         counts[1] += ns.load();
     }
 }
+
+// ------------------------------------------------------------------------------------------
+// Ray queries (AccelImpl::ray_query, src/rust/luisa_compute_backend_impl/src/cpu/accel.rs:582-800)
+// with the built-in candidate callbacks (src/tests/test_procedural.cpp:86-120), restated by
+// enumerating every candidate of every instance whose mask passes (no BVH): a surface candidate
+// is a watertight hit with t in [tmin, tmax], committed for opaque instances, otherwise per the
+// surface callback (cpu/accel.rs:378-393 disable the filter for opaque instances); a procedural
+// candidate commits the callback's distance if it lies in [tmin, tmax) (cpu/accel.rs:711-717).
+// The query's answer is the nearest committed candidate, ties to the smaller (instance, primitive).
+namespace {
+
+inline float q_len2(float a, float b) { return std::sqrt(a * a + b * b); }
+
+bool q_surface(const orc_query_policy &P, float u, float v) {
+    if (P.surface_mode == 0u) return true;  // commit
+    if (P.surface_mode != 2u) return false; // ignore
+    const float w = (1.0f - u) - v;          // cutout
+    return q_len2(w, u) < P.cutout && q_len2(u, v) < P.cutout && q_len2(v, w) < P.cutout;
+}
+
+// sphere inscribed in the AABB (centre = box centre, radius = half the smallest extent), met by
+// the object-space ray o + t d: L = c - o, tc = dot(L, d) / dot(d, d) > 0, squared distance of the
+// ray from the centre dot(L, L) - tc^2 dot(d, d) <= r^2, t = tc - sqrt((r^2 - that) / dot(d, d))
+bool q_procedural(const orc_query_policy &P, const float *box, const float o[3], const float d[3], float &t) {
+    if (P.procedural_mode != 1u) return false;
+    const float c[3] = {(box[0] + box[3]) * 0.5f, (box[1] + box[4]) * 0.5f, (box[2] + box[5]) * 0.5f};
+    const float r = 0.5f * std::fmin(std::fmin(box[3] - box[0], box[4] - box[1]), box[5] - box[2]);
+    const float L[3] = {c[0] - o[0], c[1] - o[1], c[2] - o[2]};
+    const float dd = (d[0] * d[0] + d[1] * d[1]) + d[2] * d[2];
+    const float ld = (L[0] * d[0] + L[1] * d[1]) + L[2] * d[2];
+    const float ll = (L[0] * L[0] + L[1] * L[1]) + L[2] * L[2];
+    const float tc = ld / dd;
+    if (!(tc > 0.0f)) return false;
+    const float q = ll - (tc * tc) * dd;
+    const float rr = r * r;
+    if (!(q <= rr)) return false;
+    t = tc - std::sqrt((rr - q) / dd);
+    return true;
+}
+
+// Evaluates candidate `prim` of instance `id`; true with (t, u, v) when it is committed.
+bool q_candidate(const Instance &in, const orc_query_policy &P, const float *ray, uint32_t prim, float &t, float &u,
+                 float &v) {
+    const float o[3] = {ray[0], ray[1], ray[2]}, d[3] = {ray[4], ray[5], ray[6]};
+    const float tmin = ray[3], tmax = ray[7];
+    float lo[3], ld[3];
+    xform_point(in.inv, o, lo);
+    xform_vector(in.inv, d, ld);
+    if (in.proc) {
+        if (!q_procedural(P, &in.proc->aabb[6 * (size_t)prim], lo, ld, t)) return false;
+        u = v = 0.0f;
+        return t >= tmin && t < tmax;
+    }
+    WtRay wr;
+    wt_setup(wr, lo, ld);
+    const uint32_t *ix = &in.mesh->tri[3 * (size_t)prim];
+    if (!wt_intersect(wr, &in.mesh->v[3 * (size_t)ix[0]], &in.mesh->v[3 * (size_t)ix[1]],
+                      &in.mesh->v[3 * (size_t)ix[2]], tmin, tmax, t, u, v))
+        return false;
+    return in.opaque || q_surface(P, u, v);
+}
+
+uint32_t q_prim_count(const Instance &in) {
+    return in.proc ? (uint32_t)(in.proc->aabb.size() / 6) : (uint32_t)(in.mesh ? in.mesh->tri.size() / 3 : 0);
+}
+
+} // namespace
+
+extern "C" {
+
+void *orc_procedural_create(const float *aabbs, uint64_t n) {
+    auto *p = new Procedural;
+    p->aabb.assign(aabbs, aabbs + 6 * n);
+    return p;
+}
+
+void orc_procedural_free(void *procedural) { delete static_cast<Procedural *>(procedural); }
+
+void orc_query(const void *scene, const float *rays, uint64_t nrays, uint32_t mask, const orc_query_policy *policy,
+               orc_committed_hit *out, int nthreads) {
+    const Scene &s = *static_cast<const Scene *>(scene);
+    const orc_query_policy P = *policy;
+    parallel_for(nrays, 16, nthreads, [&](uint64_t b, uint64_t e) {
+        for (uint64_t i = b; i < e; i++) {
+            const float *r = rays + 8 * i;
+            orc_committed_hit best{UINT32_MAX, UINT32_MAX, 0.0f, 0.0f, 0u, r[7]};
+            float t_best = std::numeric_limits<float>::infinity();
+            for (uint32_t id = 0; id < (uint32_t)s.inst.size(); id++) {
+                const Instance &in = s.inst[id];
+                if ((in.mask & mask & 0xffu) == 0u) continue;
+                const uint32_t np = q_prim_count(in);
+                for (uint32_t p = 0; p < np; p++) {
+                    float t, u, v;
+                    if (!q_candidate(in, P, r, p, t, u, v)) continue;
+                    const bool nearer = t != t_best ? t < t_best : (id != best.inst ? id < best.inst : p < best.prim);
+                    if (!nearer) continue;
+                    t_best = t;
+                    best = orc_committed_hit{id, p, u, v, in.proc ? 2u : 1u, t};
+                }
+            }
+            out[i] = best;
+        }
+    });
+}
+
+int orc_query_candidate(const void *scene, const float *ray, uint32_t mask, const orc_query_policy *policy,
+                        uint32_t inst, uint32_t prim, uint32_t hit_type, float *t, float *u, float *v) {
+    const Scene &s = *static_cast<const Scene *>(scene);
+    if (inst >= s.inst.size()) return 0;
+    const Instance &in = s.inst[inst];
+    if ((in.mask & mask & 0xffu) == 0u || prim >= q_prim_count(in)) return 0;
+    if (hit_type != (in.proc ? 2u : 1u)) return 0;
+    const unsigned int csr = _mm_getcsr();
+    set_ftz_daz();
+    const bool ok = q_candidate(in, *policy, ray, prim, *t, *u, *v);
+    _mm_setcsr(csr);
+    return ok ? 1 : 0;
+}
+
+} // extern "C"

@@ -89,6 +89,28 @@ void orc_trace_closest2(const void *scene, const float *rays, uint64_t nrays, ui
 void orc_trace_any(const void *scene, const float *rays, uint64_t nrays, uint32_t mask,
                    uint8_t *hit, int nthreads);
 
+/* ---- procedural primitives and ray queries (cpu/accel.rs:582-800) ----
+   A procedural primitive is a list of AABBs (6 floats each: min.xyz, max.xyz). Plain traces do
+   not see procedural instances; ray queries report their candidates to the procedural callback.
+   orc_scene_add(...) == orc_scene_add_ex(scene, mesh, NULL, affine, mask, 1). */
+void *orc_procedural_create(const float *aabbs, uint64_t n);
+void  orc_procedural_free(void *procedural);
+void  orc_scene_add_ex(void *scene, const void *mesh, const void *procedural, const float affine[12], uint32_t mask,
+                       uint32_t opaque);
+/* built-in callbacks: surface_mode 0 commit, 1 ignore, 2 cutout (|(w,u)|, |(u,v)|, |(v,w)| all
+   < cutout with w = 1 - u - v); procedural_mode 0 ignore, 1 the sphere inscribed in the AABB. */
+typedef struct orc_query_policy { uint32_t surface_mode; float cutout; uint32_t procedural_mode; uint32_t terminate_on_first; } orc_query_policy;
+typedef struct orc_committed_hit { uint32_t inst; uint32_t prim; float u; float v; uint32_t hit_type; float t; } orc_committed_hit;
+/* nearest committed candidate per ray (every candidate enumerated, no BVH); terminate_on_first
+   is ignored: any committed candidate is a valid answer of traverse_any, checked one at a time
+   with orc_query_candidate. Miss: inst = prim = UINT32_MAX, u = v = 0, hit_type 0, t = tmax. */
+void orc_query(const void *scene, const float *rays, uint64_t nrays, uint32_t mask, const orc_query_policy *policy,
+               orc_committed_hit *out, int nthreads);
+/* 1 with (t, u, v) if candidate (inst, prim) of kind hit_type (1 surface, 2 procedural) is
+   committed for `ray` (8 floats), else 0. */
+int orc_query_candidate(const void *scene, const float *ray, uint32_t mask, const orc_query_policy *policy,
+                        uint32_t inst, uint32_t prim, uint32_t hit_type, float *t, float *u, float *v);
+
 /* Independent check: double-precision Moller-Trumbore brute force over one mesh (no BVH).
    out per ray: prim (UINT32_MAX on miss), t, u, v (doubles) and second-nearest t2. */
 typedef struct orc_bf_hit { uint32_t prim; uint32_t pad; double t, u, v, t2; } orc_bf_hit;

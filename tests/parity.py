@@ -59,3 +59,59 @@ def compare_any(gpu_any, ref_any):
     if len(bad):
         raise AssertionError(f"any-hit parity failed on {len(bad)} of {len(ref_any)} rays (first {bad[:8]})")
     return dict(rays=len(ref_any), occluded=int(ref_any.sum()))
+
+
+def random_rigid(rng, lo=-50.0, hi=50.0, smin=0.5, smax=2.0):
+    # uniform rotation from a uniform unit quaternion (Shoemake 1992)
+    u1, u2, u3 = rng.random(3)
+    q = np.array([np.sqrt(1 - u1) * np.sin(2 * np.pi * u2), np.sqrt(1 - u1) * np.cos(2 * np.pi * u2),
+                  np.sqrt(u1) * np.sin(2 * np.pi * u3), np.sqrt(u1) * np.cos(2 * np.pi * u3)])
+    x, y, z, w = q
+    R = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                  [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                  [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+    s = rng.uniform(smin, smax)
+    m = np.eye(4)
+    m[:3, :3] = R * s
+    m[:3, 3] = rng.uniform(lo, hi, 3)
+    return m[:3, :4].astype(np.float32).reshape(12)
+
+
+def random_boxes(rng, n, lo, hi, smin, smax):
+    """(n, 6) float32 AABBs: centres uniform in [lo, hi]^3, per-axis extents uniform in [smin, smax]."""
+    c = rng.uniform(lo, hi, (n, 3))
+    e = rng.uniform(smin, smax, (n, 3)) * 0.5
+    return np.concatenate([c - e, c + e], axis=1).astype(np.float32)
+
+
+def compare_query(gpu, ref, scene=None, rays=None, **policy):
+    """gpu: COMMITTED_HIT_DTYPE array of the nearest-commit query; ref: the oracle's COMMITTED_DTYPE
+    answer. Hit kind, ids, t and barycentrics bit-exact; where the ids differ both answers must be
+    committed candidates (checked with the oracle when `scene` is given) at the same t within the
+    tie margin. Returns a report dict; raises on failure."""
+    g_t, r_t = gpu["committed_ray_t"], ref["t"]
+    same_id = (gpu["inst"] == ref["inst"]) & (gpu["prim"] == ref["prim"]) & (gpu["hit_type"] == ref["hit_type"])
+    bits = lambda a: np.ascontiguousarray(a, np.float32).view(np.uint32)
+    exact = same_id & (bits(g_t) == bits(r_t)) & (bits(gpu["bary"][:, 0]) == bits(ref["u"])) & \
+        (bits(gpu["bary"][:, 1]) == bits(ref["v"]))
+    bad = []
+    ties = 0
+    for i in np.flatnonzero(~exact):
+        if same_id[i]:
+            bad.append(i)  # same candidate, different numbers
+            continue
+        both_hit = gpu["hit_type"][i] != 0 and ref["hit_type"][i] != 0
+        tie = both_hit and abs(float(g_t[i]) - float(r_t[i])) <= TIE_REL * abs(float(r_t[i]))
+        if tie and scene is not None:
+            c = scene.query_candidate(rays[i], gpu["inst"][i], gpu["prim"][i], gpu["hit_type"][i], **policy)
+            tie = c is not None and np.float32(c[0]) == g_t[i]
+        if tie:
+            ties += 1
+        else:
+            bad.append(i)
+    if bad:
+        i = bad[0]
+        raise AssertionError(f"query parity failed on {len(bad)} of {len(gpu)} rays; first {i}: gpu {gpu[i]} "
+                             f"oracle {ref[i]}")
+    kinds = np.bincount(ref["hit_type"].astype(np.int64), minlength=3)
+    return dict(rays=len(gpu), miss=int(kinds[0]), surface=int(kinds[1]), procedural=int(kinds[2]), ties=ties)

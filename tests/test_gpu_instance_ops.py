@@ -8,8 +8,7 @@ import pytest
 import luisacompute_amd as lc
 from luisacompute_amd import host
 from oracle import binding as ob
-from parity import compare_any, compare_closest
-from test_gpu_instances import random_rigid
+from parity import compare_any, compare_closest, random_rigid
 
 pytestmark = pytest.mark.gpu
 

@@ -5,7 +5,7 @@ import pytest
 import luisacompute_amd as lc
 from luisacompute_amd import host
 from oracle import binding as ob
-from parity import compare_any, compare_closest
+from parity import compare_any, compare_closest, random_rigid
 
 pytestmark = pytest.mark.gpu
 
@@ -19,22 +19,6 @@ def translate(t):
     m = np.eye(4)
     m[:3, 3] = t
     return m
-
-
-def random_rigid(rng, lo=-50.0, hi=50.0, smin=0.5, smax=2.0):
-    # uniform rotation from a uniform unit quaternion (Shoemake 1992)
-    u1, u2, u3 = rng.random(3)
-    q = np.array([np.sqrt(1 - u1) * np.sin(2 * np.pi * u2), np.sqrt(1 - u1) * np.cos(2 * np.pi * u2),
-                  np.sqrt(u1) * np.sin(2 * np.pi * u3), np.sqrt(u1) * np.cos(2 * np.pi * u3)])
-    x, y, z, w = q
-    R = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
-                  [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
-                  [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
-    s = rng.uniform(smin, smax)
-    m = np.eye(4)
-    m[:3, :3] = R * s
-    m[:3, 3] = rng.uniform(lo, hi, 3)
-    return m[:3, :4].astype(np.float32).reshape(12)
 
 
 def test_rtx_scene_closed_form(device, stream):
