@@ -37,7 +37,7 @@ def parse():
     ap.add_argument("--cpu-sample-rays", type=int, default=1 << 18)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run 3 device-to-device copies of this many MiB (counter calibration) and exit")
-    ap.add_argument("--workload", default="soup", choices=["soup", "instanced", "pathtracer", "serving"],
+    ap.add_argument("--workload", default="soup", choices=["soup", "instanced", "pathtracer", "serving", "query"],
                     help="soup: the headline line; instanced: 64 ray batches over a 1,024-instance TLAS; "
                          "pathtracer: box room 1920x1080, 64 spp, depth 8 (wavefront vs megakernel); "
                          "serving: render requests over 4 resident scenes (served vs plain)")
@@ -316,6 +316,74 @@ def bench_instanced(a):
         "dtype": "f32", "data": "synthetic"}), flush=True)
 
 
+def bench_query(a):
+    """Ray queries (accel.traverse / traverse_any) on the reference's procedural test scene at scale:
+    a non-opaque triangle soup with the rounded-cutout surface callback plus procedural spheres
+    inscribed in AABBs, src/tests/test_procedural.cpp:29-120."""
+    import luisacompute_amd as lc
+    from luisacompute_amd import autotune, host
+    n_tris, n_boxes, n_rays = 1 << 20, 1 << 20, a.rays
+    verts, tris = lc.soup(n_tris, initseq=5)
+    rng = np.random.default_rng(7)
+    c = rng.uniform(0.0, 100.0, (n_boxes, 3))
+    boxes = np.concatenate([c - 0.2, c + 0.2], axis=1).astype(np.float32)
+    rays = lc.rays_box_sphere(n_rays, initseq=6)
+    dev = lc.Device(0)
+    autotune.load_default(dev)
+    s = dev.create_stream()
+    mesh = host.mesh_from_arrays(dev, s, verts, tris)
+    prim = host.procedural_from_aabbs(dev, s, boxes)
+    accel = dev.create_accel()
+    accel.emplace_back(mesh, lc.IDENTITY, opaque=False)
+    accel.emplace_back(prim, lc.IDENTITY)
+    s << accel.build()
+    rb = host.upload(dev, s, rays)
+    hb = dev.create_buffer(24, n_rays)
+    s.synchronize()
+    res = {}
+    for name, first in (("query", False), ("query_any", True)):
+        def run():
+            accel.query(s, rb, hb, n_rays, surface=lc.QUERY_SURFACE_CUTOUT, cutout=0.8,
+                        procedural=lc.QUERY_PROCEDURAL_SPHERE, terminate_on_first=first)
+        for _ in range(a.warmup):
+            run()
+        s.synchronize()
+        dev.reset_timing()
+        dev.enable_timing(True)
+        for _ in range(a.steps):
+            run()
+        s.synchronize()
+        dev.enable_timing(False)
+        ms, n = dev.timing(name)
+        res[name] = ms / max(n, 1)
+    out = np.zeros(n_rays, lc.COMMITTED_HIT_DTYPE)
+    s << hb.copy_to(out) << lc.synchronize()
+    kinds = np.bincount(out["hit_type"].astype(np.int64), minlength=3)
+    # the same rays as plain closest hits (procedural instance unseen, every triangle committed)
+    hc = dev.create_buffer(24, n_rays)
+    dev.reset_timing()
+    dev.enable_timing(True)
+    for _ in range(a.steps):
+        accel.trace_closest(s, rb, hc, n_rays)
+    s.synchronize()
+    dev.enable_timing(False)
+    tms, nt = dev.timing("trace_closest")
+    tms /= max(nt, 1)
+    print(json.dumps({
+        "metric": "Mrays/s ray queries (traverse, surface + procedural callbacks)",
+        "value": round(n_rays / (res["query"] * 1e-3) / 1e6, 2), "unit": "Mrays/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(res["query"], 3),
+        "query_any_ms": round(res["query_any"], 3),
+        "query_any_mrays_s": round(n_rays / (res["query_any"] * 1e-3) / 1e6, 2),
+        "trace_closest_same_rays_ms": round(tms, 3),
+        "trace_closest_same_rays_mrays_s": round(n_rays / (tms * 1e-3) / 1e6, 2),
+        "committed": {"miss": int(kinds[0]), "surface": int(kinds[1]), "procedural": int(kinds[2])},
+        "config": {"workload": f"non-opaque soup {n_tris:,} tris (U[0,100]^3, sigma 0.5) with the rounded cutout "
+                               f"(0.8) + {n_boxes:,} AABBs (half extent 0.2) with inscribed spheres; "
+                               f"{n_rays:,} rays origin U[0,100]^3 dir U(S^2)"},
+        "dtype": "f32", "data": "synthetic"}), flush=True)
+
+
 def copy_probe(mib):
     """Plain copy of known size: reading FETCH_SIZE / WRITE_SIZE for it calibrates the counters."""
     import luisacompute_amd as lc
@@ -376,6 +444,9 @@ def main():
         return
     if a.workload == "serving":
         bench_serving(a)
+        return
+    if a.workload == "query":
+        bench_query(a)
         return
     import luisacompute_amd as lc
     from luisacompute_amd import host

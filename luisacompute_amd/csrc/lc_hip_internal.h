@@ -232,7 +232,9 @@ enum { kQueryProceduralIgnore = 0, kQueryProceduralSphere = 1 };
 struct QueryParams {
     const float4 *rays; // 2 float4 per ray
     void *hits;         // 24-B CommittedHit per ray
-    uint64_t count;
+    const uint32_t *order; // optional: queue slot -> ray (sorted queue), else identity
+    uint32_t *counter;  // queue slot counter (zeroed by the launch)
+    uint32_t count;
     uint32_t mask;
     uint32_t first;     // terminate on the first commit (traverse_any)
     QueryGeometry geo;

@@ -60,21 +60,71 @@ struct BuiltinPolicy {
 
 constexpr int kQueryBlock = 128;
 constexpr int kQuerySlds = 8;
+constexpr int kQuerySteps = 16;     // walk steps between refills of finished lanes
+constexpr uint32_t kQueryChunk = 256; // queue slots a wave claims per atomic
 
+__global__ void k_query_zero(uint32_t *counter) {
+    if (threadIdx.x == 0) *counter = 0u;
+}
+
+// Persistent launch: a wave claims kQueryChunk queue slots per atomic and hands them to its idle
+// lanes; each lane runs kQuerySteps steps of its query between refills, so lanes whose query
+// ended take the next ray instead of idling until the wave's longest query ends. `order` (optional)
+// is the queue in traversal order (rays sorted by origin cell and direction octant).
 __global__ __launch_bounds__(kQueryBlock) void k_query(const QueryParams P) {
     __shared__ uint2 stk[kQuerySlds][kQueryBlock];
+    const uint32_t lane = threadIdx.x & 63u;
     const uint64_t glane = (uint64_t)blockIdx.x * kQueryBlock + threadIdx.x;
-    rq::LaneStack<kQueryBlock, kQuerySlds> S{stk, threadIdx.x, P.spill + glane * (uint64_t)P.spill_depth,
-                                               P.spill_depth, P.error, 0};
+    rq::QueryWalker<BuiltinPolicy, kQueryBlock, kQuerySlds> W;
+    W.S = rq::LaneStack<kQueryBlock, kQuerySlds>{stk, threadIdx.x, P.spill + glane * (uint64_t)P.spill_depth,
+                                                 P.spill_depth, P.error, 0};
     const BuiltinPolicy pol{P.surface_mode, P.cutout, P.procedural_mode};
-    const uint64_t stride = (uint64_t)gridDim.x * kQueryBlock;
-    for (uint64_t i = glane; i < P.count; i += stride) {
-        const float4 r0 = P.rays[2 * i], r1 = P.rays[2 * i + 1];
-        const rq::CommittedHit h = rq::ray_query(P.geo, S, r0, r1, P.mask, P.first != 0u, pol);
-        uint2 *o = reinterpret_cast<uint2 *>(static_cast<char *>(P.hits) + 24ull * i);
-        o[0] = make_uint2(h.inst, h.prim);
-        o[1] = make_uint2(__float_as_uint(h.u), __float_as_uint(h.v));
-        o[2] = make_uint2(h.type, __float_as_uint(h.t));
+    const uint32_t count = P.count;
+    uint32_t pool = 0, pool_end = 0; // wave-uniform: claimed, unassigned slots [pool, pool_end)
+    bool drained = false;            // wave-uniform: the queue is used up
+    bool active = false, exhausted = false;
+    uint32_t ray = 0;
+    for (;;) {
+        const uint64_t need = __ballot(!active && !exhausted);
+        if (need) {
+            const uint32_t n_need = (uint32_t)__popcll(need);
+            const uint32_t rank = (uint32_t)__popcll(need & ((1ull << lane) - 1ull));
+            const uint32_t avail = pool_end - pool;
+            uint32_t my = rank < avail ? pool + rank : 0xFFFFFFFFu;
+            if (avail >= n_need) {
+                pool += n_need;
+            } else if (drained) {
+                pool = pool_end;
+            } else {
+                const uint32_t leader = (uint32_t)__ffsll((long long)need) - 1u;
+                uint32_t b = 0;
+                if (lane == leader) b = atomicAdd(P.counter, kQueryChunk);
+                b = __builtin_amdgcn_readlane(b, (int)leader);
+                const uint32_t nb = min(b, count), ne = (uint32_t)min<uint64_t>((uint64_t)b + kQueryChunk, count);
+                drained = (uint64_t)b + kQueryChunk >= count;
+                if (rank >= avail && rank - avail < ne - nb) my = nb + (rank - avail);
+                pool = nb + min(n_need - avail, ne - nb);
+                pool_end = ne;
+            }
+            if (!active && !exhausted) {
+                if (my < count) {
+                    ray = P.order ? P.order[my] : my;
+                    W.start(P.geo, P.rays[2 * (uint64_t)ray], P.rays[2 * (uint64_t)ray + 1], P.mask, P.first != 0u);
+                    active = true;
+                } else if (drained) {
+                    exhausted = true;
+                }
+            }
+        }
+        if (!__any(active)) break;
+        if (active && W.run(P.geo, pol, kQuerySteps)) {
+            const rq::CommittedHit &h = W.hit;
+            uint2 *o = reinterpret_cast<uint2 *>(static_cast<char *>(P.hits) + 24ull * ray);
+            o[0] = make_uint2(h.inst, h.prim);
+            o[1] = make_uint2(__float_as_uint(h.u), __float_as_uint(h.v));
+            o[2] = make_uint2(h.type, __float_as_uint(h.t));
+            active = false;
+        }
     }
 }
 
@@ -91,6 +141,7 @@ int query_blocks_per_cu() {
 
 hipError_t query_launch(const QueryParams &P, hipStream_t stream, int grid) {
     if (P.count == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_query_zero, dim3(1), dim3(64), 0, stream, P.counter);
     hipLaunchKernelGGL(k_query, dim3(grid), dim3(kQueryBlock), 0, stream, P);
     return hipGetLastError();
 }

@@ -1242,6 +1242,7 @@ int lc_hip_accel_query(uint64_t device, uint64_t stream, uint64_t accel, uint64_
         P.geo.instances = a->records;
         stack_need = a->tlas_stats.depth + 2 + a->max_blas_depth + 2;
     }
+    if (ray_count > 0xF0000000u) return fail(LC_HIP_ERROR_UNSUPPORTED, "more than 0xF0000000 rays in one query");
     P.rays = reinterpret_cast<const float4 *>(static_cast<char *>(rb->ptr) + ray_offset);
     P.hits = static_cast<char *>(hb->ptr) + hit_offset;
     P.count = ray_count;
@@ -1260,7 +1261,21 @@ int lc_hip_accel_query(uint64_t device, uint64_t stream, uint64_t accel, uint64_
     LC_HIP_TRY(ensure(dev->spill, dev->spill_bytes, (size_t)grid * block * P.spill_depth * sizeof(uint2)));
     P.spill = static_cast<uint2 *>(dev->spill);
     P.error = dev->error;
+    P.counter = dev->ray_counter;
+    // the query queue is sorted like trace queues when ray sorting is on (per-ray answers of the
+    // nearest-commit query do not depend on the order)
+    const bool sort = dev->cfg[P.first ? 1 : 0].sort_rays && ray_count > 1 && !P.geo.empty;
+    if (sort) LC_HIP_TRY(ensure(dev->sort_scratch, dev->sort_bytes, ray_sort_scratch_bytes(ray_count)));
     TimedRegion tr(dev, s, P.first ? "query_any" : "query");
+    if (sort) {
+        BatchTable bt;
+        std::memset(&bt, 0, sizeof(bt));
+        bt.n = 1;
+        bt.b[0].rays = P.rays;
+        bt.b[0].mask = vis_mask;
+        hipError_t e = ray_sort(s, bt, ray_count, a->world_bounds, dev->sort_scratch, dev->sort_bytes, &P.order);
+        if (e != hipSuccess) return hip_fail(e, "query ray sort");
+    }
     hipError_t e = query_launch(P, s, grid);
     if (e != hipSuccess) return hip_fail(e, "query launch");
     return LC_HIP_OK;
