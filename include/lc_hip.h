@@ -263,6 +263,36 @@ int lc_hip_accel_query(uint64_t device, uint64_t stream, uint64_t accel, uint64_
                        uint64_t hit_buffer, size_t hit_offset, uint32_t ray_count, uint32_t vis_mask,
                        const lc_hip_query_policy *policy);
 
+/* ---- run-time shaders: create_shader / destroy_shader (device_interface.h:151-160) and
+   ShaderDispatchCommand (command.h:99-160) for kernels given as HIP source -- the form a DSL
+   codegen emits for this backend (the CUDA backend's cuda_codegen_ast.cpp:1291-1490 + NVRTC) --
+   compiled by hiprtc for gfx950 against the device library lc_hip_device.h (trace_closest /
+   trace_any, ray queries with the shader's own candidate callbacks, instance ops, make_ray,
+   offset_ray_origin, triangle_interpolate). Arguments are encoded like the CUDA backend's
+   (cuda_shader_native.cpp:73-184): in order, each aligned to 16 bytes -- buffer {pointer at the
+   offset, size in bytes}, accel {descriptor pointer, 0}, uniform its bytes -- then uint4
+   {dispatch x, y, z, 0}; the grid covers the dispatch size with blocks of the shader's block
+   size. Compile errors return LC_HIP_INVALID_HANDLE / LC_HIP_ERROR_INVALID_ARGUMENT with the
+   compiler's log in `log` (truncated to log_capacity). ---- */
+#define LC_HIP_ARGUMENT_BUFFER 0u  /* Argument::Tag (include/luisa/runtime/rhi/argument.h:9-15) */
+#define LC_HIP_ARGUMENT_UNIFORM 2u
+#define LC_HIP_ARGUMENT_ACCEL 4u
+typedef struct lc_hip_shader_argument {
+    uint32_t tag;
+    uint32_t _pad;
+    uint64_t handle; /* buffer or accel */
+    size_t offset;   /* buffer: byte offset; uniform: offset into the uniform bytes */
+    size_t size;     /* buffer: bytes bound (0: to the end of the buffer); uniform: byte count */
+} lc_hip_shader_argument;
+/* Compile only (no device needed): 0 on success with the code object size in *code_size. */
+int lc_hip_shader_compile(const char *source, char *log, size_t log_capacity, size_t *code_size);
+uint64_t lc_hip_shader_create(uint64_t device, const char *source, const char *entry, uint32_t block_x,
+                              uint32_t block_y, uint32_t block_z, char *log, size_t log_capacity);
+void lc_hip_shader_destroy(uint64_t device, uint64_t shader);
+int lc_hip_shader_dispatch(uint64_t device, uint64_t stream, uint64_t shader, const lc_hip_shader_argument *arguments,
+                           size_t argument_count, const void *uniforms, size_t uniform_size, uint32_t dispatch_x,
+                           uint32_t dispatch_y, uint32_t dispatch_z);
+
 /* ---- path tracing: the DSL program of src/tests/test_path_tracing.cpp:165-258 (camera ray,
    closest hit, emission with the balance heuristic, one area-light sample with a shadow ray
    through offset_ray_origin, cosine-weighted bounce, Russian roulette), run as a built-in

@@ -67,6 +67,13 @@ class QueryPolicy(ctypes.Structure):
                 ("terminate_on_first", ctypes.c_uint32)]
 
 
+class ShaderArgument(ctypes.Structure):
+    """lc_hip_shader_argument (Argument, include/luisa/runtime/rhi/argument.h:7-48)."""
+    _fields_ = [("tag", ctypes.c_uint32), ("_pad", ctypes.c_uint32), ("handle", ctypes.c_uint64),
+                ("offset", ctypes.c_size_t), ("size", ctypes.c_size_t)]
+    BUFFER, UNIFORM, ACCEL = 0, 2, 4
+
+
 class Modification(ctypes.Structure):
     """AccelBuildCommand::Modification (include/luisa/runtime/rhi/command.h:534-551)."""
     _fields_ = [("index", ctypes.c_uint32), ("user_id", ctypes.c_uint32), ("flags", ctypes.c_uint32),
@@ -184,6 +191,11 @@ _SIGNATURES = {
     "lc_hip_procedural_build": (ctypes.c_int, [_U64, _U64, _U64, _U32, _U64, _SZ, _SZ]),
     "lc_hip_accel_query": (ctypes.c_int, [_U64, _U64, _U64, _U64, _SZ, _U64, _SZ, _U32, _U32,
                                           ctypes.POINTER(QueryPolicy)]),
+    "lc_hip_shader_compile": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, _SZ, ctypes.POINTER(_SZ)]),
+    "lc_hip_shader_create": (_U64, [_U64, ctypes.c_char_p, ctypes.c_char_p, _U32, _U32, _U32, ctypes.c_char_p, _SZ]),
+    "lc_hip_shader_destroy": (None, [_U64, _U64]),
+    "lc_hip_shader_dispatch": (ctypes.c_int, [_U64, _U64, _U64, ctypes.POINTER(ShaderArgument), _SZ, _P, _SZ, _U32,
+                                              _U32, _U32]),
     "lc_hip_accel_create": (_U64, [_U64, ctypes.POINTER(AccelOption)]),
     "lc_hip_accel_destroy": (None, [_U64, _U64]),
     "lc_hip_accel_build": (ctypes.c_int, [_U64, _U64, _U64, _U32, _U32, ctypes.POINTER(Modification), _SZ, ctypes.c_int]),
@@ -396,6 +408,10 @@ class Device:
     def create_accel(self, option=None):
         return Accel(self, option)
 
+    def create_shader(self, source, entry, block=(128, 1, 1)):
+        """Compiles HIP source against the device library (lc_hip_device.h) with hiprtc."""
+        return Shader(self, source, entry, block)
+
     def create_event(self):
         return Event(self)
 
@@ -562,6 +578,70 @@ class Mesh:
         e = (ctypes.c_uint32 * 7)()
         _check(lib().lc_hip_mesh_validate(self.device.handle, stream.handle, self.handle, e), "mesh validate")
         return dict(zip(self.VALIDATION_CHECKS, e[:]))
+
+
+def compile_shader_source(source):
+    """Compile-only check of a shader source (no device needed); returns the code object size."""
+    log = ctypes.create_string_buffer(1 << 16)
+    size = ctypes.c_size_t()
+    rc = lib().lc_hip_shader_compile(source.encode(), log, len(log), ctypes.byref(size))
+    if rc != 0:
+        raise LCError(f"shader compile failed:\n{log.value.decode(errors='replace')}")
+    return size.value
+
+
+class _Dispatch(Command):
+    def __init__(self, shader, args, size):
+        self.shader, self.args, self.size = shader, args, size
+
+    def execute(self, stream):
+        argv = (ShaderArgument * max(1, len(self.args)))()
+        blob = bytearray()
+        for i, a in enumerate(self.args):
+            if isinstance(a, tuple) and isinstance(a[0], Buffer):  # (buffer, offset[, size])
+                argv[i] = ShaderArgument(ShaderArgument.BUFFER, 0, a[0].handle, a[1], a[2] if len(a) > 2 else 0)
+            elif isinstance(a, Buffer):
+                argv[i] = ShaderArgument(ShaderArgument.BUFFER, 0, a.handle, 0, 0)
+            elif isinstance(a, Accel):
+                argv[i] = ShaderArgument(ShaderArgument.ACCEL, 0, a.handle, 0, 0)
+            else:  # uniform: numpy scalar / array or bytes
+                b = a if isinstance(a, (bytes, bytearray)) else np.ascontiguousarray(a).tobytes()
+                argv[i] = ShaderArgument(ShaderArgument.UNIFORM, 0, 0, len(blob), len(b))
+                blob += b
+        ub = (ctypes.c_ubyte * max(1, len(blob))).from_buffer_copy(bytes(blob) or b"\0")
+        x, y, z = (tuple(self.size) + (1, 1))[:3]
+        _check(lib().lc_hip_shader_dispatch(stream.device.handle, stream.handle, self.shader.handle, argv,
+                                            len(self.args), ctypes.cast(ub, ctypes.c_void_p), len(blob), x, y, z),
+               "shader dispatch")
+
+
+class Shader:
+    """A kernel compiled at run time (create_shader, device_interface.h:151-160): HIP source with an
+    `extern "C" __global__` entry taking one Params struct (arguments in order, 16-B aligned, then
+    `uint4 ls_kid`), compiled against the device library lc_hip_device.h."""
+
+    def __init__(self, device, source, entry, block=(128, 1, 1)):
+        self.device = device
+        log = ctypes.create_string_buffer(1 << 16)
+        bx, by, bz = (tuple(block) + (1, 1))[:3]
+        h = lib().lc_hip_shader_create(device.handle, source.encode(), entry.encode(), bx, by, bz, log, len(log))
+        if h == INVALID:
+            raise LCError(f"shader create: {lib().lc_hip_last_error().decode(errors='replace')[:4000]}")
+        self.handle = h
+
+    def __del__(self):
+        if getattr(self, "handle", None) is not None and _lib is not None and self.device.handle is not None:
+            lib().lc_hip_shader_destroy(self.device.handle, self.handle)
+            self.handle = None
+
+    def __call__(self, *args):
+        """shader(args...).dispatch(nx[, ny, nz]) -> stream command (ShaderDispatchCommand)."""
+        shader = self
+
+        class _Bound:
+            def dispatch(self, *size):
+                return _Dispatch(shader, args, size)
+        return _Bound()
 
 
 class ProceduralPrimitive:

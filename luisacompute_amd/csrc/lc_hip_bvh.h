@@ -12,9 +12,16 @@
 // HPG 2017). Slot s of a node holds the child lying toward octant s (bit k of s set = positive
 // along axis k); a ray visits slots in order i ^ oct(ray), which approximates front to back.
 #pragma once
+// Included by the library's kernels and, through lc_hip_device.h, by shaders compiled at run time
+// with hiprtc: plain layouts only.
 
+#ifndef __HIPCC_RTC__
 #include <stddef.h>
+#endif
 #include <stdint.h>
+#ifndef offsetof
+#define offsetof(t, m) __builtin_offsetof(t, m)
+#endif
 
 namespace lchip {
 
@@ -86,5 +93,51 @@ static_assert(offsetof(BuildCounters, level) % 8 == 0, "level words must be 8-B 
 inline uint32_t *level_nodes_word(BuildCounters *c, uint32_t l) {
     return reinterpret_cast<uint32_t *>(&c->level[l]) + 1;
 }
+
+// An accel's user-visible instance fields on the device (the table the in-kernel instance ops
+// read and write). dirty: bit f set when field f was written since the last build.
+enum { kInstanceTransform = 0, kInstanceUserId = 1, kInstanceVisibility = 2 };
+struct InstanceState {
+    float affine[12];     // object -> world, row-major 3x4
+    uint32_t user_id;
+    uint32_t vis_mask;
+    uint32_t dirty;
+    uint32_t _pad;
+};
+static_assert(sizeof(InstanceState) == 64, "InstanceState is 64 bytes");
+
+// What a ray is traced against: one instance's BLAS (single-instance form) or TLAS + BLASes.
+struct Geometry {
+    // single-instance form
+    const WideNode *blas_nodes;
+    const TriRecord *blas_tris;
+    float inst_inv[12];
+    uint32_t inst_mask;
+    // two-level form
+    const WideNode *tlas_nodes;
+    const uint32_t *tlas_prims;
+    const InstanceRecord *instances;
+    uint32_t empty;       // nothing to traverse: every ray misses
+};
+
+// The view of an accel a ray query walks (also the device library's traces).
+struct QueryGeometry {
+    const WideNode *tlas_nodes;      // two-level form
+    const uint32_t *tlas_prims;
+    const InstanceRecord *instances; // instance 0 alone when `two` is 0
+    uint32_t two;
+    uint32_t empty;                  // nothing to traverse: every query misses
+};
+
+// Device-side descriptor of a built accel: what a run-time shader's accel argument points at
+// (lc_hip_device.h), rewritten by every accel build.
+struct DeviceAccel {
+    QueryGeometry geo;
+    InstanceState *state; // instance table of the instance ops (first word of state[count]: error word)
+    uint32_t state_count;
+    uint32_t stack_need;  // stack entries a walk can need (the host checks it against the shader stack)
+    uint32_t *error;      // device error word: bit 0 stack overflow
+    uint32_t _pad[2];
+};
 
 } // namespace lchip

@@ -1,0 +1,165 @@
+// lc_hip_device.h -- device library of shaders compiled at run time (hiprtc, gfx950): what the
+// kernels a LuisaCompute codegen emits call for the ray-tracing CallOps, and the layouts of their
+// arguments (the CUDA backend's counterparts: cuda_device_resource.h:1279-1510,1776-2241).
+//
+// A shader is HIP source with one entry `extern "C" __global__ void <name>(const Params p)` whose
+// Params struct lists the dispatch arguments in order, each aligned to 16 bytes, followed by
+// `alignas(16) uint4 ls_kid` = {dispatch size x, y, z, 0} (the argument encoding of
+// src/backends/cuda/cuda_shader_native.cpp:73-184). Buffers arrive as lchip::dev::Buffer<T>
+// {pointer at the bound offset, bound size in bytes}, accels as lchip::dev::Accel, uniforms as
+// their bytes.
+//
+// Semantics: trace_closest / trace_any are AccelImpl::trace_closest / trace_any
+// (src/rust/luisa_compute_backend_impl/src/cpu/accel.rs:449-535); query() is accel.traverse /
+// traverse_any with the shader's own candidate callbacks (ray_query.h); the instance ops read and
+// write the accel's device instance table (reads see the latest writes, writes reach traversal
+// at the accel's next build, cpu/accel.rs:537-579). The walks keep their stack in private memory
+// (kStack entries; the host checks the accel's bound at dispatch).
+#pragma once
+
+#include "lc_hip_bvh.h"
+#include "lc_hip_rtx.h"
+#include "ray_query.h"
+#include "traverse.h"
+
+namespace lchip {
+
+namespace dev {
+
+constexpr int kStack = 64;
+
+struct Ray { // lc_hip_ray / Ray (include/luisa/runtime/rtx/ray.h:10-16)
+    float origin[3];
+    float t_min;
+    float direction[3];
+    float t_max;
+};
+
+struct SurfaceHit { // SurfaceHit (hit.h:30-37); miss: inst = prim = ~0u, bary (0, 0), t = t_max
+    uint32_t inst, prim;
+    float bary[2];
+    float committed_ray_t;
+    uint32_t _pad;
+};
+
+template<class T>
+struct alignas(16) Buffer {
+    T *ptr;
+    uint64_t size_bytes;
+    __device__ __forceinline__ T &operator[](uint64_t i) const { return ptr[i]; }
+    __device__ __forceinline__ uint64_t size() const { return size_bytes / sizeof(T); }
+};
+
+struct alignas(16) Accel {
+    const DeviceAccel *desc;
+    uint64_t _pad;
+};
+
+// dispatch_id / dispatch_size of a 1-3D dispatch (ls_kid.xyz = dispatch size)
+__device__ __forceinline__ uint3 dispatch_id() {
+    return make_uint3(blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y * blockDim.y + threadIdx.y,
+                      blockIdx.z * blockDim.z + threadIdx.z);
+}
+__device__ __forceinline__ bool in_dispatch(uint4 ls_kid) {
+    const uint3 i = dispatch_id();
+    return i.x < ls_kid.x && i.y < ls_kid.y && i.z < ls_kid.z;
+}
+
+// make_ray (src/dsl/rtx/ray.cpp:7-14): t_max defaults to FLT_MAX
+__device__ __forceinline__ Ray make_ray(rtx::V3 o, rtx::V3 d, float t_min = 0.0f, float t_max = FLT_MAX) {
+    return Ray{{o.x, o.y, o.z}, t_min, {d.x, d.y, d.z}, t_max};
+}
+
+template<class S, class P>
+struct CallbackPolicy {
+    S on_surface;
+    P on_procedural;
+    __device__ __forceinline__ bool surface(uint32_t inst, uint32_t prim, float u, float v, float t) const {
+        return on_surface(inst, prim, u, v, t);
+    }
+    __device__ __forceinline__ bool procedural(uint32_t inst, uint32_t prim, const float *o, const float *d,
+                                               const float *wo, const float *wd, float4 lo, float4 hi, float tmax,
+                                               float &t) const {
+        return on_procedural(inst, prim, o, d, wo, wd, lo, hi, tmax, t);
+    }
+};
+
+struct PlainTrace { // every triangle commits, procedural primitives never do
+    __device__ __forceinline__ bool surface(uint32_t, uint32_t, float, float, float) const { return true; }
+    __device__ __forceinline__ bool procedural(uint32_t, uint32_t, const float *, const float *, const float *,
+                                               const float *, float4, float4, float, float &) const {
+        return false;
+    }
+};
+
+template<class Policy>
+__device__ __forceinline__ rq::CommittedHit walk(const Accel a, const Ray &r, uint32_t mask, bool first,
+                                                 const Policy &pol) {
+    uint2 local[kStack];
+    rq::QueryWalker<Policy, 1, 0> W;
+    W.S = rq::LaneStack<1, 0>{nullptr, 0u, local, (uint32_t)kStack, a.desc->error, 0};
+    const float4 r0 = make_float4(r.origin[0], r.origin[1], r.origin[2], r.t_min);
+    const float4 r1 = make_float4(r.direction[0], r.direction[1], r.direction[2], r.t_max);
+    const QueryGeometry Q = a.desc->geo;
+    W.start(Q, r0, r1, mask, first);
+    while (!W.run(Q, pol, 1 << 30)) {
+    }
+    return W.hit;
+}
+
+// CallOp::RAY_TRACING_TRACE_CLOSEST
+__device__ __forceinline__ SurfaceHit trace_closest(const Accel a, const Ray &r, uint32_t mask = 0xFFu) {
+    const rq::CommittedHit h = walk(a, r, mask, false, PlainTrace{});
+    return SurfaceHit{h.inst, h.prim, {h.u, h.v}, h.t, 0u};
+}
+
+// CallOp::RAY_TRACING_TRACE_ANY
+__device__ __forceinline__ bool trace_any(const Accel a, const Ray &r, uint32_t mask = 0xFFu) {
+    return walk(a, r, mask, true, PlainTrace{}).type != rq::kHitMiss;
+}
+
+// accel.traverse (any = false) / traverse_any (any = true) with candidate callbacks
+//   on_surface(inst, prim, u, v, t) -> commit?
+//   on_procedural(inst, prim, obj_o, obj_d, world_o, world_d, aabb_lo, aabb_hi, t_max, &t) -> commit t?
+template<class S, class P>
+__device__ __forceinline__ rq::CommittedHit query(const Accel a, const Ray &r, uint32_t mask, bool any, S on_surface,
+                                                  P on_procedural) {
+    return walk(a, r, mask, any, CallbackPolicy<S, P>{on_surface, on_procedural});
+}
+
+// Instance ops (RAY_TRACING_INSTANCE_TRANSFORM / _USER_ID / _VISIBILITY_MASK and SET_ forms,
+// include/luisa/ast/op.h:290-297). Out-of-range indices read zeros and are not written; they
+// set the table's error word, which fails the accel's next build.
+__device__ __forceinline__ bool instance_ok(const Accel a, uint32_t i) {
+    if (i < a.desc->state_count) return true;
+    atomicOr(reinterpret_cast<uint32_t *>(a.desc->state + a.desc->state_count), 1u);
+    return false;
+}
+__device__ __forceinline__ void instance_transform(const Accel a, uint32_t i, float out[12]) {
+    const bool ok = instance_ok(a, i);
+    for (int c = 0; c < 12; c++) out[c] = ok ? a.desc->state[i].affine[c] : 0.0f;
+}
+__device__ __forceinline__ uint32_t instance_user_id(const Accel a, uint32_t i) {
+    return instance_ok(a, i) ? a.desc->state[i].user_id : 0u;
+}
+__device__ __forceinline__ uint32_t instance_visibility_mask(const Accel a, uint32_t i) {
+    return instance_ok(a, i) ? a.desc->state[i].vis_mask : 0u;
+}
+__device__ __forceinline__ void set_instance_transform(const Accel a, uint32_t i, const float m[12]) {
+    if (!instance_ok(a, i)) return;
+    for (int c = 0; c < 12; c++) a.desc->state[i].affine[c] = m[c];
+    atomicOr(&a.desc->state[i].dirty, 1u << kInstanceTransform);
+}
+__device__ __forceinline__ void set_instance_user_id(const Accel a, uint32_t i, uint32_t id) {
+    if (!instance_ok(a, i)) return;
+    a.desc->state[i].user_id = id;
+    atomicOr(&a.desc->state[i].dirty, 1u << kInstanceUserId);
+}
+__device__ __forceinline__ void set_instance_visibility_mask(const Accel a, uint32_t i, uint32_t m) {
+    if (!instance_ok(a, i)) return;
+    a.desc->state[i].vis_mask = m;
+    atomicOr(&a.desc->state[i].dirty, 1u << kInstanceVisibility);
+}
+
+} // namespace dev
+} // namespace lchip

@@ -36,17 +36,6 @@ hipError_t bvh_validate(hipStream_t stream, const WideNode *nodes, uint32_t node
                         uint32_t tri_count, uint32_t nprims, uint32_t *marks, uint32_t *err7);
 
 // ---- kernels (instances.hip) ----
-// An accel's user-visible instance fields on the device (the table the in-kernel instance ops
-// read and write). dirty: bit f set when field f was written since the last build.
-enum { kInstanceTransform = 0, kInstanceUserId = 1, kInstanceVisibility = 2 };
-struct InstanceState {
-    float affine[12];     // object -> world, row-major 3x4
-    uint32_t user_id;
-    uint32_t vis_mask;
-    uint32_t dirty;
-    uint32_t _pad;
-};
-static_assert(sizeof(InstanceState) == 64, "InstanceState is 64 bytes");
 // Item i reads (write = false: into data) or writes (from data) field `field` of instance idx[i];
 // 12 floats per item for transforms, one uint32 otherwise. Indices >= count set *err.
 hipError_t instance_ops(hipStream_t stream, InstanceState *st, uint32_t count, uint32_t field, bool write,
@@ -72,20 +61,6 @@ struct BatchTable {
 constexpr uint32_t kFetchSegments = 8;
 constexpr uint32_t kCounterStride = 16;
 constexpr uint32_t kRayCounterWords = kFetchSegments * kCounterStride;
-
-// What a ray is traced against: one instance's BLAS (single-instance form) or TLAS + BLASes.
-struct Geometry {
-    // single-instance form
-    const WideNode *blas_nodes;
-    const TriRecord *blas_tris;
-    float inst_inv[12];
-    uint32_t inst_mask;
-    // two-level form
-    const WideNode *tlas_nodes;
-    const uint32_t *tlas_prims;
-    const InstanceRecord *instances;
-    uint32_t empty;       // nothing to traverse: every ray misses
-};
 
 struct TraceParams {
     BatchTable batches;
@@ -219,13 +194,6 @@ int serve_drain(Server *s, double timeout_s);               // 0: every request 
 void serve_stats(Server *s, ServeStats *st);
 
 // ---- ray queries (query.hip, ray_query.h) ----
-struct QueryGeometry {
-    const WideNode *tlas_nodes;      // two-level form
-    const uint32_t *tlas_prims;
-    const InstanceRecord *instances; // instance 0 alone when `two` is 0
-    uint32_t two;
-    uint32_t empty;                  // nothing to traverse: every query misses
-};
 // built-in candidate callbacks of the batched query (values of lc_hip_query_policy)
 enum { kQuerySurfaceCommit = 0, kQuerySurfaceIgnore = 1, kQuerySurfaceCutout = 2 };
 enum { kQueryProceduralIgnore = 0, kQueryProceduralSphere = 1 };
@@ -397,6 +365,27 @@ struct Accel {
     size_t state_cap = 0;
     uint32_t state_count = 0;
     bool state_written = false;
+    // descriptor a run-time shader's accel argument points at (lc_hip_device.h DeviceAccel),
+    // rewritten by every build
+    void *desc = nullptr;
+    DeviceAccel desc_host{}; // staging of the last descriptor upload
 };
+
+// ---- run-time shaders (hiprtc; shader.cpp) ----
+struct Shader {
+    Device *dev = nullptr;
+    hipModule_t module = nullptr;
+    hipFunction_t fn = nullptr;
+    uint32_t block[3] = {128, 1, 1};
+};
+// The device library headers (lc_hip_device.h and what it includes), embedded at build time.
+extern const char *const kDeviceHeaderNames[];
+extern const char *const kDeviceHeaderSources[];
+extern const int kDeviceHeaderCount;
+// Compiles `source` for gfx950 with the device library; code object into `code`, diagnostics
+// into `log`. False on a compile error.
+bool rtc_compile(const char *source, const char *name, std::vector<char> &code, std::string &log);
+// Writes the accel's DeviceAccel descriptor (allocating it on first use) on `stream`.
+int accel_write_descriptor(Accel *a, hipStream_t stream);
 
 } // namespace lchip

@@ -42,10 +42,13 @@
 
 #include "lc_hip_bvh.h"
 #include "lc_hip_internal.h"
+#include "lc_hip_rtx.h"
 #include "traverse.h"
 
 namespace lchip {
 namespace pt {
+
+using namespace rtx;
 
 constexpr uint32_t kDrawsPerSample = 64; // RNG draws reserved per sample: 2 + 5 per bounce
 constexpr float kInvPi = 0.318309886183790672f;
@@ -85,21 +88,6 @@ __device__ __forceinline__ void pcg_advance(Pcg &r, uint64_t delta) {
 }
 __device__ __forceinline__ float pcg_float(Pcg &r) { return (float)(pcg_next(r) >> 8) * 0x1p-24f; }
 
-// ---- float3 arithmetic, every operation spelled out (no contraction, fixed order) ----
-struct V3 {
-    float x, y, z;
-};
-__device__ __forceinline__ V3 v3(float x, float y, float z) { return V3{x, y, z}; }
-__device__ __forceinline__ V3 sub(V3 a, V3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
-__device__ __forceinline__ float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-__device__ __forceinline__ V3 cross(V3 a, V3 b) {
-    return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
-}
-__device__ __forceinline__ V3 normalize(V3 a) {
-    const float l = sqrtf(dot(a, a));
-    return v3(a.x / l, a.y / l, a.z / l);
-}
-
 // sin(2 pi u), cos(2 pi u) for u in [0, 1): quadrant from 4u (exact), then Taylor polynomials of
 // degree 11 / 12 on [0, pi/2) (truncation error below 6e-8).
 __device__ __forceinline__ void sincos_2pi(float u, float &s, float &c) {
@@ -118,16 +106,6 @@ __device__ __forceinline__ void sincos_2pi(float u, float &s, float &c) {
     case 2: s = -ps; c = -pc; break;
     default: s = -pc; c = ps; break;
     }
-}
-
-// offset_ray_origin (src/dsl/rtx/ray.cpp:16-24): integer ulp offset of 256 n, or 2^-16 n near 0.
-__device__ __forceinline__ float offset1(float p, float n) {
-    const int of = (int)(256.0f * n);
-    const float pi = __int_as_float(__float_as_int(p) + (p < 0.0f ? -of : of));
-    return fabsf(p) < (1.0f / 32.0f) ? p + (1.0f / 65536.0f) * n : pi;
-}
-__device__ __forceinline__ V3 offset_ray_origin(V3 p, V3 n) {
-    return v3(offset1(p.x, n.x), offset1(p.y, n.y), offset1(p.z, n.z));
 }
 
 __device__ __forceinline__ V3 to_world(const float *m, float4 v) {
@@ -172,10 +150,7 @@ __device__ __forceinline__ void shade(const PtScene &S, uint32_t depth, V3 ro, V
     const float4 *tr = reinterpret_cast<const float4 *>(I.tris + rec);
     const float4 a = tr[0], b = tr[1], c = tr[2];
     const V3 p0 = to_world(I.fwd, a), p1 = to_world(I.fwd, b), p2 = to_world(I.fwd, c);
-    // triangle_interpolate: (1 - u - v) a + u b + v c
-    const float w0 = 1.0f - bu - bv;
-    const V3 p = v3(w0 * p0.x + bu * p1.x + bv * p2.x, w0 * p0.y + bu * p1.y + bv * p2.y,
-                    w0 * p0.z + bu * p1.z + bv * p2.z);
+    const V3 p = triangle_interpolate(bu, bv, p0, p1, p2);
     const V3 n = normalize(cross(sub(p1, p0), sub(p2, p0)));
     const float cos_wo = -rd.x * n.x + -rd.y * n.y + -rd.z * n.z;
     if (cos_wo < 1e-4f) return;

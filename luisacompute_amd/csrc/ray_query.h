@@ -27,13 +27,7 @@
 // so geometry inside the AABB does not depend on how conservatively boxes are culled.
 #pragma once
 
-#include <hip/hip_runtime.h>
-
-#include <cfloat>
-#include <cstdint>
-
 #include "lc_hip_bvh.h"
-#include "lc_hip_internal.h"
 #include "traverse.h"
 
 namespace lchip {
@@ -60,14 +54,22 @@ struct LaneStack {
     uint32_t *error;
     int sp;
     __device__ __forceinline__ void push(uint2 e) {
-        if (sp < SLDS) lds[sp][tid] = e;
-        else if ((uint32_t)(sp - SLDS) < depth) spill[sp - SLDS] = e;
+        if constexpr (SLDS > 0) {
+            if (sp < SLDS) {
+                lds[sp][tid] = e;
+                sp++;
+                return;
+            }
+        }
+        if ((uint32_t)(sp - SLDS) < depth) spill[sp - SLDS] = e;
         else atomicOr(error, 1u);
         sp++;
     }
     __device__ __forceinline__ uint2 pop() {
         sp--;
-        if (sp < SLDS) return lds[sp][tid];
+        if constexpr (SLDS > 0) {
+            if (sp < SLDS) return lds[sp][tid];
+        }
         if ((uint32_t)(sp - SLDS) < depth) return spill[sp - SLDS];
         return make_uint2(0u, 0u);
     }

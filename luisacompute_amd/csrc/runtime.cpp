@@ -759,6 +759,7 @@ void lc_hip_accel_destroy(uint64_t device, uint64_t accel) {
     if (a->tlas_host) hipHostFree(a->tlas_host);
     if (a->tlas_ready) hipEventDestroy(a->tlas_ready);
     if (a->state) hipFree(a->state);
+    if (a->desc) hipFree(a->desc);
     delete a;
 }
 
@@ -879,6 +880,8 @@ int lc_hip_accel_build(uint64_t device, uint64_t stream, uint64_t accel, uint32_
         a->state_written = false;
     }
     if (update_instance_buffer_only) {
+        const int rc = accel_write_descriptor(a, s);
+        if (rc != LC_HIP_OK) return rc;
         LC_HIP_TRY(hipStreamSynchronize(s)); // host_records is reused by the next build
         return LC_HIP_OK;
     }
@@ -951,10 +954,43 @@ int lc_hip_accel_build(uint64_t device, uint64_t stream, uint64_t accel, uint32_
         LC_HIP_TRY(hipEventRecord(a->tlas_ready, s));
         a->tlas_valid = true;
     }
-    // host_records / boxes staging must outlive the async copies
+    {
+        const int rc = accel_write_descriptor(a, s);
+        if (rc != LC_HIP_OK) return rc;
+    }
+    // host_records / boxes / descriptor staging must outlive the async copies
     LC_HIP_TRY(hipStreamSynchronize(s));
     return LC_HIP_OK;
 }
+
+} // extern "C"
+
+// The DeviceAccel descriptor of lc_hip_device.h: the query view of the accel (as
+// lc_hip_accel_query builds it) plus the instance table.
+int lchip::accel_write_descriptor(Accel *a, hipStream_t s) {
+    DeviceAccel &d = a->desc_host;
+    std::memset(&d, 0, sizeof(d));
+    const uint32_t n = (uint32_t)a->inst.size();
+    if (n == 0) {
+        d.geo.empty = 1;
+    } else if (n == 1) {
+        d.geo.instances = a->records;
+        d.geo.empty = a->host_records[0].blas_nodes == 0 ? 1u : 0u;
+    } else {
+        d.geo.two = 1;
+        d.geo.tlas_nodes = a->tlas_nodes;
+        d.geo.tlas_prims = a->tlas_prims;
+        d.geo.instances = a->records;
+    }
+    d.state = a->state;
+    d.state_count = a->state_count;
+    d.error = a->dev->error;
+    if (!a->desc) LC_HIP_TRY(hipMalloc(&a->desc, sizeof(DeviceAccel)));
+    LC_HIP_TRY(hipMemcpyAsync(a->desc, &d, sizeof(d), hipMemcpyHostToDevice, s));
+    return LC_HIP_OK;
+}
+
+extern "C" {
 
 static int accel_tlas_sync(Accel *a) {
     if (!a->tlas_stats_valid && a->tlas_valid) {
@@ -1616,6 +1652,149 @@ int lc_hip_timing_reset(uint64_t device) {
     hipError_t e = collect_timings(dev);
     dev->timings.clear();
     if (e != hipSuccess) return hip_fail(e, "kernel timing");
+    return LC_HIP_OK;
+}
+
+// ---- run-time shaders (create_shader / ShaderDispatchCommand over HIP source, shader.cpp) ----
+static void copy_log(const std::string &log, char *out, size_t cap) {
+    if (!out || cap == 0) return;
+    const size_t n = std::min(cap - 1, log.size());
+    std::memcpy(out, log.data(), n);
+    out[n] = '\0';
+}
+
+int lc_hip_shader_compile(const char *source, char *log, size_t log_capacity, size_t *code_size) {
+    if (!source) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "null shader source");
+    std::vector<char> code;
+    std::string msg;
+    const bool ok = rtc_compile(source, "shader.hip", code, msg);
+    copy_log(msg, log, log_capacity);
+    if (code_size) *code_size = code.size();
+    if (!ok) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "shader compile failed: " + msg.substr(0, 2000));
+    return LC_HIP_OK;
+}
+
+uint64_t lc_hip_shader_create(uint64_t device, const char *source, const char *entry, uint32_t block_x,
+                              uint32_t block_y, uint32_t block_z, char *log, size_t log_capacity) {
+    Device *dev = obj<Device>(device);
+    if (!dev || !source || !entry) {
+        fail(LC_HIP_ERROR_INVALID_ARGUMENT, "bad shader create arguments");
+        return LC_HIP_INVALID_HANDLE;
+    }
+    const uint64_t threads = (uint64_t)block_x * block_y * block_z;
+    if (threads == 0 || threads > 1024) {
+        fail(LC_HIP_ERROR_INVALID_ARGUMENT, "shader block size must hold 1..1024 threads");
+        return LC_HIP_INVALID_HANDLE;
+    }
+    std::vector<char> code;
+    std::string msg;
+    const bool ok = rtc_compile(source, "shader.hip", code, msg);
+    copy_log(msg, log, log_capacity);
+    if (!ok) {
+        fail(LC_HIP_ERROR_INVALID_ARGUMENT, "shader compile failed: " + msg.substr(0, 2000));
+        return LC_HIP_INVALID_HANDLE;
+    }
+    DeviceGuard g(dev);
+    auto sh = std::make_unique<Shader>();
+    sh->dev = dev;
+    sh->block[0] = block_x; sh->block[1] = block_y; sh->block[2] = block_z;
+    // a failed load or lookup is reported here and taken off HIP's last-error slot, which later
+    // launches read with hipGetLastError
+    hipError_t e = hipModuleLoadData(&sh->module, code.data());
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        hip_fail(e, "shader module load");
+        return LC_HIP_INVALID_HANDLE;
+    }
+    e = hipModuleGetFunction(&sh->fn, sh->module, entry);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipModuleUnload(sh->module);
+        hip_fail(e, "shader entry lookup");
+        return LC_HIP_INVALID_HANDLE;
+    }
+    return reinterpret_cast<uint64_t>(sh.release());
+}
+
+void lc_hip_shader_destroy(uint64_t device, uint64_t shader) {
+    Device *dev = obj<Device>(device);
+    Shader *sh = obj<Shader>(shader);
+    if (!dev || !sh) return;
+    DeviceGuard g(dev);
+    hipDeviceSynchronize();
+    if (sh->module) hipModuleUnload(sh->module);
+    delete sh;
+}
+
+int lc_hip_shader_dispatch(uint64_t device, uint64_t stream, uint64_t shader, const lc_hip_shader_argument *args,
+                           size_t argc, const void *uniforms, size_t uniform_size, uint32_t dx, uint32_t dy,
+                           uint32_t dz) {
+    Device *dev = obj<Device>(device);
+    Shader *sh = obj<Shader>(shader);
+    if (!dev || !sh) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad shader dispatch handles");
+    if (argc > 0 && !args) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "null shader arguments");
+    constexpr size_t kMaxArgBytes = 4096; // kernel argument segment
+    alignas(16) unsigned char buf[kMaxArgBytes];
+    size_t off = 0;
+    auto put = [&](const void *p, size_t n) -> bool {
+        off = (off + 15) & ~size_t(15);
+        if (off + n > kMaxArgBytes) return false;
+        std::memcpy(buf + off, p, n);
+        off += n;
+        return true;
+    };
+    for (size_t i = 0; i < argc; i++) {
+        const lc_hip_shader_argument &x = args[i];
+        if (x.tag == LC_HIP_ARGUMENT_BUFFER) {
+            Buffer *b = obj<Buffer>(x.handle);
+            if (!b) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad buffer argument");
+            if (x.offset > b->bytes) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "buffer argument offset outside its buffer");
+            const uint64_t size = x.size ? x.size : b->bytes - x.offset;
+            if (x.offset + size > b->bytes) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "buffer argument outside its buffer");
+            const uint64_t binding[2] = {reinterpret_cast<uint64_t>(static_cast<char *>(b->ptr) + x.offset), size};
+            if (!put(binding, sizeof(binding))) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "shader arguments exceed 4 KiB");
+        } else if (x.tag == LC_HIP_ARGUMENT_ACCEL) {
+            Accel *a = obj<Accel>(x.handle);
+            if (!a) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad accel argument");
+            std::lock_guard<std::mutex> lk(a->mu);
+            if (!a->built || !a->desc) return fail(LC_HIP_ERROR_NOT_BUILT, "accel argument is not built");
+            uint32_t need = 0;
+            if (a->inst.size() == 1) {
+                need = a->host_records[0].blas_depth + 2;
+            } else if (a->inst.size() > 1) {
+                const int rc = accel_tlas_sync(a);
+                if (rc != LC_HIP_OK) return rc;
+                need = a->tlas_stats.depth + 2 + a->max_blas_depth + 2;
+            }
+            if (need > 64) return fail(LC_HIP_ERROR_UNSUPPORTED, "accel too deep for a shader's traversal stack (64)");
+            a->state_written = true; // the shader may write instances: fold them at the next build
+            const uint64_t binding[2] = {reinterpret_cast<uint64_t>(a->desc), 0};
+            if (!put(binding, sizeof(binding))) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "shader arguments exceed 4 KiB");
+        } else if (x.tag == LC_HIP_ARGUMENT_UNIFORM) {
+            if (!uniforms || x.offset + x.size > uniform_size)
+                return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "uniform argument outside the uniform bytes");
+            if (!put(static_cast<const char *>(uniforms) + x.offset, x.size))
+                return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "shader arguments exceed 4 KiB");
+        } else {
+            return fail(LC_HIP_ERROR_UNSUPPORTED, "shader argument kind not supported (textures, bindless arrays)");
+        }
+    }
+    const uint32_t ls_kid[4] = {dx, dy, dz, 0u};
+    if (!put(ls_kid, sizeof(ls_kid))) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "shader arguments exceed 4 KiB");
+    if ((uint64_t)dx * dy * dz == 0) return LC_HIP_OK;
+    DeviceGuard g(dev);
+    hipStream_t s = native_stream(stream);
+    const uint32_t gx = (dx + sh->block[0] - 1) / sh->block[0];
+    const uint32_t gy = (dy + sh->block[1] - 1) / sh->block[1];
+    const uint32_t gz = (dz + sh->block[2] - 1) / sh->block[2];
+    size_t size = off;
+    void *extra[] = {HIP_LAUNCH_PARAM_BUFFER_POINTER, buf, HIP_LAUNCH_PARAM_BUFFER_SIZE, &size, HIP_LAUNCH_PARAM_END};
+    hipError_t e = hipModuleLaunchKernel(sh->fn, gx, gy, gz, sh->block[0], sh->block[1], sh->block[2], 0, s, nullptr,
+                                         extra);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return hip_fail(e, "shader launch");
+    }
     return LC_HIP_OK;
 }
 

@@ -21,13 +21,17 @@
 // (-ffp-contract=off), in the same operation order as the CPU oracle.
 #pragma once
 
+#ifndef __HIPCC_RTC__ // (hiprtc provides the HIP device environment itself)
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
 #include <cstdint>
+#endif
+#ifndef FLT_MAX
+#define FLT_MAX 3.402823466e+38f
+#endif
 
 #include "lc_hip_bvh.h"
-#include "lc_hip_internal.h"
 
 namespace lchip {
 namespace trav {

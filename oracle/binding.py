@@ -88,6 +88,7 @@ def lib():
             "orc_watertight": (I, [P, P, P, P, P, F, F, P, P, P]),
             "orc_pathtrace": (None, [P, P, ctypes.POINTER(PtParams), U64, U64, P, P, I]),
             "orc_sincos_2pi": (None, [F, ctypes.POINTER(F), ctypes.POINTER(F)]),
+            "orc_offset_ray_origin": (None, [P, P, P]),
             "orc_pathtrace_rays": (None, [P, P, ctypes.POINTER(PtParams), U64, U64, P, U64, ctypes.POINTER(U64), P, U64,
                                           ctypes.POINTER(U64)]),
         }
@@ -324,3 +325,13 @@ def watertight(o, d, v0, v1, v2, tmin=0.0, tmax=1e30):
     t, u, v = ctypes.c_float(), ctypes.c_float(), ctypes.c_float()
     h = lib().orc_watertight(*[_p(x) for x in a], tmin, tmax, ctypes.byref(t), ctypes.byref(u), ctypes.byref(v))
     return bool(h), t.value, u.value, v.value
+
+
+def offset_ray_origin(p, n):
+    """offset_ray_origin (src/dsl/rtx/ray.cpp:16-24) of float32 triples; (k, 3) arrays."""
+    p = np.ascontiguousarray(p, np.float32).reshape(-1, 3)
+    n = np.ascontiguousarray(n, np.float32).reshape(-1, 3)
+    out = np.zeros_like(p)
+    for i in range(p.shape[0]):
+        lib().orc_offset_ray_origin(_p(p[i]), _p(n[i]), _p(out[i]))
+    return out

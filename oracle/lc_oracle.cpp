@@ -1014,6 +1014,11 @@ F3 pt_sample(const Scene &sc, const orc_pt_material *mats, const orc_pt_params &
 
 extern "C" void orc_sincos_2pi(float u, float *s, float *c) { pt_sincos_2pi(u, *s, *c); }
 
+extern "C" void orc_offset_ray_origin(const float p[3], const float n[3], float out[3]) {
+    const F3 o = pt_offset(f3(p[0], p[1], p[2]), f3(n[0], n[1], n[2]));
+    out[0] = o.x; out[1] = o.y; out[2] = o.z;
+}
+
 extern "C" void orc_pathtrace_rays(const void *scene, const orc_pt_material *mats, const orc_pt_params *params,
                                    uint64_t pixel_begin, uint64_t pixel_end, float *closest, uint64_t closest_cap,
                                    uint64_t *closest_count, float *shadow, uint64_t shadow_cap, uint64_t *shadow_count) {

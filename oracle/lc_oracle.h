@@ -143,6 +143,8 @@ void orc_pathtrace(const void *scene, const orc_pt_material *mats, const orc_pt_
 void orc_pathtrace_rays(const void *scene, const orc_pt_material *mats, const orc_pt_params *params,
                         uint64_t pixel_begin, uint64_t pixel_end, float *closest, uint64_t closest_cap,
                         uint64_t *closest_count, float *shadow, uint64_t shadow_cap, uint64_t *shadow_count);
+/* offset_ray_origin (src/dsl/rtx/ray.cpp:16-24), as the path tracer applies it */
+void orc_offset_ray_origin(const float p[3], const float n[3], float out[3]);
 /* the path tracer's sin(2 pi u), cos(2 pi u) (exposed for the accuracy test) */
 void orc_sincos_2pi(float u, float *s, float *c);
 

@@ -1,0 +1,143 @@
+"""Shader sources of the run-time shader tests: the kernels a DSL codegen would emit for the
+reference's ray-tracing programs, written against the device library (lc_hip_device.h)."""
+
+# hit = accel.intersect(ray, {mask}) / hit_any = accel.intersect_any(ray, {mask}) per thread
+# (src/dsl/rtx/accel.cpp:136-150)
+TRACE = r'''
+#include "lc_hip_device.h"
+using namespace lchip;
+struct Params {
+    alignas(16) dev::Buffer<dev::Ray> rays;
+    alignas(16) dev::Buffer<dev::SurfaceHit> hits;
+    alignas(16) dev::Buffer<uint8_t> any;
+    alignas(16) dev::Accel accel;
+    alignas(16) uint32_t mask;
+    alignas(16) uint4 ls_kid;
+};
+extern "C" __global__ void __launch_bounds__(128) trace_kernel(const Params p) {
+    if (!dev::in_dispatch(p.ls_kid)) return;
+    const uint32_t i = dev::dispatch_id().x;
+    const dev::Ray r = p.rays[i];
+    p.hits[i] = dev::trace_closest(p.accel, r, p.mask);
+    p.any[i] = dev::trace_any(p.accel, r, p.mask) ? 1u : 0u;
+}
+'''
+
+# accel.traverse(ray, {}).on_surface_candidate(...).on_procedural_candidate(...).trace() with the
+# callbacks of src/tests/test_procedural.cpp:86-120 written as the codegen would inline them
+QUERY = r'''
+#include "lc_hip_device.h"
+using namespace lchip;
+struct Params {
+    alignas(16) dev::Buffer<dev::Ray> rays;
+    alignas(16) dev::Buffer<rq::CommittedHit> out;
+    alignas(16) dev::Accel accel;
+    alignas(16) float cutout;
+    alignas(16) uint32_t any;
+    alignas(16) uint4 ls_kid;
+};
+extern "C" __global__ void __launch_bounds__(128) query_kernel(const Params p) {
+    if (!dev::in_dispatch(p.ls_kid)) return;
+    const uint32_t i = dev::dispatch_id().x;
+    const float cutout = p.cutout;
+    auto on_surface = [cutout](uint32_t, uint32_t, float u, float v, float) {
+        const float w = (1.0f - u) - v;
+        return sqrtf(w * w + u * u) < cutout && sqrtf(u * u + v * v) < cutout && sqrtf(v * v + w * w) < cutout;
+    };
+    auto on_procedural = [](uint32_t, uint32_t, const float *o, const float *d, const float *, const float *,
+                            float4 lo, float4 hi, float, float &t) {
+        const float c[3] = {(lo.x + hi.x) * 0.5f, (lo.y + hi.y) * 0.5f, (lo.z + hi.z) * 0.5f};
+        const float r = 0.5f * fminf(fminf(hi.x - lo.x, hi.y - lo.y), hi.z - lo.z);
+        const float L[3] = {c[0] - o[0], c[1] - o[1], c[2] - o[2]};
+        const float dd = (d[0] * d[0] + d[1] * d[1]) + d[2] * d[2];
+        const float ld = (L[0] * d[0] + L[1] * d[1]) + L[2] * d[2];
+        const float ll = (L[0] * L[0] + L[1] * L[1]) + L[2] * L[2];
+        const float tc = ld / dd;
+        if (!(tc > 0.0f)) return false;
+        const float q = ll - (tc * tc) * dd;
+        const float rr = r * r;
+        if (!(q <= rr)) return false;
+        t = tc - sqrtf((rr - q) / dd);
+        return true;
+    };
+    p.out[i] = dev::query(p.accel, p.rays[i], 0xFFu, p.any != 0u, on_surface, on_procedural);
+}
+'''
+
+# a surface callback of the shader's own: commit only even primitive ids; procedural ignored
+QUERY_EVEN = r'''
+#include "lc_hip_device.h"
+using namespace lchip;
+struct Params {
+    alignas(16) dev::Buffer<dev::Ray> rays;
+    alignas(16) dev::Buffer<rq::CommittedHit> out;
+    alignas(16) dev::Accel accel;
+    alignas(16) uint4 ls_kid;
+};
+extern "C" __global__ void __launch_bounds__(128) even_kernel(const Params p) {
+    if (!dev::in_dispatch(p.ls_kid)) return;
+    const uint32_t i = dev::dispatch_id().x;
+    auto even = [](uint32_t, uint32_t prim, float, float, float) { return (prim & 1u) == 0u; };
+    auto none = [](uint32_t, uint32_t, const float *, const float *, const float *, const float *, float4, float4,
+                   float, float &) { return false; };
+    p.out[i] = dev::query(p.accel, p.rays[i], 0xFFu, false, even, none);
+}
+'''
+
+# RAY_TRACING_INSTANCE_TRANSFORM / _USER_ID / _VISIBILITY_MASK reads, then SET_INSTANCE_USER_ID
+# and SET_INSTANCE_VISIBILITY_MASK on the instances named by `writes` (include/luisa/ast/op.h:290-297)
+INSTANCES = r'''
+#include "lc_hip_device.h"
+using namespace lchip;
+struct Params {
+    alignas(16) dev::Buffer<float> transforms;
+    alignas(16) dev::Buffer<uint32_t> ids;
+    alignas(16) dev::Accel accel;
+    alignas(16) uint32_t new_id_base;
+    alignas(16) uint32_t new_mask;
+    alignas(16) uint4 ls_kid;
+};
+extern "C" __global__ void __launch_bounds__(64) instance_kernel(const Params p) {
+    if (!dev::in_dispatch(p.ls_kid)) return;
+    const uint32_t i = dev::dispatch_id().x;
+    float m[12];
+    dev::instance_transform(p.accel, i, m);
+    for (int c = 0; c < 12; c++) p.transforms[12 * (uint64_t)i + c] = m[c];
+    p.ids[2 * i] = dev::instance_user_id(p.accel, i);
+    p.ids[2 * i + 1] = dev::instance_visibility_mask(p.accel, i);
+    if (i % 3 == 0) {
+        dev::set_instance_user_id(p.accel, i, p.new_id_base + i);
+        dev::set_instance_visibility_mask(p.accel, i, p.new_mask);
+    }
+}
+'''
+
+# make_ray / offset_ray_origin / triangle_interpolate (src/dsl/rtx/ray.cpp:7-28, hit.cpp:15-17)
+HELPERS = r'''
+#include "lc_hip_device.h"
+using namespace lchip;
+struct Params {
+    alignas(16) dev::Buffer<float> in;   // per item: p, n, w, a, b, c (3 floats each), u, v
+    alignas(16) dev::Buffer<float> out;  // per item: offset(p, n), offset(p, n, w), interpolate, make_ray (8)
+    alignas(16) uint4 ls_kid;
+};
+extern "C" __global__ void __launch_bounds__(64) helper_kernel(const Params p) {
+    if (!dev::in_dispatch(p.ls_kid)) return;
+    const uint64_t i = dev::dispatch_id().x;
+    const float *x = &p.in[20 * i];
+    const rtx::V3 P = rtx::v3(x[0], x[1], x[2]), N = rtx::v3(x[3], x[4], x[5]), W = rtx::v3(x[6], x[7], x[8]);
+    const rtx::V3 A = rtx::v3(x[9], x[10], x[11]), B = rtx::v3(x[12], x[13], x[14]), C = rtx::v3(x[15], x[16], x[17]);
+    const rtx::V3 o1 = rtx::offset_ray_origin(P, N), o2 = rtx::offset_ray_origin(P, N, W);
+    const rtx::V3 q = rtx::triangle_interpolate(x[18], x[19], A, B, C);
+    const dev::Ray r = dev::make_ray(o1, N);
+    float *y = &p.out[17 * i];
+    y[0] = o1.x; y[1] = o1.y; y[2] = o1.z;
+    y[3] = o2.x; y[4] = o2.y; y[5] = o2.z;
+    y[6] = q.x; y[7] = q.y; y[8] = q.z;
+    y[9] = r.origin[0]; y[10] = r.origin[1]; y[11] = r.origin[2]; y[12] = r.t_min;
+    y[13] = r.direction[0]; y[14] = r.direction[1]; y[15] = r.direction[2]; y[16] = r.t_max;
+}
+'''
+
+ALL = {"trace_kernel": TRACE, "query_kernel": QUERY, "even_kernel": QUERY_EVEN, "instance_kernel": INSTANCES,
+       "helper_kernel": HELPERS}
