@@ -12,10 +12,7 @@ Ray tracing from this object runs as batches over host arrays (trace_closest / t
 query_all / query_any); the per-thread DSL builtins of the reference (BuiltinFuncBuilder) have
 their counterparts in the device library of run-time shaders (lc_hip_device.h).
 """
-import numpy as np
-
-from . import (AccelOption, BUILD_PREFER_UPDATE, IDENTITY, LCError, QUERY_PROCEDURAL_SPHERE, QUERY_SURFACE_COMMIT,
-               host)
+from . import AccelOption, BUILD_PREFER_UPDATE, IDENTITY, LCError, host
 
 
 class ManagedAccel:
@@ -33,11 +30,12 @@ class ManagedAccel:
             kind, option = desc[0], AccelOption(*desc[1])
             if kind == "mesh":
                 _, _, vb, voff, vsize, vstride, tb, toff, tsize = desc
-                blas = self.device.create_mesh(vb, tb, option=option, vertex_stride=vstride, vertex_offset=voff,
-                                               vertex_size=vsize, triangle_offset=toff, triangle_size=tsize)
+                blas = self.device.create_mesh(vb.obj, tb.obj, option=option, vertex_stride=vstride,
+                                               vertex_offset=voff, vertex_size=vsize, triangle_offset=toff,
+                                               triangle_size=tsize)
             else:
                 _, _, ab, aoff, asize = desc
-                blas = self.device.create_procedural_primitive(ab, option=option, offset=aoff, size=asize)
+                blas = self.device.create_procedural_primitive(ab.obj, option=option, offset=aoff, size=asize)
             entry = self._blas[desc] = [blas, 0]
             self._pending[desc] = blas
         entry[1] += 1

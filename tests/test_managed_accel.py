@@ -82,6 +82,7 @@ def test_sharing_building_and_release():
     m.add(vb, tb)
     m.add(vb, tb)                      # same description: shared
     assert m.blas_count == 1 and len(dev.created) == 1 and len(m) == 2
+    assert dev.created[0].args[0] is vb and dev.created[0].args[1] is tb
     m.update(s)
     assert [c.what for c in s.cmds] == ["build mesh", "build accel"]
     s.cmds.clear()
@@ -108,6 +109,7 @@ def test_procedural_descriptions_and_checks():
     m.add_procedural(ab, 0, 100)       # the same range
     m.add_procedural(ab, 10, 20)
     assert m.blas_count == 2
+    assert dev.created[1].args[0] is ab
     assert dev.created[1].args[1]["offset"] == 240 and dev.created[1].args[1]["size"] == 480
     with pytest.raises(LCError):
         m.add_procedural(_Buf(16, 10))
