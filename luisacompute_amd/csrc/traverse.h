@@ -112,7 +112,10 @@ __device__ __forceinline__ uint32_t permute8(uint32_t m, uint32_t f) {
 
 __device__ __forceinline__ float ubyte(uint32_t w, int k) { return (float)((w >> (8 * k)) & 0xFFu); }
 
-// Tests the 8 children of a node; returns the hit mask in slot order (occupied slots only).
+// Tests the 8 children of a node; returns the hit mask in slot order. Empty slots hold the inverted
+// box qlo = 255, qhi = 0 (and meta 0), so the box test itself rejects them; a spurious hit on one
+// (possible only when every axis of the node has a vanishing scale) is a leaf slot of zero records
+// and changes nothing.
 __device__ __forceinline__ uint32_t intersect_node(const RaySetup &r, const uint4 w0, const uint4 w1, const uint4 w2,
                                                    const uint4 w3, const uint4 w4, float tmin, float tcull) {
     const float p[3] = {__uint_as_float(w0.x), __uint_as_float(w0.y), __uint_as_float(w0.z)};
@@ -140,7 +143,6 @@ __device__ __forceinline__ uint32_t intersect_node(const RaySetup &r, const uint
         const uint32_t wnx = s < 4 ? nx0 : nx1, wfx = s < 4 ? fx0 : fx1;
         const uint32_t wny = s < 4 ? ny0 : ny1, wfy = s < 4 ? fy0 : fy1;
         const uint32_t wnz = s < 4 ? nz0 : nz1, wfz = s < 4 ? fz0 : fz1;
-        const uint32_t wm = s < 4 ? w1.z : w1.w;
         float tnx = __builtin_fmaf(ubyte(wnx, k), a[0], bn[0]);
         float tny = __builtin_fmaf(ubyte(wny, k), a[1], bn[1]);
         float tnz = __builtin_fmaf(ubyte(wnz, k), a[2], bn[2]);
@@ -149,8 +151,7 @@ __device__ __forceinline__ uint32_t intersect_node(const RaySetup &r, const uint
         float tfz = __builtin_fmaf(ubyte(wfz, k), a[2], bf[2]);
         float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, tmin));
         float tf = fminf(fminf(tfx, tfy), fminf(tfz, tcull));
-        bool occ = ((wm >> (8 * k)) & 0xFFu) != 0u;
-        if (occ && tn <= tf) hits |= 1u << s;
+        if (tn <= tf) hits |= 1u << s;
     }
     return hits;
 }
