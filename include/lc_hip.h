@@ -276,6 +276,7 @@ int lc_hip_accel_query(uint64_t device, uint64_t stream, uint64_t accel, uint64_
    compiler's log in `log` (truncated to log_capacity). ---- */
 #define LC_HIP_ARGUMENT_BUFFER 0u  /* Argument::Tag (include/luisa/runtime/rhi/argument.h:9-15) */
 #define LC_HIP_ARGUMENT_UNIFORM 2u
+#define LC_HIP_ARGUMENT_BINDLESS_ARRAY 3u /* {slot table pointer, slot count} */
 #define LC_HIP_ARGUMENT_ACCEL 4u
 typedef struct lc_hip_shader_argument {
     uint32_t tag;
@@ -292,6 +293,27 @@ void lc_hip_shader_destroy(uint64_t device, uint64_t shader);
 int lc_hip_shader_dispatch(uint64_t device, uint64_t stream, uint64_t shader, const lc_hip_shader_argument *arguments,
                            size_t argument_count, const void *uniforms, size_t uniform_size, uint32_t dispatch_x,
                            uint32_t dispatch_y, uint32_t dispatch_z);
+
+/* ---- bindless arrays: create_bindless_array / destroy_bindless_array (device_interface.h) and
+   BindlessArrayUpdateCommand (command.h:617-680) for buffer slots: a slot holds {pointer at the
+   buffer offset, bytes from there to the buffer's end} (lc_hip_device.h dev::BindlessArray).
+   Texture slots are not supported (LC_HIP_ERROR_UNSUPPORTED). Operation values are the
+   reference's Modification::Operation (NONE, EMPLACE, REMOVE). The update is complete when the
+   call returns. ---- */
+#define LC_HIP_BINDLESS_NONE 0u
+#define LC_HIP_BINDLESS_EMPLACE 1u
+#define LC_HIP_BINDLESS_REMOVE 2u
+typedef struct lc_hip_bindless_modification {
+    size_t slot;
+    uint64_t buffer;     /* EMPLACE: buffer handle */
+    size_t offset_bytes; /* EMPLACE: byte offset into the buffer */
+    uint32_t buffer_op;
+    uint32_t texture_op; /* must be LC_HIP_BINDLESS_NONE */
+} lc_hip_bindless_modification;
+uint64_t lc_hip_bindless_create(uint64_t device, size_t slots);
+void lc_hip_bindless_destroy(uint64_t device, uint64_t array);
+int lc_hip_bindless_update(uint64_t device, uint64_t stream, uint64_t array, const lc_hip_bindless_modification *mods,
+                           size_t count);
 
 /* ---- path tracing: the DSL program of src/tests/test_path_tracing.cpp:165-258 (camera ray,
    closest hit, emission with the balance heuristic, one area-light sample with a shadow ray

@@ -71,7 +71,14 @@ class ShaderArgument(ctypes.Structure):
     """lc_hip_shader_argument (Argument, include/luisa/runtime/rhi/argument.h:7-48)."""
     _fields_ = [("tag", ctypes.c_uint32), ("_pad", ctypes.c_uint32), ("handle", ctypes.c_uint64),
                 ("offset", ctypes.c_size_t), ("size", ctypes.c_size_t)]
-    BUFFER, UNIFORM, ACCEL = 0, 2, 4
+    BUFFER, UNIFORM, BINDLESS_ARRAY, ACCEL = 0, 2, 3, 4
+
+
+class BindlessModification(ctypes.Structure):
+    """lc_hip_bindless_modification (BindlessArrayUpdateCommand::Modification, command.h:621-680)."""
+    _fields_ = [("slot", ctypes.c_size_t), ("buffer", ctypes.c_uint64), ("offset_bytes", ctypes.c_size_t),
+                ("buffer_op", ctypes.c_uint32), ("texture_op", ctypes.c_uint32)]
+    NONE, EMPLACE, REMOVE = 0, 1, 2
 
 
 class Modification(ctypes.Structure):
@@ -191,6 +198,9 @@ _SIGNATURES = {
     "lc_hip_procedural_build": (ctypes.c_int, [_U64, _U64, _U64, _U32, _U64, _SZ, _SZ]),
     "lc_hip_accel_query": (ctypes.c_int, [_U64, _U64, _U64, _U64, _SZ, _U64, _SZ, _U32, _U32,
                                           ctypes.POINTER(QueryPolicy)]),
+    "lc_hip_bindless_create": (_U64, [_U64, _SZ]),
+    "lc_hip_bindless_destroy": (None, [_U64, _U64]),
+    "lc_hip_bindless_update": (ctypes.c_int, [_U64, _U64, _U64, ctypes.POINTER(BindlessModification), _SZ]),
     "lc_hip_shader_compile": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, _SZ, ctypes.POINTER(_SZ)]),
     "lc_hip_shader_create": (_U64, [_U64, ctypes.c_char_p, ctypes.c_char_p, _U32, _U32, _U32, ctypes.c_char_p, _SZ]),
     "lc_hip_shader_destroy": (None, [_U64, _U64]),
@@ -408,6 +418,9 @@ class Device:
     def create_accel(self, option=None):
         return Accel(self, option)
 
+    def create_bindless_array(self, slots):
+        return BindlessArray(self, slots)
+
     def create_shader(self, source, entry, block=(128, 1, 1)):
         """Compiles HIP source against the device library (lc_hip_device.h) with hiprtc."""
         return Shader(self, source, entry, block)
@@ -604,6 +617,8 @@ class _Dispatch(Command):
                 argv[i] = ShaderArgument(ShaderArgument.BUFFER, 0, a.handle, 0, 0)
             elif isinstance(a, Accel):
                 argv[i] = ShaderArgument(ShaderArgument.ACCEL, 0, a.handle, 0, 0)
+            elif isinstance(a, BindlessArray):
+                argv[i] = ShaderArgument(ShaderArgument.BINDLESS_ARRAY, 0, a.handle, 0, 0)
             else:  # uniform: numpy scalar / array or bytes
                 b = a if isinstance(a, (bytes, bytearray)) else np.ascontiguousarray(a).tobytes()
                 argv[i] = ShaderArgument(ShaderArgument.UNIFORM, 0, 0, len(blob), len(b))
@@ -613,6 +628,44 @@ class _Dispatch(Command):
         _check(lib().lc_hip_shader_dispatch(stream.device.handle, stream.handle, self.shader.handle, argv,
                                             len(self.args), ctypes.cast(ub, ctypes.c_void_p), len(blob), x, y, z),
                "shader dispatch")
+
+
+class _BindlessUpdate(Command):
+    def __init__(self, array, mods):
+        self.array, self.mods = array, mods
+
+    def execute(self, stream):
+        arr = (BindlessModification * max(1, len(self.mods)))(*self.mods)
+        _check(lib().lc_hip_bindless_update(stream.device.handle, stream.handle, self.array.handle, arr,
+                                            len(self.mods)), "bindless update")
+
+
+class BindlessArray:
+    """Bindless array of buffer slots (BindlessArray, include/luisa/runtime/bindless_array.h):
+    emplace / remove record modifications, update() is the BindlessArrayUpdateCommand."""
+
+    def __init__(self, device, slots):
+        self.device, self.slots = device, int(slots)
+        self.handle = _handle(lib().lc_hip_bindless_create(device.handle, self.slots), "bindless array create")
+        self._mods, self._bound = {}, {}
+
+    def __del__(self):
+        if getattr(self, "handle", None) is not None and _lib is not None and self.device.handle is not None:
+            lib().lc_hip_bindless_destroy(self.device.handle, self.handle)
+            self.handle = None
+
+    def emplace(self, slot, buffer, offset_bytes=0):
+        self._mods[slot] = BindlessModification(slot, buffer.handle, offset_bytes, BindlessModification.EMPLACE, 0)
+        self._bound[slot] = buffer  # keeps the buffer alive while bound
+
+    def remove(self, slot):
+        self._mods[slot] = BindlessModification(slot, 0, 0, BindlessModification.REMOVE, 0)
+        self._bound.pop(slot, None)
+
+    def update(self):
+        mods = [self._mods[k] for k in sorted(self._mods)]
+        self._mods = {}
+        return _BindlessUpdate(self, mods)
 
 
 class Shader:

@@ -140,4 +140,10 @@ struct DeviceAccel {
     uint32_t _pad[2];
 };
 
+// One slot of a bindless array (buffer at its bound offset, bytes from there to the end).
+struct BindlessSlot {
+    void *ptr;
+    uint64_t size_bytes;
+};
+
 } // namespace lchip

@@ -55,6 +55,26 @@ struct alignas(16) Accel {
     uint64_t _pad;
 };
 
+// Bindless array of buffer slots (BindlessArray, include/luisa/runtime/bindless_array.h; the
+// CUDA backend's buffer reads cuda_device_resource.h): slot s = {pointer at the emplaced offset,
+// bytes to the buffer's end}; reads and writes are unchecked, like the reference's.
+struct alignas(16) BindlessArray {
+    const BindlessSlot *slots;
+    uint64_t count;
+};
+template<class T>
+__device__ __forceinline__ T bindless_buffer_read(const BindlessArray a, uint32_t slot, uint64_t i) {
+    return static_cast<const T *>(a.slots[slot].ptr)[i];
+}
+template<class T>
+__device__ __forceinline__ void bindless_buffer_write(const BindlessArray a, uint32_t slot, uint64_t i, const T &v) {
+    static_cast<T *>(a.slots[slot].ptr)[i] = v;
+}
+template<class T>
+__device__ __forceinline__ uint64_t bindless_buffer_size(const BindlessArray a, uint32_t slot) {
+    return a.slots[slot].size_bytes / sizeof(T);
+}
+
 // dispatch_id / dispatch_size of a 1-3D dispatch (ls_kid.xyz = dispatch size)
 __device__ __forceinline__ uint3 dispatch_id() {
     return make_uint3(blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y * blockDim.y + threadIdx.y,

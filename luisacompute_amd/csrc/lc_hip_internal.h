@@ -371,6 +371,13 @@ struct Accel {
     DeviceAccel desc_host{}; // staging of the last descriptor upload
 };
 
+// ---- bindless arrays (buffer slots) ----
+struct Bindless {
+    Device *dev = nullptr;
+    BindlessSlot *slots = nullptr;  // device table
+    std::vector<BindlessSlot> host; // the table as last uploaded
+};
+
 // ---- run-time shaders (hiprtc; shader.cpp) ----
 struct Shader {
     Device *dev = nullptr;

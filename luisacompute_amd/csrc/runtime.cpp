@@ -1655,6 +1655,68 @@ int lc_hip_timing_reset(uint64_t device) {
     return LC_HIP_OK;
 }
 
+// ---- bindless arrays (buffer slots): create_bindless_array + BindlessArrayUpdateCommand ----
+uint64_t lc_hip_bindless_create(uint64_t device, size_t slots) {
+    Device *dev = obj<Device>(device);
+    if (!dev || slots == 0 || slots > (1u << 24)) {
+        fail(LC_HIP_ERROR_INVALID_ARGUMENT, "bindless array needs 1..2^24 slots");
+        return LC_HIP_INVALID_HANDLE;
+    }
+    DeviceGuard g(dev);
+    auto b = std::make_unique<Bindless>();
+    b->dev = dev;
+    b->host.assign(slots, BindlessSlot{nullptr, 0});
+    hipError_t e = hipMalloc(&b->slots, slots * sizeof(BindlessSlot));
+    if (e == hipSuccess) e = hipMemset(b->slots, 0, slots * sizeof(BindlessSlot));
+    if (e != hipSuccess) {
+        if (b->slots) (void)hipFree(b->slots);
+        hip_fail(e, "bindless array");
+        return LC_HIP_INVALID_HANDLE;
+    }
+    return reinterpret_cast<uint64_t>(b.release());
+}
+
+void lc_hip_bindless_destroy(uint64_t device, uint64_t array) {
+    Device *dev = obj<Device>(device);
+    Bindless *b = obj<Bindless>(array);
+    if (!dev || !b) return;
+    DeviceGuard g(dev);
+    (void)hipDeviceSynchronize();
+    if (b->slots) (void)hipFree(b->slots);
+    delete b;
+}
+
+int lc_hip_bindless_update(uint64_t device, uint64_t stream, uint64_t array, const lc_hip_bindless_modification *mods,
+                           size_t count) {
+    Device *dev = obj<Device>(device);
+    Bindless *b = obj<Bindless>(array);
+    if (!dev || !b) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad bindless update handles");
+    if (count > 0 && !mods) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "null bindless modifications");
+    std::vector<BindlessSlot> next = b->host;
+    for (size_t i = 0; i < count; i++) {
+        const lc_hip_bindless_modification &m = mods[i];
+        if (m.slot >= next.size()) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "bindless slot out of range");
+        if (m.texture_op != LC_HIP_BINDLESS_NONE)
+            return fail(LC_HIP_ERROR_UNSUPPORTED, "bindless texture slots are not supported");
+        if (m.buffer_op == LC_HIP_BINDLESS_EMPLACE) {
+            Buffer *buf = obj<Buffer>(m.buffer);
+            if (!buf) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad bindless buffer");
+            if (m.offset_bytes > buf->bytes) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "bindless offset beyond the buffer");
+            next[m.slot] = BindlessSlot{static_cast<char *>(buf->ptr) + m.offset_bytes, buf->bytes - m.offset_bytes};
+        } else if (m.buffer_op == LC_HIP_BINDLESS_REMOVE) {
+            next[m.slot] = BindlessSlot{nullptr, 0};
+        } else if (m.buffer_op != LC_HIP_BINDLESS_NONE) {
+            return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "unknown bindless operation");
+        }
+    }
+    DeviceGuard g(dev);
+    hipStream_t s = native_stream(stream);
+    LC_HIP_TRY(hipMemcpyAsync(b->slots, next.data(), next.size() * sizeof(BindlessSlot), hipMemcpyHostToDevice, s));
+    LC_HIP_TRY(hipStreamSynchronize(s)); // the staging is this call's
+    b->host.swap(next);
+    return LC_HIP_OK;
+}
+
 // ---- run-time shaders (create_shader / ShaderDispatchCommand over HIP source, shader.cpp) ----
 static void copy_log(const std::string &log, char *out, size_t cap) {
     if (!out || cap == 0) return;
@@ -1770,13 +1832,18 @@ int lc_hip_shader_dispatch(uint64_t device, uint64_t stream, uint64_t shader, co
             a->state_written = true; // the shader may write instances: fold them at the next build
             const uint64_t binding[2] = {reinterpret_cast<uint64_t>(a->desc), 0};
             if (!put(binding, sizeof(binding))) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "shader arguments exceed 4 KiB");
+        } else if (x.tag == LC_HIP_ARGUMENT_BINDLESS_ARRAY) {
+            Bindless *b = obj<Bindless>(x.handle);
+            if (!b) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad bindless array argument");
+            const uint64_t binding[2] = {reinterpret_cast<uint64_t>(b->slots), (uint64_t)b->host.size()};
+            if (!put(binding, sizeof(binding))) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "shader arguments exceed 4 KiB");
         } else if (x.tag == LC_HIP_ARGUMENT_UNIFORM) {
             if (!uniforms || x.offset + x.size > uniform_size)
                 return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "uniform argument outside the uniform bytes");
             if (!put(static_cast<const char *>(uniforms) + x.offset, x.size))
                 return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "shader arguments exceed 4 KiB");
         } else {
-            return fail(LC_HIP_ERROR_UNSUPPORTED, "shader argument kind not supported (textures, bindless arrays)");
+            return fail(LC_HIP_ERROR_UNSUPPORTED, "shader argument kind not supported (textures)");
         }
     }
     const uint32_t ls_kid[4] = {dx, dy, dz, 0u};

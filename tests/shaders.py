@@ -141,3 +141,35 @@ extern "C" __global__ void __launch_bounds__(64) helper_kernel(const Params p) {
 
 ALL = {"trace_kernel": TRACE, "query_kernel": QUERY, "even_kernel": QUERY_EVEN, "instance_kernel": INSTANCES,
        "helper_kernel": HELPERS}
+
+# a path tracer's heap access (src/tests/test_path_tracing.cpp:212-222): per hit, the instance's
+# vertex and triangle buffers come from a bindless array (slots 2i, 2i + 1), the hit point from
+# triangle_interpolate of the three vertices
+BINDLESS = r'''
+#include "lc_hip_device.h"
+using namespace lchip;
+struct Params {
+    alignas(16) dev::Buffer<dev::Ray> rays;
+    alignas(16) dev::Buffer<float4> points;   // hit point, w = the slot's triangle count; zeros on a miss
+    alignas(16) dev::Accel accel;
+    alignas(16) dev::BindlessArray heap;
+    alignas(16) uint4 ls_kid;
+};
+extern "C" __global__ void __launch_bounds__(128) bindless_kernel(const Params p) {
+    if (!dev::in_dispatch(p.ls_kid)) return;
+    const uint32_t i = dev::dispatch_id().x;
+    const dev::SurfaceHit h = dev::trace_closest(p.accel, p.rays[i]);
+    if (h.inst == 0xFFFFFFFFu) { p.points[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f); return; }
+    const uint64_t ntri = dev::bindless_buffer_size<uint3>(p.heap, 2 * h.inst + 1);
+    if (h.prim >= ntri) { p.points[i] = make_float4(0.0f, 0.0f, 0.0f, -1.0f); return; } // reads stay in bounds
+    const uint3 t = dev::bindless_buffer_read<uint3>(p.heap, 2 * h.inst + 1, h.prim);
+    const float4 a = dev::bindless_buffer_read<float4>(p.heap, 2 * h.inst, t.x);
+    const float4 b = dev::bindless_buffer_read<float4>(p.heap, 2 * h.inst, t.y);
+    const float4 c = dev::bindless_buffer_read<float4>(p.heap, 2 * h.inst, t.z);
+    const rtx::V3 q = rtx::triangle_interpolate(h.bary[0], h.bary[1], rtx::v3(a.x, a.y, a.z), rtx::v3(b.x, b.y, b.z),
+                                                rtx::v3(c.x, c.y, c.z));
+    p.points[i] = make_float4(q.x, q.y, q.z, (float)ntri);
+}
+'''
+
+ALL["bindless_kernel"] = BINDLESS

@@ -195,3 +195,51 @@ def test_dispatch_argument_errors(device, stream):
     sh2 = device.create_shader(shaders.INSTANCES, "instance_kernel", block=(64, 1, 1))
     with pytest.raises(lc.LCError):
         stream << sh2(b, b, unbuilt, np.uint32(0), np.uint32(0)).dispatch(1)
+
+
+def test_bindless_heap_shader(device, stream):
+    """A path tracer's heap access through a bindless array of buffer slots (vertex and triangle
+    buffers per instance): hit points interpolated from the slot data equal the host's."""
+    rng = np.random.default_rng(48)
+    meshes = [lc.icosphere(2), lc.soup(500, initseq=19, extent=2.0, sigma=0.2)]
+    gm = [host.mesh_from_arrays(device, stream, v, t) for v, t in meshes]
+    accel = device.create_accel()
+    heap = device.create_bindless_array(8)
+    kinds = [0, 1, 0]
+    for i, k in enumerate(kinds):
+        accel.emplace_back(gm[k], random_rigid(rng, -3, 3, 0.5, 1.5))
+        heap.emplace(2 * i, gm[k].vb)
+        heap.emplace(2 * i + 1, gm[k].tb)
+    stream << accel.build() << heap.update()
+    rays = lc.rays_as_floats(lc.rays_box_sphere(20000, initseq=49, lo=-4.0, hi=4.0))
+    sh = device.create_shader(shaders.BINDLESS, "bindless_kernel")
+    n = rays.shape[0]
+    rb, pb = host.upload(device, stream, host.as_rays(rays)), device.create_buffer(16, n)
+
+    def run():
+        stream << sh(rb, pb, accel, heap).dispatch(n)
+        out = np.zeros((n, 4), np.float32)
+        stream << pb.copy_to(out) << lc.synchronize()
+        return out
+
+    out = run()
+    h = host.trace_closest(device, stream, accel, rays)
+    hit = h["inst"] != 0xFFFFFFFF
+    assert hit.sum() > 2000
+    assert (out[~hit] == 0).all()
+    for i in np.flatnonzero(hit)[:4000]:
+        v, t = meshes[kinds[h["inst"][i]]]
+        a, b, c = (v[t[h["prim"][i]][j], :3] for j in range(3))
+        u, w = np.float32(h["bary"][i, 0]), np.float32(h["bary"][i, 1])
+        k = (np.float32(1) - u) - w
+        np.testing.assert_array_equal(out[i, :3], (k * a + u * b) + w * c)
+        assert out[i, 3] == len(t)
+    # re-bind instance 0's triangle slot at an offset of 10 triangles: the slot size follows
+    heap.emplace(1, gm[0].tb, 12 * 10)
+    heap.remove(7)
+    stream << heap.update()
+    out = run()
+    ntri = len(meshes[0][1]) - 10
+    inside = np.flatnonzero(hit & (h["inst"] == 0) & (h["prim"] < ntri))
+    beyond = np.flatnonzero(hit & (h["inst"] == 0) & (h["prim"] >= ntri))
+    assert len(inside) and (out[inside, 3] == ntri).all() and (out[beyond, 3] == -1).all()
