@@ -275,6 +275,7 @@ int lc_hip_accel_query(uint64_t device, uint64_t stream, uint64_t accel, uint64_
    size. Compile errors return LC_HIP_INVALID_HANDLE / LC_HIP_ERROR_INVALID_ARGUMENT with the
    compiler's log in `log` (truncated to log_capacity). ---- */
 #define LC_HIP_ARGUMENT_BUFFER 0u  /* Argument::Tag (include/luisa/runtime/rhi/argument.h:9-15) */
+#define LC_HIP_ARGUMENT_TEXTURE 1u /* {pixel pointer, width, height, storage} (32 B) */
 #define LC_HIP_ARGUMENT_UNIFORM 2u
 #define LC_HIP_ARGUMENT_BINDLESS_ARRAY 3u /* {slot table pointer, slot count} */
 #define LC_HIP_ARGUMENT_ACCEL 4u
@@ -293,6 +294,33 @@ void lc_hip_shader_destroy(uint64_t device, uint64_t shader);
 int lc_hip_shader_dispatch(uint64_t device, uint64_t stream, uint64_t shader, const lc_hip_shader_argument *arguments,
                            size_t argument_count, const void *uniforms, size_t uniform_size, uint32_t dispatch_x,
                            uint32_t dispatch_y, uint32_t dispatch_z);
+
+/* ---- images: create_texture / destroy_texture (device_interface.h:125-133) for 2-D images of one
+   level, stored as linear rows of pixels, and TextureUploadCommand / TextureDownloadCommand
+   (command.h:235-330) of the whole image (rows tightly packed). Storage values are the
+   reference's PixelStorage (include/luisa/runtime/rhi/pixel.h:14-35): BYTE1..FLOAT4 (0..14);
+   block-compressed and packed formats are not supported. Shaders read and write them with the
+   reference's texel conversions (cuda_device_resource.h:225-300): unorm bytes / shorts for float
+   images, half precision rounding to nearest, integer casts for int and uint images. ---- */
+#define LC_HIP_PIXEL_BYTE1 0u
+#define LC_HIP_PIXEL_BYTE2 1u
+#define LC_HIP_PIXEL_BYTE4 2u
+#define LC_HIP_PIXEL_SHORT1 3u
+#define LC_HIP_PIXEL_SHORT2 4u
+#define LC_HIP_PIXEL_SHORT4 5u
+#define LC_HIP_PIXEL_INT1 6u
+#define LC_HIP_PIXEL_INT2 7u
+#define LC_HIP_PIXEL_INT4 8u
+#define LC_HIP_PIXEL_HALF1 9u
+#define LC_HIP_PIXEL_HALF2 10u
+#define LC_HIP_PIXEL_HALF4 11u
+#define LC_HIP_PIXEL_FLOAT1 12u
+#define LC_HIP_PIXEL_FLOAT2 13u
+#define LC_HIP_PIXEL_FLOAT4 14u
+uint64_t lc_hip_texture_create(uint64_t device, uint32_t storage, uint32_t width, uint32_t height);
+void lc_hip_texture_destroy(uint64_t device, uint64_t texture);
+int lc_hip_texture_upload(uint64_t device, uint64_t stream, uint64_t texture, const void *data);
+int lc_hip_texture_download(uint64_t device, uint64_t stream, uint64_t texture, void *data);
 
 /* ---- bindless arrays: create_bindless_array / destroy_bindless_array (device_interface.h) and
    BindlessArrayUpdateCommand (command.h:617-680) for buffer slots: a slot holds {pointer at the

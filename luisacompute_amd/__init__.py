@@ -71,7 +71,7 @@ class ShaderArgument(ctypes.Structure):
     """lc_hip_shader_argument (Argument, include/luisa/runtime/rhi/argument.h:7-48)."""
     _fields_ = [("tag", ctypes.c_uint32), ("_pad", ctypes.c_uint32), ("handle", ctypes.c_uint64),
                 ("offset", ctypes.c_size_t), ("size", ctypes.c_size_t)]
-    BUFFER, UNIFORM, BINDLESS_ARRAY, ACCEL = 0, 2, 3, 4
+    BUFFER, TEXTURE, UNIFORM, BINDLESS_ARRAY, ACCEL = 0, 1, 2, 3, 4
 
 
 class BindlessModification(ctypes.Structure):
@@ -198,6 +198,10 @@ _SIGNATURES = {
     "lc_hip_procedural_build": (ctypes.c_int, [_U64, _U64, _U64, _U32, _U64, _SZ, _SZ]),
     "lc_hip_accel_query": (ctypes.c_int, [_U64, _U64, _U64, _U64, _SZ, _U64, _SZ, _U32, _U32,
                                           ctypes.POINTER(QueryPolicy)]),
+    "lc_hip_texture_create": (_U64, [_U64, _U32, _U32, _U32]),
+    "lc_hip_texture_destroy": (None, [_U64, _U64]),
+    "lc_hip_texture_upload": (ctypes.c_int, [_U64, _U64, _U64, _P]),
+    "lc_hip_texture_download": (ctypes.c_int, [_U64, _U64, _U64, _P]),
     "lc_hip_bindless_create": (_U64, [_U64, _SZ]),
     "lc_hip_bindless_destroy": (None, [_U64, _U64]),
     "lc_hip_bindless_update": (ctypes.c_int, [_U64, _U64, _U64, ctypes.POINTER(BindlessModification), _SZ]),
@@ -418,6 +422,9 @@ class Device:
     def create_accel(self, option=None):
         return Accel(self, option)
 
+    def create_image(self, storage, width, height):
+        return Image(self, storage, width, height)
+
     def create_bindless_array(self, slots):
         return BindlessArray(self, slots)
 
@@ -617,6 +624,8 @@ class _Dispatch(Command):
                 argv[i] = ShaderArgument(ShaderArgument.BUFFER, 0, a.handle, 0, 0)
             elif isinstance(a, Accel):
                 argv[i] = ShaderArgument(ShaderArgument.ACCEL, 0, a.handle, 0, 0)
+            elif isinstance(a, Image):
+                argv[i] = ShaderArgument(ShaderArgument.TEXTURE, 0, a.handle, 0, 0)
             elif isinstance(a, BindlessArray):
                 argv[i] = ShaderArgument(ShaderArgument.BINDLESS_ARRAY, 0, a.handle, 0, 0)
             else:  # uniform: numpy scalar / array or bytes
@@ -628,6 +637,61 @@ class _Dispatch(Command):
         _check(lib().lc_hip_shader_dispatch(stream.device.handle, stream.handle, self.shader.handle, argv,
                                             len(self.args), ctypes.cast(ub, ctypes.c_void_p), len(blob), x, y, z),
                "shader dispatch")
+
+
+# PixelStorage (include/luisa/runtime/rhi/pixel.h:14-35)
+PIXEL_BYTE1, PIXEL_BYTE2, PIXEL_BYTE4, PIXEL_SHORT1, PIXEL_SHORT2, PIXEL_SHORT4 = 0, 1, 2, 3, 4, 5
+PIXEL_INT1, PIXEL_INT2, PIXEL_INT4, PIXEL_HALF1, PIXEL_HALF2, PIXEL_HALF4 = 6, 7, 8, 9, 10, 11
+PIXEL_FLOAT1, PIXEL_FLOAT2, PIXEL_FLOAT4 = 12, 13, 14
+_PIXEL_DTYPES = [np.uint8, np.uint16, np.int32, np.float16, np.float32]
+
+
+class _ImageCopy(Command):
+    def __init__(self, image, array, upload):
+        self.image, self.array, self.upload = image, array, upload
+
+    def execute(self, stream):
+        fn = lib().lc_hip_texture_upload if self.upload else lib().lc_hip_texture_download
+        _check(fn(stream.device.handle, stream.handle, self.image.handle, _ptr(self.array)),
+               "image upload" if self.upload else "image download")
+
+
+class Image:
+    """2-D image of one level (Image<T>, include/luisa/runtime/image.h) in a PixelStorage; host
+    arrays are (height, width, channels) of the storage's channel type."""
+
+    def __init__(self, device, storage, width, height):
+        self.device, self.storage, self.width, self.height = device, storage, int(width), int(height)
+        self.handle = _handle(lib().lc_hip_texture_create(device.handle, storage, self.width, self.height),
+                              "image create")
+
+    def __del__(self):
+        if getattr(self, "handle", None) is not None and _lib is not None and self.device.handle is not None:
+            lib().lc_hip_texture_destroy(self.device.handle, self.handle)
+            self.handle = None
+
+    @property
+    def dtype(self):
+        return _PIXEL_DTYPES[self.storage // 3]
+
+    @property
+    def channels(self):
+        return (1, 2, 4)[self.storage % 3]
+
+    def empty_host(self):
+        return np.zeros((self.height, self.width, self.channels), self.dtype)
+
+    def copy_from(self, array):
+        a = np.ascontiguousarray(array, self.dtype)
+        if a.size != self.width * self.height * self.channels:
+            raise LCError("image upload size mismatch")
+        return _ImageCopy(self, a, True)
+
+    def copy_to(self, array):
+        if array.dtype != self.dtype or array.size != self.width * self.height * self.channels or \
+                not array.flags.c_contiguous:
+            raise LCError("image download array mismatch")
+        return _ImageCopy(self, array, False)
 
 
 class _BindlessUpdate(Command):

@@ -55,6 +55,83 @@ struct alignas(16) Accel {
     uint64_t _pad;
 };
 
+// 2-D image (Image<T>, include/luisa/runtime/image.h): linear rows of pixels of a PixelStorage
+// (BYTE1..FLOAT4 = 0..14; channel type = storage / 3, channel count 1 / 2 / 4 = storage % 3).
+// Reads and writes convert like the reference's surfaces (cuda_device_resource.h:225-300): float
+// images see bytes / shorts as unorm (x / 255, round(saturate(x) * 255) on write) and halves
+// rounded to nearest; int / uint images cast. Channels a storage lacks read as 0. Coordinates are
+// not checked.
+struct alignas(16) Image {
+    void *data;
+    uint32_t width, height;
+    uint32_t storage;
+    uint32_t _pad[3];
+};
+
+__device__ __forceinline__ uint32_t image_channels(const Image &img) { return img.storage % 3u == 2u ? 4u : img.storage % 3u + 1u; }
+
+// raw channel c of pixel (x, y) as float (unorm / half / float) or as a 32-bit integer pattern
+__device__ __forceinline__ float image_channel_float(const Image &img, uint64_t p, uint32_t c) {
+    const uint32_t kind = img.storage / 3u, n = image_channels(img);
+    if (kind == 0u) return (float)static_cast<const unsigned char *>(img.data)[p * n + c] * (1.0f / 255.0f);
+    if (kind == 1u) return (float)static_cast<const unsigned short *>(img.data)[p * n + c] * (1.0f / 65535.0f);
+    if (kind == 2u) return (float)static_cast<const int *>(img.data)[p * n + c];
+    if (kind == 3u) return (float)static_cast<const _Float16 *>(img.data)[p * n + c];
+    return static_cast<const float *>(img.data)[p * n + c];
+}
+__device__ __forceinline__ int image_channel_int(const Image &img, uint64_t p, uint32_t c) {
+    const uint32_t kind = img.storage / 3u, n = image_channels(img);
+    if (kind == 0u) return (int)static_cast<const unsigned char *>(img.data)[p * n + c];
+    if (kind == 1u) return (int)static_cast<const unsigned short *>(img.data)[p * n + c];
+    return static_cast<const int *>(img.data)[p * n + c];
+}
+
+__device__ __forceinline__ float4 image_read_float(const Image img, uint32_t x, uint32_t y) {
+    const uint64_t p = (uint64_t)y * img.width + x;
+    const uint32_t n = image_channels(img);
+    float v[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (uint32_t c = 0; c < n; c++) v[c] = image_channel_float(img, p, c);
+    return make_float4(v[0], v[1], v[2], v[3]);
+}
+__device__ __forceinline__ uint4 image_read_uint(const Image img, uint32_t x, uint32_t y) {
+    const uint64_t p = (uint64_t)y * img.width + x;
+    const uint32_t n = image_channels(img);
+    uint32_t v[4] = {0u, 0u, 0u, 0u};
+    for (uint32_t c = 0; c < n; c++) v[c] = (uint32_t)image_channel_int(img, p, c);
+    return make_uint4(v[0], v[1], v[2], v[3]);
+}
+__device__ __forceinline__ int4 image_read_int(const Image img, uint32_t x, uint32_t y) {
+    const uint4 u = image_read_uint(img, x, y);
+    return make_int4((int)u.x, (int)u.y, (int)u.z, (int)u.w);
+}
+
+__device__ __forceinline__ void image_write_float(const Image img, uint32_t x, uint32_t y, float4 value) {
+    const uint64_t p = (uint64_t)y * img.width + x;
+    const uint32_t kind = img.storage / 3u, n = image_channels(img);
+    const float v[4] = {value.x, value.y, value.z, value.w};
+    for (uint32_t c = 0; c < n; c++) {
+        const float s = fminf(fmaxf(v[c], 0.0f), 1.0f); // saturate (NaN -> 0)
+        if (kind == 0u) static_cast<unsigned char *>(img.data)[p * n + c] = (unsigned char)roundf(s * 255.0f);
+        else if (kind == 1u) static_cast<unsigned short *>(img.data)[p * n + c] = (unsigned short)roundf(s * 65535.0f);
+        else if (kind == 2u) static_cast<int *>(img.data)[p * n + c] = (int)v[c];
+        else if (kind == 3u) static_cast<_Float16 *>(img.data)[p * n + c] = (_Float16)v[c];
+        else static_cast<float *>(img.data)[p * n + c] = v[c];
+    }
+}
+__device__ __forceinline__ void image_write_uint(const Image img, uint32_t x, uint32_t y, uint4 value) {
+    const uint64_t p = (uint64_t)y * img.width + x;
+    const uint32_t kind = img.storage / 3u, n = image_channels(img);
+    const uint32_t v[4] = {value.x, value.y, value.z, value.w};
+    for (uint32_t c = 0; c < n; c++) {
+        if (kind == 0u) static_cast<unsigned char *>(img.data)[p * n + c] = (unsigned char)v[c];
+        else if (kind == 1u) static_cast<unsigned short *>(img.data)[p * n + c] = (unsigned short)v[c];
+        else static_cast<uint32_t *>(img.data)[p * n + c] = v[c];
+    }
+}
+__device__ __forceinline__ void image_write_int(const Image img, uint32_t x, uint32_t y, int4 value) {
+    image_write_uint(img, x, y, make_uint4((uint32_t)value.x, (uint32_t)value.y, (uint32_t)value.z, (uint32_t)value.w));
+}
+
 // Bindless array of buffer slots (BindlessArray, include/luisa/runtime/bindless_array.h; the
 // CUDA backend's buffer reads cuda_device_resource.h): slot s = {pointer at the emplaced offset,
 // bytes to the buffer's end}; reads and writes are unchecked, like the reference's.

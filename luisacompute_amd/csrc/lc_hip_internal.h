@@ -371,6 +371,15 @@ struct Accel {
     DeviceAccel desc_host{}; // staging of the last descriptor upload
 };
 
+// ---- images: 2-D, linear rows, one level ----
+struct Texture {
+    Device *dev = nullptr;
+    void *ptr = nullptr;
+    uint32_t storage = 0, width = 0, height = 0, pixel_bytes = 0;
+};
+// bytes per pixel of a PixelStorage value (include/luisa/runtime/rhi/pixel.h:14-35), 0 if unsupported
+uint32_t pixel_storage_bytes(uint32_t storage);
+
 // ---- bindless arrays (buffer slots) ----
 struct Bindless {
     Device *dev = nullptr;

@@ -1655,6 +1655,72 @@ int lc_hip_timing_reset(uint64_t device) {
     return LC_HIP_OK;
 }
 
+// ---- images (2-D, one level, linear rows) ----
+} // extern "C"
+
+uint32_t lchip::pixel_storage_bytes(uint32_t storage) {
+    if (storage > LC_HIP_PIXEL_FLOAT4) return 0;
+    static const uint32_t channel_bytes[5] = {1, 2, 4, 2, 4}; // byte, short, int, half, float
+    static const uint32_t channels[3] = {1, 2, 4};
+    return channel_bytes[storage / 3] * channels[storage % 3];
+}
+
+extern "C" {
+
+uint64_t lc_hip_texture_create(uint64_t device, uint32_t storage, uint32_t width, uint32_t height) {
+    Device *dev = obj<Device>(device);
+    const uint32_t pb = pixel_storage_bytes(storage);
+    if (!dev || pb == 0 || width == 0 || height == 0 || width > 65536 || height > 65536) {
+        fail(LC_HIP_ERROR_UNSUPPORTED, "image storage or size not supported");
+        return LC_HIP_INVALID_HANDLE;
+    }
+    DeviceGuard g(dev);
+    auto t = std::make_unique<Texture>();
+    t->dev = dev;
+    t->storage = storage; t->width = width; t->height = height; t->pixel_bytes = pb;
+    const size_t bytes = (size_t)width * height * pb;
+    hipError_t e = hipMalloc(&t->ptr, bytes);
+    if (e == hipSuccess) e = hipMemset(t->ptr, 0, bytes);
+    if (e != hipSuccess) {
+        if (t->ptr) (void)hipFree(t->ptr);
+        hip_fail(e, "image allocation");
+        return LC_HIP_INVALID_HANDLE;
+    }
+    return reinterpret_cast<uint64_t>(t.release());
+}
+
+void lc_hip_texture_destroy(uint64_t device, uint64_t texture) {
+    Device *dev = obj<Device>(device);
+    Texture *t = obj<Texture>(texture);
+    if (!dev || !t) return;
+    DeviceGuard g(dev);
+    (void)hipDeviceSynchronize();
+    if (t->ptr) (void)hipFree(t->ptr);
+    delete t;
+}
+
+int lc_hip_texture_upload(uint64_t device, uint64_t stream, uint64_t texture, const void *data) {
+    Device *dev = obj<Device>(device);
+    Texture *t = obj<Texture>(texture);
+    if (!dev || !t || !data) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad image upload arguments");
+    DeviceGuard g(dev);
+    hipStream_t s = native_stream(stream);
+    // the source is consumed before the call returns, as for buffer uploads
+    LC_HIP_TRY(hipMemcpyAsync(t->ptr, data, (size_t)t->width * t->height * t->pixel_bytes, hipMemcpyHostToDevice, s));
+    LC_HIP_TRY(hipStreamSynchronize(s));
+    return LC_HIP_OK;
+}
+
+int lc_hip_texture_download(uint64_t device, uint64_t stream, uint64_t texture, void *data) {
+    Device *dev = obj<Device>(device);
+    Texture *t = obj<Texture>(texture);
+    if (!dev || !t || !data) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad image download arguments");
+    DeviceGuard g(dev);
+    hipStream_t s = native_stream(stream);
+    LC_HIP_TRY(hipMemcpyAsync(data, t->ptr, (size_t)t->width * t->height * t->pixel_bytes, hipMemcpyDeviceToHost, s));
+    return LC_HIP_OK;
+}
+
 // ---- bindless arrays (buffer slots): create_bindless_array + BindlessArrayUpdateCommand ----
 uint64_t lc_hip_bindless_create(uint64_t device, size_t slots) {
     Device *dev = obj<Device>(device);
@@ -1832,6 +1898,12 @@ int lc_hip_shader_dispatch(uint64_t device, uint64_t stream, uint64_t shader, co
             a->state_written = true; // the shader may write instances: fold them at the next build
             const uint64_t binding[2] = {reinterpret_cast<uint64_t>(a->desc), 0};
             if (!put(binding, sizeof(binding))) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "shader arguments exceed 4 KiB");
+        } else if (x.tag == LC_HIP_ARGUMENT_TEXTURE) {
+            Texture *t = obj<Texture>(x.handle);
+            if (!t) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad image argument");
+            const uint64_t binding[4] = {reinterpret_cast<uint64_t>(t->ptr), (uint64_t)t->width | ((uint64_t)t->height << 32),
+                                         (uint64_t)t->storage, 0};
+            if (!put(binding, sizeof(binding))) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "shader arguments exceed 4 KiB");
         } else if (x.tag == LC_HIP_ARGUMENT_BINDLESS_ARRAY) {
             Bindless *b = obj<Bindless>(x.handle);
             if (!b) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad bindless array argument");
@@ -1843,7 +1915,7 @@ int lc_hip_shader_dispatch(uint64_t device, uint64_t stream, uint64_t shader, co
             if (!put(static_cast<const char *>(uniforms) + x.offset, x.size))
                 return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "shader arguments exceed 4 KiB");
         } else {
-            return fail(LC_HIP_ERROR_UNSUPPORTED, "shader argument kind not supported (textures)");
+            return fail(LC_HIP_ERROR_UNSUPPORTED, "unknown shader argument kind");
         }
     }
     const uint32_t ls_kid[4] = {dx, dy, dz, 0u};

@@ -173,3 +173,34 @@ extern "C" __global__ void __launch_bounds__(128) bindless_kernel(const Params p
 '''
 
 ALL["bindless_kernel"] = BINDLESS
+
+# image reads and writes in several storages (the accumulate / hdr2ldr kernels of
+# src/tests/test_path_tracing.cpp:264-288 move pixels between FLOAT4, HALF4 and BYTE4 images)
+IMAGES = r'''
+#include "lc_hip_device.h"
+using namespace lchip;
+struct Params {
+    alignas(16) dev::Image src;   // FLOAT4
+    alignas(16) dev::Image h4;    // HALF4
+    alignas(16) dev::Image b4;    // BYTE4
+    alignas(16) dev::Image u1;    // INT1 as a uint image
+    alignas(16) dev::Image s2;    // SHORT2
+    alignas(16) dev::Buffer<float4> back;
+    alignas(16) uint4 ls_kid;
+};
+extern "C" __global__ void __launch_bounds__(64, 1) image_kernel(const Params p) {
+    if (!dev::in_dispatch(p.ls_kid)) return;
+    const uint3 id = dev::dispatch_id();
+    const float4 v = dev::image_read_float(p.src, id.x, id.y);
+    dev::image_write_float(p.h4, id.x, id.y, v);
+    dev::image_write_float(p.b4, id.x, id.y, v);
+    dev::image_write_float(p.s2, id.x, id.y, v);
+    dev::image_write_uint(p.u1, id.x, id.y, make_uint4(id.x * 7u + id.y, 0u, 0u, 0u));
+    const float4 b = dev::image_read_float(p.b4, id.x, id.y);
+    const uint4 u = dev::image_read_uint(p.u1, id.x, id.y);
+    const float4 s = dev::image_read_float(p.s2, id.x, id.y);
+    p.back[(uint64_t)id.y * p.ls_kid.x + id.x] = make_float4(b.x, b.w, (float)u.x + (float)u.y, s.y + s.z);
+}
+'''
+
+ALL["image_kernel"] = IMAGES
