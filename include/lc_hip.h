@@ -291,6 +291,20 @@ int lc_hip_shader_compile(const char *source, char *log, size_t log_capacity, si
 uint64_t lc_hip_shader_create(uint64_t device, const char *source, const char *entry, uint32_t block_x,
                               uint32_t block_y, uint32_t block_z, char *log, size_t log_capacity);
 void lc_hip_shader_destroy(uint64_t device, uint64_t shader);
+/* ShaderDispatchCommand with several dispatch sizes (3 uint32 per dispatch): launch k gets
+   ls_kid = {size k, k}. */
+int lc_hip_shader_dispatch_multiple(uint64_t device, uint64_t stream, uint64_t shader,
+                                    const lc_hip_shader_argument *arguments, size_t argument_count,
+                                    const void *uniforms, size_t uniform_size, const uint32_t *sizes, size_t count);
+/* Indirect ShaderDispatchCommand: sizes from an indirect dispatch buffer written on the device,
+   laid out as the CUDA backend's (cuda_device_resource.h:99-135): a 16-B header {count}, then
+   32-B entries {block x, y, z, 0, dispatch x, y, z, kernel id}; entries [offset, offset +
+   min(count, max_dispatch)) are launched (block sizes must be 0 or the shader's). The call waits
+   for the stream to produce the sizes. */
+int lc_hip_shader_dispatch_indirect(uint64_t device, uint64_t stream, uint64_t shader,
+                                    const lc_hip_shader_argument *arguments, size_t argument_count,
+                                    const void *uniforms, size_t uniform_size, uint64_t indirect_buffer,
+                                    uint32_t offset, uint32_t max_dispatch);
 int lc_hip_shader_dispatch(uint64_t device, uint64_t stream, uint64_t shader, const lc_hip_shader_argument *arguments,
                            size_t argument_count, const void *uniforms, size_t uniform_size, uint32_t dispatch_x,
                            uint32_t dispatch_y, uint32_t dispatch_z);

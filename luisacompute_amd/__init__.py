@@ -208,6 +208,10 @@ _SIGNATURES = {
     "lc_hip_shader_compile": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, _SZ, ctypes.POINTER(_SZ)]),
     "lc_hip_shader_create": (_U64, [_U64, ctypes.c_char_p, ctypes.c_char_p, _U32, _U32, _U32, ctypes.c_char_p, _SZ]),
     "lc_hip_shader_destroy": (None, [_U64, _U64]),
+    "lc_hip_shader_dispatch_multiple": (ctypes.c_int, [_U64, _U64, _U64, ctypes.POINTER(ShaderArgument), _SZ, _P, _SZ,
+                                                       _P, _SZ]),
+    "lc_hip_shader_dispatch_indirect": (ctypes.c_int, [_U64, _U64, _U64, ctypes.POINTER(ShaderArgument), _SZ, _P, _SZ,
+                                                       _U64, _U32, _U32]),
     "lc_hip_shader_dispatch": (ctypes.c_int, [_U64, _U64, _U64, ctypes.POINTER(ShaderArgument), _SZ, _P, _SZ, _U32,
                                               _U32, _U32]),
     "lc_hip_accel_create": (_U64, [_U64, ctypes.POINTER(AccelOption)]),
@@ -611,8 +615,9 @@ def compile_shader_source(source):
 
 
 class _Dispatch(Command):
-    def __init__(self, shader, args, size):
+    def __init__(self, shader, args, size, multiple=None, indirect=None):
         self.shader, self.args, self.size = shader, args, size
+        self.multiple, self.indirect = multiple, indirect
 
     def execute(self, stream):
         argv = (ShaderArgument * max(1, len(self.args)))()
@@ -633,10 +638,17 @@ class _Dispatch(Command):
                 argv[i] = ShaderArgument(ShaderArgument.UNIFORM, 0, 0, len(blob), len(b))
                 blob += b
         ub = (ctypes.c_ubyte * max(1, len(blob))).from_buffer_copy(bytes(blob) or b"\0")
-        x, y, z = (tuple(self.size) + (1, 1))[:3]
-        _check(lib().lc_hip_shader_dispatch(stream.device.handle, stream.handle, self.shader.handle, argv,
-                                            len(self.args), ctypes.cast(ub, ctypes.c_void_p), len(blob), x, y, z),
-               "shader dispatch")
+        common = (stream.device.handle, stream.handle, self.shader.handle, argv, len(self.args),
+                  ctypes.cast(ub, ctypes.c_void_p), len(blob))
+        if self.multiple is not None:
+            sizes = np.ascontiguousarray([(tuple(s) + (1, 1))[:3] for s in self.multiple], np.uint32).reshape(-1)
+            _check(lib().lc_hip_shader_dispatch_multiple(*common, _ptr(sizes), len(self.multiple)), "shader dispatch")
+        elif self.indirect is not None:
+            buf, offset, max_dispatch = self.indirect
+            _check(lib().lc_hip_shader_dispatch_indirect(*common, buf.handle, offset, max_dispatch), "shader dispatch")
+        else:
+            x, y, z = (tuple(self.size) + (1, 1))[:3]
+            _check(lib().lc_hip_shader_dispatch(*common, x, y, z), "shader dispatch")
 
 
 # PixelStorage (include/luisa/runtime/rhi/pixel.h:14-35)
@@ -758,6 +770,14 @@ class Shader:
         class _Bound:
             def dispatch(self, *size):
                 return _Dispatch(shader, args, size)
+
+            def dispatch_multiple(self, sizes):
+                """One launch per size (kernel ids 0..n-1)."""
+                return _Dispatch(shader, args, None, multiple=list(sizes))
+
+            def dispatch_indirect(self, indirect_buffer, offset=0, max_dispatch=0xFFFFFFFF):
+                """Sizes from an indirect dispatch buffer written on the device."""
+                return _Dispatch(shader, args, None, indirect=(indirect_buffer, offset, max_dispatch))
         return _Bound()
 
 

@@ -152,6 +152,21 @@ __device__ __forceinline__ uint64_t bindless_buffer_size(const BindlessArray a, 
     return a.slots[slot].size_bytes / sizeof(T);
 }
 
+// Indirect dispatch buffer (IndirectDispatchBuffer; the CUDA backend's LCIndirectBuffer layout,
+// cuda_device_resource.h:99-135): a 16-B header {count}, then 32-B entries {block size, 0,
+// dispatch size, kernel id}, filled by a shader and launched by lc_hip_shader_dispatch_indirect.
+__device__ __forceinline__ void indirect_set_dispatch_count(const Buffer<uint32_t> b, uint32_t count) {
+    b.ptr[0] = count;
+}
+__device__ __forceinline__ void indirect_set_dispatch_kernel(const Buffer<uint32_t> b, uint32_t index, uint3 block,
+                                                             uint3 size, uint32_t kernel_id) {
+    uint32_t *e = b.ptr + 4 + 8 * (uint64_t)index;
+    e[0] = block.x; e[1] = block.y; e[2] = block.z; e[3] = 0u;
+    e[4] = size.x; e[5] = size.y; e[6] = size.z; e[7] = kernel_id;
+}
+// kernel id of a launch of a multiple or indirect dispatch (ls_kid.w)
+__device__ __forceinline__ uint32_t kernel_id(uint4 ls_kid) { return ls_kid.w; }
+
 // dispatch_id / dispatch_size of a 1-3D dispatch (ls_kid.xyz = dispatch size)
 __device__ __forceinline__ uint3 dispatch_id() {
     return make_uint3(blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y * blockDim.y + threadIdx.y,

@@ -1854,9 +1854,12 @@ void lc_hip_shader_destroy(uint64_t device, uint64_t shader) {
     delete sh;
 }
 
-int lc_hip_shader_dispatch(uint64_t device, uint64_t stream, uint64_t shader, const lc_hip_shader_argument *args,
-                           size_t argc, const void *uniforms, size_t uniform_size, uint32_t dx, uint32_t dy,
-                           uint32_t dz) {
+// Packs the arguments (16-B aligned, in order), then launches once per dispatch size with
+// ls_kid = {size, kernel id} appended; sizes: 3 per dispatch (ids 0..count-1) or, for indirect
+// dispatches, read from the device.
+static int shader_launch(uint64_t device, uint64_t stream, uint64_t shader, const lc_hip_shader_argument *args,
+                         size_t argc, const void *uniforms, size_t uniform_size, const uint32_t *sizes,
+                         size_t count, const uint32_t *kernel_ids) {
     Device *dev = obj<Device>(device);
     Shader *sh = obj<Shader>(shader);
     if (!dev || !sh) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad shader dispatch handles");
@@ -1918,23 +1921,69 @@ int lc_hip_shader_dispatch(uint64_t device, uint64_t stream, uint64_t shader, co
             return fail(LC_HIP_ERROR_UNSUPPORTED, "unknown shader argument kind");
         }
     }
-    const uint32_t ls_kid[4] = {dx, dy, dz, 0u};
-    if (!put(ls_kid, sizeof(ls_kid))) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "shader arguments exceed 4 KiB");
-    if ((uint64_t)dx * dy * dz == 0) return LC_HIP_OK;
+    const size_t args_end = off;
     DeviceGuard g(dev);
     hipStream_t s = native_stream(stream);
-    const uint32_t gx = (dx + sh->block[0] - 1) / sh->block[0];
-    const uint32_t gy = (dy + sh->block[1] - 1) / sh->block[1];
-    const uint32_t gz = (dz + sh->block[2] - 1) / sh->block[2];
-    size_t size = off;
-    void *extra[] = {HIP_LAUNCH_PARAM_BUFFER_POINTER, buf, HIP_LAUNCH_PARAM_BUFFER_SIZE, &size, HIP_LAUNCH_PARAM_END};
-    hipError_t e = hipModuleLaunchKernel(sh->fn, gx, gy, gz, sh->block[0], sh->block[1], sh->block[2], 0, s, nullptr,
-                                         extra);
-    if (e != hipSuccess) {
-        (void)hipGetLastError();
-        return hip_fail(e, "shader launch");
+    for (size_t k = 0; k < count; k++) {
+        const uint32_t dx = sizes[3 * k], dy = sizes[3 * k + 1], dz = sizes[3 * k + 2];
+        off = args_end;
+        const uint32_t ls_kid[4] = {dx, dy, dz, kernel_ids ? kernel_ids[k] : (uint32_t)k};
+        if (!put(ls_kid, sizeof(ls_kid))) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "shader arguments exceed 4 KiB");
+        if ((uint64_t)dx * dy * dz == 0) continue;
+        const uint32_t gx = (dx + sh->block[0] - 1) / sh->block[0];
+        const uint32_t gy = (dy + sh->block[1] - 1) / sh->block[1];
+        const uint32_t gz = (dz + sh->block[2] - 1) / sh->block[2];
+        size_t size = off;
+        void *extra[] = {HIP_LAUNCH_PARAM_BUFFER_POINTER, buf, HIP_LAUNCH_PARAM_BUFFER_SIZE, &size, HIP_LAUNCH_PARAM_END};
+        hipError_t e = hipModuleLaunchKernel(sh->fn, gx, gy, gz, sh->block[0], sh->block[1], sh->block[2], 0, s,
+                                             nullptr, extra);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            return hip_fail(e, "shader launch");
+        }
     }
     return LC_HIP_OK;
+}
+
+int lc_hip_shader_dispatch(uint64_t device, uint64_t stream, uint64_t shader, const lc_hip_shader_argument *args,
+                           size_t argc, const void *uniforms, size_t uniform_size, uint32_t dx, uint32_t dy,
+                           uint32_t dz) {
+    const uint32_t size[3] = {dx, dy, dz};
+    return shader_launch(device, stream, shader, args, argc, uniforms, uniform_size, size, 1, nullptr);
+}
+
+int lc_hip_shader_dispatch_multiple(uint64_t device, uint64_t stream, uint64_t shader, const lc_hip_shader_argument *args,
+                                    size_t argc, const void *uniforms, size_t uniform_size, const uint32_t *sizes,
+                                    size_t count) {
+    if (count > 0 && !sizes) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "null dispatch sizes");
+    return shader_launch(device, stream, shader, args, argc, uniforms, uniform_size, sizes, count, nullptr);
+}
+
+int lc_hip_shader_dispatch_indirect(uint64_t device, uint64_t stream, uint64_t shader, const lc_hip_shader_argument *args,
+                                    size_t argc, const void *uniforms, size_t uniform_size, uint64_t indirect_buffer,
+                                    uint32_t offset, uint32_t max_dispatch) {
+    Device *dev = obj<Device>(device);
+    Shader *sh = obj<Shader>(shader);
+    Buffer *ib = obj<Buffer>(indirect_buffer);
+    if (!dev || !sh || !ib) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad indirect dispatch handles");
+    if (ib->bytes < 16) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "indirect buffer without a header");
+    const size_t capacity = (ib->bytes - 16) / 32;
+    // the sizes are produced on the device: wait for them (the stream's earlier work included)
+    DeviceGuard g(dev);
+    hipStream_t s = native_stream(stream);
+    std::vector<uint32_t> host((size_t)4 + 8 * capacity);
+    LC_HIP_TRY(hipMemcpyAsync(host.data(), ib->ptr, 16 + 32 * capacity, hipMemcpyDeviceToHost, s));
+    LC_HIP_TRY(hipStreamSynchronize(s));
+    const size_t n = std::min<size_t>({(size_t)host[0], (size_t)max_dispatch, capacity > offset ? capacity - offset : 0});
+    std::vector<uint32_t> sizes(3 * n), ids(n);
+    for (size_t k = 0; k < n; k++) {
+        const uint32_t *e = &host[4 + 8 * (offset + k)]; // {block x, y, z, pad, size x, y, z, kernel id}
+        if ((e[0] || e[1] || e[2]) && (e[0] != sh->block[0] || e[1] != sh->block[1] || e[2] != sh->block[2]))
+            return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "indirect dispatch block size differs from the shader's");
+        sizes[3 * k] = e[4]; sizes[3 * k + 1] = e[5]; sizes[3 * k + 2] = e[6];
+        ids[k] = e[7];
+    }
+    return shader_launch(device, stream, shader, args, argc, uniforms, uniform_size, sizes.data(), n, ids.data());
 }
 
 } // extern "C"

@@ -204,3 +204,38 @@ extern "C" __global__ void __launch_bounds__(64, 1) image_kernel(const Params p)
 '''
 
 ALL["image_kernel"] = IMAGES
+
+# indirect dispatch: a producer fills the indirect buffer, a consumer records its kernel id and
+# dispatch id (the pattern of src/tests/test_indirect.cpp)
+INDIRECT_PRODUCER = r'''
+#include "lc_hip_device.h"
+using namespace lchip;
+struct Params {
+    alignas(16) dev::Buffer<uint32_t> indirect;
+    alignas(16) uint32_t count;
+    alignas(16) uint4 ls_kid;
+};
+extern "C" __global__ void __launch_bounds__(64) producer(const Params p) {
+    if (!dev::in_dispatch(p.ls_kid)) return;
+    const uint32_t i = dev::dispatch_id().x;
+    if (i == 0) dev::indirect_set_dispatch_count(p.indirect, p.count);
+    dev::indirect_set_dispatch_kernel(p.indirect, i, make_uint3(64, 1, 1), make_uint3(3 * i + 1, 1, 1), 10 + i);
+}
+'''
+INDIRECT_CONSUMER = r'''
+#include "lc_hip_device.h"
+using namespace lchip;
+struct Params {
+    alignas(16) dev::Buffer<uint32_t> out;
+    alignas(16) uint4 ls_kid;
+};
+extern "C" __global__ void __launch_bounds__(64) consumer(const Params p) {
+    if (!dev::in_dispatch(p.ls_kid)) return;
+    const uint32_t i = dev::dispatch_id().x;
+    const uint32_t k = dev::kernel_id(p.ls_kid);
+    p.out[64 * (k % 16) + i] = 1000u * k + i + 1u;
+}
+'''
+
+ALL["producer"] = INDIRECT_PRODUCER
+ALL["consumer"] = INDIRECT_CONSUMER

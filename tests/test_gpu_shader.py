@@ -243,3 +243,27 @@ def test_bindless_heap_shader(device, stream):
     inside = np.flatnonzero(hit & (h["inst"] == 0) & (h["prim"] < ntri))
     beyond = np.flatnonzero(hit & (h["inst"] == 0) & (h["prim"] >= ntri))
     assert len(inside) and (out[inside, 3] == ntri).all() and (out[beyond, 3] == -1).all()
+
+
+def test_multiple_and_indirect_dispatch(device, stream):
+    prod = device.create_shader(shaders.INDIRECT_PRODUCER, "producer", block=(64, 1, 1))
+    cons = device.create_shader(shaders.INDIRECT_CONSUMER, "consumer", block=(64, 1, 1))
+    out = device.create_buffer(4, 64 * 16)
+    zeros = np.zeros(64 * 16, np.uint32)
+    # multiple dispatch: launch k has kernel id k
+    stream << out.copy_from(zeros) << cons(out).dispatch_multiple([(5,), (2,), (9,)])
+    got = np.zeros(64 * 16, np.uint32)
+    stream << out.copy_to(got) << lc.synchronize()
+    want = zeros.copy()
+    for k, n in enumerate((5, 2, 9)):
+        want[64 * k:64 * k + n] = 1000 * k + np.arange(n) + 1
+    np.testing.assert_array_equal(got, want)
+    # indirect: 6 entries written on the device, entries [1, 1 + min(6, 4)) launched
+    ind = device.create_buffer(4, 4 + 8 * 8)
+    stream << out.copy_from(zeros) << prod(ind, np.uint32(6)).dispatch(6) << cons(out).dispatch_indirect(ind, 1, 4)
+    stream << out.copy_to(got) << lc.synchronize()
+    want = zeros.copy()
+    for i in range(1, 5):
+        k, n = 10 + i, 3 * i + 1
+        want[64 * (k % 16):64 * (k % 16) + n] = 1000 * k + np.arange(n) + 1
+    np.testing.assert_array_equal(got, want)
