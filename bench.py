@@ -356,9 +356,29 @@ def bench_query(a):
         dev.enable_timing(False)
         ms, n = dev.timing(name)
         res[name] = ms / max(n, 1)
+    accel.query(s, rb, hb, n_rays, surface=lc.QUERY_SURFACE_CUTOUT, cutout=0.8,
+                procedural=lc.QUERY_PROCEDURAL_SPHERE, terminate_on_first=False)
     out = np.zeros(n_rays, lc.COMMITTED_HIT_DTYPE)
     s << hb.copy_to(out) << lc.synchronize()
     kinds = np.bincount(out["hit_type"].astype(np.int64), minlength=3)
+    # the same query as a run-time shader (hiprtc, one query per thread, callbacks in the shader)
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import shaders
+    sh = dev.create_shader(shaders.QUERY, "query_kernel")
+    hs = dev.create_buffer(24, n_rays)
+
+    def run_shader():
+        s << sh(rb, hs, accel, np.float32(0.8), np.uint32(0)).dispatch(n_rays)
+    run_shader()
+    s.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        run_shader()
+    s.synchronize()
+    shader_ms = (time.perf_counter() - t0) * 1e3 / a.steps
+    out_sh = np.zeros(n_rays, lc.COMMITTED_HIT_DTYPE)
+    s << hs.copy_to(out_sh) << lc.synchronize()
+    shader_same = bool((out_sh.view(np.uint32) == out.view(np.uint32)).all())
     # the same rays as plain closest hits (procedural instance unseen, every triangle committed)
     hc = dev.create_buffer(24, n_rays)
     dev.reset_timing()
@@ -378,6 +398,9 @@ def bench_query(a):
         "trace_closest_same_rays_ms": round(tms, 3),
         "trace_closest_same_rays_mrays_s": round(n_rays / (tms * 1e-3) / 1e6, 2),
         "committed": {"miss": int(kinds[0]), "surface": int(kinds[1]), "procedural": int(kinds[2])},
+        "shader_query_ms": round(shader_ms, 3),
+        "shader_query_mrays_s": round(n_rays / (shader_ms * 1e-3) / 1e6, 2),
+        "shader_query_same_answers": shader_same,
         "config": {"workload": f"non-opaque soup {n_tris:,} tris (U[0,100]^3, sigma 0.5) with the rounded cutout "
                                f"(0.8) + {n_boxes:,} AABBs (half extent 0.2) with inscribed spheres; "
                                f"{n_rays:,} rays origin U[0,100]^3 dir U(S^2)"},
