@@ -19,8 +19,9 @@
 // of the visiting order; ties at equal t go to the smaller (instance, primitive) pair, as in
 // trace_closest.
 //
-// The walk is one ray per lane without the wave-uniform speculation of the trace walker: a
-// lane finishes a node's leaf records before it visits the next node. It is resumable
+// The walk is one ray per lane. Nearest-commit queries take wave-uniform steps (leaf records or
+// nodes, as the closest-hit walker of traverse.h); first-commit queries let each lane finish a
+// node's leaf records before its next node (a commit ends them). The walk is resumable
 // (QueryWalker::run advances a bounded number of steps), so persistent kernels refill the lanes
 // whose query finished, as the trace kernel does. Procedural BLASes hold one
 // ProcRecord per leaf slot; the callback receives the object-space ray (and the world-space one),
@@ -147,7 +148,18 @@ struct QueryWalker {
 
     __device__ __forceinline__ bool run(const QueryGeometry &Q, const Policy &pol, int steps) {
         for (int it = 0; it < steps; it++) {
-            if (t_mask != 0u) {
+            bool leaf_step = t_mask != 0u;
+            if (!first) {
+                // nearest-commit queries take wave-uniform steps like the closest-hit walker
+                // (traverse.h): leaf records when at least half of the live lanes hold some,
+                // otherwise nodes (lanes holding records visit a node too and push them)
+                if (t_mask == 0u && (g_mask & 0xFFu) == 0u && S.sp == 0) return true;
+                const uint32_t n_live = (uint32_t)__popcll(__ballot(1));
+                const uint32_t n_leaf = (uint32_t)__popcll(__ballot(t_mask != 0u));
+                leaf_step = 2u * n_leaf >= n_live;
+                if (leaf_step && t_mask == 0u) continue;
+            }
+            if (leaf_step) {
                 const uint32_t b = (uint32_t)__builtin_ctz(t_mask);
                 t_mask &= t_mask - 1u;
                 if (!in_blas) {
@@ -184,6 +196,7 @@ struct QueryWalker {
                 continue;
             }
             if ((g_mask & 0xFFu) == 0u) {
+                if (t_mask != 0u) continue; // holds leaf records and has no node to visit
                 if (in_blas && S.sp == sp_inst) {
                     if (!Q.two) return true;
                     // leave the instance: back to the world-space ray and the TLAS
@@ -199,10 +212,15 @@ struct QueryWalker {
                     t_mask = e.y & ~trav::kInstGroup;
                     continue;
                 }
+                if (e.y & trav::kTriGroup) {
+                    t_base = e.x;
+                    t_mask = e.y & ~trav::kTriGroup;
+                    continue;
+                }
                 g_base = e.x;
                 g_mask = e.y;
             }
-            // visit the next node of the current group; its leaf records come before its children
+            // visit the next node of the current group
             uint32_t hm = g_mask & 0xFFu;
             const uint32_t im = (g_mask >> 8) & 0xFFu;
             const uint32_t i = (uint32_t)__builtin_ctz(hm);
@@ -225,8 +243,12 @@ struct QueryWalker {
                 const uint32_t m = ((s < 4 ? w1.z : w1.w) >> (8 * (s & 3))) & 0xFFu;
                 nt |= ((1u << (m >> 5)) - 1u) << (m & 31u);
             }
-            t_base = w1.y;
-            t_mask = nt;
+            if (nt != 0u) {
+                // records still held go to the stack (instances in the TLAS, leaf records in a BLAS)
+                if (t_mask != 0u) S.push(make_uint2(t_base, t_mask | (in_blas ? trav::kTriGroup : trav::kInstGroup)));
+                t_base = w1.y;
+                t_mask = nt;
+            }
         }
         return false;
     }

@@ -1237,8 +1237,8 @@ int lc_hip_accel_trace_count(uint64_t device, uint64_t stream, uint64_t accel, u
 
 // Ray queries (accel.traverse / traverse_any, include/luisa/dsl/rtx/ray_query.h:11-243) over a
 // batch with the built-in candidate callbacks of query.hip. Unlike plain traces they see the
-// procedural instances too; stack entries: per wide level one node group (the walk finishes a
-// node's leaf records before its children), plus the TLAS leaf entries.
+// procedural instances too; stack entries: per wide level one node group and one held group of
+// leaf records (wave-uniform steps), plus the TLAS leaf entries.
 int lc_hip_accel_query(uint64_t device, uint64_t stream, uint64_t accel, uint64_t ray_buffer, size_t ray_offset,
                        uint64_t hit_buffer, size_t hit_offset, uint32_t ray_count, uint32_t vis_mask,
                        const lc_hip_query_policy *policy) {
@@ -1268,7 +1268,7 @@ int lc_hip_accel_query(uint64_t device, uint64_t stream, uint64_t accel, uint64_
         const InstanceRecord &r = a->host_records[0];
         P.geo.instances = a->records;
         P.geo.empty = r.blas_nodes == 0 ? 1u : 0u;
-        stack_need = r.blas_depth + 2;
+        stack_need = 2 * (r.blas_depth + 2); // a node group and a held leaf group per level
     } else {
         int rc = accel_tlas_sync(a);
         if (rc != LC_HIP_OK) return rc;
@@ -1276,7 +1276,7 @@ int lc_hip_accel_query(uint64_t device, uint64_t stream, uint64_t accel, uint64_
         P.geo.tlas_nodes = a->tlas_nodes;
         P.geo.tlas_prims = a->tlas_prims;
         P.geo.instances = a->records;
-        stack_need = a->tlas_stats.depth + 2 + a->max_blas_depth + 2;
+        stack_need = 2 * (a->tlas_stats.depth + 2 + a->max_blas_depth + 2);
     }
     if (ray_count > 0xF0000000u) return fail(LC_HIP_ERROR_UNSUPPORTED, "more than 0xF0000000 rays in one query");
     P.rays = reinterpret_cast<const float4 *>(static_cast<char *>(rb->ptr) + ray_offset);
@@ -1891,11 +1891,11 @@ static int shader_launch(uint64_t device, uint64_t stream, uint64_t shader, cons
             if (!a->built || !a->desc) return fail(LC_HIP_ERROR_NOT_BUILT, "accel argument is not built");
             uint32_t need = 0;
             if (a->inst.size() == 1) {
-                need = a->host_records[0].blas_depth + 2;
+                need = 2 * (a->host_records[0].blas_depth + 2);
             } else if (a->inst.size() > 1) {
                 const int rc = accel_tlas_sync(a);
                 if (rc != LC_HIP_OK) return rc;
-                need = a->tlas_stats.depth + 2 + a->max_blas_depth + 2;
+                need = 2 * (a->tlas_stats.depth + 2 + a->max_blas_depth + 2);
             }
             if (need > 64) return fail(LC_HIP_ERROR_UNSUPPORTED, "accel too deep for a shader's traversal stack (64)");
             a->state_written = true; // the shader may write instances: fold them at the next build
