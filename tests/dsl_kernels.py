@@ -1,0 +1,47 @@
+"""Kernels written in the Python DSL (luisacompute_amd.dsl) for the DSL tests."""
+from luisacompute_amd import dsl
+from luisacompute_amd.dsl import Accel, Buffer, float_, uint
+
+
+@dsl.kernel
+def trace_kernel(rays: Buffer("Ray"), hits: Buffer("SurfaceHit"), occluded: Buffer("uint"), accel: Accel,
+                 mask: uint):
+    i = dispatch_id().x
+    r = rays.read(i)
+    hits.write(i, accel.trace_closest(r, mask))
+    occluded.write(i, 1 if accel.trace_any(r, mask) else 0)
+
+
+@dsl.kernel(block=(64, 1, 1))
+def control_kernel(xs: Buffer("float"), out: Buffer("float"), n_terms: uint, scale: float_):
+    """Loops, branches and math: out[i] = scale * sum_{k < n_terms, k != 3} f(x, k)."""
+    i = dispatch_id().x
+    x = xs.read(i)
+    acc = 0.0
+    for k in range(n_terms):
+        if k == 3:
+            continue
+        term = x * k if x > 0.5 else x - k
+        if k > 6 and term < -5.0:
+            break
+        acc += term
+    j = 0
+    while j < 2:
+        acc = acc * 0.5 + abs(x)
+        j += 1
+    out.write(i, scale * acc)
+
+
+@dsl.kernel
+def shade_kernel(rays: Buffer("Ray"), points: Buffer("float4"), accel: Accel):
+    """Hit point from the ray and t, an offset origin and a shadow ray toward +z."""
+    i = dispatch_id().x
+    r = rays.read(i)
+    h = accel.trace_closest(r, 255)
+    if hit_miss(h):
+        points.write(i, float4(0.0, 0.0, 0.0, -1.0))
+        return
+    p = ray_origin(r) + ray_direction(r) * h.committed_ray_t
+    q = offset_ray_origin(p, normalize(ray_direction(r)) * -1.0)
+    blocked = accel.trace_any(make_ray(q, float3(0.0, 0.0, 1.0), 0.0, 100.0), 255)
+    points.write(i, float4(q.x, q.y, q.z, 1.0 if blocked else 0.0))

@@ -1,0 +1,89 @@
+"""GPU tests of Python-DSL kernels (AST -> HIP source -> hiprtc): answers equal the library's
+entry points and a float32 restatement of the same program."""
+import numpy as np
+import pytest
+
+import luisacompute_amd as lc
+from luisacompute_amd import host
+from oracle import binding as ob
+from parity import random_rigid
+import dsl_kernels
+
+pytestmark = pytest.mark.gpu
+f32 = np.float32
+
+
+def _scene(device, stream, seed):
+    rng = np.random.default_rng(seed)
+    meshes = [lc.icosphere(3), lc.soup(1500, initseq=21, extent=2.0, sigma=0.2)]
+    gm = [host.mesh_from_arrays(device, stream, v, t) for v, t in meshes]
+    accel = device.create_accel()
+    for i in range(5):
+        accel.emplace_back(gm[i % 2], random_rigid(rng, -3, 3, 0.5, 1.5), visibility_mask=1 << (i % 3))
+    stream << accel.build()
+    return accel, gm
+
+
+def test_dsl_trace_kernel(device, stream):
+    accel, keep = _scene(device, stream, 70)
+    rays = lc.rays_as_floats(lc.rays_box_sphere(30000, initseq=71, lo=-4.0, hi=4.0))
+    n = rays.shape[0]
+    k = dsl_kernels.trace_kernel.compile(device)
+    rb = host.upload(device, stream, host.as_rays(rays))
+    hb, ob_ = device.create_buffer(24, n), device.create_buffer(4, n)
+    for mask in (0xFF, 0x02):
+        stream << k(rb, hb, ob_, accel, mask).dispatch(n)
+        hits, occ = np.zeros(n, lc.HIT_DTYPE), np.zeros(n, np.uint32)
+        stream << hb.copy_to(hits) << ob_.copy_to(occ) << lc.synchronize()
+        g = host.trace_closest(device, stream, accel, rays, vis_mask=mask)
+        np.testing.assert_array_equal(hits.view(np.uint32).reshape(-1, 6)[:, :5], g.view(np.uint32).reshape(-1, 6)[:, :5])
+        np.testing.assert_array_equal(occ, host.trace_any(device, stream, accel, rays, vis_mask=mask))
+        assert (g["inst"] != 0xFFFFFFFF).sum() > 300
+
+
+def test_dsl_control_flow_kernel(device, stream):
+    rng = np.random.default_rng(72)
+    xs = rng.uniform(-1.0, 2.0, 5000).astype(f32)
+    n_terms, scale = 11, f32(0.75)
+    k = dsl_kernels.control_kernel.compile(device)
+    xb, outb = host.upload(device, stream, xs), device.create_buffer(4, len(xs))
+    stream << k(xb, outb, n_terms, scale).dispatch(len(xs))
+    out = np.zeros(len(xs), f32)
+    stream << outb.copy_to(out) << lc.synchronize()
+    want = np.zeros(len(xs), f32)
+    for i, x in enumerate(xs):
+        acc = f32(0.0)
+        for kk in range(n_terms):
+            if kk == 3:
+                continue
+            term = f32(x * f32(kk)) if x > 0.5 else f32(x - f32(kk))
+            if kk > 6 and term < f32(-5.0):
+                break
+            acc = f32(acc + term)
+        for _ in range(2):
+            acc = f32(f32(acc * f32(0.5)) + abs(x))
+        want[i] = f32(scale * acc)
+    np.testing.assert_array_equal(out, want)
+
+
+def test_dsl_shade_kernel(device, stream):
+    accel, keep = _scene(device, stream, 73)
+    rays = lc.rays_as_floats(lc.rays_box_sphere(20000, initseq=74, lo=-4.0, hi=4.0))
+    n = rays.shape[0]
+    k = dsl_kernels.shade_kernel.compile(device)
+    rb, pb = host.upload(device, stream, host.as_rays(rays)), device.create_buffer(16, n)
+    stream << k(rb, pb, accel).dispatch(n)
+    pts = np.zeros((n, 4), f32)
+    stream << pb.copy_to(pts) << lc.synchronize()
+    g = host.trace_closest(device, stream, accel, rays)
+    hit = g["inst"] != 0xFFFFFFFF
+    assert (pts[~hit, 3] == -1).all() and hit.sum() > 2000
+    o, d, t = rays[hit, :3], rays[hit, 4:7], g["committed_ray_t"][hit]
+    p = o + d * t[:, None]
+    l = np.sqrt((d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2])
+    nrm = (d / l[:, None]) * f32(-1.0)
+    q = ob.offset_ray_origin(p, nrm)
+    np.testing.assert_array_equal(pts[hit, :3], q)
+    shadow = np.zeros((hit.sum(), 8), f32)
+    shadow[:, :3], shadow[:, 6], shadow[:, 7] = q, 1.0, 100.0
+    np.testing.assert_array_equal(pts[hit, 3], host.trace_any(device, stream, accel, shadow).astype(f32))
