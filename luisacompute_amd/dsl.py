@@ -25,7 +25,12 @@ tmax]), offset_ray_origin(p, n), triangle_interpolate(bary, a, b, c), dot, cross
 sqrt, abs, min, max, floor, ceil, sin, cos, exp, log, pow, select(c, a, b), ray_origin(r),
 ray_direction(r), hit_miss(h); buffer.read(i) / buffer.write(i, v) / buffer.size();
 accel.trace_closest(ray, mask) / trace_any(ray, mask) / instance_user_id(i) /
-instance_visibility_mask(i); image.read(x, y) / image.write(x, y, v) (float images);
+instance_visibility_mask(i); accel.query_all / query_any(ray, mask, on_surface, on_procedural)
+(accel.traverse / traverse_any) with callbacks defined as nested functions of one parameter, the
+candidate: on_surface(c) gets c.inst, c.prim, c.bary, c.t and returns whether to commit;
+on_procedural(c) gets c.inst, c.prim, c.origin, c.direction (object space), c.aabb_min,
+c.aabb_max, c.t_max and returns the distance to commit (outside [t_min, t_max): none); either may
+be None; the result has inst, prim, u, v, type (0 miss, 1 surface, 2 procedural), t. image.read(x, y) / image.write(x, y, v) (float images);
 heap.buffer_read(type, slot, i) with type a string naming the element type.
 """
 import ast
@@ -100,6 +105,27 @@ __device__ __forceinline__ float3 ray_direction(const dev::Ray &r) {
 __device__ __forceinline__ float2 hit_bary(const dev::SurfaceHit &h) { return make_float2(h.bary[0], h.bary[1]); }
 __device__ __forceinline__ bool hit_miss(const dev::SurfaceHit &h) { return h.inst == 0xFFFFFFFFu; }
 template<class T> __device__ __forceinline__ T select(bool c, T a, T b) { return c ? a : b; }
+// ray-query candidates as the DSL callbacks see them
+struct SurfaceCandidate { uint32_t inst, prim; float2 bary; float t; };
+struct ProceduralCandidate { uint32_t inst, prim; float3 origin, direction, aabb_min, aabb_max; float t_max; };
+template<class F> __device__ __forceinline__ auto surface_cb(F f) {
+    return [f](uint32_t inst, uint32_t prim, float u, float v, float t) {
+        return (bool)f(SurfaceCandidate{inst, prim, make_float2(u, v), t});
+    };
+}
+template<class F> __device__ __forceinline__ auto procedural_cb(F f) {
+    return [f](uint32_t inst, uint32_t prim, const float *o, const float *d, const float *, const float *, float4 lo,
+               float4 hi, float tmax, float &t) {
+        t = (float)f(ProceduralCandidate{inst, prim, make_float3(o[0], o[1], o[2]), make_float3(d[0], d[1], d[2]),
+                                         make_float3(lo.x, lo.y, lo.z), make_float3(hi.x, hi.y, hi.z), tmax});
+        return true; // a distance outside [t_min, t_max) commits nothing
+    };
+}
+__device__ __forceinline__ auto no_surface() { return [](uint32_t, uint32_t, float, float, float) { return false; }; }
+__device__ __forceinline__ auto no_procedural() {
+    return [](uint32_t, uint32_t, const float *, const float *, const float *, const float *, float4, float4, float,
+              float &) { return false; };
+}
 } // namespace lcdsl
 '''
 
@@ -125,6 +151,7 @@ class _Codegen(ast.NodeVisitor):
         self.lines = []
         self.indent = 1
         self.scopes = [set(params)]
+        self.in_callback = 0
 
     def emit(self, line):
         self.lines.append("    " * self.indent + line)
@@ -208,9 +235,28 @@ class _Codegen(ast.NodeVisitor):
         self.emit("continue;")
 
     def visit_Return(self, node):
+        if self.in_callback:
+            if node.value is None:
+                self.error(node, "callbacks return a value")
+            self.emit(f"return {self.expr(node.value)};")
+            return
         if node.value is not None:
             self.error(node, "kernels return nothing")
         self.emit("return;")
+
+    def visit_FunctionDef(self, node):
+        # a ray-query callback: one parameter (the candidate), printed as a lambda
+        if len(node.args.args) != 1 or node.decorator_list:
+            self.error(node, "nested functions are ray-query callbacks of one parameter (the candidate)")
+        c = node.args.args[0].arg
+        self.scopes[-1].add(node.name)
+        self.emit(f"auto {node.name} = [&](const auto &{c}) {{")
+        self.in_callback += 1
+        self.scopes.append({c})
+        self.block(node.body)
+        self.scopes.pop()
+        self.in_callback -= 1
+        self.emit("};")
 
     def visit_Pass(self, node):
         pass
@@ -220,6 +266,8 @@ class _Codegen(ast.NodeVisitor):
 
     # -- expressions
     def expr(self, node):
+        if isinstance(node, ast.Constant) and node.value is None:
+            return "nullptr"
         if isinstance(node, ast.Constant):
             v = node.value
             if isinstance(v, bool):
@@ -286,6 +334,16 @@ class _Codegen(ast.NodeVisitor):
             if kind is Accel:
                 if m in ("trace_closest", "trace_any", "instance_user_id", "instance_visibility_mask"):
                     return f"lchip::dev::{m}({obj}, {', '.join(args)})"
+                if m in ("query_all", "query_any"):
+                    # accel.traverse / traverse_any (ray_query.h): (ray, mask, on_surface, on_procedural);
+                    # a callback may be None
+                    if len(node.args) != 4:
+                        self.error(node, f"{m}(ray, mask, on_surface, on_procedural)")
+                    none = lambda a: isinstance(a, ast.Constant) and a.value is None
+                    sfc = "lcdsl::no_surface()" if none(node.args[2]) else f"lcdsl::surface_cb({args[2]})"
+                    prc = "lcdsl::no_procedural()" if none(node.args[3]) else f"lcdsl::procedural_cb({args[3]})"
+                    first = "true" if m == "query_any" else "false"
+                    return f"lchip::dev::query({obj}, {args[0]}, {args[1]}, {first}, {sfc}, {prc})"
             if kind is Image:
                 if m == "read":
                     return f"lchip::dev::image_read_float({obj}, {args[0]}, {args[1]})"

@@ -45,3 +45,37 @@ def shade_kernel(rays: Buffer("Ray"), points: Buffer("float4"), accel: Accel):
     q = offset_ray_origin(p, normalize(ray_direction(r)) * -1.0)
     blocked = accel.trace_any(make_ray(q, float3(0.0, 0.0, 1.0), 0.0, 100.0), 255)
     points.write(i, float4(q.x, q.y, q.z, 1.0 if blocked else 0.0))
+
+
+@dsl.kernel
+def query_kernel(rays: Buffer("Ray"), out: Buffer("CommittedHit"), accel: Accel, cutout: float_, any_hit: uint):
+    """The callbacks of src/tests/test_procedural.cpp:86-120 written in the DSL."""
+    i = dispatch_id().x
+
+    def on_surface(c):
+        w = (1.0 - c.bary.x) - c.bary.y
+        u = c.bary.x
+        v = c.bary.y
+        return sqrt(w * w + u * u) < cutout and sqrt(u * u + v * v) < cutout and sqrt(v * v + w * w) < cutout
+
+    def on_procedural(c):
+        lo = c.aabb_min
+        hi = c.aabb_max
+        r = 0.5 * min(min(hi.x - lo.x, hi.y - lo.y), hi.z - lo.z)
+        center = float3((lo.x + hi.x) * 0.5, (lo.y + hi.y) * 0.5, (lo.z + hi.z) * 0.5)
+        L = center - c.origin
+        d = c.direction
+        dd = dot(d, d)
+        tc = dot(L, d) / dd
+        if not (tc > 0.0):
+            return -1.0
+        q = dot(L, L) - (tc * tc) * dd
+        rr = r * r
+        if not (q <= rr):
+            return -1.0
+        return tc - sqrt((rr - q) / dd)
+
+    if any_hit != 0:
+        out.write(i, accel.query_any(rays.read(i), 255, on_surface, on_procedural))
+    else:
+        out.write(i, accel.query_all(rays.read(i), 255, on_surface, on_procedural))

@@ -87,3 +87,28 @@ def test_dsl_shade_kernel(device, stream):
     shadow = np.zeros((hit.sum(), 8), f32)
     shadow[:, :3], shadow[:, 6], shadow[:, 7] = q, 1.0, 100.0
     np.testing.assert_array_equal(pts[hit, 3], host.trace_any(device, stream, accel, shadow).astype(f32))
+
+
+@pytest.mark.parametrize("any_hit", [False, True])
+def test_dsl_query_kernel(device, stream, any_hit):
+    """Ray queries with callbacks written in the DSL equal the built-in query (nearest commit) or
+    report committed candidates (first commit)."""
+    from test_gpu_shader import _query_scene
+    accel, sc, keep = _query_scene(device, stream, seed=75)
+    rays = lc.rays_as_floats(lc.rays_box_sphere(30000, initseq=76, lo=-4.0, hi=4.0))
+    n = rays.shape[0]
+    k = dsl_kernels.query_kernel.compile(device)
+    rb, qb = host.upload(device, stream, host.as_rays(rays)), device.create_buffer(24, n)
+    stream << k(rb, qb, accel, 0.8, 1 if any_hit else 0).dispatch(n)
+    out = np.zeros(n, lc.COMMITTED_HIT_DTYPE)
+    stream << qb.copy_to(out) << lc.synchronize()
+    pol = dict(surface=lc.QUERY_SURFACE_CUTOUT, cutout=0.8, procedural=lc.QUERY_PROCEDURAL_SPHERE)
+    ref = host.query(device, stream, accel, rays, **pol)
+    assert (ref["hit_type"] == 1).sum() > 300 and (ref["hit_type"] == 2).sum() > 300
+    if not any_hit:
+        np.testing.assert_array_equal(out.view(np.uint32), ref.view(np.uint32))
+    else:
+        np.testing.assert_array_equal(out["hit_type"] != 0, ref["hit_type"] != 0)
+        for i in np.flatnonzero(out["hit_type"] != 0)[:2000]:
+            c = sc.query_candidate(rays[i], out["inst"][i], out["prim"][i], out["hit_type"][i], **pol)
+            assert c is not None and f32(c[0]) == out["committed_ray_t"][i]
