@@ -15,7 +15,8 @@ printed as HIP source directly, against the device library of run-time shaders
     stream << shader(ray_buffer, hit_buffer, accel, 0xFF).dispatch(n)
 
 Supported: parameters annotated with Buffer(element), Accel, Image, BindlessArray or a scalar type
-(uint, int, float: uniforms); assignments (the first one in a block declares with `auto`),
+(uint, int_, float_, ulong, float3_: uniforms); nested functions (helpers and ray-query callbacks,
+printed as lambdas capturing the kernel's variables by reference); assignments (the first one in a block declares with `auto`),
 augmented assignments, if / elif / else, for over range(...), while, break, continue, return,
 conditional expressions, arithmetic, comparisons, `and` / `or` / `not`, calls of the builtins below
 and attribute access (hit.inst, v.x, ...). C++ infers the variable types.
@@ -74,6 +75,8 @@ BindlessArray = _Resource("lchip::dev::BindlessArray")
 uint = _Scalar("uint32_t", np.uint32)
 int_ = _Scalar("int32_t", np.int32)
 float_ = _Scalar("float", np.float32)
+ulong = _Scalar("uint64_t", np.uint64)
+float3_ = _Scalar("float3", np.float32)  # a 3-float uniform
 
 _ELEMENT_TYPES = {"Ray": "lchip::dev::Ray", "SurfaceHit": "lchip::dev::SurfaceHit",
                   "CommittedHit": "lchip::rq::CommittedHit", "uint": "uint32_t", "int": "int32_t",
@@ -138,7 +141,7 @@ _FUNCS = {"float2": "make_float2", "float3": "make_float3", "float4": "make_floa
           "triangle_interpolate": "lcdsl::triangle_interpolate", "make_ray": "lcdsl::make_ray",
           "ray_origin": "lcdsl::ray_origin", "ray_direction": "lcdsl::ray_direction", "hit_miss": "lcdsl::hit_miss",
           "hit_bary": "lcdsl::hit_bary", "select": "lcdsl::select", "uint": "(uint32_t)", "int": "(int32_t)",
-          "float": "(float)"}
+          "float": "(float)", "ulong": "(uint64_t)", "isnan": "isnan"}
 
 _BINOPS = {ast.Add: "+", ast.Sub: "-", ast.Mult: "*", ast.Div: "/", ast.Mod: "%", ast.BitAnd: "&", ast.BitOr: "|",
            ast.BitXor: "^", ast.LShift: "<<", ast.RShift: ">>"}
@@ -146,8 +149,9 @@ _CMPOPS = {ast.Eq: "==", ast.NotEq: "!=", ast.Lt: "<", ast.LtE: "<=", ast.Gt: ">
 
 
 class _Codegen(ast.NodeVisitor):
-    def __init__(self, params):
+    def __init__(self, params, constants=None):
         self.params = params  # name -> annotation
+        self.constants = constants or {}  # numbers the kernel names from its module
         self.lines = []
         self.indent = 1
         self.scopes = [set(params)]
@@ -236,23 +240,22 @@ class _Codegen(ast.NodeVisitor):
 
     def visit_Return(self, node):
         if self.in_callback:
-            if node.value is None:
-                self.error(node, "callbacks return a value")
-            self.emit(f"return {self.expr(node.value)};")
+            self.emit("return;" if node.value is None else f"return {self.expr(node.value)};")
             return
         if node.value is not None:
             self.error(node, "kernels return nothing")
         self.emit("return;")
 
     def visit_FunctionDef(self, node):
-        # a ray-query callback: one parameter (the candidate), printed as a lambda
-        if len(node.args.args) != 1 or node.decorator_list:
-            self.error(node, "nested functions are ray-query callbacks of one parameter (the candidate)")
-        c = node.args.args[0].arg
+        # a nested function (a helper, or a ray-query callback of one parameter, the candidate):
+        # a lambda capturing the kernel's variables by reference, parameters by value
+        if node.decorator_list or node.args.vararg or node.args.kwarg or node.args.defaults:
+            self.error(node, "nested functions take plain parameters")
+        names = [a.arg for a in node.args.args]
         self.scopes[-1].add(node.name)
-        self.emit(f"auto {node.name} = [&](const auto &{c}) {{")
+        self.emit(f"auto {node.name} = [&](" + ", ".join(f"auto {n}" for n in names) + ") {")
         self.in_callback += 1
-        self.scopes.append({c})
+        self.scopes.append(set(names))
         self.block(node.body)
         self.scopes.pop()
         self.in_callback -= 1
@@ -268,12 +271,14 @@ class _Codegen(ast.NodeVisitor):
     def expr(self, node):
         if isinstance(node, ast.Constant) and node.value is None:
             return "nullptr"
+        if isinstance(node, ast.Constant) and isinstance(node.value, str):
+            return '"' + node.value + '"'  # only as the element type of a bindless read
         if isinstance(node, ast.Constant):
             v = node.value
             if isinstance(v, bool):
                 return "true" if v else "false"
             if isinstance(v, int):
-                return f"{v}u" if v > 0x7FFFFFFF else str(v)
+                return f"{v}ull" if v > 0xFFFFFFFF else (f"{v}u" if v > 0x7FFFFFFF else str(v))
             if isinstance(v, float):
                 return f"{np.float32(v)!r}".replace("np.float32(", "").rstrip(")") + "f" if np.isfinite(v) else \
                     ("FLT_MAX" if v > 0 else "-FLT_MAX")
@@ -281,6 +286,9 @@ class _Codegen(ast.NodeVisitor):
         if isinstance(node, ast.Name):
             if node.id in self.params and isinstance(self.params[node.id], (Buffer, _Resource, _Scalar)):
                 return f"lcp_.{node.id}"
+            if not self.declared(node.id) and isinstance(self.constants.get(node.id), (int, float)) and \
+                    not isinstance(self.constants.get(node.id), bool):
+                return self.expr(ast.Constant(self.constants[node.id]))
             return node.id
         if isinstance(node, ast.BinOp):
             op = _BINOPS.get(type(node.op))
@@ -318,6 +326,8 @@ class _Codegen(ast.NodeVisitor):
                 return "lchip::dev::dispatch_id()"
             if f.id == "dispatch_size":
                 return "make_uint3(lcp_.ls_kid.x, lcp_.ls_kid.y, lcp_.ls_kid.z)"
+            if self.declared(f.id) and f.id not in self.params:  # a nested function
+                return f"{f.id}({', '.join(args)})"
             name = _FUNCS.get(f.id)
             if name is None:
                 self.error(node, f"unknown function {f.id}")
@@ -374,7 +384,7 @@ class Kernel:
             if not isinstance(ann, (Buffer, _Resource, _Scalar)):
                 raise LCError(f"dsl: parameter {a.arg} needs a Buffer / Accel / Image / BindlessArray / scalar annotation")
             self.params[a.arg] = ann
-        gen = _Codegen(self.params)
+        gen = _Codegen(self.params, {k: v for k, v in fn.__globals__.items() if isinstance(v, (int, float))})
         for st in fdef.body:
             gen.visit(st)
         fields = "\n".join(f"    alignas(16) {ann.decl()} {n};" for n, ann in self.params.items())
