@@ -67,7 +67,7 @@ def bench_pathtracer(a):
     9 instances), 1920x1080 at 64 spp, max depth 8, Russian roulette from bounce 3. One step = one
     full render. Mrays/s counts every ray traced (camera, bounce and shadow rays)."""
     import luisacompute_amd as lc
-    from luisacompute_amd import autotune, workloads
+    from luisacompute_amd import autotune, host, workloads
     room = workloads.BoxRoom()
     dev = lc.Device(0)
     autotune.load_default(dev)
@@ -106,6 +106,35 @@ def bench_pathtracer(a):
     mk_ms, mc, ms_ = (None, nc, ns) if a.pt_no_mega else run(lc.PT_MEGAKERNEL, max(1, a.steps // 2), 1)
     rays = nc + ns
     paths = w * h * spp
+    # the same program written in the Python DSL (AST -> HIP source -> hiprtc; one thread per
+    # pixel): the AST-codegen megakernel, at the same image size and spp, same image
+    dsl_ms = dsl_same = None
+    if not a.pt_no_mega:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import dsl_kernels
+        heap = dev.create_bindless_array(2 * len(meshes))
+        for i, m in enumerate(meshes):
+            heap.emplace(2 * i, m.vb)
+            heap.emplace(2 * i + 1, m.tb)
+        s << heap.update()
+        xf = host.upload(dev, s, np.concatenate([accel.instance_info(i)[0] for i in range(len(meshes))]).astype(np.float32))
+        mats = np.array([list(room.albedo[i]) + [1.0 if room.emissive[i] else 0.0] for i in range(len(meshes))], np.float32)
+        mb = host.upload(dev, s, mats)
+        vec = lambda name: np.array(getattr(scene, name)[:], np.float32)
+        kdsl = dsl_kernels.path_tracer.compile(dev)
+        dimg = dev.create_buffer(16, w * h)
+        dsl_cmd = kdsl(dimg, accel, heap, xf, mb, vec("light_position"), vec("light_u"), vec("light_v"),
+                       vec("light_emission"), vec("camera_origin"), vec("camera_right"), vec("camera_up"),
+                       vec("camera_forward"), spp, 0, 8, 3, 42, 3 << 32)
+        s << dsl_cmd.dispatch(w, h)
+        s.synchronize()
+        t0 = time.perf_counter()
+        s << dsl_cmd.dispatch(w, h)
+        s.synchronize()
+        dsl_ms = (time.perf_counter() - t0) * 1e3
+        dimage = np.zeros((h, w, 4), np.float32)
+        s << dimg.copy_to(dimage) << lc.synchronize()
+        dsl_same = bool((dimage.view(np.uint32) == image.view(np.uint32)).all())
     out = {
         "metric": "Mrays/s path tracing, box room 1920x1080 64 spp depth 8",
         "value": round(rays / (wf_ms * 1e-3) / 1e6, 2), "unit": "Mrays/s", "n_gpus": 1, "steps": a.steps,
@@ -117,6 +146,9 @@ def bench_pathtracer(a):
         "megakernel_mrays_s": mk_ms and round((mc + ms_) / (mk_ms * 1e-3) / 1e6, 2),
         "wavefront_over_megakernel": mk_ms and round(mk_ms / wf_ms, 3),
         "same_ray_counts": (mc, ms_) == (nc, ns),
+        "dsl_megakernel_ms": dsl_ms and round(dsl_ms, 3),
+        "dsl_megakernel_mrays_s": dsl_ms and round(rays / (dsl_ms * 1e-3) / 1e6, 2),
+        "dsl_megakernel_same_image": dsl_same,
         "image_mean": float(image[..., :3].mean()),
         "config": {"workload": f"box room {room.triangle_count} tris, {w}x{h}, {spp} spp, depth 8, RR from 3, "
                                f"chunk_spp {a.pt_chunk or 'auto'}, sort {a.pt_sort}, fetch {a.pt_fetch}, hipGraph replay"},
