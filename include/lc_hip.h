@@ -277,7 +277,7 @@ int lc_hip_accel_query(uint64_t device, uint64_t stream, uint64_t accel, uint64_
 #define LC_HIP_ARGUMENT_BUFFER 0u  /* Argument::Tag (include/luisa/runtime/rhi/argument.h:9-15) */
 #define LC_HIP_ARGUMENT_TEXTURE 1u /* {pixel pointer, width, height, storage} (32 B) */
 #define LC_HIP_ARGUMENT_UNIFORM 2u
-#define LC_HIP_ARGUMENT_BINDLESS_ARRAY 3u /* {slot table pointer, slot count} */
+#define LC_HIP_ARGUMENT_BINDLESS_ARRAY 3u /* {buffer slots, slot count, texture slots, 0} (32 B) */
 #define LC_HIP_ARGUMENT_ACCEL 4u
 typedef struct lc_hip_shader_argument {
     uint32_t tag;
@@ -337,20 +337,26 @@ int lc_hip_texture_upload(uint64_t device, uint64_t stream, uint64_t texture, co
 int lc_hip_texture_download(uint64_t device, uint64_t stream, uint64_t texture, void *data);
 
 /* ---- bindless arrays: create_bindless_array / destroy_bindless_array (device_interface.h) and
-   BindlessArrayUpdateCommand (command.h:617-680) for buffer slots: a slot holds {pointer at the
-   buffer offset, bytes from there to the buffer's end} (lc_hip_device.h dev::BindlessArray).
-   Texture slots are not supported (LC_HIP_ERROR_UNSUPPORTED). Operation values are the
-   reference's Modification::Operation (NONE, EMPLACE, REMOVE). The update is complete when the
-   call returns. ---- */
+   BindlessArrayUpdateCommand (command.h:617-680): a buffer slot holds {pointer at the buffer
+   offset, bytes from there to the buffer's end}, a 2-D texture slot an image (lc_hip_texture_*)
+   with a sampler code (Sampler::code, include/luisa/runtime/rhi/sampler.h: filter << 2 |
+   address; filters POINT, LINEAR_POINT, LINEAR_LINEAR, ANISOTROPIC, addresses EDGE, REPEAT,
+   MIRROR, ZERO) (lc_hip_device.h dev::BindlessArray). Sampling is exact single-precision point or
+   bilinear filtering on normalized coordinates (no mip levels: the linear filters are bilinear,
+   anisotropic too). Operation values are the reference's Modification::Operation (NONE, EMPLACE,
+   REMOVE). The update is complete when the call returns. ---- */
 #define LC_HIP_BINDLESS_NONE 0u
 #define LC_HIP_BINDLESS_EMPLACE 1u
 #define LC_HIP_BINDLESS_REMOVE 2u
 typedef struct lc_hip_bindless_modification {
     size_t slot;
-    uint64_t buffer;     /* EMPLACE: buffer handle */
-    size_t offset_bytes; /* EMPLACE: byte offset into the buffer */
+    uint64_t buffer;     /* buffer EMPLACE: buffer handle */
+    size_t offset_bytes; /* buffer EMPLACE: byte offset into the buffer */
     uint32_t buffer_op;
-    uint32_t texture_op; /* must be LC_HIP_BINDLESS_NONE */
+    uint32_t texture_op; /* 2-D texture slot operation */
+    uint64_t texture;    /* texture EMPLACE: image handle */
+    uint32_t sampler;    /* texture EMPLACE: Sampler::code */
+    uint32_t _pad;
 } lc_hip_bindless_modification;
 uint64_t lc_hip_bindless_create(uint64_t device, size_t slots);
 void lc_hip_bindless_destroy(uint64_t device, uint64_t array);

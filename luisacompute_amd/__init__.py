@@ -77,8 +77,18 @@ class ShaderArgument(ctypes.Structure):
 class BindlessModification(ctypes.Structure):
     """lc_hip_bindless_modification (BindlessArrayUpdateCommand::Modification, command.h:621-680)."""
     _fields_ = [("slot", ctypes.c_size_t), ("buffer", ctypes.c_uint64), ("offset_bytes", ctypes.c_size_t),
-                ("buffer_op", ctypes.c_uint32), ("texture_op", ctypes.c_uint32)]
+                ("buffer_op", ctypes.c_uint32), ("texture_op", ctypes.c_uint32), ("texture", ctypes.c_uint64),
+                ("sampler", ctypes.c_uint32), ("_pad", ctypes.c_uint32)]
     NONE, EMPLACE, REMOVE = 0, 1, 2
+
+
+# Sampler (include/luisa/runtime/rhi/sampler.h): code = filter << 2 | address
+FILTER_POINT, FILTER_LINEAR_POINT, FILTER_LINEAR_LINEAR, FILTER_ANISOTROPIC = 0, 1, 2, 3
+ADDRESS_EDGE, ADDRESS_REPEAT, ADDRESS_MIRROR, ADDRESS_ZERO = 0, 1, 2, 3
+
+
+def sampler_code(filter, address):
+    return (filter << 2) | address
 
 
 class Modification(ctypes.Structure):
@@ -730,13 +740,32 @@ class BindlessArray:
             lib().lc_hip_bindless_destroy(self.device.handle, self.handle)
             self.handle = None
 
+    def _mod(self, slot):
+        m = self._mods.get(slot)
+        if m is None:
+            m = self._mods[slot] = BindlessModification(slot, 0, 0, 0, 0, 0, 0, 0)
+        return m
+
     def emplace(self, slot, buffer, offset_bytes=0):
-        self._mods[slot] = BindlessModification(slot, buffer.handle, offset_bytes, BindlessModification.EMPLACE, 0)
+        m = self._mod(slot)
+        m.buffer, m.offset_bytes, m.buffer_op = buffer.handle, offset_bytes, BindlessModification.EMPLACE
         self._bound[slot] = buffer  # keeps the buffer alive while bound
 
     def remove(self, slot):
-        self._mods[slot] = BindlessModification(slot, 0, 0, BindlessModification.REMOVE, 0)
+        m = self._mod(slot)
+        m.buffer, m.offset_bytes, m.buffer_op = 0, 0, BindlessModification.REMOVE
         self._bound.pop(slot, None)
+
+    def emplace_texture(self, slot, image, sampler):
+        """2-D texture slot: image with a sampler code (sampler_code(filter, address))."""
+        m = self._mod(slot)
+        m.texture, m.sampler, m.texture_op = image.handle, sampler, BindlessModification.EMPLACE
+        self._bound[("tex", slot)] = image
+
+    def remove_texture(self, slot):
+        m = self._mod(slot)
+        m.texture, m.sampler, m.texture_op = 0, 0, BindlessModification.REMOVE
+        self._bound.pop(("tex", slot), None)
 
     def update(self):
         mods = [self._mods[k] for k in sorted(self._mods)]

@@ -145,5 +145,12 @@ struct BindlessSlot {
     void *ptr;
     uint64_t size_bytes;
 };
+// A 2-D texture slot of a bindless array: the image's pixels and the slot's sampler code
+// (filter << 2 | address, include/luisa/runtime/rhi/sampler.h:7-31); data null when empty.
+struct BindlessTex2D {
+    void *data;
+    uint32_t width, height, storage, sampler;
+    uint32_t _pad[2];
+};
 
 } // namespace lchip

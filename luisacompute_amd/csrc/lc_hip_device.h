@@ -138,6 +138,8 @@ __device__ __forceinline__ void image_write_int(const Image img, uint32_t x, uin
 struct alignas(16) BindlessArray {
     const BindlessSlot *slots;
     uint64_t count;
+    const BindlessTex2D *tex;
+    uint64_t _pad;
 };
 template<class T>
 __device__ __forceinline__ T bindless_buffer_read(const BindlessArray a, uint32_t slot, uint64_t i) {
@@ -166,6 +168,62 @@ __device__ __forceinline__ void indirect_set_dispatch_kernel(const Buffer<uint32
 }
 // kernel id of a launch of a multiple or indirect dispatch (ls_kid.w)
 __device__ __forceinline__ uint32_t kernel_id(uint4 ls_kid) { return ls_kid.w; }
+
+// 2-D texture slots: reads by texel, and sampling on normalized coordinates with the slot's sampler
+// (Sampler::code = filter << 2 | address, include/luisa/runtime/rhi/sampler.h): POINT takes the
+// texel floor(uv * size); the linear filters (no mip levels: LINEAR_POINT, LINEAR_LINEAR and
+// ANISOTROPIC alike) blend the four texels around uv * size - 0.5 with exact single-precision
+// weights, rows first. Addresses: EDGE clamps, REPEAT wraps, MIRROR reflects every other period,
+// ZERO reads 0 outside. Float images only (the conversions of image_read_float).
+__device__ __forceinline__ int tex_address(int i, int n, uint32_t address) {
+    if (address == 0u) return i < 0 ? 0 : (i >= n ? n - 1 : i);
+    if (address == 1u) { const int m = i % n; return m < 0 ? m + n : m; }
+    if (address == 2u) {
+        int m = i % (2 * n);
+        if (m < 0) m += 2 * n;
+        return m < n ? m : 2 * n - 1 - m;
+    }
+    return (i < 0 || i >= n) ? -1 : i;
+}
+__device__ __forceinline__ float4 tex_texel(const BindlessTex2D &t, int x, int y) {
+    if (x < 0 || y < 0) return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const Image img{t.data, t.width, t.height, t.storage, {0u, 0u, 0u}};
+    return image_read_float(img, (uint32_t)x, (uint32_t)y);
+}
+__device__ __forceinline__ float4 bindless_tex2d_read(const BindlessArray a, uint32_t slot, uint32_t x, uint32_t y) {
+    return tex_texel(a.tex[slot], (int)x, (int)y);
+}
+__device__ __forceinline__ uint2 bindless_tex2d_size(const BindlessArray a, uint32_t slot) {
+    return make_uint2(a.tex[slot].width, a.tex[slot].height);
+}
+__device__ __forceinline__ float4 bindless_tex2d_sample(const BindlessArray a, uint32_t slot, float u, float v) {
+    const BindlessTex2D t = a.tex[slot];
+    const int w = (int)t.width, h = (int)t.height;
+    const uint32_t filter = t.sampler >> 2, address = t.sampler & 3u;
+    if (filter == 0u) {
+        const int x = tex_address((int)floorf(u * (float)w), w, address);
+        const int y = tex_address((int)floorf(v * (float)h), h, address);
+        return tex_texel(t, y < 0 ? -1 : x, y);
+    }
+    const float fx = u * (float)w - 0.5f, fy = v * (float)h - 0.5f;
+    const float x0 = floorf(fx), y0 = floorf(fy);
+    const float ax = fx - x0, ay = fy - y0;
+    const int i0 = (int)x0, j0 = (int)y0;
+    const int xa = tex_address(i0, w, address), xb = tex_address(i0 + 1, w, address);
+    const int ya = tex_address(j0, h, address), yb = tex_address(j0 + 1, h, address);
+    const float4 t00 = tex_texel(t, ya < 0 ? -1 : xa, ya), t10 = tex_texel(t, ya < 0 ? -1 : xb, ya);
+    const float4 t01 = tex_texel(t, yb < 0 ? -1 : xa, yb), t11 = tex_texel(t, yb < 0 ? -1 : xb, yb);
+    const float bx = 1.0f - ax, by = 1.0f - ay;
+    const float c00[4] = {t00.x, t00.y, t00.z, t00.w}, c10[4] = {t10.x, t10.y, t10.z, t10.w};
+    const float c01[4] = {t01.x, t01.y, t01.z, t01.w}, c11[4] = {t11.x, t11.y, t11.z, t11.w};
+    float r[4];
+    for (int c = 0; c < 4; c++) {
+        const float top = c00[c] * bx + c10[c] * ax;
+        const float bot = c01[c] * bx + c11[c] * ax;
+        r[c] = top * by + bot * ay;
+    }
+    return make_float4(r[0], r[1], r[2], r[3]);
+}
 
 // dispatch_id / dispatch_size of a 1-3D dispatch (ls_kid.xyz = dispatch size)
 __device__ __forceinline__ uint3 dispatch_id() {

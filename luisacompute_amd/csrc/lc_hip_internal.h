@@ -383,8 +383,10 @@ uint32_t pixel_storage_bytes(uint32_t storage);
 // ---- bindless arrays (buffer slots) ----
 struct Bindless {
     Device *dev = nullptr;
-    BindlessSlot *slots = nullptr;  // device table
-    std::vector<BindlessSlot> host; // the table as last uploaded
+    BindlessSlot *slots = nullptr;  // device table of buffer slots
+    BindlessTex2D *tex = nullptr;   // device table of 2-D texture slots
+    std::vector<BindlessSlot> host; // the tables as last uploaded
+    std::vector<BindlessTex2D> host_tex;
 };
 
 // ---- run-time shaders (hiprtc; shader.cpp) ----

@@ -1732,10 +1732,14 @@ uint64_t lc_hip_bindless_create(uint64_t device, size_t slots) {
     auto b = std::make_unique<Bindless>();
     b->dev = dev;
     b->host.assign(slots, BindlessSlot{nullptr, 0});
+    b->host_tex.assign(slots, BindlessTex2D{nullptr, 0, 0, 0, 0, {0, 0}});
     hipError_t e = hipMalloc(&b->slots, slots * sizeof(BindlessSlot));
     if (e == hipSuccess) e = hipMemset(b->slots, 0, slots * sizeof(BindlessSlot));
+    if (e == hipSuccess) e = hipMalloc(&b->tex, slots * sizeof(BindlessTex2D));
+    if (e == hipSuccess) e = hipMemset(b->tex, 0, slots * sizeof(BindlessTex2D));
     if (e != hipSuccess) {
         if (b->slots) (void)hipFree(b->slots);
+        if (b->tex) (void)hipFree(b->tex);
         hip_fail(e, "bindless array");
         return LC_HIP_INVALID_HANDLE;
     }
@@ -1749,6 +1753,7 @@ void lc_hip_bindless_destroy(uint64_t device, uint64_t array) {
     DeviceGuard g(dev);
     (void)hipDeviceSynchronize();
     if (b->slots) (void)hipFree(b->slots);
+    if (b->tex) (void)hipFree(b->tex);
     delete b;
 }
 
@@ -1759,11 +1764,20 @@ int lc_hip_bindless_update(uint64_t device, uint64_t stream, uint64_t array, con
     if (!dev || !b) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad bindless update handles");
     if (count > 0 && !mods) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "null bindless modifications");
     std::vector<BindlessSlot> next = b->host;
+    std::vector<BindlessTex2D> next_tex = b->host_tex;
     for (size_t i = 0; i < count; i++) {
         const lc_hip_bindless_modification &m = mods[i];
         if (m.slot >= next.size()) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "bindless slot out of range");
-        if (m.texture_op != LC_HIP_BINDLESS_NONE)
-            return fail(LC_HIP_ERROR_UNSUPPORTED, "bindless texture slots are not supported");
+        if (m.texture_op == LC_HIP_BINDLESS_EMPLACE) {
+            Texture *t = obj<Texture>(m.texture);
+            if (!t) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad bindless texture");
+            if (m.sampler > 15u) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "unknown sampler code");
+            next_tex[m.slot] = BindlessTex2D{t->ptr, t->width, t->height, t->storage, m.sampler, {0, 0}};
+        } else if (m.texture_op == LC_HIP_BINDLESS_REMOVE) {
+            next_tex[m.slot] = BindlessTex2D{nullptr, 0, 0, 0, 0, {0, 0}};
+        } else if (m.texture_op != LC_HIP_BINDLESS_NONE) {
+            return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "unknown bindless texture operation");
+        }
         if (m.buffer_op == LC_HIP_BINDLESS_EMPLACE) {
             Buffer *buf = obj<Buffer>(m.buffer);
             if (!buf) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad bindless buffer");
@@ -1778,8 +1792,10 @@ int lc_hip_bindless_update(uint64_t device, uint64_t stream, uint64_t array, con
     DeviceGuard g(dev);
     hipStream_t s = native_stream(stream);
     LC_HIP_TRY(hipMemcpyAsync(b->slots, next.data(), next.size() * sizeof(BindlessSlot), hipMemcpyHostToDevice, s));
+    LC_HIP_TRY(hipMemcpyAsync(b->tex, next_tex.data(), next_tex.size() * sizeof(BindlessTex2D), hipMemcpyHostToDevice, s));
     LC_HIP_TRY(hipStreamSynchronize(s)); // the staging is this call's
     b->host.swap(next);
+    b->host_tex.swap(next_tex);
     return LC_HIP_OK;
 }
 
@@ -1910,7 +1926,8 @@ static int shader_launch(uint64_t device, uint64_t stream, uint64_t shader, cons
         } else if (x.tag == LC_HIP_ARGUMENT_BINDLESS_ARRAY) {
             Bindless *b = obj<Bindless>(x.handle);
             if (!b) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad bindless array argument");
-            const uint64_t binding[2] = {reinterpret_cast<uint64_t>(b->slots), (uint64_t)b->host.size()};
+            const uint64_t binding[4] = {reinterpret_cast<uint64_t>(b->slots), (uint64_t)b->host.size(),
+                                         reinterpret_cast<uint64_t>(b->tex), 0};
             if (!put(binding, sizeof(binding))) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "shader arguments exceed 4 KiB");
         } else if (x.tag == LC_HIP_ARGUMENT_UNIFORM) {
             if (!uniforms || x.offset + x.size > uniform_size)

@@ -32,7 +32,8 @@ candidate: on_surface(c) gets c.inst, c.prim, c.bary, c.t and returns whether to
 on_procedural(c) gets c.inst, c.prim, c.origin, c.direction (object space), c.aabb_min,
 c.aabb_max, c.t_max and returns the distance to commit (outside [t_min, t_max): none); either may
 be None; the result has inst, prim, u, v, type (0 miss, 1 surface, 2 procedural), t. image.read(x, y) / image.write(x, y, v) (float images);
-heap.buffer_read(type, slot, i) with type a string naming the element type.
+heap.buffer_read(type, slot, i) with type a string naming the element type; heap.tex2d_sample(slot,
+uv) / tex2d_read(slot, x, y) / tex2d_size(slot).
 """
 import ast
 import inspect
@@ -359,6 +360,12 @@ class _Codegen(ast.NodeVisitor):
                     return f"lchip::dev::image_read_float({obj}, {args[0]}, {args[1]})"
                 if m == "write":
                     return f"lchip::dev::image_write_float({obj}, {args[0]}, {args[1]}, {args[2]})"
+            if kind is BindlessArray and m == "tex2d_sample":
+                return f"lchip::dev::bindless_tex2d_sample({obj}, {args[0]}, {args[1]}.x, {args[1]}.y)"
+            if kind is BindlessArray and m == "tex2d_read":
+                return f"lchip::dev::bindless_tex2d_read({obj}, {args[0]}, {args[1]}, {args[2]})"
+            if kind is BindlessArray and m == "tex2d_size":
+                return f"lchip::dev::bindless_tex2d_size({obj}, {args[0]})"
             if kind is BindlessArray and m == "buffer_read":
                 el = node.args[0]
                 if not (isinstance(el, ast.Constant) and isinstance(el.value, str)):

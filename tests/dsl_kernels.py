@@ -259,3 +259,11 @@ def path_tracer(image: Buffer("float4"), accel: Accel, heap: dsl.BindlessArray, 
         total = float3(total.x + L.x, total.y + L.y, total.z + L.z)
     fs = float(spp)
     image.write(pixel, float4(total.x / fs, total.y / fs, total.z / fs, 1.0))
+
+
+@dsl.kernel(block=(64, 1, 1))
+def sample_kernel(uv: Buffer("float2"), out: Buffer("float4"), heap: dsl.BindlessArray, n_slots: uint):
+    """Samples texture slots 0..n_slots-1 of the heap at uv[i] (out[i * n_slots + k])."""
+    i = dispatch_id().x
+    for k in range(n_slots):
+        out.write(i * n_slots + k, heap.tex2d_sample(k, uv.read(i)))

@@ -7,6 +7,7 @@ import luisacompute_amd as lc
 import shaders
 
 pytestmark = pytest.mark.gpu
+f32 = np.float32
 
 
 def _unorm_write(v, scale):
@@ -54,3 +55,77 @@ def test_image_errors(device, stream):
     img = device.create_image(lc.PIXEL_FLOAT1, 4, 4)
     with pytest.raises(lc.LCError):
         img.copy_from(np.zeros(15, np.float32))
+
+
+def _np_sample(tex, u, v, code):
+    """Exact float32 restatement of the device sampling (point / bilinear, four addresses)."""
+    h, w = tex.shape[:2]
+    filt, addr = code >> 2, code & 3
+
+    def wrap(i, n):
+        if addr == 0:
+            return np.clip(i, 0, n - 1)
+        if addr == 1:
+            return np.mod(i, n)
+        if addr == 2:
+            m = np.mod(i, 2 * n)
+            return np.where(m < n, m, 2 * n - 1 - m)
+        return np.where((i < 0) | (i >= n), -1, i)
+
+    def texel(x, y):
+        ok = (x >= 0) & (y >= 0)
+        return np.where(ok[:, None], tex[np.clip(y, 0, h - 1), np.clip(x, 0, w - 1)], f32(0))
+
+    if filt == 0:
+        x = wrap(np.floor(u * f32(w)).astype(np.int64), w)
+        y = wrap(np.floor(v * f32(h)).astype(np.int64), h)
+        return texel(np.where(y < 0, -1, x), y)
+    fx, fy = u * f32(w) - f32(0.5), v * f32(h) - f32(0.5)
+    x0, y0 = np.floor(fx), np.floor(fy)
+    ax, ay = (fx - x0)[:, None], (fy - y0)[:, None]
+    i0, j0 = x0.astype(np.int64), y0.astype(np.int64)
+    xa, xb, ya, yb = wrap(i0, w), wrap(i0 + 1, w), wrap(j0, h), wrap(j0 + 1, h)
+    t00, t10 = texel(np.where(ya < 0, -1, xa), ya), texel(np.where(ya < 0, -1, xb), ya)
+    t01, t11 = texel(np.where(yb < 0, -1, xa), yb), texel(np.where(yb < 0, -1, xb), yb)
+    bx, by = f32(1) - ax, f32(1) - ay
+    top, bot = t00 * bx + t10 * ax, t01 * bx + t11 * ax
+    return top * by + bot * ay
+
+
+def test_bindless_texture_sampling(device, stream):
+    import dsl_kernels
+    from luisacompute_amd import host
+    rng = np.random.default_rng(61)
+    w, h = 13, 7
+    tex = rng.uniform(0, 1, (h, w, 4)).astype(f32)
+    img = device.create_image(lc.PIXEL_FLOAT4, w, h)
+    stream << img.copy_from(tex)
+    codes = [lc.sampler_code(f, a) for f in range(4) for a in range(4)]
+    heap = device.create_bindless_array(len(codes))
+    for k, c in enumerate(codes):
+        heap.emplace_texture(k, img, c)
+    stream << heap.update()
+    n = 4096
+    uv = rng.uniform(-1.5, 2.5, (n, 2)).astype(f32)
+    uv[:64] = ((np.arange(64)[:, None] % np.array([w, h])) + 0.5) / np.array([w, h], f32)  # texel centres
+    k = dsl_kernels.sample_kernel.compile(device)
+    ub, ob_ = host.upload(device, stream, uv), device.create_buffer(16, n * len(codes))
+    stream << k(ub, ob_, heap, len(codes)).dispatch(n)
+    out = np.zeros((n, len(codes), 4), f32)
+    stream << ob_.copy_to(out) << lc.synchronize()
+    for j, c in enumerate(codes):
+        np.testing.assert_array_equal(out[:, j], _np_sample(tex, uv[:, 0], uv[:, 1], c), err_msg=f"sampler {c}")
+    # properties that do not restate the formula: texel centres return the texel for every filter
+    cx, cy = (np.arange(64) % w), (np.arange(64) % h)
+    for j in range(len(codes)):
+        np.testing.assert_allclose(out[:64, j], tex[cy, cx], rtol=0, atol=2e-7)
+    # zero address: outside the unit square reads 0 for point sampling
+    far = (uv[:, 0] < -0.1) | (uv[:, 0] > 1.1) | (uv[:, 1] < -0.2) | (uv[:, 1] > 1.2)
+    assert (out[far, codes.index(lc.sampler_code(0, lc.ADDRESS_ZERO))] == 0).all()
+    # repeat: u and u + 1 sample the same texel (point filter, away from texel edges)
+    j = codes.index(lc.sampler_code(0, lc.ADDRESS_REPEAT))
+    fr = np.mod(uv[:, 0] * w, 1.0)
+    inner = (fr > 0.01) & (fr < 0.99) & (np.mod(uv[:, 1] * h, 1.0) > 0.01) & (np.mod(uv[:, 1] * h, 1.0) < 0.99)
+    xs = np.floor(uv[:, 0] * w).astype(int) % w
+    ys = np.floor(uv[:, 1] * h).astype(int) % h
+    np.testing.assert_array_equal(out[inner, j], tex[ys[inner], xs[inner]])
