@@ -489,7 +489,7 @@ def main():
         return
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (rays and scenes would shard "
-              f"by replicating the BVH per GPU, see DESIGN.md)", file=sys.stderr)
+              f"by replicating the BVH per GPU)", file=sys.stderr)
         sys.exit(2)
     if a.workload == "instanced":
         bench_instanced(a)
