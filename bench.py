@@ -585,11 +585,12 @@ def main():
     # HBM bytes per launch of the traversal kernel: FETCH_SIZE + WRITE_SIZE from two rocprofv3
     # --pmc runs of this command, each scaled by what the counter reads for a 1 GiB device copy
     # (profiles/trace_closest_soup16m_traffic.md)
-    traffic = None
+    traffic = l2_hit = None
     tpath = os.path.join(ROOT, "profiles", "trace_closest_soup16m_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            tj = json.load(f)
+        traffic, l2_hit = tj.get("hbm_bytes_per_launch"), tj.get("l2_hit_rate")
 
     result = {
         "metric": "Mrays/s incoherent closest-hit @16M tris (BVH build included)",
@@ -635,6 +636,7 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
+            "l2_hit_rate": l2_hit,
             "bytes_per_ray": round(bytes_per_ray, 2),
             "compulsory_floor_gbs": round(floor_gbs, 2),
         },
