@@ -254,6 +254,13 @@ struct Device {
     uint32_t *ray_counter = nullptr;
     uint32_t *error = nullptr;
     unsigned long long *stats = nullptr;
+    // build arena of compacting BLAS builds (guarded by mu): worst-case wide nodes, and the build
+    // scratch of primitives without allow_update; free again once arena_done has completed
+    WideNode *arena_nodes = nullptr;
+    size_t arena_node_cap = 0;
+    void *arena_scratch = nullptr;
+    size_t arena_scratch_bytes = 0;
+    hipEvent_t arena_done = nullptr;
     // kernel timing with HIP events on the launch stream
     bool timing = false;
     struct Pending {
