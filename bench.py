@@ -571,11 +571,13 @@ def main():
     dev.enable_timing(False)
     refit_ms, nr = dev.timing("mesh_refit")
     refit_ms /= max(nr, 1)
+    umem = umesh.memory()
     del umesh
 
     # algorithmic bytes: counter-mode run of the same traversal on the same BVH and rays
     n_nodes, n_tris = accel.trace_count(s, rb, a.rays)
     st = mesh.stats()
+    mem = mesh.memory()
     nodes_per_ray = n_nodes / a.rays
     tris_per_ray = n_tris / a.rays
     bytes_per_ray = RAY_BYTES + HIT_BYTES + NODE_BYTES * nodes_per_ray + TRI_BYTES * tris_per_ray
@@ -624,7 +626,8 @@ def main():
         "trace_any_ms": round(any_ms, 3),
         "trace_any_mrays_s": round(a.rays / (any_ms * 1e-3) / 1e6, 2),
         "bvh": {"wide_nodes": st.node_count, "depth": st.depth, "node_bytes": st.node_bytes,
-                "tri_bytes": st.primitive_bytes},
+                "tri_bytes": st.primitive_bytes, "held_bytes": mem["total_bytes"],
+                "held_bytes_allow_update": umem["total_bytes"], "build_arena_bytes": dev.build_arena_bytes()},
         "nodes_per_ray": round(nodes_per_ray, 3),
         "tris_per_ray": round(tris_per_ray, 3),
         "roofline": {

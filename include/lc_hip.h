@@ -470,6 +470,24 @@ int lc_hip_mesh_stats(uint64_t device, uint64_t mesh, lc_hip_bvh_stats *out);
    slot box, 4 each triangle record referenced once, 5 each primitive covered once, 6 empty slots
    empty. All zero for a well-formed tree. */
 int lc_hip_mesh_validate(uint64_t device, uint64_t stream, uint64_t mesh, uint32_t errors[7]);
+/* Device memory a built primitive holds (meshes and procedural primitives). With
+   AccelOption::allow_compaction (include/luisa/runtime/rhi/resource.h:114) a build goes through the
+   device's build arena and keeps only the used wide nodes, and no build scratch unless
+   allow_update (the OptiX build compacts into a buffer that only grows,
+   src/backends/cuda/cuda_mesh.cpp:42-89); without it the primitive keeps the worst-case node
+   array and the build scratch. */
+typedef struct lc_hip_memory_usage {
+    uint64_t node_bytes;      /* wide-node array */
+    uint64_t primitive_bytes; /* leaf records (procedural: + build boxes and leaf ids) */
+    uint64_t scratch_bytes;   /* build scratch kept for refits */
+    uint64_t total_bytes;
+} lc_hip_memory_usage;
+int lc_hip_mesh_memory(uint64_t device, uint64_t mesh, lc_hip_memory_usage *out);
+/* The device's build arena (worst-case nodes, and the scratch of builds without allow_update) is
+   kept for the next compacting build; these report its size and release it. No counterpart in
+   the reference (its temporary build buffers come from a pool). */
+uint64_t lc_hip_device_build_arena_bytes(uint64_t device);
+int lc_hip_device_release_build_arena(uint64_t device);
 int lc_hip_accel_stats(uint64_t device, uint64_t accel, lc_hip_bvh_stats *out);
 /* kind 0 = closest, 1 = any. block in {64,128,256}; lds_stack in {8,16,24}; steps >= 1;
    blocks_per_cu 0 = occupancy limit. */

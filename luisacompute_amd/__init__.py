@@ -116,6 +116,12 @@ class BvhStats(ctypes.Structure):
                 ("primitive_bytes", ctypes.c_uint64)]
 
 
+class MemoryUsage(ctypes.Structure):
+    """lc_hip_memory_usage: device bytes a built primitive holds."""
+    _fields_ = [("node_bytes", ctypes.c_uint64), ("primitive_bytes", ctypes.c_uint64),
+                ("scratch_bytes", ctypes.c_uint64), ("total_bytes", ctypes.c_uint64)]
+
+
 class PtInstance(ctypes.Structure):
     """lc_hip_pt_instance: material of one accel instance (geometry comes from its mesh)."""
     _fields_ = [("albedo", ctypes.c_float * 3), ("emissive", ctypes.c_uint32)]
@@ -236,6 +242,9 @@ _SIGNATURES = {
                                                 ctypes.POINTER(_U64), ctypes.POINTER(_U64)]),
     "lc_hip_mesh_stats": (ctypes.c_int, [_U64, _U64, ctypes.POINTER(BvhStats)]),
     "lc_hip_mesh_validate": (ctypes.c_int, [_U64, _U64, _U64, ctypes.POINTER(_U32)]),
+    "lc_hip_mesh_memory": (ctypes.c_int, [_U64, _U64, ctypes.POINTER(MemoryUsage)]),
+    "lc_hip_device_build_arena_bytes": (_U64, [_U64]),
+    "lc_hip_device_release_build_arena": (ctypes.c_int, [_U64]),
     "lc_hip_accel_stats": (ctypes.c_int, [_U64, _U64, ctypes.POINTER(BvhStats)]),
     "lc_hip_trace_set_config": (ctypes.c_int, [_U64, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "lc_hip_trace_get_config": (ctypes.c_int, [_U64, ctypes.c_int] + [ctypes.POINTER(ctypes.c_int)] * 4),
@@ -405,6 +414,13 @@ class Device:
 
     def warp_size(self):
         return lib().lc_hip_device_warp_size(self.handle)
+
+    def build_arena_bytes(self):
+        """Bytes of the build arena kept for compacting BLAS builds."""
+        return lib().lc_hip_device_build_arena_bytes(self.handle)
+
+    def release_build_arena(self):
+        _check(lib().lc_hip_device_release_build_arena(self.handle), "release build arena")
 
     def query(self, key):
         out = ctypes.create_string_buffer(256)
@@ -603,6 +619,10 @@ class Mesh:
         s = BvhStats()
         _check(lib().lc_hip_mesh_stats(self.device.handle, self.handle, ctypes.byref(s)), "mesh stats")
         return s
+
+    def memory(self):
+        """Device bytes held: {node_bytes, primitive_bytes, scratch_bytes, total_bytes}."""
+        return _memory(self, "mesh memory")
 
     VALIDATION_CHECKS = ("child_index", "triangle_range", "leaf_in_slot_box", "child_in_slot_box",
                          "record_once", "primitive_once", "empty_slot")
@@ -834,6 +854,16 @@ class ProceduralPrimitive:
         s = BvhStats()
         _check(lib().lc_hip_mesh_stats(self.device.handle, self.handle, ctypes.byref(s)), "procedural stats")
         return s
+
+    def memory(self):
+        """Device bytes held: {node_bytes, primitive_bytes, scratch_bytes, total_bytes}."""
+        return _memory(self, "procedural memory")
+
+
+def _memory(primitive, what):
+    u = MemoryUsage()
+    _check(lib().lc_hip_mesh_memory(primitive.device.handle, primitive.handle, ctypes.byref(u)), what)
+    return {name: getattr(u, name) for name, _ in MemoryUsage._fields_}
 
 
 class Accel:

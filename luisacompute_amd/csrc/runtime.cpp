@@ -185,6 +185,60 @@ void copy_counters_async(BuildCounters *dctr, HostCounters *host, uint32_t *erro
     hipMemcpyAsync(&host->error, error, sizeof(uint32_t), hipMemcpyDeviceToHost, s);
 }
 
+// Compacting builds (allow_compaction). cuda_mesh.cpp:42-89 builds into a temporary buffer, reads
+// the compacted size back and copies into the primitive's own buffer, which only grows. Here the
+// tree is built into the device's arena of worst-case wide nodes (n nodes for n primitives), and,
+// for a primitive without allow_update (no refit will need the build's scratch), into the arena's
+// build scratch; the host then waits for the node count and the used nodes are copied into the
+// primitive's own array. A built primitive thus holds node_count nodes instead of n and no build
+// scratch. Called with dev->mu held.
+struct BuildTarget {
+    WideNode *nodes;
+    void *scratch;
+    size_t scratch_bytes;
+};
+
+hipError_t arena_acquire(Device *dev, Mesh *m, uint32_t n, BuildTarget &t) {
+    // the previous compaction's copy out of the arena has finished before the arena is reused
+    hipError_t e = hipEventSynchronize(dev->arena_done);
+    if (e != hipSuccess) return e;
+    void *p = dev->arena_nodes;
+    size_t cap = dev->arena_node_cap * sizeof(WideNode);
+    e = ensure(p, cap, bvh_node_capacity(n) * sizeof(WideNode));
+    dev->arena_nodes = static_cast<WideNode *>(p);
+    dev->arena_node_cap = cap / sizeof(WideNode);
+    if (e != hipSuccess) return e;
+    t.nodes = dev->arena_nodes;
+    if (m->allow_update) {
+        e = ensure(m->scratch, m->scratch_bytes, bvh_scratch_bytes(n));
+        t.scratch = m->scratch;
+        t.scratch_bytes = m->scratch_bytes;
+    } else {
+        e = ensure(dev->arena_scratch, dev->arena_scratch_bytes, bvh_scratch_bytes(n));
+        t.scratch = dev->arena_scratch;
+        t.scratch_bytes = dev->arena_scratch_bytes;
+    }
+    return e;
+}
+
+// After the build's launches and counter copies on s: waits for the counters, grows the
+// primitive's node array to the node count if needed and copies the nodes out of the arena.
+int arena_compact(Device *dev, Mesh *m, hipStream_t s) {
+    LC_HIP_TRY(hipStreamSynchronize(s));
+    const uint32_t count = m->host->node_count;
+    if (count == 0 || count > dev->arena_node_cap) return fail(LC_HIP_ERROR_DEVICE_FAULT, "BVH node count out of range");
+    if (!m->nodes || m->node_cap < count) {
+        if (m->nodes) LC_HIP_TRY(hipFree(m->nodes));
+        m->nodes = nullptr;
+        m->node_cap = 0;
+        LC_HIP_TRY(hipMalloc(&m->nodes, (size_t)count * sizeof(WideNode)));
+        m->node_cap = count;
+    }
+    LC_HIP_TRY(hipMemcpyAsync(m->nodes, dev->arena_nodes, (size_t)count * sizeof(WideNode), hipMemcpyDeviceToDevice, s));
+    LC_HIP_TRY(hipEventRecord(dev->arena_done, s));
+    return LC_HIP_OK;
+}
+
 } // namespace
 
 extern "C" {
@@ -226,6 +280,11 @@ uint64_t lc_hip_device_create(int32_t index) {
     dev->stats = reinterpret_cast<unsigned long long *>(static_cast<char *>(p) + 576);
     dev->cfg[0] = TraceConfig{256, 16, 32, 0};
     dev->cfg[1] = TraceConfig{256, 16, 32, 0};
+    if (hipEventCreateWithFlags(&dev->arena_done, hipEventDisableTiming) != hipSuccess) {
+        hipFree(dev->ray_counter);
+        fail(LC_HIP_ERROR_HIP, "device event creation failed");
+        return LC_HIP_INVALID_HANDLE;
+    }
     return handle(dev.release());
 }
 
@@ -239,6 +298,9 @@ void lc_hip_device_destroy(uint64_t device) {
     if (dev->spill) hipFree(dev->spill);
     if (dev->sort_scratch) hipFree(dev->sort_scratch);
     if (dev->ray_counter) hipFree(dev->ray_counter);
+    if (dev->arena_nodes) hipFree(dev->arena_nodes);
+    if (dev->arena_scratch) hipFree(dev->arena_scratch);
+    if (dev->arena_done) hipEventDestroy(dev->arena_done);
     delete dev;
 }
 
@@ -559,11 +621,13 @@ int lc_hip_procedural_build(uint64_t device, uint64_t stream, uint64_t primitive
     std::lock_guard<std::mutex> lk(m->mu);
     DeviceGuard g(dev);
     hipStream_t s = native_stream(stream);
-    void *p;
-    size_t cap;
-    p = m->nodes; cap = m->node_cap * sizeof(WideNode);
-    LC_HIP_TRY(ensure(p, cap, bvh_node_capacity(n) * sizeof(WideNode)));
-    m->nodes = static_cast<WideNode *>(p); m->node_cap = cap / sizeof(WideNode);
+    if (!m->allow_compaction) {
+        void *p = m->nodes;
+        size_t cap = m->node_cap * sizeof(WideNode);
+        LC_HIP_TRY(ensure(p, cap, bvh_node_capacity(n) * sizeof(WideNode)));
+        m->nodes = static_cast<WideNode *>(p); m->node_cap = cap / sizeof(WideNode);
+        LC_HIP_TRY(ensure(m->scratch, m->scratch_bytes, bvh_scratch_bytes(n)));
+    }
     const size_t pn = std::max<size_t>(n, 1);
     if (pn > m->proc_cap) {
         if (m->proc) hipFree(m->proc);
@@ -575,7 +639,6 @@ int lc_hip_procedural_build(uint64_t device, uint64_t stream, uint64_t primitive
         LC_HIP_TRY(hipMalloc(&m->proc_ids, pn * sizeof(uint32_t)));
         m->proc_cap = pn;
     }
-    LC_HIP_TRY(ensure(m->scratch, m->scratch_bytes, bvh_scratch_bytes(n)));
     m->ntris = n;
     m->built = true;
     m->stats_valid = false;
@@ -583,16 +646,22 @@ int lc_hip_procedural_build(uint64_t device, uint64_t stream, uint64_t primitive
     if (n == 0) return LC_HIP_OK;
     {
         std::lock_guard<std::mutex> dl(dev->mu);
+        BuildTarget t{m->nodes, m->scratch, m->scratch_bytes};
+        if (m->allow_compaction) LC_HIP_TRY(arena_acquire(dev, m, n, t));
         LC_HIP_TRY(hipMemsetAsync(dev->error, 0, sizeof(uint32_t), s));
         TimedRegion tr(dev, s, "procedural_build");
         const float *aabb = reinterpret_cast<const float *>(static_cast<const char *>(ab->ptr) + aabb_offset);
         LC_HIP_TRY(aabbs_to_boxes(s, aabb, n, m->proc_boxes));
         BuildCounters *ctr = nullptr;
-        hipError_t e = bvh_build(s, m->scratch, m->scratch_bytes, nullptr, 0, 0, nullptr, m->proc_boxes, n, m->nodes,
+        hipError_t e = bvh_build(s, t.scratch, t.scratch_bytes, nullptr, 0, 0, nullptr, m->proc_boxes, n, t.nodes,
                                  nullptr, m->proc_ids, &ctr, 0, 1u, dev->cu_count, dev->error, &m->host->phase_count);
         if (e != hipSuccess) return hip_fail(e, "procedural bvh_build");
         LC_HIP_TRY(proc_records(s, m->proc_ids, m->proc_boxes, n, m->proc));
         copy_counters_async(ctr, m->host, dev->error, s);
+        if (m->allow_compaction) {
+            const int rc = arena_compact(dev, m, s);
+            if (rc != LC_HIP_OK) return rc;
+        }
     }
     LC_HIP_TRY(hipEventRecord(m->ready, s));
     return LC_HIP_OK;
@@ -653,16 +722,17 @@ int lc_hip_mesh_build(uint64_t device, uint64_t stream, uint64_t mesh, uint32_t 
         LC_HIP_TRY(hipEventRecord(m->ready, s));
         return LC_HIP_OK;
     }
-    size_t ncap = bvh_node_capacity(n);
     void *p;
     size_t cap;
-    p = m->nodes; cap = m->node_cap * sizeof(WideNode);
-    LC_HIP_TRY(ensure(p, cap, ncap * sizeof(WideNode)));
-    m->nodes = static_cast<WideNode *>(p); m->node_cap = cap / sizeof(WideNode);
+    if (!m->allow_compaction) {
+        p = m->nodes; cap = m->node_cap * sizeof(WideNode);
+        LC_HIP_TRY(ensure(p, cap, bvh_node_capacity(n) * sizeof(WideNode)));
+        m->nodes = static_cast<WideNode *>(p); m->node_cap = cap / sizeof(WideNode);
+        LC_HIP_TRY(ensure(m->scratch, m->scratch_bytes, bvh_scratch_bytes(n)));
+    }
     p = m->tris; cap = m->tri_cap * sizeof(TriRecord);
     LC_HIP_TRY(ensure(p, cap, std::max<size_t>(n, 1) * sizeof(TriRecord)));
     m->tris = static_cast<TriRecord *>(p); m->tri_cap = cap / sizeof(TriRecord);
-    LC_HIP_TRY(ensure(m->scratch, m->scratch_bytes, bvh_scratch_bytes(n)));
     m->ntris = n;
     m->built = true;
     m->stats_valid = false;
@@ -670,14 +740,20 @@ int lc_hip_mesh_build(uint64_t device, uint64_t stream, uint64_t mesh, uint32_t 
     if (n == 0) return LC_HIP_OK;
     {
         std::lock_guard<std::mutex> dl(dev->mu);
+        BuildTarget t{m->nodes, m->scratch, m->scratch_bytes};
+        if (m->allow_compaction) LC_HIP_TRY(arena_acquire(dev, m, n, t));
         LC_HIP_TRY(hipMemsetAsync(dev->error, 0, sizeof(uint32_t), s));
         TimedRegion tr(dev, s, "mesh_build");
         BuildCounters *ctr = nullptr;
-        hipError_t e = bvh_build(s, m->scratch, m->scratch_bytes, vaddr, vertex_stride, nverts, taddr, nullptr, n,
-                                 m->nodes, m->tris, nullptr, &ctr, dev->split_small, leaf_size_for(m->hint),
+        hipError_t e = bvh_build(s, t.scratch, t.scratch_bytes, vaddr, vertex_stride, nverts, taddr, nullptr, n,
+                                 t.nodes, m->tris, nullptr, &ctr, dev->split_small, leaf_size_for(m->hint),
                                  dev->cu_count, dev->error, &m->host->phase_count);
         if (e != hipSuccess) return hip_fail(e, "bvh_build");
         copy_counters_async(ctr, m->host, dev->error, s);
+        if (m->allow_compaction) {
+            const int rc = arena_compact(dev, m, s);
+            if (rc != LC_HIP_OK) return rc;
+        }
     }
     LC_HIP_TRY(hipEventRecord(m->ready, s));
     return LC_HIP_OK;
@@ -697,6 +773,41 @@ int lc_hip_mesh_stats(uint64_t device, uint64_t mesh, lc_hip_bvh_stats *out) {
     std::memcpy(out->bounds, m->stats.bounds, sizeof(out->bounds));
     out->node_bytes = (uint64_t)m->stats.node_count * sizeof(WideNode);
     out->primitive_bytes = (uint64_t)m->stats.tri_count * sizeof(TriRecord);
+    return LC_HIP_OK;
+}
+
+int lc_hip_mesh_memory(uint64_t device, uint64_t mesh, lc_hip_memory_usage *out) {
+    Device *dev = obj<Device>(device);
+    Mesh *m = obj<Mesh>(mesh);
+    if (!dev || !m || !out) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad mesh memory handles");
+    std::lock_guard<std::mutex> lk(m->mu);
+    out->node_bytes = (uint64_t)m->node_cap * sizeof(WideNode);
+    out->primitive_bytes = (uint64_t)m->tri_cap * sizeof(TriRecord) +
+                           (uint64_t)m->proc_cap * (sizeof(ProcRecord) + 2 * sizeof(float4) + sizeof(uint32_t));
+    out->scratch_bytes = m->scratch ? (uint64_t)m->scratch_bytes : 0;
+    out->total_bytes = out->node_bytes + out->primitive_bytes + out->scratch_bytes;
+    return LC_HIP_OK;
+}
+
+uint64_t lc_hip_device_build_arena_bytes(uint64_t device) {
+    Device *dev = obj<Device>(device);
+    if (!dev) return 0;
+    std::lock_guard<std::mutex> dl(dev->mu);
+    return (uint64_t)dev->arena_node_cap * sizeof(WideNode) + (dev->arena_scratch ? dev->arena_scratch_bytes : 0);
+}
+
+int lc_hip_device_release_build_arena(uint64_t device) {
+    Device *dev = obj<Device>(device);
+    if (!dev) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad device handle");
+    DeviceGuard g(dev);
+    std::lock_guard<std::mutex> dl(dev->mu);
+    LC_HIP_TRY(hipEventSynchronize(dev->arena_done));
+    if (dev->arena_nodes) LC_HIP_TRY(hipFree(dev->arena_nodes));
+    dev->arena_nodes = nullptr;
+    dev->arena_node_cap = 0;
+    if (dev->arena_scratch) LC_HIP_TRY(hipFree(dev->arena_scratch));
+    dev->arena_scratch = nullptr;
+    dev->arena_scratch_bytes = 0;
     return LC_HIP_OK;
 }
 
