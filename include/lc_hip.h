@@ -332,6 +332,11 @@ int lc_hip_shader_dispatch(uint64_t device, uint64_t stream, uint64_t shader, co
 #define LC_HIP_PIXEL_FLOAT2 13u
 #define LC_HIP_PIXEL_FLOAT4 14u
 uint64_t lc_hip_texture_create(uint64_t device, uint32_t storage, uint32_t width, uint32_t height);
+/* Volume of one level (Volume<T>, include/luisa/runtime/volume.h; create_texture with dimension 3,
+   device_interface.h:125-128): width x height x depth pixels, x fastest, then y, then z. Upload,
+   download, destroy and shader binding (LC_HIP_ARGUMENT_TEXTURE) as for 2-D images; bindless
+   texture slots take 2-D images only. */
+uint64_t lc_hip_volume_create(uint64_t device, uint32_t storage, uint32_t width, uint32_t height, uint32_t depth);
 void lc_hip_texture_destroy(uint64_t device, uint64_t texture);
 int lc_hip_texture_upload(uint64_t device, uint64_t stream, uint64_t texture, const void *data);
 int lc_hip_texture_download(uint64_t device, uint64_t stream, uint64_t texture, void *data);

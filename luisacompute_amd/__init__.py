@@ -215,6 +215,7 @@ _SIGNATURES = {
     "lc_hip_accel_query": (ctypes.c_int, [_U64, _U64, _U64, _U64, _SZ, _U64, _SZ, _U32, _U32,
                                           ctypes.POINTER(QueryPolicy)]),
     "lc_hip_texture_create": (_U64, [_U64, _U32, _U32, _U32]),
+    "lc_hip_volume_create": (_U64, [_U64, _U32, _U32, _U32, _U32]),
     "lc_hip_texture_destroy": (None, [_U64, _U64]),
     "lc_hip_texture_upload": (ctypes.c_int, [_U64, _U64, _U64, _P]),
     "lc_hip_texture_download": (ctypes.c_int, [_U64, _U64, _U64, _P]),
@@ -454,6 +455,9 @@ class Device:
 
     def create_image(self, storage, width, height):
         return Image(self, storage, width, height)
+
+    def create_volume(self, storage, width, height, depth):
+        return Volume(self, storage, width, height, depth)
 
     def create_bindless_array(self, slots):
         return BindlessArray(self, slots)
@@ -733,6 +737,32 @@ class Image:
         if array.dtype != self.dtype or array.size != self.width * self.height * self.channels or \
                 not array.flags.c_contiguous:
             raise LCError("image download array mismatch")
+        return _ImageCopy(self, array, False)
+
+
+class Volume(Image):
+    """Volume of one level (Volume<T>, include/luisa/runtime/volume.h) in a PixelStorage; host
+    arrays are (depth, height, width, channels) of the storage's channel type."""
+
+    def __init__(self, device, storage, width, height, depth):
+        self.device, self.storage = device, storage
+        self.width, self.height, self.depth = int(width), int(height), int(depth)
+        self.handle = _handle(lib().lc_hip_volume_create(device.handle, storage, self.width, self.height, self.depth),
+                              "volume create")
+
+    def empty_host(self):
+        return np.zeros((self.depth, self.height, self.width, self.channels), self.dtype)
+
+    def copy_from(self, array):
+        a = np.ascontiguousarray(array, self.dtype)
+        if a.size != self.depth * self.width * self.height * self.channels:
+            raise LCError("volume upload size mismatch")
+        return _ImageCopy(self, a, True)
+
+    def copy_to(self, array):
+        if array.dtype != self.dtype or array.size != self.depth * self.width * self.height * self.channels or \
+                not array.flags.c_contiguous:
+            raise LCError("volume download array mismatch")
         return _ImageCopy(self, array, False)
 
 

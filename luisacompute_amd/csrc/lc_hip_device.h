@@ -65,8 +65,12 @@ struct alignas(16) Image {
     void *data;
     uint32_t width, height;
     uint32_t storage;
-    uint32_t _pad[3];
+    uint32_t depth; // 1 for 2-D images
+    uint32_t _pad[2];
 };
+// Volume (Volume<T>, include/luisa/runtime/volume.h): the same layout with depth slices of
+// height rows; pixel (x, y, z) sits at (z * height + y) * width + x.
+using Volume = Image;
 
 __device__ __forceinline__ uint32_t image_channels(const Image &img) { return img.storage % 3u == 2u ? 4u : img.storage % 3u + 1u; }
 
@@ -86,27 +90,24 @@ __device__ __forceinline__ int image_channel_int(const Image &img, uint64_t p, u
     return static_cast<const int *>(img.data)[p * n + c];
 }
 
-__device__ __forceinline__ float4 image_read_float(const Image img, uint32_t x, uint32_t y) {
-    const uint64_t p = (uint64_t)y * img.width + x;
+__device__ __forceinline__ float4 pixel_read_float(const Image img, uint64_t p) {
     const uint32_t n = image_channels(img);
     float v[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     for (uint32_t c = 0; c < n; c++) v[c] = image_channel_float(img, p, c);
     return make_float4(v[0], v[1], v[2], v[3]);
 }
-__device__ __forceinline__ uint4 image_read_uint(const Image img, uint32_t x, uint32_t y) {
-    const uint64_t p = (uint64_t)y * img.width + x;
+__device__ __forceinline__ uint4 pixel_read_uint(const Image img, uint64_t p) {
     const uint32_t n = image_channels(img);
     uint32_t v[4] = {0u, 0u, 0u, 0u};
     for (uint32_t c = 0; c < n; c++) v[c] = (uint32_t)image_channel_int(img, p, c);
     return make_uint4(v[0], v[1], v[2], v[3]);
 }
-__device__ __forceinline__ int4 image_read_int(const Image img, uint32_t x, uint32_t y) {
-    const uint4 u = image_read_uint(img, x, y);
+__device__ __forceinline__ int4 pixel_read_int(const Image img, uint64_t p) {
+    const uint4 u = pixel_read_uint(img, p);
     return make_int4((int)u.x, (int)u.y, (int)u.z, (int)u.w);
 }
 
-__device__ __forceinline__ void image_write_float(const Image img, uint32_t x, uint32_t y, float4 value) {
-    const uint64_t p = (uint64_t)y * img.width + x;
+__device__ __forceinline__ void pixel_write_float(const Image img, uint64_t p, float4 value) {
     const uint32_t kind = img.storage / 3u, n = image_channels(img);
     const float v[4] = {value.x, value.y, value.z, value.w};
     for (uint32_t c = 0; c < n; c++) {
@@ -118,8 +119,7 @@ __device__ __forceinline__ void image_write_float(const Image img, uint32_t x, u
         else static_cast<float *>(img.data)[p * n + c] = v[c];
     }
 }
-__device__ __forceinline__ void image_write_uint(const Image img, uint32_t x, uint32_t y, uint4 value) {
-    const uint64_t p = (uint64_t)y * img.width + x;
+__device__ __forceinline__ void pixel_write_uint(const Image img, uint64_t p, uint4 value) {
     const uint32_t kind = img.storage / 3u, n = image_channels(img);
     const uint32_t v[4] = {value.x, value.y, value.z, value.w};
     for (uint32_t c = 0; c < n; c++) {
@@ -128,9 +128,26 @@ __device__ __forceinline__ void image_write_uint(const Image img, uint32_t x, ui
         else static_cast<uint32_t *>(img.data)[p * n + c] = v[c];
     }
 }
-__device__ __forceinline__ void image_write_int(const Image img, uint32_t x, uint32_t y, int4 value) {
-    image_write_uint(img, x, y, make_uint4((uint32_t)value.x, (uint32_t)value.y, (uint32_t)value.z, (uint32_t)value.w));
+__device__ __forceinline__ void pixel_write_int(const Image img, uint64_t p, int4 value) {
+    pixel_write_uint(img, p, make_uint4((uint32_t)value.x, (uint32_t)value.y, (uint32_t)value.z, (uint32_t)value.w));
 }
+
+__device__ __forceinline__ uint64_t image_pixel(const Image &img, uint32_t x, uint32_t y) { return (uint64_t)y * img.width + x; }
+__device__ __forceinline__ uint64_t volume_pixel(const Volume &v, uint32_t x, uint32_t y, uint32_t z) {
+    return ((uint64_t)z * v.height + y) * v.width + x;
+}
+__device__ __forceinline__ float4 image_read_float(const Image img, uint32_t x, uint32_t y) { return pixel_read_float(img, image_pixel(img, x, y)); }
+__device__ __forceinline__ uint4 image_read_uint(const Image img, uint32_t x, uint32_t y) { return pixel_read_uint(img, image_pixel(img, x, y)); }
+__device__ __forceinline__ int4 image_read_int(const Image img, uint32_t x, uint32_t y) { return pixel_read_int(img, image_pixel(img, x, y)); }
+__device__ __forceinline__ void image_write_float(const Image img, uint32_t x, uint32_t y, float4 value) { pixel_write_float(img, image_pixel(img, x, y), value); }
+__device__ __forceinline__ void image_write_uint(const Image img, uint32_t x, uint32_t y, uint4 value) { pixel_write_uint(img, image_pixel(img, x, y), value); }
+__device__ __forceinline__ void image_write_int(const Image img, uint32_t x, uint32_t y, int4 value) { pixel_write_int(img, image_pixel(img, x, y), value); }
+__device__ __forceinline__ float4 volume_read_float(const Volume v, uint32_t x, uint32_t y, uint32_t z) { return pixel_read_float(v, volume_pixel(v, x, y, z)); }
+__device__ __forceinline__ uint4 volume_read_uint(const Volume v, uint32_t x, uint32_t y, uint32_t z) { return pixel_read_uint(v, volume_pixel(v, x, y, z)); }
+__device__ __forceinline__ int4 volume_read_int(const Volume v, uint32_t x, uint32_t y, uint32_t z) { return pixel_read_int(v, volume_pixel(v, x, y, z)); }
+__device__ __forceinline__ void volume_write_float(const Volume v, uint32_t x, uint32_t y, uint32_t z, float4 value) { pixel_write_float(v, volume_pixel(v, x, y, z), value); }
+__device__ __forceinline__ void volume_write_uint(const Volume v, uint32_t x, uint32_t y, uint32_t z, uint4 value) { pixel_write_uint(v, volume_pixel(v, x, y, z), value); }
+__device__ __forceinline__ void volume_write_int(const Volume v, uint32_t x, uint32_t y, uint32_t z, int4 value) { pixel_write_int(v, volume_pixel(v, x, y, z), value); }
 
 // Bindless array of buffer slots (BindlessArray, include/luisa/runtime/bindless_array.h; the
 // CUDA backend's buffer reads cuda_device_resource.h): slot s = {pointer at the emplaced offset,
@@ -187,7 +204,7 @@ __device__ __forceinline__ int tex_address(int i, int n, uint32_t address) {
 }
 __device__ __forceinline__ float4 tex_texel(const BindlessTex2D &t, int x, int y) {
     if (x < 0 || y < 0) return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    const Image img{t.data, t.width, t.height, t.storage, {0u, 0u, 0u}};
+    const Image img{t.data, t.width, t.height, t.storage, 1u, {0u, 0u}};
     return image_read_float(img, (uint32_t)x, (uint32_t)y);
 }
 __device__ __forceinline__ float4 bindless_tex2d_read(const BindlessArray a, uint32_t slot, uint32_t x, uint32_t y) {

@@ -383,6 +383,7 @@ struct Texture {
     Device *dev = nullptr;
     void *ptr = nullptr;
     uint32_t storage = 0, width = 0, height = 0, pixel_bytes = 0;
+    uint32_t depth = 1; // > 1 for volumes (3-D images)
 };
 // bytes per pixel of a PixelStorage value (include/luisa/runtime/rhi/pixel.h:14-35), 0 if unsupported
 uint32_t pixel_storage_bytes(uint32_t storage);

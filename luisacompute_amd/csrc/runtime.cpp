@@ -1778,18 +1778,19 @@ uint32_t lchip::pixel_storage_bytes(uint32_t storage) {
 
 extern "C" {
 
-uint64_t lc_hip_texture_create(uint64_t device, uint32_t storage, uint32_t width, uint32_t height) {
+static uint64_t texture_create(uint64_t device, uint32_t storage, uint32_t width, uint32_t height, uint32_t depth) {
     Device *dev = obj<Device>(device);
     const uint32_t pb = pixel_storage_bytes(storage);
-    if (!dev || pb == 0 || width == 0 || height == 0 || width > 65536 || height > 65536) {
+    if (!dev || pb == 0 || width == 0 || height == 0 || depth == 0 || width > 65536 || height > 65536 ||
+        depth > 65536 || (size_t)width * height * depth > (1ull << 34)) {
         fail(LC_HIP_ERROR_UNSUPPORTED, "image storage or size not supported");
         return LC_HIP_INVALID_HANDLE;
     }
     DeviceGuard g(dev);
     auto t = std::make_unique<Texture>();
     t->dev = dev;
-    t->storage = storage; t->width = width; t->height = height; t->pixel_bytes = pb;
-    const size_t bytes = (size_t)width * height * pb;
+    t->storage = storage; t->width = width; t->height = height; t->depth = depth; t->pixel_bytes = pb;
+    const size_t bytes = (size_t)width * height * depth * pb;
     hipError_t e = hipMalloc(&t->ptr, bytes);
     if (e == hipSuccess) e = hipMemset(t->ptr, 0, bytes);
     if (e != hipSuccess) {
@@ -1798,6 +1799,14 @@ uint64_t lc_hip_texture_create(uint64_t device, uint32_t storage, uint32_t width
         return LC_HIP_INVALID_HANDLE;
     }
     return reinterpret_cast<uint64_t>(t.release());
+}
+
+uint64_t lc_hip_texture_create(uint64_t device, uint32_t storage, uint32_t width, uint32_t height) {
+    return texture_create(device, storage, width, height, 1);
+}
+
+uint64_t lc_hip_volume_create(uint64_t device, uint32_t storage, uint32_t width, uint32_t height, uint32_t depth) {
+    return texture_create(device, storage, width, height, depth);
 }
 
 void lc_hip_texture_destroy(uint64_t device, uint64_t texture) {
@@ -1817,7 +1826,7 @@ int lc_hip_texture_upload(uint64_t device, uint64_t stream, uint64_t texture, co
     DeviceGuard g(dev);
     hipStream_t s = native_stream(stream);
     // the source is consumed before the call returns, as for buffer uploads
-    LC_HIP_TRY(hipMemcpyAsync(t->ptr, data, (size_t)t->width * t->height * t->pixel_bytes, hipMemcpyHostToDevice, s));
+    LC_HIP_TRY(hipMemcpyAsync(t->ptr, data, (size_t)t->width * t->height * t->depth * t->pixel_bytes, hipMemcpyHostToDevice, s));
     LC_HIP_TRY(hipStreamSynchronize(s));
     return LC_HIP_OK;
 }
@@ -1828,7 +1837,7 @@ int lc_hip_texture_download(uint64_t device, uint64_t stream, uint64_t texture, 
     if (!dev || !t || !data) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad image download arguments");
     DeviceGuard g(dev);
     hipStream_t s = native_stream(stream);
-    LC_HIP_TRY(hipMemcpyAsync(data, t->ptr, (size_t)t->width * t->height * t->pixel_bytes, hipMemcpyDeviceToHost, s));
+    LC_HIP_TRY(hipMemcpyAsync(data, t->ptr, (size_t)t->width * t->height * t->depth * t->pixel_bytes, hipMemcpyDeviceToHost, s));
     return LC_HIP_OK;
 }
 
@@ -1882,6 +1891,7 @@ int lc_hip_bindless_update(uint64_t device, uint64_t stream, uint64_t array, con
         if (m.texture_op == LC_HIP_BINDLESS_EMPLACE) {
             Texture *t = obj<Texture>(m.texture);
             if (!t) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad bindless texture");
+            if (t->depth != 1) return fail(LC_HIP_ERROR_UNSUPPORTED, "bindless texture slots hold 2-D images");
             if (m.sampler > 15u) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "unknown sampler code");
             next_tex[m.slot] = BindlessTex2D{t->ptr, t->width, t->height, t->storage, m.sampler, {0, 0}};
         } else if (m.texture_op == LC_HIP_BINDLESS_REMOVE) {
@@ -2032,7 +2042,7 @@ static int shader_launch(uint64_t device, uint64_t stream, uint64_t shader, cons
             Texture *t = obj<Texture>(x.handle);
             if (!t) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad image argument");
             const uint64_t binding[4] = {reinterpret_cast<uint64_t>(t->ptr), (uint64_t)t->width | ((uint64_t)t->height << 32),
-                                         (uint64_t)t->storage, 0};
+                                         (uint64_t)t->storage | ((uint64_t)t->depth << 32), 0};
             if (!put(binding, sizeof(binding))) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "shader arguments exceed 4 KiB");
         } else if (x.tag == LC_HIP_ARGUMENT_BINDLESS_ARRAY) {
             Bindless *b = obj<Bindless>(x.handle);

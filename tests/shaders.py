@@ -205,6 +205,31 @@ extern "C" __global__ void __launch_bounds__(64, 1) image_kernel(const Params p)
 
 ALL["image_kernel"] = IMAGES
 
+# volumes: read a FLOAT2 volume, write a BYTE4 and an INT1 volume, read them back into a buffer
+VOLUMES = r'''
+#include "lc_hip_device.h"
+using namespace lchip;
+struct Params {
+    alignas(16) dev::Volume src;  // FLOAT2
+    alignas(16) dev::Volume b4;   // BYTE4
+    alignas(16) dev::Volume i1;   // INT1
+    alignas(16) dev::Buffer<float4> back;
+    alignas(16) uint4 ls_kid;
+};
+extern "C" __global__ void __launch_bounds__(64, 1) volume_kernel(const Params p) {
+    if (!dev::in_dispatch(p.ls_kid)) return;
+    const uint3 id = dev::dispatch_id();
+    const float4 v = dev::volume_read_float(p.src, id.x, id.y, id.z);
+    dev::volume_write_float(p.b4, id.x, id.y, id.z, make_float4(v.x, v.y, 0.5f, 1.0f));
+    dev::volume_write_int(p.i1, id.x, id.y, id.z, make_int4((int)(id.x + 10u * id.y + 100u * id.z) - 500, 0, 0, 0));
+    const float4 b = dev::volume_read_float(p.b4, id.x, id.y, id.z);
+    const int4 i = dev::volume_read_int(p.i1, id.x, id.y, id.z);
+    p.back[((uint64_t)id.z * p.ls_kid.y + id.y) * p.ls_kid.x + id.x] = make_float4(b.x, v.w, (float)i.x, (float)p.src.depth);
+}
+'''
+
+ALL["volume_kernel"] = VOLUMES
+
 # indirect dispatch: a producer fills the indirect buffer, a consumer records its kernel id and
 # dispatch id (the pattern of src/tests/test_indirect.cpp)
 INDIRECT_PRODUCER = r'''

@@ -129,3 +129,34 @@ def test_bindless_texture_sampling(device, stream):
     xs = np.floor(uv[:, 0] * w).astype(int) % w
     ys = np.floor(uv[:, 1] * h).astype(int) % h
     np.testing.assert_array_equal(out[inner, j], tex[ys[inner], xs[inner]])
+
+
+def test_volume_shader(device, stream):
+    """Volumes (Volume<T>): upload / download, float reads of FLOAT2 (missing channels 0),
+    unorm byte writes and reads, int writes, 3-D dispatch coordinates."""
+    w, h, d = 19, 7, 5
+    rng = np.random.default_rng(12)
+    src = rng.uniform(-0.2, 1.2, (d, h, w, 2)).astype(np.float32)
+    vs = device.create_volume(lc.PIXEL_FLOAT2, w, h, d)
+    vb = device.create_volume(lc.PIXEL_BYTE4, w, h, d)
+    vi = device.create_volume(lc.PIXEL_INT1, w, h, d)
+    back = device.create_buffer(16, w * h * d)
+    stream << vs.copy_from(src)
+    sh = device.create_shader(shaders.VOLUMES, "volume_kernel", block=(8, 4, 2))
+    stream << sh(vs, vb, vi, back).dispatch(w, h, d)
+    ob4, oi1, os2 = vb.empty_host(), vi.empty_host(), vs.empty_host()
+    bk = np.zeros((d, h, w, 4), np.float32)
+    stream << vb.copy_to(ob4) << vi.copy_to(oi1) << vs.copy_to(os2) << back.copy_to(bk) << lc.synchronize()
+    np.testing.assert_array_equal(os2.view(np.uint32), src.view(np.uint32))
+    rgba = np.concatenate([src, np.full((d, h, w, 1), 0.5, np.float32), np.ones((d, h, w, 1), np.float32)], axis=-1)
+    np.testing.assert_array_equal(ob4, _unorm_write(rgba, 255.0).astype(np.uint8))
+    zz, yy, xx = np.mgrid[0:d, 0:h, 0:w]
+    np.testing.assert_array_equal(oi1[..., 0], (xx + 10 * yy + 100 * zz - 500).astype(np.int32))
+    np.testing.assert_array_equal(bk[..., 0], ob4[..., 0].astype(np.float32) * np.float32(1.0 / 255.0))
+    np.testing.assert_array_equal(bk[..., 1], np.zeros((d, h, w), np.float32))
+    np.testing.assert_array_equal(bk[..., 2], (xx + 10 * yy + 100 * zz - 500).astype(np.float32))
+    np.testing.assert_array_equal(bk[..., 3], np.full((d, h, w), d, np.float32))
+    bl = device.create_bindless_array(2)
+    with pytest.raises(lc.LCError):
+        bl.emplace_texture(0, vs, lc.sampler_code(lc.FILTER_POINT, lc.ADDRESS_EDGE))
+        stream << bl.update() << lc.synchronize()
