@@ -14,7 +14,7 @@ printed as HIP source directly, against the device library of run-time shaders
     shader = shade.compile(device)
     stream << shader(ray_buffer, hit_buffer, accel, 0xFF).dispatch(n)
 
-Supported: parameters annotated with Buffer(element), Accel, Image, BindlessArray or a scalar type
+Supported: parameters annotated with Buffer(element), Accel, Image, Volume, BindlessArray or a scalar type
 (uint, int_, float_, ulong, float3_: uniforms); nested functions (helpers and ray-query callbacks,
 printed as lambdas capturing the kernel's variables by reference); assignments (the first one in a block declares with `auto`),
 augmented assignments, if / elif / else, for over range(...), while, break, continue, return,
@@ -72,6 +72,7 @@ class _Scalar:
 
 Accel = _Resource("lchip::dev::Accel")
 Image = _Resource("lchip::dev::Image")
+Volume = _Resource("lchip::dev::Volume")
 BindlessArray = _Resource("lchip::dev::BindlessArray")
 uint = _Scalar("uint32_t", np.uint32)
 int_ = _Scalar("int32_t", np.int32)
@@ -360,6 +361,11 @@ class _Codegen(ast.NodeVisitor):
                     return f"lchip::dev::image_read_float({obj}, {args[0]}, {args[1]})"
                 if m == "write":
                     return f"lchip::dev::image_write_float({obj}, {args[0]}, {args[1]}, {args[2]})"
+            if kind is Volume:
+                if m == "read":
+                    return f"lchip::dev::volume_read_float({obj}, {args[0]}, {args[1]}, {args[2]})"
+                if m == "write":
+                    return f"lchip::dev::volume_write_float({obj}, {args[0]}, {args[1]}, {args[2]}, {args[3]})"
             if kind is BindlessArray and m == "tex2d_sample":
                 return f"lchip::dev::bindless_tex2d_sample({obj}, {args[0]}, {args[1]}.x, {args[1]}.y)"
             if kind is BindlessArray and m == "tex2d_read":
@@ -389,7 +395,7 @@ class Kernel:
         for a in fdef.args.args:
             ann = hints.get(a.arg)
             if not isinstance(ann, (Buffer, _Resource, _Scalar)):
-                raise LCError(f"dsl: parameter {a.arg} needs a Buffer / Accel / Image / BindlessArray / scalar annotation")
+                raise LCError(f"dsl: parameter {a.arg} needs a Buffer / Accel / Image / Volume / BindlessArray / scalar annotation")
             self.params[a.arg] = ann
         gen = _Codegen(self.params, {k: v for k, v in fn.__globals__.items() if isinstance(v, (int, float))})
         for st in fdef.body:

@@ -267,3 +267,11 @@ def sample_kernel(uv: Buffer("float2"), out: Buffer("float4"), heap: dsl.Bindles
     i = dispatch_id().x
     for k in range(n_slots):
         out.write(i * n_slots + k, heap.tex2d_sample(k, uv.read(i)))
+
+
+@dsl.kernel(block=(8, 4, 2))
+def volume_kernel(src: dsl.Volume, dst: dsl.Volume, scale: float_):
+    """dst(x, y, z) = (scale * src.xy, z, 1) over a 3-D dispatch."""
+    p = dispatch_id()
+    v = src.read(p.x, p.y, p.z)
+    dst.write(p.x, p.y, p.z, float4(v.x * scale, v.y * scale, float(p.z), 1.0))

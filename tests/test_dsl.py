@@ -7,7 +7,8 @@ import dsl_kernels
 
 
 @pytest.mark.parametrize("k", [dsl_kernels.trace_kernel, dsl_kernels.control_kernel, dsl_kernels.shade_kernel,
-                               dsl_kernels.query_kernel, dsl_kernels.path_tracer])
+                               dsl_kernels.query_kernel, dsl_kernels.path_tracer,
+                               dsl_kernels.volume_kernel])
 def test_dsl_kernels_compile(k):
     assert "extern \"C\" __global__" in k.source
     assert lc.compile_shader_source(k.source) > 1000

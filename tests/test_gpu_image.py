@@ -160,3 +160,22 @@ def test_volume_shader(device, stream):
     with pytest.raises(lc.LCError):
         bl.emplace_texture(0, vs, lc.sampler_code(lc.FILTER_POINT, lc.ADDRESS_EDGE))
         stream << bl.update() << lc.synchronize()
+
+
+def test_dsl_volume_kernel(device, stream):
+    """A Python DSL kernel over volumes (3-D dispatch) equals its numpy restatement."""
+    import dsl_kernels
+    w, h, d = 13, 6, 4
+    src = np.random.default_rng(13).uniform(-2.0, 2.0, (d, h, w, 2)).astype(np.float32)
+    vs = device.create_volume(lc.PIXEL_FLOAT2, w, h, d)
+    vd = device.create_volume(lc.PIXEL_FLOAT4, w, h, d)
+    stream << vs.copy_from(src)
+    k = dsl_kernels.volume_kernel.compile(device)
+    stream << k(vs, vd, 0.5).dispatch(w, h, d)
+    out = vd.empty_host()
+    stream << vd.copy_to(out) << lc.synchronize()
+    zz = np.broadcast_to(np.arange(d, dtype=np.float32)[:, None, None], (d, h, w))
+    np.testing.assert_array_equal(out[..., 0], src[..., 0] * np.float32(0.5))
+    np.testing.assert_array_equal(out[..., 1], src[..., 1] * np.float32(0.5))
+    np.testing.assert_array_equal(out[..., 2], zz)
+    np.testing.assert_array_equal(out[..., 3], np.ones((d, h, w), np.float32))
