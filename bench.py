@@ -110,8 +110,7 @@ def bench_pathtracer(a):
     # pixel): the AST-codegen megakernel, at the same image size and spp, same image
     dsl_ms = dsl_same = None
     if not a.pt_no_mega:
-        sys.path.insert(0, os.path.join(ROOT, "tests"))
-        import dsl_kernels
+        from luisacompute_amd import programs
         heap = dev.create_bindless_array(2 * len(meshes))
         for i, m in enumerate(meshes):
             heap.emplace(2 * i, m.vb)
@@ -121,7 +120,7 @@ def bench_pathtracer(a):
         mats = np.array([list(room.albedo[i]) + [1.0 if room.emissive[i] else 0.0] for i in range(len(meshes))], np.float32)
         mb = host.upload(dev, s, mats)
         vec = lambda name: np.array(getattr(scene, name)[:], np.float32)
-        kdsl = dsl_kernels.path_tracer.compile(dev)
+        kdsl = programs.path_tracer.compile(dev)
         dimg = dev.create_buffer(16, w * h)
         dsl_cmd = kdsl(dimg, accel, heap, xf, mb, vec("light_position"), vec("light_u"), vec("light_v"),
                        vec("light_emission"), vec("camera_origin"), vec("camera_right"), vec("camera_up"),
@@ -394,9 +393,8 @@ def bench_query(a):
     s << hb.copy_to(out) << lc.synchronize()
     kinds = np.bincount(out["hit_type"].astype(np.int64), minlength=3)
     # the same query as a run-time shader (hiprtc, one query per thread, callbacks in the shader)
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import shaders
-    sh = dev.create_shader(shaders.QUERY, "query_kernel")
+    from luisacompute_amd import programs
+    sh = dev.create_shader(programs.QUERY, "query_kernel")
     hs = dev.create_buffer(24, n_rays)
 
     def run_shader():

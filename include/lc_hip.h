@@ -146,7 +146,13 @@ int lc_hip_stream_synchronize(uint64_t device, uint64_t stream);
 int lc_hip_stream_add_callback(uint64_t device, uint64_t stream, void (*fn)(void *), void *user);
 void *lc_hip_stream_native(uint64_t device, uint64_t stream);
 
-/* ---- events: timeline fences (device_interface.h:163-168; CUDAEvent, cuda_event.cpp:21-63) ---- */
+/* ---- events: timeline fences (device_interface.h:163-168; CUDAEvent, cuda_event.cpp:21-63, a
+   timeline semaphore). signal(stream, v) issues value v at the stream's current point; the
+   timeline has reached v once a signal of a value >= v completed on the GPU (the initial value is
+   0). wait(stream, v) orders the stream's later work after that point; synchronize(v) blocks the
+   caller until it; is_completed(v) returns 1 if reached, else 0. A wait or synchronize for a
+   value no signal >= v was issued for yet blocks the calling thread until another thread issues
+   one (the GPU never spins on a timeline value). ---- */
 uint64_t lc_hip_event_create(uint64_t device);
 void lc_hip_event_destroy(uint64_t device, uint64_t event);
 int lc_hip_event_signal(uint64_t device, uint64_t event, uint64_t stream, uint64_t value);

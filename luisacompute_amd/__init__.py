@@ -519,7 +519,9 @@ class Stream:
     def __init__(self, device, tag=1):
         self.device = device
         self.handle = _handle(lib().lc_hip_stream_create(device.handle, tag), "stream create")
-        self._callbacks = []
+        self._thunk = None
+        self._pending = {}
+        self._next_callback = 0
 
     def __del__(self):
         if getattr(self, "handle", None) is not None and _lib is not None and self.device.handle is not None:
@@ -535,13 +537,26 @@ class Stream:
 
     def synchronize(self):
         _check(lib().lc_hip_stream_synchronize(self.device.handle, self.handle), "stream synchronize")
-        self._callbacks = []
 
     def add_callback(self, fn):
-        cb = ctypes.CFUNCTYPE(None, ctypes.c_void_p)(lambda _: fn())
-        self._callbacks.append(cb)
-        _check(lib().lc_hip_stream_add_callback(self.device.handle, self.handle, ctypes.cast(cb, ctypes.c_void_p), None),
-               "stream callback")
+        """Runs fn() on a backend thread once the work enqueued so far has completed
+        (CUDAStream::callback, src/backends/cuda/cuda_stream.cpp:142-170); callbacks of one stream
+        run in the order they were added."""
+        if self._thunk is None:
+            # one C thunk per stream, alive as long as the stream; `user` names the Python callable
+            def run(user):
+                f = self._pending.pop(user, None)
+                if f is not None:
+                    f()
+            self._thunk = ctypes.CFUNCTYPE(None, ctypes.c_void_p)(run)
+        self._next_callback += 1
+        key = self._next_callback
+        self._pending[key] = fn
+        rc = lib().lc_hip_stream_add_callback(self.device.handle, self.handle, ctypes.cast(self._thunk, ctypes.c_void_p),
+                                              ctypes.c_void_p(key))
+        if rc != 0:
+            self._pending.pop(key, None)
+        _check(rc, "stream callback")
 
     def native(self):
         return lib().lc_hip_stream_native(self.device.handle, self.handle)
@@ -570,6 +585,24 @@ class Event:
 
     def is_completed(self, value):
         return lib().lc_hip_event_is_completed(self.device.handle, self.handle, value) == 1
+
+    # stream commands, as the reference's Event::signal() / wait() (src/runtime/event.cpp:7-15)
+    def signaled(self, value):
+        return _EventCommand(self, value, True)
+
+    def waited(self, value):
+        return _EventCommand(self, value, False)
+
+
+class _EventCommand(Command):
+    def __init__(self, event, value, signal):
+        self.event, self.value, self.is_signal = event, value, signal
+
+    def execute(self, stream):
+        if self.is_signal:
+            self.event.signal(stream, self.value)
+        else:
+            self.event.wait(stream, self.value)
 
 
 class Buffer:

@@ -239,6 +239,22 @@ struct TimingStat {
     uint64_t launches = 0;
 };
 
+// Working memory of the trace, query and build launches of one stream: launches on different
+// streams never share it (streams run concurrently); launches on one stream are ordered by the
+// stream. words: [0, 512) the per-segment slot counters, 512 the error word, 576 the two 64-bit
+// counter-mode totals.
+struct StreamScratch {
+    uint32_t *words = nullptr;
+    uint32_t *ray_counter = nullptr;
+    uint32_t *error = nullptr;
+    unsigned long long *stats = nullptr;
+    void *spill = nullptr;
+    size_t spill_bytes = 0;
+    void *sort = nullptr;
+    size_t sort_bytes = 0;
+};
+constexpr size_t kStreamWordsBytes = 1024;
+
 struct Device {
     int index = 0;
     int cu_count = 0;
@@ -246,14 +262,10 @@ struct Device {
     std::mutex mu;
     TraceConfig cfg[2]; // [closest, any]
     uint32_t split_small = 1;
-    // per-device working memory shared by launches (guarded by mu)
-    void *spill = nullptr;
-    size_t spill_bytes = 0;
-    void *sort_scratch = nullptr;
-    size_t sort_bytes = 0;
-    uint32_t *ray_counter = nullptr;
+    // error word of run-time shaders' traversal stacks (DeviceAccel::error)
     uint32_t *error = nullptr;
-    unsigned long long *stats = nullptr;
+    // per-stream working memory of trace / query / build launches (guarded by mu)
+    std::map<hipStream_t, StreamScratch *> scratch;
     // build arena of compacting BLAS builds (guarded by mu): worst-case wide nodes, and the build
     // scratch of primitives without allow_update; free again once arena_done has completed
     WideNode *arena_nodes = nullptr;
@@ -271,6 +283,11 @@ struct Device {
     std::map<std::string, TimingStat> timings;
     void *pt = nullptr; // path-tracer workspace (pathtracer.hip), released with the device
 };
+
+// The stream's working memory, created on first use (caller holds dev->mu); null when the
+// allocation fails. Released with the stream (after it drained) or with the device.
+StreamScratch *stream_scratch(Device *dev, hipStream_t s);
+void stream_scratch_release(Device *dev, hipStream_t s);
 
 struct Buffer {
     Device *dev = nullptr;
@@ -368,14 +385,21 @@ struct Accel {
     // out-of-range-index flag word); written at every build, folded back into `inst` before the
     // next build when instance writes were issued
     InstanceState *state = nullptr;
-    std::vector<InstanceState> state_host; // staging of the last upload
     size_t state_cap = 0;
     uint32_t state_count = 0;
     bool state_written = false;
     // descriptor a run-time shader's accel argument points at (lc_hip_device.h DeviceAccel),
     // rewritten by every build
     void *desc = nullptr;
-    DeviceAccel desc_host{}; // staging of the last descriptor upload
+    DeviceAccel desc_host{};
+    // pinned staging of a build's uploads (instance records, instance table, TLAS input boxes,
+    // descriptor): the copies read it after the build call returned; `staged` marks their end and
+    // is waited for before the next build writes it again
+    char *stage = nullptr;
+    size_t stage_cap = 0;
+    hipEvent_t staged = nullptr;
+    bool stage_pending = false;
+    uint32_t tlas_count = 0; // instance count of the last full build (its TLAS, or its single instance)
 };
 
 // ---- images: 2-D, linear rows, one level ----
@@ -411,7 +435,8 @@ extern const int kDeviceHeaderCount;
 // Compiles `source` for gfx950 with the device library; code object into `code`, diagnostics
 // into `log`. False on a compile error.
 bool rtc_compile(const char *source, const char *name, std::vector<char> &code, std::string &log);
-// Writes the accel's DeviceAccel descriptor (allocating it on first use) on `stream`.
-int accel_write_descriptor(Accel *a, hipStream_t stream);
+// Writes the accel's DeviceAccel descriptor (allocating it on first use) on `stream`, copying
+// from `staging` (pinned, sizeof(DeviceAccel) bytes, untouched until the copy ran).
+int accel_write_descriptor(Accel *a, hipStream_t stream, void *staging);
 
 } // namespace lchip

@@ -239,7 +239,45 @@ int arena_compact(Device *dev, Mesh *m, hipStream_t s) {
     return LC_HIP_OK;
 }
 
+void free_stream_scratch(StreamScratch *S) {
+    if (S->words) hipFree(S->words);
+    if (S->spill) hipFree(S->spill);
+    if (S->sort) hipFree(S->sort);
+    delete S;
+}
+
 } // namespace
+
+StreamScratch *lchip::stream_scratch(Device *dev, hipStream_t s) {
+    auto it = dev->scratch.find(s);
+    if (it != dev->scratch.end()) return it->second;
+    auto *S = new StreamScratch;
+    void *p = nullptr;
+    if (hipMalloc(&p, kStreamWordsBytes) != hipSuccess) {
+        delete S;
+        return nullptr;
+    }
+    // ordered before any launch on s that uses the words
+    if (hipMemsetAsync(p, 0, kStreamWordsBytes, s) != hipSuccess) {
+        hipFree(p);
+        delete S;
+        return nullptr;
+    }
+    S->words = static_cast<uint32_t *>(p);
+    S->ray_counter = S->words;
+    S->error = S->words + kRayCounterWords;
+    S->stats = reinterpret_cast<unsigned long long *>(static_cast<char *>(p) + 576);
+    dev->scratch.emplace(s, S);
+    return S;
+}
+
+void lchip::stream_scratch_release(Device *dev, hipStream_t s) {
+    std::lock_guard<std::mutex> lk(dev->mu);
+    auto it = dev->scratch.find(s);
+    if (it == dev->scratch.end()) return;
+    free_stream_scratch(it->second);
+    dev->scratch.erase(it);
+}
 
 extern "C" {
 
@@ -269,19 +307,16 @@ uint64_t lc_hip_device_create(int32_t index) {
     dev->name = prop.gcnArchName;
     DeviceGuard g(dev.get());
     void *p = nullptr;
-    // [0, 512): per-segment ray slot counters; 512: error word; 576: two 64-bit counter-mode stats
-    if (hipMalloc(&p, 1024) != hipSuccess) {
+    if (hipMalloc(&p, 256) != hipSuccess) {
         fail(LC_HIP_ERROR_OUT_OF_MEMORY, "device scratch allocation failed");
         return LC_HIP_INVALID_HANDLE;
     }
-    hipMemset(p, 0, 1024);
-    dev->ray_counter = static_cast<uint32_t *>(p);
-    dev->error = dev->ray_counter + kRayCounterWords;
-    dev->stats = reinterpret_cast<unsigned long long *>(static_cast<char *>(p) + 576);
+    hipMemset(p, 0, 256);
+    dev->error = static_cast<uint32_t *>(p);
     dev->cfg[0] = TraceConfig{256, 16, 32, 0};
     dev->cfg[1] = TraceConfig{256, 16, 32, 0};
     if (hipEventCreateWithFlags(&dev->arena_done, hipEventDisableTiming) != hipSuccess) {
-        hipFree(dev->ray_counter);
+        hipFree(dev->error);
         fail(LC_HIP_ERROR_HIP, "device event creation failed");
         return LC_HIP_INVALID_HANDLE;
     }
@@ -295,9 +330,9 @@ void lc_hip_device_destroy(uint64_t device) {
     hipDeviceSynchronize();
     collect_timings(dev);
     pt_release(dev);
-    if (dev->spill) hipFree(dev->spill);
-    if (dev->sort_scratch) hipFree(dev->sort_scratch);
-    if (dev->ray_counter) hipFree(dev->ray_counter);
+    for (auto &kv : dev->scratch) free_stream_scratch(kv.second);
+    dev->scratch.clear();
+    if (dev->error) hipFree(dev->error);
     if (dev->arena_nodes) hipFree(dev->arena_nodes);
     if (dev->arena_scratch) hipFree(dev->arena_scratch);
     if (dev->arena_done) hipEventDestroy(dev->arena_done);
@@ -427,6 +462,7 @@ void lc_hip_stream_destroy(uint64_t device, uint64_t stream) {
     st->qcv.notify_all();
     if (st->worker.joinable()) st->worker.join();
     pt_release_stream(dev, st->s);
+    stream_scratch_release(dev, st->s);
     hipStreamDestroy(st->s);
     delete st;
 }
@@ -457,12 +493,50 @@ int lc_hip_stream_add_callback(uint64_t device, uint64_t stream, void (*fn)(void
 
 void *lc_hip_stream_native(uint64_t, uint64_t stream) { return native_stream(stream); }
 
-// ---- events: a timeline value maps to the HIP event recorded by the signal of that value ----
+// ---- events: timeline fences. The reference's event is a timeline semaphore
+// (src/backends/cuda/cuda_event.cpp:21-63, src/runtime/event.cpp:49-57): signal(stream, v) sets
+// the value to v when the stream reaches that point, wait(stream, v) holds the stream until the
+// value is at least v, is_completed / synchronize test / wait for that on the host. Here a signal
+// of v records a HIP event on its stream; the timeline has reached v once a signal of a value
+// >= v has completed. A wait or synchronize for a value no signal >= v has been issued for yet
+// blocks the calling thread until another thread issues one (the GPU never spins on a value). ----
 struct Event {
     Device *dev;
     std::mutex mu;
-    std::map<uint64_t, hipEvent_t> signals;
+    std::condition_variable cv;
+    std::map<uint64_t, hipEvent_t> signals; // issued, not yet known complete, by value
+    uint64_t issued = 0;                    // highest value signalled so far
+    uint64_t reached = 0;                   // highest value known complete
+    bool closing = false;
 };
+
+// Moves completed signals into `reached` (caller holds e->mu).
+static void event_poll(Event *e) {
+    for (auto it = e->signals.begin(); it != e->signals.end();) {
+        if (hipEventQuery(it->second) == hipSuccess) {
+            e->reached = std::max(e->reached, it->first);
+            hipEventDestroy(it->second);
+            it = e->signals.erase(it);
+        } else {
+            ++it;
+        }
+    }
+    // signals at or below the reached value add nothing
+    for (auto it = e->signals.begin(); it != e->signals.end() && it->first <= e->reached;) {
+        hipEventDestroy(it->second);
+        it = e->signals.erase(it);
+    }
+}
+
+// Blocks until a signal of a value >= v has been issued; returns the HIP event of the smallest
+// such signal, or null when the timeline has already reached v (caller holds lk on e->mu).
+static hipEvent_t event_await_issue(Event *e, uint64_t v, std::unique_lock<std::mutex> &lk) {
+    e->cv.wait(lk, [&] { return e->closing || e->issued >= v || e->reached >= v; });
+    event_poll(e);
+    if (e->reached >= v) return nullptr;
+    auto it = e->signals.lower_bound(v);
+    return it == e->signals.end() ? nullptr : it->second;
+}
 
 uint64_t lc_hip_event_create(uint64_t device) {
     Device *dev = obj<Device>(device);
@@ -479,10 +553,16 @@ void lc_hip_event_destroy(uint64_t device, uint64_t event) {
     Event *e = obj<Event>(event);
     if (!e || !obj<Device>(device)) return;
     DeviceGuard g(e->dev);
-    for (auto &kv : e->signals) {
-        hipEventSynchronize(kv.second);
-        hipEventDestroy(kv.second);
+    {
+        std::lock_guard<std::mutex> lk(e->mu);
+        e->closing = true;
+        for (auto &kv : e->signals) {
+            hipEventSynchronize(kv.second);
+            hipEventDestroy(kv.second);
+        }
+        e->signals.clear();
     }
+    e->cv.notify_all();
     delete e;
 }
 
@@ -492,45 +572,35 @@ int lc_hip_event_signal(uint64_t device, uint64_t event, uint64_t stream, uint64
     DeviceGuard g(e->dev);
     hipEvent_t h;
     LC_HIP_TRY(hipEventCreateWithFlags(&h, hipEventDisableTiming));
-    LC_HIP_TRY(hipEventRecord(h, native_stream(stream)));
-    std::lock_guard<std::mutex> lk(e->mu);
-    auto it = e->signals.find(value);
-    if (it != e->signals.end()) {
-        hipEventDestroy(it->second);
-        it->second = h;
-    } else {
-        e->signals.emplace(value, h);
-    }
-    // drop fences of older values that have completed
-    for (auto i = e->signals.begin(); i != e->signals.end() && i->first < value;) {
-        if (hipEventQuery(i->second) == hipSuccess) {
-            hipEventDestroy(i->second);
-            i = e->signals.erase(i);
-        } else {
-            ++i;
+    {
+        hipError_t r = hipEventRecord(h, native_stream(stream));
+        if (r != hipSuccess) {
+            hipEventDestroy(h);
+            return hip_fail(r, "hipEventRecord");
         }
     }
-    return LC_HIP_OK;
-}
-
-static hipEvent_t event_for(Event *e, uint64_t value, bool &done) {
-    std::lock_guard<std::mutex> lk(e->mu);
-    auto it = e->signals.lower_bound(value);
-    if (it == e->signals.end()) {
-        // no signal at or above value pending: completed if an older one was pruned
-        done = true;
-        return nullptr;
+    {
+        std::lock_guard<std::mutex> lk(e->mu);
+        auto it = e->signals.find(value);
+        if (it != e->signals.end()) {
+            hipEventDestroy(it->second);
+            it->second = h;
+        } else {
+            e->signals.emplace(value, h);
+        }
+        e->issued = std::max(e->issued, value);
+        event_poll(e);
     }
-    done = false;
-    return it->second;
+    e->cv.notify_all();
+    return LC_HIP_OK;
 }
 
 int lc_hip_event_wait(uint64_t device, uint64_t event, uint64_t stream, uint64_t value) {
     Event *e = obj<Event>(event);
     if (!e || !obj<Device>(device)) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad event");
     DeviceGuard g(e->dev);
-    bool done;
-    hipEvent_t h = event_for(e, value, done);
+    std::unique_lock<std::mutex> lk(e->mu);
+    hipEvent_t h = event_await_issue(e, value, lk);
     if (h) LC_HIP_TRY(hipStreamWaitEvent(native_stream(stream), h, 0));
     return LC_HIP_OK;
 }
@@ -539,9 +609,11 @@ int lc_hip_event_synchronize(uint64_t device, uint64_t event, uint64_t value) {
     Event *e = obj<Event>(event);
     if (!e || !obj<Device>(device)) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad event");
     DeviceGuard g(e->dev);
-    bool done;
-    hipEvent_t h = event_for(e, value, done);
-    if (h) LC_HIP_TRY(hipEventSynchronize(h));
+    std::unique_lock<std::mutex> lk(e->mu);
+    hipEvent_t h = event_await_issue(e, value, lk);
+    if (!h) return LC_HIP_OK;
+    LC_HIP_TRY(hipEventSynchronize(h));
+    event_poll(e);
     return LC_HIP_OK;
 }
 
@@ -549,10 +621,9 @@ int lc_hip_event_is_completed(uint64_t device, uint64_t event, uint64_t value) {
     Event *e = obj<Event>(event);
     if (!e || !obj<Device>(device)) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad event");
     DeviceGuard g(e->dev);
-    bool done;
-    hipEvent_t h = event_for(e, value, done);
-    if (!h) return 1;
-    return hipEventQuery(h) == hipSuccess ? 1 : 0;
+    std::lock_guard<std::mutex> lk(e->mu);
+    event_poll(e);
+    return e->reached >= value ? 1 : 0;
 }
 
 // ---- meshes ----
@@ -648,16 +719,18 @@ int lc_hip_procedural_build(uint64_t device, uint64_t stream, uint64_t primitive
         std::lock_guard<std::mutex> dl(dev->mu);
         BuildTarget t{m->nodes, m->scratch, m->scratch_bytes};
         if (m->allow_compaction) LC_HIP_TRY(arena_acquire(dev, m, n, t));
-        LC_HIP_TRY(hipMemsetAsync(dev->error, 0, sizeof(uint32_t), s));
+        StreamScratch *S = stream_scratch(dev, s);
+        if (!S) return fail(LC_HIP_ERROR_OUT_OF_MEMORY, "stream scratch");
+        LC_HIP_TRY(hipMemsetAsync(S->error, 0, sizeof(uint32_t), s));
         TimedRegion tr(dev, s, "procedural_build");
         const float *aabb = reinterpret_cast<const float *>(static_cast<const char *>(ab->ptr) + aabb_offset);
         LC_HIP_TRY(aabbs_to_boxes(s, aabb, n, m->proc_boxes));
         BuildCounters *ctr = nullptr;
         hipError_t e = bvh_build(s, t.scratch, t.scratch_bytes, nullptr, 0, 0, nullptr, m->proc_boxes, n, t.nodes,
-                                 nullptr, m->proc_ids, &ctr, 0, 1u, dev->cu_count, dev->error, &m->host->phase_count);
+                                 nullptr, m->proc_ids, &ctr, 0, 1u, dev->cu_count, S->error, &m->host->phase_count);
         if (e != hipSuccess) return hip_fail(e, "procedural bvh_build");
         LC_HIP_TRY(proc_records(s, m->proc_ids, m->proc_boxes, n, m->proc));
-        copy_counters_async(ctr, m->host, dev->error, s);
+        copy_counters_async(ctr, m->host, S->error, s);
         if (m->allow_compaction) {
             const int rc = arena_compact(dev, m, s);
             if (rc != LC_HIP_OK) return rc;
@@ -710,13 +783,15 @@ int lc_hip_mesh_build(uint64_t device, uint64_t stream, uint64_t mesh, uint32_t 
         m->build_epoch++;
         {
             std::lock_guard<std::mutex> dl(dev->mu);
-            LC_HIP_TRY(hipMemsetAsync(dev->error, 0, sizeof(uint32_t), s));
+            StreamScratch *S = stream_scratch(dev, s);
+            if (!S) return fail(LC_HIP_ERROR_OUT_OF_MEMORY, "stream scratch");
+            LC_HIP_TRY(hipMemsetAsync(S->error, 0, sizeof(uint32_t), s));
             TimedRegion tr(dev, s, "mesh_refit");
             BuildCounters *ctr = nullptr;
             hipError_t e = bvh_refit(s, m->scratch, m->scratch_bytes, vaddr, vertex_stride, nverts, taddr, n, depth,
-                                     m->nodes, m->tris, &ctr, dev->cu_count, dev->error);
+                                     m->nodes, m->tris, &ctr, dev->cu_count, S->error);
             if (e != hipSuccess) return hip_fail(e, "bvh_refit");
-            copy_counters_async(ctr, m->host, dev->error, s);
+            copy_counters_async(ctr, m->host, S->error, s);
         }
         m->refits++;
         LC_HIP_TRY(hipEventRecord(m->ready, s));
@@ -742,14 +817,16 @@ int lc_hip_mesh_build(uint64_t device, uint64_t stream, uint64_t mesh, uint32_t 
         std::lock_guard<std::mutex> dl(dev->mu);
         BuildTarget t{m->nodes, m->scratch, m->scratch_bytes};
         if (m->allow_compaction) LC_HIP_TRY(arena_acquire(dev, m, n, t));
-        LC_HIP_TRY(hipMemsetAsync(dev->error, 0, sizeof(uint32_t), s));
+        StreamScratch *S = stream_scratch(dev, s);
+        if (!S) return fail(LC_HIP_ERROR_OUT_OF_MEMORY, "stream scratch");
+        LC_HIP_TRY(hipMemsetAsync(S->error, 0, sizeof(uint32_t), s));
         TimedRegion tr(dev, s, "mesh_build");
         BuildCounters *ctr = nullptr;
         hipError_t e = bvh_build(s, t.scratch, t.scratch_bytes, vaddr, vertex_stride, nverts, taddr, nullptr, n,
                                  t.nodes, m->tris, nullptr, &ctr, dev->split_small, leaf_size_for(m->hint),
-                                 dev->cu_count, dev->error, &m->host->phase_count);
+                                 dev->cu_count, S->error, &m->host->phase_count);
         if (e != hipSuccess) return hip_fail(e, "bvh_build");
-        copy_counters_async(ctr, m->host, dev->error, s);
+        copy_counters_async(ctr, m->host, S->error, s);
         if (m->allow_compaction) {
             const int rc = arena_compact(dev, m, s);
             if (rc != LC_HIP_OK) return rc;
@@ -849,7 +926,8 @@ uint64_t lc_hip_accel_create(uint64_t device, const lc_hip_accel_option *option)
         a->allow_update = option->allow_update != 0;
     }
     if (hipHostMalloc((void **)&a->tlas_host, sizeof(HostCounters), hipHostMallocDefault) != hipSuccess ||
-        hipEventCreateWithFlags(&a->tlas_ready, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&a->tlas_ready, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&a->staged, hipEventDisableTiming) != hipSuccess) {
         fail(LC_HIP_ERROR_OUT_OF_MEMORY, "accel host counters");
         return LC_HIP_INVALID_HANDLE;
     }
@@ -869,6 +947,8 @@ void lc_hip_accel_destroy(uint64_t device, uint64_t accel) {
     if (a->tlas_scratch) hipFree(a->tlas_scratch);
     if (a->tlas_host) hipHostFree(a->tlas_host);
     if (a->tlas_ready) hipEventDestroy(a->tlas_ready);
+    if (a->staged) hipEventDestroy(a->staged);
+    if (a->stage) hipHostFree(a->stage);
     if (a->state) hipFree(a->state);
     if (a->desc) hipFree(a->desc);
     delete a;
@@ -897,6 +977,25 @@ static int fold_instance_writes(Accel *a) {
 static void xform_point_h(const float m[12], const float p[3], float out[3]) {
     for (int r = 0; r < 3; r++) out[r] = m[4 * r + 0] * p[0] + m[4 * r + 1] * p[1] + m[4 * r + 2] * p[2] + m[4 * r + 3];
 }
+
+// Pinned staging of at least `bytes` for a build's uploads, after the previous build's copies
+// out of it have run.
+static int stage_reserve(Accel *a, size_t bytes) {
+    if (a->stage_pending) {
+        LC_HIP_TRY(hipEventSynchronize(a->staged));
+        a->stage_pending = false;
+    }
+    if (a->stage_cap >= bytes) return LC_HIP_OK;
+    if (a->stage) hipHostFree(a->stage);
+    a->stage = nullptr;
+    a->stage_cap = 0;
+    const size_t cap = std::max<size_t>(bytes, 4096);
+    LC_HIP_TRY(hipHostMalloc((void **)&a->stage, cap, hipHostMallocDefault));
+    a->stage_cap = cap;
+    return LC_HIP_OK;
+}
+
+static size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
 // The request is not consulted: an update rebuilds the TLAS over the instance boxes (a few
 // thousand instances at most; the boxes move with the transforms anyway).
@@ -963,44 +1062,60 @@ int lc_hip_accel_build(uint64_t device, uint64_t stream, uint64_t accel, uint32_
         }
     }
     a->max_blas_depth = max_depth;
+    // staging layout: records | instance table | TLAS boxes | descriptor
+    const size_t rec_bytes = (size_t)instance_count * sizeof(InstanceRecord);
+    const size_t state_bytes = ((size_t)instance_count + 1) * sizeof(InstanceState);
+    const size_t box_bytes = 2 * sizeof(float4) * (size_t)instance_count;
+    const size_t off_state = align_up(rec_bytes, 256);
+    const size_t off_box = off_state + align_up(state_bytes, 256);
+    const size_t off_desc = off_box + align_up(box_bytes, 256);
+    {
+        const int rc = stage_reserve(a, off_desc + sizeof(DeviceAccel));
+        if (rc != LC_HIP_OK) return rc;
+    }
     void *p = a->records;
     size_t cap = a->record_cap * sizeof(InstanceRecord);
     LC_HIP_TRY(ensure(p, cap, std::max<size_t>(instance_count, 1) * sizeof(InstanceRecord)));
     a->records = static_cast<InstanceRecord *>(p);
     a->record_cap = cap / sizeof(InstanceRecord);
     if (instance_count) {
-        LC_HIP_TRY(hipMemcpyAsync(a->records, a->host_records.data(), instance_count * sizeof(InstanceRecord),
-                                  hipMemcpyHostToDevice, s));
+        std::memcpy(a->stage, a->host_records.data(), rec_bytes);
+        LC_HIP_TRY(hipMemcpyAsync(a->records, a->stage, rec_bytes, hipMemcpyHostToDevice, s));
     }
     // the device-side instance table the in-kernel instance ops see (state[count] = error word)
     {
         void *sp = a->state;
         size_t scap = a->state_cap * sizeof(InstanceState);
-        LC_HIP_TRY(ensure(sp, scap, ((size_t)instance_count + 1) * sizeof(InstanceState)));
+        LC_HIP_TRY(ensure(sp, scap, state_bytes));
         a->state = static_cast<InstanceState *>(sp);
         a->state_cap = scap / sizeof(InstanceState);
-        a->state_host.assign((size_t)instance_count + 1, InstanceState{});
+        InstanceState *st = reinterpret_cast<InstanceState *>(a->stage + off_state);
+        for (uint32_t i = 0; i <= instance_count; i++) st[i] = InstanceState{};
         for (uint32_t i = 0; i < instance_count; i++) {
-            std::memcpy(a->state_host[i].affine, a->inst[i].affine, sizeof(float) * 12);
-            a->state_host[i].user_id = a->inst[i].user_id;
-            a->state_host[i].vis_mask = a->inst[i].vis_mask;
+            std::memcpy(st[i].affine, a->inst[i].affine, sizeof(float) * 12);
+            st[i].user_id = a->inst[i].user_id;
+            st[i].vis_mask = a->inst[i].vis_mask;
         }
-        LC_HIP_TRY(hipMemcpyAsync(a->state, a->state_host.data(), a->state_host.size() * sizeof(InstanceState),
-                                  hipMemcpyHostToDevice, s));
+        LC_HIP_TRY(hipMemcpyAsync(a->state, st, state_bytes, hipMemcpyHostToDevice, s));
         a->state_count = instance_count;
         a->state_written = false;
     }
-    if (update_instance_buffer_only) {
-        const int rc = accel_write_descriptor(a, s);
+    // update_instance_buffer_only (command.h:531-616) refreshes the instance records over the TLAS
+    // of the last build; when the instance count differs from that build's (or nothing was built)
+    // that TLAS does not describe the instances, so the accel is built in full instead
+    if (update_instance_buffer_only && a->built && instance_count == a->tlas_count) {
+        const int rc = accel_write_descriptor(a, s, a->stage + off_desc);
         if (rc != LC_HIP_OK) return rc;
-        LC_HIP_TRY(hipStreamSynchronize(s)); // host_records is reused by the next build
+        LC_HIP_TRY(hipEventRecord(a->staged, s));
+        a->stage_pending = true;
         return LC_HIP_OK;
     }
     a->built = true;
+    a->tlas_count = instance_count;
     a->tlas_valid = false;
     a->tlas_stats_valid = false;
     // world boxes of instances: 8 transformed BLAS-box corners, padded by a relative epsilon
-    std::vector<float> boxes(8 * (size_t)instance_count);
+    float *boxes = reinterpret_cast<float *>(a->stage + off_box);
     {
         float wlo[3] = {3e38f, 3e38f, 3e38f}, whi[3] = {-3e38f, -3e38f, -3e38f};
         for (uint32_t i = 0; i < instance_count; i++) {
@@ -1035,9 +1150,9 @@ int lc_hip_accel_build(uint64_t device, uint64_t stream, uint64_t accel, uint32_
     }
     if (instance_count > 1) {
         p = a->boxes; cap = a->box_cap;
-        LC_HIP_TRY(ensure(p, cap, boxes.size() * sizeof(float)));
+        LC_HIP_TRY(ensure(p, cap, box_bytes));
         a->boxes = static_cast<float4 *>(p); a->box_cap = cap;
-        LC_HIP_TRY(hipMemcpyAsync(a->boxes, boxes.data(), boxes.size() * sizeof(float), hipMemcpyHostToDevice, s));
+        LC_HIP_TRY(hipMemcpyAsync(a->boxes, boxes, box_bytes, hipMemcpyHostToDevice, s));
         size_t ncap = bvh_node_capacity(instance_count);
         if (a->tlas_cap < ncap) {
             if (a->tlas_nodes) hipFree(a->tlas_nodes);
@@ -1052,25 +1167,28 @@ int lc_hip_accel_build(uint64_t device, uint64_t stream, uint64_t accel, uint32_
         LC_HIP_TRY(ensure(a->tlas_scratch, a->tlas_scratch_bytes, bvh_scratch_bytes(instance_count)));
         {
             std::lock_guard<std::mutex> dl(dev->mu);
-            LC_HIP_TRY(hipMemsetAsync(dev->error, 0, sizeof(uint32_t), s));
+            StreamScratch *S = stream_scratch(dev, s);
+            if (!S) return fail(LC_HIP_ERROR_OUT_OF_MEMORY, "stream scratch");
+            LC_HIP_TRY(hipMemsetAsync(S->error, 0, sizeof(uint32_t), s));
             TimedRegion tr(dev, s, "accel_build");
             BuildCounters *ctr = nullptr;
             hipError_t e = bvh_build(s, a->tlas_scratch, a->tlas_scratch_bytes, nullptr, 0, 0, nullptr, a->boxes,
                                      instance_count, a->tlas_nodes, nullptr, a->tlas_prims, &ctr, 0,
-                                     leaf_size_for(a->hint), std::max(1, dev->cu_count), dev->error,
+                                     leaf_size_for(a->hint), std::max(1, dev->cu_count), S->error,
                                      &a->tlas_host->phase_count);
             if (e != hipSuccess) return hip_fail(e, "tlas build");
-            copy_counters_async(ctr, a->tlas_host, dev->error, s);
+            copy_counters_async(ctr, a->tlas_host, S->error, s);
         }
         LC_HIP_TRY(hipEventRecord(a->tlas_ready, s));
         a->tlas_valid = true;
     }
     {
-        const int rc = accel_write_descriptor(a, s);
+        const int rc = accel_write_descriptor(a, s, a->stage + off_desc);
         if (rc != LC_HIP_OK) return rc;
     }
-    // host_records / boxes / descriptor staging must outlive the async copies
-    LC_HIP_TRY(hipStreamSynchronize(s));
+    // the staging is reused by the next build once these copies ran (no host wait here)
+    LC_HIP_TRY(hipEventRecord(a->staged, s));
+    a->stage_pending = true;
     return LC_HIP_OK;
 }
 
@@ -1078,7 +1196,7 @@ int lc_hip_accel_build(uint64_t device, uint64_t stream, uint64_t accel, uint32_
 
 // The DeviceAccel descriptor of lc_hip_device.h: the query view of the accel (as
 // lc_hip_accel_query builds it) plus the instance table.
-int lchip::accel_write_descriptor(Accel *a, hipStream_t s) {
+int lchip::accel_write_descriptor(Accel *a, hipStream_t s, void *staging) {
     DeviceAccel &d = a->desc_host;
     std::memset(&d, 0, sizeof(d));
     const uint32_t n = (uint32_t)a->inst.size();
@@ -1097,7 +1215,8 @@ int lchip::accel_write_descriptor(Accel *a, hipStream_t s) {
     d.state_count = a->state_count;
     d.error = a->dev->error;
     if (!a->desc) LC_HIP_TRY(hipMalloc(&a->desc, sizeof(DeviceAccel)));
-    LC_HIP_TRY(hipMemcpyAsync(a->desc, &d, sizeof(d), hipMemcpyHostToDevice, s));
+    std::memcpy(staging, &d, sizeof(d));
+    LC_HIP_TRY(hipMemcpyAsync(a->desc, staging, sizeof(d), hipMemcpyHostToDevice, s));
     return LC_HIP_OK;
 }
 
@@ -1257,27 +1376,32 @@ static int trace_launch_batches(Device *dev, Accel *a, hipStream_t s, const lc_h
     if (grc != LC_HIP_OK) return grc;
     std::lock_guard<std::mutex> dl(dev->mu);
     const TraceConfig &cfg = dev->cfg[any ? 1 : 0];
-    int bpc = cfg.blocks_per_cu > 0 ? cfg.blocks_per_cu : trace_blocks_per_cu(any, two, counting, cfg.block, cfg.slds);
+    // the counting form runs the timed form's grid (same waves, same queue hand-out)
+    int bpc = cfg.blocks_per_cu > 0 ? cfg.blocks_per_cu : trace_blocks_per_cu(any, two, false, cfg.block, cfg.slds);
     uint64_t want = (total + cfg.block - 1) / cfg.block;
     int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)bpc * dev->cu_count, want));
     P.spill_depth = stack_need > (uint32_t)cfg.slds ? stack_need - (uint32_t)cfg.slds : 0u;
     size_t spill_bytes = (size_t)grid * cfg.block * P.spill_depth * sizeof(uint2);
-    LC_HIP_TRY(ensure(dev->spill, dev->spill_bytes, spill_bytes));
-    P.spill = static_cast<uint2 *>(dev->spill);
-    P.ray_counter = dev->ray_counter;
-    P.error = dev->error;
-    P.stats = dev->stats;
+    StreamScratch *S = stream_scratch(dev, s);
+    if (!S) return fail(LC_HIP_ERROR_OUT_OF_MEMORY, "stream scratch");
+    LC_HIP_TRY(ensure(S->spill, S->spill_bytes, spill_bytes));
+    P.spill = static_cast<uint2 *>(S->spill);
+    P.ray_counter = S->ray_counter;
+    P.error = S->error;
+    P.stats = S->stats;
     P.steps = cfg.steps;
     P.fetch_chunk = cfg.fetch_chunk;
     P.fetch_segments = cfg.fetch_segments;
-    if (counting) LC_HIP_TRY(hipMemsetAsync(dev->stats, 0, 2 * sizeof(unsigned long long), s));
-    const bool sort = cfg.sort_rays && !counting && total > 1 && !P.geo.empty; // per-ray results do not depend on order
-    if (sort) LC_HIP_TRY(ensure(dev->sort_scratch, dev->sort_bytes, ray_sort_scratch_bytes(P.count)));
+    if (counting) LC_HIP_TRY(hipMemsetAsync(S->stats, 0, 2 * sizeof(unsigned long long), s));
+    // per-ray results do not depend on the order; the counting form traverses the same sorted
+    // queue as the timed form, so its node / triangle totals describe that launch
+    const bool sort = cfg.sort_rays && total > 1 && !P.geo.empty;
+    if (sort) LC_HIP_TRY(ensure(S->sort, S->sort_bytes, ray_sort_scratch_bytes(P.count)));
     {
         // "trace_*": the whole entry point (sort + traversal); "trace_*_kernel": traversal only
         TimedRegion tr(dev, s, counting ? "trace_count" : (any ? "trace_any" : "trace_closest"));
         if (sort) {
-            hipError_t e = ray_sort(s, P.batches, P.count, a->world_bounds, dev->sort_scratch, dev->sort_bytes, &P.order);
+            hipError_t e = ray_sort(s, P.batches, P.count, a->world_bounds, S->sort, S->sort_bytes, &P.order);
             if (e != hipSuccess) return hip_fail(e, "ray sort");
         }
         TimedRegion tk(dev, s, counting ? "trace_count_kernel" : (any ? "trace_any_kernel" : "trace_closest_kernel"));
@@ -1286,7 +1410,7 @@ static int trace_launch_batches(Device *dev, Accel *a, hipStream_t s, const lc_h
     }
     if (counting) {
         unsigned long long st[2] = {0, 0};
-        LC_HIP_TRY(hipMemcpyAsync(st, dev->stats, sizeof(st), hipMemcpyDeviceToHost, s));
+        LC_HIP_TRY(hipMemcpyAsync(st, S->stats, sizeof(st), hipMemcpyDeviceToHost, s));
         LC_HIP_TRY(hipStreamSynchronize(s));
         if (n_nodes) *n_nodes = st[0];
         if (n_tris) *n_tris = st[1];
@@ -1405,14 +1529,16 @@ int lc_hip_accel_query(uint64_t device, uint64_t stream, uint64_t accel, uint64_
         1, std::min<uint64_t>((uint64_t)query_blocks_per_cu() * std::max(1, dev->cu_count), want));
     const uint32_t slds = (uint32_t)query_lds_entries();
     P.spill_depth = stack_need > slds ? stack_need - slds : 0u;
-    LC_HIP_TRY(ensure(dev->spill, dev->spill_bytes, (size_t)grid * block * P.spill_depth * sizeof(uint2)));
-    P.spill = static_cast<uint2 *>(dev->spill);
-    P.error = dev->error;
-    P.counter = dev->ray_counter;
+    StreamScratch *S = stream_scratch(dev, s);
+    if (!S) return fail(LC_HIP_ERROR_OUT_OF_MEMORY, "stream scratch");
+    LC_HIP_TRY(ensure(S->spill, S->spill_bytes, (size_t)grid * block * P.spill_depth * sizeof(uint2)));
+    P.spill = static_cast<uint2 *>(S->spill);
+    P.error = S->error;
+    P.counter = S->ray_counter;
     // the query queue is sorted like trace queues when ray sorting is on (per-ray answers of the
     // nearest-commit query do not depend on the order)
     const bool sort = dev->cfg[P.first ? 1 : 0].sort_rays && ray_count > 1 && !P.geo.empty;
-    if (sort) LC_HIP_TRY(ensure(dev->sort_scratch, dev->sort_bytes, ray_sort_scratch_bytes(ray_count)));
+    if (sort) LC_HIP_TRY(ensure(S->sort, S->sort_bytes, ray_sort_scratch_bytes(ray_count)));
     TimedRegion tr(dev, s, P.first ? "query_any" : "query");
     if (sort) {
         BatchTable bt;
@@ -1420,7 +1546,7 @@ int lc_hip_accel_query(uint64_t device, uint64_t stream, uint64_t accel, uint64_
         bt.n = 1;
         bt.b[0].rays = P.rays;
         bt.b[0].mask = vis_mask;
-        hipError_t e = ray_sort(s, bt, ray_count, a->world_bounds, dev->sort_scratch, dev->sort_bytes, &P.order);
+        hipError_t e = ray_sort(s, bt, ray_count, a->world_bounds, S->sort, S->sort_bytes, &P.order);
         if (e != hipSuccess) return hip_fail(e, "query ray sort");
     }
     hipError_t e = query_launch(P, s, grid);

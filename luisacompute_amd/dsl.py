@@ -90,6 +90,21 @@ _PRELUDE = r'''
 #include "lc_hip_device.h"
 namespace lcdsl {
 using namespace lchip;
+// Python '/' converts both operands to float first (src/py/luisa/builtin.py:191-194); '//' is the
+// C division of the operand types.
+__device__ __forceinline__ float as_real(int32_t x) { return (float)x; }
+__device__ __forceinline__ float as_real(uint32_t x) { return (float)x; }
+__device__ __forceinline__ float as_real(int64_t x) { return (float)x; }
+__device__ __forceinline__ float as_real(uint64_t x) { return (float)x; }
+__device__ __forceinline__ float as_real(float x) { return x; }
+__device__ __forceinline__ float2 as_real(float2 x) { return x; }
+__device__ __forceinline__ float3 as_real(float3 x) { return x; }
+__device__ __forceinline__ float4 as_real(float4 x) { return x; }
+__device__ __forceinline__ float2 as_real(int2 x) { return make_float2((float)x.x, (float)x.y); }
+__device__ __forceinline__ float3 as_real(int3 x) { return make_float3((float)x.x, (float)x.y, (float)x.z); }
+__device__ __forceinline__ float2 as_real(uint2 x) { return make_float2((float)x.x, (float)x.y); }
+__device__ __forceinline__ float3 as_real(uint3 x) { return make_float3((float)x.x, (float)x.y, (float)x.z); }
+template<class A, class B> __device__ __forceinline__ auto true_div(A a, B b) { return as_real(a) / as_real(b); }
 __device__ __forceinline__ float3 v3f(rtx::V3 v) { return make_float3(v.x, v.y, v.z); }
 __device__ __forceinline__ rtx::V3 fv3(float3 v) { return rtx::v3(v.x, v.y, v.z); }
 __device__ __forceinline__ float dot(float3 a, float3 b) { return rtx::dot(fv3(a), fv3(b)); }
@@ -145,7 +160,7 @@ _FUNCS = {"float2": "make_float2", "float3": "make_float3", "float4": "make_floa
           "hit_bary": "lcdsl::hit_bary", "select": "lcdsl::select", "uint": "(uint32_t)", "int": "(int32_t)",
           "float": "(float)", "ulong": "(uint64_t)", "isnan": "isnan"}
 
-_BINOPS = {ast.Add: "+", ast.Sub: "-", ast.Mult: "*", ast.Div: "/", ast.Mod: "%", ast.BitAnd: "&", ast.BitOr: "|",
+_BINOPS = {ast.Add: "+", ast.Sub: "-", ast.Mult: "*", ast.FloorDiv: "/", ast.Mod: "%", ast.BitAnd: "&", ast.BitOr: "|",
            ast.BitXor: "^", ast.LShift: "<<", ast.RShift: ">>"}
 _CMPOPS = {ast.Eq: "==", ast.NotEq: "!=", ast.Lt: "<", ast.LtE: "<=", ast.Gt: ">", ast.GtE: ">="}
 
@@ -197,6 +212,10 @@ class _Codegen(ast.NodeVisitor):
             self.error(node, "unsupported assignment target")
 
     def visit_AugAssign(self, node):
+        if isinstance(node.op, ast.Div):
+            t = self.expr(node.target)
+            self.emit(f"{t} = true_div({t}, {self.expr(node.value)});")
+            return
         op = _BINOPS.get(type(node.op))
         if op is None:
             self.error(node, "unsupported operator")
@@ -293,6 +312,8 @@ class _Codegen(ast.NodeVisitor):
                 return self.expr(ast.Constant(self.constants[node.id]))
             return node.id
         if isinstance(node, ast.BinOp):
+            if isinstance(node.op, ast.Div):
+                return f"true_div({self.expr(node.left)}, {self.expr(node.right)})"
             op = _BINOPS.get(type(node.op))
             if op is None:
                 self.error(node, "unsupported operator")

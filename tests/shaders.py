@@ -1,5 +1,6 @@
 """Shader sources of the run-time shader tests: the kernels a DSL codegen would emit for the
 reference's ray-tracing programs, written against the device library (lc_hip_device.h)."""
+from luisacompute_amd.programs import QUERY  # noqa: F401  (the benchmark's query shader)
 
 # hit = accel.intersect(ray, {mask}) / hit_any = accel.intersect_any(ray, {mask}) per thread
 # (src/dsl/rtx/accel.cpp:136-150)
@@ -20,47 +21,6 @@ extern "C" __global__ void __launch_bounds__(128) trace_kernel(const Params p) {
     const dev::Ray r = p.rays[i];
     p.hits[i] = dev::trace_closest(p.accel, r, p.mask);
     p.any[i] = dev::trace_any(p.accel, r, p.mask) ? 1u : 0u;
-}
-'''
-
-# accel.traverse(ray, {}).on_surface_candidate(...).on_procedural_candidate(...).trace() with the
-# callbacks of src/tests/test_procedural.cpp:86-120 written as the codegen would inline them
-QUERY = r'''
-#include "lc_hip_device.h"
-using namespace lchip;
-struct Params {
-    alignas(16) dev::Buffer<dev::Ray> rays;
-    alignas(16) dev::Buffer<rq::CommittedHit> out;
-    alignas(16) dev::Accel accel;
-    alignas(16) float cutout;
-    alignas(16) uint32_t any;
-    alignas(16) uint4 ls_kid;
-};
-extern "C" __global__ void __launch_bounds__(128) query_kernel(const Params p) {
-    if (!dev::in_dispatch(p.ls_kid)) return;
-    const uint32_t i = dev::dispatch_id().x;
-    const float cutout = p.cutout;
-    auto on_surface = [cutout](uint32_t, uint32_t, float u, float v, float) {
-        const float w = (1.0f - u) - v;
-        return sqrtf(w * w + u * u) < cutout && sqrtf(u * u + v * v) < cutout && sqrtf(v * v + w * w) < cutout;
-    };
-    auto on_procedural = [](uint32_t, uint32_t, const float *o, const float *d, const float *, const float *,
-                            float4 lo, float4 hi, float, float &t) {
-        const float c[3] = {(lo.x + hi.x) * 0.5f, (lo.y + hi.y) * 0.5f, (lo.z + hi.z) * 0.5f};
-        const float r = 0.5f * fminf(fminf(hi.x - lo.x, hi.y - lo.y), hi.z - lo.z);
-        const float L[3] = {c[0] - o[0], c[1] - o[1], c[2] - o[2]};
-        const float dd = (d[0] * d[0] + d[1] * d[1]) + d[2] * d[2];
-        const float ld = (L[0] * d[0] + L[1] * d[1]) + L[2] * d[2];
-        const float ll = (L[0] * L[0] + L[1] * L[1]) + L[2] * L[2];
-        const float tc = ld / dd;
-        if (!(tc > 0.0f)) return false;
-        const float q = ll - (tc * tc) * dd;
-        const float rr = r * r;
-        if (!(q <= rr)) return false;
-        t = tc - sqrtf((rr - q) / dd);
-        return true;
-    };
-    p.out[i] = dev::query(p.accel, p.rays[i], 0xFFu, p.any != 0u, on_surface, on_procedural);
 }
 '''
 

@@ -23,3 +23,9 @@ def test_dsl_errors():
         @dsl.kernel
         def bad2(out: dsl.Buffer("float")):
             out.write(0, frobnicate(1.0))
+
+
+def test_dsl_true_division():
+    src = dsl_kernels.division_kernel.source
+    assert "true_div(" in src
+    assert lc.compile_shader_source(src) > 1000

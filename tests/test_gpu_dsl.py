@@ -144,3 +144,17 @@ def test_dsl_path_tracer_equals_the_library_path_tracer(device, stream):
     print(f"[dsl path tracer] {n_closest} closest + {n_shadow} shadow rays; mean {got[..., :3].mean():.4f}", flush=True)
     assert got[..., :3].mean() > 0.01
     np.testing.assert_array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+def test_dsl_integer_division(device, stream):
+    """'/' of two unsigned integers is float division, '//' truncating integer division."""
+    n, k = 1000, 7
+    kern = dsl_kernels.division_kernel.compile(device)
+    a, b = device.create_buffer(4, n), device.create_buffer(4, n)
+    stream << kern(a, b, k).dispatch(n)
+    ft, fl = np.zeros(n, np.float32), np.zeros(n, np.uint32)
+    stream << a.copy_to(ft) << b.copy_to(fl) << lc.synchronize()
+    i = np.arange(n, dtype=np.uint32)
+    np.testing.assert_array_equal(ft, i.astype(np.float32) / np.float32(k))
+    np.testing.assert_array_equal(fl, i // k)
+    assert ft[3] != 0.0  # 3 / 7 does not truncate to 0
