@@ -1,5 +1,6 @@
 /*
- * lc_hip.h -- C ABI of the MI355X (gfx950) ray-tracing backend, library lc-backend-hip.so.
+ * lc_hip.h -- C ABI of the MI355X (gfx950) ray-tracing backend, library libluisa_compute_backend_hip.so.
+ * The reference's own pure-C backend entry point (luisa_compute_lib_interface) is in lc_api.h.
  *
  * Plain C: handles are uint64_t (the backend object pointer, as in the reference where a
  * resource handle is the backend object's address), sizes are size_t, buffers are device
@@ -127,6 +128,10 @@ int lc_hip_device_query(uint64_t device, const char *key, char *out, size_t capa
    BufferCreationInfo {handle, native_handle, element_stride, total_size_bytes} ---- */
 uint64_t lc_hip_buffer_create(uint64_t device, size_t element_stride, size_t element_count, void **native_handle);
 void lc_hip_buffer_destroy(uint64_t device, uint64_t buffer);
+/* create_buffer with external memory (device_interface.h:116-118, the `external_memory` argument):
+   a buffer over existing device memory of element_stride * element_count bytes, which the
+   caller keeps alive; destroy releases the handle only. */
+uint64_t lc_hip_buffer_create_external(uint64_t device, size_t element_stride, size_t element_count, void *device_ptr);
 /* BufferUploadCommand / BufferDownloadCommand / BufferCopyCommand (command.h:128-233). The
    upload source is consumed before the call returns; a download destination is valid after the
    stream is synchronized. */
@@ -346,6 +351,14 @@ uint64_t lc_hip_volume_create(uint64_t device, uint32_t storage, uint32_t width,
 void lc_hip_texture_destroy(uint64_t device, uint64_t texture);
 int lc_hip_texture_upload(uint64_t device, uint64_t stream, uint64_t texture, const void *data);
 int lc_hip_texture_download(uint64_t device, uint64_t stream, uint64_t texture, void *data);
+/* BufferToTextureCopyCommand / TextureToBufferCopyCommand (command.h:235-301) of the whole image:
+   its pixels, tightly packed, at buffer_offset; to_texture != 0 copies buffer -> image. */
+int lc_hip_texture_buffer_copy(uint64_t device, uint64_t stream, uint64_t texture, uint64_t buffer, size_t buffer_offset,
+                               int to_texture);
+/* TextureCopyCommand (command.h:303-336) of a whole image onto one of the same size and pixel size. */
+int lc_hip_texture_copy(uint64_t device, uint64_t stream, uint64_t src, uint64_t dst);
+/* Storage (PixelStorage) and size (width, height, depth) of an image. */
+int lc_hip_texture_info(uint64_t device, uint64_t texture, uint32_t *storage, uint32_t size[3]);
 
 /* ---- bindless arrays: create_bindless_array / destroy_bindless_array (device_interface.h) and
    BindlessArrayUpdateCommand (command.h:617-680): a buffer slot holds {pointer at the buffer

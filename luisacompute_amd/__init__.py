@@ -1,7 +1,7 @@
 """luisacompute_amd -- MI355X (gfx950) ray-tracing backend for the LuisaCompute programming model.
 
 Python host mirror of the reference runtime's ray-tracing path over the C ABI of
-lc-backend-hip.so (include/lc_hip.h). Names and argument meaning follow the reference:
+libluisa_compute_backend_hip.so (include/lc_hip.h). Names and argument meaning follow the reference:
 
     reference (C++)                                   here
     Context::create_device("cuda")                   Device() / create_device("hip")
@@ -25,7 +25,7 @@ import os
 import numpy as np
 
 _PKG = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_PKG, "lc-backend-hip.so")
+LIB_PATH = os.path.join(_PKG, "libluisa_compute_backend_hip.so")
 INVALID = 0xFFFFFFFFFFFFFFFF
 
 RAY_DTYPE = np.dtype([("origin", "<f4", (3,)), ("t_min", "<f4"), ("direction", "<f4", (3,)), ("t_max", "<f4")],
@@ -281,7 +281,7 @@ EXPORTED_SYMBOLS = sorted(_SIGNATURES)
 
 
 def lib():
-    """Loads lc-backend-hip.so (raises if it has not been built: no fallback exists)."""
+    """Loads libluisa_compute_backend_hip.so (raises if it has not been built: no fallback exists)."""
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):

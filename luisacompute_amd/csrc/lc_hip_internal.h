@@ -294,6 +294,7 @@ struct Buffer {
     void *ptr = nullptr;
     size_t bytes = 0;
     size_t elem_stride = 0;
+    bool owned = true; // false: a view of external device memory (not freed by destroy)
 };
 
 struct Stream {

@@ -1,4 +1,4 @@
-// runtime.cpp -- host side of lc-backend-hip.so: devices, buffers, streams, events, mesh/accel
+// runtime.cpp -- host side of libluisa_compute_backend_hip.so: devices, buffers, streams, events, mesh/accel
 // builds and trace launches behind the C ABI of include/lc_hip.h.
 //
 // Structure follows the reference's CUDA backend (CUDADevice / CUDAStream / CUDAMesh /
@@ -384,12 +384,34 @@ uint64_t lc_hip_buffer_create(uint64_t device, size_t element_stride, size_t ele
     return handle(b.release());
 }
 
+uint64_t lc_hip_buffer_create_external(uint64_t device, size_t element_stride, size_t element_count, void *device_ptr) {
+    Device *dev = obj<Device>(device);
+    if (!dev || element_stride == 0 || !device_ptr) {
+        fail(LC_HIP_ERROR_INVALID_ARGUMENT, "bad external buffer arguments");
+        return LC_HIP_INVALID_HANDLE;
+    }
+    DeviceGuard g(dev);
+    hipPointerAttribute_t attr{};
+    if (hipPointerGetAttributes(&attr, device_ptr) != hipSuccess || attr.type != hipMemoryTypeDevice) {
+        (void)hipGetLastError();
+        fail(LC_HIP_ERROR_INVALID_ARGUMENT, "external memory is not device memory");
+        return LC_HIP_INVALID_HANDLE;
+    }
+    auto b = std::make_unique<Buffer>();
+    b->dev = dev;
+    b->elem_stride = element_stride;
+    b->bytes = element_stride * element_count;
+    b->ptr = device_ptr;
+    b->owned = false;
+    return handle(b.release());
+}
+
 void lc_hip_buffer_destroy(uint64_t device, uint64_t buffer) {
     Device *dev = obj<Device>(device);
     Buffer *b = obj<Buffer>(buffer);
     if (!dev || !b) return;
     DeviceGuard g(dev);
-    hipFree(b->ptr);
+    if (b->owned) hipFree(b->ptr);
     delete b;
 }
 
@@ -1964,6 +1986,46 @@ int lc_hip_texture_download(uint64_t device, uint64_t stream, uint64_t texture, 
     DeviceGuard g(dev);
     hipStream_t s = native_stream(stream);
     LC_HIP_TRY(hipMemcpyAsync(data, t->ptr, (size_t)t->width * t->height * t->depth * t->pixel_bytes, hipMemcpyDeviceToHost, s));
+    return LC_HIP_OK;
+}
+
+int lc_hip_texture_buffer_copy(uint64_t device, uint64_t stream, uint64_t texture, uint64_t buffer, size_t buffer_offset,
+                               int to_texture) {
+    Device *dev = obj<Device>(device);
+    Texture *t = obj<Texture>(texture);
+    Buffer *b = obj<Buffer>(buffer);
+    if (!dev || !t || !b) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad image / buffer copy handles");
+    const size_t bytes = (size_t)t->width * t->height * t->depth * t->pixel_bytes;
+    if (buffer_offset > b->bytes || bytes > b->bytes - buffer_offset)
+        return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "image copy outside the buffer");
+    DeviceGuard g(dev);
+    char *bp = static_cast<char *>(b->ptr) + buffer_offset;
+    LC_HIP_TRY(hipMemcpyAsync(to_texture ? t->ptr : (void *)bp, to_texture ? (const void *)bp : t->ptr, bytes,
+                              hipMemcpyDeviceToDevice, native_stream(stream)));
+    return LC_HIP_OK;
+}
+
+int lc_hip_texture_copy(uint64_t device, uint64_t stream, uint64_t src, uint64_t dst) {
+    Device *dev = obj<Device>(device);
+    Texture *a = obj<Texture>(src), *b = obj<Texture>(dst);
+    if (!dev || !a || !b) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad image copy handles");
+    if (a->width != b->width || a->height != b->height || a->depth != b->depth || a->pixel_bytes != b->pixel_bytes)
+        return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "image copy between images of different sizes");
+    DeviceGuard g(dev);
+    LC_HIP_TRY(hipMemcpyAsync(b->ptr, a->ptr, (size_t)a->width * a->height * a->depth * a->pixel_bytes,
+                              hipMemcpyDeviceToDevice, native_stream(stream)));
+    return LC_HIP_OK;
+}
+
+int lc_hip_texture_info(uint64_t device, uint64_t texture, uint32_t *storage, uint32_t size[3]) {
+    Texture *t = obj<Texture>(texture);
+    if (!obj<Device>(device) || !t) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad image handle");
+    if (storage) *storage = t->storage;
+    if (size) {
+        size[0] = t->width;
+        size[1] = t->height;
+        size[2] = t->depth;
+    }
     return LC_HIP_OK;
 }
 

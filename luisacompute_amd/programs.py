@@ -228,3 +228,37 @@ extern "C" __global__ void __launch_bounds__(128) query_kernel(const Params p) {
     p.out[i] = dev::query(p.accel, p.rays[i], 0xFFu, p.any != 0u, on_surface, on_procedural);
 }
 '''
+
+
+class PathTracerBindings:
+    """The resources `path_tracer` reads for one scene: heap slots 2i / 2i + 1 = vertex / triangle
+    buffers of instance i's mesh, xforms = the instances' object -> world affines, materials =
+    (albedo, emissive) per instance (the lc_hip_pt_instance list the library path tracer shades
+    the same accel with). `args(image, spp, ...)` binds one request."""
+
+    def __init__(self, device, stream, accel, pt_instances, scene):
+        import numpy as np
+        from . import host
+        n = len(accel)
+        if len(pt_instances) != n:
+            raise ValueError("one material per accel instance")
+        self.accel, self.scene = accel, scene
+        self.heap = device.create_bindless_array(2 * n)
+        for i, m in enumerate(accel._meshes):
+            self.heap.emplace(2 * i, m.vb)
+            self.heap.emplace(2 * i + 1, m.tb)
+        stream << self.heap.update()
+        xf = np.concatenate([accel.instance_info(i)[0] for i in range(n)]).astype(np.float32)
+        self.xforms = host.upload(device, stream, xf)
+        mats = np.array([list(p.albedo) + [1.0 if p.emissive else 0.0] for p in pt_instances], np.float32)
+        self.materials = host.upload(device, stream, mats)
+
+    def vec(self, name):
+        import numpy as np
+        return np.array(getattr(self.scene, name)[:], np.float32)
+
+    def args(self, image, spp, sample_offset=0, max_depth=8, rr_depth=3, seed=42, sequence=0):
+        v = self.vec
+        return (image, self.accel, self.heap, self.xforms, self.materials, v("light_position"), v("light_u"),
+                v("light_v"), v("light_emission"), v("camera_origin"), v("camera_right"), v("camera_up"),
+                v("camera_forward"), spp, sample_offset, max_depth, rr_depth, seed, sequence)

@@ -1,5 +1,5 @@
 """Synthetic workloads of the benchmark configurations, built from the PCG32 generators of
-lc-backend-hip.so (stream initstate 42, initseq = workload number).
+libluisa_compute_backend_hip.so (stream initstate 42, initseq = workload number).
 
   icosphere  : level-6 icosphere (81,920 triangles), 1,048,576 rays from r = 3 toward the unit ball
   soup       : 16,777,216-triangle soup in [0,100]^3 (sigma 0.5), 8,388,608 rays in the cube
