@@ -192,6 +192,7 @@ _SIGNATURES = {
     "lc_hip_device_query": (ctypes.c_int, [_U64, ctypes.c_char_p, ctypes.c_char_p, _SZ]),
     "lc_hip_buffer_create": (_U64, [_U64, _SZ, _SZ, ctypes.POINTER(_P)]),
     "lc_hip_buffer_destroy": (None, [_U64, _U64]),
+    "lc_hip_buffer_create_external": (_U64, [_U64, _SZ, _SZ, _P]),
     "lc_hip_buffer_upload": (ctypes.c_int, [_U64, _U64, _U64, _SZ, _SZ, _P]),
     "lc_hip_buffer_download": (ctypes.c_int, [_U64, _U64, _U64, _SZ, _SZ, _P]),
     "lc_hip_buffer_copy": (ctypes.c_int, [_U64, _U64, _U64, _SZ, _U64, _SZ, _SZ]),
@@ -219,6 +220,9 @@ _SIGNATURES = {
     "lc_hip_texture_destroy": (None, [_U64, _U64]),
     "lc_hip_texture_upload": (ctypes.c_int, [_U64, _U64, _U64, _P]),
     "lc_hip_texture_download": (ctypes.c_int, [_U64, _U64, _U64, _P]),
+    "lc_hip_texture_buffer_copy": (ctypes.c_int, [_U64, _U64, _U64, _U64, _SZ, ctypes.c_int]),
+    "lc_hip_texture_copy": (ctypes.c_int, [_U64, _U64, _U64, _U64]),
+    "lc_hip_texture_info": (ctypes.c_int, [_U64, _U64, _P, _P]),
     "lc_hip_bindless_create": (_U64, [_U64, _SZ]),
     "lc_hip_bindless_destroy": (None, [_U64, _U64]),
     "lc_hip_bindless_update": (ctypes.c_int, [_U64, _U64, _U64, ctypes.POINTER(BindlessModification), _SZ]),
