@@ -297,8 +297,16 @@ typedef struct lc_hip_shader_argument {
     size_t offset;   /* buffer: byte offset; uniform: offset into the uniform bytes */
     size_t size;     /* buffer: bytes bound (0: to the end of the buffer); uniform: byte count */
 } lc_hip_shader_argument;
-/* Compile only (no device needed): 0 on success with the code object size in *code_size. */
+/* Compile only (no device needed): 0 on success with the code object size in *code_size.
+   Compiles go through the code-object cache (cuda_device.cpp:555-604 caches PTX the same way):
+   a 64-bit hash of the compile options, the device library and the source names the entry, kept
+   in memory and on disk (kernel_<hash>.hsaco under LUISA_HIP_CACHE_DIR, default
+   $HOME/.cache/luisacompute_amd, "0" = no disk cache; LUISA_DUMP_SOURCE=1 writes the source
+   beside it). */
 int lc_hip_shader_compile(const char *source, char *log, size_t log_capacity, size_t *code_size);
+/* The cache key of a source, and the process's cache hits (memory or disk) and misses (hiprtc runs). */
+uint64_t lc_hip_shader_cache_key(const char *source);
+int lc_hip_shader_cache_stats(uint64_t *hits, uint64_t *misses);
 uint64_t lc_hip_shader_create(uint64_t device, const char *source, const char *entry, uint32_t block_x,
                               uint32_t block_y, uint32_t block_z, char *log, size_t log_capacity);
 void lc_hip_shader_destroy(uint64_t device, uint64_t shader);
@@ -527,6 +535,14 @@ int lc_hip_trace_set_fetch(uint64_t device, int kind, uint32_t chunk, uint32_t s
 int lc_hip_trace_get_fetch(uint64_t device, int kind, uint32_t *chunk, uint32_t *segments);
 /* Build knob: 1 = also open multi-triangle leaf candidates while a node has free slots. */
 int lc_hip_build_set_split_small(uint64_t device, int enable);
+/* Build knob: SAH refinement of FAST_TRACE builds (the radix tree's clusters of <= 64 primitives
+   rebuilt by binned SAH). 1 = on (the default, unless the environment sets LUISA_HIP_BVH_SAH=0),
+   0 = off. FAST_BUILD builds never refine. */
+int lc_hip_build_set_sah(uint64_t device, int enable);
+/* Build knob: children per wide BVH node, 2..8 (default 8, or LUISA_HIP_BVH_WIDTH); the node
+   format keeps 8 slots, narrower trees leave slots empty. The getter returns the width. */
+int lc_hip_build_set_width(uint64_t device, int width);
+int lc_hip_build_get_width(uint64_t device);
 /* Kernel timing with HIP events recorded on the launch stream around each build/trace. */
 int lc_hip_timing_enable(uint64_t device, int enable);
 int lc_hip_timing_get(uint64_t device, const char *name, double *total_ms, uint64_t *launches);

@@ -227,6 +227,8 @@ _SIGNATURES = {
     "lc_hip_bindless_destroy": (None, [_U64, _U64]),
     "lc_hip_bindless_update": (ctypes.c_int, [_U64, _U64, _U64, ctypes.POINTER(BindlessModification), _SZ]),
     "lc_hip_shader_compile": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, _SZ, ctypes.POINTER(_SZ)]),
+    "lc_hip_shader_cache_key": (ctypes.c_uint64, [ctypes.c_char_p]),
+    "lc_hip_shader_cache_stats": (ctypes.c_int, [ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]),
     "lc_hip_shader_create": (_U64, [_U64, ctypes.c_char_p, ctypes.c_char_p, _U32, _U32, _U32, ctypes.c_char_p, _SZ]),
     "lc_hip_shader_destroy": (None, [_U64, _U64]),
     "lc_hip_shader_dispatch_multiple": (ctypes.c_int, [_U64, _U64, _U64, ctypes.POINTER(ShaderArgument), _SZ, _P, _SZ,
@@ -257,6 +259,9 @@ _SIGNATURES = {
     "lc_hip_trace_set_fetch": (ctypes.c_int, [_U64, ctypes.c_int, _U32, _U32]),
     "lc_hip_trace_get_fetch": (ctypes.c_int, [_U64, ctypes.c_int, ctypes.POINTER(_U32), ctypes.POINTER(_U32)]),
     "lc_hip_build_set_split_small": (ctypes.c_int, [_U64, ctypes.c_int]),
+    "lc_hip_build_set_sah": (ctypes.c_int, [_U64, ctypes.c_int]),
+    "lc_hip_build_set_width": (ctypes.c_int, [_U64, ctypes.c_int]),
+    "lc_hip_build_get_width": (ctypes.c_int, [_U64]),
     "lc_hip_timing_enable": (ctypes.c_int, [_U64, ctypes.c_int]),
     "lc_hip_timing_get": (ctypes.c_int, [_U64, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_U64)]),
     "lc_hip_timing_reset": (ctypes.c_int, [_U64]),
@@ -501,6 +506,17 @@ class Device:
     def set_build_split_small(self, enable):
         _check(lib().lc_hip_build_set_split_small(self.handle, 1 if enable else 0), "build config")
 
+    def set_build_width(self, width):
+        """Children per wide BVH node (2..8) of later builds."""
+        _check(lib().lc_hip_build_set_width(self.handle, int(width)), "build config")
+
+    def build_width(self):
+        return lib().lc_hip_build_get_width(self.handle)
+
+    def set_build_sah(self, enable):
+        """SAH refinement of FAST_TRACE builds (on by default)."""
+        _check(lib().lc_hip_build_set_sah(self.handle, 1 if enable else 0), "build config")
+
     def enable_timing(self, on=True):
         _check(lib().lc_hip_timing_enable(self.handle, 1 if on else 0), "timing")
 
@@ -673,6 +689,17 @@ class Mesh:
         e = (ctypes.c_uint32 * 7)()
         _check(lib().lc_hip_mesh_validate(self.device.handle, stream.handle, self.handle, e), "mesh validate")
         return dict(zip(self.VALIDATION_CHECKS, e[:]))
+
+
+def shader_cache_stats():
+    """(hits, misses) of the process's shader code-object cache."""
+    h, m = ctypes.c_uint64(), ctypes.c_uint64()
+    lib().lc_hip_shader_cache_stats(ctypes.byref(h), ctypes.byref(m))
+    return h.value, m.value
+
+
+def shader_cache_key(source):
+    return lib().lc_hip_shader_cache_key(source.encode())
 
 
 def compile_shader_source(source):

@@ -524,8 +524,240 @@ __device__ __forceinline__ uint32_t ord_u32(float f) {
     return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
 
+// ---- SAH refinement (FAST_TRACE builds). The radix tree is cut into clusters: the highest
+// nodes with at most kSahCluster primitives (their parent has more). Each cluster's subtree is
+// rebuilt top-down by binned SAH over its primitives (one wave per cluster), and its primitives
+// are reordered into the new tree's leaf order, so every node still covers a contiguous run of
+// positions and the rest of the build (bottom-up boxes from the cluster roots, collapse) is
+// unchanged. The rebuilt subtree keeps the radix tree's index rule: a node split after position
+// g has its left child at index g and its right child at g + 1 (leaf or internal), so its
+// internal nodes take the indices the old subtree's internal nodes held, f + 1 .. l - 1, and the
+// cluster root keeps its own index (its parent's link is untouched).
+constexpr uint32_t kSahCluster = 64; // primitives per cluster (one per lane of a wave)
+constexpr int kSahBins = 16;         // bins per axis: 15 split candidates per axis, 45 lanes
+
+// Cluster roots in position order: the thread at position `pos` climbs from leaf pos while the
+// parent starts at pos and holds at most kSahCluster primitives; it reports the node it stops at
+// when that node's parent is larger (or the node is the root). Every cluster is reported by the
+// thread at its first position, so the list comes out ordered by first leaf.
+__global__ __launch_bounds__(kPhaseBlock) void k_sah_cut(uint32_t n, BinaryTree bt, uint32_t *pend, uint32_t *bcount) {
+    const uint32_t pos = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t found = kNoEntry;
+    if (pos < n) {
+        uint32_t ref = pos | kLeafBit, f = pos, l = pos;
+        for (;;) {
+            if (ref == 0u) { // the root: the whole input is one cluster
+                found = ref;
+                break;
+            }
+            const uint2 link = (ref & kLeafBit) ? bt.parent_leaf[pos] : bt.parent_int[ref];
+            const uint32_t p = link.x, side = link.y >> 31, end = link.y & ~kLeafBit;
+            const uint32_t pf = side ? end : f, pl = side ? l : end;
+            if (pl - pf + 1u > kSahCluster) {
+                found = ref;
+                break;
+            }
+            if (pf != pos) break; // inside a cluster that starts before pos
+            ref = p;
+            f = pf;
+            l = pl;
+        }
+    }
+    compact_deferred(found, pend, bcount);
+}
+
+__device__ __forceinline__ float wave_sum_min(float v, bool on) { return wave_min(on ? v : FLT_MAX); }
+__device__ __forceinline__ float wave_sum_max(float v, bool on) { return wave_max(on ? v : -FLT_MAX); }
+
+// Bin of a doubled centroid (NaN and values below the first boundary go to bin 0).
+__device__ __forceinline__ int sah_bin(float cv, float cmin, float scale) {
+    const float x = (cv - cmin) * scale;
+    return x >= 1.0f ? min(kSahBins - 1, (int)fminf(x, (float)kSahBins)) : 0;
+}
+
+// One wave (one workgroup of 64) per cluster, clusters taken in a grid-stride loop. Primitive
+// boxes and doubled centroids sit in LDS by local index (the cluster's input order); s_perm maps
+// positions to local indices. A task (positions [a, b], node index) is split by the lanes: lane
+// L < 45 scores candidate (axis L / 15, boundary L % 15 + 1) over the task's primitives, the
+// cheapest area(left) * n_left + area(right) * n_right wins (lowest lane on ties), and a stable
+// partition reorders the positions. Tasks of two primitives need no search. Centroids that
+// coincide on every axis are split at the middle position.
+__global__ __launch_bounds__(64) void k_sah_clusters(const uint32_t *entries, const uint32_t *count_ptr, BinaryTree bt,
+                                                     uint32_t *ids, const TriRecord *prec, TriRecord *srec,
+                                                     const float4 *pbox) {
+    __shared__ float s_box[6][kSahCluster];
+    __shared__ float s_cen[3][kSahCluster];
+    __shared__ uint32_t s_perm[kSahCluster];
+    __shared__ uint32_t s_pos[kSahCluster];
+    __shared__ uint2 s_task[kSahCluster];
+    const uint32_t lane = threadIdx.x;
+    const uint32_t count = *count_ptr;
+    for (uint32_t c = blockIdx.x; c < count; c += gridDim.x) {
+        const uint32_t ref = entries[c];
+        if (ref & kLeafBit) { // a single primitive: its record goes to its position unchanged
+            const uint32_t pos = ref & ~kLeafBit;
+            if (srec && lane < 3u) {
+                const float4 *src = reinterpret_cast<const float4 *>(prec + ids[pos]);
+                reinterpret_cast<float4 *>(srec + pos)[lane] = src[lane];
+            }
+            continue;
+        }
+        const uint2 rg = bt.range[ref];
+        const uint32_t f = rg.x, m = rg.y - rg.x + 1u; // 2 <= m <= kSahCluster
+        const bool mine = lane < m;
+        uint32_t id = 0;
+        float4 r0 = make_float4(0.f, 0.f, 0.f, 0.f), r1 = r0, r2 = r0;
+        float3 lo = make_float3(0.f, 0.f, 0.f), hi = lo;
+        if (mine) {
+            id = ids[f + lane];
+            if (srec) {
+                const float4 *src = reinterpret_cast<const float4 *>(prec + id);
+                r0 = src[0];
+                r1 = src[1];
+                r2 = src[2];
+                lo = make_float3(fminf(r0.x, fminf(r1.x, r2.x)), fminf(r0.y, fminf(r1.y, r2.y)), fminf(r0.z, fminf(r1.z, r2.z)));
+                hi = make_float3(fmaxf(r0.x, fmaxf(r1.x, r2.x)), fmaxf(r0.y, fmaxf(r1.y, r2.y)), fmaxf(r0.z, fmaxf(r1.z, r2.z)));
+            } else {
+                stored_box(pbox, id, lo, hi);
+            }
+            s_box[0][lane] = lo.x; s_box[1][lane] = lo.y; s_box[2][lane] = lo.z;
+            s_box[3][lane] = hi.x; s_box[4][lane] = hi.y; s_box[5][lane] = hi.z;
+            s_cen[0][lane] = lo.x + hi.x; s_cen[1][lane] = lo.y + hi.y; s_cen[2][lane] = lo.z + hi.z;
+            s_perm[lane] = lane;
+        }
+        if (lane == 0u) s_task[0] = make_uint2(m - 1u, ref); // positions [0, m - 1] packed as b, root index
+        __syncthreads();
+        int sp = 1;
+        while (sp > 0) { // uniform: every lane pops the same task
+            sp--;
+            const uint2 task = s_task[sp];
+            const uint32_t a = task.x >> 16, b = task.x & 0xFFFFu, idx = task.y;
+            const uint32_t mt = b - a + 1u;
+            // the task's primitives: lane j holds position a + j
+            const bool on = lane < mt;
+            const uint32_t pj = on ? s_perm[a + lane] : 0u;
+            uint32_t nl = 1u;
+            float3 llo, lhi, rlo, rhi;
+            bool by_bins = false;
+            uint32_t best_axis = 0, best_k = 0;
+            float cmin[3] = {0.f, 0.f, 0.f}, cscale[3] = {0.f, 0.f, 0.f};
+            if (mt > 2u) {
+                for (int ax = 0; ax < 3; ax++) {
+                    const float cv = s_cen[ax][pj];
+                    const float mn = wave_sum_min(cv, on), mx = wave_sum_max(cv, on);
+                    cmin[ax] = mn;
+                    cscale[ax] = mx > mn ? fminf((float)kSahBins / (mx - mn), FLT_MAX) : 0.0f;
+                }
+                // candidate of this lane
+                float cost = FLT_MAX;
+                float3 clo = make_float3(FLT_MAX, FLT_MAX, FLT_MAX), chi = make_float3(-FLT_MAX, -FLT_MAX, -FLT_MAX);
+                float3 dlo = clo, dhi = chi;
+                const uint32_t ax = lane / (kSahBins - 1), k = lane % (kSahBins - 1) + 1u;
+                if (lane < 3u * (kSahBins - 1) && cscale[ax] > 0.0f) {
+                    uint32_t cl = 0;
+                    for (uint32_t j = 0; j < mt; j++) {
+                        const uint32_t q = s_perm[a + j];
+                        const float cv = s_cen[ax][q];
+                        const int bin = sah_bin(cv, cmin[ax], cscale[ax]);
+                        const float3 blo = make_float3(s_box[0][q], s_box[1][q], s_box[2][q]);
+                        const float3 bhi = make_float3(s_box[3][q], s_box[4][q], s_box[5][q]);
+                        if ((uint32_t)bin < k) {
+                            cl++;
+                            clo = make_float3(fminf(clo.x, blo.x), fminf(clo.y, blo.y), fminf(clo.z, blo.z));
+                            chi = make_float3(fmaxf(chi.x, bhi.x), fmaxf(chi.y, bhi.y), fmaxf(chi.z, bhi.z));
+                        } else {
+                            dlo = make_float3(fminf(dlo.x, blo.x), fminf(dlo.y, blo.y), fminf(dlo.z, blo.z));
+                            dhi = make_float3(fmaxf(dhi.x, bhi.x), fmaxf(dhi.y, bhi.y), fmaxf(dhi.z, bhi.z));
+                        }
+                    }
+                    if (cl > 0u && cl < mt)
+                        cost = box_area(clo, chi) * (float)cl + box_area(dlo, dhi) * (float)(mt - cl);
+                }
+                // cheapest candidate, lowest lane on ties
+                uint64_t key = ((uint64_t)ord_u32(cost) << 32) | lane;
+                for (int o = 32; o > 0; o >>= 1) {
+                    const uint64_t w = __shfl_xor(key, o, 64);
+                    key = w < key ? w : key;
+                }
+                const uint32_t win = (uint32_t)key;
+                if (__shfl(cost, (int)win, 64) < FLT_MAX) {
+                    by_bins = true;
+                    best_axis = win / (kSahBins - 1);
+                    best_k = win % (kSahBins - 1) + 1u;
+                    llo = make_float3(__shfl(clo.x, (int)win, 64), __shfl(clo.y, (int)win, 64), __shfl(clo.z, (int)win, 64));
+                    lhi = make_float3(__shfl(chi.x, (int)win, 64), __shfl(chi.y, (int)win, 64), __shfl(chi.z, (int)win, 64));
+                    rlo = make_float3(__shfl(dlo.x, (int)win, 64), __shfl(dlo.y, (int)win, 64), __shfl(dlo.z, (int)win, 64));
+                    rhi = make_float3(__shfl(dhi.x, (int)win, 64), __shfl(dhi.y, (int)win, 64), __shfl(dhi.z, (int)win, 64));
+                }
+            }
+            // partition the task's positions: left = bins below best_k (or the first half)
+            bool left;
+            if (by_bins) {
+                const float cv = s_cen[best_axis][pj];
+                left = sah_bin(cv, cmin[best_axis], cscale[best_axis]) < (int)best_k;
+            } else {
+                left = lane < mt / 2u;
+            }
+            const uint64_t lmask = __ballot(on && left), rmask = __ballot(on && !left);
+            nl = (uint32_t)__popcll(lmask);
+            const uint64_t below = (1ull << lane) - 1ull;
+            if (!by_bins && mt == 2u) { // two primitives: one each side
+                const uint32_t ql = s_perm[a], qr = s_perm[a + 1u];
+                llo = make_float3(s_box[0][ql], s_box[1][ql], s_box[2][ql]);
+                lhi = make_float3(s_box[3][ql], s_box[4][ql], s_box[5][ql]);
+                rlo = make_float3(s_box[0][qr], s_box[1][qr], s_box[2][qr]);
+                rhi = make_float3(s_box[3][qr], s_box[4][qr], s_box[5][qr]);
+            } else if (!by_bins) { // boxes of the halves
+                const float3 blo = make_float3(s_box[0][pj], s_box[1][pj], s_box[2][pj]);
+                const float3 bhi = make_float3(s_box[3][pj], s_box[4][pj], s_box[5][pj]);
+                const bool il = on && left, ir = on && !left;
+                llo = make_float3(wave_sum_min(blo.x, il), wave_sum_min(blo.y, il), wave_sum_min(blo.z, il));
+                lhi = make_float3(wave_sum_max(bhi.x, il), wave_sum_max(bhi.y, il), wave_sum_max(bhi.z, il));
+                rlo = make_float3(wave_sum_min(blo.x, ir), wave_sum_min(blo.y, ir), wave_sum_min(blo.z, ir));
+                rhi = make_float3(wave_sum_max(bhi.x, ir), wave_sum_max(bhi.y, ir), wave_sum_max(bhi.z, ir));
+            }
+            __syncthreads(); // every lane has read s_perm for this task
+            if (on) {
+                const uint32_t r = left ? (uint32_t)__popcll(lmask & below) : nl + (uint32_t)__popcll(rmask & below);
+                s_perm[a + r] = pj;
+            }
+            // the node: split after position g; children at g (left) and g + 1 (right)
+            const uint32_t g = a + nl - 1u;
+            const uint32_t lref = nl == 1u ? ((f + a) | kLeafBit) : (f + g);
+            const uint32_t rref = mt - nl == 1u ? ((f + b) | kLeafBit) : (f + g + 1u);
+            if (lane == 0u) {
+                bt.child[idx] = make_uint2(lref, rref);
+                bt.range[idx] = make_uint2(f + a, f + b);
+                float4 *cb = bt.cbox + 4 * (uint64_t)idx;
+                cb[0] = make_float4(llo.x, llo.y, llo.z, 0.0f);
+                cb[1] = make_float4(lhi.x, lhi.y, lhi.z, 0.0f);
+                cb[2] = make_float4(rlo.x, rlo.y, rlo.z, 0.0f);
+                cb[3] = make_float4(rhi.x, rhi.y, rhi.z, 0.0f);
+                if (mt - nl >= 2u) s_task[sp] = make_uint2(((a + nl) << 16) | b, f + g + 1u);
+                if (nl >= 2u) s_task[sp + (mt - nl >= 2u ? 1 : 0)] = make_uint2((a << 16) | g, f + g);
+            }
+            sp += (mt - nl >= 2u ? 1 : 0) + (nl >= 2u ? 1 : 0);
+            __syncthreads(); // s_perm and the task stack written
+        }
+        // primitives to their final positions
+        if (mine) s_pos[s_perm[lane]] = lane;
+        __syncthreads();
+        if (mine) {
+            const uint32_t pos = f + s_pos[lane];
+            ids[pos] = id;
+            if (srec) {
+                float4 *dst = reinterpret_cast<float4 *>(srec + pos);
+                dst[0] = r0;
+                dst[1] = r1;
+                dst[2] = r2;
+            }
+        }
+        __syncthreads(); // LDS reused by the next cluster
+    }
+}
+
 // One wide level. Each group of 8 lanes turns one work item (binary ref, wide node index) into a
-// WideNode; lane k of the group holds child candidate k. All shuffles are executed by every lane
+// WideNode of at most `width` (2..8) children; lane k of the group holds child candidate k. All shuffles are executed by every lane
 // of a group (groups without an item compute on empty data), and each group's control flow is
 // uniform across its 8 lanes; the loop over items is uniform across the block.
 //
@@ -542,7 +774,7 @@ __device__ __forceinline__ uint32_t ord_u32(float f) {
 __global__ __launch_bounds__(kCollapseBlock) void k_collapse(LeafBoxes lb, BinaryTree bt,
                                                      BuildCounters *ctr, uint32_t level, const uint2 *qin,
                                                      uint2 *qout, WideOut out, uint32_t split_small,
-                                                     uint32_t leaf_max) {
+                                                     uint32_t leaf_max, uint32_t width) {
     __shared__ __align__(16) uint32_t s_node[kCollapseBlock / 8][20];
     __shared__ uint32_t s_wave[2][kCollapseBlock / 64];
     __shared__ unsigned long long s_base;
@@ -600,7 +832,7 @@ __global__ __launch_bounds__(kCollapseBlock) void k_collapse(LeafBoxes lb, Binar
                 if (prio) key = ((uint64_t)prio << 40) | ((uint64_t)__float_as_uint(fmaxf(box_area(lo, hi), 0.0f)) << 3) | (7u - lane);
             }
             const uint64_t best = g8_max_u64(key);
-            if (nc >= 8 || best == 0) break; // uniform within the group
+            if (nc >= width || best == 0) break; // uniform within the group
             const uint32_t winner = 7u - (uint32_t)(best & 7u);
             const uint32_t wref = __shfl(ref, (int)winner, 8);
             const uint32_t wlast = __shfl(last, (int)winner, 8);
@@ -1092,8 +1324,10 @@ hipError_t bvh_validate(hipStream_t stream, const WideNode *nodes, uint32_t node
 hipError_t bvh_build(hipStream_t stream, void *scratch, size_t scratch_bytes, const void *vbase, uint64_t vstride,
                      uint64_t nverts, const uint32_t *tri, const float4 *boxes, uint32_t n, WideNode *nodes,
                      TriRecord *tris, uint32_t *prim_out, BuildCounters **counters_out, uint32_t split_small,
-                     uint32_t leaf_max, int cu_count, uint32_t *error_flag, uint32_t *pinned_word) {
+                     uint32_t leaf_max, uint32_t sah_refine, uint32_t width, int cu_count, uint32_t *error_flag,
+                     uint32_t *pinned_word) {
     if (leaf_max < 1 || leaf_max > kLeafMax) return hipErrorInvalidValue;
+    if (width < 2 || width > 8) return hipErrorInvalidValue;
     if (scratch_bytes < bvh_scratch_bytes(n)) return hipErrorInvalidValue;
     // the collapse grid fills the CUs once (its loop strides over the level's items)
     static const int collapse_blocks_per_cu = [] {
@@ -1130,12 +1364,15 @@ hipError_t bvh_build(hipStream_t stream, void *scratch, size_t scratch_bytes, co
     hipError_t e = rocprim::radix_sort_pairs(temp, tb, kb, vb, n, 0, 30, stream);
     if (e != hipSuccess) return e;
     const uint32_t *keys = kb.current();
-    const uint32_t *ids = vb.current();
+    uint32_t *ids = vb.current();
+    // SAH refinement of the radix tree's clusters (below); needs at least one split to choose
+    const bool sah = sah_refine != 0u && n > 2u;
     // leaf boxes: triangle builds keep their records in sorted order (written by the leaf phase,
-    // or here when the build is too small for it); box builds read the input boxes
+    // by the SAH cluster pass, or here when the build is too small for the leaf phase); box
+    // builds read the input boxes
     TriRecord *srec = boxes ? nullptr : L.srec;
     const LeafBoxes lb{boxes, ids, srec};
-    if (srec && n <= kLastBlock)
+    if (srec && n <= kLastBlock && !sah)
         hipLaunchKernelGGL(k_sorted_records, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, stream,
                            (const TriRecord *)prec, ids, srec, n);
     if (n == 1) {
@@ -1148,6 +1385,30 @@ hipError_t bvh_build(hipStream_t stream, void *scratch, size_t scratch_bytes, co
         const uint32_t *entries = nullptr;
         uint32_t count = n;
         int which = 0;
+        if (sah) {
+            // clusters in position order -> ent[0]; each rebuilt by binned SAH, its primitives
+            // reordered (ids, and the sorted records of a triangle build); the bottom-up phases
+            // then start from the cluster roots
+            const uint32_t nblocks = (n + kPhaseBlock - 1) / kPhaseBlock;
+            hipLaunchKernelGGL(k_sah_cut, dim3(nblocks), dim3(kPhaseBlock), 0, stream, n, bt, pend, L.bcount);
+            tb = temp_bytes;
+            e = rocprim::exclusive_scan(temp, tb, (const uint32_t *)L.bcount, L.boff, 0u, (size_t)nblocks,
+                                        rocprim::plus<uint32_t>(), stream);
+            if (e != hipSuccess) return e;
+            hipLaunchKernelGGL(k_gather_deferred, dim3(nblocks), dim3(kPhaseBlock), 0, stream, (const uint32_t *)pend,
+                               (const uint32_t *)L.bcount, (const uint32_t *)L.boff, kPhaseBlock, nblocks, ent[which],
+                               dcount);
+            const uint32_t grid_sah = std::max<uint32_t>(1u, std::min<uint32_t>((uint32_t)std::max(1, cu_count) * 32u, n / 2u));
+            hipLaunchKernelGGL(k_sah_clusters, dim3(grid_sah), dim3(64), 0, stream, (const uint32_t *)ent[which],
+                               (const uint32_t *)dcount, bt, ids, (const TriRecord *)prec, srec, boxes);
+            e = hipMemcpyAsync(pinned_word, dcount, sizeof(uint32_t), hipMemcpyDeviceToHost, stream);
+            if (e != hipSuccess) return e;
+            e = hipStreamSynchronize(stream);
+            if (e != hipSuccess) return e;
+            count = *pinned_word;
+            entries = ent[which];
+            which ^= 1;
+        }
         for (;;) {
             if (count <= kLastBlock) {
                 uint32_t blk = (count + 63) / 64 * 64;
@@ -1201,7 +1462,7 @@ hipError_t bvh_build(hipStream_t stream, void *scratch, size_t scratch_bytes, co
         const uint2 *qin = (level & 1) ? q1 : q0;
         uint2 *qout = (level & 1) ? q0 : q1;
         hipLaunchKernelGGL(k_collapse, dim3(grid_collapse), dim3(kCollapseBlock), 0, stream, lb, bt, ctr, level, qin,
-                           qout, out, split_small, leaf_max);
+                           qout, out, split_small, leaf_max, width);
         if ((level + 1) % kChunk == 0 && level + 1 < kCollapseLaunches) {
             e = hipMemcpyAsync(pinned_word, level_nodes_word(ctr, level + 1), sizeof(uint32_t), hipMemcpyDeviceToHost,
                                stream);

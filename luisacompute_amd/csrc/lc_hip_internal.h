@@ -24,7 +24,8 @@ size_t bvh_node_capacity(uint32_t n);
 hipError_t bvh_build(hipStream_t stream, void *scratch, size_t scratch_bytes, const void *vbase, uint64_t vstride,
                      uint64_t nverts, const uint32_t *tri, const float4 *boxes, uint32_t n, WideNode *nodes,
                      TriRecord *tris, uint32_t *prim_out, BuildCounters **counters_out, uint32_t split_small,
-                     uint32_t leaf_max, int cu_count, uint32_t *error_flag, uint32_t *pinned_word);
+                     uint32_t leaf_max, uint32_t sah_refine, uint32_t width, int cu_count, uint32_t *error_flag,
+                     uint32_t *pinned_word);
 hipError_t bvh_refit(hipStream_t stream, void *scratch, size_t scratch_bytes, const void *vbase, uint64_t vstride,
                      uint64_t nverts, const uint32_t *tri, uint32_t n, uint32_t depth, WideNode *nodes,
                      TriRecord *tris, BuildCounters **counters_out, int cu_count, uint32_t *error_flag);
@@ -262,6 +263,8 @@ struct Device {
     std::mutex mu;
     TraceConfig cfg[2]; // [closest, any]
     uint32_t split_small = 1;
+    uint32_t sah_refine = 1; // SAH refinement of FAST_TRACE builds (LUISA_HIP_BVH_SAH=0: off)
+    uint32_t bvh_width = 8;  // children per wide node, 2..8 (LUISA_HIP_BVH_WIDTH)
     // error word of run-time shaders' traversal stacks (DeviceAccel::error)
     uint32_t *error = nullptr;
     // per-stream working memory of trace / query / build launches (guarded by mu)
@@ -436,6 +439,8 @@ extern const int kDeviceHeaderCount;
 // Compiles `source` for gfx950 with the device library; code object into `code`, diagnostics
 // into `log`. False on a compile error.
 bool rtc_compile(const char *source, const char *name, std::vector<char> &code, std::string &log);
+uint64_t rtc_cache_key(const char *source);
+void rtc_cache_stats(uint64_t *hits, uint64_t *misses);
 // Writes the accel's DeviceAccel descriptor (allocating it on first use) on `stream`, copying
 // from `staging` (pinned, sizeof(DeviceAccel) bytes, untouched until the copy ran).
 int accel_write_descriptor(Accel *a, hipStream_t stream, void *staging);

@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -177,6 +178,9 @@ void invert_affine(const float m[12], float out[12]) {
 // primitive leaves (more wide nodes, fewer triangle tests: path tracer 2.56 -> 3.28 G rays/s);
 // FAST_BUILD allows 3 (fewer nodes, 19% shorter build of 16.7M triangles).
 uint32_t leaf_size_for(uint32_t hint) { return hint == LC_HIP_ACCEL_FAST_BUILD ? 3u : 1u; }
+// Binned-SAH refinement of the radix tree's clusters for every hint but FAST_BUILD (the reference
+// CPU backend builds Embree's high-quality SAH for FAST_TRACE, cpu/accel.rs:58-62).
+uint32_t sah_for(const Device *dev, uint32_t hint) { return hint != LC_HIP_ACCEL_FAST_BUILD && dev->sah_refine ? 1u : 0u; }
 
 void copy_counters_async(BuildCounters *dctr, HostCounters *host, uint32_t *error, hipStream_t s) {
     hipMemcpyAsync(host, dctr, offsetof(BuildCounters, level), hipMemcpyDeviceToHost, s);
@@ -313,6 +317,8 @@ uint64_t lc_hip_device_create(int32_t index) {
     }
     hipMemset(p, 0, 256);
     dev->error = static_cast<uint32_t *>(p);
+    if (const char *v = std::getenv("LUISA_HIP_BVH_SAH")) dev->sah_refine = std::atoi(v) != 0 ? 1u : 0u;
+    if (const char *v = std::getenv("LUISA_HIP_BVH_WIDTH")) dev->bvh_width = (uint32_t)std::min(8, std::max(2, std::atoi(v)));
     dev->cfg[0] = TraceConfig{256, 16, 32, 0};
     dev->cfg[1] = TraceConfig{256, 16, 32, 0};
     if (hipEventCreateWithFlags(&dev->arena_done, hipEventDisableTiming) != hipSuccess) {
@@ -749,7 +755,8 @@ int lc_hip_procedural_build(uint64_t device, uint64_t stream, uint64_t primitive
         LC_HIP_TRY(aabbs_to_boxes(s, aabb, n, m->proc_boxes));
         BuildCounters *ctr = nullptr;
         hipError_t e = bvh_build(s, t.scratch, t.scratch_bytes, nullptr, 0, 0, nullptr, m->proc_boxes, n, t.nodes,
-                                 nullptr, m->proc_ids, &ctr, 0, 1u, dev->cu_count, S->error, &m->host->phase_count);
+                                 nullptr, m->proc_ids, &ctr, 0, 1u, sah_for(dev, m->hint), dev->bvh_width, dev->cu_count,
+                                 S->error, &m->host->phase_count);
         if (e != hipSuccess) return hip_fail(e, "procedural bvh_build");
         LC_HIP_TRY(proc_records(s, m->proc_ids, m->proc_boxes, n, m->proc));
         copy_counters_async(ctr, m->host, S->error, s);
@@ -846,7 +853,8 @@ int lc_hip_mesh_build(uint64_t device, uint64_t stream, uint64_t mesh, uint32_t 
         BuildCounters *ctr = nullptr;
         hipError_t e = bvh_build(s, t.scratch, t.scratch_bytes, vaddr, vertex_stride, nverts, taddr, nullptr, n,
                                  t.nodes, m->tris, nullptr, &ctr, dev->split_small, leaf_size_for(m->hint),
-                                 dev->cu_count, S->error, &m->host->phase_count);
+                                 sah_for(dev, m->hint), dev->bvh_width, dev->cu_count, S->error,
+                                 &m->host->phase_count);
         if (e != hipSuccess) return hip_fail(e, "bvh_build");
         copy_counters_async(ctr, m->host, S->error, s);
         if (m->allow_compaction) {
@@ -1196,8 +1204,9 @@ int lc_hip_accel_build(uint64_t device, uint64_t stream, uint64_t accel, uint32_
             BuildCounters *ctr = nullptr;
             hipError_t e = bvh_build(s, a->tlas_scratch, a->tlas_scratch_bytes, nullptr, 0, 0, nullptr, a->boxes,
                                      instance_count, a->tlas_nodes, nullptr, a->tlas_prims, &ctr, 0,
-                                     leaf_size_for(a->hint), std::max(1, dev->cu_count), S->error,
-                                     &a->tlas_host->phase_count);
+                                     leaf_size_for(a->hint), sah_for(dev, a->hint), dev->bvh_width,
+                                     std::max(1, dev->cu_count),
+                                     S->error, &a->tlas_host->phase_count);
             if (e != hipSuccess) return hip_fail(e, "tlas build");
             copy_counters_async(ctr, a->tlas_host, S->error, s);
         }
@@ -1876,6 +1885,29 @@ int lc_hip_trace_set_ray_sort(uint64_t device, int kind, int enable) {
     return LC_HIP_OK;
 }
 
+int lc_hip_build_set_width(uint64_t device, int width) {
+    Device *dev = obj<Device>(device);
+    if (!dev) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad device");
+    if (width < 2 || width > 8) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "BVH width must be 2..8");
+    std::lock_guard<std::mutex> lk(dev->mu);
+    dev->bvh_width = (uint32_t)width;
+    return LC_HIP_OK;
+}
+
+int lc_hip_build_get_width(uint64_t device) {
+    Device *dev = obj<Device>(device);
+    if (!dev) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad device");
+    return (int)dev->bvh_width;
+}
+
+int lc_hip_build_set_sah(uint64_t device, int enable) {
+    Device *dev = obj<Device>(device);
+    if (!dev) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad device");
+    std::lock_guard<std::mutex> lk(dev->mu);
+    dev->sah_refine = enable ? 1u : 0u;
+    return LC_HIP_OK;
+}
+
 int lc_hip_build_set_split_small(uint64_t device, int enable) {
     Device *dev = obj<Device>(device);
     if (!dev) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad device");
@@ -2115,6 +2147,13 @@ static void copy_log(const std::string &log, char *out, size_t cap) {
     std::memcpy(out, log.data(), n);
     out[n] = '\0';
 }
+
+int lc_hip_shader_cache_stats(uint64_t *hits, uint64_t *misses) {
+    rtc_cache_stats(hits, misses);
+    return LC_HIP_OK;
+}
+
+uint64_t lc_hip_shader_cache_key(const char *source) { return source ? rtc_cache_key(source) : 0; }
 
 int lc_hip_shader_compile(const char *source, char *log, size_t log_capacity, size_t *code_size) {
     if (!source) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "null shader source");
