@@ -4,6 +4,7 @@ and a "hip" device from the LibInterface, resources from the DeviceInterface tab
 CommandLists (include/luisa/rust/api_types.h:416-485) with a completion callback. The accel built
 through the table is traced and checked against the oracle."""
 import ctypes as C
+import sys
 import threading
 
 import numpy as np
@@ -24,7 +25,12 @@ FORCE_BUILD = 1
 def api():
     dll, li = lcapi.lib_interface(lc.LIB_PATH)
     logs = []
-    cb = lcapi.LoggerCallback(lambda m: logs.append((m.level.decode(), m.message.decode())))
+
+    def log(m):
+        logs.append((m.level.decode(), m.message.decode()))
+        if logs[-1][0] == "E":  # printed at once: an error is followed by an abort
+            print(f"[backend E] {logs[-1][1]}", file=sys.stderr, flush=True)
+    cb = lcapi.LoggerCallback(log)
     li.set_logger_callback(cb)
     ctx = li.create_context(b"/tmp")
     dev = li.create_device(ctx, b"hip", b'{"index": 0}')
@@ -108,7 +114,10 @@ def test_build_and_trace_through_command_lists(api):
     ib = dev.create_buffer(d, C.c_void_p(tri_t.ptr), len(tris), None)
     assert vb.element_stride == 16 and ib.element_stride == 12
     aabbs = np.array([[10, 10, 10, 11, 11, 11]], np.float32)
-    ab = dev.create_buffer(d, C.c_void_p(lcapi.ir_void().ptr), aabbs.nbytes, None)
+    # the IrType objects must outlive the calls that read them (a temporary's memory is freed
+    # before the call runs)
+    void_t = lcapi.ir_void()
+    ab = dev.create_buffer(d, C.c_void_p(void_t.ptr), aabbs.nbytes, None)
     st = dev.create_stream(d, COMPUTE).handle
     opt = lcapi.AccelOption(0, False, False)
     mesh = dev.create_mesh(d, C.byref(opt)).handle
@@ -133,8 +142,8 @@ def test_build_and_trace_through_command_lists(api):
                                _upload(ab.resource.handle, aabbs), mb, pb, acb], lambda user: done.set())
     assert done.wait(60), "dispatch callback did not run"
     rays = lc.rays_box_sphere(50000, initseq=80, lo=-3.0, hi=7.0)
-    rbuf = dev.create_buffer(d, C.c_void_p(lcapi.ir_void().ptr), rays.nbytes, None)
-    hbuf = dev.create_buffer(d, C.c_void_p(lcapi.ir_void().ptr), 24 * len(rays), None)
+    rbuf = dev.create_buffer(d, C.c_void_p(void_t.ptr), rays.nbytes, None)
+    hbuf = dev.create_buffer(d, C.c_void_p(void_t.ptr), 24 * len(rays), None)
     keep2 = _dispatch(dev, st, [_upload(rbuf.resource.handle, rays)])
     rc = dll.lc_hip_accel_trace_closest(C.c_uint64(d.value), C.c_uint64(st), C.c_uint64(accel),
                                         C.c_uint64(rbuf.resource.handle), C.c_size_t(0), C.c_uint64(hbuf.resource.handle),
@@ -171,7 +180,8 @@ def test_events_and_textures(api):
     t0 = dev.create_texture(d, 29, 2, w, h, 1, 1, False, False).handle
     t1 = dev.create_texture(d, 29, 2, w, h, 1, 1, False, False).handle
     assert t0 != 2 ** 64 - 1 and t1 != 2 ** 64 - 1
-    buf = dev.create_buffer(d, C.c_void_p(lcapi.ir_void().ptr), pix.nbytes, None).resource.handle
+    void_t = lcapi.ir_void()
+    buf = dev.create_buffer(d, C.c_void_p(void_t.ptr), pix.nbytes, None).resource.handle
     up = lcapi.Command(tag=lcapi.CMD_TEXTURE_UPLOAD)
     up.u.texture_upload = lcapi.TextureTransfer(lcapi.Handle(t0), 14, 0, (C.c_uint32 * 3)(w, h, 1), pix.ctypes.data)
     t2b = lcapi.Command(tag=lcapi.CMD_TEXTURE_TO_BUFFER_COPY)
