@@ -310,6 +310,11 @@ int lc_hip_shader_cache_stats(uint64_t *hits, uint64_t *misses);
 uint64_t lc_hip_shader_create(uint64_t device, const char *source, const char *entry, uint32_t block_x,
                               uint32_t block_y, uint32_t block_z, char *log, size_t log_capacity);
 void lc_hip_shader_destroy(uint64_t device, uint64_t shader);
+/* shader_argument_usage (device_interface.h:159): how the kernel uses argument `index`, Usage
+   values of include/luisa/ast/usage.h (0 none, 1 read, 2 write, 3 read-write). A frontend that
+   built the kernel's AST records them with set_argument_usage; unrecorded arguments answer 3. */
+int lc_hip_shader_set_argument_usage(uint64_t device, uint64_t shader, const uint32_t *usage, size_t count);
+int lc_hip_shader_argument_usage(uint64_t device, uint64_t shader, size_t index);
 /* ShaderDispatchCommand with several dispatch sizes (3 uint32 per dispatch): launch k gets
    ls_kid = {size k, k}. */
 int lc_hip_shader_dispatch_multiple(uint64_t device, uint64_t stream, uint64_t shader,

@@ -230,6 +230,8 @@ _SIGNATURES = {
     "lc_hip_shader_cache_key": (ctypes.c_uint64, [ctypes.c_char_p]),
     "lc_hip_shader_cache_stats": (ctypes.c_int, [ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]),
     "lc_hip_shader_create": (_U64, [_U64, ctypes.c_char_p, ctypes.c_char_p, _U32, _U32, _U32, ctypes.c_char_p, _SZ]),
+    "lc_hip_shader_set_argument_usage": (ctypes.c_int, [_U64, _U64, ctypes.POINTER(_U32), _SZ]),
+    "lc_hip_shader_argument_usage": (ctypes.c_int, [_U64, _U64, _SZ]),
     "lc_hip_shader_destroy": (None, [_U64, _U64]),
     "lc_hip_shader_dispatch_multiple": (ctypes.c_int, [_U64, _U64, _U64, ctypes.POINTER(ShaderArgument), _SZ, _P, _SZ,
                                                        _P, _SZ]),
@@ -905,6 +907,19 @@ class Shader:
         if getattr(self, "handle", None) is not None and _lib is not None and self.device.handle is not None:
             lib().lc_hip_shader_destroy(self.device.handle, self.handle)
             self.handle = None
+
+    def set_argument_usage(self, usages):
+        """Records the kernel's per-argument Usage (0 none, 1 read, 2 write, 3 both)."""
+        arr = (ctypes.c_uint32 * max(1, len(usages)))(*usages)
+        _check(lib().lc_hip_shader_set_argument_usage(self.device.handle, self.handle, arr, len(usages)),
+               "shader argument usage")
+
+    def argument_usage(self, index):
+        """shader_argument_usage (device_interface.h:159)."""
+        r = lib().lc_hip_shader_argument_usage(self.device.handle, self.handle, index)
+        if r < 0:
+            raise LCError(f"shader argument usage: {lib().lc_hip_last_error().decode()}")
+        return r
 
     def __call__(self, *args):
         """shader(args...).dispatch(nx[, ny, nz]) -> stream command (ShaderDispatchCommand)."""

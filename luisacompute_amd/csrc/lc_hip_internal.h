@@ -431,6 +431,7 @@ struct Shader {
     hipModule_t module = nullptr;
     hipFunction_t fn = nullptr;
     uint32_t block[3] = {128, 1, 1};
+    std::vector<uint32_t> usage; // per argument: Usage (1 read, 2 write), from the frontend's AST
 };
 // The device library headers (lc_hip_device.h and what it includes), embedded at build time.
 extern const char *const kDeviceHeaderNames[];

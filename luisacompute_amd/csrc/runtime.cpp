@@ -2208,6 +2208,22 @@ uint64_t lc_hip_shader_create(uint64_t device, const char *source, const char *e
     return reinterpret_cast<uint64_t>(sh.release());
 }
 
+int lc_hip_shader_set_argument_usage(uint64_t device, uint64_t shader, const uint32_t *usage, size_t count) {
+    Shader *sh = obj<Shader>(shader);
+    if (!obj<Device>(device) || !sh || (count && !usage)) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad shader usage arguments");
+    for (size_t i = 0; i < count; i++)
+        if (usage[i] > 3u) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "usage is NONE, READ, WRITE or READ_WRITE");
+    sh->usage.assign(usage, usage + count);
+    return LC_HIP_OK;
+}
+
+int lc_hip_shader_argument_usage(uint64_t device, uint64_t shader, size_t index) {
+    Shader *sh = obj<Shader>(shader);
+    if (!obj<Device>(device) || !sh) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad shader");
+    if (index >= sh->usage.size()) return 3; // unknown: read and write, as a conservative answer
+    return (int)sh->usage[index];
+}
+
 void lc_hip_shader_destroy(uint64_t device, uint64_t shader) {
     Device *dev = obj<Device>(device);
     Shader *sh = obj<Shader>(shader);

@@ -26,6 +26,14 @@ def test_dsl_errors():
 
 
 def test_dsl_true_division():
-    src = dsl_kernels.division_kernel.source
-    assert "true_div(" in src
-    assert lc.compile_shader_source(src) > 1000
+    """'/' converts both integer operands to float; '//' divides in the operands' type."""
+    from luisacompute_amd import lcast as A
+    k = dsl_kernels.division_kernel
+    writes = []
+    A.walk_expressions(k.function.body, lambda e: writes.append(e) if isinstance(e, A.CallExpr) and
+                       e.op == A.CallOp.BUFFER_WRITE else None)
+    true_div, floor_div = (w.arguments[2] for w in writes)
+    assert isinstance(true_div, A.BinaryExpr) and true_div.op == A.BinaryOp.DIV and true_div.type is A.FLOAT
+    assert all(isinstance(x, A.CastExpr) and x.type is A.FLOAT for x in (true_div.lhs, true_div.rhs))
+    assert isinstance(floor_div, A.BinaryExpr) and floor_div.op == A.BinaryOp.DIV and floor_div.type is A.UINT
+    assert lc.compile_shader_source(k.source) > 1000
