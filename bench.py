@@ -41,18 +41,32 @@ def parse():
                     help="soup: the headline line; instanced: 64 ray batches over a 1,024-instance TLAS; "
                          "pathtracer: box room 1920x1080, 64 spp, depth 8 (wavefront vs megakernel); "
                          "serving: render requests over 4 resident scenes (served vs plain)")
-    ap.add_argument("--serve-requests", type=int, default=48)
+    ap.add_argument("--serve-requests", type=int, default=200)
+    ap.add_argument("--no-serving", action="store_true", help="soup line: skip the serving leg")
     ap.add_argument("--serve-rate", type=float, default=0.0, help="arrivals/s (0: 0.8 x plain saturated throughput)")
     ap.add_argument("--serve-budget-mib", type=int, default=1024)
-    ap.add_argument("--serve-wave-paths", type=int, default=32 << 20)
+    ap.add_argument("--serve-wave-paths", type=int, default=None, help="paths per shared wave (default: tuned)")
     ap.add_argument("--pt-width", type=int, default=1920)
     ap.add_argument("--pt-height", type=int, default=1080)
     ap.add_argument("--pt-spp", type=int, default=64)
-    ap.add_argument("--pt-chunk", type=int, default=0, help="wavefront samples per wave (0: about 32M paths)")
-    ap.add_argument("--pt-sort", type=int, default=0, help="wavefront: bucket-sort continuation rays")
-    ap.add_argument("--pt-fetch", default="128,8", help="wavefront traces: slots per claim (0: exact), queue segments")
+    ap.add_argument("--pt-chunk", type=int, default=None, help="wavefront samples per wave (0: about 32M paths; default: tuned)")
+    ap.add_argument("--pt-sort", type=int, default=None, help="wavefront: bucket-sort continuation rays (default: tuned)")
+    ap.add_argument("--pt-fetch", default=None, help="wavefront traces: slots per claim (0: exact), queue segments (default: tuned)")
     ap.add_argument("--pt-no-mega", action="store_true", help="skip the megakernel comparison")
-    return ap.parse_args()
+    a = ap.parse_args()
+    # per-call knobs the autotuner chose (luisacompute_amd/autotune_table.json) unless given
+    from luisacompute_amd import autotune
+    table = autotune.load_table()
+    chunk, sort, fetch = autotune.pathtracer_options(table)
+    if a.pt_chunk is None:
+        a.pt_chunk = chunk
+    if a.pt_sort is None:
+        a.pt_sort = sort
+    if a.pt_fetch is None:
+        a.pt_fetch = f"{fetch[0]},{fetch[1]}"
+    if a.serve_wave_paths is None:
+        a.serve_wave_paths = autotune.serving_options(table)
+    return a
 
 
 def ncores_here():
@@ -175,30 +189,35 @@ def bench_pathtracer(a):
     print(json.dumps(out), flush=True)
 
 
-def bench_serving(a):
+def serving_measure(a, n):
     """Render serving over four resident scenes (icosphere L6, 16.7M-triangle soup, box room,
     1,024-instance TLAS). Requests: scene U{4}, square resolution U{256,512,1024,2048}, spp
     U{1,4,16,64}, depth 8, PCG32 stream (42, 5). Two disciplines on the same requests:
-      plain  : one megakernel dispatch per request, in arrival order (one stream);
-      served : the serving loop (HBM-budget admission, continuous batching into shared waves,
-               scenes on their own streams).
+      plain  : the path tracer written in the DSL (AST -> HIP -> hiprtc megakernel), one dispatch
+               per request in arrival order on one stream;
+      served : the serving loop (HBM-budget admission, continuous batching into shared wavefront
+               waves, scenes on their own streams).
     Throughput is measured with every request queued at t = 0 (requests/s = N / makespan);
     latency (p50/p99 of completion - arrival) with Poisson arrivals at a fixed rate, 0.8 x the
-    plain discipline's saturated throughput."""
+    plain discipline's saturated throughput. Returns the result dict."""
     import threading
     import luisacompute_amd as lc
-    from luisacompute_amd import autotune, workloads
+    from luisacompute_amd import autotune, programs, workloads
     dev = lc.Device(0)
     autotune.load_default(dev)
     s = dev.create_stream()
     t0 = time.perf_counter()
     scenes = workloads.serving_scenes(dev, s)
+    binds = [programs.PathTracerBindings(dev, s, acc, inst, sc) for _, acc, inst, sc in scenes]
+    kdsl = programs.path_tracer.compile(dev)
     s.synchronize()
     t_setup = time.perf_counter() - t0
-    n = a.serve_requests
     base = workloads.serving_requests(n, 0.0)
     bufs = [dev.create_buffer(16, r["res"] * r["res"]) for r in base]
-    cnts = [dev.create_buffer(8, 2) for _ in base]
+
+    def dispatch_plain(r):
+        cmd = kdsl(*binds[r["scene"]].args(bufs[r["index"]], r["spp"], 0, 8, 3, 42, r["sequence"]))
+        s << cmd.dispatch(r["res"], r["res"])
 
     def plain(reqs, rate_mode):
         lat = []
@@ -208,22 +227,10 @@ def bench_serving(a):
                 wait = t_start + r["arrival"] - time.perf_counter()
                 if wait > 0:
                     time.sleep(wait)
-            name, acc, inst, sc = scenes[r["scene"]]
-            o = lc.PtOptions()
-            o.width = o.height = r["res"]
-            o.spp, o.max_depth, o.rr_depth, o.seed, o.sequence, o.mode = r["spp"], 8, 3, 42, r["sequence"], lc.PT_MEGAKERNEL
-            acc.pathtrace(s, inst, sc, o, bufs[r["index"]], counters=cnts[r["index"]])
+            dispatch_plain(r)
             s.synchronize()
             lat.append(time.perf_counter() - t_start - r["arrival"])
         return time.perf_counter() - t_start, lat
-
-    def plain_rays(reqs):
-        tot = 0
-        for r in reqs:
-            c = np.zeros(2, np.uint64)
-            s << cnts[r["index"]].copy_to(c) << lc.synchronize()
-            tot += int(c.sum())
-        return tot
 
     fetch = tuple(int(x) for x in a.pt_fetch.split(","))
 
@@ -262,16 +269,12 @@ def bench_serving(a):
         srv.close()
         return makespan, lat, rays, out
 
-    # warm the plain path (kernel load) on a tiny request per scene
-    for _, acc, inst, sc in scenes:
-        o = lc.PtOptions()
-        o.width = o.height = 16
-        o.spp, o.max_depth, o.rr_depth, o.seed, o.sequence, o.mode = 1, 8, 3, 42, 1, lc.PT_MEGAKERNEL
-        acc.pathtrace(s, inst, sc, o, bufs[0])
+    # warm the plain path (module load) on a tiny request per scene
+    for k in range(len(scenes)):
+        dispatch_plain({"scene": k, "index": 0, "spp": 1, "sequence": 1, "res": 16})
     s.synchronize()
     print(f"[serving] setup {t_setup:.1f} s; {n} requests; plain backlog", file=sys.stderr, flush=True)
     mk_plain, _ = plain(base, False)
-    rays_plain = plain_rays(base)
     print(f"[serving] plain backlog {mk_plain:.2f} s; served backlog", file=sys.stderr, flush=True)
     mk_served, _, rays_served, info_b = served(base, False)
     rate = a.serve_rate or 0.8 * n / mk_plain
@@ -284,21 +287,34 @@ def bench_serving(a):
     def pct(x, q):
         return round(float(np.percentile(np.asarray(x), q)) * 1e3, 2)
 
-    out = {
-        "metric": "served renders: requests/s (saturated) and p50/p99 latency at a fixed Poisson rate",
-        "value": round(n / mk_served, 3), "unit": "requests/s", "n_gpus": 1, "steps": n, "warmup": 1,
-        "ms_per_step": round(mk_served / n * 1e3, 2), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "plain_requests_s": round(n / mk_plain, 3), "served_over_plain": round(mk_plain / mk_served, 3),
-        "served_mrays_s": round(rays_served / mk_served / 1e6, 1), "plain_mrays_s": round(rays_plain / mk_plain / 1e6, 1),
+    return {
+        "requests": n,
+        "served_requests_s": round(n / mk_served, 3), "plain_requests_s": round(n / mk_plain, 3),
+        "served_over_plain": round(mk_plain / mk_served, 3),
+        # both disciplines trace the same paths (each served image equals its alone render)
+        "served_mrays_s": round(rays_served / mk_served / 1e6, 1), "plain_mrays_s": round(rays_served / mk_plain / 1e6, 1),
         "rate_per_s": round(rate, 3),
         "latency_ms": {"served_p50": pct(lat_served, 50), "served_p99": pct(lat_served, 99),
                        "plain_p50": pct(lat_plain, 50), "plain_p99": pct(lat_plain, 99)},
+        "plain": "DSL path tracer (AST -> HIP -> hiprtc megakernel), one dispatch per request",
         "served_backlog": info_b, "served_at_rate": info_r,
-        "config": {"workload": f"{n} requests over 4 resident scenes (icosphere L6, soup 16.7M, box room 3.5M, "
-                               f"1,024-instance TLAS); res U{{256..2048}}^2, spp U{{1,4,16,64}}, depth 8; "
-                               f"budget {a.serve_budget_mib} MiB, wave {a.serve_wave_paths} paths"},
+        "workload": f"{n} requests over 4 resident scenes (icosphere L6, soup 16.7M, box room 3.5M, "
+                    f"1,024-instance TLAS); res U{{256..2048}}^2, spp U{{1,4,16,64}}, depth 8; "
+                    f"budget {a.serve_budget_mib} MiB, wave {a.serve_wave_paths} paths",
     }
+
+
+def bench_serving(a):
+    """The serving measurement alone as its own line (requests/s is the value)."""
+    r = serving_measure(a, a.serve_requests)
+    out = {
+        "metric": "served renders: requests/s (saturated) and p50/p99 latency at a fixed Poisson rate",
+        "value": r["served_requests_s"], "unit": "requests/s", "n_gpus": 1, "steps": r["requests"], "warmup": 1,
+        "ms_per_step": round(1e3 / r["served_requests_s"], 2), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": r.pop("workload")},
+    }
+    out.update(r)
     print(json.dumps(out), flush=True)
 
 
@@ -473,11 +489,68 @@ def cpu_baseline(verts, tris, rays_f, sample_rays, total_rays):
         "value": round(total_rays / est / 1e6, 4),
         "unit": "Mrays/s",
         "cores": ncores,
+        "cpu_model": cpu_model(),
         "kind": "port",
         "sample": (f"full {len(tris)}-triangle BVH build ({t_build:.2f} s) + closest-hit of the first "
                    f"{sample_rays} rays ({t_trace:.2f} s, {sample_rays / t_trace / 1e6:.3f} Mrays/s); "
                    f"job time estimated as build + trace scaled to {total_rays} rays"),
     }
+
+
+def build_hints(dev, s, mesh, rb, n_rays):
+    """The soup's BLAS under each build hint: build ms, wide nodes, node visits and triangle
+    tests per ray and closest-hit ms. FAST_BUILD (LBVH, 3-triangle leaves) is the timed step's;
+    FAST_TRACE (single-triangle leaves) with and without the SAH refinement of its clusters."""
+    import luisacompute_amd as lc
+    out = {}
+    for name, hint, sah in (("fast_build", lc.AccelOption.FAST_BUILD, True),
+                            ("fast_trace_lbvh", lc.AccelOption.FAST_TRACE, False),
+                            ("fast_trace_sah", lc.AccelOption.FAST_TRACE, True)):
+        dev.set_build_sah(sah)
+        m = dev.create_mesh(mesh.vb, mesh.tb, option=lc.AccelOption(hint, 1, 0))
+        acc = dev.create_accel()
+        acc.emplace_back(m, lc.IDENTITY)
+        s << m.build() << acc.build()
+        s.synchronize()
+        dev.reset_timing()
+        dev.enable_timing(True)
+        for _ in range(3):
+            s << m.build()
+        s << acc.build()
+        for _ in range(3):
+            acc.trace_closest(s, rb, _hits(dev, n_rays), n_rays)
+        s.synchronize()
+        dev.enable_timing(False)
+        bms, nb = dev.timing("mesh_build")
+        tms, nt = dev.timing("trace_closest")
+        nn, ntr = acc.trace_count(s, rb, n_rays)
+        st = m.stats()
+        out[name] = {"build_ms": round(bms / max(nb, 1), 3), "trace_closest_ms": round(tms / max(nt, 1), 3),
+                     "wide_nodes": st.node_count, "depth": st.depth,
+                     "nodes_per_ray": round(nn / n_rays, 3), "tris_per_ray": round(ntr / n_rays, 3)}
+        del acc, m
+    dev.set_build_sah(True)
+    return out
+
+
+_HITS = {}
+
+
+def _hits(dev, n):
+    if n not in _HITS:
+        _HITS[n] = dev.create_buffer(24, n)
+    return _HITS[n]
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def main():
@@ -572,7 +645,8 @@ def main():
     umem = umesh.memory()
     del umesh
 
-    # algorithmic bytes: counter-mode run of the same traversal on the same BVH and rays
+    # algorithmic bytes: counter-mode run of the same traversal (same sorted queue, grid and
+    # wave-uniform steps) on the same BVH and rays
     n_nodes, n_tris = accel.trace_count(s, rb, a.rays)
     st = mesh.stats()
     mem = mesh.memory()
@@ -591,6 +665,8 @@ def main():
         with open(tpath) as f:
             tj = json.load(f)
         traffic, l2_hit = tj.get("hbm_bytes_per_launch"), tj.get("l2_hit_rate")
+
+    hints = build_hints(dev, s, mesh, rb, a.rays)
 
     result = {
         "metric": "Mrays/s incoherent closest-hit @16M tris (BVH build included)",
@@ -638,15 +714,28 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
             "l2_hit_rate": l2_hit,
+            "traffic_source": "profiles/trace_closest_soup16m_traffic.json: FETCH_SIZE + WRITE_SIZE of k_trace<closest> "
+                              "from rocprofv3 --pmc runs of this command, scaled by a 1 GiB copy's counts"
+                              if traffic is not None else None,
+            # the HBM roofline is the one priced here; the counter profile shows the kernel is
+            # limited by VALU issue (divergent lanes), not by bandwidth
+            "limiter": "valu_issue",
             "bytes_per_ray": round(bytes_per_ray, 2),
             "compulsory_floor_gbs": round(floor_gbs, 2),
         },
         "input_gen_s": round(gen_s, 2),
         "trace_config": {"closest": dev.trace_config("closest"), "any": dev.trace_config("any"),
                          "source": "luisacompute_amd/autotune_table.json" if tuned else "defaults"},
+        "build_hints": hints,
     }
     if not a.no_cpu:
         result["cpu_baseline"] = cpu_baseline(verts, tris, lc.rays_as_floats(rays), a.cpu_sample_rays, a.rays)
+    if not a.no_serving:
+        # second half of the metric: served renders (requests/s, p50/p99) beside the plain
+        # per-request DSL megakernel, on the four resident scenes
+        del mesh, accel, rb, hb, hab
+        dev.release_build_arena()
+        result["serving"] = serving_measure(a, a.serve_requests)
     print(json.dumps(result), flush=True)
 
 
