@@ -246,6 +246,16 @@ int lc_hip_accel_trace_count(uint64_t device, uint64_t stream, uint64_t accel, u
                              uint32_t ray_count, uint32_t vis_mask, int any, uint64_t *nodes_visited,
                              uint64_t *triangles_tested);
 
+/* Closest hits that allocate nothing. No counterpart in the reference: for callers that cannot
+   allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+   hipMalloc). The same per-ray results as lc_hip_accel_trace_closest; its working memory (slot
+   counters, traversal stack spill) is a static 32 MiB array per GPU held until the process ends,
+   and rays are traced unsorted. Fails with LC_HIP_ERROR_UNSUPPORTED when one workgroup's stacks
+   need more than that array. Calls that overlap on one GPU share the array and race. */
+int lc_hip_accel_trace_closest_noalloc(uint64_t device, uint64_t stream, uint64_t accel, uint64_t ray_buffer,
+                                       size_t ray_offset, uint64_t hit_buffer, size_t hit_offset, uint32_t ray_count,
+                                       uint32_t vis_mask);
+
 /* ---- ray queries: RayQueryAll / RayQueryAny (accel.traverse / traverse_any with
    on_surface_candidate / on_procedural_candidate, include/luisa/dsl/rtx/ray_query.h:11-243;
    semantics of AccelImpl::ray_query, src/rust/luisa_compute_backend_impl/src/cpu/accel.rs:582-800)

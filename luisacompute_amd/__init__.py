@@ -246,6 +246,7 @@ _SIGNATURES = {
     "lc_hip_accel_instance_read": (ctypes.c_int, [_U64, _U64, _U64, _U32, _U64, _SZ, _U64, _SZ, _U32]),
     "lc_hip_accel_instance_write": (ctypes.c_int, [_U64, _U64, _U64, _U32, _U64, _SZ, _U64, _SZ, _U32]),
     "lc_hip_accel_trace_closest": (ctypes.c_int, [_U64, _U64, _U64, _U64, _SZ, _U64, _SZ, _U32, _U32]),
+    "lc_hip_accel_trace_closest_noalloc": (ctypes.c_int, [_U64, _U64, _U64, _U64, _SZ, _U64, _SZ, _U32, _U32]),
     "lc_hip_accel_trace_any": (ctypes.c_int, [_U64, _U64, _U64, _U64, _SZ, _U64, _SZ, _U32, _U32]),
     "lc_hip_accel_trace_count": (ctypes.c_int, [_U64, _U64, _U64, _U64, _SZ, _U32, _U32, ctypes.c_int,
                                                 ctypes.POINTER(_U64), ctypes.POINTER(_U64)]),

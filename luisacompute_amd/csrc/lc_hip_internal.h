@@ -221,6 +221,11 @@ hipError_t query_launch(const QueryParams &P, hipStream_t stream, int grid);
 
 bool trace_config_supported(int block, int slds);
 int trace_blocks_per_cu(bool any, bool two, bool count, int block, int slds);
+// Working memory of the no-allocation trace (trace.hip): a static device array of
+// kNoAllocScratchBytes per GPU; the lookup runs once per device, with that device current.
+constexpr size_t kNoAllocScratchBytes = 32ull << 20;
+constexpr size_t kNoAllocWordsBytes = 4096; // slot counters, error word, stats; the spill area follows
+hipError_t noalloc_scratch_lookup(void **device_ptr);
 hipError_t trace_launch(const TraceParams &P, hipStream_t stream, bool any, bool two, bool count, int grid, int block,
                         int slds);
 
@@ -285,6 +290,8 @@ struct Device {
     std::vector<Pending> pending;
     std::map<std::string, TimingStat> timings;
     void *pt = nullptr; // path-tracer workspace (pathtracer.hip), released with the device
+    // this GPU's copy of the no-allocation trace's static scratch array (null if the lookup failed)
+    void *noalloc_scratch = nullptr;
 };
 
 // The stream's working memory, created on first use (caller holds dev->mu); null when the

@@ -317,6 +317,11 @@ uint64_t lc_hip_device_create(int32_t index) {
     }
     hipMemset(p, 0, 256);
     dev->error = static_cast<uint32_t *>(p);
+    // the no-allocation trace's scratch array on this GPU (its entry point fails without it)
+    if (noalloc_scratch_lookup(&dev->noalloc_scratch) != hipSuccess) {
+        dev->noalloc_scratch = nullptr;
+        (void)hipGetLastError();
+    }
     if (const char *v = std::getenv("LUISA_HIP_BVH_SAH")) dev->sah_refine = std::atoi(v) != 0 ? 1u : 0u;
     if (const char *v = std::getenv("LUISA_HIP_BVH_WIDTH")) dev->bvh_width = (uint32_t)std::min(8, std::max(2, std::atoi(v)));
     dev->cfg[0] = TraceConfig{256, 16, 32, 0};
@@ -1499,6 +1504,69 @@ int lc_hip_accel_trace_count(uint64_t device, uint64_t stream, uint64_t accel, u
                              uint64_t *triangles_tested) {
     const lc_hip_ray_batch b{ray_buffer, ray_offset, 0, 0, ray_count, vis_mask};
     return trace_batches(device, stream, accel, &b, 1, any != 0, true, nodes_visited, triangles_tested);
+}
+
+// Closest hits without allocating: the working memory that lc_hip_accel_trace_closest takes from
+// the stream's buffers (slot counters, error word, stack spill) comes from this GPU's static
+// scratch array, and the rays are traced in queue order (the ray sort's scratch grows with the
+// ray count). Per-ray results equal lc_hip_accel_trace_closest's. The grid shrinks when the
+// spill area cannot hold a full grid's stacks. Calls that overlap on one GPU share the array
+// and race.
+int lc_hip_accel_trace_closest_noalloc(uint64_t device, uint64_t stream, uint64_t accel, uint64_t ray_buffer,
+                                       size_t ray_offset, uint64_t hit_buffer, size_t hit_offset, uint32_t ray_count,
+                                       uint32_t vis_mask) {
+    Device *dev = obj<Device>(device);
+    Accel *a = obj<Accel>(accel);
+    Buffer *rb = obj<Buffer>(ray_buffer), *hb = obj<Buffer>(hit_buffer);
+    if (!dev || !a || !rb || !hb) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad trace handles");
+    if (!dev->noalloc_scratch) return fail(LC_HIP_ERROR_UNSUPPORTED, "no-allocation scratch array unavailable on this device");
+    if (ray_offset % 16 != 0) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "ray offset must be 16-byte aligned");
+    if (hit_offset % 8 != 0) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "hit offset must be 8-byte aligned");
+    if (ray_offset + (size_t)ray_count * 32 > rb->bytes) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "rays outside the ray buffer");
+    if (hit_offset + (size_t)ray_count * 24 > hb->bytes) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "hits outside the hit buffer");
+    if (ray_count > 0xF0000000u) return fail(LC_HIP_ERROR_UNSUPPORTED, "more than 0xF0000000 rays in one launch");
+    std::lock_guard<std::mutex> lk(a->mu);
+    if (!a->built) return fail(LC_HIP_ERROR_NOT_BUILT, "accel is not built");
+    DeviceGuard g(dev);
+    hipStream_t s = native_stream(stream);
+    if (ray_count == 0) return LC_HIP_OK;
+    TraceParams P;
+    std::memset(&P, 0, sizeof(P));
+    P.batches.n = 1;
+    P.batches.b[0].rays = reinterpret_cast<const float4 *>(static_cast<char *>(rb->ptr) + ray_offset);
+    P.batches.b[0].out = static_cast<char *>(hb->ptr) + hit_offset;
+    P.batches.b[0].start = 0;
+    P.batches.b[0].mask = vis_mask;
+    P.count = ray_count;
+    bool two = false;
+    uint32_t stack_need = 0;
+    const int grc = accel_geometry(a, P.geo, two, stack_need);
+    if (grc != LC_HIP_OK) return grc;
+    std::lock_guard<std::mutex> dl(dev->mu);
+    const TraceConfig &cfg = dev->cfg[0];
+    const int bpc = cfg.blocks_per_cu > 0 ? cfg.blocks_per_cu : trace_blocks_per_cu(false, two, false, cfg.block, cfg.slds);
+    const uint64_t want = ((uint64_t)ray_count + cfg.block - 1) / cfg.block;
+    uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)bpc * dev->cu_count, want));
+    P.spill_depth = stack_need > (uint32_t)cfg.slds ? stack_need - (uint32_t)cfg.slds : 0u;
+    const size_t spill_per_block = (size_t)cfg.block * P.spill_depth * sizeof(uint2);
+    const size_t spill_cap = kNoAllocScratchBytes - kNoAllocWordsBytes;
+    if (spill_per_block > spill_cap)
+        return fail(LC_HIP_ERROR_UNSUPPORTED, "traversal stacks of one workgroup need " + std::to_string(spill_per_block) +
+                                                  " spill bytes, more than the no-allocation scratch holds (" +
+                                                  std::to_string(spill_cap) + ")");
+    if (spill_per_block) grid = std::min<uint64_t>(grid, spill_cap / spill_per_block);
+    static_assert(576 + 2 * sizeof(unsigned long long) <= kNoAllocWordsBytes, "no-allocation words");
+    char *base = static_cast<char *>(dev->noalloc_scratch);
+    P.ray_counter = reinterpret_cast<uint32_t *>(base);
+    P.error = reinterpret_cast<uint32_t *>(base) + kRayCounterWords;
+    P.stats = reinterpret_cast<unsigned long long *>(base + 576);
+    P.spill = reinterpret_cast<uint2 *>(base + kNoAllocWordsBytes);
+    P.steps = cfg.steps;
+    P.fetch_chunk = cfg.fetch_chunk;
+    P.fetch_segments = cfg.fetch_segments;
+    const hipError_t e = trace_launch(P, s, false, two, false, (int)grid, cfg.block, cfg.slds);
+    if (e != hipSuccess) return hip_fail(e, "trace launch");
+    return LC_HIP_OK;
 }
 
 // Ray queries (accel.traverse / traverse_any, include/luisa/dsl/rtx/ray_query.h:11-243) over a

@@ -16,12 +16,43 @@
 
 #include <cfloat>
 #include <cstdint>
+#include <map>
+#include <mutex>
 
 #include "lc_hip_bvh.h"
 #include "lc_hip_internal.h"
 #include "traverse.h"
 
+// Working memory of lc_hip_accel_trace_closest_noalloc: the launch's slot counters, error word
+// and stats in the first kNoAllocWordsBytes, then the per-lane stack spill area. Each GPU holds
+// its own copy for the life of the process; host code reaches it only through the address that
+// noalloc_scratch_lookup returns for the current device.
+static __device__ __align__(256) unsigned char scratch[lchip::kNoAllocScratchBytes];
+
 namespace lchip {
+
+// Address of the current device's copy of `scratch`, looked up on the first call for each device
+// (with that device current) and kept per device index for later calls.
+hipError_t noalloc_scratch_lookup(void **device_ptr) {
+    static std::mutex mu;
+    static std::map<int, void *> by_device;
+    *device_ptr = nullptr;
+    int index = 0;
+    hipError_t e = hipGetDevice(&index);
+    if (e != hipSuccess) return e;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = by_device.find(index);
+    if (it != by_device.end()) {
+        *device_ptr = it->second;
+        return hipSuccess;
+    }
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    by_device.emplace(index, p);
+    *device_ptr = p;
+    return hipSuccess;
+}
 
 // Template parameters: ANY (trace_any), TWO (TLAS + BLAS; otherwise one instance's BLAS),
 // COUNT (count node visits / triangle tests), BLOCK threads, SLDS stack entries in LDS.
