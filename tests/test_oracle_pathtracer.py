@@ -174,3 +174,34 @@ def test_box_room_geometry():
         else:
             assert np.all(n[:, 1] < 0)
     assert np.all((room.albedo[:8] >= 0.1) & (room.albedo[:8] < 0.9))
+
+
+def test_offset_ray_origin_closed_forms():
+    """offset_ray_origin (src/dsl/rtx/ray.cpp:16-24; Waechter and Binder, "A Fast and Robust Method
+    for Avoiding Self-Intersection", Ray Tracing Gems 2019) pinned on values fixed by its three
+    constants: components with |p| < 1/32 move by n / 65536 in floating point; the others move by
+    trunc(256 n) units in the last place, away from zero where n points that way."""
+    f = np.float32
+
+    def off(p, n):
+        return ob.offset_ray_origin(np.array([p], np.float32), np.array([n], np.float32))[0]
+    # near the origin: p + n * 2^-16
+    got = off((0.01, 0.0, -0.02), (1.0, 0.0, 0.0))
+    assert got[0] == f(0.01) + f(2.0 ** -16) and got[1] == 0.0 and got[2] == f(-0.02)
+    # 0.5 has an ulp of 2^-24: 256 ulps = 2^-16; -3.0 (ulp 2^-22) moved further negative by 2^-14
+    got = off((-3.0, 1.0, 0.5), (-1.0, 0.0, 1.0))
+    assert got[0] == f(-3.0 - 2.0 ** -14) and got[1] == 1.0 and got[2] == f(0.5 + 2.0 ** -16)
+    # |p| = 1/32 exactly is outside the float branch: 256 ulps of 2^-5 (ulp 2^-28) = 2^-20;
+    # just below it the float branch adds 2^-16
+    got = off((1.0 / 32.0, 0.0, 0.0), (1.0, 0.0, 0.0))
+    assert got[0] == f(1.0 / 32.0 + 2.0 ** -20)
+    below = np.nextafter(f(1.0 / 32.0), f(0.0))
+    got = off((below, 0.0, 0.0), (1.0, 0.0, 0.0))
+    assert got[0] == below + f(2.0 ** -16)
+    # the integer offset truncates 256 n: n = 0.5 -> 128 ulps, n = 0.999 -> 255 ulps, n = -0.999
+    # on a positive coordinate -> 255 steps of the bit pattern toward zero (below 4.0 the ulp is
+    # 2^-22)
+    got = off((1.0, 2.0, 4.0), (0.5, 0.999, -0.999))
+    assert got[0] == f(1.0 + 128 * 2.0 ** -23)
+    assert got[1] == f(2.0 + 255 * 2.0 ** -22)
+    assert got[2] == f(4.0 - 255 * 2.0 ** -22)
