@@ -37,7 +37,7 @@ def parse():
     ap.add_argument("--cpu-sample-rays", type=int, default=1 << 18)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run 3 device-to-device copies of this many MiB (counter calibration) and exit")
-    ap.add_argument("--workload", default="soup", choices=["soup", "instanced", "pathtracer", "serving", "query"],
+    ap.add_argument("--workload", default="soup", choices=["soup", "instanced", "pathtracer", "serving", "query", "build"],
                     help="soup: the headline line; instanced: 64 ray batches over a 1,024-instance TLAS; "
                          "pathtracer: box room 1920x1080, 64 spp, depth 8 (wavefront vs megakernel); "
                          "serving: render requests over 4 resident scenes (served vs plain)")
@@ -453,6 +453,40 @@ def bench_query(a):
         "dtype": "f32", "data": "synthetic"}), flush=True)
 
 
+def bench_build(a):
+    """BLAS builds of the soup (16,777,216 triangles) under each hint: FAST_BUILD (LBVH, 3-triangle
+    leaves), FAST_TRACE without and with the binned-SAH refinement; `steps` timed builds each."""
+    import luisacompute_amd as lc
+    from luisacompute_amd import host
+    verts, tris = lc.soup(a.tris)
+    dev = lc.Device(0)
+    s = dev.create_stream()
+    res = {}
+    for name, hint, sah in (("fast_build", lc.AccelOption.FAST_BUILD, True),
+                            ("fast_trace_lbvh", lc.AccelOption.FAST_TRACE, False),
+                            ("fast_trace_sah", lc.AccelOption.FAST_TRACE, True)):
+        dev.set_build_sah(sah)
+        m = host.mesh_from_arrays(dev, s, verts, tris, build=False, option=lc.AccelOption(hint, 1, 0))
+        for _ in range(max(1, a.warmup)):
+            s << m.build()
+        s.synchronize()
+        dev.reset_timing()
+        dev.enable_timing(True)
+        for _ in range(a.steps):
+            s << m.build()
+        s.synchronize()
+        dev.enable_timing(False)
+        ms, n = dev.timing("mesh_build")
+        st = m.stats()
+        res[name] = {"build_ms": round(ms / max(n, 1), 3), "wide_nodes": st.node_count, "depth": st.depth}
+        del m
+    dev.set_build_sah(True)
+    print(json.dumps({"metric": "BLAS build ms, soup 16.7M triangles, per hint", "value": res["fast_build"]["build_ms"],
+                      "unit": "ms", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "higher_is_better": False,
+                      "hints": res, "config": {"workload": f"soup {a.tris:,} tris (U[0,100]^3, sigma 0.5)"},
+                      "dtype": "f32", "data": "synthetic"}), flush=True)
+
+
 def copy_probe(mib):
     """Plain copy of known size: reading FETCH_SIZE / WRITE_SIZE for it calibrates the counters."""
     import luisacompute_amd as lc
@@ -573,6 +607,9 @@ def main():
         return
     if a.workload == "query":
         bench_query(a)
+        return
+    if a.workload == "build":
+        bench_build(a)
         return
     import luisacompute_amd as lc
     from luisacompute_amd import host

@@ -534,6 +534,7 @@ __device__ __forceinline__ uint32_t ord_u32(float f) {
 // internal nodes take the indices the old subtree's internal nodes held, f + 1 .. l - 1, and the
 // cluster root keeps its own index (its parent's link is untouched).
 constexpr uint32_t kSahCluster = 64; // primitives per cluster (one per lane of a wave)
+static_assert(kSahCluster == 64, "k_sah_clusters runs one wave64 per workgroup");
 constexpr int kSahBins = 16;         // bins per axis: 15 split candidates per axis, 45 lanes
 
 // Cluster roots in position order: the thread at position `pos` climbs from leaf pos while the
@@ -566,8 +567,29 @@ __global__ __launch_bounds__(kPhaseBlock) void k_sah_cut(uint32_t n, BinaryTree 
     compact_deferred(found, pend, bcount);
 }
 
-__device__ __forceinline__ float wave_sum_min(float v, bool on) { return wave_min(on ? v : FLT_MAX); }
-__device__ __forceinline__ float wave_sum_max(float v, bool on) { return wave_max(on ? v : -FLT_MAX); }
+// Wave64 reductions through DPP row operations (VALU lane moves, no LDS round trips): pairs,
+// quads, half rows, rows, then rows 1 / 3 take lane 15 / 47 and rows 2 / 3 lane 31; lane 63 ends
+// with the whole wave's value, broadcast with readlane. Every lane of the wave takes part.
+template<bool MAX, int CTRL, int ROW_MASK>
+__device__ __forceinline__ float dpp_step(float v) {
+    const int o = __builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), CTRL, ROW_MASK, 0xF, false);
+    return MAX ? fmaxf(v, __int_as_float(o)) : fminf(v, __int_as_float(o));
+}
+template<bool MAX>
+__device__ __forceinline__ float wave_reduce(float v) {
+    v = dpp_step<MAX, 0xB1, 0xF>(v);  // quad_perm [1,0,3,2]
+    v = dpp_step<MAX, 0x4E, 0xF>(v);  // quad_perm [2,3,0,1]
+    v = dpp_step<MAX, 0x141, 0xF>(v); // row_half_mirror
+    v = dpp_step<MAX, 0x140, 0xF>(v); // row_mirror
+    v = dpp_step<MAX, 0x142, 0xA>(v); // row_bcast:15 into rows 1 and 3
+    v = dpp_step<MAX, 0x143, 0xC>(v); // row_bcast:31 into rows 2 and 3
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+__device__ __forceinline__ float wave_sum_min(float v, bool on) { return wave_reduce<false>(on ? v : FLT_MAX); }
+__device__ __forceinline__ float wave_sum_max(float v, bool on) { return wave_reduce<true>(on ? v : -FLT_MAX); }
+__device__ __forceinline__ float lane_value(float v, uint32_t lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), (int)lane));
+}
 
 // Bin of a doubled centroid (NaN and values below the first boundary go to bin 0).
 __device__ __forceinline__ int sah_bin(float cv, float cmin, float scale) {
@@ -575,19 +597,27 @@ __device__ __forceinline__ int sah_bin(float cv, float cmin, float scale) {
     return x >= 1.0f ? min(kSahBins - 1, (int)fminf(x, (float)kSahBins)) : 0;
 }
 
-// One wave (one workgroup of 64) per cluster, clusters taken in a grid-stride loop. Primitive
-// boxes and doubled centroids sit in LDS by local index (the cluster's input order); s_perm maps
-// positions to local indices. A task (positions [a, b], node index) is split by the lanes: lane
-// L < 45 scores candidate (axis L / 15, boundary L % 15 + 1) over the task's primitives, the
-// cheapest area(left) * n_left + area(right) * n_right wins (lowest lane on ties), and a stable
-// partition reorders the positions. Tasks of two primitives need no search. Centroids that
-// coincide on every axis are split at the middle position.
+// Orders the LDS accesses of one wave (LDS operations of a wave complete in order; this keeps the
+// compiler from moving them across the point). Unlike __syncthreads it does not wait for the
+// wave's global stores, so a workgroup of exactly one wave uses it in place of a barrier.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// One wave (one workgroup of 64) per cluster, clusters taken in a grid-stride loop. The cluster's
+// primitives sit in LDS in position order (box, doubled centroid, local index) and move when a
+// task is partitioned, so the candidate scans read consecutive LDS words. A task (positions
+// [a, b], node index) is split by the lanes: lane L < 45 scores candidate (axis L / 15, boundary
+// L % 15 + 1) over the task's primitives, the cheapest area(left) * n_left + area(right) * n_right
+// wins (lowest lane on ties), and a stable partition reorders the positions. Tasks of two
+// primitives need no search. Centroids that coincide on every axis are split at the middle.
 __global__ __launch_bounds__(64) void k_sah_clusters(const uint32_t *entries, const uint32_t *count_ptr, BinaryTree bt,
                                                      uint32_t *ids, const TriRecord *prec, TriRecord *srec,
                                                      const float4 *pbox) {
     __shared__ float s_box[6][kSahCluster];
     __shared__ float s_cen[3][kSahCluster];
-    __shared__ uint32_t s_perm[kSahCluster];
+    __shared__ uint32_t s_loc[kSahCluster];
     __shared__ uint32_t s_pos[kSahCluster];
     __shared__ uint2 s_task[kSahCluster];
     const uint32_t lane = threadIdx.x;
@@ -607,8 +637,8 @@ __global__ __launch_bounds__(64) void k_sah_clusters(const uint32_t *entries, co
         const bool mine = lane < m;
         uint32_t id = 0;
         float4 r0 = make_float4(0.f, 0.f, 0.f, 0.f), r1 = r0, r2 = r0;
-        float3 lo = make_float3(0.f, 0.f, 0.f), hi = lo;
         if (mine) {
+            float3 lo, hi;
             id = ids[f + lane];
             if (srec) {
                 const float4 *src = reinterpret_cast<const float4 *>(prec + id);
@@ -623,19 +653,23 @@ __global__ __launch_bounds__(64) void k_sah_clusters(const uint32_t *entries, co
             s_box[0][lane] = lo.x; s_box[1][lane] = lo.y; s_box[2][lane] = lo.z;
             s_box[3][lane] = hi.x; s_box[4][lane] = hi.y; s_box[5][lane] = hi.z;
             s_cen[0][lane] = lo.x + hi.x; s_cen[1][lane] = lo.y + hi.y; s_cen[2][lane] = lo.z + hi.z;
-            s_perm[lane] = lane;
+            s_loc[lane] = lane;
         }
         if (lane == 0u) s_task[0] = make_uint2(m - 1u, ref); // positions [0, m - 1] packed as b, root index
-        __syncthreads();
+        wave_sync();
         int sp = 1;
         while (sp > 0) { // uniform: every lane pops the same task
             sp--;
             const uint2 task = s_task[sp];
             const uint32_t a = task.x >> 16, b = task.x & 0xFFFFu, idx = task.y;
             const uint32_t mt = b - a + 1u;
-            // the task's primitives: lane j holds position a + j
+            // lane j holds the primitive at position a + j
             const bool on = lane < mt;
-            const uint32_t pj = on ? s_perm[a + lane] : 0u;
+            const uint32_t pj = on ? a + lane : a;
+            const float3 blo = make_float3(s_box[0][pj], s_box[1][pj], s_box[2][pj]);
+            const float3 bhi = make_float3(s_box[3][pj], s_box[4][pj], s_box[5][pj]);
+            const float bc[3] = {s_cen[0][pj], s_cen[1][pj], s_cen[2][pj]};
+            const uint32_t bloc = s_loc[pj];
             uint32_t nl = 1u;
             float3 llo, lhi, rlo, rhi;
             bool by_bins = false;
@@ -643,58 +677,52 @@ __global__ __launch_bounds__(64) void k_sah_clusters(const uint32_t *entries, co
             float cmin[3] = {0.f, 0.f, 0.f}, cscale[3] = {0.f, 0.f, 0.f};
             if (mt > 2u) {
                 for (int ax = 0; ax < 3; ax++) {
-                    const float cv = s_cen[ax][pj];
-                    const float mn = wave_sum_min(cv, on), mx = wave_sum_max(cv, on);
+                    const float mn = wave_sum_min(bc[ax], on), mx = wave_sum_max(bc[ax], on);
                     cmin[ax] = mn;
                     cscale[ax] = mx > mn ? fminf((float)kSahBins / (mx - mn), FLT_MAX) : 0.0f;
                 }
-                // candidate of this lane
+                // candidate of this lane: a scan of the task's positions
                 float cost = FLT_MAX;
                 float3 clo = make_float3(FLT_MAX, FLT_MAX, FLT_MAX), chi = make_float3(-FLT_MAX, -FLT_MAX, -FLT_MAX);
                 float3 dlo = clo, dhi = chi;
                 const uint32_t ax = lane / (kSahBins - 1), k = lane % (kSahBins - 1) + 1u;
                 if (lane < 3u * (kSahBins - 1) && cscale[ax] > 0.0f) {
+                    const float *cen = s_cen[ax];
                     uint32_t cl = 0;
-                    for (uint32_t j = 0; j < mt; j++) {
-                        const uint32_t q = s_perm[a + j];
-                        const float cv = s_cen[ax][q];
-                        const int bin = sah_bin(cv, cmin[ax], cscale[ax]);
-                        const float3 blo = make_float3(s_box[0][q], s_box[1][q], s_box[2][q]);
-                        const float3 bhi = make_float3(s_box[3][q], s_box[4][q], s_box[5][q]);
-                        if ((uint32_t)bin < k) {
+#pragma unroll 4
+                    for (uint32_t q = a; q <= b; q++) {
+                        const float3 qlo = make_float3(s_box[0][q], s_box[1][q], s_box[2][q]);
+                        const float3 qhi = make_float3(s_box[3][q], s_box[4][q], s_box[5][q]);
+                        if ((uint32_t)sah_bin(cen[q], cmin[ax], cscale[ax]) < k) {
                             cl++;
-                            clo = make_float3(fminf(clo.x, blo.x), fminf(clo.y, blo.y), fminf(clo.z, blo.z));
-                            chi = make_float3(fmaxf(chi.x, bhi.x), fmaxf(chi.y, bhi.y), fmaxf(chi.z, bhi.z));
+                            clo = make_float3(fminf(clo.x, qlo.x), fminf(clo.y, qlo.y), fminf(clo.z, qlo.z));
+                            chi = make_float3(fmaxf(chi.x, qhi.x), fmaxf(chi.y, qhi.y), fmaxf(chi.z, qhi.z));
                         } else {
-                            dlo = make_float3(fminf(dlo.x, blo.x), fminf(dlo.y, blo.y), fminf(dlo.z, blo.z));
-                            dhi = make_float3(fmaxf(dhi.x, bhi.x), fmaxf(dhi.y, bhi.y), fmaxf(dhi.z, bhi.z));
+                            dlo = make_float3(fminf(dlo.x, qlo.x), fminf(dlo.y, qlo.y), fminf(dlo.z, qlo.z));
+                            dhi = make_float3(fmaxf(dhi.x, qhi.x), fmaxf(dhi.y, qhi.y), fmaxf(dhi.z, qhi.z));
                         }
                     }
                     if (cl > 0u && cl < mt)
                         cost = box_area(clo, chi) * (float)cl + box_area(dlo, dhi) * (float)(mt - cl);
                 }
-                // cheapest candidate, lowest lane on ties
-                uint64_t key = ((uint64_t)ord_u32(cost) << 32) | lane;
-                for (int o = 32; o > 0; o >>= 1) {
-                    const uint64_t w = __shfl_xor(key, o, 64);
-                    key = w < key ? w : key;
-                }
-                const uint32_t win = (uint32_t)key;
-                if (__shfl(cost, (int)win, 64) < FLT_MAX) {
+                // cheapest candidate, lowest lane on ties (NaN costs never win)
+                const float best = wave_sum_min(cost == cost ? cost : FLT_MAX, true);
+                const uint64_t at = __ballot(cost == best);
+                const uint32_t win = at ? (uint32_t)__ffsll((long long)at) - 1u : 0u;
+                if (best < FLT_MAX && at != 0u && win < 3u * (kSahBins - 1)) {
                     by_bins = true;
                     best_axis = win / (kSahBins - 1);
                     best_k = win % (kSahBins - 1) + 1u;
-                    llo = make_float3(__shfl(clo.x, (int)win, 64), __shfl(clo.y, (int)win, 64), __shfl(clo.z, (int)win, 64));
-                    lhi = make_float3(__shfl(chi.x, (int)win, 64), __shfl(chi.y, (int)win, 64), __shfl(chi.z, (int)win, 64));
-                    rlo = make_float3(__shfl(dlo.x, (int)win, 64), __shfl(dlo.y, (int)win, 64), __shfl(dlo.z, (int)win, 64));
-                    rhi = make_float3(__shfl(dhi.x, (int)win, 64), __shfl(dhi.y, (int)win, 64), __shfl(dhi.z, (int)win, 64));
+                    llo = make_float3(lane_value(clo.x, win), lane_value(clo.y, win), lane_value(clo.z, win));
+                    lhi = make_float3(lane_value(chi.x, win), lane_value(chi.y, win), lane_value(chi.z, win));
+                    rlo = make_float3(lane_value(dlo.x, win), lane_value(dlo.y, win), lane_value(dlo.z, win));
+                    rhi = make_float3(lane_value(dhi.x, win), lane_value(dhi.y, win), lane_value(dhi.z, win));
                 }
             }
             // partition the task's positions: left = bins below best_k (or the first half)
             bool left;
             if (by_bins) {
-                const float cv = s_cen[best_axis][pj];
-                left = sah_bin(cv, cmin[best_axis], cscale[best_axis]) < (int)best_k;
+                left = sah_bin(bc[best_axis], cmin[best_axis], cscale[best_axis]) < (int)best_k;
             } else {
                 left = lane < mt / 2u;
             }
@@ -702,24 +730,24 @@ __global__ __launch_bounds__(64) void k_sah_clusters(const uint32_t *entries, co
             nl = (uint32_t)__popcll(lmask);
             const uint64_t below = (1ull << lane) - 1ull;
             if (!by_bins && mt == 2u) { // two primitives: one each side
-                const uint32_t ql = s_perm[a], qr = s_perm[a + 1u];
-                llo = make_float3(s_box[0][ql], s_box[1][ql], s_box[2][ql]);
-                lhi = make_float3(s_box[3][ql], s_box[4][ql], s_box[5][ql]);
-                rlo = make_float3(s_box[0][qr], s_box[1][qr], s_box[2][qr]);
-                rhi = make_float3(s_box[3][qr], s_box[4][qr], s_box[5][qr]);
+                llo = make_float3(s_box[0][a], s_box[1][a], s_box[2][a]);
+                lhi = make_float3(s_box[3][a], s_box[4][a], s_box[5][a]);
+                rlo = make_float3(s_box[0][b], s_box[1][b], s_box[2][b]);
+                rhi = make_float3(s_box[3][b], s_box[4][b], s_box[5][b]);
             } else if (!by_bins) { // boxes of the halves
-                const float3 blo = make_float3(s_box[0][pj], s_box[1][pj], s_box[2][pj]);
-                const float3 bhi = make_float3(s_box[3][pj], s_box[4][pj], s_box[5][pj]);
                 const bool il = on && left, ir = on && !left;
                 llo = make_float3(wave_sum_min(blo.x, il), wave_sum_min(blo.y, il), wave_sum_min(blo.z, il));
                 lhi = make_float3(wave_sum_max(bhi.x, il), wave_sum_max(bhi.y, il), wave_sum_max(bhi.z, il));
                 rlo = make_float3(wave_sum_min(blo.x, ir), wave_sum_min(blo.y, ir), wave_sum_min(blo.z, ir));
                 rhi = make_float3(wave_sum_max(bhi.x, ir), wave_sum_max(bhi.y, ir), wave_sum_max(bhi.z, ir));
             }
-            __syncthreads(); // every lane has read s_perm for this task
-            if (on) {
-                const uint32_t r = left ? (uint32_t)__popcll(lmask & below) : nl + (uint32_t)__popcll(rmask & below);
-                s_perm[a + r] = pj;
+            wave_sync(); // every lane has read the task's positions
+            if (on && mt > 2u) { // move the primitives to their partitioned positions
+                const uint32_t r = a + (left ? (uint32_t)__popcll(lmask & below) : nl + (uint32_t)__popcll(rmask & below));
+                s_box[0][r] = blo.x; s_box[1][r] = blo.y; s_box[2][r] = blo.z;
+                s_box[3][r] = bhi.x; s_box[4][r] = bhi.y; s_box[5][r] = bhi.z;
+                s_cen[0][r] = bc[0]; s_cen[1][r] = bc[1]; s_cen[2][r] = bc[2];
+                s_loc[r] = bloc;
             }
             // the node: split after position g; children at g (left) and g + 1 (right)
             const uint32_t g = a + nl - 1u;
@@ -737,11 +765,11 @@ __global__ __launch_bounds__(64) void k_sah_clusters(const uint32_t *entries, co
                 if (nl >= 2u) s_task[sp + (mt - nl >= 2u ? 1 : 0)] = make_uint2((a << 16) | g, f + g);
             }
             sp += (mt - nl >= 2u ? 1 : 0) + (nl >= 2u ? 1 : 0);
-            __syncthreads(); // s_perm and the task stack written
+            wave_sync(); // positions and the task stack written
         }
         // primitives to their final positions
-        if (mine) s_pos[s_perm[lane]] = lane;
-        __syncthreads();
+        if (mine) s_pos[s_loc[lane]] = lane;
+        wave_sync();
         if (mine) {
             const uint32_t pos = f + s_pos[lane];
             ids[pos] = id;
@@ -752,7 +780,7 @@ __global__ __launch_bounds__(64) void k_sah_clusters(const uint32_t *entries, co
                 dst[2] = r2;
             }
         }
-        __syncthreads(); // LDS reused by the next cluster
+        wave_sync(); // LDS reused by the next cluster
     }
 }
 
@@ -1398,7 +1426,14 @@ hipError_t bvh_build(hipStream_t stream, void *scratch, size_t scratch_bytes, co
             hipLaunchKernelGGL(k_gather_deferred, dim3(nblocks), dim3(kPhaseBlock), 0, stream, (const uint32_t *)pend,
                                (const uint32_t *)L.bcount, (const uint32_t *)L.boff, kPhaseBlock, nblocks, ent[which],
                                dcount);
-            const uint32_t grid_sah = std::max<uint32_t>(1u, std::min<uint32_t>((uint32_t)std::max(1, cu_count) * 32u, n / 2u));
+            // one pass of resident workgroups (a second partial pass would double the time)
+            static const int sah_blocks_per_cu = [] {
+                int b = 0;
+                if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_sah_clusters, 64, 0) != hipSuccess || b < 1) b = 1;
+                return b;
+            }();
+            const uint32_t grid_sah = std::max<uint32_t>(
+                1u, std::min<uint32_t>((uint32_t)std::max(1, cu_count) * (uint32_t)sah_blocks_per_cu, n / 2u));
             hipLaunchKernelGGL(k_sah_clusters, dim3(grid_sah), dim3(64), 0, stream, (const uint32_t *)ent[which],
                                (const uint32_t *)dcount, bt, ids, (const TriRecord *)prec, srec, boxes);
             e = hipMemcpyAsync(pinned_word, dcount, sizeof(uint32_t), hipMemcpyDeviceToHost, stream);
