@@ -53,6 +53,7 @@ def parse():
     ap.add_argument("--pt-sort", type=int, default=None, help="wavefront: bucket-sort continuation rays (default: tuned)")
     ap.add_argument("--pt-fetch", default=None, help="wavefront traces: slots per claim (0: exact), queue segments (default: tuned)")
     ap.add_argument("--pt-no-mega", action="store_true", help="skip the megakernel comparison")
+    ap.add_argument("--pt-no-dsl-wavefront", action="store_true", help="skip the DSL wavefront form")
     a = ap.parse_args()
     # per-call knobs the autotuner chose (luisacompute_amd/autotune_table.json) unless given
     from luisacompute_amd import autotune
@@ -148,6 +149,27 @@ def bench_pathtracer(a):
         dimage = np.zeros((h, w, 4), np.float32)
         s << dimg.copy_to(dimage) << lc.synchronize()
         dsl_same = bool((dimage.view(np.uint32) == image.view(np.uint32)).all())
+    # the wavefront form written in the DSL (camera / shade / resolve kernels over queues, the
+    # library's persistent trace on the device-counted queues; one sample's launches captured
+    # into a graph and replayed per sample): same image as the DSL megakernel
+    dwf_ms = dwf_same = None
+    if not a.pt_no_dsl_wavefront:
+        from luisacompute_amd import programs
+        bind = programs.PathTracerBindings(dev, s, accel, inst, scene)
+        wfd = programs.WavefrontPathTracer(dev, s, bind, w, h)
+        wimg = dev.create_buffer(16, w * h)
+        wfd.render(s, wimg, spp, sequence=3 << 32)  # first render: eager sample + capture
+        s.synchronize()
+        reps = max(1, a.steps // 2)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            wfd.render(s, wimg, spp, sequence=3 << 32)
+        s.synchronize()
+        dwf_ms = (time.perf_counter() - t0) / reps * 1e3
+        wimage = np.zeros((h, w, 4), np.float32)
+        s << wimg.copy_to(wimage) << lc.synchronize()
+        dwf_same = bool((wimage.view(np.uint32) == image.view(np.uint32)).all())
+        del wfd
     out = {
         "metric": "Mrays/s path tracing, box room 1920x1080 64 spp depth 8",
         "value": round(rays / (wf_ms * 1e-3) / 1e6, 2), "unit": "Mrays/s", "n_gpus": 1, "steps": a.steps,
@@ -162,6 +184,10 @@ def bench_pathtracer(a):
         "dsl_megakernel_ms": dsl_ms and round(dsl_ms, 3),
         "dsl_megakernel_mrays_s": dsl_ms and round(rays / (dsl_ms * 1e-3) / 1e6, 2),
         "dsl_megakernel_same_image": dsl_same,
+        "dsl_wavefront_ms": dwf_ms and round(dwf_ms, 3),
+        "dsl_wavefront_mrays_s": dwf_ms and round(rays / (dwf_ms * 1e-3) / 1e6, 2),
+        "dsl_wavefront_over_dsl_megakernel": (dwf_ms and dsl_ms) and round(dsl_ms / dwf_ms, 3),
+        "dsl_wavefront_same_image": dwf_same,
         "image_mean": float(image[..., :3].mean()),
         "config": {"workload": f"box room {room.triangle_count} tris, {w}x{h}, {spp} spp, depth 8, RR from 3, "
                                f"chunk_spp {a.pt_chunk or 'auto'}, sort {a.pt_sort}, fetch {a.pt_fetch}, hipGraph replay"},

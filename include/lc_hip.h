@@ -151,6 +151,19 @@ int lc_hip_stream_synchronize(uint64_t device, uint64_t stream);
 int lc_hip_stream_add_callback(uint64_t device, uint64_t stream, void (*fn)(void *), void *user);
 void *lc_hip_stream_native(uint64_t device, uint64_t stream);
 
+/* ---- stream capture into HIP graphs. The reference's CUDA backend encodes each CommandList it is
+   given (CUDAStream::dispatch, cuda_stream.cpp:194-203); a sequence submitted many times is
+   recorded once here and replayed. Between begin and end the stream's shader dispatches, traces and
+   copies are recorded, not run; traces run unsorted and within the working memory the stream already
+   holds (run the sequence once before capturing it; otherwise LC_HIP_ERROR_UNSUPPORTED). Host
+   waits, callbacks, events and builds are not capturable. A graph replays on any stream; its traces
+   use the capturing stream's working memory, so replays must not overlap that stream's own traces.
+   Destroy graphs before their device. ---- */
+int lc_hip_stream_capture_begin(uint64_t device, uint64_t stream);
+int lc_hip_stream_capture_end(uint64_t device, uint64_t stream, uint64_t *graph);
+int lc_hip_graph_launch(uint64_t device, uint64_t stream, uint64_t graph);
+void lc_hip_graph_destroy(uint64_t device, uint64_t graph);
+
 /* ---- events: timeline fences (device_interface.h:163-168; CUDAEvent, cuda_event.cpp:21-63, a
    timeline semaphore). signal(stream, v) issues value v at the stream's current point; the
    timeline has reached v once a signal of a value >= v completed on the GPU (the initial value is
@@ -241,6 +254,17 @@ int lc_hip_accel_trace_closest_batched(uint64_t device, uint64_t stream, uint64_
                                        const lc_hip_ray_batch *batches, size_t batch_count);
 int lc_hip_accel_trace_any_batched(uint64_t device, uint64_t stream, uint64_t accel, const lc_hip_ray_batch *batches,
                                    size_t batch_count);
+/* Traces over a queue whose length a kernel produced on the device: rays and hit slots for
+   `capacity` rays; the launch traces the first min(count, capacity), count being the uint32 at
+   count_offset of count_buffer when the launch runs (a kernel dispatched over an
+   IndirectDispatchBuffer calling accel.trace_closest / trace_any, include/luisa/runtime/
+   dispatch_buffer.h; cuda_device_resource.h:1783-1829). Rays are traced in queue order. */
+int lc_hip_accel_trace_closest_indirect(uint64_t device, uint64_t stream, uint64_t accel, uint64_t ray_buffer,
+                                        size_t ray_offset, uint64_t hit_buffer, size_t hit_offset, uint32_t capacity,
+                                        uint64_t count_buffer, size_t count_offset, uint32_t vis_mask);
+int lc_hip_accel_trace_any_indirect(uint64_t device, uint64_t stream, uint64_t accel, uint64_t ray_buffer,
+                                    size_t ray_offset, uint64_t hit_buffer, size_t hit_offset, uint32_t capacity,
+                                    uint64_t count_buffer, size_t count_offset, uint32_t vis_mask);
 /* Counter-mode run of the same traversal: total BVH nodes visited and triangles tested. */
 int lc_hip_accel_trace_count(uint64_t device, uint64_t stream, uint64_t accel, uint64_t ray_buffer, size_t ray_offset,
                              uint32_t ray_count, uint32_t vis_mask, int any, uint64_t *nodes_visited,

@@ -201,6 +201,10 @@ _SIGNATURES = {
     "lc_hip_stream_synchronize": (ctypes.c_int, [_U64, _U64]),
     "lc_hip_stream_add_callback": (ctypes.c_int, [_U64, _U64, _P, _P]),
     "lc_hip_stream_native": (_P, [_U64, _U64]),
+    "lc_hip_stream_capture_begin": (ctypes.c_int, [_U64, _U64]),
+    "lc_hip_stream_capture_end": (ctypes.c_int, [_U64, _U64, ctypes.POINTER(_U64)]),
+    "lc_hip_graph_launch": (ctypes.c_int, [_U64, _U64, _U64]),
+    "lc_hip_graph_destroy": (None, [_U64, _U64]),
     "lc_hip_event_create": (_U64, [_U64]),
     "lc_hip_event_destroy": (None, [_U64, _U64]),
     "lc_hip_event_signal": (ctypes.c_int, [_U64, _U64, _U64, _U64]),
@@ -247,6 +251,10 @@ _SIGNATURES = {
     "lc_hip_accel_instance_write": (ctypes.c_int, [_U64, _U64, _U64, _U32, _U64, _SZ, _U64, _SZ, _U32]),
     "lc_hip_accel_trace_closest": (ctypes.c_int, [_U64, _U64, _U64, _U64, _SZ, _U64, _SZ, _U32, _U32]),
     "lc_hip_accel_trace_closest_noalloc": (ctypes.c_int, [_U64, _U64, _U64, _U64, _SZ, _U64, _SZ, _U32, _U32]),
+    "lc_hip_accel_trace_closest_indirect": (ctypes.c_int, [_U64, _U64, _U64, _U64, _SZ, _U64, _SZ, _U32, _U64, _SZ,
+                                                           _U32]),
+    "lc_hip_accel_trace_any_indirect": (ctypes.c_int, [_U64, _U64, _U64, _U64, _SZ, _U64, _SZ, _U32, _U64, _SZ,
+                                                       _U32]),
     "lc_hip_accel_trace_any": (ctypes.c_int, [_U64, _U64, _U64, _U64, _SZ, _U64, _SZ, _U32, _U32]),
     "lc_hip_accel_trace_count": (ctypes.c_int, [_U64, _U64, _U64, _U64, _SZ, _U32, _U32, ctypes.c_int,
                                                 ctypes.POINTER(_U64), ctypes.POINTER(_U64)]),
@@ -583,6 +591,47 @@ class Stream:
 
     def native(self):
         return lib().lc_hip_stream_native(self.device.handle, self.handle)
+
+    def capture(self):
+        """Records the launches issued on this stream inside the `with` block into a Graph instead
+        of running them (lc_hip_stream_capture_begin / _end): `with stream.capture() as g: ...`,
+        then `g.graph.launch(stream)`. Run the sequence once before capturing it."""
+        return _Capture(self)
+
+
+class Graph:
+    """A captured launch sequence (a HIP graph); launch(stream) replays it."""
+
+    def __init__(self, device, handle):
+        self.device, self.handle = device, handle
+
+    def launch(self, stream):
+        _check(lib().lc_hip_graph_launch(self.device.handle, stream.handle, self.handle), "graph launch")
+
+    def __del__(self):
+        if getattr(self, "handle", None) is not None and _lib is not None and self.device.handle is not None:
+            lib().lc_hip_graph_destroy(self.device.handle, self.handle)
+            self.handle = None
+
+
+class _Capture:
+    def __init__(self, stream):
+        self.stream, self.graph = stream, None
+
+    def __enter__(self):
+        _check(lib().lc_hip_stream_capture_begin(self.stream.device.handle, self.stream.handle), "capture begin")
+        return self
+
+    def __exit__(self, exc_type, exc, tb):
+        h = ctypes.c_uint64(0)
+        rc = lib().lc_hip_stream_capture_end(self.stream.device.handle, self.stream.handle, ctypes.byref(h))
+        if exc_type is not None:  # the block failed: drop what was recorded, report the block's error
+            if rc == 0:
+                lib().lc_hip_graph_destroy(self.stream.device.handle, h.value)
+            return False
+        _check(rc, "capture end")
+        self.graph = Graph(self.stream.device, h.value)
+        return False
 
 
 class Event:
@@ -1086,6 +1135,24 @@ class Accel:
         n = ray_buffer.count if count is None else count
         _check(lib().lc_hip_accel_trace_any(self.device.handle, stream.handle, self.handle, ray_buffer.handle,
                                             ray_offset, hit_buffer.handle, hit_offset, n, vis_mask), "trace_any")
+
+    def trace_closest_indirect(self, stream, ray_buffer, hit_buffer, count_buffer, count_index=0, capacity=None,
+                               vis_mask=0xFF, ray_offset=0, hit_offset=0):
+        """Closest hits of the first min(count, capacity) rays, count = count_buffer[count_index]
+        (uint32) read on the device when the launch runs (a queue a kernel appended to)."""
+        n = ray_buffer.count if capacity is None else capacity
+        _check(lib().lc_hip_accel_trace_closest_indirect(self.device.handle, stream.handle, self.handle,
+                                                         ray_buffer.handle, ray_offset, hit_buffer.handle, hit_offset,
+                                                         n, count_buffer.handle, 4 * count_index, vis_mask),
+               "trace_closest_indirect")
+
+    def trace_any_indirect(self, stream, ray_buffer, hit_buffer, count_buffer, count_index=0, capacity=None,
+                           vis_mask=0xFF, ray_offset=0, hit_offset=0):
+        n = ray_buffer.count if capacity is None else capacity
+        _check(lib().lc_hip_accel_trace_any_indirect(self.device.handle, stream.handle, self.handle,
+                                                     ray_buffer.handle, ray_offset, hit_buffer.handle, hit_offset,
+                                                     n, count_buffer.handle, 4 * count_index, vis_mask),
+               "trace_any_indirect")
 
     # RayQueryAll / RayQueryAny (accel.traverse / traverse_any) over a batch with built-in callbacks;
     # hits: COMMITTED_HIT_DTYPE per ray

@@ -258,6 +258,11 @@ struct StreamScratch {
     size_t spill_bytes = 0;
     void *sort = nullptr;
     size_t sort_bytes = 0;
+    // graphs captured on the stream replay launches that name these buffers: while any such
+    // graph (or a capture in progress) holds a pin, a buffer that has to grow is retired instead
+    // of freed, and the scratch outlives its stream; retired buffers go when the last pin does
+    uint32_t pins = 0;
+    std::vector<void *> retired;
 };
 constexpr size_t kStreamWordsBytes = 1024;
 
@@ -292,6 +297,25 @@ struct Device {
     void *pt = nullptr; // path-tracer workspace (pathtracer.hip), released with the device
     // this GPU's copy of the no-allocation trace's static scratch array (null if the lookup failed)
     void *noalloc_scratch = nullptr;
+    // stream captures in progress (guarded by mu): the stream's scratch, pinned for the capture
+    std::map<hipStream_t, StreamScratch *> capturing;
+    // scratch of destroyed streams still pinned by graphs (guarded by mu)
+    std::vector<StreamScratch *> orphans;
+    // destroys requested while a capture was open (their frees and device-wide waits would
+    // invalidate it); run when the last capture on the device ends (guarded by mu)
+    std::vector<std::function<void()>> deferred;
+};
+
+// Defers a destroy while a capture is open on the device: true when `fn` was queued (run when the
+// last capture ends), false when the caller should destroy now.
+bool defer_while_capturing(Device *dev, std::function<void()> fn);
+
+// A captured sequence of launches (lc_hip_stream_capture_end), replayed by lc_hip_graph_launch.
+struct Graph {
+    Device *dev = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    StreamScratch *pinned = nullptr; // the capturing stream's scratch, which the launches name
 };
 
 // The stream's working memory, created on first use (caller holds dev->mu); null when the

@@ -40,6 +40,16 @@ int hip_fail(hipError_t e, const char *what) {
         if (_e != hipSuccess) return hip_fail(_e, #expr);     \
     } while (0)
 
+// Grows a stream's working buffer (grow(), below); a stream under capture cannot grow.
+#define LC_HIP_GROW(args)                                                                                     \
+    do {                                                                                                      \
+        hipError_t _e = grow args;                                                                            \
+        if (_e == hipErrorStreamCaptureUnsupported)                                                           \
+            return fail(LC_HIP_ERROR_UNSUPPORTED, "a captured launch needs more working memory than the "    \
+                                                  "stream holds: run the work once outside the capture first"); \
+        if (_e != hipSuccess) return hip_fail(_e, "stream working memory");                                   \
+    } while (0)
+
 template<typename T>
 T *obj(uint64_t h) { return (h == 0 || h == LC_HIP_INVALID_HANDLE) ? nullptr : reinterpret_cast<T *>(h); }
 
@@ -79,14 +89,35 @@ hipError_t ensure(void *&ptr, size_t &cap, size_t bytes) {
     return e;
 }
 
-// Timed region on a stream: records HIP events around the launches when timing is on.
+bool capturing(hipStream_t s) {
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    return hipStreamIsCapturing(s, &st) == hipSuccess && st != hipStreamCaptureStatusNone;
+}
+
+// Grows one of a stream's working buffers. While graphs pin the scratch, the old buffer is
+// retired (their launches may name it) rather than freed. A stream being captured cannot grow:
+// the launches it records must name memory that exists when they replay, so the work is run
+// once outside the capture first.
+hipError_t grow(StreamScratch *S, hipStream_t s, void *&ptr, size_t &cap, size_t bytes) {
+    if (cap >= bytes && ptr) return hipSuccess;
+    if (capturing(s)) return hipErrorStreamCaptureUnsupported;
+    if (ptr && S->pins) {
+        S->retired.push_back(ptr);
+        ptr = nullptr;
+        cap = 0;
+    }
+    return ensure(ptr, cap, bytes);
+}
+
+// Timed region on a stream: records HIP events around the launches when timing is on (not
+// while the stream is being captured: a captured region has no time of its own).
 struct TimedRegion {
     Device *dev;
     hipStream_t s;
     const char *name;
     hipEvent_t a = nullptr, b = nullptr;
     TimedRegion(Device *d, hipStream_t st, const char *n) : dev(d), s(st), name(n) {
-        if (dev->timing) {
+        if (dev->timing && !capturing(s)) {
             hipEventCreate(&a);
             hipEventCreate(&b);
             hipEventRecord(a, s);
@@ -247,7 +278,22 @@ void free_stream_scratch(StreamScratch *S) {
     if (S->words) hipFree(S->words);
     if (S->spill) hipFree(S->spill);
     if (S->sort) hipFree(S->sort);
+    for (void *p : S->retired) hipFree(p);
     delete S;
+}
+
+// Drops one graph's (or capture's) pin on a stream's scratch (caller holds dev->mu, and the
+// launches that named retired buffers have completed).
+void unpin(Device *dev, StreamScratch *S) {
+    if (!S || S->pins == 0) return;
+    if (--S->pins > 0) return;
+    for (void *p : S->retired) hipFree(p);
+    S->retired.clear();
+    auto it = std::find(dev->orphans.begin(), dev->orphans.end(), S);
+    if (it != dev->orphans.end()) {
+        dev->orphans.erase(it);
+        free_stream_scratch(S);
+    }
 }
 
 } // namespace
@@ -279,9 +325,33 @@ void lchip::stream_scratch_release(Device *dev, hipStream_t s) {
     std::lock_guard<std::mutex> lk(dev->mu);
     auto it = dev->scratch.find(s);
     if (it == dev->scratch.end()) return;
-    free_stream_scratch(it->second);
+    if (it->second->pins) {
+        dev->orphans.push_back(it->second); // graphs still name it: freed with their last pin
+    } else {
+        free_stream_scratch(it->second);
+    }
     dev->scratch.erase(it);
 }
+
+bool lchip::defer_while_capturing(Device *dev, std::function<void()> fn) {
+    std::lock_guard<std::mutex> lk(dev->mu);
+    if (dev->capturing.empty()) return false;
+    dev->deferred.push_back(std::move(fn));
+    return true;
+}
+
+namespace {
+// Runs the destroys deferred during captures once none is open (caller does not hold dev->mu).
+void run_deferred(Device *dev) {
+    std::vector<std::function<void()>> fns;
+    {
+        std::lock_guard<std::mutex> lk(dev->mu);
+        if (!dev->capturing.empty()) return;
+        fns.swap(dev->deferred);
+    }
+    for (auto &f : fns) f();
+}
+} // namespace
 
 extern "C" {
 
@@ -337,12 +407,31 @@ uint64_t lc_hip_device_create(int32_t index) {
 void lc_hip_device_destroy(uint64_t device) {
     Device *dev = obj<Device>(device);
     if (!dev) return;
+    DeviceGuard g0(dev);
+    {
+        // destroys still waiting for a capture that was never ended
+        std::vector<std::function<void()>> fns;
+        {
+            std::lock_guard<std::mutex> lk(dev->mu);
+            for (auto &kv : dev->capturing) { // captures left open: discard what they recorded
+                hipGraph_t gr = nullptr;
+                if (hipStreamEndCapture(kv.first, &gr) == hipSuccess && gr) hipGraphDestroy(gr);
+                (void)hipGetLastError();
+                unpin(dev, kv.second);
+            }
+            dev->capturing.clear();
+            fns.swap(dev->deferred);
+        }
+        for (auto &f : fns) f();
+    }
     DeviceGuard g(dev);
     hipDeviceSynchronize();
     collect_timings(dev);
     pt_release(dev);
     for (auto &kv : dev->scratch) free_stream_scratch(kv.second);
     dev->scratch.clear();
+    for (StreamScratch *S : dev->orphans) free_stream_scratch(S);
+    dev->orphans.clear();
     if (dev->error) hipFree(dev->error);
     if (dev->arena_nodes) hipFree(dev->arena_nodes);
     if (dev->arena_scratch) hipFree(dev->arena_scratch);
@@ -421,6 +510,7 @@ void lc_hip_buffer_destroy(uint64_t device, uint64_t buffer) {
     Device *dev = obj<Device>(device);
     Buffer *b = obj<Buffer>(buffer);
     if (!dev || !b) return;
+    if (defer_while_capturing(dev, [device, buffer] { lc_hip_buffer_destroy(device, buffer); })) return;
     DeviceGuard g(dev);
     if (b->owned) hipFree(b->ptr);
     delete b;
@@ -486,7 +576,29 @@ void lc_hip_stream_destroy(uint64_t device, uint64_t stream) {
     Device *dev = obj<Device>(device);
     Stream *st = obj<Stream>(stream);
     if (!dev || !st) return;
+    {
+        // another stream's capture is open: destroy this one when it ends
+        std::lock_guard<std::mutex> dl(dev->mu);
+        bool other = false;
+        for (auto &kv : dev->capturing) other = other || kv.first != st->s;
+        if (other) {
+            dev->deferred.push_back([device, stream] { lc_hip_stream_destroy(device, stream); });
+            return;
+        }
+    }
     DeviceGuard g(dev);
+    {
+        // a capture left open: discard what it recorded
+        std::lock_guard<std::mutex> dl(dev->mu);
+        auto it = dev->capturing.find(st->s);
+        if (it != dev->capturing.end()) {
+            hipGraph_t gr = nullptr;
+            if (hipStreamEndCapture(st->s, &gr) == hipSuccess && gr) hipGraphDestroy(gr);
+            (void)hipGetLastError();
+            unpin(dev, it->second);
+            dev->capturing.erase(it);
+        }
+    }
     hipStreamSynchronize(st->s);
     {
         std::lock_guard<std::mutex> lk(st->qmu);
@@ -498,6 +610,7 @@ void lc_hip_stream_destroy(uint64_t device, uint64_t stream) {
     stream_scratch_release(dev, st->s);
     hipStreamDestroy(st->s);
     delete st;
+    run_deferred(dev);
 }
 
 int lc_hip_stream_synchronize(uint64_t device, uint64_t stream) {
@@ -525,6 +638,98 @@ int lc_hip_stream_add_callback(uint64_t device, uint64_t stream, void (*fn)(void
 }
 
 void *lc_hip_stream_native(uint64_t, uint64_t stream) { return native_stream(stream); }
+
+// ---- stream capture into HIP graphs. The reference's CUDA backend encodes every CommandList it
+// is given again (CUDAStream::dispatch, src/backends/cuda/cuda_stream.cpp:194-203); a caller
+// that submits the same launches many times (a wavefront path tracer's per-sample sequence)
+// records them once here and replays the graph. Between begin and end the stream's launches are
+// recorded, not run: shader dispatches, traces (unsorted, and only within the working memory the
+// stream already holds: run the sequence once outside the capture first) and buffer copies. Host
+// synchronisation, callbacks, events and builds are not capturable. A graph replays on any
+// stream; its launches use the capturing stream's working memory, so replays must not overlap
+// that stream's own traces. ----
+int lc_hip_stream_capture_begin(uint64_t device, uint64_t stream) {
+    Device *dev = obj<Device>(device);
+    Stream *st = obj<Stream>(stream);
+    if (!dev || !st) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad capture handles");
+    DeviceGuard g(dev);
+    std::lock_guard<std::mutex> dl(dev->mu);
+    if (dev->capturing.count(st->s)) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "the stream is already being captured");
+    StreamScratch *S = stream_scratch(dev, st->s); // created (and cleared) before the capture
+    if (!S) return fail(LC_HIP_ERROR_OUT_OF_MEMORY, "stream scratch");
+    LC_HIP_TRY(hipStreamBeginCapture(st->s, hipStreamCaptureModeRelaxed));
+    S->pins++;
+    dev->capturing.emplace(st->s, S);
+    return LC_HIP_OK;
+}
+
+static int capture_end(uint64_t device, uint64_t stream, uint64_t *graph);
+
+int lc_hip_stream_capture_end(uint64_t device, uint64_t stream, uint64_t *graph) {
+    const int rc = capture_end(device, stream, graph);
+    if (Device *dev = obj<Device>(device)) run_deferred(dev);
+    return rc;
+}
+
+static int capture_end(uint64_t device, uint64_t stream, uint64_t *graph) {
+    Device *dev = obj<Device>(device);
+    Stream *st = obj<Stream>(stream);
+    if (!dev || !st || !graph) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad capture handles");
+    *graph = LC_HIP_INVALID_HANDLE;
+    DeviceGuard g(dev);
+    std::lock_guard<std::mutex> dl(dev->mu);
+    auto it = dev->capturing.find(st->s);
+    if (it == dev->capturing.end()) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "the stream is not being captured");
+    StreamScratch *S = it->second;
+    dev->capturing.erase(it);
+    hipGraph_t gr = nullptr;
+    hipError_t e = hipStreamEndCapture(st->s, &gr);
+    if (e != hipSuccess || !gr) {
+        if (gr) hipGraphDestroy(gr);
+        unpin(dev, S);
+        (void)hipGetLastError();
+        return hip_fail(e != hipSuccess ? e : hipErrorStreamCaptureInvalidated, "stream capture");
+    }
+    hipGraphExec_t ex = nullptr;
+    e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        hipGraphDestroy(gr);
+        unpin(dev, S);
+        return hip_fail(e, "graph instantiate");
+    }
+    auto *G = new Graph;
+    G->dev = dev;
+    G->graph = gr;
+    G->exec = ex;
+    G->pinned = S; // the capture's pin passes to the graph
+    *graph = handle(G);
+    return LC_HIP_OK;
+}
+
+int lc_hip_graph_launch(uint64_t device, uint64_t stream, uint64_t graph) {
+    Device *dev = obj<Device>(device);
+    Graph *G = obj<Graph>(graph);
+    if (!dev || !G || G->dev != dev) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad graph handles");
+    DeviceGuard g(dev);
+    LC_HIP_TRY(hipGraphLaunch(G->exec, native_stream(stream)));
+    return LC_HIP_OK;
+}
+
+void lc_hip_graph_destroy(uint64_t device, uint64_t graph) {
+    Device *dev = obj<Device>(device);
+    Graph *G = obj<Graph>(graph);
+    if (!dev || !G || G->dev != dev) return;
+    if (defer_while_capturing(dev, [device, graph] { lc_hip_graph_destroy(device, graph); })) return;
+    DeviceGuard g(dev);
+    hipDeviceSynchronize(); // replays in flight may still name retired buffers
+    hipGraphExecDestroy(G->exec);
+    hipGraphDestroy(G->graph);
+    {
+        std::lock_guard<std::mutex> dl(dev->mu);
+        unpin(dev, G->pinned);
+    }
+    delete G;
+}
 
 // ---- events: timeline fences. The reference's event is a timeline semaphore
 // (src/backends/cuda/cuda_event.cpp:21-63, src/runtime/event.cpp:49-57): signal(stream, v) sets
@@ -585,6 +790,7 @@ uint64_t lc_hip_event_create(uint64_t device) {
 void lc_hip_event_destroy(uint64_t device, uint64_t event) {
     Event *e = obj<Event>(event);
     if (!e || !obj<Device>(device)) return;
+    if (defer_while_capturing(e->dev, [device, event] { lc_hip_event_destroy(device, event); })) return;
     DeviceGuard g(e->dev);
     {
         std::lock_guard<std::mutex> lk(e->mu);
@@ -686,6 +892,7 @@ void lc_hip_mesh_destroy(uint64_t device, uint64_t mesh) {
     Device *dev = obj<Device>(device);
     Mesh *m = obj<Mesh>(mesh);
     if (!dev || !m) return;
+    if (defer_while_capturing(dev, [device, mesh] { lc_hip_mesh_destroy(device, mesh); })) return;
     DeviceGuard g(dev);
     hipDeviceSynchronize();
     if (m->nodes) hipFree(m->nodes);
@@ -973,6 +1180,7 @@ void lc_hip_accel_destroy(uint64_t device, uint64_t accel) {
     Device *dev = obj<Device>(device);
     Accel *a = obj<Accel>(accel);
     if (!dev || !a) return;
+    if (defer_while_capturing(dev, [device, accel] { lc_hip_accel_destroy(device, accel); })) return;
     DeviceGuard g(dev);
     hipDeviceSynchronize();
     if (a->records) hipFree(a->records);
@@ -1372,9 +1580,12 @@ int lc_hip_accel_instance_write(uint64_t device, uint64_t stream, uint64_t accel
     return instance_batch(device, stream, accel, field, true, index_buffer, index_offset, in_buffer, in_offset, count);
 }
 
-// One launch over up to kMaxBatches ray batches (queue = the batches back to back).
+// One launch over up to kMaxBatches ray batches (queue = the batches back to back). With
+// `count_ptr` (one batch) the batch's ray count is its capacity and the launch traces the first
+// min(*count_ptr, capacity) rays, the count being read on the device when the launch runs.
 static int trace_launch_batches(Device *dev, Accel *a, hipStream_t s, const lc_hip_ray_batch *batches, size_t nb,
-                                bool any, bool counting, uint64_t *n_nodes, uint64_t *n_tris) {
+                                bool any, bool counting, uint64_t *n_nodes, uint64_t *n_tris,
+                                const uint32_t *count_ptr = nullptr) {
     TraceParams P;
     std::memset(&P, 0, sizeof(P));
     uint64_t total = 0;
@@ -1403,6 +1614,7 @@ static int trace_launch_batches(Device *dev, Accel *a, hipStream_t s, const lc_h
     if (total > 0xF0000000ull) return fail(LC_HIP_ERROR_UNSUPPORTED, "more than 0xF0000000 rays in one launch");
     P.batches.n = (uint32_t)nb;
     P.count = (uint32_t)total;
+    P.count_ptr = count_ptr;
     if (counting && n_nodes) *n_nodes = 0;
     if (counting && n_tris) *n_tris = 0;
     if (total == 0) return LC_HIP_OK;
@@ -1420,7 +1632,7 @@ static int trace_launch_batches(Device *dev, Accel *a, hipStream_t s, const lc_h
     size_t spill_bytes = (size_t)grid * cfg.block * P.spill_depth * sizeof(uint2);
     StreamScratch *S = stream_scratch(dev, s);
     if (!S) return fail(LC_HIP_ERROR_OUT_OF_MEMORY, "stream scratch");
-    LC_HIP_TRY(ensure(S->spill, S->spill_bytes, spill_bytes));
+    LC_HIP_GROW((S, s, S->spill, S->spill_bytes, spill_bytes));
     P.spill = static_cast<uint2 *>(S->spill);
     P.ray_counter = S->ray_counter;
     P.error = S->error;
@@ -1430,9 +1642,12 @@ static int trace_launch_batches(Device *dev, Accel *a, hipStream_t s, const lc_h
     P.fetch_segments = cfg.fetch_segments;
     if (counting) LC_HIP_TRY(hipMemsetAsync(S->stats, 0, 2 * sizeof(unsigned long long), s));
     // per-ray results do not depend on the order; the counting form traverses the same sorted
-    // queue as the timed form, so its node / triangle totals describe that launch
-    const bool sort = cfg.sort_rays && total > 1 && !P.geo.empty;
-    if (sort) LC_HIP_TRY(ensure(S->sort, S->sort_bytes, ray_sort_scratch_bytes(P.count)));
+    // queue as the timed form, so its node / triangle totals describe that launch. Queues counted
+    // on the device are traced in the order their producer wrote them, and so are captured
+    // launches (the radix sort clears its digit counts with memset nodes, which graphs here do
+    // not replay reliably)
+    const bool sort = cfg.sort_rays && total > 1 && !P.geo.empty && !count_ptr && !capturing(s);
+    if (sort) LC_HIP_GROW((S, s, S->sort, S->sort_bytes, ray_sort_scratch_bytes(P.count)));
     {
         // "trace_*": the whole entry point (sort + traversal); "trace_*_kernel": traversal only
         TimedRegion tr(dev, s, counting ? "trace_count" : (any ? "trace_any" : "trace_closest"));
@@ -1504,6 +1719,45 @@ int lc_hip_accel_trace_count(uint64_t device, uint64_t stream, uint64_t accel, u
                              uint64_t *triangles_tested) {
     const lc_hip_ray_batch b{ray_buffer, ray_offset, 0, 0, ray_count, vis_mask};
     return trace_batches(device, stream, accel, &b, 1, any != 0, true, nodes_visited, triangles_tested);
+}
+
+// Traces over a queue whose length is produced on the GPU (a shader's appends): `capacity` rays
+// lie in the ray buffer and hit slots in the hit buffer; the launch traces the first
+// min(count, capacity) of them, count being the uint32 at count_offset of count_buffer when the
+// launch runs. Stands in for a kernel dispatched over an IndirectDispatchBuffer whose body calls
+// accel.trace_closest / trace_any (IndirectDispatchBuffer, include/luisa/runtime/dispatch_buffer.h;
+// lc_accel_trace_closest, src/backends/cuda/cuda_builtin/cuda_device_resource.h:1783-1829). The
+// queue is traced in its own order (no ray sort).
+static int trace_indirect(uint64_t device, uint64_t stream, uint64_t accel, uint64_t ray_buffer, size_t ray_offset,
+                          uint64_t hit_buffer, size_t hit_offset, uint32_t capacity, uint64_t count_buffer,
+                          size_t count_offset, uint32_t vis_mask, bool any) {
+    Device *dev = obj<Device>(device);
+    Accel *a = obj<Accel>(accel);
+    Buffer *cb = obj<Buffer>(count_buffer);
+    if (!dev || !a || !cb) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad indirect trace handles");
+    if (count_offset % 4 != 0 || count_offset + 4 > cb->bytes)
+        return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "count outside the count buffer (uint32, 4-byte aligned)");
+    std::lock_guard<std::mutex> lk(a->mu);
+    if (!a->built) return fail(LC_HIP_ERROR_NOT_BUILT, "accel is not built");
+    DeviceGuard g(dev);
+    hipStream_t s = native_stream(stream);
+    const lc_hip_ray_batch b{ray_buffer, ray_offset, hit_buffer, hit_offset, capacity, vis_mask};
+    const uint32_t *count_ptr = reinterpret_cast<const uint32_t *>(static_cast<const char *>(cb->ptr) + count_offset);
+    return trace_launch_batches(dev, a, s, &b, 1, any, false, nullptr, nullptr, count_ptr);
+}
+
+int lc_hip_accel_trace_closest_indirect(uint64_t device, uint64_t stream, uint64_t accel, uint64_t ray_buffer,
+                                        size_t ray_offset, uint64_t hit_buffer, size_t hit_offset, uint32_t capacity,
+                                        uint64_t count_buffer, size_t count_offset, uint32_t vis_mask) {
+    return trace_indirect(device, stream, accel, ray_buffer, ray_offset, hit_buffer, hit_offset, capacity, count_buffer,
+                          count_offset, vis_mask, false);
+}
+
+int lc_hip_accel_trace_any_indirect(uint64_t device, uint64_t stream, uint64_t accel, uint64_t ray_buffer,
+                                    size_t ray_offset, uint64_t hit_buffer, size_t hit_offset, uint32_t capacity,
+                                    uint64_t count_buffer, size_t count_offset, uint32_t vis_mask) {
+    return trace_indirect(device, stream, accel, ray_buffer, ray_offset, hit_buffer, hit_offset, capacity, count_buffer,
+                          count_offset, vis_mask, true);
 }
 
 // Closest hits without allocating: the working memory that lc_hip_accel_trace_closest takes from
@@ -1630,14 +1884,14 @@ int lc_hip_accel_query(uint64_t device, uint64_t stream, uint64_t accel, uint64_
     P.spill_depth = stack_need > slds ? stack_need - slds : 0u;
     StreamScratch *S = stream_scratch(dev, s);
     if (!S) return fail(LC_HIP_ERROR_OUT_OF_MEMORY, "stream scratch");
-    LC_HIP_TRY(ensure(S->spill, S->spill_bytes, (size_t)grid * block * P.spill_depth * sizeof(uint2)));
+    LC_HIP_GROW((S, s, S->spill, S->spill_bytes, (size_t)grid * block * P.spill_depth * sizeof(uint2)));
     P.spill = static_cast<uint2 *>(S->spill);
     P.error = S->error;
     P.counter = S->ray_counter;
     // the query queue is sorted like trace queues when ray sorting is on (per-ray answers of the
-    // nearest-commit query do not depend on the order)
-    const bool sort = dev->cfg[P.first ? 1 : 0].sort_rays && ray_count > 1 && !P.geo.empty;
-    if (sort) LC_HIP_TRY(ensure(S->sort, S->sort_bytes, ray_sort_scratch_bytes(ray_count)));
+    // nearest-commit query do not depend on the order); not under capture, as for traces
+    const bool sort = dev->cfg[P.first ? 1 : 0].sort_rays && ray_count > 1 && !P.geo.empty && !capturing(s);
+    if (sort) LC_HIP_GROW((S, s, S->sort, S->sort_bytes, ray_sort_scratch_bytes(ray_count)));
     TimedRegion tr(dev, s, P.first ? "query_any" : "query");
     if (sort) {
         BatchTable bt;
@@ -1795,6 +2049,7 @@ void lc_hip_server_destroy(uint64_t device, uint64_t server) {
     Device *dev = obj<Device>(device);
     Server *s = obj<Server>(server);
     if (!dev || !s) return;
+    if (defer_while_capturing(dev, [device, server] { lc_hip_server_destroy(device, server); })) return;
     DeviceGuard g(dev);
     serve_destroy(s);
 }
@@ -2061,6 +2316,7 @@ void lc_hip_texture_destroy(uint64_t device, uint64_t texture) {
     Device *dev = obj<Device>(device);
     Texture *t = obj<Texture>(texture);
     if (!dev || !t) return;
+    if (defer_while_capturing(dev, [device, texture] { lc_hip_texture_destroy(device, texture); })) return;
     DeviceGuard g(dev);
     (void)hipDeviceSynchronize();
     if (t->ptr) (void)hipFree(t->ptr);
@@ -2158,6 +2414,7 @@ void lc_hip_bindless_destroy(uint64_t device, uint64_t array) {
     Device *dev = obj<Device>(device);
     Bindless *b = obj<Bindless>(array);
     if (!dev || !b) return;
+    if (defer_while_capturing(dev, [device, array] { lc_hip_bindless_destroy(device, array); })) return;
     DeviceGuard g(dev);
     (void)hipDeviceSynchronize();
     if (b->slots) (void)hipFree(b->slots);
@@ -2296,6 +2553,7 @@ void lc_hip_shader_destroy(uint64_t device, uint64_t shader) {
     Device *dev = obj<Device>(device);
     Shader *sh = obj<Shader>(shader);
     if (!dev || !sh) return;
+    if (defer_while_capturing(dev, [device, shader] { lc_hip_shader_destroy(device, shader); })) return;
     DeviceGuard g(dev);
     hipDeviceSynchronize();
     if (sh->module) hipModuleUnload(sh->module);
@@ -2382,10 +2640,11 @@ static int shader_launch(uint64_t device, uint64_t stream, uint64_t shader, cons
         const uint32_t gx = (dx + sh->block[0] - 1) / sh->block[0];
         const uint32_t gy = (dy + sh->block[1] - 1) / sh->block[1];
         const uint32_t gz = (dz + sh->block[2] - 1) / sh->block[2];
-        size_t size = off;
-        void *extra[] = {HIP_LAUNCH_PARAM_BUFFER_POINTER, buf, HIP_LAUNCH_PARAM_BUFFER_SIZE, &size, HIP_LAUNCH_PARAM_END};
+        // the kernel's one argument is the packed struct (its size, `off`, comes from the code
+        // object's argument metadata); passed as a kernel parameter, which stream capture records
+        void *params[] = {buf};
         hipError_t e = hipModuleLaunchKernel(sh->fn, gx, gy, gz, sh->block[0], sh->block[1], sh->block[2], 0, s,
-                                             nullptr, extra);
+                                             params, nullptr);
         if (e != hipSuccess) {
             (void)hipGetLastError();
             return hip_fail(e, "shader launch");

@@ -37,3 +37,23 @@ def test_dsl_true_division():
     assert all(isinstance(x, A.CastExpr) and x.type is A.FLOAT for x in (true_div.lhs, true_div.rhs))
     assert isinstance(floor_div, A.BinaryExpr) and floor_div.op == A.BinaryOp.DIV and floor_div.type is A.UINT
     assert lc.compile_shader_source(k.source) > 1000
+
+
+@pytest.mark.parametrize("name", __import__("luisacompute_amd.programs", fromlist=["x"]).WavefrontPathTracer.KERNELS)
+def test_wavefront_kernels_compile(name):
+    from luisacompute_amd import programs
+    k = getattr(programs, name)
+    assert "extern \"C\" __global__" in k.source
+    assert lc.compile_shader_source(k.source) > 1000
+
+
+def test_wavefront_shade_argument_usage():
+    """The shade kernel's queues and path state carry the usage its AST gives them."""
+    from luisacompute_amd import lcast as A, programs
+    k = programs.wf_shade
+    names = list(k.params)
+    use = lambda n: k.argument_usage(names.index(n))
+    assert use("rays_in") == A.Usage.READ and use("hits") == A.Usage.READ
+    assert use("rays_out") == A.Usage.WRITE and use("shadow_rays") == A.Usage.WRITE
+    assert use("counters") == A.Usage.READ_WRITE and use("throughput") == A.Usage.READ_WRITE
+    assert use("rng") == A.Usage.READ_WRITE and use("radiance") == A.Usage.READ_WRITE
