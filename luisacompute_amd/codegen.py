@@ -202,7 +202,7 @@ class _Printer:
         self.names = {}
         self.query_depth = 0
         used = set()
-        for v in fn.arguments + fn.local_variables:
+        for v in fn.arguments + fn.local_variables + fn.shared_variables:
             base = (v.name or f"t{v.uid}").replace(".", "_")
             prefix = "a_" if v in fn.arguments and not kernel else "v_"
             n = prefix + base
@@ -427,6 +427,8 @@ class _Printer:
         self.query_depth -= 1
 
     def declarations(self):
+        for v in self.fn.shared_variables:  # not initialised (a shared variable has no initialiser)
+            self.line(f"__shared__ {type_name(v.type)} {self.var(v)};")
         for v in self.fn.local_variables:
             self.line(f"{type_name(v.type)} {self.var(v)}{{}};")
 

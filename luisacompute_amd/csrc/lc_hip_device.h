@@ -28,6 +28,14 @@ namespace dev {
 
 constexpr int kStack = 64;
 
+// array<T, N> of the AST (local and shared arrays; type.h's ARRAY)
+template<class T, uint32_t N>
+struct Array {
+    T v[N];
+    __device__ __forceinline__ T &operator[](uint32_t i) { return v[i]; }
+    __device__ __forceinline__ const T &operator[](uint32_t i) const { return v[i]; }
+};
+
 struct Ray { // lc_hip_ray / Ray (include/luisa/runtime/rtx/ray.h:10-16)
     float origin[3];
     float t_min;

@@ -34,7 +34,7 @@ namespace {
 
 const char *const kRtcOptions[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off",
                                    "-fgpu-flush-denormals-to-zero", "-Xclang", "-target-feature", "-Xclang",
-                                   "-packed-fp32-ops", "-Wno-unused-function"};
+                                   "-packed-fp32-ops", "-Wno-unused-function", "-Wno-parentheses-equality"};
 constexpr int kRtcOptionCount = (int)(sizeof(kRtcOptions) / sizeof(kRtcOptions[0]));
 constexpr uint64_t kCacheMagic = 0x31434F5350494843ull; // "CHIPSOC1": cache file format 1
 

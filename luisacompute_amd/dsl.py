@@ -34,7 +34,8 @@ Builtins: dispatch_id(), dispatch_size(), thread_id(), block_id(), warp_lane_id(
 int2/3/4, uint2/3/4, float(), int(), uint(), ulong(), bool(); sqrt, rsqrt, abs, min, max, clamp,
 lerp, saturate, step, fma, floor, ceil, fract, trunc, round, sin, cos, tan, asin, acos, atan,
 atan2, exp, exp2, log, log2, log10, pow (and **), copysign, isnan, isinf, dot, cross, normalize,
-length, length_squared, reduce_sum/min/max, select(c, a, b) (= c ? a : b), synchronize_block();
+length, length_squared, reduce_sum/min/max, select(c, a, b) (= c ? a : b), synchronize_block() (or
+sync_block()); name = shared_array("type", n) declares a workgroup-shared array (kernels only);
 warp_active_count_bits(b), warp_prefix_count_bits(b), warp_is_first_active_lane(),
 warp_read_first_active_lane(x), warp_read_lane(x, i); make_ray(o, d[, t_min, t_max]),
 offset_ray_origin(p, n), triangle_interpolate(bary, a, b, c), ray_origin(r), ray_direction(r),
@@ -332,7 +333,29 @@ class _Builder:
     def s_Assign(self, node):
         if len(node.targets) != 1:
             self.error(node, "one assignment target at a time")
+        v = node.value
+        if isinstance(v, ast.Call) and isinstance(v.func, ast.Name) and v.func.id == "shared_array":
+            self.shared_array(node.targets[0], v, node)
+            return
         self.assign(node.targets[0], self.expr(node.value), node)
+
+    def shared_array(self, target, call, node):
+        """name = shared_array("elem", n): an array of n elements shared by the workgroup's threads
+        (SharedArrayType(n, dtype)() of the reference Python frontend, src/py/luisa/array.py:73-95);
+        not initialised; indexed like an array, with atomic_* methods for integer elements."""
+        if not self.kernel:
+            self.error(node, "shared arrays belong to kernels")
+        a = call.args
+        if not (isinstance(target, ast.Name) and len(a) == 2 and isinstance(a[0], ast.Constant) and
+                isinstance(a[0].value, str) and isinstance(a[1], ast.Constant) and isinstance(a[1].value, int)
+                and a[1].value > 0):
+            self.error(node, 'name = shared_array("element type", positive size)')
+        if self.lookup_soft(target.id) is not None:
+            self.error(node, f"{target.id} is already defined")
+        el = Type.of(a[0].value)
+        if not el.is_scalar and not el.is_vector:
+            self.error(node, "shared arrays hold scalars or vectors")
+        self.names[target.id] = self.fn.shared(Type.array(el, a[1].value), target.id)
 
     def assign(self, target, value, node):
         if value is None or value.type is None:
@@ -659,7 +682,7 @@ class _Builder:
         f = node.func
         if isinstance(f, ast.Attribute) and isinstance(f.value, ast.Name):
             v = self.lookup(f.value.id, node)
-            if isinstance(v, A.Variable) and v.type.is_resource:
+            if isinstance(v, A.Variable) and (v.type.is_resource or v.tag == A.Variable.Tag.SHARED):
                 return self.method(v, f.attr, node)
         if isinstance(f, ast.Name):
             v = self.lookup(f.id, node)
@@ -737,7 +760,7 @@ class _Builder:
             t = self.common(args[1], args[2], node)
             return A.CallExpr(t, CallOp.SELECT, [self.cast(args[2], t, node), self.cast(args[1], t, node),
                                                  self.to_bool(args[0], node)])
-        if name == "synchronize_block":
+        if name in ("synchronize_block", "sync_block"):  # sync_block: src/py/luisa/builtin.py:371
             return A.CallExpr(None, CallOp.SYNCHRONIZE_BLOCK, [])
         if name in ("warp_active_count_bits", "warp_prefix_count_bits"):
             op = CallOp.WARP_ACTIVE_COUNT_BITS if name == "warp_active_count_bits" else CallOp.WARP_PREFIX_COUNT_BITS
@@ -795,6 +818,13 @@ class _Builder:
         # string arguments name types (bindless buffer_read)
         args = [None if isinstance(a, ast.Constant) and isinstance(a.value, str) else self.expr(a) for a in node.args]
         res = A.RefExpr(v)
+        if v.tag == A.Variable.Tag.SHARED:
+            el = t.element
+            if m in _ATOMIC_METHODS and el.is_int and el.is_scalar:
+                if len(args) != 2:
+                    self.error(node, f"{m}(index, value)")
+                return A.CallExpr(el, _ATOMIC_METHODS[m], [res, self._index(args, node), self.cast(args[1], el, node)])
+            self.error(node, f"shared arrays of {el.description} have no method {m}")
         if t.tag == Type.Tag.BUFFER:
             el = t.element
             if m == "read":

@@ -629,6 +629,7 @@ class Function:
         self.arguments = []
         self.builtin_variables = []
         self.local_variables = []
+        self.shared_variables = []
         self.body = ScopeStmt()
         self.return_type = None
         self.custom_callables = []
@@ -659,6 +660,14 @@ class Function:
     def local(self, type_, name):
         v = self._new(type_, Variable.Tag.LOCAL, name)
         self.local_variables.append(v)
+        return v
+
+    def shared(self, type_, name):
+        """A workgroup-shared variable (FunctionBuilder::shared; kernels only)."""
+        if self.tag != Function.Tag.KERNEL:
+            raise ValueError("shared variables belong to kernels")
+        v = self._new(type_, Variable.Tag.SHARED, name)
+        self.shared_variables.append(v)
         return v
 
     def builtin(self, tag):

@@ -202,8 +202,8 @@ def path_tracer(image: Buffer("float4"), accel: Accel, heap: dsl.BindlessArray, 
 # pixel's sum in sample order. Path state (ray, throughput and pdf, radiance, PCG state) lives in
 # per-path buffers between the kernels; every value is computed by the megakernel's expressions in
 # the same order, so the image equals the megakernel's bit for bit.
-# counters: [0], [1] continuation queue lengths, [2] shadow queue length; frame[0]: the wave's
-# first sample.
+# counters (one 128-B line each, so that their atomics do not serialise on one line): [0] and [32]
+# the continuation queue lengths, [64] the shadow queue length; frame[0]: the wave's first sample.
 
 @dsl.kernel(block=(256, 1, 1))
 def wf_clear(total: Buffer("float4"), frame: Buffer("uint"), n: uint):
@@ -282,11 +282,11 @@ def wf_camera(rays: Buffer("Ray"), paths: Buffer("uint"), throughput: Buffer("fl
 def wf_reset(counters: Buffer("uint"), out_queue: uint):
     """Empties the queue the next wf_shade appends continuation rays to, and the shadow queue."""
     if dispatch_id().x == 0:
-        counters.write(out_queue, 0)
-        counters.write(2, 0)
+        counters.write(32 * out_queue, 0)
+        counters.write(64, 0)
 
 
-@dsl.kernel(block=(256, 1, 1))
+@dsl.kernel(block=(512, 1, 1))
 def wf_shade(rays_in: Buffer("Ray"), paths_in: Buffer("uint"), hits: Buffer("SurfaceHit"), rays_out: Buffer("Ray"),
              paths_out: Buffer("uint"), shadow_rays: Buffer("Ray"), shadow_paths: Buffer("uint"),
              shadow_contrib: Buffer("float4"), throughput: Buffer("float4"), radiance: Buffer("float4"),
@@ -294,20 +294,23 @@ def wf_shade(rays_in: Buffer("Ray"), paths_in: Buffer("uint"), hits: Buffer("Sur
              materials: Buffer("float4"), light_position: dsl.float3_, light_u: dsl.float3_, light_v: dsl.float3_,
              light_emission: dsl.float3_, depth: uint, max_depth: uint, rr_depth: uint, in_queue: uint):
     """One bounce for the entries of queue `in_queue`: the megakernel's loop body from the closest
-    hit to the shadow ray for each entry's path; continuation and shadow rays are appended to
-    their queues (one atomic per wave and queue, slots in lane order)."""
+    hit to the shadow ray for each entry's path (a one-pass loop whose `break`s are the
+    megakernel's early exits, so that every thread reaches the appends). Continuation and shadow
+    rays are appended with one global atomic per workgroup and queue: lanes count through a
+    shared array, one thread reserves the workgroup's slots (the dispatch covers whole
+    workgroups)."""
     j = dispatch_id().x
-    if j >= counters.read(in_queue):
-        return
-    p = paths_in.read(j)
-    h = hits.read(j)
-    if hit_miss(h):
-        return
-    r = rays_in.read(j)
-    o = ray_origin(r)
-    d = ray_direction(r)
-    state = rng.read(2 * p)
-    inc = rng.read(2 * p + 1)
+    shadow = False
+    cont = False
+    p = uint(0)
+    pp = float3(0.0, 0.0, 0.0)
+    n = float3(0.0, 0.0, 0.0)
+    wi = float3(0.0, 0.0, 0.0)
+    nd = float3(0.0, 0.0, 0.0)
+    d_light = 0.0
+    contrib = float3(0.0, 0.0, 0.0)
+    state = ulong(0)
+    inc = ulong(0)
 
     def pcg_next():
         old = state
@@ -325,124 +328,147 @@ def wf_shade(rays_in: Buffer("Ray"), paths_in: Buffer("uint"), hits: Buffer("Sur
                       xforms.read(m + 4) * v.x + xforms.read(m + 5) * v.y + xforms.read(m + 6) * v.z + xforms.read(m + 7),
                       xforms.read(m + 8) * v.x + xforms.read(m + 9) * v.y + xforms.read(m + 10) * v.z + xforms.read(m + 11))
 
-    tp = throughput.read(p)
-    beta = float3(tp.x, tp.y, tp.z)
-    pdf = tp.w
-    lr = radiance.read(p)
-    L = float3(lr.x, lr.y, lr.z)
-    lc_ = cross(light_u, light_v)
-    light_area = length(lc_)
-    light_n = normalize(lc_)
-    inst = h.inst
-    tri = heap.buffer_read("uint3", 2 * inst + 1, h.prim)
-    p0 = to_world(inst, heap.buffer_read("float4", 2 * inst, tri.x))
-    p1 = to_world(inst, heap.buffer_read("float4", 2 * inst, tri.y))
-    p2 = to_world(inst, heap.buffer_read("float4", 2 * inst, tri.z))
-    pt = triangle_interpolate(hit_bary(h), p0, p1, p2)
-    n = normalize(cross(p1 - p0, p2 - p0))
-    cos_wo = -d.x * n.x + -d.y * n.y + -d.z * n.z
-    if cos_wo < 1e-4:
-        return
-    mat = materials.read(inst)
-    if mat.w != 0.0:
-        if depth == 0:
-            L = float3(L.x + light_emission.x, L.y + light_emission.y, L.z + light_emission.z)
-        else:
-            dp = pt - o
-            pdf_light = dot(dp, dp) / (light_area * cos_wo)
-            mis = pdf / max(pdf + pdf_light, 1e-4)
-            L = float3(L.x + mis * beta.x * light_emission.x, L.y + mis * beta.y * light_emission.y,
-                       L.z + mis * beta.z * light_emission.z)
-        radiance.write(p, float4(L.x, L.y, L.z, 0.0))
-        return
-    # light sample
-    ulx = pcg_float()
-    uly = pcg_float()
-    pl = float3(light_position.x + ulx * light_u.x + uly * light_v.x,
-                light_position.y + ulx * light_u.y + uly * light_v.y,
-                light_position.z + ulx * light_u.z + uly * light_v.z)
-    pp = offset_ray_origin(pt, n)
-    ppl = offset_ray_origin(pl, light_n)
-    dl = ppl - pp
-    d_light = sqrt(dot(dl, dl))
-    wi = normalize(dl)
-    cos_wi_light = dot(wi, n)
-    cos_light = -dot(light_n, wi)
-    shadow = cos_wi_light > 1e-4 and cos_light > 1e-4
-    contrib = float3(0.0, 0.0, 0.0)
-    if shadow:
-        pdf_l = (d_light * d_light) / (light_area * cos_light)
-        pdf_b = cos_wi_light * INV_PI
-        mis_l = pdf_l / max(pdf_l + pdf_b, 1e-4)
-        den = max(pdf_l, 1e-4)
-        contrib = float3(beta.x * (mat.x * INV_PI * cos_wi_light) * mis_l * light_emission.x / den,
-                         beta.y * (mat.y * INV_PI * cos_wi_light) * mis_l * light_emission.y / den,
-                         beta.z * (mat.z * INV_PI * cos_wi_light) * mis_l * light_emission.z / den)
-    # cosine-weighted direction in the make_onb frame
-    ux = pcg_float()
-    uy = pcg_float()
-    rr_ = sqrt(ux)
-    x4 = uy * 4.0
-    k = int(x4)
-    f = x4 - float(k)
-    t = f * 1.57079632679489662
-    t2 = t * t
-    ps = t * (1.0 + t2 * (-1.0 / 6.0 + t2 * (1.0 / 120.0 + t2 * (-1.0 / 5040.0 + t2 * (1.0 / 362880.0 +
-                                                                                      t2 * (-1.0 / 39916800.0))))))
-    pc = 1.0 + t2 * (-1.0 / 2.0 + t2 * (1.0 / 24.0 + t2 * (-1.0 / 720.0 + t2 * (1.0 / 40320.0 + t2 * (
-        -1.0 / 3628800.0 + t2 * (1.0 / 479001600.0))))))
-    q4 = k & 3
-    sp = ps
-    cp = pc
-    if q4 == 1:
-        sp = pc
-        cp = -ps
-    elif q4 == 2:
-        sp = -ps
-        cp = -pc
-    elif q4 == 3:
-        sp = -pc
-        cp = ps
-    wl = float3(rr_ * cp, rr_ * sp, sqrt(1.0 - ux))
-    bn = normalize(float3(-n.y, n.x, 0.0) if abs(n.x) > abs(n.z) else float3(0.0, -n.z, n.y))
-    tn = normalize(cross(bn, n))
-    nd = float3(wl.x * tn.x + wl.y * bn.x + wl.z * n.x, wl.x * tn.y + wl.y * bn.y + wl.z * n.y,
-                wl.x * tn.z + wl.y * bn.z + wl.z * n.z)
-    pdf = abs(wl.z) * INV_PI
-    beta = float3(beta.x * mat.x, beta.y * mat.y, beta.z * mat.z)
-    cont = depth + 1 < max_depth
-    if depth >= rr_depth:
-        lum = 0.212671 * beta.x + 0.715160 * beta.y + 0.072169 * beta.z
-        if lum == 0.0:
-            cont = False
-        else:
-            qq = max(lum, 0.05)
-            rr = pcg_float()
-            if rr >= qq:
+    while True:
+        if j >= counters.read(32 * in_queue):
+            break
+        p = paths_in.read(j)
+        h = hits.read(j)
+        if hit_miss(h):
+            break
+        r = rays_in.read(j)
+        o = ray_origin(r)
+        d = ray_direction(r)
+        state = rng.read(2 * p)
+        inc = rng.read(2 * p + 1)
+        tp = throughput.read(p)
+        beta = float3(tp.x, tp.y, tp.z)
+        pdf = tp.w
+        lr = radiance.read(p)
+        L = float3(lr.x, lr.y, lr.z)
+        lc_ = cross(light_u, light_v)
+        light_area = length(lc_)
+        light_n = normalize(lc_)
+        inst = h.inst
+        tri = heap.buffer_read("uint3", 2 * inst + 1, h.prim)
+        p0 = to_world(inst, heap.buffer_read("float4", 2 * inst, tri.x))
+        p1 = to_world(inst, heap.buffer_read("float4", 2 * inst, tri.y))
+        p2 = to_world(inst, heap.buffer_read("float4", 2 * inst, tri.z))
+        pt = triangle_interpolate(hit_bary(h), p0, p1, p2)
+        n = normalize(cross(p1 - p0, p2 - p0))
+        cos_wo = -d.x * n.x + -d.y * n.y + -d.z * n.z
+        if cos_wo < 1e-4:
+            break
+        mat = materials.read(inst)
+        if mat.w != 0.0:
+            if depth == 0:
+                L = float3(L.x + light_emission.x, L.y + light_emission.y, L.z + light_emission.z)
+            else:
+                dp = pt - o
+                pdf_light = dot(dp, dp) / (light_area * cos_wo)
+                mis = pdf / max(pdf + pdf_light, 1e-4)
+                L = float3(L.x + mis * beta.x * light_emission.x, L.y + mis * beta.y * light_emission.y,
+                           L.z + mis * beta.z * light_emission.z)
+            radiance.write(p, float4(L.x, L.y, L.z, 0.0))
+            break
+        # light sample
+        ulx = pcg_float()
+        uly = pcg_float()
+        pl = float3(light_position.x + ulx * light_u.x + uly * light_v.x,
+                    light_position.y + ulx * light_u.y + uly * light_v.y,
+                    light_position.z + ulx * light_u.z + uly * light_v.z)
+        pp = offset_ray_origin(pt, n)
+        ppl = offset_ray_origin(pl, light_n)
+        dl = ppl - pp
+        d_light = sqrt(dot(dl, dl))
+        wi = normalize(dl)
+        cos_wi_light = dot(wi, n)
+        cos_light = -dot(light_n, wi)
+        shadow = cos_wi_light > 1e-4 and cos_light > 1e-4
+        if shadow:
+            pdf_l = (d_light * d_light) / (light_area * cos_light)
+            pdf_b = cos_wi_light * INV_PI
+            mis_l = pdf_l / max(pdf_l + pdf_b, 1e-4)
+            den = max(pdf_l, 1e-4)
+            contrib = float3(beta.x * (mat.x * INV_PI * cos_wi_light) * mis_l * light_emission.x / den,
+                             beta.y * (mat.y * INV_PI * cos_wi_light) * mis_l * light_emission.y / den,
+                             beta.z * (mat.z * INV_PI * cos_wi_light) * mis_l * light_emission.z / den)
+        # cosine-weighted direction in the make_onb frame
+        ux = pcg_float()
+        uy = pcg_float()
+        rr_ = sqrt(ux)
+        x4 = uy * 4.0
+        k = int(x4)
+        f = x4 - float(k)
+        t = f * 1.57079632679489662
+        t2 = t * t
+        ps = t * (1.0 + t2 * (-1.0 / 6.0 + t2 * (1.0 / 120.0 + t2 * (-1.0 / 5040.0 + t2 * (1.0 / 362880.0 +
+                                                                                          t2 * (-1.0 / 39916800.0))))))
+        pc = 1.0 + t2 * (-1.0 / 2.0 + t2 * (1.0 / 24.0 + t2 * (-1.0 / 720.0 + t2 * (1.0 / 40320.0 + t2 * (
+            -1.0 / 3628800.0 + t2 * (1.0 / 479001600.0))))))
+        q4 = k & 3
+        sp = ps
+        cp = pc
+        if q4 == 1:
+            sp = pc
+            cp = -ps
+        elif q4 == 2:
+            sp = -ps
+            cp = -pc
+        elif q4 == 3:
+            sp = -pc
+            cp = ps
+        wl = float3(rr_ * cp, rr_ * sp, sqrt(1.0 - ux))
+        bn = normalize(float3(-n.y, n.x, 0.0) if abs(n.x) > abs(n.z) else float3(0.0, -n.z, n.y))
+        tn = normalize(cross(bn, n))
+        nd = float3(wl.x * tn.x + wl.y * bn.x + wl.z * n.x, wl.x * tn.y + wl.y * bn.y + wl.z * n.y,
+                    wl.x * tn.z + wl.y * bn.z + wl.z * n.z)
+        pdf = abs(wl.z) * INV_PI
+        beta = float3(beta.x * mat.x, beta.y * mat.y, beta.z * mat.z)
+        cont = depth + 1 < max_depth
+        if depth >= rr_depth:
+            lum = 0.212671 * beta.x + 0.715160 * beta.y + 0.072169 * beta.z
+            if lum == 0.0:
                 cont = False
             else:
-                iq = 1.0 / qq
-                beta = float3(beta.x * iq, beta.y * iq, beta.z * iq)
-    throughput.write(p, float4(beta.x, beta.y, beta.z, pdf))
-    rng.write(2 * p, state)
-    # appends: one atomic per wave and queue, slots in lane order
+                qq = max(lum, 0.05)
+                rr = pcg_float()
+                if rr >= qq:
+                    cont = False
+                else:
+                    iq = 1.0 / qq
+                    beta = float3(beta.x * iq, beta.y * iq, beta.z * iq)
+        throughput.write(p, float4(beta.x, beta.y, beta.z, pdf))
+        rng.write(2 * p, state)
+        break
+    # appends: lanes count per wave into the shared counts, one thread reserves the workgroup's
+    # slots of each queue, lanes write in wave order within the workgroup's range
+    wg = shared_array("uint", 4)
+    tid = thread_id().x
+    if tid == 0:
+        wg[0] = 0
+        wg[1] = 0
+    sync_block()
     ns = warp_active_count_bits(shadow)
-    sbase = uint(0)
+    nc = warp_active_count_bits(cont)
+    sw = uint(0)
+    cw = uint(0)
     if warp_is_first_active_lane():
-        sbase = counters.atomic_fetch_add(2, ns)
-    sbase = warp_read_first_active_lane(sbase)
+        sw = wg.atomic_fetch_add(0, ns)
+        cw = wg.atomic_fetch_add(1, nc)
+    sw = warp_read_first_active_lane(sw)
+    cw = warp_read_first_active_lane(cw)
+    sync_block()
+    if tid == 0:
+        wg[2] = counters.atomic_fetch_add(64, wg[0])
+        wg[3] = counters.atomic_fetch_add(32 * (1 - in_queue), wg[1])
+    sync_block()
     if shadow:
-        si = sbase + warp_prefix_count_bits(shadow)
+        si = wg[2] + sw + warp_prefix_count_bits(shadow)
         shadow_rays.write(si, make_ray(offset_ray_origin(pp, n), wi, 0.0, d_light))
         shadow_paths.write(si, p)
         shadow_contrib.write(si, float4(contrib.x, contrib.y, contrib.z, 0.0))
-    nc = warp_active_count_bits(cont)
-    cbase = uint(0)
-    if warp_is_first_active_lane():
-        cbase = counters.atomic_fetch_add(1 - in_queue, nc)
-    cbase = warp_read_first_active_lane(cbase)
     if cont:
-        ci = cbase + warp_prefix_count_bits(cont)
+        ci = wg[3] + cw + warp_prefix_count_bits(cont)
         rays_out.write(ci, make_ray(pp, nd, 0.0, FLT_MAX))
         paths_out.write(ci, p)
 
@@ -453,7 +479,7 @@ def wf_resolve(shadow_paths: Buffer("uint"), shadow_contrib: Buffer("float4"), o
     """Adds the contribution of every unoccluded shadow ray to its path (at most one shadow ray per
     path and bounce, so the updates do not collide)."""
     j = dispatch_id().x
-    if j >= counters.read(2):
+    if j >= counters.read(64):
         return
     if occluded.read(j) == 0:
         p = shadow_paths.read(j)
@@ -599,7 +625,7 @@ class WavefrontPathTracer:
         self.throughput, self.radiance = B(16, m), B(16, m)
         self.rng = B(8, 2 * m)
         self.total = B(16, n)
-        self.counters, self.frame = B(4, 4), B(4, 4)
+        self.counters, self.frame = B(4, 96), B(4, 4)
         g = globals()
         self.k = {name: g[name].compile(device) for name in self.KERNELS}
         self._waves = {}
@@ -615,6 +641,8 @@ class WavefrontPathTracer:
         cmds = [lambda s, c=cam: s << c]
         # the callables hold the buffers themselves, not self (no reference cycle through self)
         hits, ctr, srays, occ = self.hits, self.counters, self.shadow_rays, self.occluded
+        # wf_shade's workgroups all reach its barriers: whole workgroups, the tail idle
+        shade_threads = -(-m // 512) * 512
         for depth in range(max_depth):
             qi = depth & 1
             reset = k["wf_reset"](self.counters, 1 - qi).dispatch(1)
@@ -622,13 +650,14 @@ class WavefrontPathTracer:
                                   self.shadow_rays, self.shadow_paths, self.shadow_contrib, self.throughput,
                                   self.radiance, self.rng, self.counters, b.heap, b.xforms, b.materials,
                                   v("light_position"), v("light_u"), v("light_v"), v("light_emission"), depth,
-                                  max_depth, rr_depth, qi).dispatch(m)
+                                  max_depth, rr_depth, qi).dispatch(shade_threads)
             resolve = k["wf_resolve"](self.shadow_paths, self.shadow_contrib, self.occluded, self.radiance,
                                       self.counters).dispatch(m)
             cmds += [lambda s, c=reset: s << c,
-                     lambda s, r=self.rays[qi], q=qi: accel.trace_closest_indirect(s, r, hits, ctr, q, capacity=m),
+                     lambda s, r=self.rays[qi], q=qi: accel.trace_closest_indirect(s, r, hits, ctr, 32 * q,
+                                                                                   capacity=m),
                      lambda s, c=shade: s << c,
-                     lambda s: accel.trace_any_indirect(s, srays, occ, ctr, 2, capacity=m),
+                     lambda s: accel.trace_any_indirect(s, srays, occ, ctr, 64, capacity=m),
                      lambda s, c=resolve: s << c]
         acc = k["wf_accumulate"](self.radiance, self.total, self.frame, n, samples).dispatch(n)
         cmds.append(lambda s, c=acc: s << c)
