@@ -1528,15 +1528,12 @@ hipError_t proc_records(hipStream_t stream, const uint32_t *ids, const float4 *b
     return hipGetLastError();
 }
 
-// Refits a triangle BLAS built by bvh_build over the same scratch (its counters hold the level
-// sizes) for new vertex positions; depth = the build's wide depth.
-hipError_t bvh_refit(hipStream_t stream, void *scratch, size_t scratch_bytes, const void *vbase, uint64_t vstride,
-                     uint64_t nverts, const uint32_t *tri, uint32_t n, uint32_t depth, WideNode *nodes,
-                     TriRecord *tris, BuildCounters **counters_out, int cu_count, uint32_t *error_flag) {
-    if (scratch_bytes < bvh_scratch_bytes(n) || nverts == 0) return hipErrorInvalidValue;
-    ScratchLayout L(scratch, n);
-    *counters_out = L.ctr;
-    if (n == 0) return hipSuccess;
+namespace {
+// The refit's launches over the build's counters (level sizes; the root box is rewritten) and an
+// array of 2 float4 per wide node for the nodes' exact boxes.
+hipError_t refit_launches(hipStream_t stream, BuildCounters *ctr, float4 *exact, const void *vbase, uint64_t vstride,
+                          uint64_t nverts, const uint32_t *tri, uint32_t n, uint32_t depth, WideNode *nodes,
+                          TriRecord *tris, int cu_count, uint32_t *error_flag) {
     static const int refit_blocks_per_cu = [] {
         int b = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_refit_level, kCollapseBlock, 0) != hipSuccess || b < 1) b = 1;
@@ -1547,8 +1544,43 @@ hipError_t bvh_refit(hipStream_t stream, void *scratch, size_t scratch_bytes, co
                        stream, src, tris, n);
     const int grid = std::max(1, cu_count) * refit_blocks_per_cu;
     for (uint32_t level = depth; level-- > 0;)
-        hipLaunchKernelGGL(k_refit_level, dim3(grid), dim3(kCollapseBlock), 0, stream, L.ctr, level, nodes, tris, L.bt.cbox);
+        hipLaunchKernelGGL(k_refit_level, dim3(grid), dim3(kCollapseBlock), 0, stream, ctr, level, nodes, tris, exact);
     return hipGetLastError();
+}
+} // namespace
+
+// Refits a triangle BLAS built by bvh_build over the same scratch (its counters hold the level
+// sizes) for new vertex positions; depth = the build's wide depth.
+hipError_t bvh_refit(hipStream_t stream, void *scratch, size_t scratch_bytes, const void *vbase, uint64_t vstride,
+                     uint64_t nverts, const uint32_t *tri, uint32_t n, uint32_t depth, WideNode *nodes,
+                     TriRecord *tris, BuildCounters **counters_out, int cu_count, uint32_t *error_flag) {
+    if (scratch_bytes < bvh_scratch_bytes(n) || nverts == 0) return hipErrorInvalidValue;
+    ScratchLayout L(scratch, n);
+    *counters_out = L.ctr;
+    if (n == 0) return hipSuccess;
+    return refit_launches(stream, L.ctr, L.bt.cbox, vbase, vstride, nverts, tri, n, depth, nodes, tris, cu_count,
+                          error_flag);
+}
+
+// What a refit keeps of a build whose scratch is not kept: its counters (in a slot rounded up to
+// 256 B), then 2 float4 per wide node (32 B per node instead of the build scratch's ~240 B per
+// primitive).
+constexpr size_t kRefitCounterSlot = (sizeof(BuildCounters) + 255) & ~size_t(255);
+
+size_t bvh_refit_state_bytes(uint32_t node_count) {
+    return kRefitCounterSlot + 2 * (size_t)std::max<uint32_t>(node_count, 1u) * sizeof(float4);
+}
+
+BuildCounters *bvh_refit_state_counters(void *state) { return static_cast<BuildCounters *>(state); }
+
+hipError_t bvh_refit_from_state(hipStream_t stream, void *state, size_t state_bytes, uint32_t node_count,
+                                const void *vbase, uint64_t vstride, uint64_t nverts, const uint32_t *tri, uint32_t n,
+                                uint32_t depth, WideNode *nodes, TriRecord *tris, int cu_count, uint32_t *error_flag) {
+    if (!state || state_bytes < bvh_refit_state_bytes(node_count) || nverts == 0) return hipErrorInvalidValue;
+    if (n == 0) return hipSuccess;
+    float4 *exact = reinterpret_cast<float4 *>(static_cast<char *>(state) + kRefitCounterSlot);
+    return refit_launches(stream, static_cast<BuildCounters *>(state), exact, vbase, vstride, nverts, tri, n, depth,
+                          nodes, tris, cu_count, error_flag);
 }
 
 } // namespace lchip

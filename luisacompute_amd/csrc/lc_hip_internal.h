@@ -29,6 +29,13 @@ hipError_t bvh_build(hipStream_t stream, void *scratch, size_t scratch_bytes, co
 hipError_t bvh_refit(hipStream_t stream, void *scratch, size_t scratch_bytes, const void *vbase, uint64_t vstride,
                      uint64_t nverts, const uint32_t *tri, uint32_t n, uint32_t depth, WideNode *nodes,
                      TriRecord *tris, BuildCounters **counters_out, int cu_count, uint32_t *error_flag);
+// Refit state of a compacting build with allow_update (the build's scratch is not kept): the
+// counters followed by 2 float4 per wide node; copy the build's counters into it (same stream).
+size_t bvh_refit_state_bytes(uint32_t node_count);
+BuildCounters *bvh_refit_state_counters(void *state);
+hipError_t bvh_refit_from_state(hipStream_t stream, void *state, size_t state_bytes, uint32_t node_count,
+                                const void *vbase, uint64_t vstride, uint64_t nverts, const uint32_t *tri, uint32_t n,
+                                uint32_t depth, WideNode *nodes, TriRecord *tris, int cu_count, uint32_t *error_flag);
 // Procedural primitives: packed 24-B AABBs -> 2 float4 per box; leaf records from the build's
 // leaf-order primitive ids.
 hipError_t aabbs_to_boxes(hipStream_t stream, const float *aabb, uint32_t n, float4 *boxes);
@@ -363,6 +370,11 @@ struct Mesh {
     TriRecord *tris = nullptr;
     void *scratch = nullptr;
     size_t scratch_bytes = 0;
+    // compacting builds with allow_update: what a refit needs (bvh_refit_state_bytes), sized for
+    // refit_nodes wide nodes
+    void *refit_state = nullptr;
+    size_t refit_state_bytes = 0;
+    uint32_t refit_nodes = 0;
     HostCounters *host = nullptr; // pinned
     hipEvent_t ready = nullptr;   // counters copied to host
     bool built = false;

@@ -222,11 +222,11 @@ void copy_counters_async(BuildCounters *dctr, HostCounters *host, uint32_t *erro
 
 // Compacting builds (allow_compaction). cuda_mesh.cpp:42-89 builds into a temporary buffer, reads
 // the compacted size back and copies into the primitive's own buffer, which only grows. Here the
-// tree is built into the device's arena of worst-case wide nodes (n nodes for n primitives), and,
-// for a primitive without allow_update (no refit will need the build's scratch), into the arena's
-// build scratch; the host then waits for the node count and the used nodes are copied into the
-// primitive's own array. A built primitive thus holds node_count nodes instead of n and no build
-// scratch. Called with dev->mu held.
+// tree is built into the device's arena of worst-case wide nodes (n nodes for n primitives) with
+// the arena's build scratch; the host then waits for the node count and the used nodes are copied
+// into the primitive's own array. A built primitive thus holds node_count nodes instead of n and
+// no build scratch; with allow_update it also keeps the refit state (the build's counters and 32 B
+// per wide node), copied out of the arena scratch with the nodes. Called with dev->mu held.
 struct BuildTarget {
     WideNode *nodes;
     void *scratch;
@@ -244,21 +244,23 @@ hipError_t arena_acquire(Device *dev, Mesh *m, uint32_t n, BuildTarget &t) {
     dev->arena_node_cap = cap / sizeof(WideNode);
     if (e != hipSuccess) return e;
     t.nodes = dev->arena_nodes;
-    if (m->allow_update) {
-        e = ensure(m->scratch, m->scratch_bytes, bvh_scratch_bytes(n));
-        t.scratch = m->scratch;
-        t.scratch_bytes = m->scratch_bytes;
-    } else {
-        e = ensure(dev->arena_scratch, dev->arena_scratch_bytes, bvh_scratch_bytes(n));
-        t.scratch = dev->arena_scratch;
-        t.scratch_bytes = dev->arena_scratch_bytes;
+    e = ensure(dev->arena_scratch, dev->arena_scratch_bytes, bvh_scratch_bytes(n));
+    t.scratch = dev->arena_scratch;
+    t.scratch_bytes = dev->arena_scratch_bytes;
+    // a primitive that compacted before keeps no scratch of its own
+    if (m->scratch) {
+        hipFree(m->scratch);
+        m->scratch = nullptr;
+        m->scratch_bytes = 0;
     }
     return e;
 }
 
 // After the build's launches and counter copies on s: waits for the counters, grows the
-// primitive's node array to the node count if needed and copies the nodes out of the arena.
-int arena_compact(Device *dev, Mesh *m, hipStream_t s) {
+// primitive's node array to the node count if needed and copies the nodes out of the arena; with
+// allow_update (triangle meshes) also the refit state: the build's counters (`ctr`, in the arena
+// scratch) into a per-primitive array with room for the nodes' exact boxes.
+int arena_compact(Device *dev, Mesh *m, hipStream_t s, const BuildCounters *ctr) {
     LC_HIP_TRY(hipStreamSynchronize(s));
     const uint32_t count = m->host->node_count;
     if (count == 0 || count > dev->arena_node_cap) return fail(LC_HIP_ERROR_DEVICE_FAULT, "BVH node count out of range");
@@ -270,6 +272,12 @@ int arena_compact(Device *dev, Mesh *m, hipStream_t s) {
         m->node_cap = count;
     }
     LC_HIP_TRY(hipMemcpyAsync(m->nodes, dev->arena_nodes, (size_t)count * sizeof(WideNode), hipMemcpyDeviceToDevice, s));
+    if (m->allow_update && !m->procedural && ctr) {
+        LC_HIP_TRY(ensure(m->refit_state, m->refit_state_bytes, bvh_refit_state_bytes(count)));
+        LC_HIP_TRY(hipMemcpyAsync(bvh_refit_state_counters(m->refit_state), ctr, sizeof(BuildCounters),
+                                  hipMemcpyDeviceToDevice, s));
+        m->refit_nodes = count;
+    }
     LC_HIP_TRY(hipEventRecord(dev->arena_done, s));
     return LC_HIP_OK;
 }
@@ -901,6 +909,7 @@ void lc_hip_mesh_destroy(uint64_t device, uint64_t mesh) {
     if (m->proc_boxes) hipFree(m->proc_boxes);
     if (m->proc_ids) hipFree(m->proc_ids);
     if (m->scratch) hipFree(m->scratch);
+    if (m->refit_state) hipFree(m->refit_state);
     if (m->host) hipHostFree(m->host);
     if (m->ready) hipEventDestroy(m->ready);
     delete m;
@@ -973,7 +982,7 @@ int lc_hip_procedural_build(uint64_t device, uint64_t stream, uint64_t primitive
         LC_HIP_TRY(proc_records(s, m->proc_ids, m->proc_boxes, n, m->proc));
         copy_counters_async(ctr, m->host, S->error, s);
         if (m->allow_compaction) {
-            const int rc = arena_compact(dev, m, s);
+            const int rc = arena_compact(dev, m, s, nullptr);
             if (rc != LC_HIP_OK) return rc;
         }
     }
@@ -1029,8 +1038,15 @@ int lc_hip_mesh_build(uint64_t device, uint64_t stream, uint64_t mesh, uint32_t 
             LC_HIP_TRY(hipMemsetAsync(S->error, 0, sizeof(uint32_t), s));
             TimedRegion tr(dev, s, "mesh_refit");
             BuildCounters *ctr = nullptr;
-            hipError_t e = bvh_refit(s, m->scratch, m->scratch_bytes, vaddr, vertex_stride, nverts, taddr, n, depth,
-                                     m->nodes, m->tris, &ctr, dev->cu_count, S->error);
+            hipError_t e;
+            if (m->allow_compaction) { // the refit state kept by the compacting build
+                ctr = bvh_refit_state_counters(m->refit_state);
+                e = bvh_refit_from_state(s, m->refit_state, m->refit_state_bytes, m->refit_nodes, vaddr, vertex_stride,
+                                         nverts, taddr, n, depth, m->nodes, m->tris, dev->cu_count, S->error);
+            } else {
+                e = bvh_refit(s, m->scratch, m->scratch_bytes, vaddr, vertex_stride, nverts, taddr, n, depth, m->nodes,
+                              m->tris, &ctr, dev->cu_count, S->error);
+            }
             if (e != hipSuccess) return hip_fail(e, "bvh_refit");
             copy_counters_async(ctr, m->host, S->error, s);
         }
@@ -1070,7 +1086,7 @@ int lc_hip_mesh_build(uint64_t device, uint64_t stream, uint64_t mesh, uint32_t 
         if (e != hipSuccess) return hip_fail(e, "bvh_build");
         copy_counters_async(ctr, m->host, S->error, s);
         if (m->allow_compaction) {
-            const int rc = arena_compact(dev, m, s);
+            const int rc = arena_compact(dev, m, s, ctr);
             if (rc != LC_HIP_OK) return rc;
         }
     }
@@ -1103,7 +1119,7 @@ int lc_hip_mesh_memory(uint64_t device, uint64_t mesh, lc_hip_memory_usage *out)
     out->node_bytes = (uint64_t)m->node_cap * sizeof(WideNode);
     out->primitive_bytes = (uint64_t)m->tri_cap * sizeof(TriRecord) +
                            (uint64_t)m->proc_cap * (sizeof(ProcRecord) + 2 * sizeof(float4) + sizeof(uint32_t));
-    out->scratch_bytes = m->scratch ? (uint64_t)m->scratch_bytes : 0;
+    out->scratch_bytes = (m->scratch ? (uint64_t)m->scratch_bytes : 0) + (m->refit_state ? (uint64_t)m->refit_state_bytes : 0);
     out->total_bytes = out->node_bytes + out->primitive_bytes + out->scratch_bytes;
     return LC_HIP_OK;
 }

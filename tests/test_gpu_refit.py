@@ -13,11 +13,11 @@ from parity import compare_any, compare_closest
 pytestmark = pytest.mark.gpu
 
 
-def _mesh(device, stream, verts, tris, allow_update=True):
+def _mesh(device, stream, verts, tris, allow_update=True, compact=True):
     vb = device.create_buffer(16, verts.shape[0])
     tb = device.create_buffer(12, tris.shape[0])
     stream << vb.copy_from(np.ascontiguousarray(verts, np.float32)) << tb.copy_from(np.ascontiguousarray(tris, np.uint32))
-    opt = lc.AccelOption(lc.AccelOption.FAST_TRACE, 1, 1 if allow_update else 0)
+    opt = lc.AccelOption(lc.AccelOption.FAST_TRACE, 1 if compact else 0, 1 if allow_update else 0)
     mesh = device.create_mesh(vb, tb, option=opt)
     accel = device.create_accel()
     accel.emplace_back(mesh, lc.IDENTITY)
@@ -47,14 +47,28 @@ def _deform(verts, k):
     return v
 
 
-def test_refit_matches_oracle_on_moved_geometry(device, stream):
+@pytest.mark.parametrize("compact", [True, False])
+def test_refit_matches_oracle_on_moved_geometry(device, stream, compact):
+    """Compacting meshes refit from the kept refit state (the build's counters and 32 B per wide
+    node; another mesh's build reuses the arena scratch in between), others from their own build
+    scratch."""
     verts, tris = lc.icosphere(5)
     rays = lc.rays_orbit_to_ball(1 << 16)
-    mesh, accel, vb = _mesh(device, stream, verts, tris)
+    mesh, accel, vb = _mesh(device, stream, verts, tris, compact=compact)
     device.enable_timing(True)
     device.reset_timing()
     stream << mesh.build() << accel.build()
     _check(device, stream, accel, verts, tris, rays, "refit base")
+    m = mesh.memory()
+    if compact:
+        # refit state only: the build's counters (a slot of a few 256-B units) + 2 float4 per wide node
+        slot = m["scratch_bytes"] - 32 * mesh.stats().node_count
+        assert 256 <= slot <= 4096 and slot % 256 == 0, m
+    # another compacting build overwrites the device's arena scratch before the refits
+    ov, ot = lc.soup(30000, initseq=97, extent=3.0, sigma=0.2)
+    other, other_accel, _ = _mesh(device, stream, ov, ot)
+    stream << other.build()
+    stream.synchronize()
     depth0 = mesh.stats().depth
     for k in (1, 2):
         moved = _deform(verts, k)
@@ -65,7 +79,7 @@ def test_refit_matches_oracle_on_moved_geometry(device, stream):
         assert all(v == 0 for v in errs.values()), errs
         st = mesh.stats()
         assert st.depth == depth0 and st.leaf_primitive_count == len(tris)
-    assert device.timing("mesh_build")[1] == 1
+    assert device.timing("mesh_build")[1] == 2
     assert device.timing("mesh_refit")[1] == 2
     device.enable_timing(False)
 
