@@ -710,7 +710,9 @@ def main():
 
     # algorithmic bytes: counter-mode run of the same traversal (same sorted queue, grid and
     # wave-uniform steps) on the same BVH and rays
-    n_nodes, n_tris = accel.trace_count(s, rb, a.rays)
+    steps = accel.trace_steps(s, rb, a.rays)
+    n_nodes, n_tris = steps.nodes_visited, steps.triangles_tested
+    lane_slots = 64 * max(steps.wave_steps, 1)
     st = mesh.stats()
     mem = mesh.memory()
     nodes_per_ray = n_nodes / a.rays
@@ -783,6 +785,13 @@ def main():
             # the HBM roofline is the one priced here; the counter profile shows the kernel is
             # limited by VALU issue (divergent lanes), not by bandwidth
             "limiter": "valu_issue",
+            # lane accounting of the counting run: of the wave iterations' lane slots, the share
+            # with a ray, and the shares taking a node visit / a triangle test
+            "lane_steps": {"live": round(steps.live_lane_steps / lane_slots, 4),
+                           "node_visits": round(n_nodes / lane_slots, 4),
+                           "triangle_tests": round(n_tris / lane_slots, 4),
+                           "wave_iterations_per_ray": round(steps.wave_steps * 64 / a.rays, 2),
+                           "refill_rounds": steps.refill_rounds},
             "bytes_per_ray": round(bytes_per_ray, 2),
             "compulsory_floor_gbs": round(floor_gbs, 2),
         },

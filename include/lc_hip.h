@@ -269,6 +269,20 @@ int lc_hip_accel_trace_any_indirect(uint64_t device, uint64_t stream, uint64_t a
 int lc_hip_accel_trace_count(uint64_t device, uint64_t stream, uint64_t accel, uint64_t ray_buffer, size_t ray_offset,
                              uint32_t ray_count, uint32_t vis_mask, int any, uint64_t *nodes_visited,
                              uint64_t *triangles_tested);
+/* The same counter-mode run with its lane accounting (no reference counterpart: measurement).
+   A lane takes one traversal step per loop iteration (a node visit, a triangle test, or nothing
+   when the wave's step kind does not match what it holds); live_lane_steps counts the iterations
+   lanes spent with a ray, wave_steps the iterations waves issued (x 64 = lane slots), and
+   refill_rounds the times waves handed new rays to their idle lanes. */
+typedef struct lc_hip_trace_steps {
+    uint64_t nodes_visited;
+    uint64_t triangles_tested;
+    uint64_t live_lane_steps;
+    uint64_t wave_steps;
+    uint64_t refill_rounds;
+} lc_hip_trace_steps;
+int lc_hip_accel_trace_steps(uint64_t device, uint64_t stream, uint64_t accel, uint64_t ray_buffer, size_t ray_offset,
+                             uint32_t ray_count, uint32_t vis_mask, int any, lc_hip_trace_steps *out);
 
 /* Closest hits that allocate nothing. No counterpart in the reference: for callers that cannot
    allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
@@ -572,6 +586,11 @@ int lc_hip_trace_set_ray_sort(uint64_t device, int kind, int enable);
    (segments 1) or from a segment of the queue per XCD (segments 8). */
 int lc_hip_trace_set_fetch(uint64_t device, int kind, uint32_t chunk, uint32_t segments);
 int lc_hip_trace_get_fetch(uint64_t device, int kind, uint32_t *chunk, uint32_t *segments);
+/* Traversal step policy: closest hit takes a triangle step when 4 x (lanes holding triangles) >=
+   tri_quarters x (live lanes), tri_quarters in 1..4 (default 2); a wave stops stepping early to
+   refill once fewer than refill_below of its 64 lanes hold a ray (0 = off, the default). */
+int lc_hip_trace_set_steps(uint64_t device, int kind, uint32_t tri_quarters, uint32_t refill_below);
+int lc_hip_trace_get_steps(uint64_t device, int kind, uint32_t *tri_quarters, uint32_t *refill_below);
 /* Build knob: 1 = also open multi-triangle leaf candidates while a node has free slots. */
 int lc_hip_build_set_split_small(uint64_t device, int enable);
 /* Build knob: SAH refinement of FAST_TRACE builds (the radix tree's clusters of <= 64 primitives

@@ -61,6 +61,13 @@ BUILD_PREFER_UPDATE = 0
 BUILD_FORCE_BUILD = 1
 
 
+class TraceSteps(ctypes.Structure):
+    """lc_hip_trace_steps: lane accounting of a counting traversal run."""
+    _fields_ = [("nodes_visited", ctypes.c_uint64), ("triangles_tested", ctypes.c_uint64),
+                ("live_lane_steps", ctypes.c_uint64), ("wave_steps", ctypes.c_uint64),
+                ("refill_rounds", ctypes.c_uint64)]
+
+
 class QueryPolicy(ctypes.Structure):
     """lc_hip_query_policy: built-in surface / procedural candidate callbacks of a ray query."""
     _fields_ = [("surface_mode", ctypes.c_uint32), ("cutout", ctypes.c_float), ("procedural_mode", ctypes.c_uint32),
@@ -258,6 +265,8 @@ _SIGNATURES = {
     "lc_hip_accel_trace_any": (ctypes.c_int, [_U64, _U64, _U64, _U64, _SZ, _U64, _SZ, _U32, _U32]),
     "lc_hip_accel_trace_count": (ctypes.c_int, [_U64, _U64, _U64, _U64, _SZ, _U32, _U32, ctypes.c_int,
                                                 ctypes.POINTER(_U64), ctypes.POINTER(_U64)]),
+    "lc_hip_accel_trace_steps": (ctypes.c_int, [_U64, _U64, _U64, _U64, _SZ, _U32, _U32, ctypes.c_int,
+                                                ctypes.POINTER(TraceSteps)]),
     "lc_hip_mesh_stats": (ctypes.c_int, [_U64, _U64, ctypes.POINTER(BvhStats)]),
     "lc_hip_mesh_validate": (ctypes.c_int, [_U64, _U64, _U64, ctypes.POINTER(_U32)]),
     "lc_hip_mesh_memory": (ctypes.c_int, [_U64, _U64, ctypes.POINTER(MemoryUsage)]),
@@ -269,6 +278,8 @@ _SIGNATURES = {
     "lc_hip_trace_set_ray_sort": (ctypes.c_int, [_U64, ctypes.c_int, ctypes.c_int]),
     "lc_hip_trace_set_fetch": (ctypes.c_int, [_U64, ctypes.c_int, _U32, _U32]),
     "lc_hip_trace_get_fetch": (ctypes.c_int, [_U64, ctypes.c_int, ctypes.POINTER(_U32), ctypes.POINTER(_U32)]),
+    "lc_hip_trace_set_steps": (ctypes.c_int, [_U64, ctypes.c_int, _U32, _U32]),
+    "lc_hip_trace_get_steps": (ctypes.c_int, [_U64, ctypes.c_int, ctypes.POINTER(_U32), ctypes.POINTER(_U32)]),
     "lc_hip_build_set_split_small": (ctypes.c_int, [_U64, ctypes.c_int]),
     "lc_hip_build_set_sah": (ctypes.c_int, [_U64, ctypes.c_int]),
     "lc_hip_build_set_width": (ctypes.c_int, [_U64, ctypes.c_int]),
@@ -513,6 +524,18 @@ class Device:
         _check(lib().lc_hip_trace_get_fetch(self.handle, 1 if kind == "any" else 0, ctypes.byref(c), ctypes.byref(s)),
                "fetch")
         return c.value, s.value
+
+    def set_trace_steps(self, kind, tri_quarters, refill_below=0):
+        """Traversal step policy: triangle-step threshold (1..4 quarters of the live lanes) and
+        early refill below that many live lanes (0 off)."""
+        _check(lib().lc_hip_trace_set_steps(self.handle, 1 if kind == "any" else 0, tri_quarters, refill_below),
+               "set_trace_steps")
+
+    def trace_steps(self, kind):
+        q, r = ctypes.c_uint32(), ctypes.c_uint32()
+        _check(lib().lc_hip_trace_get_steps(self.handle, 1 if kind == "any" else 0, ctypes.byref(q), ctypes.byref(r)),
+               "trace_steps")
+        return q.value, r.value
 
     def set_build_split_small(self, enable):
         _check(lib().lc_hip_build_set_split_small(self.handle, 1 if enable else 0), "build config")
@@ -1180,6 +1203,16 @@ class Accel:
         _check(lib().lc_hip_pathtrace(self.device.handle, stream.handle, self.handle, arr, len(instances),
                                       ctypes.byref(scene), ctypes.byref(options), image_buffer.handle, image_offset,
                                       counters.handle if counters is not None else 0), "pathtrace")
+
+    def trace_steps(self, stream, ray_buffer, count=None, vis_mask=0xFF, any_hit=False, ray_offset=0):
+        """The counting run's lane accounting (TraceSteps): node visits, triangle tests, live
+        lane-iterations, wave iterations and refill rounds."""
+        n = ray_buffer.count if count is None else count
+        out = TraceSteps()
+        _check(lib().lc_hip_accel_trace_steps(self.device.handle, stream.handle, self.handle, ray_buffer.handle,
+                                              ray_offset, n, vis_mask, 1 if any_hit else 0, ctypes.byref(out)),
+               "trace_steps")
+        return out
 
     def trace_count(self, stream, ray_buffer, count=None, vis_mask=0xFF, any_hit=False, ray_offset=0):
         n = ray_buffer.count if count is None else count

@@ -5,6 +5,10 @@ Stages (each on the best configuration of the stages before it):
   1. traversal, per trace kind (closest / any) on the 16.7M-triangle soup, searched jointly: block
      size, LDS stack depth, steps between ray refills, sorted or unsorted queue, and how the launch
      hands out queue slots (slots per claim, one queue or a segment per XCD);
+  1b. the traversal step policy on the best stage-1 launch shape and hand-out: the share of live
+     lanes that must hold triangles for a closest-hit wave to take a triangle step (1..3 quarters),
+     the early refill of waves whose live lanes fall below a count (off, 32, 48), and the steps
+     between refills again;
   2. BVH width (children per wide node, 4 / 6 / 8) on the soup step it serves: BLAS build +
      closest-hit trace;
   3. the wavefront path tracer on the box room: samples per wave (queue chunking), the fetch of its
@@ -32,6 +36,8 @@ STEPS = (8, 16, 32, 128)
 BLOCKS_PER_CU = (0,)
 SORT = (False, True)
 FETCH = ((0, 1), (64, 1), (256, 1), (64, 8), (256, 8))  # (slots per claim, queue segments)
+TRI_QUARTERS = (1, 2, 3)
+REFILL_BELOW = (0, 32, 48)
 WIDTHS = (4, 6, 8)
 PT_CHUNKS = (0, 2, 4, 8, 16)        # samples per wave (0: about 32M paths per wave)
 PT_FETCH = ((0, 1), (64, 8), (128, 8), (256, 8))
@@ -48,6 +54,7 @@ def apply(device, table):
             device.set_trace_config(kind, c["block"], c["lds_stack"], c["steps"], c.get("blocks_per_cu", 0))
             device.set_ray_sort(kind, c.get("sort_rays", False))
             device.set_fetch(kind, c.get("fetch_chunk", 0), c.get("fetch_segments", 1))
+            device.set_trace_steps(kind, c.get("tri_quarters", 2), c.get("refill_below", 0))
     b = table.get("bvh")
     if b and "width" in b:
         device.set_build_width(b["width"])
@@ -117,6 +124,28 @@ def tune(device, stream, accel, ray_buffer, hit_buffer, any_buffer, count, repea
         device.set_ray_sort(kind, c["sort_rays"])
         device.set_fetch(kind, c["fetch_chunk"], c["fetch_segments"])
         result[kind] = dict(c, ms=round(ms, 4), mrays_s=round(count / ms / 1e3, 2))
+    return result
+
+
+def tune_steps(device, stream, accel, ray_buffer, hit_buffer, any_buffer, count, result, repeats=3, log=print):
+    """Stage 1b: the step policy on the best stage-1 configuration of each kind (the triangle-step
+    threshold applies to closest hit only)."""
+    for kind, buf in (("closest", hit_buffer), ("any", any_buffer)):
+        fn = (lambda: accel.trace_closest(stream, ray_buffer, buf, count)) if kind == "closest" else \
+             (lambda: accel.trace_any(stream, ray_buffer, buf, count))
+        c = result[kind]
+        timed = []
+        for tq, rb, steps in itertools.product(TRI_QUARTERS if kind == "closest" else (2,), REFILL_BELOW, STEPS):
+            device.set_trace_config(kind, c["block"], c["lds_stack"], steps, c["blocks_per_cu"])
+            device.set_trace_steps(kind, tq, rb)
+            ms = _time(device, stream, fn, "trace_" + kind, repeats)
+            log(f"  {kind:7s} tri_quarters={tq} refill_below={rb:2d} steps={steps:3d}: {ms:.3f} ms "
+                f"({count / ms / 1e3:.1f} Mrays/s)")
+            timed.append((ms, tq, rb, steps))
+        ms, tq, rb, steps = min(timed)
+        device.set_trace_config(kind, c["block"], c["lds_stack"], steps, c["blocks_per_cu"])
+        device.set_trace_steps(kind, tq, rb)
+        c.update(steps=steps, tri_quarters=tq, refill_below=rb, ms=round(ms, 4), mrays_s=round(count / ms / 1e3, 2))
     return result
 
 
@@ -229,6 +258,8 @@ def main():
     ab = dev.create_buffer(1, a.rays)
     log("[autotune] stage 1: traversal")
     res = tune(dev, s, accel, rb, hb, ab, a.rays, log=log)
+    log("[autotune] stage 1b: step policy")
+    res = tune_steps(dev, s, accel, rb, hb, ab, a.rays, res, log=log)
     log("[autotune] stage 2: BVH width")
     res["bvh"] = tune_width(dev, s, mesh, accel, rb, hb, a.rays, log=log)
     del accel, mesh, rb, hb, ab

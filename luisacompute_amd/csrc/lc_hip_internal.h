@@ -85,6 +85,10 @@ struct TraceParams {
     uint32_t hit_record;  // closest hits carry the leaf record index in their padding word
     uint32_t fetch_chunk;    // queue slots a wave claims per atomic (0: exactly what its idle lanes need)
     uint32_t fetch_segments; // 1: one queue counter; kFetchSegments: one segment + counter per XCD
+    uint32_t tri_quarters;   // closest hit: a step tests triangles when 4 x (lanes holding some) >=
+                             // tri_quarters x (live lanes)
+    uint32_t refill_below;   // a wave stops its steps early to refill once fewer lanes than this
+                             // hold a ray (0: refill only after `steps` steps)
 };
 
 // ---- kernels (raysort.hip) ----
@@ -245,6 +249,8 @@ struct TraceConfig {
     bool sort_rays = false; // traverse a queue sorted by origin Morton code + direction octant
     uint32_t fetch_chunk = 0;    // see TraceParams
     uint32_t fetch_segments = 1;
+    uint32_t tri_quarters = 2;   // see TraceParams
+    uint32_t refill_below = 0;
 };
 
 struct TimingStat {

@@ -890,6 +890,8 @@ static int run_wave(Device *dev, hipStream_t s, Workspace &W, const PtBinding &B
             P.steps = cc.steps;
             P.fetch_chunk = opt.fetch_chunk;
             P.fetch_segments = opt.fetch_segments;
+            P.tri_quarters = cc.tri_quarters;
+            P.refill_below = cc.refill_below;
             P.stats = nullptr;
             P.error = dev->error;
             P.hit_record = 1u;
@@ -905,6 +907,7 @@ static int run_wave(Device *dev, hipStream_t s, Workspace &W, const PtBinding &B
             A.ray_counter = Q.ctr + kCtrAny;
             A.spill_depth = sd_a;
             A.steps = ca.steps;
+            A.refill_below = ca.refill_below;
             A.hit_record = 0u;
             e = trace_launch(A, s, true, two, false, grid_a, ca.block, ca.slds);
             if (e != hipSuccess) return e;

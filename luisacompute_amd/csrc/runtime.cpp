@@ -1600,8 +1600,7 @@ int lc_hip_accel_instance_write(uint64_t device, uint64_t stream, uint64_t accel
 // `count_ptr` (one batch) the batch's ray count is its capacity and the launch traces the first
 // min(*count_ptr, capacity) rays, the count being read on the device when the launch runs.
 static int trace_launch_batches(Device *dev, Accel *a, hipStream_t s, const lc_hip_ray_batch *batches, size_t nb,
-                                bool any, bool counting, uint64_t *n_nodes, uint64_t *n_tris,
-                                const uint32_t *count_ptr = nullptr) {
+                                bool any, bool counting, uint64_t counts[5], const uint32_t *count_ptr = nullptr) {
     TraceParams P;
     std::memset(&P, 0, sizeof(P));
     uint64_t total = 0;
@@ -1631,8 +1630,7 @@ static int trace_launch_batches(Device *dev, Accel *a, hipStream_t s, const lc_h
     P.batches.n = (uint32_t)nb;
     P.count = (uint32_t)total;
     P.count_ptr = count_ptr;
-    if (counting && n_nodes) *n_nodes = 0;
-    if (counting && n_tris) *n_tris = 0;
+    if (counting && counts) std::memset(counts, 0, 5 * sizeof(uint64_t));
     if (total == 0) return LC_HIP_OK;
     bool two = false;
     uint32_t stack_need = 0;
@@ -1656,7 +1654,9 @@ static int trace_launch_batches(Device *dev, Accel *a, hipStream_t s, const lc_h
     P.steps = cfg.steps;
     P.fetch_chunk = cfg.fetch_chunk;
     P.fetch_segments = cfg.fetch_segments;
-    if (counting) LC_HIP_TRY(hipMemsetAsync(S->stats, 0, 2 * sizeof(unsigned long long), s));
+    P.tri_quarters = cfg.tri_quarters;
+    P.refill_below = cfg.refill_below;
+    if (counting) LC_HIP_TRY(hipMemsetAsync(S->stats, 0, 5 * sizeof(unsigned long long), s));
     // per-ray results do not depend on the order; the counting form traverses the same sorted
     // queue as the timed form, so its node / triangle totals describe that launch. Queues counted
     // on the device are traced in the order their producer wrote them, and so are captured
@@ -1676,17 +1676,17 @@ static int trace_launch_batches(Device *dev, Accel *a, hipStream_t s, const lc_h
         if (e != hipSuccess) return hip_fail(e, "trace launch");
     }
     if (counting) {
-        unsigned long long st[2] = {0, 0};
+        unsigned long long st[5] = {0, 0, 0, 0, 0};
         LC_HIP_TRY(hipMemcpyAsync(st, S->stats, sizeof(st), hipMemcpyDeviceToHost, s));
         LC_HIP_TRY(hipStreamSynchronize(s));
-        if (n_nodes) *n_nodes = st[0];
-        if (n_tris) *n_tris = st[1];
+        if (counts)
+            for (int k = 0; k < 5; k++) counts[k] = st[k];
     }
     return LC_HIP_OK;
 }
 
 static int trace_batches(uint64_t device, uint64_t stream, uint64_t accel, const lc_hip_ray_batch *batches, size_t nb,
-                         bool any, bool counting, uint64_t *n_nodes, uint64_t *n_tris) {
+                         bool any, bool counting, uint64_t counts[5]) {
     Device *dev = obj<Device>(device);
     Accel *a = obj<Accel>(accel);
     if (!dev || !a) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad trace handles");
@@ -1695,46 +1695,67 @@ static int trace_batches(uint64_t device, uint64_t stream, uint64_t accel, const
     if (!a->built) return fail(LC_HIP_ERROR_NOT_BUILT, "accel is not built");
     DeviceGuard g(dev);
     hipStream_t s = native_stream(stream);
-    uint64_t nodes = 0, tris = 0;
+    uint64_t sum[5] = {0, 0, 0, 0, 0};
     for (size_t k = 0; k < nb; k += kMaxBatches) {
-        uint64_t cn = 0, ct = 0;
-        int rc = trace_launch_batches(dev, a, s, batches + k, std::min<size_t>(kMaxBatches, nb - k), any, counting, &cn, &ct);
+        uint64_t c[5] = {0, 0, 0, 0, 0};
+        int rc = trace_launch_batches(dev, a, s, batches + k, std::min<size_t>(kMaxBatches, nb - k), any, counting, c);
         if (rc != LC_HIP_OK) return rc;
-        nodes += cn;
-        tris += ct;
+        for (int j = 0; j < 5; j++) sum[j] += c[j];
     }
-    if (n_nodes) *n_nodes = nodes;
-    if (n_tris) *n_tris = tris;
+    if (counts)
+        for (int j = 0; j < 5; j++) counts[j] = sum[j];
     return LC_HIP_OK;
 }
 
 int lc_hip_accel_trace_closest(uint64_t device, uint64_t stream, uint64_t accel, uint64_t ray_buffer, size_t ray_offset,
                                uint64_t hit_buffer, size_t hit_offset, uint32_t ray_count, uint32_t vis_mask) {
     const lc_hip_ray_batch b{ray_buffer, ray_offset, hit_buffer, hit_offset, ray_count, vis_mask};
-    return trace_batches(device, stream, accel, &b, 1, false, false, nullptr, nullptr);
+    return trace_batches(device, stream, accel, &b, 1, false, false, nullptr);
 }
 
 int lc_hip_accel_trace_any(uint64_t device, uint64_t stream, uint64_t accel, uint64_t ray_buffer, size_t ray_offset,
                            uint64_t hit_buffer, size_t hit_offset, uint32_t ray_count, uint32_t vis_mask) {
     const lc_hip_ray_batch b{ray_buffer, ray_offset, hit_buffer, hit_offset, ray_count, vis_mask};
-    return trace_batches(device, stream, accel, &b, 1, true, false, nullptr, nullptr);
+    return trace_batches(device, stream, accel, &b, 1, true, false, nullptr);
 }
 
 int lc_hip_accel_trace_closest_batched(uint64_t device, uint64_t stream, uint64_t accel, const lc_hip_ray_batch *batches,
                                        size_t batch_count) {
-    return trace_batches(device, stream, accel, batches, batch_count, false, false, nullptr, nullptr);
+    return trace_batches(device, stream, accel, batches, batch_count, false, false, nullptr);
 }
 
 int lc_hip_accel_trace_any_batched(uint64_t device, uint64_t stream, uint64_t accel, const lc_hip_ray_batch *batches,
                                    size_t batch_count) {
-    return trace_batches(device, stream, accel, batches, batch_count, true, false, nullptr, nullptr);
+    return trace_batches(device, stream, accel, batches, batch_count, true, false, nullptr);
 }
 
 int lc_hip_accel_trace_count(uint64_t device, uint64_t stream, uint64_t accel, uint64_t ray_buffer, size_t ray_offset,
                              uint32_t ray_count, uint32_t vis_mask, int any, uint64_t *nodes_visited,
                              uint64_t *triangles_tested) {
     const lc_hip_ray_batch b{ray_buffer, ray_offset, 0, 0, ray_count, vis_mask};
-    return trace_batches(device, stream, accel, &b, 1, any != 0, true, nodes_visited, triangles_tested);
+    uint64_t c[5] = {0, 0, 0, 0, 0};
+    const int rc = trace_batches(device, stream, accel, &b, 1, any != 0, true, c);
+    if (nodes_visited) *nodes_visited = c[0];
+    if (triangles_tested) *triangles_tested = c[1];
+    return rc;
+}
+
+// Lane accounting of the counting form (no reference counterpart: measurement of the persistent
+// traversal): node visits and triangle tests next to the loop iterations lanes spent live and the
+// wave iterations issued, so useful lane-steps / (64 x wave iterations) is the kernel's lane
+// utilisation in steps.
+int lc_hip_accel_trace_steps(uint64_t device, uint64_t stream, uint64_t accel, uint64_t ray_buffer, size_t ray_offset,
+                             uint32_t ray_count, uint32_t vis_mask, int any, lc_hip_trace_steps *out) {
+    if (!out) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "null trace steps output");
+    const lc_hip_ray_batch b{ray_buffer, ray_offset, 0, 0, ray_count, vis_mask};
+    uint64_t c[5] = {0, 0, 0, 0, 0};
+    const int rc = trace_batches(device, stream, accel, &b, 1, any != 0, true, c);
+    out->nodes_visited = c[0];
+    out->triangles_tested = c[1];
+    out->live_lane_steps = c[2];
+    out->wave_steps = c[3];
+    out->refill_rounds = c[4];
+    return rc;
 }
 
 // Traces over a queue whose length is produced on the GPU (a shader's appends): `capacity` rays
@@ -1759,7 +1780,7 @@ static int trace_indirect(uint64_t device, uint64_t stream, uint64_t accel, uint
     hipStream_t s = native_stream(stream);
     const lc_hip_ray_batch b{ray_buffer, ray_offset, hit_buffer, hit_offset, capacity, vis_mask};
     const uint32_t *count_ptr = reinterpret_cast<const uint32_t *>(static_cast<const char *>(cb->ptr) + count_offset);
-    return trace_launch_batches(dev, a, s, &b, 1, any, false, nullptr, nullptr, count_ptr);
+    return trace_launch_batches(dev, a, s, &b, 1, any, false, nullptr, count_ptr);
 }
 
 int lc_hip_accel_trace_closest_indirect(uint64_t device, uint64_t stream, uint64_t accel, uint64_t ray_buffer,
@@ -1834,6 +1855,8 @@ int lc_hip_accel_trace_closest_noalloc(uint64_t device, uint64_t stream, uint64_
     P.steps = cfg.steps;
     P.fetch_chunk = cfg.fetch_chunk;
     P.fetch_segments = cfg.fetch_segments;
+    P.tri_quarters = cfg.tri_quarters;
+    P.refill_below = cfg.refill_below;
     const hipError_t e = trace_launch(P, s, false, two, false, (int)grid, cfg.block, cfg.slds);
     if (e != hipSuccess) return hip_fail(e, "trace launch");
     return LC_HIP_OK;
@@ -2192,6 +2215,30 @@ int lc_hip_trace_set_fetch(uint64_t device, int kind, uint32_t chunk, uint32_t s
     std::lock_guard<std::mutex> lk(dev->mu);
     dev->cfg[kind].fetch_chunk = chunk;
     dev->cfg[kind].fetch_segments = segments;
+    return LC_HIP_OK;
+}
+
+// Traversal step policy of a trace kind (autotuned): the triangle-step threshold of closest-hit
+// traversal (4 x lanes holding triangles >= tri_quarters x live lanes, 1..4) and the early refill
+// (a wave refills once fewer than refill_below of its 64 lanes hold a ray; 0 off).
+int lc_hip_trace_set_steps(uint64_t device, int kind, uint32_t tri_quarters, uint32_t refill_below) {
+    Device *dev = obj<Device>(device);
+    if (!dev || kind < 0 || kind > 1) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "bad trace kind");
+    if (tri_quarters < 1 || tri_quarters > 4 || refill_below > 64)
+        return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "tri_quarters in 1..4, refill_below in 0..64");
+    std::lock_guard<std::mutex> dl(dev->mu);
+    dev->cfg[kind].tri_quarters = tri_quarters;
+    dev->cfg[kind].refill_below = refill_below;
+    return LC_HIP_OK;
+}
+
+int lc_hip_trace_get_steps(uint64_t device, int kind, uint32_t *tri_quarters, uint32_t *refill_below) {
+    Device *dev = obj<Device>(device);
+    if (!dev || kind < 0 || kind > 1 || !tri_quarters || !refill_below)
+        return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "bad trace kind");
+    std::lock_guard<std::mutex> dl(dev->mu);
+    *tri_quarters = dev->cfg[kind].tri_quarters;
+    *refill_below = dev->cfg[kind].refill_below;
     return LC_HIP_OK;
 }
 

@@ -87,6 +87,7 @@ void k_trace(const TraceParams P) {
     uint32_t pool = 0, pool_end = 0; // wave-uniform: unclaimed slots [pool, pool_end)
     bool drained = false;            // wave-uniform: every segment is used up
     bool active = false, exhausted = false;
+    uint64_t rounds = 0; // counting form: refill rounds of this wave
     uint32_t ray_id = 0; // index within the batch
     uint32_t batch = 0;  // batch of the current ray
     for (;;) {
@@ -149,7 +150,8 @@ void k_trace(const TraceParams P) {
             }
         }
         if (!__any(active)) break;
-        if (active && W.run(P.geo, P.steps)) {
+        if (COUNT && lane == (uint32_t)__ffsll((long long)__ballot(1)) - 1u) rounds++;
+        if (active && W.run(P.geo, P.steps, P.tri_quarters, P.refill_below, !drained)) {
             // the counting form has no output buffers (out is null)
             if (COUNT) {
             } else if (ANY) {
@@ -162,14 +164,11 @@ void k_trace(const TraceParams P) {
         }
     }
     if (COUNT) {
-        uint64_t n_nodes = W.n_nodes, n_tris = W.n_tris;
-        for (int o = 32; o > 0; o >>= 1) {
-            n_nodes += __shfl_xor(n_nodes, o, 64);
-            n_tris += __shfl_xor(n_tris, o, 64);
-        }
-        if (lane == 0) {
-            atomicAdd(reinterpret_cast<unsigned long long *>(P.stats), (unsigned long long)n_nodes);
-            atomicAdd(reinterpret_cast<unsigned long long *>(P.stats) + 1, (unsigned long long)n_tris);
+        // stats: node visits, triangle tests, live lane-iterations, wave iterations, refill rounds
+        uint64_t v[5] = {W.n_nodes, W.n_tris, W.n_live, W.n_wave, rounds};
+        for (int k = 0; k < 5; k++) {
+            for (int o = 32; o > 0; o >>= 1) v[k] += __shfl_xor(v[k], o, 64);
+            if (lane == 0) atomicAdd(reinterpret_cast<unsigned long long *>(P.stats) + k, (unsigned long long)v[k]);
         }
     }
 }

@@ -39,12 +39,22 @@ namespace trav {
 struct RaySetup {
     float o[3], d[3];
     float inv[3];
-    int kx, ky, kz;
+    uint32_t kxyz; // watertight axis permutation: kx | ky << 2 | kz << 4
     float Sx, Sy, Sz;
     uint32_t oct;
 };
 
 __device__ __forceinline__ float sel3(const float v[3], int k) { return k == 0 ? v[0] : (k == 1 ? v[1] : v[2]); }
+
+// (x, y, z)[k] for a lane-varying k in 0..2 as two flat selects on scalars: no array for the
+// compiler to index (which would move it to scratch memory) and no branches (a nested conditional
+// compiles to exec-masked branches, about 25 instructions per selection)
+__device__ __forceinline__ float pick3(float x, float y, float z, uint32_t k) {
+    float r = z;
+    r = k == 1u ? y : r;
+    r = k == 0u ? x : r;
+    return r;
+}
 
 __device__ __forceinline__ void ray_setup(RaySetup &r) {
     float ax = fabsf(r.d[0]), ay = fabsf(r.d[1]), az = fabsf(r.d[2]);
@@ -53,7 +63,7 @@ __device__ __forceinline__ void ray_setup(RaySetup &r) {
     int ky = kx + 1 == 3 ? 0 : kx + 1;
     float dkz = sel3(r.d, kz);
     if (dkz < 0.0f) { int t = kx; kx = ky; ky = t; }
-    r.kx = kx; r.ky = ky; r.kz = kz;
+    r.kxyz = (uint32_t)kx | ((uint32_t)ky << 2) | ((uint32_t)kz << 4);
     r.Sx = sel3(r.d, kx) / dkz;
     r.Sy = sel3(r.d, ky) / dkz;
     r.Sz = 1.0f / dkz;
@@ -68,12 +78,15 @@ __device__ __forceinline__ void ray_setup(RaySetup &r) {
 // Watertight test; identical operation order to the oracle's wt_intersect.
 __device__ __forceinline__ bool wt_intersect(const RaySetup &r, const float4 &a, const float4 &b, const float4 &c,
                                              float tmin, float tmax, float &t_out, float &u_out, float &v_out) {
-    const float v0[3] = {a.x, a.y, a.z}, v1[3] = {b.x, b.y, b.z}, v2[3] = {c.x, c.y, c.z};
-    const int kx = r.kx, ky = r.ky, kz = r.kz;
-    const float okx = sel3(r.o, kx), oky = sel3(r.o, ky), okz = sel3(r.o, kz);
-    const float A_x = sel3(v0, kx) - okx, A_y = sel3(v0, ky) - oky, A_z = sel3(v0, kz) - okz;
-    const float B_x = sel3(v1, kx) - okx, B_y = sel3(v1, ky) - oky, B_z = sel3(v1, kz) - okz;
-    const float C_x = sel3(v2, kx) - okx, C_y = sel3(v2, ky) - oky, C_z = sel3(v2, kz) - okz;
+    // vertex - origin per axis, then the components in (kx, ky, kz) order (selecting component k
+    // of the difference equals the difference of the k components)
+    const uint32_t kx = r.kxyz & 3u, ky = (r.kxyz >> 2) & 3u, kz = r.kxyz >> 4;
+    const float ax = a.x - r.o[0], ay = a.y - r.o[1], az = a.z - r.o[2];
+    const float A_x = pick3(ax, ay, az, kx), A_y = pick3(ax, ay, az, ky), A_z = pick3(ax, ay, az, kz);
+    const float bx = b.x - r.o[0], by = b.y - r.o[1], bz = b.z - r.o[2];
+    const float B_x = pick3(bx, by, bz, kx), B_y = pick3(bx, by, bz, ky), B_z = pick3(bx, by, bz, kz);
+    const float cx = c.x - r.o[0], cy = c.y - r.o[1], cz = c.z - r.o[2];
+    const float C_x = pick3(cx, cy, cz, kx), C_y = pick3(cx, cy, cz, ky), C_z = pick3(cx, cy, cz, kz);
     const float Ax = A_x - r.Sx * A_z, Ay = A_y - r.Sy * A_z;
     const float Bx = B_x - r.Sx * B_z, By = B_y - r.Sy * B_z;
     const float Cx = C_x - r.Sx * C_z, Cy = C_y - r.Sy * C_z;
@@ -216,12 +229,14 @@ struct Walker {
     uint32_t cur_inst;
     const WideNode *nodes;
     const TriRecord *tris;
-    uint64_t n_nodes, n_tris;
+    // counting form only: node visits, triangle tests, loop iterations this lane spent live in
+    // run(), and wave iterations (counted by the wave's first live lane)
+    uint64_t n_nodes, n_tris, n_live, n_wave;
 
     __device__ __forceinline__ void init(const Geometry &G, uint2 (*s)[BLOCK], uint32_t t, uint2 *spill_lane,
                                          uint32_t depth, uint32_t *err) {
         stk = s; tid = t; spill = spill_lane; spill_depth = depth; error = err;
-        sp = 0; n_nodes = 0; n_tris = 0;
+        sp = 0; n_nodes = 0; n_tris = 0; n_live = 0; n_wave = 0;
         // single-instance form: the BLAS pointers never change (kept uniform)
         in_blas = !TWO;
         nodes = TWO ? G.tlas_nodes : G.blas_nodes;
@@ -272,17 +287,29 @@ struct Walker {
         }
     }
 
-    // Runs up to `steps` traversal steps; true when the ray is finished.
-    __device__ __forceinline__ bool run(const Geometry &G, int steps) {
+    // Runs up to `steps` traversal steps; true when the ray is finished. Closest hit takes a
+    // triangle step when 4 x (lanes holding triangles) >= tri_quarters x (live lanes) (0: the
+    // default kTriQuarters); with
+    // refill_below > 0 the wave stops early (false for every live lane) once fewer lanes than that
+    // hold a ray, so that the caller can refill the idle ones (can_refill: slots are left).
+    __device__ __forceinline__ bool run(const Geometry &G, int steps, uint32_t tri_quarters = kTriQuarters,
+                                        uint32_t refill_below = 0u, bool can_refill = false) {
         constexpr bool SPEC = ANY ? kSpeculateAny : kSpeculate;
         for (int it = 0; it < steps; it++) {
+            if (COUNT) {
+                n_live++;
+                if (__lane_id() == (uint32_t)__ffsll((long long)__ballot(1)) - 1u) n_wave++;
+            }
+            if (refill_below != 0u && can_refill && it > 0 && (uint32_t)__popcll(__ballot(1)) < refill_below)
+                return false;
             const float tcull = ANY ? tmax : fminf(tmax, t_best);
             bool tri_step = t_mask != 0u;
             if (SPEC) {
                 if (t_mask == 0u && (g_mask & 0xFFu) == 0u && sp == 0) return true;
                 const uint32_t n_live = (uint32_t)__popcll(__ballot(1));
                 const uint32_t n_tri = (uint32_t)__popcll(__ballot(t_mask != 0u));
-                tri_step = 4u * n_tri >= (uint32_t)(ANY ? kTriQuartersAny : kTriQuarters) * n_live;
+                const uint32_t tq = ANY ? (uint32_t)kTriQuartersAny : (tri_quarters ? tri_quarters : (uint32_t)kTriQuarters);
+                tri_step = 4u * n_tri >= tq * n_live;
                 if (tri_step && t_mask == 0u) continue;
                 if (!tri_step && !kSpeculateNodes && t_mask != 0u) continue;
             }

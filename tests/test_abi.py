@@ -48,7 +48,8 @@ def test_c_struct_layouts_match_binding(tmp_path):
              "lc_hip_server_options": ctypes.sizeof(lc.ServerOptions),
              "lc_hip_render_request": ctypes.sizeof(lc.RenderRequest),
              "lc_hip_request_status": ctypes.sizeof(lc.RequestStatus),
-             "lc_hip_serving_stats": ctypes.sizeof(lc.ServingStats)}
+             "lc_hip_serving_stats": ctypes.sizeof(lc.ServingStats),
+             "lc_hip_trace_steps": ctypes.sizeof(lc.TraceSteps)}
     src = tmp_path / "sizes.c"
     src.write_text('#include <stdio.h>\n#include "lc_hip.h"\nint main(void) {\n' +
                    "".join(f'    printf("{k} %zu\\n", sizeof({k}));\n' for k in pairs) + "    return 0;\n}\n")
