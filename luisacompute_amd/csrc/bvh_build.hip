@@ -24,6 +24,7 @@
 
 #include <cfloat>
 #include <cstdint>
+#include <cstdlib>
 
 #include "lc_hip_bvh.h"
 #include "lc_hip_internal.h"
@@ -34,7 +35,10 @@ namespace {
 
 constexpr int kBlock = 256;
 // Collapse workgroup: groups of 8 lanes, one queue/triangle reservation per workgroup iteration.
-constexpr int kCollapseBlock = 256;
+// collapse workgroup (8 lanes per wide node; 512 measured best: 4.39 ms builds vs 4.90 at 256 and
+// 4.62 at 1024, the workgroup's one reservation atomic per pass against occupancy) and the refit's
+constexpr int kCollapseBlock = 512;
+constexpr int kRefitBlock = 256;
 constexpr uint32_t kNoEntry = 0xFFFFFFFFu;
 // bottom-up phases: entries per workgroup, the single-workgroup last phase, leaves per run of
 // the first phase
@@ -482,42 +486,65 @@ __device__ __forceinline__ uint32_t pick_exponent(float lo, float hi) {
         int b = ex + 127;
         e = (uint32_t)max(1, min(254, b));
     }
-    while (e < 254 && (double)lo + 255.0 * (double)__int_as_float((int)(e << 23)) < (double)hi) e++;
+    // 2^(b-127) > ext / 255 up to the rounding of ext and of the division, so one step up at most
+    // restores lo + 255 s >= hi (checked exactly in double precision)
+    if (e < 254 && (double)lo + 255.0 * (double)__int_as_float((int)(e << 23)) < (double)hi) e++;
     return e;
 }
 
-__device__ __forceinline__ uint32_t quant_lo(float v, float p, float s) {
-    float q = floorf((v - p) / s);
+// 1 / s for the power of two s = 2^(e - 127): exact, so (v - p) * inv equals (v - p) / s.
+__device__ __forceinline__ float inv_scale(uint32_t e) {
+    return e <= 253u ? __int_as_float((int)((254u - e) << 23)) : 1.0f / __int_as_float((int)(e << 23));
+}
+
+// Conservative quantised bounds of v in node frame (p, s): the largest q with p + q s <= v and
+// the smallest with p + q s >= v, within [0, 255]. The float quotient is off by at most one unit
+// (the rounding of v - p is far below s), so one exact double-precision check corrects it.
+__device__ __forceinline__ uint32_t quant_lo(float v, float p, float s, float inv) {
+    float q = floorf((v - p) * inv);
     q = fminf(fmaxf(q, 0.0f), 255.0f);
     uint32_t qi = (uint32_t)q;
-    while (qi > 0 && (double)p + (double)qi * (double)s > (double)v) qi--;
+    if (qi > 0 && (double)p + (double)qi * (double)s > (double)v) qi--;
     return qi;
 }
 
-__device__ __forceinline__ uint32_t quant_hi(float v, float p, float s) {
-    float q = ceilf((v - p) / s);
+__device__ __forceinline__ uint32_t quant_hi(float v, float p, float s, float inv) {
+    float q = ceilf((v - p) * inv);
     q = fminf(fmaxf(q, 0.0f), 255.0f);
     uint32_t qi = (uint32_t)q;
-    while (qi < 255 && (double)p + (double)qi * (double)s < (double)v) qi++;
+    if (qi < 255 && (double)p + (double)qi * (double)s < (double)v) qi++;
     return qi;
 }
 
-// Reductions over the 8 lanes of a collapse group.
+// Reductions over the 8 lanes of a collapse group through DPP lane moves (no LDS round trips):
+// pairs and quads by quad permutes, then the two quads of the group by the half-row mirror (lane
+// i and 7 - i lie in different quads). The 8 lanes of a group are active together.
+template<int CTRL>
+__device__ __forceinline__ uint32_t g8_move(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, CTRL, 0xF, 0xF, false);
+}
 __device__ __forceinline__ float g8_min(float v) {
-    v = fminf(v, __shfl_xor(v, 1, 8)); v = fminf(v, __shfl_xor(v, 2, 8)); return fminf(v, __shfl_xor(v, 4, 8));
+    v = fminf(v, __uint_as_float(g8_move<0xB1>(__float_as_uint(v))));
+    v = fminf(v, __uint_as_float(g8_move<0x4E>(__float_as_uint(v))));
+    return fminf(v, __uint_as_float(g8_move<0x141>(__float_as_uint(v))));
 }
 __device__ __forceinline__ float g8_max(float v) {
-    v = fmaxf(v, __shfl_xor(v, 1, 8)); v = fmaxf(v, __shfl_xor(v, 2, 8)); return fmaxf(v, __shfl_xor(v, 4, 8));
+    v = fmaxf(v, __uint_as_float(g8_move<0xB1>(__float_as_uint(v))));
+    v = fmaxf(v, __uint_as_float(g8_move<0x4E>(__float_as_uint(v))));
+    return fmaxf(v, __uint_as_float(g8_move<0x141>(__float_as_uint(v))));
 }
 __device__ __forceinline__ uint32_t g8_or(uint32_t v) {
-    v |= __shfl_xor(v, 1, 8); v |= __shfl_xor(v, 2, 8); return v | __shfl_xor(v, 4, 8);
+    v |= g8_move<0xB1>(v);
+    v |= g8_move<0x4E>(v);
+    return v | g8_move<0x141>(v);
+}
+template<int CTRL>
+__device__ __forceinline__ uint64_t g8_max_step(uint64_t v) {
+    const uint64_t w = ((uint64_t)g8_move<CTRL>((uint32_t)(v >> 32)) << 32) | g8_move<CTRL>((uint32_t)v);
+    return w > v ? w : v;
 }
 __device__ __forceinline__ uint64_t g8_max_u64(uint64_t v) {
-    for (int o = 1; o < 8; o <<= 1) {
-        uint64_t w = __shfl_xor(v, o, 8);
-        v = w > v ? w : v;
-    }
-    return v;
+    return g8_max_step<0x141>(g8_max_step<0x4E>(g8_max_step<0xB1>(v)));
 }
 __device__ __forceinline__ uint32_t ord_u32(float f) {
     uint32_t u = __float_as_uint(f);
@@ -799,12 +826,13 @@ __global__ __launch_bounds__(64) void k_sah_clusters(const uint32_t *entries, co
 // reserved with one 64-bit atomic per workgroup iteration. Leaf slots list their primitives in
 // out.prim_out in triangle-record order: primitive ids for a box build, sorted positions for a
 // triangle build, whose records are then copied from the sorted records (k_tri_records).
-__global__ __launch_bounds__(kCollapseBlock) void k_collapse(LeafBoxes lb, BinaryTree bt,
+template<int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_collapse(LeafBoxes lb, BinaryTree bt,
                                                      BuildCounters *ctr, uint32_t level, const uint2 *qin,
                                                      uint2 *qout, WideOut out, uint32_t split_small,
                                                      uint32_t leaf_max, uint32_t width) {
-    __shared__ __align__(16) uint32_t s_node[kCollapseBlock / 8][20];
-    __shared__ uint32_t s_wave[2][kCollapseBlock / 64];
+    __shared__ __align__(16) uint32_t s_node[BLOCK / 8][20];
+    __shared__ uint32_t s_wave[2][BLOCK / 64];
     __shared__ unsigned long long s_base;
     const uint32_t count = (uint32_t)(ctr->level[level] >> 32);
     if (count == 0) return;
@@ -884,6 +912,7 @@ __global__ __launch_bounds__(kCollapseBlock) void k_collapse(LeafBoxes lb, Binar
         const float3 nhi = make_float3(g8_max(hi.x), g8_max(hi.y), g8_max(hi.z));
         const uint32_t ex = pick_exponent(nlo.x, nhi.x), ey = pick_exponent(nlo.y, nhi.y), ez = pick_exponent(nlo.z, nhi.z);
         const float sx = __int_as_float((int)(ex << 23)), sy = __int_as_float((int)(ey << 23)), sz = __int_as_float((int)(ez << 23));
+        const float ix = inv_scale(ex), iy = inv_scale(ey), iz = inv_scale(ez);
         // ---- octant slots: repeatedly give the best remaining (child, free slot) pair its slot,
         // scoring dot(child centre - node centre, octant direction)
         const float3 pc = make_float3(0.5f * (nlo.x + nhi.x), 0.5f * (nlo.y + nhi.y), 0.5f * (nlo.z + nhi.z));
@@ -911,12 +940,16 @@ __global__ __launch_bounds__(kCollapseBlock) void k_collapse(LeafBoxes lb, Binar
         const bool internal = valid && cnt > leaf_max;
         const uint32_t imask = g8_or(internal ? (1u << slot) : 0u);
         const uint32_t leafcnt = (valid && !internal) ? cnt : 0u;
-        uint32_t toff = 0, ttotal = 0;
-        for (int k = 0; k < 8; k++) {
-            const uint32_t sk = __shfl(slot, k, 8), ck = __shfl(leafcnt, k, 8);
-            ttotal += ck;
-            if (sk < slot) toff += ck;
-        }
+        // each lane's leaf count in the nibble of its slot (counts <= kLeafMax < 16, slots
+        // distinct), OR-reduced: the group's counts by slot; toff = the nibbles below this slot
+        static_assert(kLeafMax < 16, "leaf counts packed in nibbles");
+        const uint32_t by_slot = g8_or(leafcnt ? leafcnt << (4u * slot) : 0u);
+        const auto nibble_sum = [](uint32_t x) {
+            x = (x & 0x0F0F0F0Fu) + ((x >> 4) & 0x0F0F0F0Fu);
+            return (x * 0x01010101u) >> 24;
+        };
+        const uint32_t ttotal = nibble_sum(by_slot);
+        const uint32_t toff = slot < 8u ? nibble_sum(by_slot & ((1u << (4u * slot)) - 1u)) : ttotal;
         // ---- reservations: inclusive scan of the 8 groups' counts (uniform within a group) over
         // the wave, wave totals through LDS, one 64-bit atomic per block for the next level's
         // queue slots (upper half) and this level's triangle records (lower half)
@@ -934,7 +967,7 @@ __global__ __launch_bounds__(kCollapseBlock) void k_collapse(LeafBoxes lb, Binar
         __syncthreads();
         if (threadIdx.x == 0) {
             uint32_t bi = 0, btri = 0;
-            for (uint32_t w = 0; w < kCollapseBlock / 64; w++) {
+            for (uint32_t w = 0; w < BLOCK / 64; w++) {
                 bi += s_wave[0][w];
                 btri += s_wave[1][w];
             }
@@ -985,12 +1018,12 @@ __global__ __launch_bounds__(kCollapseBlock) void k_collapse(LeafBoxes lb, Binar
         }
         if (valid) {
             nb[24 + slot] = (uint8_t)meta;
-            nb[32 + slot] = (uint8_t)quant_lo(lo.x, nlo.x, sx);
-            nb[40 + slot] = (uint8_t)quant_lo(lo.y, nlo.y, sy);
-            nb[48 + slot] = (uint8_t)quant_lo(lo.z, nlo.z, sz);
-            nb[56 + slot] = (uint8_t)quant_hi(hi.x, nlo.x, sx);
-            nb[64 + slot] = (uint8_t)quant_hi(hi.y, nlo.y, sy);
-            nb[72 + slot] = (uint8_t)quant_hi(hi.z, nlo.z, sz);
+            nb[32 + slot] = (uint8_t)quant_lo(lo.x, nlo.x, sx, ix);
+            nb[40 + slot] = (uint8_t)quant_lo(lo.y, nlo.y, sy, iy);
+            nb[48 + slot] = (uint8_t)quant_lo(lo.z, nlo.z, sz, iz);
+            nb[56 + slot] = (uint8_t)quant_hi(hi.x, nlo.x, sx, ix);
+            nb[64 + slot] = (uint8_t)quant_hi(hi.y, nlo.y, sy, iy);
+            nb[72 + slot] = (uint8_t)quant_hi(hi.z, nlo.z, sz, iz);
         }
         __syncthreads();
         if (active && lane < 5u)
@@ -1046,9 +1079,9 @@ __global__ __launch_bounds__(kBlock) void k_refit_records(PrimSource src, TriRec
 }
 
 // One wide level; 8 lanes per node, lane k = slot k. exact[2 i], exact[2 i + 1]: node i's box.
-__global__ __launch_bounds__(kCollapseBlock) void k_refit_level(BuildCounters *ctr, uint32_t level, WideNode *nodes,
+__global__ __launch_bounds__(kRefitBlock) void k_refit_level(BuildCounters *ctr, uint32_t level, WideNode *nodes,
                                                                 const TriRecord *tris, float4 *exact) {
-    __shared__ __align__(16) uint32_t s_node[kCollapseBlock / 8][20];
+    __shared__ __align__(16) uint32_t s_node[kRefitBlock / 8][20];
     const uint32_t count = (uint32_t)(ctr->level[level] >> 32);
     uint32_t start = 0;
     for (uint32_t j = 0; j < level; j++) start += (uint32_t)(ctr->level[j] >> 32);
@@ -1089,6 +1122,7 @@ __global__ __launch_bounds__(kCollapseBlock) void k_refit_level(BuildCounters *c
         const float3 nhi = make_float3(g8_max(hi.x), g8_max(hi.y), g8_max(hi.z));
         const uint32_t ex = pick_exponent(nlo.x, nhi.x), ey = pick_exponent(nlo.y, nhi.y), ez = pick_exponent(nlo.z, nhi.z);
         const float sx = __int_as_float((int)(ex << 23)), sy = __int_as_float((int)(ey << 23)), sz = __int_as_float((int)(ez << 23));
+        const float ix = inv_scale(ex), iy = inv_scale(ey), iz = inv_scale(ez);
         uint32_t *nw = s_node[threadIdx.x >> 3];
         uint8_t *nb = reinterpret_cast<uint8_t *>(nw);
         for (uint32_t w = 8u + lane; w < 20u; w += 8u) nw[w] = w < 14u ? 0xFFFFFFFFu : 0u;
@@ -1104,12 +1138,12 @@ __global__ __launch_bounds__(kCollapseBlock) void k_refit_level(BuildCounters *c
         }
         __syncthreads();
         if (active && meta != 0u) {
-            nb[32 + lane] = (uint8_t)quant_lo(lo.x, nlo.x, sx);
-            nb[40 + lane] = (uint8_t)quant_lo(lo.y, nlo.y, sy);
-            nb[48 + lane] = (uint8_t)quant_lo(lo.z, nlo.z, sz);
-            nb[56 + lane] = (uint8_t)quant_hi(hi.x, nlo.x, sx);
-            nb[64 + lane] = (uint8_t)quant_hi(hi.y, nlo.y, sy);
-            nb[72 + lane] = (uint8_t)quant_hi(hi.z, nlo.z, sz);
+            nb[32 + lane] = (uint8_t)quant_lo(lo.x, nlo.x, sx, ix);
+            nb[40 + lane] = (uint8_t)quant_lo(lo.y, nlo.y, sy, iy);
+            nb[48 + lane] = (uint8_t)quant_lo(lo.z, nlo.z, sz, iz);
+            nb[56 + lane] = (uint8_t)quant_hi(hi.x, nlo.x, sx, ix);
+            nb[64 + lane] = (uint8_t)quant_hi(hi.y, nlo.y, sy, iy);
+            nb[72 + lane] = (uint8_t)quant_hi(hi.z, nlo.z, sz, iz);
         }
         __syncthreads();
         if (active && lane < 5u)
@@ -1262,6 +1296,35 @@ size_t scan_temp_bytes(uint32_t n) {
 // Workgroups of a bottom-up phase (runs of >= kPhaseBlock entries or leaves).
 size_t phase_blocks(size_t m) { return m / kPhaseBlock + 1; }
 
+// Workgroup size of the collapse (LUISA_HIP_COLLAPSE_BLOCK: 256, 512 or 1024 threads, 8 per wide
+// node) and its resident workgroups per CU. A workgroup reserves its nodes' next-level queue slots
+// and triangle ranges with one atomic per pass over its items.
+struct CollapseShape {
+    int block;
+    int blocks_per_cu;
+};
+
+template<int B>
+int collapse_occupancy() {
+    int b = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_collapse<B>, B, 0) != hipSuccess || b < 1) b = 1;
+    return b;
+}
+
+CollapseShape collapse_shape() {
+    static const CollapseShape shape = [] {
+        int block = kCollapseBlock;
+        if (const char *v = std::getenv("LUISA_HIP_COLLAPSE_BLOCK")) {
+            const int b = std::atoi(v);
+            if (b == 256 || b == 512 || b == 1024) block = b;
+        }
+        const int bpc = block == 1024 ? collapse_occupancy<1024>() : (block == 512 ? collapse_occupancy<512>()
+                                                                                    : collapse_occupancy<256>());
+        return CollapseShape{block, bpc};
+    }();
+    return shape;
+}
+
 // The scratch arrays of a build over n primitives (bvh_scratch_bytes(n) bytes). The counters
 // stay valid after the build; a refit reuses them and the cbox array.
 struct ScratchLayout {
@@ -1358,12 +1421,8 @@ hipError_t bvh_build(hipStream_t stream, void *scratch, size_t scratch_bytes, co
     if (width < 2 || width > 8) return hipErrorInvalidValue;
     if (scratch_bytes < bvh_scratch_bytes(n)) return hipErrorInvalidValue;
     // the collapse grid fills the CUs once (its loop strides over the level's items)
-    static const int collapse_blocks_per_cu = [] {
-        int b = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_collapse, kCollapseBlock, 0) != hipSuccess || b < 1) b = 1;
-        return b;
-    }();
-    const int grid_collapse = std::max(1, cu_count) * collapse_blocks_per_cu;
+    const CollapseShape cs = collapse_shape();
+    const int grid_collapse = std::max(1, cu_count) * cs.blocks_per_cu;
     ScratchLayout L(scratch, n);
     uint32_t *k0 = L.k0, *k1 = L.k1, *i0 = L.i0, *i1 = L.i1;
     BinaryTree bt = L.bt;
@@ -1496,8 +1555,15 @@ hipError_t bvh_build(hipStream_t stream, void *scratch, size_t scratch_bytes, co
     for (uint32_t level = 0; level < kCollapseLaunches; level++) {
         const uint2 *qin = (level & 1) ? q1 : q0;
         uint2 *qout = (level & 1) ? q0 : q1;
-        hipLaunchKernelGGL(k_collapse, dim3(grid_collapse), dim3(kCollapseBlock), 0, stream, lb, bt, ctr, level, qin,
-                           qout, out, split_small, leaf_max, width);
+        if (cs.block == 1024)
+            hipLaunchKernelGGL(k_collapse<1024>, dim3(grid_collapse), dim3(1024), 0, stream, lb, bt, ctr, level, qin,
+                               qout, out, split_small, leaf_max, width);
+        else if (cs.block == 512)
+            hipLaunchKernelGGL(k_collapse<512>, dim3(grid_collapse), dim3(512), 0, stream, lb, bt, ctr, level, qin,
+                               qout, out, split_small, leaf_max, width);
+        else
+            hipLaunchKernelGGL(k_collapse<256>, dim3(grid_collapse), dim3(256), 0, stream, lb, bt, ctr, level, qin,
+                               qout, out, split_small, leaf_max, width);
         if ((level + 1) % kChunk == 0 && level + 1 < kCollapseLaunches) {
             e = hipMemcpyAsync(pinned_word, level_nodes_word(ctr, level + 1), sizeof(uint32_t), hipMemcpyDeviceToHost,
                                stream);
@@ -1536,7 +1602,7 @@ hipError_t refit_launches(hipStream_t stream, BuildCounters *ctr, float4 *exact,
                           TriRecord *tris, int cu_count, uint32_t *error_flag) {
     static const int refit_blocks_per_cu = [] {
         int b = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_refit_level, kCollapseBlock, 0) != hipSuccess || b < 1) b = 1;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_refit_level, kRefitBlock, 0) != hipSuccess || b < 1) b = 1;
         return b;
     }();
     PrimSource src{static_cast<const char *>(vbase), vstride, nverts, tri, nullptr, error_flag};
@@ -1544,7 +1610,7 @@ hipError_t refit_launches(hipStream_t stream, BuildCounters *ctr, float4 *exact,
                        stream, src, tris, n);
     const int grid = std::max(1, cu_count) * refit_blocks_per_cu;
     for (uint32_t level = depth; level-- > 0;)
-        hipLaunchKernelGGL(k_refit_level, dim3(grid), dim3(kCollapseBlock), 0, stream, ctr, level, nodes, tris, exact);
+        hipLaunchKernelGGL(k_refit_level, dim3(grid), dim3(kRefitBlock), 0, stream, ctr, level, nodes, tris, exact);
     return hipGetLastError();
 }
 } // namespace
