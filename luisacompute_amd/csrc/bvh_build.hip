@@ -403,8 +403,17 @@ __global__ __launch_bounds__(B) void k_bounds_leaves(LeafBoxes lb, const TriReco
                                                      BinaryTree bt, uint32_t *pend, uint32_t *bcount) {
     __shared__ float s_box[B][2][6];
     __shared__ uint32_t s_arrived[B];
+    // parent links of the run's leaves and of its internal nodes (a node completed inside the run
+    // has its range, and so its index, inside the run), read once with coalesced loads: the climb
+    // then takes its links from LDS instead of one dependent global load per step
+    __shared__ uint2 s_leaf_link[B], s_int_link[B];
     const uint32_t L0 = blockIdx.x * B, L1 = min(L0 + (uint32_t)B - 1u, n - 1u);
     s_arrived[threadIdx.x] = 0u;
+    {
+        const uint32_t i = L0 + threadIdx.x;
+        if (i <= L1) s_leaf_link[threadIdx.x] = bt.parent_leaf[i];
+        if (i <= L1 && i + 1u < n) s_int_link[threadIdx.x] = bt.parent_int[i]; // internal nodes 0..n-2
+    }
     __syncthreads();
     const uint32_t leaf = L0 + threadIdx.x;
     uint32_t deferred = kNoEntry;
@@ -414,7 +423,7 @@ __global__ __launch_bounds__(B) void k_bounds_leaves(LeafBoxes lb, const TriReco
         if (srec) copy_record(prec + lb.ids[leaf], srec + leaf, lo, hi); // triangle records in sorted order
         else stored_box(lb.pbox, lb.ids[leaf], lo, hi);
         for (;;) {
-            const uint2 link = (cur & kLeafBit) ? bt.parent_leaf[f] : bt.parent_int[cur];
+            const uint2 link = (cur & kLeafBit) ? s_leaf_link[f - L0] : s_int_link[cur - L0];
             const uint32_t p = link.x, side = link.y >> 31, end = link.y & ~kLeafBit;
             const uint32_t pf = side ? end : f, pl = side ? l : end;
             if (pf < L0 || pl > L1) {
