@@ -3,11 +3,14 @@
 // Incoherent rays make every node and triangle fetch of a wave miss the XCD-local L2 (measured
 // on the 16.7M-triangle soup: ~2x the algorithmic bytes reach memory). Ordering the queue by a
 // key of where the ray starts and where it points lets neighbouring lanes and consecutive
-// waves descend the same subtrees, so most of those fetches hit in L2. The key is the 27-bit
-// Morton code of the origin quantised (9 bits per axis) in the scene box, followed by the 3-bit
-// direction octant; the queue is the stable radix-sorted permutation of ray indices. Results are
-// still written to each ray's own slot.
+// waves descend the same subtrees, so most of those fetches hit in L2. The key is the Morton code
+// of the origin quantised in the scene box (7 bits per axis by default: 21 bits, three 8-bit radix
+// passes with the octant), followed by the 3-bit direction octant; the queue is the stable
+// radix-sorted permutation of ray indices. Results are still written to each ray's own slot.
+// Measured on the 16.7M-triangle soup: 9 bits per axis (four passes) gives the same traversal time
+// and a 0.07 ms slower sort; octant-major keys make the traversal 30% slower.
 #include <hip/hip_runtime.h>
+#include <cstdlib>
 #include <cstring>
 #include <rocprim/device/device_radix_sort.hpp>
 
@@ -29,7 +32,7 @@ __device__ __forceinline__ uint32_t expand9(uint32_t v) {
 }
 
 __global__ __launch_bounds__(256) void k_ray_keys(const BatchTable batches, uint32_t n, float3 lo, float3 scale,
-                                                  uint32_t *keys, uint32_t *idx) {
+                                                  uint32_t axis_bits, uint32_t *keys, uint32_t *idx) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t b = find_batch(batches, i);
@@ -40,7 +43,8 @@ __global__ __launch_bounds__(256) void k_ray_keys(const BatchTable batches, uint
     uint32_t c[3];
     for (int a = 0; a < 3; a++) c[a] = (uint32_t)fminf(fmaxf(q[a], 0.0f), 511.0f); // NaN -> 0
     const uint32_t oct = (d.x < 0.0f ? 1u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 4u : 0u);
-    keys[i] = ((expand9(c[0]) << 2 | expand9(c[1]) << 1 | expand9(c[2])) << 3) | oct;
+    const uint32_t m = expand9(c[0]) << 2 | expand9(c[1]) << 1 | expand9(c[2]); // 27-bit Morton code
+    keys[i] = ((m >> (27u - 3u * axis_bits)) << 3) | oct;
     idx[i] = i;
 }
 
@@ -74,9 +78,15 @@ hipError_t ray_sort(hipStream_t stream, const BatchTable &batches, uint32_t n, c
     sc.x = ext[0] > 0.0f ? 512.0f / ext[0] : 0.0f;
     sc.y = ext[1] > 0.0f ? 512.0f / ext[1] : 0.0f;
     sc.z = ext[2] > 0.0f ? 512.0f / ext[2] : 0.0f;
-    hipLaunchKernelGGL(k_ray_keys, dim3((n + 255) / 256), dim3(256), 0, stream, batches, n, lo, sc, k0, i0);
+    // origin bits per axis (LUISA_HIP_RAY_SORT_AXIS_BITS, 1..9; 7)
+    static const uint32_t axis_bits = [] {
+        const char *v = std::getenv("LUISA_HIP_RAY_SORT_AXIS_BITS");
+        const int b = v ? std::atoi(v) : 7;
+        return (uint32_t)(b >= 1 && b <= 9 ? b : 7);
+    }();
+    hipLaunchKernelGGL(k_ray_keys, dim3((n + 255) / 256), dim3(256), 0, stream, batches, n, lo, sc, axis_bits, k0, i0);
     rocprim::double_buffer<uint32_t> kb(k0, k1), vb(i0, i1);
-    hipError_t e = rocprim::radix_sort_pairs(temp, tb, kb, vb, n, 0, 30, stream);
+    hipError_t e = rocprim::radix_sort_pairs(temp, tb, kb, vb, n, 0, (int)(3u * axis_bits + 3u), stream);
     if (e != hipSuccess) return e;
     *order_out = vb.current();
     return hipGetLastError();
