@@ -151,6 +151,13 @@ int lc_hip_stream_synchronize(uint64_t device, uint64_t stream);
 int lc_hip_stream_add_callback(uint64_t device, uint64_t stream, void (*fn)(void *), void *user);
 void *lc_hip_stream_native(uint64_t device, uint64_t stream);
 
+/* DeviceInterface::set_name (device_interface.h:189; tags of Resource::Tag, resource.h:179-200):
+   named meshes (3), procedural primitives (5), accels (7) and shaders (10) bracket their builds
+   and dispatches with roctx marker ranges "<name>::build" / "::update" / "::compact" /
+   "::dispatch" (rocprofv3 --marker-trace), as the CUDA backend does with NVTX
+   (cuda_accel.cpp:28,68,101); other resources ignore the name. */
+int lc_hip_set_name(uint64_t device, uint32_t resource_tag, uint64_t handle, const char *name);
+
 /* ---- stream capture into HIP graphs. The reference's CUDA backend encodes each CommandList it is
    given (CUDAStream::dispatch, cuda_stream.cpp:194-203); a sequence submitted many times is
    recorded once here and replayed. Between begin and end the stream's shader dispatches, traces and

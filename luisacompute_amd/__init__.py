@@ -208,6 +208,7 @@ _SIGNATURES = {
     "lc_hip_stream_synchronize": (ctypes.c_int, [_U64, _U64]),
     "lc_hip_stream_add_callback": (ctypes.c_int, [_U64, _U64, _P, _P]),
     "lc_hip_stream_native": (_P, [_U64, _U64]),
+    "lc_hip_set_name": (ctypes.c_int, [_U64, _U32, _U64, ctypes.c_char_p]),
     "lc_hip_stream_capture_begin": (ctypes.c_int, [_U64, _U64]),
     "lc_hip_stream_capture_end": (ctypes.c_int, [_U64, _U64, ctypes.POINTER(_U64)]),
     "lc_hip_graph_launch": (ctypes.c_int, [_U64, _U64, _U64]),
@@ -1221,6 +1222,22 @@ class Accel:
                                               ray_offset, n, vis_mask, 1 if any_hit else 0, ctypes.byref(a),
                                               ctypes.byref(b)), "trace_count")
         return a.value, b.value
+
+
+def _set_name_for(tag):
+    def set_name(self, name):
+        """Resource::set_name (DeviceInterface::set_name): the builds / dispatches of a named
+        resource are bracketed by marker ranges "<name>::build" etc. (rocprofv3 --marker-trace)."""
+        _check(lib().lc_hip_set_name(self.device.handle, tag, self.handle, str(name).encode()), "set_name")
+        return self
+    return set_name
+
+
+# Resource::Tag values (include/luisa/runtime/rhi/resource.h:179-200)
+Mesh.set_name = _set_name_for(3)
+ProceduralPrimitive.set_name = _set_name_for(5)
+Accel.set_name = _set_name_for(7)
+Shader.set_name = _set_name_for(10)
 
 
 # ------------------------------------------------------------------ render serving

@@ -367,6 +367,7 @@ struct HostCounters {
 
 struct Mesh {
     Device *dev = nullptr;
+    std::string name; // DeviceInterface::set_name: build ranges are marked when set
     uint32_t hint = 0;
     bool allow_compaction = true, allow_update = false;
     std::mutex mu;
@@ -411,6 +412,7 @@ struct InstanceHost {
 
 struct Accel {
     Device *dev = nullptr;
+    std::string name; // DeviceInterface::set_name: build ranges are marked when set
     uint32_t hint = 0;
     bool allow_compaction = true, allow_update = false;
     std::mutex mu;
@@ -477,6 +479,7 @@ struct Bindless {
 // ---- run-time shaders (hiprtc; shader.cpp) ----
 struct Shader {
     Device *dev = nullptr;
+    std::string name; // DeviceInterface::set_name: dispatch ranges are marked when set
     hipModule_t module = nullptr;
     hipFunction_t fn = nullptr;
     uint32_t block[3] = {128, 1, 1};

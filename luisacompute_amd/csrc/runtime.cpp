@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <rocprofiler-sdk-roctx/roctx.h>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -128,6 +129,23 @@ struct TimedRegion {
             hipEventRecord(b, s);
             dev->pending.push_back({name, a, b});
         }
+    }
+};
+
+// Host-side marker range (roctx, shown by rocprofv3 --marker-trace) around the work of a named
+// resource, as the CUDA backend brackets a named accel's build with NVTX ranges
+// (src/backends/cuda/cuda_accel.cpp:28,68,101). Nothing happens for unnamed resources.
+struct MarkerRange {
+    bool on = false;
+    MarkerRange(const std::string &name, const char *what) {
+        if (!name.empty()) {
+            const std::string label = name + "::" + what;
+            roctxRangePushA(label.c_str());
+            on = true;
+        }
+    }
+    ~MarkerRange() {
+        if (on) roctxRangePop();
     }
 };
 
@@ -261,6 +279,7 @@ hipError_t arena_acquire(Device *dev, Mesh *m, uint32_t n, BuildTarget &t) {
 // allow_update (triangle meshes) also the refit state: the build's counters (`ctr`, in the arena
 // scratch) into a per-primitive array with room for the nodes' exact boxes.
 int arena_compact(Device *dev, Mesh *m, hipStream_t s, const BuildCounters *ctr) {
+    MarkerRange mr(m->name, "compact");
     LC_HIP_TRY(hipStreamSynchronize(s));
     const uint32_t count = m->host->node_count;
     if (count == 0 || count > dev->arena_node_cap) return fail(LC_HIP_ERROR_DEVICE_FAULT, "BVH node count out of range");
@@ -647,6 +666,34 @@ int lc_hip_stream_add_callback(uint64_t device, uint64_t stream, void (*fn)(void
 
 void *lc_hip_stream_native(uint64_t, uint64_t stream) { return native_stream(stream); }
 
+// DeviceInterface::set_name (include/luisa/runtime/rhi/device_interface.h:189). Meshes, procedural
+// primitives, accels and shaders (Resource::Tag MESH 3, PROCEDURAL_PRIMITIVE 5, ACCEL 7, SHADER 10,
+// include/luisa/runtime/rhi/resource.h:179-200) keep the name and bracket their builds / dispatches
+// with marker ranges "<name>::build" / "::update" / "::compact" / "::dispatch"; other resources
+// accept it and ignore it, as backends without a naming facility do.
+int lc_hip_set_name(uint64_t device, uint32_t resource_tag, uint64_t handle, const char *name) {
+    Device *dev = obj<Device>(device);
+    if (!dev || !obj<char>(handle)) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad set_name handles");
+    const std::string n = name ? name : "";
+    switch (resource_tag) {
+        case 3u:
+        case 5u: {
+            Mesh *m = obj<Mesh>(handle);
+            std::lock_guard<std::mutex> lk(m->mu);
+            m->name = n;
+            return LC_HIP_OK;
+        }
+        case 7u: {
+            Accel *a = obj<Accel>(handle);
+            std::lock_guard<std::mutex> lk(a->mu);
+            a->name = n;
+            return LC_HIP_OK;
+        }
+        case 10u: obj<Shader>(handle)->name = n; return LC_HIP_OK;
+        default: return LC_HIP_OK;
+    }
+}
+
 // ---- stream capture into HIP graphs. The reference's CUDA backend encodes every CommandList it
 // is given again (CUDAStream::dispatch, src/backends/cuda/cuda_stream.cpp:194-203); a caller
 // that submits the same launches many times (a wavefront path tracer's per-sample sequence)
@@ -965,6 +1012,7 @@ int lc_hip_procedural_build(uint64_t device, uint64_t stream, uint64_t primitive
     m->build_epoch++;
     if (n == 0) return LC_HIP_OK;
     {
+        MarkerRange mr(m->name, "build");
         std::lock_guard<std::mutex> dl(dev->mu);
         BuildTarget t{m->nodes, m->scratch, m->scratch_bytes};
         if (m->allow_compaction) LC_HIP_TRY(arena_acquire(dev, m, n, t));
@@ -1026,6 +1074,7 @@ int lc_hip_mesh_build(uint64_t device, uint64_t stream, uint64_t mesh, uint32_t 
     m->t_addr = taddr;
     m->t_size = triangle_size;
     if (refit) {
+        MarkerRange mr(m->name, "update");
         const int rc = mesh_stats_sync(m); // the wide depth of the build being refitted
         if (rc != LC_HIP_OK) return rc;
         const uint32_t depth = m->stats.depth;
@@ -1071,6 +1120,7 @@ int lc_hip_mesh_build(uint64_t device, uint64_t stream, uint64_t mesh, uint32_t 
     m->build_epoch++;
     if (n == 0) return LC_HIP_OK;
     {
+        MarkerRange mr(m->name, "build");
         std::lock_guard<std::mutex> dl(dev->mu);
         BuildTarget t{m->nodes, m->scratch, m->scratch_bytes};
         if (m->allow_compaction) LC_HIP_TRY(arena_acquire(dev, m, n, t));
@@ -1265,6 +1315,7 @@ int lc_hip_accel_build(uint64_t device, uint64_t stream, uint64_t accel, uint32_
     if (!dev || !a) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad accel build handles");
     if (nmods > 0 && !mods) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "null modifications");
     std::lock_guard<std::mutex> lk(a->mu);
+    MarkerRange mr(a->name, update_instance_buffer_only ? "update" : "build");
     DeviceGuard g(dev);
     hipStream_t s = native_stream(stream);
     // instance writes issued on the device since the last build come first (cpu/accel.rs:335-343
@@ -2692,6 +2743,7 @@ static int shader_launch(uint64_t device, uint64_t stream, uint64_t shader, cons
         }
     }
     const size_t args_end = off;
+    MarkerRange mr(sh->name, "dispatch");
     DeviceGuard g(dev);
     hipStream_t s = native_stream(stream);
     for (size_t k = 0; k < count; k++) {
