@@ -265,13 +265,25 @@ int lc_hip_accel_trace_any_batched(uint64_t device, uint64_t stream, uint64_t ac
    `capacity` rays; the launch traces the first min(count, capacity), count being the uint32 at
    count_offset of count_buffer when the launch runs (a kernel dispatched over an
    IndirectDispatchBuffer calling accel.trace_closest / trace_any, include/luisa/runtime/
-   dispatch_buffer.h; cuda_device_resource.h:1783-1829). Rays are traced in queue order. */
+   dispatch_buffer.h; cuda_device_resource.h:1783-1829). With ray sorting on, the first
+   min(count, capacity) rays are sorted on the device like any trace queue. */
 int lc_hip_accel_trace_closest_indirect(uint64_t device, uint64_t stream, uint64_t accel, uint64_t ray_buffer,
                                         size_t ray_offset, uint64_t hit_buffer, size_t hit_offset, uint32_t capacity,
                                         uint64_t count_buffer, size_t count_offset, uint32_t vis_mask);
 int lc_hip_accel_trace_any_indirect(uint64_t device, uint64_t stream, uint64_t accel, uint64_t ray_buffer,
                                     size_t ray_offset, uint64_t hit_buffer, size_t hit_offset, uint32_t capacity,
                                     uint64_t count_buffer, size_t count_offset, uint32_t vis_mask);
+/* lcub DeviceRadixSort::SortPairs (include/luisa/backends/ext/cuda/lcub/device_radix_sort.h:15-27)
+   for uint32 keys and uint32 values: a stable sort of the pairs by key bits [begin_bit, end_bit).
+   The first call is the temp_storage_size query (num_items < 2^31). The sort reads keys_in and
+   values_in, writes keys_out and values_out, and uses temp_buffer (all distinct buffers);
+   values_in == 0 sorts the item indices 0..num_items-1. count_buffer != 0: only the first
+   min(count, num_items) items are sorted, count being the uint32 at count_offset when the sort
+   runs (later output words are left as they were). Allocates nothing; can be captured. */
+int lc_hip_radix_sort_pairs_temp_bytes(uint32_t num_items, size_t *temp_bytes);
+int lc_hip_radix_sort_pairs(uint64_t device, uint64_t stream, uint64_t temp_buffer, uint64_t keys_in,
+                            uint64_t keys_out, uint64_t values_in, uint64_t values_out, uint32_t num_items,
+                            int begin_bit, int end_bit, uint64_t count_buffer, size_t count_offset);
 /* Counter-mode run of the same traversal: total BVH nodes visited and triangles tested. */
 int lc_hip_accel_trace_count(uint64_t device, uint64_t stream, uint64_t accel, uint64_t ray_buffer, size_t ray_offset,
                              uint32_t ray_count, uint32_t vis_mask, int any, uint64_t *nodes_visited,

@@ -264,6 +264,9 @@ _SIGNATURES = {
     "lc_hip_accel_trace_any_indirect": (ctypes.c_int, [_U64, _U64, _U64, _U64, _SZ, _U64, _SZ, _U32, _U64, _SZ,
                                                        _U32]),
     "lc_hip_accel_trace_any": (ctypes.c_int, [_U64, _U64, _U64, _U64, _SZ, _U64, _SZ, _U32, _U32]),
+    "lc_hip_radix_sort_pairs_temp_bytes": (ctypes.c_int, [_U32, ctypes.POINTER(ctypes.c_size_t)]),
+    "lc_hip_radix_sort_pairs": (ctypes.c_int, [_U64, _U64, _U64, _U64, _U64, _U64, _U64, _U32, ctypes.c_int,
+                                               ctypes.c_int, _U64, _SZ]),
     "lc_hip_accel_trace_count": (ctypes.c_int, [_U64, _U64, _U64, _U64, _SZ, _U32, _U32, ctypes.c_int,
                                                 ctypes.POINTER(_U64), ctypes.POINTER(_U64)]),
     "lc_hip_accel_trace_steps": (ctypes.c_int, [_U64, _U64, _U64, _U64, _SZ, _U32, _U32, ctypes.c_int,
@@ -765,6 +768,30 @@ class Mesh:
         e = (ctypes.c_uint32 * 7)()
         _check(lib().lc_hip_mesh_validate(self.device.handle, stream.handle, self.handle, e), "mesh validate")
         return dict(zip(self.VALIDATION_CHECKS, e[:]))
+
+
+class DeviceRadixSort:
+    """lcub DeviceRadixSort (include/luisa/backends/ext/cuda/lcub/device_radix_sort.h:15-27) for
+    uint32 keys and values, on the library's own radix kernels."""
+
+    @staticmethod
+    def temp_storage_size(num_items):
+        b = ctypes.c_size_t()
+        _check(lib().lc_hip_radix_sort_pairs_temp_bytes(int(num_items), ctypes.byref(b)), "radix sort temp bytes")
+        return b.value
+
+    @staticmethod
+    def sort_pairs(stream, temp, keys_in, keys_out, values_in, values_out, num_items, begin_bit=0, end_bit=32,
+                   count_buffer=None, count_index=0):
+        """Stable sort of num_items (key, value) pairs by key bits [begin_bit, end_bit). values_in None
+        sorts the item indices. count_buffer: only the first min(count_buffer[count_index],
+        num_items) items are sorted, the count being read on the device when the sort runs."""
+        dev = stream.device
+        _check(lib().lc_hip_radix_sort_pairs(dev.handle, stream.handle, temp.handle, keys_in.handle, keys_out.handle,
+                                             values_in.handle if values_in is not None else 0, values_out.handle,
+                                             int(num_items), int(begin_bit), int(end_bit),
+                                             count_buffer.handle if count_buffer is not None else 0,
+                                             4 * int(count_index)), "radix sort")
 
 
 def shader_cache_stats():

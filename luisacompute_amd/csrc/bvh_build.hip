@@ -19,8 +19,6 @@
 // The same builder serves the TLAS (primitives = instance world boxes).
 #include <hip/hip_runtime.h>
 #include <cstring>
-#include <rocprim/device/device_radix_sort.hpp>
-#include <rocprim/device/device_scan.hpp>
 
 #include <cfloat>
 #include <cstdint>
@@ -200,7 +198,7 @@ __device__ __forceinline__ uint32_t expand10(uint32_t v) {
 // 30-bit Morton code of the box centroid in the centroid bounds; a triangle build also writes
 // each triangle's record in primitive order (prec), read back in sorted order by the leaf phase.
 __global__ __launch_bounds__(kBlock) void k_morton(PrimSource src, uint32_t n, const BuildCounters *ctr,
-                                                   uint32_t *keys, uint32_t *ids, TriRecord *prec) {
+                                                   uint32_t *keys, TriRecord *prec) {
     uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     float lo[3], sc[3];
@@ -221,7 +219,6 @@ __global__ __launch_bounds__(kBlock) void k_morton(PrimSource src, uint32_t n, c
         q[a] = (uint32_t)f;
     }
     keys[p] = expand10(q[0]) << 2 | expand10(q[1]) << 1 | expand10(q[2]);
-    ids[p] = p;
 }
 
 // Common-prefix length of sorted keys i and j (-1 outside [0, n)); equal keys continue with the
@@ -1287,21 +1284,6 @@ T *carve(char *&p, size_t count) {
     return r;
 }
 
-size_t sort_temp_bytes(uint32_t n) {
-    size_t bytes = 0;
-    rocprim::double_buffer<uint32_t> k(nullptr, nullptr);
-    rocprim::double_buffer<uint32_t> v(nullptr, nullptr);
-    rocprim::radix_sort_pairs(nullptr, bytes, k, v, n, 0, 30, (hipStream_t)0);
-    return bytes;
-}
-
-size_t scan_temp_bytes(uint32_t n) {
-    size_t bytes = 0;
-    rocprim::exclusive_scan(nullptr, bytes, (const uint32_t *)nullptr, (uint32_t *)nullptr, 0u, (size_t)n,
-                            rocprim::plus<uint32_t>(), (hipStream_t)0);
-    return bytes;
-}
-
 // Workgroups of a bottom-up phase (runs of >= kPhaseBlock entries or leaves).
 size_t phase_blocks(size_t m) { return m / kPhaseBlock + 1; }
 
@@ -1368,7 +1350,7 @@ struct ScratchLayout {
         boff = carve<uint32_t>(p, phase_blocks(m));
         ctr = carve<BuildCounters>(p, 1);
         dcount = carve<uint32_t>(p, 1);
-        temp_bytes = std::max(sort_temp_bytes((uint32_t)m), scan_temp_bytes((uint32_t)phase_blocks(m)));
+        temp_bytes = radix_sort_count_bytes((uint32_t)m);
         temp = carve<char>(p, temp_bytes);
     }
 };
@@ -1392,7 +1374,7 @@ size_t bvh_scratch_bytes(uint32_t n) {
     b += 2 * m * sizeof(uint2) + 256 * 2;           // collapse queues
     b += sizeof(BuildCounters) + 256;               // counters
     b += sizeof(uint32_t) + 256;                    // entry count
-    b += std::max(sort_temp_bytes((uint32_t)m), scan_temp_bytes((uint32_t)phase_blocks(m))) + 256; // rocPRIM temp
+    b += radix_sort_count_bytes((uint32_t)m) + 256; // digit counts of the key sort
     return b;
 }
 
@@ -1453,14 +1435,13 @@ hipError_t bvh_build(hipStream_t stream, void *scratch, size_t scratch_bytes, co
     const uint32_t grid = (n + kBlock - 1) / kBlock;
     hipLaunchKernelGGL(k_centroid_bounds, dim3(std::min<uint32_t>(grid, 2048)), dim3(kBlock), 0, stream, src, n, ctr);
     TriRecord *prec = boxes ? nullptr : L.prec;
-    hipLaunchKernelGGL(k_morton, dim3(grid), dim3(kBlock), 0, stream, src, n, ctr, k0, i0, prec);
-    rocprim::double_buffer<uint32_t> kb(k0, k1);
-    rocprim::double_buffer<uint32_t> vb(i0, i1);
-    size_t tb = temp_bytes;
-    hipError_t e = rocprim::radix_sort_pairs(temp, tb, kb, vb, n, 0, 30, stream);
+    hipLaunchKernelGGL(k_morton, dim3(grid), dim3(kBlock), 0, stream, src, n, ctr, k0, prec);
+    // stable sort of the primitive indices by key (k0 -> k1, ids -> i1); the sort's ping-pong
+    // buffers are i0 and the first entry list, which the bottom-up phases fill later
+    hipError_t e = radix_sort_pairs_with(stream, temp, temp_bytes, ent[0], i0, k0, k1, nullptr, i1, n, nullptr, 0u, 30u);
     if (e != hipSuccess) return e;
-    const uint32_t *keys = kb.current();
-    uint32_t *ids = vb.current();
+    const uint32_t *keys = k1;
+    uint32_t *ids = i1;
     // SAH refinement of the radix tree's clusters (below); needs at least one split to choose
     const bool sah = sah_refine != 0u && n > 2u;
     // leaf boxes: triangle builds keep their records in sorted order (written by the leaf phase,
@@ -1487,9 +1468,7 @@ hipError_t bvh_build(hipStream_t stream, void *scratch, size_t scratch_bytes, co
             // then start from the cluster roots
             const uint32_t nblocks = (n + kPhaseBlock - 1) / kPhaseBlock;
             hipLaunchKernelGGL(k_sah_cut, dim3(nblocks), dim3(kPhaseBlock), 0, stream, n, bt, pend, L.bcount);
-            tb = temp_bytes;
-            e = rocprim::exclusive_scan(temp, tb, (const uint32_t *)L.bcount, L.boff, 0u, (size_t)nblocks,
-                                        rocprim::plus<uint32_t>(), stream);
+            e = exclusive_scan_u32(stream, L.bcount, L.boff, nblocks);
             if (e != hipSuccess) return e;
             hipLaunchKernelGGL(k_gather_deferred, dim3(nblocks), dim3(kPhaseBlock), 0, stream, (const uint32_t *)pend,
                                (const uint32_t *)L.bcount, (const uint32_t *)L.boff, kPhaseBlock, nblocks, ent[which],
@@ -1528,9 +1507,7 @@ hipError_t bvh_build(hipStream_t stream, void *scratch, size_t scratch_bytes, co
             else
                 hipLaunchKernelGGL(k_bounds_phase<false>, dim3(nblocks), dim3(kPhaseBlock), 0, stream, lb, n, bt,
                                    entries, count, pend, L.bcount, ctr, q0, 0u);
-            tb = temp_bytes;
-            e = rocprim::exclusive_scan(temp, tb, (const uint32_t *)L.bcount, L.boff, 0u, (size_t)nblocks,
-                                        rocprim::plus<uint32_t>(), stream);
+            e = exclusive_scan_u32(stream, L.bcount, L.boff, nblocks);
             if (e != hipSuccess) return e;
             hipLaunchKernelGGL(k_gather_deferred, dim3(nblocks), dim3(run), 0, stream, (const uint32_t *)pend,
                                (const uint32_t *)L.bcount, (const uint32_t *)L.boff, run, nblocks, ent[which], dcount);

@@ -91,10 +91,29 @@ struct TraceParams {
                              // hold a ray (0: refill only after `steps` steps)
 };
 
+// ---- kernels (radix.hip) ----
+// Stable sort of (key, value) pairs by key bits [begin_bit, end_bit); `vals_in` null sorts the
+// identity (item indices). Items are the first min(*count_ptr, cap) when count_ptr is given.
+// keys_in is left unchanged and must not alias keys_out; cap < 2^31.
+size_t radix_sort_temp_bytes(uint32_t cap);
+hipError_t radix_sort_pairs(hipStream_t stream, void *temp, size_t temp_bytes, const uint32_t *keys_in,
+                            uint32_t *keys_out, const uint32_t *vals_in, uint32_t *vals_out, uint32_t cap,
+                            const uint32_t *count_ptr, uint32_t begin_bit, uint32_t end_bit);
+// The same with the ping-pong buffers given (cap words each, distinct from the inputs and outputs);
+// count_temp holds radix_sort_count_bytes(cap) bytes of digit counts.
+size_t radix_sort_count_bytes(uint32_t cap);
+hipError_t radix_sort_pairs_with(hipStream_t stream, void *count_temp, size_t count_bytes, uint32_t *alt_keys,
+                                 uint32_t *alt_vals, const uint32_t *keys_in, uint32_t *keys_out,
+                                 const uint32_t *vals_in, uint32_t *vals_out, uint32_t cap, const uint32_t *count_ptr,
+                                 uint32_t begin_bit, uint32_t end_bit);
+// out[i] = in[0] + ... + in[i-1] (one workgroup); in and out may alias.
+hipError_t exclusive_scan_u32(hipStream_t stream, const uint32_t *in, uint32_t *out, uint32_t n);
+
 // ---- kernels (raysort.hip) ----
+// count_ptr (device word, may be null): only the first min(*count_ptr, n) rays are sorted.
 size_t ray_sort_scratch_bytes(uint32_t n);
-hipError_t ray_sort(hipStream_t stream, const BatchTable &batches, uint32_t n, const float bounds[6], void *scratch,
-                    size_t scratch_bytes, const uint32_t **order_out);
+hipError_t ray_sort(hipStream_t stream, const BatchTable &batches, uint32_t n, const uint32_t *count_ptr,
+                    const float bounds[6], void *scratch, size_t scratch_bytes, const uint32_t **order_out);
 
 // Queue index -> batch (batches ordered by start).
 __host__ __device__ inline uint32_t find_batch(const BatchTable &t, uint32_t q) {

@@ -6,13 +6,14 @@
 // waves descend the same subtrees, so most of those fetches hit in L2. The key is the Morton code
 // of the origin quantised in the scene box (7 bits per axis by default: 21 bits, three 8-bit radix
 // passes with the octant), followed by the 3-bit direction octant; the queue is the stable
-// radix-sorted permutation of ray indices. Results are still written to each ray's own slot.
+// radix-sorted permutation of ray indices (the library's own radix sort, radix.hip, which also
+// sorts queues whose length a kernel wrote to device memory, and inside graph captures). Results
+// are still written to each ray's own slot.
 // Measured on the 16.7M-triangle soup: 9 bits per axis (four passes) gives the same traversal time
 // and a 0.07 ms slower sort; octant-major keys make the traversal 30% slower.
 #include <hip/hip_runtime.h>
 #include <cstdlib>
 #include <cstring>
-#include <rocprim/device/device_radix_sort.hpp>
 
 #include <cstdint>
 
@@ -31,10 +32,10 @@ __device__ __forceinline__ uint32_t expand9(uint32_t v) {
     return v;
 }
 
-__global__ __launch_bounds__(256) void k_ray_keys(const BatchTable batches, uint32_t n, float3 lo, float3 scale,
-                                                  uint32_t axis_bits, uint32_t *keys, uint32_t *idx) {
+__global__ __launch_bounds__(256) void k_ray_keys(const BatchTable batches, uint32_t n, const uint32_t *count_ptr,
+                                                  float3 lo, float3 scale, uint32_t axis_bits, uint32_t *keys) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= (count_ptr ? min(*count_ptr, n) : n)) return;
     const uint32_t b = find_batch(batches, i);
     const float4 *rays = batches.b[b].rays;
     const uint32_t r = i - batches.b[b].start;
@@ -45,33 +46,24 @@ __global__ __launch_bounds__(256) void k_ray_keys(const BatchTable batches, uint
     const uint32_t oct = (d.x < 0.0f ? 1u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 4u : 0u);
     const uint32_t m = expand9(c[0]) << 2 | expand9(c[1]) << 1 | expand9(c[2]); // 27-bit Morton code
     keys[i] = ((m >> (27u - 3u * axis_bits)) << 3) | oct;
-    idx[i] = i;
-}
-
-size_t sort_temp(uint32_t n) {
-    size_t bytes = 0;
-    rocprim::double_buffer<uint32_t> k(nullptr, nullptr), v(nullptr, nullptr);
-    rocprim::radix_sort_pairs(nullptr, bytes, k, v, n, 0, 30, (hipStream_t)0);
-    return bytes;
 }
 
 size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
 } // namespace
 
-size_t ray_sort_scratch_bytes(uint32_t n) { return 4 * align256((size_t)n * 4) + align256(sort_temp(n)); }
+size_t ray_sort_scratch_bytes(uint32_t n) { return 3 * align256((size_t)n * 4) + radix_sort_temp_bytes(n); }
 
 // Sorts ray indices by (origin Morton code in [lo, hi], direction octant). Returns the order
-// (n indices) inside `scratch`.
-hipError_t ray_sort(hipStream_t stream, const BatchTable &batches, uint32_t n, const float bounds[6], void *scratch,
-                    size_t scratch_bytes, const uint32_t **order_out) {
+// (n indices, or the first *count_ptr when count_ptr is given) inside `scratch`.
+hipError_t ray_sort(hipStream_t stream, const BatchTable &batches, uint32_t n, const uint32_t *count_ptr,
+                    const float bounds[6], void *scratch, size_t scratch_bytes, const uint32_t **order_out) {
     if (scratch_bytes < ray_sort_scratch_bytes(n)) return hipErrorInvalidValue;
     char *p = static_cast<char *>(scratch);
     const size_t a = align256((size_t)n * 4);
-    uint32_t *k0 = reinterpret_cast<uint32_t *>(p), *k1 = reinterpret_cast<uint32_t *>(p + a);
-    uint32_t *i0 = reinterpret_cast<uint32_t *>(p + 2 * a), *i1 = reinterpret_cast<uint32_t *>(p + 3 * a);
-    void *temp = p + 4 * a;
-    size_t tb = sort_temp(n);
+    uint32_t *keys = reinterpret_cast<uint32_t *>(p), *sorted = reinterpret_cast<uint32_t *>(p + a);
+    uint32_t *order = reinterpret_cast<uint32_t *>(p + 2 * a);
+    void *temp = p + 3 * a;
     float3 lo = make_float3(bounds[0], bounds[1], bounds[2]);
     float3 sc;
     float ext[3] = {bounds[3] - bounds[0], bounds[4] - bounds[1], bounds[5] - bounds[2]};
@@ -84,11 +76,12 @@ hipError_t ray_sort(hipStream_t stream, const BatchTable &batches, uint32_t n, c
         const int b = v ? std::atoi(v) : 7;
         return (uint32_t)(b >= 1 && b <= 9 ? b : 7);
     }();
-    hipLaunchKernelGGL(k_ray_keys, dim3((n + 255) / 256), dim3(256), 0, stream, batches, n, lo, sc, axis_bits, k0, i0);
-    rocprim::double_buffer<uint32_t> kb(k0, k1), vb(i0, i1);
-    hipError_t e = rocprim::radix_sort_pairs(temp, tb, kb, vb, n, 0, (int)(3u * axis_bits + 3u), stream);
+    hipLaunchKernelGGL(k_ray_keys, dim3((n + 255) / 256), dim3(256), 0, stream, batches, n, count_ptr, lo, sc,
+                       axis_bits, keys);
+    hipError_t e = radix_sort_pairs(stream, temp, radix_sort_temp_bytes(n), keys, sorted, nullptr, order, n, count_ptr,
+                                    0u, 3u * axis_bits + 3u);
     if (e != hipSuccess) return e;
-    *order_out = vb.current();
+    *order_out = order;
     return hipGetLastError();
 }
 

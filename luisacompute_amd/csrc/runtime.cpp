@@ -1710,16 +1710,17 @@ static int trace_launch_batches(Device *dev, Accel *a, hipStream_t s, const lc_h
     if (counting) LC_HIP_TRY(hipMemsetAsync(S->stats, 0, 5 * sizeof(unsigned long long), s));
     // per-ray results do not depend on the order; the counting form traverses the same sorted
     // queue as the timed form, so its node / triangle totals describe that launch. Queues counted
-    // on the device are traced in the order their producer wrote them, and so are captured
-    // launches (the radix sort clears its digit counts with memset nodes, which graphs here do
-    // not replay reliably)
-    const bool sort = cfg.sort_rays && total > 1 && !P.geo.empty && !count_ptr && !capturing(s);
+    // on the device sort their first *count_ptr rays (the sort reads the count when it runs). A
+    // captured launch sorts when the stream already holds the sort's working memory (captures
+    // cannot allocate) and otherwise traces the queue in its own order.
+    bool sort = cfg.sort_rays && total > 1 && !P.geo.empty;
+    if (sort && capturing(s) && S->sort_bytes < ray_sort_scratch_bytes(P.count)) sort = false;
     if (sort) LC_HIP_GROW((S, s, S->sort, S->sort_bytes, ray_sort_scratch_bytes(P.count)));
     {
         // "trace_*": the whole entry point (sort + traversal); "trace_*_kernel": traversal only
         TimedRegion tr(dev, s, counting ? "trace_count" : (any ? "trace_any" : "trace_closest"));
         if (sort) {
-            hipError_t e = ray_sort(s, P.batches, P.count, a->world_bounds, S->sort, S->sort_bytes, &P.order);
+            hipError_t e = ray_sort(s, P.batches, P.count, count_ptr, a->world_bounds, S->sort, S->sort_bytes, &P.order);
             if (e != hipSuccess) return hip_fail(e, "ray sort");
         }
         TimedRegion tk(dev, s, counting ? "trace_count_kernel" : (any ? "trace_any_kernel" : "trace_closest_kernel"));
@@ -1846,6 +1847,53 @@ int lc_hip_accel_trace_any_indirect(uint64_t device, uint64_t stream, uint64_t a
                                     uint64_t count_buffer, size_t count_offset, uint32_t vis_mask) {
     return trace_indirect(device, stream, accel, ray_buffer, ray_offset, hit_buffer, hit_offset, capacity, count_buffer,
                           count_offset, vis_mask, true);
+}
+
+// DeviceRadixSort::SortPairs of lcub (include/luisa/backends/ext/cuda/lcub/device_radix_sort.h:15-27)
+// for uint32 keys and values: the temp_storage_size query and the sort command, on the library's
+// own radix kernels (radix.hip). The command allocates nothing and records into a graph.
+int lc_hip_radix_sort_pairs_temp_bytes(uint32_t num_items, size_t *temp_bytes) {
+    if (!temp_bytes) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "null temp_bytes");
+    if (num_items > 0x7FFFFFFFu) return fail(LC_HIP_ERROR_UNSUPPORTED, "more than 2^31 - 1 items");
+    *temp_bytes = radix_sort_temp_bytes(num_items);
+    return LC_HIP_OK;
+}
+
+int lc_hip_radix_sort_pairs(uint64_t device, uint64_t stream, uint64_t temp_buffer, uint64_t keys_in,
+                            uint64_t keys_out, uint64_t values_in, uint64_t values_out, uint32_t num_items,
+                            int begin_bit, int end_bit, uint64_t count_buffer, size_t count_offset) {
+    Device *dev = obj<Device>(device);
+    Buffer *tb = obj<Buffer>(temp_buffer), *ki = obj<Buffer>(keys_in), *ko = obj<Buffer>(keys_out);
+    Buffer *vi = obj<Buffer>(values_in), *vo = obj<Buffer>(values_out);
+    if (!dev || !tb || !ki || !ko || !vo || (values_in && !vi)) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad sort handles");
+    if (begin_bit < 0 || end_bit > 32 || begin_bit >= end_bit)
+        return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "bits must satisfy 0 <= begin_bit < end_bit <= 32");
+    if (num_items > 0x7FFFFFFFu) return fail(LC_HIP_ERROR_UNSUPPORTED, "more than 2^31 - 1 items");
+    const size_t bytes = (size_t)num_items * 4;
+    if (ki->bytes < bytes || ko->bytes < bytes || vo->bytes < bytes || (vi && vi->bytes < bytes))
+        return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "a key or value buffer holds fewer than num_items words");
+    if (tb->bytes < radix_sort_temp_bytes(num_items))
+        return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "temp buffer smaller than lc_hip_radix_sort_pairs_temp_bytes");
+    if (ki == ko || (vi && vi == vo) || tb == ki || tb == ko || tb == vo || (vi && tb == vi) || ko == vo)
+        return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "inputs, outputs and temp storage must be distinct buffers");
+    const uint32_t *count_ptr = nullptr;
+    if (count_buffer) {
+        Buffer *cb = obj<Buffer>(count_buffer);
+        if (!cb) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad count buffer");
+        if (count_offset % 4 != 0 || count_offset + 4 > cb->bytes)
+            return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "count outside the count buffer (uint32, 4-byte aligned)");
+        count_ptr = reinterpret_cast<const uint32_t *>(static_cast<const char *>(cb->ptr) + count_offset);
+    }
+    if (num_items == 0) return LC_HIP_OK;
+    DeviceGuard g(dev);
+    hipStream_t s = native_stream(stream);
+    TimedRegion tr(dev, s, "radix_sort_pairs");
+    hipError_t e = radix_sort_pairs(s, tb->ptr, tb->bytes, static_cast<const uint32_t *>(ki->ptr),
+                                    static_cast<uint32_t *>(ko->ptr), vi ? static_cast<const uint32_t *>(vi->ptr) : nullptr,
+                                    static_cast<uint32_t *>(vo->ptr), num_items, count_ptr, (uint32_t)begin_bit,
+                                    (uint32_t)end_bit);
+    if (e != hipSuccess) return hip_fail(e, "radix sort");
+    return LC_HIP_OK;
 }
 
 // Closest hits without allocating: the working memory that lc_hip_accel_trace_closest takes from
@@ -1979,8 +2027,9 @@ int lc_hip_accel_query(uint64_t device, uint64_t stream, uint64_t accel, uint64_
     P.error = S->error;
     P.counter = S->ray_counter;
     // the query queue is sorted like trace queues when ray sorting is on (per-ray answers of the
-    // nearest-commit query do not depend on the order); not under capture, as for traces
-    const bool sort = dev->cfg[P.first ? 1 : 0].sort_rays && ray_count > 1 && !P.geo.empty && !capturing(s);
+    // nearest-commit query do not depend on the order); under capture as for traces
+    bool sort = dev->cfg[P.first ? 1 : 0].sort_rays && ray_count > 1 && !P.geo.empty;
+    if (sort && capturing(s) && S->sort_bytes < ray_sort_scratch_bytes(ray_count)) sort = false;
     if (sort) LC_HIP_GROW((S, s, S->sort, S->sort_bytes, ray_sort_scratch_bytes(ray_count)));
     TimedRegion tr(dev, s, P.first ? "query_any" : "query");
     if (sort) {
@@ -1989,7 +2038,7 @@ int lc_hip_accel_query(uint64_t device, uint64_t stream, uint64_t accel, uint64_
         bt.n = 1;
         bt.b[0].rays = P.rays;
         bt.b[0].mask = vis_mask;
-        hipError_t e = ray_sort(s, bt, ray_count, a->world_bounds, S->sort, S->sort_bytes, &P.order);
+        hipError_t e = ray_sort(s, bt, ray_count, nullptr, a->world_bounds, S->sort, S->sort_bytes, &P.order);
         if (e != hipSuccess) return hip_fail(e, "query ray sort");
     }
     hipError_t e = query_launch(P, s, grid);
