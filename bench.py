@@ -152,24 +152,31 @@ def bench_pathtracer(a):
     # the wavefront form written in the DSL (camera / shade / resolve kernels over queues, the
     # library's persistent trace on the device-counted queues; one sample's launches captured
     # into a graph and replayed per sample): same image as the DSL megakernel
-    dwf_ms = dwf_same = None
+    # (queues traced in the order the shade kernel appended them, and sorted on the device by
+    # the library's radix sort before each trace)
+    dwf_ms = dwf_same = dwfs_ms = dwfs_same = None
     if not a.pt_no_dsl_wavefront:
         from luisacompute_amd import programs
         bind = programs.PathTracerBindings(dev, s, accel, inst, scene)
-        wfd = programs.WavefrontPathTracer(dev, s, bind, w, h)
-        wimg = dev.create_buffer(16, w * h)
-        wfd.render(s, wimg, spp, sequence=3 << 32)  # first render: eager sample + capture
-        s.synchronize()
-        reps = max(1, a.steps // 2)
-        t0 = time.perf_counter()
-        for _ in range(reps):
-            wfd.render(s, wimg, spp, sequence=3 << 32)
-        s.synchronize()
-        dwf_ms = (time.perf_counter() - t0) / reps * 1e3
-        wimage = np.zeros((h, w, 4), np.float32)
-        s << wimg.copy_to(wimage) << lc.synchronize()
-        dwf_same = bool((wimage.view(np.uint32) == image.view(np.uint32)).all())
-        del wfd
+        for sort_queues in (False, True):
+            wfd = programs.WavefrontPathTracer(dev, s, bind, w, h, sort_queues=sort_queues)
+            wimg = dev.create_buffer(16, w * h)
+            wfd.render(s, wimg, spp, sequence=3 << 32)  # first render: eager sample + capture
+            s.synchronize()
+            reps = max(1, a.steps // 2)
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                wfd.render(s, wimg, spp, sequence=3 << 32)
+            s.synchronize()
+            ms = (time.perf_counter() - t0) / reps * 1e3
+            wimage = np.zeros((h, w, 4), np.float32)
+            s << wimg.copy_to(wimage) << lc.synchronize()
+            same = bool((wimage.view(np.uint32) == image.view(np.uint32)).all())
+            if sort_queues:
+                dwfs_ms, dwfs_same = ms, same
+            else:
+                dwf_ms, dwf_same = ms, same
+            del wfd
     out = {
         "metric": "Mrays/s path tracing, box room 1920x1080 64 spp depth 8",
         "value": round(rays / (wf_ms * 1e-3) / 1e6, 2), "unit": "Mrays/s", "n_gpus": 1, "steps": a.steps,
@@ -188,6 +195,8 @@ def bench_pathtracer(a):
         "dsl_wavefront_mrays_s": dwf_ms and round(rays / (dwf_ms * 1e-3) / 1e6, 2),
         "dsl_wavefront_over_dsl_megakernel": (dwf_ms and dsl_ms) and round(dsl_ms / dwf_ms, 3),
         "dsl_wavefront_same_image": dwf_same,
+        "dsl_wavefront_sorted_queues_ms": dwfs_ms and round(dwfs_ms, 3),
+        "dsl_wavefront_sorted_queues_same_image": dwfs_same,
         "image_mean": float(image[..., :3].mean()),
         "config": {"workload": f"box room {room.triangle_count} tris, {w}x{h}, {spp} spp, depth 8, RR from 3, "
                                f"chunk_spp {a.pt_chunk or 'auto'}, sort {a.pt_sort}, fetch {a.pt_fetch}, hipGraph replay"},

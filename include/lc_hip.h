@@ -598,8 +598,10 @@ int lc_hip_accel_stats(uint64_t device, uint64_t accel, lc_hip_bvh_stats *out);
 int lc_hip_trace_set_config(uint64_t device, int kind, int block, int lds_stack, int steps, int blocks_per_cu);
 int lc_hip_trace_get_config(uint64_t device, int kind, int *block, int *lds_stack, int *steps, int *blocks_per_cu);
 /* 1 = traverse rays in the order of a sort by (origin Morton code in the scene box, direction
-   octant) so neighbouring lanes share subtrees; results still land in each ray's own slot. */
+   octant) so neighbouring lanes share subtrees; results still land in each ray's own slot.
+   Applies to plain, batched, device-counted (indirect) and captured traces and to ray queries. */
 int lc_hip_trace_set_ray_sort(uint64_t device, int kind, int enable);
+int lc_hip_trace_get_ray_sort(uint64_t device, int kind, int *enable);
 /* How a persistent trace launch hands out queue slots: a wave claims `chunk` slots per atomic
    (0: exactly what its idle lanes need, one atomic per refill round) from one queue counter
    (segments 1) or from a segment of the queue per XCD (segments 8). */

@@ -280,6 +280,7 @@ _SIGNATURES = {
     "lc_hip_trace_set_config": (ctypes.c_int, [_U64, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "lc_hip_trace_get_config": (ctypes.c_int, [_U64, ctypes.c_int] + [ctypes.POINTER(ctypes.c_int)] * 4),
     "lc_hip_trace_set_ray_sort": (ctypes.c_int, [_U64, ctypes.c_int, ctypes.c_int]),
+    "lc_hip_trace_get_ray_sort": (ctypes.c_int, [_U64, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
     "lc_hip_trace_set_fetch": (ctypes.c_int, [_U64, ctypes.c_int, _U32, _U32]),
     "lc_hip_trace_get_fetch": (ctypes.c_int, [_U64, ctypes.c_int, ctypes.POINTER(_U32), ctypes.POINTER(_U32)]),
     "lc_hip_trace_set_steps": (ctypes.c_int, [_U64, ctypes.c_int, _U32, _U32]),
@@ -518,6 +519,11 @@ class Device:
     def set_ray_sort(self, kind, enable):
         """Traverse rays in sorted-queue order (origin Morton code, direction octant)."""
         _check(lib().lc_hip_trace_set_ray_sort(self.handle, 1 if kind == "any" else 0, 1 if enable else 0), "ray sort")
+
+    def ray_sort(self, kind):
+        v = ctypes.c_int()
+        _check(lib().lc_hip_trace_get_ray_sort(self.handle, 1 if kind == "any" else 0, ctypes.byref(v)), "ray sort")
+        return bool(v.value)
 
     def set_fetch(self, kind, chunk, segments):
         """Queue-slot hand-out of persistent trace launches (kind 0 closest, 1 any)."""

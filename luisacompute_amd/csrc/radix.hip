@@ -85,8 +85,8 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t &t
     return before + incl - v;
 }
 
-// Tile layout: wave w of tile t holds items t*kRsTile + w*64*kRsItems + j*64 + lane, j < kRsItems,
-// so the order (wave, j, lane) is the input order.
+// Tile layout of the ranking: wave w of tile t holds items t*kRsTile + w*64*kRsItems + j*64 + lane,
+// j < kRsItems, so the order (wave, j, lane) is the input order.
 __global__ __launch_bounds__(kRsBlock) void k_rs_upsweep(const uint32_t *keys, uint32_t cap, const uint32_t *count_ptr,
                                                          uint32_t shift, uint32_t mask, uint32_t n_tiles,
                                                          uint32_t *counts) {
@@ -98,19 +98,28 @@ __global__ __launch_bounds__(kRsBlock) void k_rs_upsweep(const uint32_t *keys, u
     __syncthreads();
     const uint32_t tile0 = blockIdx.x * kRsTile;
     if (tile0 < n) {
-        const uint32_t base = tile0 + w * 64u * kRsItems + lane;
+        // counts do not depend on the order: 16-B loads, thread t holding words 4 (t + 256 k) + e
         uint32_t d[kRsItems];
 #pragma unroll
-        for (int j = 0; j < kRsItems; j++) {
-            const uint32_t i = base + j * 64u;
-            d[j] = i < n ? (keys[i] >> shift) & mask : 0xFFFFFFFFu;
+        for (int k = 0; k < kRsItems / 4; k++) {
+            const uint32_t i = tile0 + 4u * (t + kRsBlock * k);
+            uint32_t v[4];
+            if (i + 3u < n) {
+                const uint4 x = *reinterpret_cast<const uint4 *>(keys + i);
+                v[0] = x.x, v[1] = x.y, v[2] = x.z, v[3] = x.w;
+            } else {
+#pragma unroll
+                for (int e = 0; e < 4; e++) v[e] = i + e < n ? keys[i + e] : 0u;
+            }
+#pragma unroll
+            for (int e = 0; e < 4; e++) d[4 * k + e] = i + e < n ? (v[e] >> shift) & mask : 0xFFFFFFFFu;
         }
 #pragma unroll
         for (int j = 0; j < kRsItems; j++) {
             const bool valid = d[j] != 0xFFFFFFFFu;
-            if (__ballot(valid) == 0) break; // items are in index order: the rest are past the count
+            if (__ballot(valid) == 0) continue; // past the count
             const uint32_t first = __builtin_amdgcn_readfirstlane(d[j]);
-            if (__ballot(d[j] == first) == ~0ull) { // whole wave valid, one digit (sorted runs, equal keys)
+            if (first != 0xFFFFFFFFu && __ballot(d[j] == first) == ~0ull) { // whole wave valid, one digit
                 if (lane == 0) s_hist[w][first] += 64u;
             } else if (valid) {
                 atomicAdd(&s_hist[w][d[j]], 1u);
@@ -295,6 +304,10 @@ hipError_t radix_sort_pairs_with(hipStream_t stream, void *count_temp, size_t co
     if (cap == 0) return hipSuccess;
     if (!count_temp || count_bytes < radix_sort_count_bytes(cap) || !alt_k || !alt_v || !keys_in || !keys_out ||
         !vals_out)
+        return hipErrorInvalidValue;
+    // the digit counts read the keys 16 B at a time
+    if ((reinterpret_cast<uintptr_t>(keys_in) | reinterpret_cast<uintptr_t>(keys_out) |
+         reinterpret_cast<uintptr_t>(alt_k)) % 16 != 0)
         return hipErrorInvalidValue;
     const uint32_t nt = rs_tiles(cap);
     uint32_t *counts = static_cast<uint32_t *>(count_temp);

@@ -2371,6 +2371,14 @@ int lc_hip_trace_set_ray_sort(uint64_t device, int kind, int enable) {
     return LC_HIP_OK;
 }
 
+int lc_hip_trace_get_ray_sort(uint64_t device, int kind, int *enable) {
+    Device *dev = obj<Device>(device);
+    if (!dev || kind < 0 || kind > 1 || !enable) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "bad trace config target");
+    std::lock_guard<std::mutex> lk(dev->mu);
+    *enable = dev->cfg[kind].sort_rays ? 1 : 0;
+    return LC_HIP_OK;
+}
+
 int lc_hip_build_set_width(uint64_t device, int width) {
     Device *dev = obj<Device>(device);
     if (!dev) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad device");

@@ -603,12 +603,15 @@ class WavefrontPathTracer:
     device-counted queues. With graph=True a wave's launch sequence (camera, max_depth x {reset,
     closest hits, shade, shadow any-hits, resolve}, accumulate) runs once eagerly, is then captured
     into a graph (Stream.capture) and replayed for the following waves; the first sample of a wave
-    advances in device memory, so every replay is the same launch sequence."""
+    advances in device memory, so every replay is the same launch sequence. sort_queues: the
+    traces sort their queues (origin cell, direction octant; the library's radix sort, which
+    reads the device-side count) before traversal; the image is the same either way."""
 
     KERNELS = ("wf_clear", "wf_camera", "wf_reset", "wf_shade", "wf_resolve", "wf_accumulate", "wf_finish")
 
-    def __init__(self, device, stream, bindings, width, height, samples_per_wave=None):
+    def __init__(self, device, stream, bindings, width, height, samples_per_wave=None, sort_queues=False):
         self.device, self.b = device, bindings
+        self.sort_queues = bool(sort_queues)
         self.width, self.height = width, height
         n = self.n = width * height
         k = samples_per_wave or max(1, min(64, (32 << 20) // n))
@@ -641,6 +644,18 @@ class WavefrontPathTracer:
         cmds = [lambda s, c=cam: s << c]
         # the callables hold the buffers themselves, not self (no reference cycle through self)
         hits, ctr, srays, occ = self.hits, self.counters, self.shadow_rays, self.occluded
+        dev, sort = self.device, self.sort_queues
+
+        def traced(kind, fn):
+            def call(s):
+                was = dev.ray_sort(kind)
+                dev.set_ray_sort(kind, sort)
+                try:
+                    fn(s)
+                finally:
+                    dev.set_ray_sort(kind, was)
+            return call
+
         # wf_shade's workgroups all reach its barriers: whole workgroups, the tail idle
         shade_threads = -(-m // 512) * 512
         for depth in range(max_depth):
@@ -654,10 +669,10 @@ class WavefrontPathTracer:
             resolve = k["wf_resolve"](self.shadow_paths, self.shadow_contrib, self.occluded, self.radiance,
                                       self.counters).dispatch(m)
             cmds += [lambda s, c=reset: s << c,
-                     lambda s, r=self.rays[qi], q=qi: accel.trace_closest_indirect(s, r, hits, ctr, 32 * q,
-                                                                                   capacity=m),
+                     traced("closest", lambda s, r=self.rays[qi], q=qi: accel.trace_closest_indirect(
+                         s, r, hits, ctr, 32 * q, capacity=m)),
                      lambda s, c=shade: s << c,
-                     lambda s: accel.trace_any_indirect(s, srays, occ, ctr, 64, capacity=m),
+                     traced("any", lambda s: accel.trace_any_indirect(s, srays, occ, ctr, 64, capacity=m)),
                      lambda s, c=resolve: s << c]
         acc = k["wf_accumulate"](self.radiance, self.total, self.frame, n, samples).dispatch(n)
         cmds.append(lambda s, c=acc: s << c)

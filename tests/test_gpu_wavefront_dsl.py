@@ -114,12 +114,13 @@ def _room(device, stream):
     return room, accel, meshes, inst
 
 
-@pytest.mark.parametrize("w,h,spp,max_depth,per_wave", [(64, 48, 4, 8, None), (64, 48, 4, 8, 1),
-                                                         (200, 120, 5, 5, 2)])
-def test_dsl_wavefront_equals_dsl_megakernel(device, w, h, spp, max_depth, per_wave):
+@pytest.mark.parametrize("w,h,spp,max_depth,per_wave,sort", [(64, 48, 4, 8, None, False), (64, 48, 4, 8, 1, True),
+                                                              (200, 120, 5, 5, 2, True), (200, 120, 5, 5, 2, False)])
+def test_dsl_wavefront_equals_dsl_megakernel(device, w, h, spp, max_depth, per_wave, sort):
     """The wavefront form renders the megakernel's image bit for bit: eagerly, and with a wave's
     launches captured once and replayed (renders of one wave, of one sample per wave, and of
-    two-sample waves with a one-sample remainder; eager work between the renders)."""
+    two-sample waves with a one-sample remainder; eager work between the renders), with the
+    traces' queues sorted on the device or traced in queue order."""
     s = device.create_stream()
     room, accel, meshes, inst = _room(device, s)
     bind = programs.PathTracerBindings(device, s, accel, inst, room.scene())
@@ -130,15 +131,17 @@ def test_dsl_wavefront_equals_dsl_megakernel(device, w, h, spp, max_depth, per_w
     want = np.zeros((h, w, 4), f32)
     s << img.copy_to(want) << lc.synchronize()
     assert want[..., :3].mean() > 0.01
-    wf = programs.WavefrontPathTracer(device, s, bind, w, h, samples_per_wave=per_wave)
+    wf = programs.WavefrontPathTracer(device, s, bind, w, h, samples_per_wave=per_wave, sort_queues=sort)
+    before = (device.ray_sort("closest"), device.ray_sort("any"))
     for label, graph in (("eager", False), ("graph", True), ("graph, replay", True)):
-        print(f"[dsl wavefront] {w}x{h} spp={spp} depth={max_depth} per wave {wf.samples_per_wave} {label}",
-              flush=True)
+        print(f"[dsl wavefront] {w}x{h} spp={spp} depth={max_depth} per wave {wf.samples_per_wave} sort={sort} "
+              f"{label}", flush=True)
         out = device.create_buffer(16, w * h)
         wf.render(s, out, spp, max_depth=max_depth, sequence=seq, graph=graph)
         got = np.zeros((h, w, 4), f32)
         s << out.copy_to(got) << lc.synchronize()
         np.testing.assert_array_equal(got.view(np.uint32), want.view(np.uint32))
+        assert (device.ray_sort("closest"), device.ray_sort("any")) == before  # the device setting is kept
         # eager work between renders on the same stream
         s << mega(*bind.args(img, 1, max_depth=max_depth, sequence=seq)).dispatch(w, h)
         s.synchronize()
