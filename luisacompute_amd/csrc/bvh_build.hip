@@ -303,7 +303,9 @@ __device__ __forceinline__ void publish_root(BuildCounters *ctr, uint2 *queue0, 
 // Deferred entries of a phase, compacted per workgroup in thread order (which is first-leaf
 // order: a thread's final subtree contains its starting entry): out[blockIdx.x * blockDim.x + r],
 // count in bcount[blockIdx.x]. Every thread of the workgroup calls it once.
-__device__ __forceinline__ void compact_deferred(uint32_t deferred, uint32_t *out, uint32_t *bcount) {
+__device__ __forceinline__ void compact_deferred(uint32_t deferred, uint32_t *out, uint32_t *bcount,
+                                                 uint32_t chunk = 0xFFFFFFFFu) {
+    if (chunk == 0xFFFFFFFFu) chunk = blockIdx.x;
     __shared__ uint32_t s_wave_count[16];
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const bool has = deferred != kNoEntry;
@@ -316,22 +318,65 @@ __device__ __forceinline__ void compact_deferred(uint32_t deferred, uint32_t *ou
         if (w < wave) base += s_wave_count[w];
         total += s_wave_count[w];
     }
-    if (has) out[blockIdx.x * blockDim.x + base + rank] = deferred;
-    if (threadIdx.x == 0u) bcount[blockIdx.x] = total;
+    if (has) out[chunk * blockDim.x + base + rank] = deferred;
+    if (threadIdx.x == 0u) bcount[chunk] = total;
+}
+
+// One thread's climb from entry `cur` (a leaf ref or an internal node): boxes go up while the
+// parent's key range lies inside [L0, L1]; the first child to arrive at a parent stops, the second
+// merges the sibling's box and goes on. Returns the entry to defer (kNoEntry: stopped at a
+// parent its sibling completes, or published the root).
+template<bool AGENT>
+__device__ __forceinline__ uint32_t climb(const LeafBoxes &lb, const BinaryTree &bt, uint32_t cur, uint32_t L0,
+                                          uint32_t L1, BuildCounters *ctr, uint2 *queue0) {
+    float3 lo, hi;
+    uint32_t f, l;
+    if (cur & kLeafBit) {
+        f = l = cur & ~kLeafBit;
+        leaf_box(lb, f, lo, hi);
+    } else {
+        const uint2 rg = bt.range[cur];
+        f = rg.x;
+        l = rg.y;
+        const float4 *cb = bt.cbox + 4 * (uint64_t)cur;
+        float4 a = cb[0], b = cb[1], c = cb[2], d = cb[3];
+        lo = make_float3(fminf(a.x, c.x), fminf(a.y, c.y), fminf(a.z, c.z));
+        hi = make_float3(fmaxf(b.x, d.x), fmaxf(b.y, d.y), fmaxf(b.z, d.z));
+    }
+    for (;;) {
+        if (cur == 0u) { // internal node 0 spans every leaf: the root
+            publish_root(ctr, queue0, 0u, lo, hi);
+            return kNoEntry;
+        }
+        const uint2 link = (cur & kLeafBit) ? bt.parent_leaf[f] : bt.parent_int[cur];
+        const uint32_t p = link.x, side = link.y >> 31, end = link.y & ~kLeafBit;
+        const uint32_t pf = side ? end : f, pl = side ? l : end;
+        if (pf < L0 || pl > L1) return cur;
+        bt.cbox[4 * (uint64_t)p + 2 * side] = make_float4(lo.x, lo.y, lo.z, 0.0f);
+        bt.cbox[4 * (uint64_t)p + 2 * side + 1] = make_float4(hi.x, hi.y, hi.z, 0.0f);
+        const uint32_t first = AGENT
+            ? __hip_atomic_exchange(&bt.slots[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT)
+            : __hip_atomic_exchange(&bt.slots[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (first == 0u) return kNoEntry; // the sibling's thread completes p
+        const float4 slo = bt.cbox[4 * (uint64_t)p + 2 * (1u - side)];
+        const float4 shi = bt.cbox[4 * (uint64_t)p + 2 * (1u - side) + 1];
+        lo = make_float3(fminf(lo.x, slo.x), fminf(lo.y, slo.y), fminf(lo.z, slo.z));
+        hi = make_float3(fmaxf(hi.x, shi.x), fmaxf(hi.y, shi.y), fmaxf(hi.z, shi.z));
+        cur = p;
+        f = pf;
+        l = pl;
+    }
 }
 
 // Bottom-up boxes, one phase. The workgroup owns entries [e0, e1) -- complete subtrees ordered by
 // first leaf -- and so the leaf range [L0, L1]. A thread climbs from its entry while the parent's
 // key range lies inside [L0, L1]; then both children of that parent are climbed by threads of the
-// same workgroup, and the handoff (the first child to arrive stops, the second merges the boxes
-// and goes on) needs only workgroup-scope ordering. A parent whose range leaves the workgroup is
-// deferred: the child becomes an entry of the next phase (compact_deferred into `pend`).
-// whole != 0: the workgroup owns all leaves (last phase, one workgroup) or, with agent scope, the
-// fallback for inputs whose phases stop shrinking; nothing is deferred then.
-template<bool AGENT>
-__global__ void k_bounds_phase(LeafBoxes lb, uint32_t n, BinaryTree bt,
-                               const uint32_t *entries, uint32_t m, uint32_t *pend, uint32_t *bcount,
-                               BuildCounters *ctr, uint2 *queue0, uint32_t whole) {
+// same workgroup, and the handoff needs only workgroup-scope ordering. A parent whose range leaves
+// the workgroup is deferred: the child becomes an entry of the next pass (compact_deferred into
+// `pend`). whole != 0: the workgroup owns all leaves (a build of at most kLastBlock primitives,
+// one workgroup); nothing is deferred then.
+__global__ void k_bounds_phase(LeafBoxes lb, uint32_t n, BinaryTree bt, const uint32_t *entries, uint32_t m,
+                               uint32_t *pend, uint32_t *bcount, BuildCounters *ctr, uint2 *queue0, uint32_t whole) {
     const uint32_t e0 = blockIdx.x * blockDim.x;
     if (e0 >= m) return; // whole workgroups only
     const uint32_t e1 = min(e0 + blockDim.x, m);
@@ -342,50 +387,51 @@ __global__ void k_bounds_phase(LeafBoxes lb, uint32_t n, BinaryTree bt,
     }
     const uint32_t e = e0 + threadIdx.x;
     uint32_t deferred = kNoEntry;
-    if (e < e1) {
-        uint32_t cur = entries ? entries[e] : (e | kLeafBit);
-        float3 lo, hi;
-        uint32_t f, l;
-        if (cur & kLeafBit) {
-            f = l = cur & ~kLeafBit;
-            leaf_box(lb, f, lo, hi);
-        } else {
-            const uint2 rg = bt.range[cur];
-            f = rg.x;
-            l = rg.y;
-            const float4 *cb = bt.cbox + 4 * (uint64_t)cur;
-            float4 a = cb[0], b = cb[1], c = cb[2], d = cb[3];
-            lo = make_float3(fminf(a.x, c.x), fminf(a.y, c.y), fminf(a.z, c.z));
-            hi = make_float3(fmaxf(b.x, d.x), fmaxf(b.y, d.y), fmaxf(b.z, d.z));
-        }
-        for (;;) {
-            if (cur == 0u) { // internal node 0 spans every leaf: the root
-                publish_root(ctr, queue0, 0u, lo, hi);
-                break;
-            }
-            const uint2 link = (cur & kLeafBit) ? bt.parent_leaf[f] : bt.parent_int[cur];
-            const uint32_t p = link.x, side = link.y >> 31, end = link.y & ~kLeafBit;
-            const uint32_t pf = side ? end : f, pl = side ? l : end;
-            if (pf < L0 || pl > L1) {
-                deferred = cur;
-                break;
-            }
-            bt.cbox[4 * (uint64_t)p + 2 * side] = make_float4(lo.x, lo.y, lo.z, 0.0f);
-            bt.cbox[4 * (uint64_t)p + 2 * side + 1] = make_float4(hi.x, hi.y, hi.z, 0.0f);
-            const uint32_t first = AGENT
-                ? __hip_atomic_exchange(&bt.slots[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT)
-                : __hip_atomic_exchange(&bt.slots[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
-            if (first == 0u) break; // the sibling's thread completes p
-            const float4 slo = bt.cbox[4 * (uint64_t)p + 2 * (1u - side)];
-            const float4 shi = bt.cbox[4 * (uint64_t)p + 2 * (1u - side) + 1];
-            lo = make_float3(fminf(lo.x, slo.x), fminf(lo.y, slo.y), fminf(lo.z, slo.z));
-            hi = make_float3(fmaxf(hi.x, shi.x), fmaxf(hi.y, shi.y), fmaxf(hi.z, shi.z));
-            cur = p;
-            f = pf;
-            l = pl;
-        }
-    }
+    if (e < e1) deferred = climb<false>(lb, bt, entries ? entries[e] : (e | kLeafBit), L0, L1, ctr, queue0);
     if (!whole) compact_deferred(deferred, pend, bcount);
+}
+
+// The bottom-up passes after the first run over entry lists whose length a kernel left in device
+// memory (`count`), so the host never reads it. k_bounds_phase_dev: a phase as k_bounds_phase,
+// workgroup w taking the chunks of kPhaseBlock entries w, w + gridDim.x, ... (chunk c's deferred
+// entries go to pend[c * kPhaseBlock ...], their number to bcount[c]). k_bounds_last: one
+// workgroup climbs from every remaining entry to the root (threads stride over the entries; the
+// handoffs stay in the workgroup). k_gather_dev: the ordered compaction of a phase's deferred
+// entries into the next list and its count.
+__global__ __launch_bounds__(kPhaseBlock) void k_bounds_phase_dev(LeafBoxes lb, uint32_t n, BinaryTree bt,
+                                                                  const uint32_t *entries, const uint32_t *count,
+                                                                  uint32_t *pend, uint32_t *bcount, BuildCounters *ctr,
+                                                                  uint2 *queue0) {
+    const uint32_t m = *count;
+    const uint32_t chunks = (m + kPhaseBlock - 1u) / kPhaseBlock;
+    for (uint32_t c = blockIdx.x; c < chunks; c += gridDim.x) {
+        const uint32_t e0 = c * kPhaseBlock, e1 = min(e0 + kPhaseBlock, m);
+        const uint32_t L0 = ref_first(bt, entries[e0]), L1 = ref_last(bt, entries[e1 - 1u]);
+        const uint32_t e = e0 + threadIdx.x;
+        uint32_t deferred = kNoEntry;
+        if (e < e1) deferred = climb<false>(lb, bt, entries[e], L0, L1, ctr, queue0);
+        compact_deferred(deferred, pend, bcount, c);
+        __syncthreads(); // compact_deferred's LDS counts are rewritten by the next chunk
+    }
+}
+
+__global__ __launch_bounds__(kLastBlock) void k_bounds_last(LeafBoxes lb, uint32_t n, BinaryTree bt,
+                                                            const uint32_t *entries, const uint32_t *count,
+                                                            BuildCounters *ctr, uint2 *queue0) {
+    const uint32_t m = *count;
+    for (uint32_t e = threadIdx.x; e < m; e += blockDim.x) climb<false>(lb, bt, entries[e], 0u, n - 1u, ctr, queue0);
+}
+
+__global__ __launch_bounds__(kPhaseBlock) void k_gather_dev(const uint32_t *src, const uint32_t *bcount,
+                                                            const uint32_t *boff, const uint32_t *count,
+                                                            uint32_t *dst, uint32_t *total) {
+    const uint32_t chunks = (*count + kPhaseBlock - 1u) / kPhaseBlock;
+    if (chunks == 0u && blockIdx.x == 0u && threadIdx.x == 0u) *total = 0u;
+    for (uint32_t c = blockIdx.x; c < chunks; c += gridDim.x) {
+        const uint32_t k = bcount[c], o = boff[c];
+        for (uint32_t t = threadIdx.x; t < k; t += blockDim.x) dst[o + t] = src[(uint64_t)c * kPhaseBlock + t];
+        if (c == chunks - 1u && threadIdx.x == 0u) *total = o + k;
+    }
 }
 
 // First bottom-up phase: workgroup b owns leaves [b B, b B + B) and climbs from every leaf. A node
@@ -832,20 +878,32 @@ __global__ __launch_bounds__(64) void k_sah_clusters(const uint32_t *entries, co
 // reserved with one 64-bit atomic per workgroup iteration. Leaf slots list their primitives in
 // out.prim_out in triangle-record order: primitive ids for a box build, sorted positions for a
 // triangle build, whose records are then copied from the sorted records (k_tri_records).
-template<int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_collapse(LeafBoxes lb, BinaryTree bt,
-                                                     BuildCounters *ctr, uint32_t level, const uint2 *qin,
-                                                     uint2 *qout, WideOut out, uint32_t split_small,
+//
+// Level l reads its work items from q[l & 1] and appends the next level's to q[(l + 1) & 1].
+// TAIL: one workgroup walks levels `level`, `level` + 1, ... until one is empty: the levels below
+// the ones launched over the whole GPU, present only in deep trees, with no host read of a level's
+// count. The level words are read with agent-scope atomic loads (the tail's own reservations
+// update them while it runs); a level's queue entries are written and read by the same workgroup,
+// ordered by the barrier that ends each item round.
+template<int BLOCK, bool TAIL>
+__global__ __launch_bounds__(BLOCK) void k_collapse(LeafBoxes lb, BinaryTree bt, BuildCounters *ctr, uint32_t level,
+                                                     uint2 *q0, uint2 *q1, WideOut out, uint32_t split_small,
                                                      uint32_t leaf_max, uint32_t width) {
     __shared__ __align__(16) uint32_t s_node[BLOCK / 8][20];
     __shared__ uint32_t s_wave[2][BLOCK / 64];
     __shared__ unsigned long long s_base;
-    const uint32_t count = (uint32_t)(ctr->level[level] >> 32);
+    const auto level_word = [&](uint32_t l) {
+        return __hip_atomic_load(&ctr->level[l], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    };
+  for (uint32_t lv = level; lv < (TAIL ? kCollapseLaunches : level + 1u); lv++) {
+    const uint2 *qin = (lv & 1u) ? q1 : q0;
+    uint2 *qout = (lv & 1u) ? q0 : q1;
+    const uint32_t count = (uint32_t)(level_word(lv) >> 32);
     if (count == 0) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0) atomicMax(&ctr->depth, level + 1);
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicMax(&ctr->depth, lv + 1);
     uint32_t next_start = 0, tri_start = 0;
-    for (uint32_t j = 0; j <= level; j++) {
-        const uint64_t lw = ctr->level[j];
+    for (uint32_t j = 0; j <= lv; j++) {
+        const uint64_t lw = level_word(j);
         next_start += (uint32_t)(lw >> 32);
         tri_start += (uint32_t)lw;
     }
@@ -979,7 +1037,7 @@ __global__ __launch_bounds__(BLOCK) void k_collapse(LeafBoxes lb, BinaryTree bt,
             }
             unsigned long long b = 0;
             if (bi | btri)
-                b = atomicAdd(reinterpret_cast<unsigned long long *>(&ctr->level[level + 1]),
+                b = atomicAdd(reinterpret_cast<unsigned long long *>(&ctr->level[lv + 1]),
                               ((unsigned long long)bi << 32) | btri);
             s_base = b;
         }
@@ -1036,6 +1094,7 @@ __global__ __launch_bounds__(BLOCK) void k_collapse(LeafBoxes lb, BinaryTree bt,
             reinterpret_cast<uint4 *>(out.nodes + item.y)[lane] = reinterpret_cast<const uint4 *>(nw)[lane];
         __syncthreads();
     }
+  }
 }
 
 // Totals of the per-level words (one thread).
@@ -1298,7 +1357,7 @@ struct CollapseShape {
 template<int B>
 int collapse_occupancy() {
     int b = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_collapse<B>, B, 0) != hipSuccess || b < 1) b = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_collapse<B, false>, B, 0) != hipSuccess || b < 1) b = 1;
     return b;
 }
 
@@ -1349,7 +1408,7 @@ struct ScratchLayout {
         bcount = carve<uint32_t>(p, phase_blocks(m));
         boff = carve<uint32_t>(p, phase_blocks(m));
         ctr = carve<BuildCounters>(p, 1);
-        dcount = carve<uint32_t>(p, 1);
+        dcount = carve<uint32_t>(p, 2);
         temp_bytes = radix_sort_count_bytes((uint32_t)m);
         temp = carve<char>(p, temp_bytes);
     }
@@ -1373,7 +1432,7 @@ size_t bvh_scratch_bytes(uint32_t n) {
     b += 2 * phase_blocks(m) * sizeof(uint32_t) + 256 * 2; // phase workgroup counts + scan
     b += 2 * m * sizeof(uint2) + 256 * 2;           // collapse queues
     b += sizeof(BuildCounters) + 256;               // counters
-    b += sizeof(uint32_t) + 256;                    // entry count
+    b += 2 * sizeof(uint32_t) + 256;                // entry counts of the two lists
     b += radix_sort_count_bytes((uint32_t)m) + 256; // digit counts of the key sort
     return b;
 }
@@ -1400,14 +1459,12 @@ hipError_t bvh_validate(hipStream_t stream, const WideNode *nodes, uint32_t node
 
 // Builds a wide BVH over n primitives (triangles if boxes == nullptr, else boxes).
 // Output arrays: nodes (bvh_node_capacity(n) entries), tris (n entries) or prim_out (n entries).
-// The counters land in `counters_out` (device) for the caller to read back. `pinned_word` is a
-// host-pinned word used to read entry and level counts: the stream is synchronized once per
-// bottom-up phase (about log_64(n) phases) and once per chunk of collapse levels.
+// The counters land in `counters_out` (device) for the caller to read back. The build only
+// enqueues work: every count a step depends on is read on the device, so the host never waits.
 hipError_t bvh_build(hipStream_t stream, void *scratch, size_t scratch_bytes, const void *vbase, uint64_t vstride,
                      uint64_t nverts, const uint32_t *tri, const float4 *boxes, uint32_t n, WideNode *nodes,
                      TriRecord *tris, uint32_t *prim_out, BuildCounters **counters_out, uint32_t split_small,
-                     uint32_t leaf_max, uint32_t sah_refine, uint32_t width, int cu_count, uint32_t *error_flag,
-                     uint32_t *pinned_word) {
+                     uint32_t leaf_max, uint32_t sah_refine, uint32_t width, int cu_count, uint32_t *error_flag) {
     if (leaf_max < 1 || leaf_max > kLeafMax) return hipErrorInvalidValue;
     if (width < 2 || width > 8) return hipErrorInvalidValue;
     if (scratch_bytes < bvh_scratch_bytes(n)) return hipErrorInvalidValue;
@@ -1456,22 +1513,26 @@ hipError_t bvh_build(hipStream_t stream, void *scratch, size_t scratch_bytes, co
         hipLaunchKernelGGL(k_single_leaf_root, dim3(1), dim3(1), 0, stream, lb, ctr, q0);
     } else {
         hipLaunchKernelGGL(k_karras, dim3((n - 1 + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, n, keys, bt);
-        // Bottom-up phases. Entries of phase 0 are the leaves in sorted order (entries == null).
-        // Each phase leaves its deferred entries compacted per workgroup in `pend`; a scan of the
-        // workgroup counts places them in the next entry list.
-        const uint32_t *entries = nullptr;
-        uint32_t count = n;
-        int which = 0;
+        // Bottom-up boxes without host reads: a build of at most kLastBlock primitives is one
+        // workgroup. Otherwise a first pass (the leaf phase over 512-leaf runs, or the SAH pass
+        // over the clusters) leaves the entries for the rest of the tree in ent[0] and their count
+        // in dcount[0] (a scan of per-workgroup counts places them in order), then phases over
+        // chunks of 256 entries (workgroup-scope handoffs) shrink the list, alternating between
+        // ent[0] / dcount[0] and ent[1] / dcount[1], and one workgroup finishes from what remains.
+        // The number of phases comes from an estimate of the first list's length (about 18
+        // deferred entries per 512-leaf run, or 1 per SAH cluster of ~32 primitives, each phase
+        // shrinking its list ~16x); it sets only how much the last workgroup has to do.
+        const uint32_t phase_grid = (uint32_t)std::max(1, cu_count) * 4u;
+        uint64_t estimate = 0;
         if (sah) {
             // clusters in position order -> ent[0]; each rebuilt by binned SAH, its primitives
-            // reordered (ids, and the sorted records of a triangle build); the bottom-up phases
-            // then start from the cluster roots
+            // reordered (ids, and the sorted records of a triangle build)
             const uint32_t nblocks = (n + kPhaseBlock - 1) / kPhaseBlock;
             hipLaunchKernelGGL(k_sah_cut, dim3(nblocks), dim3(kPhaseBlock), 0, stream, n, bt, pend, L.bcount);
             e = exclusive_scan_u32(stream, L.bcount, L.boff, nblocks);
             if (e != hipSuccess) return e;
             hipLaunchKernelGGL(k_gather_deferred, dim3(nblocks), dim3(kPhaseBlock), 0, stream, (const uint32_t *)pend,
-                               (const uint32_t *)L.bcount, (const uint32_t *)L.boff, kPhaseBlock, nblocks, ent[which],
+                               (const uint32_t *)L.bcount, (const uint32_t *)L.boff, kPhaseBlock, nblocks, ent[0],
                                dcount);
             // one pass of resident workgroups (a second partial pass would double the time)
             static const int sah_blocks_per_cu = [] {
@@ -1481,84 +1542,66 @@ hipError_t bvh_build(hipStream_t stream, void *scratch, size_t scratch_bytes, co
             }();
             const uint32_t grid_sah = std::max<uint32_t>(
                 1u, std::min<uint32_t>((uint32_t)std::max(1, cu_count) * (uint32_t)sah_blocks_per_cu, n / 2u));
-            hipLaunchKernelGGL(k_sah_clusters, dim3(grid_sah), dim3(64), 0, stream, (const uint32_t *)ent[which],
+            hipLaunchKernelGGL(k_sah_clusters, dim3(grid_sah), dim3(64), 0, stream, (const uint32_t *)ent[0],
                                (const uint32_t *)dcount, bt, ids, (const TriRecord *)prec, srec, boxes);
-            e = hipMemcpyAsync(pinned_word, dcount, sizeof(uint32_t), hipMemcpyDeviceToHost, stream);
-            if (e != hipSuccess) return e;
-            e = hipStreamSynchronize(stream);
-            if (e != hipSuccess) return e;
-            count = *pinned_word;
-            entries = ent[which];
-            which ^= 1;
-        }
-        for (;;) {
-            if (count <= kLastBlock) {
-                uint32_t blk = (count + 63) / 64 * 64;
-                hipLaunchKernelGGL(k_bounds_phase<false>, dim3(1), dim3(blk), 0, stream, lb, n, bt, entries, count,
-                                   pend, L.bcount, ctr, q0, 1u);
-                break;
-            }
-            const uint32_t run = entries ? kPhaseBlock : kLeafRun;
-            const uint32_t nblocks = ((entries ? count : n) + run - 1) / run;
-            if (!entries)
-                hipLaunchKernelGGL(k_bounds_leaves<kLeafRun>, dim3(nblocks), dim3(kLeafRun), 0, stream, lb,
-                                   (const TriRecord *)prec, srec, n,
-                                   bt, pend, L.bcount);
-            else
-                hipLaunchKernelGGL(k_bounds_phase<false>, dim3(nblocks), dim3(kPhaseBlock), 0, stream, lb, n, bt,
-                                   entries, count, pend, L.bcount, ctr, q0, 0u);
+            estimate = n / 32u;
+        } else if (n <= kLastBlock) {
+            const uint32_t blk = (n + 63) / 64 * 64;
+            hipLaunchKernelGGL(k_bounds_phase, dim3(1), dim3(blk), 0, stream, lb, n, bt, (const uint32_t *)nullptr, n,
+                               pend, L.bcount, ctr, q0, 1u);
+        } else {
+            const uint32_t nblocks = (n + kLeafRun - 1) / kLeafRun;
+            hipLaunchKernelGGL(k_bounds_leaves<kLeafRun>, dim3(nblocks), dim3(kLeafRun), 0, stream, lb,
+                               (const TriRecord *)prec, srec, n, bt, pend, L.bcount);
             e = exclusive_scan_u32(stream, L.bcount, L.boff, nblocks);
             if (e != hipSuccess) return e;
-            hipLaunchKernelGGL(k_gather_deferred, dim3(nblocks), dim3(run), 0, stream, (const uint32_t *)pend,
-                               (const uint32_t *)L.bcount, (const uint32_t *)L.boff, run, nblocks, ent[which], dcount);
-            e = hipMemcpyAsync(pinned_word, dcount, sizeof(uint32_t), hipMemcpyDeviceToHost, stream);
-            if (e != hipSuccess) return e;
-            e = hipStreamSynchronize(stream);
-            if (e != hipSuccess) return e;
-            const uint32_t next = *pinned_word;
-            entries = ent[which];
-            which ^= 1;
-            if (next == 0) break; // the root was reached in this phase
-            if (next > count - count / 8) {
-                // Phases stopped shrinking (sibling subtrees keep straddling workgroup
-                // boundaries): finish with agent-scope handoffs over all remaining entries.
-                hipLaunchKernelGGL(k_bounds_phase<true>, dim3((next + kPhaseBlock - 1) / kPhaseBlock), dim3(kPhaseBlock),
-                                   0, stream, lb, n, bt, entries, next, pend, L.bcount, ctr, q0, 1u);
-                break;
+            hipLaunchKernelGGL(k_gather_deferred, dim3(nblocks), dim3(kLeafRun), 0, stream, (const uint32_t *)pend,
+                               (const uint32_t *)L.bcount, (const uint32_t *)L.boff, kLeafRun, nblocks, ent[0], dcount);
+            estimate = (uint64_t)nblocks * 18u;
+        }
+        if (sah || n > kLastBlock) {
+            uint32_t w = 0;
+            for (uint64_t c = estimate; c > 4u * kLastBlock; c /= 16u) {
+                hipLaunchKernelGGL(k_bounds_phase_dev, dim3(phase_grid), dim3(kPhaseBlock), 0, stream, lb, n, bt,
+                                   (const uint32_t *)ent[w], (const uint32_t *)(dcount + w), pend, L.bcount, ctr, q0);
+                e = exclusive_scan_u32_counted(stream, L.bcount, L.boff, dcount + w, kPhaseBlock);
+                if (e != hipSuccess) return e;
+                hipLaunchKernelGGL(k_gather_dev, dim3(phase_grid), dim3(kPhaseBlock), 0, stream, (const uint32_t *)pend,
+                                   (const uint32_t *)L.bcount, (const uint32_t *)L.boff, (const uint32_t *)(dcount + w),
+                                   ent[w ^ 1u], dcount + (w ^ 1u));
+                w ^= 1u;
             }
-            count = next;
+            hipLaunchKernelGGL(k_bounds_last, dim3(1), dim3(kLastBlock), 0, stream, lb, n, bt, (const uint32_t *)ent[w],
+                               (const uint32_t *)(dcount + w), ctr, q0);
         }
     }
     // triangle BLAS: leaf sorted positions go to `pend` (free after the bottom-up phases) and the
     // records are copied from the sorted records in one pass after the last level
     WideOut out{nodes, tris, tris ? pend : prim_out};
     // A root-to-leaf path of the radix tree crosses each of the 30 key bits and 32 index bits at
-    // most once, and every wide level descends at least one binary level, so the wide depth is
-    // at most 63 < kCollapseLaunches. Levels are issued in chunks; after each chunk the work
-    // count of the next level is read back and the loop stops once it is zero. The host checks
-    // that level kCollapseLaunches holds no nodes after the build.
-    const uint32_t kChunk = 6;
-    for (uint32_t level = 0; level < kCollapseLaunches; level++) {
-        const uint2 *qin = (level & 1) ? q1 : q0;
-        uint2 *qout = (level & 1) ? q0 : q1;
+    // most once (a binned-SAH cluster adds at most 63 levels below its root), and every wide level
+    // descends at least one binary level, so the wide depth stays below kCollapseLaunches. The
+    // first `head` levels (the expected depth, log8 of the primitive count, + 4) are launched over
+    // the whole GPU; a launch whose level is empty returns at once. One workgroup then walks any
+    // deeper levels. No level count is read on the host; the host checks after the build that
+    // level kCollapseLaunches holds no nodes.
+    uint32_t head = 4;
+    for (uint64_t cap = 1; cap < n; cap *= 8) head++;
+    head = std::min<uint32_t>(head, kCollapseLaunches);
+    for (uint32_t level = 0; level < head; level++) {
         if (cs.block == 1024)
-            hipLaunchKernelGGL(k_collapse<1024>, dim3(grid_collapse), dim3(1024), 0, stream, lb, bt, ctr, level, qin,
-                               qout, out, split_small, leaf_max, width);
+            hipLaunchKernelGGL((k_collapse<1024, false>), dim3(grid_collapse), dim3(1024), 0, stream, lb, bt, ctr, level,
+                               q0, q1, out, split_small, leaf_max, width);
         else if (cs.block == 512)
-            hipLaunchKernelGGL(k_collapse<512>, dim3(grid_collapse), dim3(512), 0, stream, lb, bt, ctr, level, qin,
-                               qout, out, split_small, leaf_max, width);
+            hipLaunchKernelGGL((k_collapse<512, false>), dim3(grid_collapse), dim3(512), 0, stream, lb, bt, ctr, level,
+                               q0, q1, out, split_small, leaf_max, width);
         else
-            hipLaunchKernelGGL(k_collapse<256>, dim3(grid_collapse), dim3(256), 0, stream, lb, bt, ctr, level, qin,
-                               qout, out, split_small, leaf_max, width);
-        if ((level + 1) % kChunk == 0 && level + 1 < kCollapseLaunches) {
-            e = hipMemcpyAsync(pinned_word, level_nodes_word(ctr, level + 1), sizeof(uint32_t), hipMemcpyDeviceToHost,
-                               stream);
-            if (e != hipSuccess) return e;
-            e = hipStreamSynchronize(stream);
-            if (e != hipSuccess) return e;
-            if (*pinned_word == 0) break;
-        }
+            hipLaunchKernelGGL((k_collapse<256, false>), dim3(grid_collapse), dim3(256), 0, stream, lb, bt, ctr, level,
+                               q0, q1, out, split_small, leaf_max, width);
     }
+    if (head < kCollapseLaunches)
+        hipLaunchKernelGGL((k_collapse<512, true>), dim3(1), dim3(512), 0, stream, lb, bt, ctr, head, q0, q1, out,
+                           split_small, leaf_max, width);
     hipLaunchKernelGGL(k_finish_counts, dim3(1), dim3(1), 0, stream, ctr);
     if (tris)
         hipLaunchKernelGGL(k_tri_records, dim3(std::min<uint32_t>(grid, 1u << 20)), dim3(kBlock), 0, stream, pend,

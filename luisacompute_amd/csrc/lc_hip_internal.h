@@ -24,8 +24,7 @@ size_t bvh_node_capacity(uint32_t n);
 hipError_t bvh_build(hipStream_t stream, void *scratch, size_t scratch_bytes, const void *vbase, uint64_t vstride,
                      uint64_t nverts, const uint32_t *tri, const float4 *boxes, uint32_t n, WideNode *nodes,
                      TriRecord *tris, uint32_t *prim_out, BuildCounters **counters_out, uint32_t split_small,
-                     uint32_t leaf_max, uint32_t sah_refine, uint32_t width, int cu_count, uint32_t *error_flag,
-                     uint32_t *pinned_word);
+                     uint32_t leaf_max, uint32_t sah_refine, uint32_t width, int cu_count, uint32_t *error_flag);
 hipError_t bvh_refit(hipStream_t stream, void *scratch, size_t scratch_bytes, const void *vbase, uint64_t vstride,
                      uint64_t nverts, const uint32_t *tri, uint32_t n, uint32_t depth, WideNode *nodes,
                      TriRecord *tris, BuildCounters **counters_out, int cu_count, uint32_t *error_flag);
@@ -108,6 +107,9 @@ hipError_t radix_sort_pairs_with(hipStream_t stream, void *count_temp, size_t co
                                  uint32_t begin_bit, uint32_t end_bit);
 // out[i] = in[0] + ... + in[i-1] (one workgroup); in and out may alias.
 hipError_t exclusive_scan_u32(hipStream_t stream, const uint32_t *in, uint32_t *out, uint32_t n);
+// The same over ceil(*count_ptr / div) words, the count read on the device.
+hipError_t exclusive_scan_u32_counted(hipStream_t stream, const uint32_t *in, uint32_t *out, const uint32_t *count_ptr,
+                                      uint32_t div);
 
 // ---- kernels (raysort.hip) ----
 // count_ptr (device word, may be null): only the first min(*count_ptr, n) rays are sorted.
@@ -381,7 +383,6 @@ struct HostCounters {
     float bounds[6];
     uint32_t tail_level_count; // nodes at wide level kCollapseLaunches: must be 0
     uint32_t error;
-    uint32_t phase_count;      // bottom-up phase entry count read back during a build
 };
 
 struct Mesh {

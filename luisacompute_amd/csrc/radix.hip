@@ -253,8 +253,11 @@ __global__ __launch_bounds__(kRsBlock) void k_rs_downsweep(const uint32_t *keys_
     }
 }
 
-__global__ __launch_bounds__(kScanBlock) void k_exclusive_scan(const uint32_t *in, uint32_t *out, uint32_t n) {
+// count_ptr != null: n = ceil(*count_ptr / div) (a list's chunk count, known on the device only).
+__global__ __launch_bounds__(kScanBlock) void k_exclusive_scan(const uint32_t *in, uint32_t *out, uint32_t n,
+                                                               const uint32_t *count_ptr, uint32_t div) {
     __shared__ uint32_t s_wave[kScanBlock / 64];
+    if (count_ptr) n = (uint32_t)(((uint64_t)*count_ptr + div - 1u) / div);
     uint32_t carry = 0;
     for (uint32_t c0 = 0; c0 < n; c0 += (uint32_t)(kScanBlock * kScanItems)) {
         const uint32_t b = c0 + threadIdx.x * kScanItems;
@@ -340,7 +343,15 @@ hipError_t radix_sort_pairs_with(hipStream_t stream, void *count_temp, size_t co
 hipError_t exclusive_scan_u32(hipStream_t stream, const uint32_t *in, uint32_t *out, uint32_t n) {
     if (n == 0) return hipSuccess;
     if (!in || !out) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_exclusive_scan, dim3(1), dim3(kScanBlock), 0, stream, in, out, n);
+    hipLaunchKernelGGL(k_exclusive_scan, dim3(1), dim3(kScanBlock), 0, stream, in, out, n, (const uint32_t *)nullptr,
+                       1u);
+    return hipGetLastError();
+}
+
+hipError_t exclusive_scan_u32_counted(hipStream_t stream, const uint32_t *in, uint32_t *out, const uint32_t *count_ptr,
+                                      uint32_t div) {
+    if (!in || !out || !count_ptr || div == 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_exclusive_scan, dim3(1), dim3(kScanBlock), 0, stream, in, out, 0u, count_ptr, div);
     return hipGetLastError();
 }
 
