@@ -288,6 +288,7 @@ _SIGNATURES = {
     "lc_hip_build_set_split_small": (ctypes.c_int, [_U64, ctypes.c_int]),
     "lc_hip_build_set_sah": (ctypes.c_int, [_U64, ctypes.c_int]),
     "lc_hip_build_set_width": (ctypes.c_int, [_U64, ctypes.c_int]),
+    "lc_hip_build_set_collapse_head": (ctypes.c_int, [_U64, ctypes.c_int]),
     "lc_hip_build_get_width": (ctypes.c_int, [_U64]),
     "lc_hip_timing_enable": (ctypes.c_int, [_U64, ctypes.c_int]),
     "lc_hip_timing_get": (ctypes.c_int, [_U64, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_U64)]),
@@ -556,6 +557,10 @@ class Device:
 
     def build_width(self):
         return lib().lc_hip_build_get_width(self.handle)
+
+    def set_build_collapse_head(self, levels):
+        """Wide levels launched over the whole GPU before one workgroup walks the rest (0: automatic)."""
+        _check(lib().lc_hip_build_set_collapse_head(self.handle, int(levels)), "build config")
 
     def set_build_sah(self, enable):
         """SAH refinement of FAST_TRACE builds (on by default)."""

@@ -24,7 +24,8 @@ size_t bvh_node_capacity(uint32_t n);
 hipError_t bvh_build(hipStream_t stream, void *scratch, size_t scratch_bytes, const void *vbase, uint64_t vstride,
                      uint64_t nverts, const uint32_t *tri, const float4 *boxes, uint32_t n, WideNode *nodes,
                      TriRecord *tris, uint32_t *prim_out, BuildCounters **counters_out, uint32_t split_small,
-                     uint32_t leaf_max, uint32_t sah_refine, uint32_t width, int cu_count, uint32_t *error_flag);
+                     uint32_t leaf_max, uint32_t sah_refine, uint32_t width, int cu_count, uint32_t *error_flag,
+                     uint32_t collapse_head);
 hipError_t bvh_refit(hipStream_t stream, void *scratch, size_t scratch_bytes, const void *vbase, uint64_t vstride,
                      uint64_t nverts, const uint32_t *tri, uint32_t n, uint32_t depth, WideNode *nodes,
                      TriRecord *tris, BuildCounters **counters_out, int cu_count, uint32_t *error_flag);
@@ -309,6 +310,7 @@ struct Device {
     uint32_t split_small = 1;
     uint32_t sah_refine = 1; // SAH refinement of FAST_TRACE builds (LUISA_HIP_BVH_SAH=0: off)
     uint32_t bvh_width = 8;  // children per wide node, 2..8 (LUISA_HIP_BVH_WIDTH)
+    uint32_t collapse_head = 0; // wide levels launched over the GPU before the one-workgroup tail (0: automatic)
     // error word of run-time shaders' traversal stacks (DeviceAccel::error)
     uint32_t *error = nullptr;
     // per-stream working memory of trace / query / build launches (guarded by mu)

@@ -1025,7 +1025,7 @@ int lc_hip_procedural_build(uint64_t device, uint64_t stream, uint64_t primitive
         BuildCounters *ctr = nullptr;
         hipError_t e = bvh_build(s, t.scratch, t.scratch_bytes, nullptr, 0, 0, nullptr, m->proc_boxes, n, t.nodes,
                                  nullptr, m->proc_ids, &ctr, 0, 1u, sah_for(dev, m->hint), dev->bvh_width, dev->cu_count,
-                                 S->error);
+                                 S->error, dev->collapse_head);
         if (e != hipSuccess) return hip_fail(e, "procedural bvh_build");
         LC_HIP_TRY(proc_records(s, m->proc_ids, m->proc_boxes, n, m->proc));
         copy_counters_async(ctr, m->host, S->error, s);
@@ -1131,7 +1131,7 @@ int lc_hip_mesh_build(uint64_t device, uint64_t stream, uint64_t mesh, uint32_t 
         BuildCounters *ctr = nullptr;
         hipError_t e = bvh_build(s, t.scratch, t.scratch_bytes, vaddr, vertex_stride, nverts, taddr, nullptr, n,
                                  t.nodes, m->tris, nullptr, &ctr, dev->split_small, leaf_size_for(m->hint),
-                                 sah_for(dev, m->hint), dev->bvh_width, dev->cu_count, S->error);
+                                 sah_for(dev, m->hint), dev->bvh_width, dev->cu_count, S->error, dev->collapse_head);
         if (e != hipSuccess) return hip_fail(e, "bvh_build");
         copy_counters_async(ctr, m->host, S->error, s);
         if (m->allow_compaction) {
@@ -1485,7 +1485,7 @@ int lc_hip_accel_build(uint64_t device, uint64_t stream, uint64_t accel, uint32_
                                      instance_count, a->tlas_nodes, nullptr, a->tlas_prims, &ctr, 0,
                                      leaf_size_for(a->hint), sah_for(dev, a->hint), dev->bvh_width,
                                      std::max(1, dev->cu_count),
-                                     S->error);
+                                     S->error, dev->collapse_head);
             if (e != hipSuccess) return hip_fail(e, "tlas build");
             copy_counters_async(ctr, a->tlas_host, S->error, s);
         }
@@ -2384,6 +2384,16 @@ int lc_hip_build_set_width(uint64_t device, int width) {
     if (width < 2 || width > 8) return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "BVH width must be 2..8");
     std::lock_guard<std::mutex> lk(dev->mu);
     dev->bvh_width = (uint32_t)width;
+    return LC_HIP_OK;
+}
+
+int lc_hip_build_set_collapse_head(uint64_t device, int levels) {
+    Device *dev = obj<Device>(device);
+    if (!dev) return fail(LC_HIP_ERROR_INVALID_HANDLE, "bad device");
+    if (levels < 0 || levels > (int)kCollapseLaunches)
+        return fail(LC_HIP_ERROR_INVALID_ARGUMENT, "collapse head levels must be 0 (automatic) .. 98");
+    std::lock_guard<std::mutex> lk(dev->mu);
+    dev->collapse_head = (uint32_t)levels;
     return LC_HIP_OK;
 }
 
