@@ -125,6 +125,18 @@ __device__ __forceinline__ uint32_t permute8(uint32_t m, uint32_t f) {
 
 __device__ __forceinline__ float ubyte(uint32_t w, int k) { return (float)((w >> (8 * k)) & 0xFFu); }
 
+// Quantised planes as IEEE halves: byte q of a plane word becomes the low byte of a half whose high
+// byte is 0x64, i.e. the half 1024 + q (exact). One byte permute makes the halves of two planes,
+// and a fused multiply-add takes a half operand as it is (v_fma_mix_f32), so a plane costs half a
+// permute and one FMA instead of a byte conversion and an FMA.
+__device__ __forceinline__ uint32_t plane_halves(uint32_t w, int pair) {
+    return __builtin_amdgcn_perm(0x64646464u, w, pair ? 0x07030702u : 0x07010700u);
+}
+__device__ __forceinline__ float plane_1024(uint32_t halves, int hi) {
+    const uint16_t h = hi ? (uint16_t)(halves >> 16) : (uint16_t)(halves & 0xFFFFu);
+    return (float)__builtin_bit_cast(_Float16, h);
+}
+
 // Tests the 8 children of a node; returns the hit mask in slot order. Empty slots hold the inverted
 // box qlo = 255, qhi = 0 (and meta 0), so the box test itself rejects them; a spurious hit on one
 // (possible only when every axis of the node has a vanishing scale) is a leaf slot of zero records
@@ -137,11 +149,14 @@ __device__ __forceinline__ uint32_t intersect_node(const RaySetup &r, const uint
         float s = __uint_as_float(((w0.w >> (8 * k)) & 0xFFu) << 23);
         a[k] = s * r.inv[k];
         float b = (p[k] - r.o[k]) * r.inv[k];
-        // |error| of fma(q, a, b) against the exact slab distance is below 6 ulp of
-        // 255|a| + |b|; 2^-21 (16 ulp) keeps the test conservative.
-        float e = 0x1p-21f * (255.0f * fabsf(a[k]) + fabsf(b));
-        bn[k] = b - e;
-        bf[k] = b + e;
+        // A plane's distance q a + b is computed as (1024 + q) a + (b - 1024 a): 1024 a is exact
+        // (a power-of-two scale times inv), the subtraction and the FMA each round once. The
+        // errors of b, of b - e - 1024 a and of the FMA stay below 8 units of 2^-24 (1024|a| +
+        // |b|); the slack e = 2^-20 (1024|a| + |b|) (16 such units) keeps the test conservative.
+        float e = 0x1p-20f * (1024.0f * fabsf(a[k]) + fabsf(b));
+        const float shift = 1024.0f * a[k];
+        bn[k] = (b - e) - shift;
+        bf[k] = (b + e) - shift;
     }
     // near/far plane words per axis (x: lo = w2.xy hi = w3.zw; y: lo = w2.zw hi = w4.xy;
     // z: lo = w3.xy hi = w4.zw); a negative direction swaps them.
@@ -156,12 +171,13 @@ __device__ __forceinline__ uint32_t intersect_node(const RaySetup &r, const uint
         const uint32_t wnx = s < 4 ? nx0 : nx1, wfx = s < 4 ? fx0 : fx1;
         const uint32_t wny = s < 4 ? ny0 : ny1, wfy = s < 4 ? fy0 : fy1;
         const uint32_t wnz = s < 4 ? nz0 : nz1, wfz = s < 4 ? fz0 : fz1;
-        float tnx = __builtin_fmaf(ubyte(wnx, k), a[0], bn[0]);
-        float tny = __builtin_fmaf(ubyte(wny, k), a[1], bn[1]);
-        float tnz = __builtin_fmaf(ubyte(wnz, k), a[2], bn[2]);
-        float tfx = __builtin_fmaf(ubyte(wfx, k), a[0], bf[0]);
-        float tfy = __builtin_fmaf(ubyte(wfy, k), a[1], bf[1]);
-        float tfz = __builtin_fmaf(ubyte(wfz, k), a[2], bf[2]);
+        const int pair = k >> 1, hi = k & 1;
+        float tnx = __builtin_fmaf(plane_1024(plane_halves(wnx, pair), hi), a[0], bn[0]);
+        float tny = __builtin_fmaf(plane_1024(plane_halves(wny, pair), hi), a[1], bn[1]);
+        float tnz = __builtin_fmaf(plane_1024(plane_halves(wnz, pair), hi), a[2], bn[2]);
+        float tfx = __builtin_fmaf(plane_1024(plane_halves(wfx, pair), hi), a[0], bf[0]);
+        float tfy = __builtin_fmaf(plane_1024(plane_halves(wfy, pair), hi), a[1], bf[1]);
+        float tfz = __builtin_fmaf(plane_1024(plane_halves(wfz, pair), hi), a[2], bf[2]);
         float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, tmin));
         float tf = fminf(fminf(tfx, tfy), fminf(tfz, tcull));
         if (tn <= tf) hits |= 1u << s;
