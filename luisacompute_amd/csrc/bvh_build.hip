@@ -943,7 +943,9 @@ __global__ __launch_bounds__(BLOCK) void k_collapse(LeafBoxes lb, BinaryTree bt,
                 valid = true;
             }
         }
-        // ---- open the widest child (must-split before may-split) until 8 children
+        // ---- open the widest child (must-split before may-split) until 8 children; while two
+        // slots are free the runner-up opens in the same round (its loads overlap the winner's:
+        // half the dependent load rounds)
         for (int it = 0; it < 7; it++) {
             uint64_t key = 0;
             if (valid) {
@@ -954,21 +956,29 @@ __global__ __launch_bounds__(BLOCK) void k_collapse(LeafBoxes lb, BinaryTree bt,
             const uint64_t best = g8_max_u64(key);
             if (nc >= width || best == 0) break; // uniform within the group
             const uint32_t winner = 7u - (uint32_t)(best & 7u);
-            const uint32_t wref = __shfl(ref, (int)winner, 8);
-            const uint32_t wlast = __shfl(last, (int)winner, 8);
-            if (lane == winner || lane == nc) {
-                const uint32_t side = lane == winner ? 0u : 1u;
-                const uint2 ch = bt.child[wref];
-                const float4 a = bt.cbox[4 * (uint64_t)wref + 2 * side], b = bt.cbox[4 * (uint64_t)wref + 2 * side + 1];
+            const uint64_t best2 = g8_max_u64(nc + 2u <= width && lane != winner ? key : 0ull);
+            const bool two = best2 != 0;
+            const uint32_t runner = 7u - (uint32_t)(best2 & 7u);
+            const uint32_t wref = __shfl(ref, (int)winner, 8), wlast = __shfl(last, (int)winner, 8);
+            const uint32_t rref = __shfl(ref, (int)runner, 8), rlast = __shfl(last, (int)runner, 8);
+            // lanes: the winner and lane nc take the winner's children, the runner-up and lane
+            // nc + 1 the runner-up's (side 0 stays in the opened child's lane)
+            const bool wside = lane == winner || lane == nc;
+            const bool rside = two && (lane == runner || lane == nc + 1u);
+            if (wside || rside) {
+                const uint32_t side = (lane == winner || (two && lane == runner)) ? 0u : 1u;
+                const uint32_t pref = wside ? wref : rref, plast = wside ? wlast : rlast;
+                const uint2 ch = bt.child[pref];
+                const float4 a = bt.cbox[4 * (uint64_t)pref + 2 * side], b = bt.cbox[4 * (uint64_t)pref + 2 * side + 1];
                 const uint32_t gamma = ch.x & ~kLeafBit;
                 ref = side ? ch.y : ch.x;
                 if (side) first = gamma + 1u;
-                last = side ? wlast : gamma;
+                last = side ? plast : gamma;
                 lo = make_float3(a.x, a.y, a.z);
                 hi = make_float3(b.x, b.y, b.z);
                 valid = true;
             }
-            nc++;
+            nc += two ? 2u : 1u;
         }
         const uint32_t cnt = valid ? last - first + 1u : 0u;
         // ---- node box, origin and scales
