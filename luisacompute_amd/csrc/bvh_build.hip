@@ -221,15 +221,6 @@ __global__ __launch_bounds__(kBlock) void k_morton(PrimSource src, uint32_t n, c
     keys[p] = expand10(q[0]) << 2 | expand10(q[1]) << 1 | expand10(q[2]);
 }
 
-// Common-prefix length of sorted keys i and j (-1 outside [0, n)); equal keys continue with the
-// common prefix of the indices, so every pair of positions is ordered (Karras 2012, sec. 4).
-__device__ __forceinline__ int prefix_len(const uint32_t *keys, int n, int i, int j) {
-    if (j < 0 || j >= n) return -1;
-    uint32_t a = keys[i], b = keys[j];
-    if (a != b) return __clz((int)(a ^ b));
-    return 32 + __clz(i ^ j);
-}
-
 struct BinaryTree {
     float4 *cbox;          // 4 per internal node: left lo, left hi, right lo, right hi
     uint2 *child;          // per internal node: left ref, right ref
@@ -251,23 +242,43 @@ __device__ __forceinline__ uint2 parent_link(uint32_t parent, uint32_t end, uint
 // binary search over prefix lengths and splits it at the highest differing bit (Karras,
 // "Maximizing Parallelism in the Construction of BVHs, Octrees, and k-d Trees", HPG 2012).
 // No synchronisation: every node depends on the sorted keys only. Root = internal node 0.
+// Most nodes span a few keys, so the workgroup first copies the keys of its nodes and kBlock more
+// on each side into LDS; the searches read keys there and go to memory only beyond that window.
 __global__ __launch_bounds__(kBlock) void k_karras(uint32_t n, const uint32_t *keys, BinaryTree bt) {
+    __shared__ uint32_t s_keys[3 * kBlock];
     const int N = (int)n;
+    const int w0 = (int)(blockIdx.x * blockDim.x) - (int)kBlock; // position of s_keys[0]
+    for (int k = (int)threadIdx.x; k < 3 * (int)kBlock; k += (int)blockDim.x) {
+        const int g = w0 + k;
+        s_keys[k] = (g >= 0 && g < N) ? keys[g] : 0u;
+    }
+    __syncthreads();
     const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (i >= N - 1) return;
-    const int d = prefix_len(keys, N, i, i + 1) > prefix_len(keys, N, i, i - 1) ? 1 : -1;
-    const int dmin = prefix_len(keys, N, i, i - d);
+    const uint32_t ki = s_keys[i - w0];
+    // common-prefix length of sorted keys i and j (-1 outside [0, N)); equal keys continue with
+    // the common prefix of the positions, so every pair of positions is ordered (Karras 2012,
+    // sec. 4). The key of j comes from the window when it lies there.
+    const auto plen = [&](int j) -> int {
+        if (j < 0 || j >= N) return -1;
+        const int k = j - w0;
+        const uint32_t kj = (k >= 0 && k < 3 * (int)kBlock) ? s_keys[k] : keys[j];
+        if (ki != kj) return __clz((int)(ki ^ kj));
+        return 32 + __clz(i ^ j);
+    };
+    const int d = plen(i + 1) > plen(i - 1) ? 1 : -1;
+    const int dmin = plen(i - d);
     int lmax = 2;
-    while (prefix_len(keys, N, i, i + lmax * d) > dmin) lmax <<= 1;
+    while (plen(i + lmax * d) > dmin) lmax <<= 1;
     int l = 0;
     for (int t = lmax >> 1; t >= 1; t >>= 1)
-        if (prefix_len(keys, N, i, i + (l + t) * d) > dmin) l += t;
+        if (plen(i + (l + t) * d) > dmin) l += t;
     const int j = i + l * d;
-    const int dnode = prefix_len(keys, N, i, j);
+    const int dnode = plen(j);
     int s = 0, t = l;
     do {
         t = (t + 1) >> 1;
-        if (prefix_len(keys, N, i, i + (s + t) * d) > dnode) s += t;
+        if (plen(i + (s + t) * d) > dnode) s += t;
     } while (t > 1);
     const int gamma = i + s * d + min(d, 0);
     const int lo = min(i, j), hi = max(i, j);
