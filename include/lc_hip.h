@@ -623,7 +623,7 @@ int lc_hip_build_set_sah(uint64_t device, int enable);
 int lc_hip_build_set_width(uint64_t device, int width);
 int lc_hip_build_get_width(uint64_t device);
 /* Build knob: wide levels of the collapse launched over the whole GPU (0 = automatic:
-   ceil(log8 n) + 4); deeper levels are walked by one workgroup. Same BVH either way (node
+   ceil(log_width n) + 4); deeper levels are walked by one workgroup. Same BVH either way (node
    numbering within a level may differ); a small value only moves work to that workgroup. */
 int lc_hip_build_set_collapse_head(uint64_t device, int levels);
 /* Kernel timing with HIP events recorded on the launch stream around each build/trace. */

@@ -1603,12 +1603,12 @@ hipError_t bvh_build(hipStream_t stream, void *scratch, size_t scratch_bytes, co
     // A root-to-leaf path of the radix tree crosses each of the 30 key bits and 32 index bits at
     // most once (a binned-SAH cluster adds at most 63 levels below its root), and every wide level
     // descends at least one binary level, so the wide depth stays below kCollapseLaunches. The
-    // first `head` levels (by default the expected depth, log8 of the primitive count, + 4) are launched over
+    // first `head` levels (by default the expected depth, log_width of the primitive count, + 4) are launched over
     // the whole GPU; a launch whose level is empty returns at once. One workgroup then walks any
     // deeper levels. No level count is read on the host; the host checks after the build that
     // level kCollapseLaunches holds no nodes.
     uint32_t head = 4;
-    for (uint64_t cap = 1; cap < n; cap *= 8) head++;
+    for (uint64_t cap = 1; cap < n; cap *= std::max<uint32_t>(width, 2u)) head++;
     if (collapse_head > 0) head = collapse_head; // set by the caller (lc_hip_build_set_collapse_head)
     head = std::min<uint32_t>(head, kCollapseLaunches);
     for (uint32_t level = 0; level < head; level++) {
